@@ -1,0 +1,46 @@
+// cf_kernels.h — device buffers of one scene and the kernel launchers.
+#pragma once
+#include "cf_device.h"
+
+namespace cf {
+
+// Every device buffer of one scene (the `buf` dictionary of js/fluid.js:52-68
+// plus the level-0 multigrid arrays).  Passed by value as a kernel argument.
+struct Fields {
+  const cf_params* P;  // the 160-B parameter block in device memory
+  int nx, ny;
+  float *U, *Uh, *U2, *V, *Vh, *V2, *T, *Th, *T2;
+  float4 *Y, *Yh, *Y2;
+  float2 *SQ, *cellG;
+  float *fx, *fy, *pieces;
+  int *acc, *faceW;
+  float4* rad;
+  float *p0, *rhs0, *ax0, *ay0;
+};
+
+// One multigrid level (the `Level` uniform of shaders/pressure.wgsl:7 and its arrays).
+struct Level {
+  int nx, ny;
+  float h2;
+  float *p, *rhs, *ax, *ay;
+};
+
+void launch_advect_fwd(const Fields& F, hipStream_t s);
+void launch_advect_corr(const Fields& F, hipStream_t s);
+void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s);
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, hipStream_t s);
+void launch_forces(const Fields& F, hipStream_t s);
+void launch_coef_rhs(const Fields& F, hipStream_t s);
+void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s);
+void launch_relax(const Level& L, int color, hipStream_t s);
+void launch_restrict_res(const Level& f, const Level& c, hipStream_t s);
+void launch_prolong(const Level& f, const Level& c, hipStream_t s);
+void launch_coarsest(const Level& L, int iters, hipStream_t s);
+void launch_project(const Fields& F, hipStream_t s);
+void launch_geometry(const Fields& F, hipStream_t s);
+void launch_radiation(const Fields& F, int ntargets, hipStream_t s);
+void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s);
+unsigned div_error_blocks(const Fields& F);
+void launch_div_error(const Fields& F, float* part, hipStream_t s);
+
+}  // namespace cf

@@ -1,0 +1,833 @@
+// cf_kernels.hip — gfx950 kernels of the campfire gas step.
+//
+// One thread per cell or face, workgroups of 32x8 threads (four wave64s, each
+// wave two 32-cell rows so every row access is a contiguous 128-B segment),
+// a 1-D launch remapped so each XCD owns a contiguous band of tiles.  The
+// integer accumulators are reduced over 8x8 cell tiles exactly as the
+// reference's 8x8 workgroups do (shaders/sources.wgsl:160-172), so `acc` gets
+// the same set of integer contributions.
+#include "cf_kernels.h"
+
+namespace cf {
+
+// -------------------------------------------------------------------------
+// tile decode: TX x TY threads per workgroup over an (ex x ey) index domain
+// -------------------------------------------------------------------------
+template <int TX, int TY>
+__device__ __forceinline__ bool tile_ij(int ex, int ey, int& i, int& j) {
+  const int gx = (ex + TX - 1) / TX, gy = (ey + TY - 1) / TY;
+  const int t = xcd_tile(blockIdx.x, gx * gy);
+  i = (t % gx) * TX + threadIdx.x;
+  j = (t / gx) * TY + threadIdx.y;
+  return true;
+}
+
+// -------------------------------------------------------------------------
+// advection, forward pass: scalForward + velForward (shaders/advect.wgsl:36-47,68-76)
+// -------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_advect_fwd(Fields F) {
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  const float dt = P.dt, dx = P.dx;
+  if (i < nx && j < ny) {
+    const int k = j * nx + i;
+    float2 d = trace(F.U, F.V, nx, ny, dx, (float)i + K(0.5), (float)j + K(0.5), dt);
+    F.Th[k] = sC(F.T, nx, ny, P.Tamb, d.x, d.y);
+    F.Yh[k] = sC4(F.Y, nx, ny, make_float4(0.f, P.YOamb, 0.f, 0.f), d.x, d.y);
+  }
+  if (i <= nx && j < ny) {
+    float2 d = trace(F.U, F.V, nx, ny, dx, (float)i, (float)j + K(0.5), dt);
+    F.Uh[j * (nx + 1) + i] = sU(F.U, nx, ny, d.x, d.y);
+  }
+  if (i < nx && j <= ny) {
+    float2 d = trace(F.U, F.V, nx, ny, dx, (float)i + K(0.5), (float)j, dt);
+    F.Vh[j * nx + i] = sV(F.V, nx, ny, d.x, d.y);
+  }
+}
+
+// -------------------------------------------------------------------------
+// advection, MacCormack correction: scalCorrect + velCorrect (:49-66,78-103)
+// -------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_advect_corr(Fields F) {
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  const float dt = P.dt, dx = P.dx;
+  if (i < nx && j < ny) {
+    const int k = j * nx + i;
+    const float4 aY = make_float4(0.f, P.YOamb, 0.f, 0.f);
+    const float gx = (float)i + K(0.5), gy = (float)j + K(0.5);
+    float2 d = trace(F.U, F.V, nx, ny, dx, gx, gy, dt);
+    float2 f = trace(F.U, F.V, nx, ny, dx, gx, gy, -dt);
+    float2 mT = mmC(F.T, nx, ny, P.Tamb, d.x, d.y);
+    float4 lo, hi;
+    mmC4(F.Y, nx, ny, aY, d.x, d.y, lo, hi);
+    const float Thk = F.Th[k];
+    const float4 Yhk = F.Yh[k], Yk = F.Y[k];
+    const float Tc = Thk + K(0.5) * (F.T[k] - sC(F.Th, nx, ny, P.Tamb, f.x, f.y));
+    const float4 sb = sC4(F.Yh, nx, ny, aY, f.x, f.y);
+    const float4 Yc = make_float4(Yhk.x + K(0.5) * (Yk.x - sb.x), Yhk.y + K(0.5) * (Yk.y - sb.y),
+                                  Yhk.z + K(0.5) * (Yk.z - sb.z), Yhk.w + K(0.5) * (Yk.w - sb.w));
+    const bool inT = Tc >= mT.x && Tc <= mT.y;
+    const bool inY = Yc.x >= lo.x && Yc.y >= lo.y && Yc.z >= lo.z && Yc.w >= lo.w &&
+                     Yc.x <= hi.x && Yc.y <= hi.y && Yc.z <= hi.z && Yc.w <= hi.w;
+    const float mixC = fminf(Yc.x, Yc.y / P.sO2);
+    const float mixL = fminf(Yhk.x, Yhk.y / P.sO2);
+    if (inT && inY && mixC <= mixL + K(1e-5)) {
+      F.T2[k] = Tc; F.Y2[k] = Yc;
+    } else {
+      F.T2[k] = Thk; F.Y2[k] = Yhk;
+    }
+  }
+  if (i <= nx && j < ny) {
+    const int k = j * (nx + 1) + i;
+    const float gx = (float)i, gy = (float)j + K(0.5);
+    float2 b = trace(F.U, F.V, nx, ny, dx, gx, gy, -dt);
+    float back = sU(F.Uh, nx, ny, b.x, b.y);
+    float2 d = trace(F.U, F.V, nx, ny, dx, gx, gy, dt);
+    float2 m = mmU(F.U, nx, ny, d.x, d.y);
+    F.U2[k] = clampf(F.Uh[k] + K(0.5) * (F.U[k] - back), m.x, m.y);
+  }
+  if (i < nx && j <= ny) {
+    const int k = j * nx + i;
+    const float gx = (float)i + K(0.5), gy = (float)j;
+    float2 b = trace(F.U, F.V, nx, ny, dx, gx, gy, -dt);
+    float back = sV(F.Vh, nx, ny, b.x, b.y);
+    float2 d = trace(F.U, F.V, nx, ny, dx, gx, gy, dt);
+    float2 m = mmV(F.V, nx, ny, d.x, d.y);
+    F.V2[k] = clampf(F.Vh[k] + K(0.5) * (F.V[k] - back), m.x, m.y);
+  }
+}
+
+// -------------------------------------------------------------------------
+// explicit diffusion of one cell (shaders/diffuse.wgsl:18-50)
+// -------------------------------------------------------------------------
+__device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny, const float* __restrict__ Tin,
+                                             const float4* __restrict__ Yin, const float* __restrict__ fx,
+                                             const float* __restrict__ fy, float dts, int i, int j,
+                                             float& To, float4& Yo) {
+  const int k = j * nx + i;
+  const float T0 = Tin[k];
+  const float4 Y0 = Yin[k];
+  float sT = K(0);
+  float4 sY = make_float4(0.f, 0.f, 0.f, 0.f);
+#define CF_DIFF_NB(cond, fidx, farr, n)                                       \
+  if (cond) {                                                                 \
+    const float f = farr[fidx];                                               \
+    if (f > K(0)) {                                                           \
+      const float Tn = Tin[n];                                                \
+      const float4 Yn = Yin[n];                                               \
+      const float a = f * rhoDOf(P, K(0.5) * (T0 + Tn));                      \
+      sT += a * (Tn - T0);                                                    \
+      sY.x += a * (Yn.x - Y0.x); sY.y += a * (Yn.y - Y0.y);                   \
+      sY.z += a * (Yn.z - Y0.z); sY.w += a * (Yn.w - Y0.w);                   \
+    }                                                                         \
+  }
+  CF_DIFF_NB(i > 0, j * (nx + 1) + i, fx, k - 1)          // west
+  CF_DIFF_NB(i < nx - 1, j * (nx + 1) + i + 1, fx, k + 1) // east
+  CF_DIFF_NB(j > 0, j * nx + i, fy, k - nx)               // south
+  CF_DIFF_NB(j < ny - 1, (j + 1) * nx + i, fy, k + nx)    // north
+#undef CF_DIFF_NB
+  const float c = dts / (rhoOf(P, T0) * P.dx * P.dx);
+  To = T0 + c * sT;
+  Yo = make_float4(fmaxf(Y0.x + c * sY.x * K(1), K(0)), fmaxf(Y0.y + c * sY.y * K(1), K(0)),
+                   fmaxf(Y0.z + c * sY.z * K(1), K(0)), fmaxf(Y0.w + c * sY.w * K(0), K(0)));
+}
+
+__global__ __launch_bounds__(256) void k_diffuse(Fields F, const float* __restrict__ Tin, const float4* __restrict__ Yin,
+                                                 float* __restrict__ Tout, float4* __restrict__ Yout, float dts) {
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx, ny, i, j);
+  if (i >= nx || j >= ny) return;
+  float To;
+  float4 Yo;
+  diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, dts, i, j, To, Yo);
+  Tout[j * nx + i] = To;
+  Yout[j * nx + i] = Yo;
+}
+
+// -------------------------------------------------------------------------
+// pointwise sources of one cell (shaders/sources.wgsl:41-157)
+// -------------------------------------------------------------------------
+__device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F, int i, int j, float Tc, float4 Yc,
+                                             float& Tout, float4& Yout, float2& SQout,
+                                             float& qrel, float& qrad, float& tpeak) {
+  const int nx = P.nx;
+  const int k = j * nx + i;
+  const float dx = P.dx, dt = P.dt, cp = P.cp, sO2 = P.sO2, dHc = P.dHc;
+  const float2 g = F.cellG[k];
+  const float sdf = g.x;
+  // ids come from the geometry pass (piece*4 + face) or are SOLID_NONE;
+  // anything else is treated as no solid rather than indexed
+  uint32_t sid = __float_as_uint(g.y);
+  if (sid >= (uint32_t)(CF_MAX_PIECES * 4)) sid = CF_SOLID_NONE;
+  float massSrc = K(0);
+  const float cellA = dx * dx;
+  qrel = K(0); qrad = K(0); tpeak = K(0);
+  if (sdf < K(0) && sid != CF_SOLID_NONE) {
+    // inside wood: hold the face temperature (:50-55)
+    Tout = F.pieces[(sid >> 2) * CF_PIECE_FLOATS + 8 + (sid & 3u)];
+    Yout = Yc;
+    SQout = make_float2(0.f, 0.f);
+    return;
+  }
+  if (sid != CF_SOLID_NONE && sdf < K(2.0) * dx) {
+    // wood coupling over the interface band (:57-94)
+    const uint32_t pi = sid >> 2, fi = sid & 3u;
+    const float* pc = F.pieces + pi * CF_PIECE_FLOATS;
+    const float Ts = pc[8 + fi];
+    const float band = (float)F.faceW[sid] / K(1.0e8);
+    const float w = fmaxf(K(0), K(1) - fabsf(sdf - dx) / dx) / dx * (K(2.0) * pc[3]) / fmaxf(band, K(1e-6));
+    const float rho = rhoOf(P, Tc);
+    const float kg = rhoDOf(P, K(0.5) * (Tc + Ts)) * cp;
+    const float dist = fmaxf(sdf, K(0.5) * dx);
+    const float rate = kg * w / (dist * rho * cp);
+    const float dTc = (Ts - Tc) * (K(1) - expf(-rate * dt));
+    Tc += dTc;
+    const float toWood = -rho * cp * dTc * cellA;
+    atomicAdd(&F.acc[(pi * 4u + fi) * 2u], iround(toWood * K(1e6)));
+    const float mf = pc[12 + fi] * w;
+    if (mf > K(0)) {
+      const float madd = mf * dt;
+      const float r = rhoOf(P, Tc);
+      Yc = make_float4((r * Yc.x + madd) / (r + madd), (r * Yc.y + K(0)) / (r + madd),
+                       (r * Yc.z + K(0)) / (r + madd), (r * Yc.w + K(0)) / (r + madd));
+      Tc = (r * Tc + madd * Ts) / (r + madd);
+      massSrc += mf;
+    }
+    const float cov = pc[16 + fi];
+    if (cov > K(0) && Yc.y > K(0)) {
+      const float r = rhoOf(P, Tc);
+      const float kc = P.charA * expf(-P.charTa / Ts) * cov;
+      const float mO2 = fminf(kc * r * Yc.y * w * dt, K(0.9) * r * Yc.y);
+      const float mC = mO2 / P.charO2;
+      Yc = make_float4((r * Yc.x + K(0)) / (r + mC), (r * Yc.y + -mO2) / (r + mC),
+                       (r * Yc.z + (mC + mO2)) / (r + mC), (r * Yc.w + K(0)) / (r + mC));
+      Tc = (r * Tc + mC * Ts) / (r + mC);
+      massSrc += mC / dt;
+      atomicAdd(&F.acc[(pi * 4u + fi) * 2u + 1u], iround(mC * cellA * K(1e12)));
+    }
+  }
+  // test sources (:96-107)
+  const float x = ((float)i + K(0.5)) * dx, y = ((float)j + K(0.5)) * dx;
+  if (P.heaterT > K(0) && x > P.heaterX0 && x < P.heaterX1 && y < P.heaterY1) Tc = P.heaterT;
+  if (P.burnerFlux > K(0) && j == 0 && x > P.burnerX0 && x < P.burnerX1) {
+    const float mf = P.burnerFlux / dx;
+    const float madd = mf * dt;
+    const float r = rhoOf(P, Tc);
+    Yc = make_float4((r * Yc.x + madd) / (r + madd), (r * Yc.y + K(0)) / (r + madd),
+                     (r * Yc.z + K(0)) / (r + madd), (r * Yc.w + K(0)) / (r + madd));
+    Tc = (r * Tc + madd * P.burnerT) / (r + madd);
+    massSrc += mf;
+  }
+  // one-step Arrhenius, 8 substeps with rho frozen (:109-120)
+  const float rho0 = rhoOf(P, Tc);
+  const float h = dt / (float)8;
+  float burned = K(0);
+#pragma unroll 1
+  for (int s = 0; s < 8; s++) {
+    const float wr = P.reactB * rho0 * Yc.x * Yc.y * expf(-P.reactTa / Tc);
+    const float dF = fminf(wr * h, fminf(Yc.x, Yc.y / sO2));
+    Yc.x += -dF; Yc.y += -sO2 * dF; Yc.z += (K(1) + sO2) * dF; Yc.w += K(0);
+    Tc += dF * dHc / cp;
+    burned += dF;
+  }
+  // soot formation and oxidation (:122-127)
+  const float dSf = fminf(P.sootFormA * expf(-P.sootFormTa / Tc) * Yc.x * dt, Yc.x);
+  Yc.x += -dSf; Yc.y += K(0); Yc.z += K(0); Yc.w += dSf;
+  const float dSo = fminf(P.sootOxA * Yc.w * Yc.y * expf(-P.sootOxTa / Tc) * dt, fminf(Yc.w, Yc.y / sO2));
+  Yc.x += K(0); Yc.y += -sO2 * dSo; Yc.z += (K(1) + sO2) * dSo; Yc.w += -dSo;
+  Tc += dSo * dHc / cp;
+  burned += dSo;
+  qrel = rho0 * burned * dHc / dt;
+  // optically thin loss, linearised implicit (:130-136)
+  const float r1 = rhoOf(P, Tc);
+  const float fv = r1 * Yc.w / P.sootDensity;
+  const float kap = P.kappaSootC * fv * Tc + P.kappaGas * Yc.z;
+  const float T3 = Tc * Tc * Tc;
+  qrad = K(4.0) * kap * P.sigma * (T3 * Tc - powf(P.Tamb, K(4.0)));
+  const float dTr = qrad * dt / (r1 * cp) / (K(1) + K(16.0) * kap * P.sigma * T3 * dt / (r1 * cp));
+  Tc -= dTr;
+  // entrainment through the front/back faces (:140-148)
+  const float ucx = K(0.5) * (F.U[j * (nx + 1) + i] + F.U[j * (nx + 1) + i + 1]);
+  const float ucy = K(0.5) * (F.V[j * nx + i] + F.V[(j + 1) * nx + i]);
+  const float me = K(2.0) * P.exchAlpha * sqrtf(ucx * ucx + ucy * ucy) * P.rhoAmb / P.depth;
+  if (me > K(0)) {
+    const float madd = me * dt;
+    const float r = rhoOf(P, Tc);
+    Yc = make_float4((r * Yc.x + madd * K(0)) / (r + madd), (r * Yc.y + madd * P.YOamb) / (r + madd),
+                     (r * Yc.z + madd * K(0)) / (r + madd), (r * Yc.w + madd * K(0)) / (r + madd));
+    Tc = (r * Tc + madd * P.Tamb) / (r + madd);
+    massSrc += me;
+  }
+  // write (:150-156)
+  Tc = fmaxf(Tc, K(0.5) * P.Tamb);
+  Yc = make_float4(fmaxf(Yc.x, K(0)), fmaxf(Yc.y, K(0)), fmaxf(Yc.z, K(0)), fmaxf(Yc.w, K(0)));
+  const float S = massSrc / rhoOf(P, Tc) + logf(Tc / F.T2[k]) / dt;
+  Tout = Tc;
+  Yout = Yc;
+  tpeak = Tc;
+  SQout = make_float2(S, qrel);
+}
+
+// sources, optionally fused with the last diffusion substep (which then reads
+// Tin/Yin): a pointwise composition, identical to running the two in turn.
+template <bool FUSE>
+__global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restrict__ Tin,
+                                                 const float4* __restrict__ Yin, float dts) {
+  __shared__ float red[3][8][33];
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx, ny, i, j);
+  float qrel = K(0), qrad = K(0), tpeak = K(0);
+  if (i < nx && j < ny) {
+    const int k = j * nx + i;
+    float Tc;
+    float4 Yc;
+    if (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, dts, i, j, Tc, Yc);
+    else { Tc = F.T[k]; Yc = F.Y[k]; }
+    float To;
+    float4 Yo;
+    float2 sq;
+    sources_cell(P, F, i, j, Tc, Yc, To, Yo, sq, qrel, qrad, tpeak);
+    F.T[k] = To;
+    F.Y[k] = Yo;
+    F.SQ[k] = sq;
+  }
+  // 8x8-tile reduction in the reference's tree order (:160-172)
+  const float dx = P.dx, dt = P.dt;
+  red[0][threadIdx.y][threadIdx.x] = qrel * dx * dx * dt;
+  red[1][threadIdx.y][threadIdx.x] = qrad * dx * dx * dt;
+  red[2][threadIdx.y][threadIdx.x] = tpeak;
+  __syncthreads();
+  const int lin = threadIdx.y * 32 + threadIdx.x;
+  const int w = lin >> 6, lane = lin & 63;
+  const int lx = lane & 7, ly = lane >> 3;
+  float vq = red[0][ly][w * 8 + lx], vr = red[1][ly][w * 8 + lx], vt = red[2][ly][w * 8 + lx];
+  vq = wave_tree_sum(vq);
+  vr = wave_tree_sum(vr);
+  vt = wave_tree_max(vt);
+  if (lane == 0) {
+    atomicAdd(&F.acc[CF_ACC_GLOBAL], iround(vq * K(1e3)));
+    atomicAdd(&F.acc[CF_ACC_GLOBAL + 1], iround(vr * K(1e3)));
+    atomicMax(&F.acc[CF_ACC_GLOBAL + 2], (int)vt);
+  }
+}
+
+// -------------------------------------------------------------------------
+// momentum: viscosity, buoyancy, entrainment drag (shaders/forces.wgsl:34-101)
+// -------------------------------------------------------------------------
+__device__ __forceinline__ float Tcl(const float* __restrict__ T, int nx, int ny, int i, int j) {
+  return T[iclamp(j, 0, ny - 1) * nx + iclamp(i, 0, nx - 1)];
+}
+__device__ __forceinline__ float exchangef(const cf_params& P, float spd, float Tf) {
+  return expf(K(-2.0) * P.exchAlpha * spd * (Tf / P.Tamb) / P.depth * P.dt);
+}
+__global__ __launch_bounds__(256) void k_forces(Fields F) {
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  const float dt = P.dt, dx = P.dx;
+  const float idx2 = K(1) / (dx * dx);
+  const float* __restrict__ T = F.T;
+  const float* __restrict__ U2 = F.U2;
+  const float* __restrict__ V2 = F.V2;
+  const float* __restrict__ fx = F.fx;
+  const float* __restrict__ fy = F.fy;
+  auto u2 = [&](int a, int b) { const int q = b * (nx + 1) + a; return fx[q] > K(0) ? U2[q] : K(0); };
+  auto v2 = [&](int a, int b) { const int q = b * nx + a; return fy[q] > K(0) ? V2[q] : K(0); };
+  if (i <= nx && j < ny) {
+    const int k = j * (nx + 1) + i;
+    float out;
+    if (!(fx[k] > K(0))) out = K(0);
+    else if (i == 0) out = fminf(U2[j * (nx + 1) + 1], K(0));
+    else if (i == nx) out = fmaxf(U2[j * (nx + 1) + nx - 1], K(0));
+    else {
+      const float u = U2[k];
+      const float TL = Tcl(T, nx, ny, i - 1, j), TR = Tcl(T, nx, ny, i, j);
+      const float rhoF = rhoOf(P, K(0.5) * (TL + TR));
+      const float muE = muOf(P, TR), muW = muOf(P, TL);
+      float uN = u, muN = muOf(P, K(0.5) * (TL + TR));
+      if (j < ny - 1) { uN = u2(i, j + 1); muN = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j + 1) + Tcl(T, nx, ny, i, j + 1))); }
+      float uS = -u, muS = muOf(P, K(0.5) * (TL + TR));
+      if (j > 0) { uS = u2(i, j - 1); muS = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i, j - 1))); }
+      const float visc = (muE * (u2(i + 1, j) - u) + muW * (u2(i - 1, j) - u) + muN * (uN - u) + muS * (uS - u)) * idx2 / rhoF;
+      const float vAvg = K(0.25) * (V2[j * nx + i - 1] + V2[j * nx + i] + V2[(j + 1) * nx + i - 1] + V2[(j + 1) * nx + i]);
+      out = (u + dt * visc) * exchangef(P, sqrtf(u * u + vAvg * vAvg), K(0.5) * (TL + TR));
+    }
+    F.U[k] = out;
+  }
+  if (i < nx && j <= ny) {
+    const int k = j * nx + i;
+    float out;
+    if (j == 0 || !(fy[k] > K(0))) out = K(0);
+    else if (j == ny) out = fmaxf(V2[(ny - 1) * nx + i], K(0));
+    else {
+      const float v = V2[k];
+      const float TB = Tcl(T, nx, ny, i, j - 1), TT = Tcl(T, nx, ny, i, j);
+      const float Tf = K(0.5) * (TB + TT);
+      const float rhoF = rhoOf(P, Tf);
+      const float muN = muOf(P, TT), muS = muOf(P, TB);
+      float vE = v, muE = muOf(P, Tf);
+      if (i < nx - 1) { vE = v2(i + 1, j); muE = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i + 1, j - 1) + Tcl(T, nx, ny, i + 1, j))); }
+      float vW = v, muW = muOf(P, Tf);
+      if (i > 0) { vW = v2(i - 1, j); muW = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i - 1, j))); }
+      const float visc = (muN * (v2(i, j + 1) - v) + muS * (v2(i, j - 1) - v) + muE * (vE - v) + muW * (vW - v)) * idx2 / rhoF;
+      const float buoy = P.grav * (Tf / P.Tamb - K(1));
+      const float uAvg = K(0.25) * (U2[(j - 1) * (nx + 1) + i] + U2[(j - 1) * (nx + 1) + i + 1] + U2[j * (nx + 1) + i] + U2[j * (nx + 1) + i + 1]);
+      out = (v + dt * (visc + buoy)) * exchangef(P, sqrtf(uAvg * uAvg + v * v), Tf);
+    }
+    F.V[k] = out;
+  }
+}
+
+// -------------------------------------------------------------------------
+// projection set-up: coef + rhs0 in one pass (shaders/pressure.wgsl:30-54)
+// -------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_coef_rhs(Fields F) {
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  const float* __restrict__ T = F.T;
+  if (i <= nx && j < ny) {
+    const int k = j * (nx + 1) + i;
+    const float Tf = K(0.5) * (T[j * nx + iclamp(i - 1, 0, nx - 1)] + T[j * nx + min(i, nx - 1)]);
+    F.ax0[k] = F.fx[k] * invRho(P, Tf);
+  }
+  if (i < nx && j <= ny) {
+    const int k = j * nx + i;
+    const float Tf = K(0.5) * (T[iclamp(j - 1, 0, ny - 1) * nx + i] + T[min(j, ny - 1) * nx + i]);
+    F.ay0[k] = F.fy[k] * invRho(P, Tf);
+  }
+  if (i < nx && j < ny) {
+    const float fe = F.fx[j * (nx + 1) + i + 1], fw = F.fx[j * (nx + 1) + i];
+    const float fn = F.fy[(j + 1) * nx + i], fs = F.fy[j * nx + i];
+    const float div = (fe * F.U[j * (nx + 1) + i + 1] - fw * F.U[j * (nx + 1) + i] + fn * F.V[(j + 1) * nx + i] -
+                       fs * F.V[j * nx + i]) / P.dx;
+    const float open = fe + fw + fn + fs;
+    F.rhs0[j * nx + i] = open > K(0) ? (div - F.SQ[j * nx + i].x) / P.dt : K(0);
+  }
+}
+
+// coefficient restriction to the next coarser level (:57-67)
+__global__ __launch_bounds__(256) void k_coef_restrict(Level f, Level c) {
+  int I, J;
+  tile_ij<32, 8>(c.nx + 1, c.ny + 1, I, J);
+  const int nx = f.nx, nxc = c.nx, nyc = c.ny;
+  if (I <= nxc && J < nyc) c.ax[J * (nxc + 1) + I] = K(0.5) * (f.ax[(2 * J) * (nx + 1) + 2 * I] + f.ax[(2 * J + 1) * (nx + 1) + 2 * I]);
+  if (I < nxc && J <= nyc) c.ay[J * nxc + I] = K(0.5) * (f.ay[(2 * J) * nx + 2 * I] + f.ay[(2 * J) * nx + 2 * I + 1]);
+}
+
+// 5-point stencil: sum a*p_nb and the diagonal (:77-85)
+__device__ __forceinline__ void stencil(const Level& L, const float* __restrict__ p, int i, int j, float& off, float& diag) {
+  const int nx = L.nx, ny = L.ny;
+  const float w = L.ax[j * (nx + 1) + i], e = L.ax[j * (nx + 1) + i + 1];
+  const float s = L.ay[j * nx + i], n = L.ay[(j + 1) * nx + i];
+  off = K(0); diag = K(0);
+  if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else { diag += K(2.0) * w; }
+  if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else { diag += K(2.0) * e; }
+  if (j > 0) { off += s * p[(j - 1) * nx + i]; diag += s; }
+  if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else { diag += K(2.0) * n; }
+}
+
+// one red or black half-sweep (:87-94); thread per cell of that colour
+__global__ __launch_bounds__(256) void k_relax(Level L, int color) {
+  int ii, j;
+  tile_ij<32, 8>((L.nx + 1) / 2, L.ny, ii, j);
+  const int i = 2 * ii + ((j + color) & 1);
+  if (i >= L.nx || j >= L.ny) return;
+  float off, diag;
+  stencil(L, L.p, i, j, off, diag);
+  const int k = j * L.nx + i;
+  L.p[k] = diag > K(0) ? (off - L.rhs[k] * L.h2) / diag : K(0);
+}
+
+__device__ __forceinline__ float residual(const Level& L, int i, int j) {
+  float off, diag;
+  stencil(L, L.p, i, j, off, diag);
+  if (diag <= K(0)) return K(0);
+  return L.rhs[j * L.nx + i] - (off - diag * L.p[j * L.nx + i]) / L.h2;
+}
+
+// residual restricted to the coarse rhs; coarse p = 0 (:104-113)
+__global__ __launch_bounds__(256) void k_restrict_res(Level f, Level c) {
+  int I, J;
+  tile_ij<32, 8>(c.nx, c.ny, I, J);
+  if (I >= c.nx || J >= c.ny) return;
+  const float r = residual(f, 2 * I, 2 * J) + residual(f, 2 * I + 1, 2 * J) + residual(f, 2 * I, 2 * J + 1) +
+                  residual(f, 2 * I + 1, 2 * J + 1);
+  c.rhs[J * c.nx + I] = K(0.25) * r;
+  c.p[J * c.nx + I] = K(0);
+}
+
+// bilinear coarse correction (:116-129)
+__global__ __launch_bounds__(256) void k_prolong(Level f, Level c) {
+  int i, j;
+  tile_ij<32, 8>(f.nx, f.ny, i, j);
+  if (i >= f.nx || j >= f.ny) return;
+  const int nxc = c.nx, nyc = c.ny;
+  const int I = i / 2, J = j / 2;
+  const int I2 = iclamp(I + ((i & 1) == 1 ? 1 : -1), 0, nxc - 1);
+  const int J2 = iclamp(J + ((j & 1) == 1 ? 1 : -1), 0, nyc - 1);
+  const float* __restrict__ pc = c.p;
+  const float cc = K(0.5625) * pc[J * nxc + I] + K(0.1875) * (pc[J * nxc + I2] + pc[J2 * nxc + I]) + K(0.0625) * pc[J2 * nxc + I2];
+  const int nx = f.nx;
+  // same diagonal as stencil(): w-term, e-term, then s, n (order kept)
+  float d2 = K(0);
+  {
+    const float w = f.ax[j * (nx + 1) + i], e = f.ax[j * (nx + 1) + i + 1];
+    const float s = f.ay[j * nx + i], n = f.ay[(j + 1) * nx + i];
+    if (i > 0) d2 += w; else d2 += K(2.0) * w;
+    if (i < nx - 1) d2 += e; else d2 += K(2.0) * e;
+    if (j > 0) d2 += s;
+    if (j < f.ny - 1) d2 += n; else d2 += K(2.0) * n;
+  }
+  f.p[j * nx + i] += d2 > K(0) ? cc : K(0);
+}
+
+// coarsest level: `iters` red-black sweeps from zero in one workgroup (:131-156)
+__global__ __launch_bounds__(256) void k_coarsest(Level L, int iters) {
+  __shared__ float wp[256];
+  const int n = L.nx * L.ny;
+  const int k = threadIdx.x;
+  const int nx = L.nx, ny = L.ny;
+  const int i = k % nx, j = k / nx;
+  float w = 0, e = 0, s = 0, nn = 0, r = 0;
+  if (k < n) {
+    wp[k] = K(0);
+    w = L.ax[j * (nx + 1) + i]; e = L.ax[j * (nx + 1) + i + 1];
+    s = L.ay[j * nx + i]; nn = L.ay[(j + 1) * nx + i];
+    r = L.rhs[k] * L.h2;
+  }
+  __syncthreads();
+  for (int it = 0; it < iters; it++) {
+    for (int c = 0; c < 2; c++) {
+      if (k < n && ((i + j) & 1) == c) {
+        float off = K(0), diag = K(0);
+        if (i > 0) { off += w * wp[k - 1]; diag += w; } else { diag += K(2.0) * w; }
+        if (i < nx - 1) { off += e * wp[k + 1]; diag += e; } else { diag += K(2.0) * e; }
+        if (j > 0) { off += s * wp[k - nx]; diag += s; }
+        if (j < ny - 1) { off += nn * wp[k + nx]; diag += nn; } else { diag += K(2.0) * nn; }
+        wp[k] = diag > K(0) ? (off - r) / diag : K(0);
+      }
+      __syncthreads();
+    }
+  }
+  if (k < n) L.p[k] = wp[k];
+}
+
+// velocity correction u -= dt (1/rho) grad p (:159-180)
+__global__ __launch_bounds__(256) void k_project(Fields F) {
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  const float dt = P.dt, dx = P.dx;
+  const float* __restrict__ p = F.p0;
+  const float* __restrict__ T = F.T;
+  if (i <= nx && j < ny) {
+    const int k = j * (nx + 1) + i;
+    if (F.fx[k] > K(0)) {
+      const float pl = i > 0 ? p[j * nx + max(i - 1, 0)] : -p[j * nx];
+      const float pr = i < nx ? p[j * nx + min(i, nx - 1)] : -p[j * nx + nx - 1];
+      const float Tf = K(0.5) * (T[j * nx + iclamp(i - 1, 0, nx - 1)] + T[j * nx + min(i, nx - 1)]);
+      F.U[k] -= dt * invRho(P, Tf) * (pr - pl) / dx;
+    } else F.U[k] = K(0);
+  }
+  if (i < nx && j <= ny) {
+    const int k = j * nx + i;
+    if (j > 0 && F.fy[k] > K(0)) {
+      const float pb = p[(j - 1) * nx + i];
+      const float pt = j < ny ? p[min(j, ny - 1) * nx + i] : -p[(ny - 1) * nx + i];
+      const float Tf = K(0.5) * (T[(j - 1) * nx + i] + T[min(j, ny - 1) * nx + i]);
+      F.V[k] -= dt * invRho(P, Tf) * (pt - pb) / dx;
+    } else F.V[k] = K(0);
+  }
+}
+
+// -------------------------------------------------------------------------
+// geometry: rasterise the pieces (shaders/geometry.wgsl:20-52, common.wgsl:47-62)
+// -------------------------------------------------------------------------
+__device__ __forceinline__ float nearest(const cf_params& P, const float* __restrict__ pieces, float px, float py,
+                                         uint32_t& bid) {
+  float best = K(1e9);
+  bid = CF_SOLID_NONE;
+  const int np = min(P.nPieces, CF_MAX_PIECES);
+  for (int n = 0; n < np; n++) {
+    const float* pc = pieces + n * CF_PIECE_FLOATS;
+    if (!(pc[7] > K(0))) continue;
+    const float c = cosf(pc[2]), s = sinf(pc[2]);
+    const float dxp = px - pc[0], dyp = py - pc[1];
+    const float lx = c * dxp + s * dyp, ly = -s * dxp + c * dyp;
+    const float qx = fabsf(lx) - pc[3], qy = fabsf(ly) - pc[3];
+    const float mxq = fmaxf(qx, K(0)), myq = fmaxf(qy, K(0));
+    const float d = sqrtf(mxq * mxq + myq * myq) + fminf(fmaxf(qx, qy), K(0));
+    if (d < best) {
+      best = d;
+      uint32_t face;
+      if (fabsf(lx) > fabsf(ly)) face = lx > K(0) ? 0u : 2u;
+      else face = ly > K(0) ? 1u : 3u;
+      bid = (uint32_t)n * 4u + face;
+    }
+  }
+  return best;
+}
+__global__ __launch_bounds__(256) void k_geometry(Fields F) {
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  const float dx = P.dx;
+  uint32_t id;
+  if (i < nx && j < ny) {
+    const float d = nearest(P, F.pieces, ((float)i + K(0.5)) * dx, ((float)j + K(0.5)) * dx, id);
+    F.cellG[j * nx + i] = make_float2(d, __uint_as_float(id));
+    if (id != CF_SOLID_NONE && d >= K(0) && d < K(2.0) * dx) {
+      const float w = fmaxf(K(0), K(1) - fabsf(d - dx) / dx) / dx;
+      atomicAdd(&F.faceW[id], iround(w * dx * dx * K(1.0e8)));
+    }
+  }
+  if (i <= nx && j < ny) {
+    const float d = nearest(P, F.pieces, (float)i * dx, ((float)j + K(0.5)) * dx, id);
+    F.fx[j * (nx + 1) + i] = clampf(K(0.5) + d / dx, K(0), K(1));
+  }
+  if (i < nx && j <= ny) {
+    const float d = nearest(P, F.pieces, ((float)i + K(0.5)) * dx, (float)j * dx, id);
+    F.fy[j * nx + i] = j == 0 ? K(0) : clampf(K(0.5) + d / dx, K(0), K(1));
+  }
+}
+
+// -------------------------------------------------------------------------
+// radiation onto wood faces and floor probes (shaders/radiation.wgsl:28-158)
+// one wave64 per target, one stratified ray per lane
+// -------------------------------------------------------------------------
+#define CF_PI K(3.14159265)
+__device__ __forceinline__ float fractf_(float x) { return x - floorf(x); }
+__device__ __forceinline__ float planck580(float T) {
+  const float l = K(580e-9);
+  const float x = K(1.438777e-2) / (l * T);
+  if (x > K(80.0)) return K(0);
+  return K(1.191042e-16) / (l * l * l * l * l) / (expf(x) - K(1)) * K(1e-9);
+}
+__device__ __forceinline__ float hitBox(float ox, float oy, float dxr, float dyr, const float* pc, float& face) {
+  const float c = cosf(pc[2]), s = sinf(pc[2]);
+  const float qx = ox - pc[0], qy = oy - pc[1];
+  const float lox = c * qx + s * qy, loy = -s * qx + c * qy;
+  const float ldx = c * dxr + s * dyr, ldy = -s * dxr + c * dyr;
+  const float h = pc[3];
+  const float invx = K(1) / (fabsf(ldx) < K(1e-9) ? K(1e-9) : ldx);
+  const float invy = K(1) / (fabsf(ldy) < K(1e-9) ? K(1e-9) : ldy);
+  const float t1x = (-h - lox) * invx, t1y = (-h - loy) * invy;
+  const float t2x = (h - lox) * invx, t2y = (h - loy) * invy;
+  const float tminx = fminf(t1x, t2x), tminy = fminf(t1y, t2y), tmaxx = fmaxf(t1x, t2x), tmaxy = fmaxf(t1y, t2y);
+  const float tn = fmaxf(tminx, tminy), tf = fminf(tmaxx, tmaxy);
+  if (tf < K(0) || tn > tf || tn < K(1e-5)) { face = K(0); return K(-1); }
+  if (tminx > tminy) face = ldx < K(0) ? K(0) : K(2);
+  else face = ldy < K(0) ? K(1) : K(3);
+  return tn;
+}
+__device__ __forceinline__ void gasAt(const cf_params& P, const float* __restrict__ T, const float4* __restrict__ Y,
+                                      float px, float py, float& g0, float& g1) {
+  const int i = toint(px / P.dx), j = toint(py / P.dx);
+  if (i < 0 || j < 0 || i >= P.nx || j >= P.ny) { g0 = K(0); g1 = K(0); return; }
+  const int k = j * P.nx + i;
+  const float Tg = T[k];
+  const float4 Yg = Y[k];
+  const float fv = rhoOf(P, Tg) * Yg.w / P.sootDensity;
+  const float kap = P.kappaSootC * fv * Tg + P.kappaGas * Yg.z;
+  const float T2 = Tg * Tg;
+  g0 = kap * P.sigma * (T2 * T2 - powf(P.Tamb, K(4.0))) / CF_PI;
+  g1 = K(6.3) * fv / K(580e-9) * planck580(Tg);
+}
+__global__ __launch_bounds__(64) void k_radiation(Fields F) {
+  const cf_params P = *F.P;
+  const int tgt = blockIdx.x;
+  const int li = threadIdx.x;
+  const int nFaces = min(P.nPieces, CF_MAX_PIECES) * 4;
+  float ox = 0, oy = 0, nrx = 0, nry = 1;
+  bool valid = false;
+  if (tgt < nFaces) {
+    const float* pc = F.pieces + (tgt / 4) * CF_PIECE_FLOATS;
+    const int f = tgt % 4;
+    if (pc[7] > K(0)) {
+      const float c = cosf(pc[2]), s = sinf(pc[2]);
+      float lnx = 1, lny = 0;
+      if (f == 1) { lnx = 0; lny = 1; } else if (f == 2) { lnx = -1; lny = 0; } else if (f == 3) { lnx = 0; lny = -1; }
+      nrx = c * lnx - s * lny; nry = s * lnx + c * lny;
+      const float tgx = -nry, tgy = nrx;
+      const float u = (fractf_((float)li * K(0.618034) + (float)P.frame * K(0.1234)) - K(0.5)) * K(1.6) * pc[3];
+      const float hh = pc[3] + K(1e-4);
+      ox = pc[0] + nrx * hh + tgx * u;
+      oy = pc[1] + nry * hh + tgy * u;
+      valid = true;
+    }
+  } else {
+    const int fi = tgt - nFaces;
+    if (fi < CF_N_FLOOR) {
+      ox = ((float)fi + K(0.5)) / (float)CF_N_FLOOR * (float)P.nx * P.dx;
+      oy = K(1e-4);
+      nrx = 0; nry = 1;
+      valid = true;
+    }
+  }
+  float dq = K(0), dv = K(0);
+  if (valid) {
+    const float jit = fractf_((float)P.frame * K(0.7548777) + (float)tgt * K(0.5698403));
+    const float a = (((float)li + jit) / K(64) - K(0.5)) * CF_PI;
+    const float cosn = cosf(a);
+    const float base = atan2f(nry, nrx);
+    float dxr = cosf(base + a), dyr = sinf(base + a);
+    float J = K(0), Jv = K(0);
+    float Twall = P.Tamb;
+    bool bounced = false;
+    const float ds = P.dx;
+    const float W = (float)P.nx * P.dx, H = (float)P.ny * P.dx;
+    for (int step = 0; step < 4096; step++) {
+      float tHit = K(1e9), hitT = K(0);
+      for (int n = 0; n < min(P.nPieces, CF_MAX_PIECES); n++) {
+        const float* pc = F.pieces + n * CF_PIECE_FLOATS;
+        if (!(pc[7] > K(0))) continue;
+        float face;
+        const float hx = hitBox(ox, oy, dxr, dyr, pc, face);
+        if (hx > K(0) && hx < tHit) { tHit = hx; hitT = pc[8 + (int)face]; }
+      }
+      float tFloor = K(1e9);
+      if (dyr < K(-1e-6)) tFloor = -oy / dyr;
+      const float tEnd = fminf(tHit, tFloor);
+      float tx = K(1e9), ty = K(1e9);
+      if (dxr > K(1e-6)) tx = (W - ox) / dxr; else if (dxr < K(-1e-6)) tx = -ox / dxr;
+      if (dyr > K(1e-6)) ty = (H - oy) / dyr;
+      const float tOut = fminf(fminf(tx, ty), tEnd);
+      const int ns = toint(ceilf(tOut / ds));
+      for (int m = 0; m < ns; m++) {
+        const float t = ((float)m + K(0.5)) * tOut / (float)ns;
+        float g0, g1;
+        gasAt(P, F.T, F.Y, ox + dxr * t, oy + dyr * t, g0, g1);
+        J += g0 * tOut / (float)ns;
+        Jv += g1 * tOut / (float)ns;
+      }
+      if (tHit < tFloor && tHit < K(1e8)) { Twall = hitT; break; }
+      if (tFloor < K(1e8) && tFloor <= tHit && !bounced) {
+        ox = ox + dxr * tFloor + K(0);
+        oy = oy + dyr * tFloor + K(1e-5);
+        dyr = -dyr;
+        bounced = true;
+        continue;
+      }
+      Twall = P.Tamb;
+      break;
+    }
+    const float w = CF_PI / K(64) * cosn;
+    const float T2 = Twall * Twall;
+    dq = w * (K(0.5) * P.sigma * T2 * T2 + K(2.0) * J);
+    dv = w * (K(2.0) * Jv);
+  }
+  dq = wave_tree_sum(dq);
+  dv = wave_tree_sum(dv);
+  if (li == 0 && valid) {
+    const float4 old = F.rad[tgt];
+    const float a = old.x <= K(0) ? K(1.0) : K(0.35);
+    F.rad[tgt] = make_float4(mixf(old.x, dq, a), mixf(old.y, dv, a), K(0), K(0));
+  }
+}
+
+// -------------------------------------------------------------------------
+// utilities
+// -------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; q < n; q += stride) a[q] = v;
+}
+
+// |div(f u) - S| after projection (the H.divErr diagnostic, dev/gas_test.html:101-111):
+// per-workgroup max (as uint bits) and sum
+__global__ __launch_bounds__(256) void k_div_error(Fields F, float* __restrict__ part) {
+  __shared__ float smax[256], ssum[256];
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  int i, j;
+  tile_ij<32, 8>(nx, ny, i, j);
+  float e = K(0);
+  if (i < nx && j < ny) {
+    const int iu = j * (nx + 1) + i, iv = j * nx + i;
+    const float div = (F.fx[iu + 1] * F.U[iu + 1] - F.fx[iu] * F.U[iu] + F.fy[iv + nx] * F.V[iv + nx] - F.fy[iv] * F.V[iv]) / P.dx;
+    e = fabsf(div - F.SQ[iv].x);
+  }
+  const int t = threadIdx.y * 32 + threadIdx.x;
+  smax[t] = e; ssum[t] = e;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) { smax[t] = fmaxf(smax[t], smax[t + s]); ssum[t] += ssum[t + s]; }
+    __syncthreads();
+  }
+  if (t == 0) { part[2 * blockIdx.x] = smax[0]; part[2 * blockIdx.x + 1] = ssum[0]; }
+}
+
+// ---- launchers ----------------------------------------------------------
+static inline unsigned nblk(int ex, int ey) { return (unsigned)(((ex + 31) / 32) * ((ey + 7) / 8)); }
+static const dim3 B328(32, 8);
+
+void launch_advect_fwd(const Fields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_advect_fwd, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+}
+void launch_advect_corr(const Fields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_advect_corr, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+}
+void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s) {
+  hipLaunchKernelGGL(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout, dts);
+}
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, hipStream_t s) {
+  if (fuse) hipLaunchKernelGGL(k_sources<true>, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, dts);
+  else hipLaunchKernelGGL(k_sources<false>, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, dts);
+}
+void launch_forces(const Fields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+}
+void launch_coef_rhs(const Fields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+}
+void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_coef_restrict, dim3(nblk(c.nx + 1, c.ny + 1)), B328, 0, s, f, c);
+}
+void launch_relax(const Level& L, int color, hipStream_t s) {
+  hipLaunchKernelGGL(k_relax, dim3(nblk((L.nx + 1) / 2, L.ny)), B328, 0, s, L, color);
+}
+void launch_restrict_res(const Level& f, const Level& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict_res, dim3(nblk(c.nx, c.ny)), B328, 0, s, f, c);
+}
+void launch_prolong(const Level& f, const Level& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_prolong, dim3(nblk(f.nx, f.ny)), B328, 0, s, f, c);
+}
+void launch_coarsest(const Level& L, int iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_coarsest, dim3(1), dim3(256), 0, s, L, iters);
+}
+void launch_project(const Fields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_project, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+}
+void launch_geometry(const Fields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_geometry, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+}
+void launch_radiation(const Fields& F, int ntargets, hipStream_t s) {
+  if (ntargets > 0) hipLaunchKernelGGL(k_radiation, dim3(ntargets), dim3(64), 0, s, F);
+}
+void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  unsigned b = (unsigned)((n + 255) / 256);
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(256), 0, s, (uint32_t*)a, v, n);
+}
+unsigned div_error_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
+void launch_div_error(const Fields& F, float* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_div_error, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, part);
+}
+
+}  // namespace cf

@@ -1,0 +1,803 @@
+// cf_sim.hip — host side of the C ABI (include/campfire_hip.h).
+//
+// Owns one scene: every device buffer in one allocation, one HIP stream,
+// pinned upload rings for the parameter block and the pieces, the 3-deep
+// pinned readback ring, and the step as a captured hipGraph that is replayed
+// once per step.  The schedule of encodeStep (js/fluid.js:225-253) and of the
+// V-cycle (js/fluid.js:294-312) is kept; consecutive reference dispatches
+// that are pointwise compositions of each other run as one kernel.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "cf_kernels.h"
+
+using cf::Fields;
+using cf::Level;
+
+#define MAXLEV 16
+#define UP_SLOTS 4
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return set_err(CF_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+// PARAM_FIELDS (js/fluid.js:13-24)
+static const char* PNAME[CF_NPARAMS] = {
+    "nx", "ny", "dx", "dt", "Tamb", "rhoAmb", "YOamb", "grav",
+    "cp", "Pr", "viscMult", "diffMult", "reactB", "reactTa", "dHc", "sO2",
+    "sootFormA", "sootFormTa", "sootOxA", "sootOxTa",
+    "kappaSootC", "kappaGas", "exchAlpha", "depth",
+    "charA", "charTa", "nPieces", "frame",
+    "heaterX0", "heaterX1", "heaterY1", "heaterT",
+    "burnerX0", "burnerX1", "burnerFlux", "burnerT",
+    "sootDensity", "sigma", "charO2", "time"};
+static const char PKIND[CF_NPARAMS] = {
+    'i', 'i', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f',
+    'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'i', 'u',
+    'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f', 'f'};
+static_assert(sizeof(cf_params) == 160, "cf_params must be the 160-B uniform");
+
+// Physical constants used by the host (js/config.js:12-47)
+static const double PHYS_Tamb = 293.15;
+static double phys_rhoAmb() { return 101325.0 * 0.0289 / (8.314462 * PHYS_Tamb); }
+
+struct ProfTag {
+  const char* name;
+  double ms;
+  int launches;
+};
+
+struct cf_sim {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int nx = 0, ny = 0, nC = 0, nU = 0, nV = 0;
+  double dx = 0, dt = 0, time = 0;
+  uint32_t frame = 0;
+  double pv[CF_NPARAMS];
+  cf_params P;
+  float dsub = 0;
+  int diffSub = 1;
+  int nlev = 0, lnx[MAXLEV], lny[MAXLEV];
+  float lh2[MAXLEV];
+  cf_schedule sch;
+  void* block = nullptr;
+  size_t block_bytes = 0;
+  Fields F;
+  Level L[MAXLEV];
+  cf_params* dP = nullptr;
+  // upload rings
+  cf_params* hP = nullptr;
+  float* hPieces = nullptr;
+  hipEvent_t evP[UP_SLOTS], evPc[UP_SLOTS];
+  bool evP_used[UP_SLOTS], evPc_used[UP_SLOTS];
+  int upP = 0, upPc = 0;
+  // readback ring (js/fluid.js:157-159)
+  uint8_t* ring = nullptr;
+  hipEvent_t ringEv[3];
+  bool busy[3];
+  int stageIdx = 0;
+  // diagnostics
+  float* dPart = nullptr;
+  std::vector<float> hPart;
+  // graph of one step
+  hipGraphExec_t gexec = nullptr;
+  int gkey[6] = {-1, -1, -1, -1, -1, -1};
+  bool use_graph = true;
+  // profiling
+  bool profiling = false;
+  std::vector<hipEvent_t> pev;
+  std::vector<int> ptag;
+  std::vector<ProfTag> tags;
+  int last_vcycles = 0;
+  float last_resid = -1;
+};
+
+static const size_t RING_BYTES = CF_ACC_LEN * 4 + CF_RAD_LEN * 16 + CF_MAX_PIECES * 4 * 4;
+
+// ---- profiling of launch sites -------------------------------------------------
+static void mark(cf_sim* s, const char* tag) {
+  if (!s->profiling) return;
+  int id = -1;
+  for (size_t q = 0; q < s->tags.size(); q++)
+    if (!strcmp(s->tags[q].name, tag)) { id = (int)q; break; }
+  if (id < 0) { s->tags.push_back({tag, 0.0, 0}); id = (int)s->tags.size() - 1; }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  hipEventRecord(e, s->st);
+  s->pev.push_back(e);
+  s->ptag.push_back(id);
+}
+
+// ---- step schedule ---------------------------------------------------------
+static void enqueue_vcycle(cf_sim* s, int nu) {  // js/fluid.js:294-312
+  const int L = s->nlev;
+  for (int l = 0; l < L - 1; l++) {
+    for (int q = 0; q < nu; q++) {
+      cf::launch_relax(s->L[l], 0, s->st); mark(s, "relax");
+      cf::launch_relax(s->L[l], 1, s->st); mark(s, "relax");
+    }
+    cf::launch_restrict_res(s->L[l], s->L[l + 1], s->st); mark(s, "restrict_res");
+  }
+  cf::launch_coarsest(s->L[L - 1], s->sch.coarse_iters, s->st); mark(s, "coarsest");
+  for (int l = L - 2; l >= 0; l--) {
+    cf::launch_prolong(s->L[l], s->L[l + 1], s->st); mark(s, "prolong");
+    for (int q = 0; q < nu; q++) {
+      cf::launch_relax(s->L[l], 1, s->st); mark(s, "relax");
+      cf::launch_relax(s->L[l], 0, s->st); mark(s, "relax");
+    }
+  }
+}
+
+static void enqueue_coef(cf_sim* s) {
+  cf::launch_coef_rhs(s->F, s->st); mark(s, "coef_rhs");
+  for (int l = 0; l < s->nlev - 1; l++) {
+    cf::launch_coef_restrict(s->L[l], s->L[l + 1], s->st); mark(s, "coef_restrict");
+  }
+}
+
+// stage granularity of encodeStepProfiled (js/fluid.js:256-292), unfused
+static int enqueue_stage(cf_sim* s, int stage) {
+  Fields& F = s->F;
+  switch (stage) {
+    case 0:
+      cf::launch_advect_fwd(F, s->st); mark(s, "advect_fwd");
+      cf::launch_advect_corr(F, s->st); mark(s, "advect_corr");
+      break;
+    case 1:
+      cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
+      for (int q = 1; q < s->diffSub; q += 2) {
+        cf::launch_diffuse(F, F.T, F.Y, F.Th, F.Yh, s->dsub, s->st); mark(s, "diffuse");
+        cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
+      }
+      break;
+    case 2: cf::launch_sources(F, false, nullptr, nullptr, 0.f, s->st); mark(s, "sources"); break;
+    case 3: cf::launch_forces(F, s->st); mark(s, "forces"); break;
+    case 4: enqueue_coef(s); break;
+    case 5:
+      for (int c = 0; c < s->sch.vcycles; c++) enqueue_vcycle(s, s->sch.nu);
+      break;
+    case 6: cf::launch_project(F, s->st); mark(s, "project"); break;
+    default: return set_err(CF_EINVAL, "stage %d out of range 0..6", stage);
+  }
+  return CF_OK;
+}
+
+// one step (js/fluid.js:225-253); the last diffusion substep runs inside sources
+static void enqueue_step(cf_sim* s) {
+  Fields& F = s->F;
+  enqueue_stage(s, 0);
+  if (s->diffSub == 1) {
+    cf::launch_sources(F, true, F.T2, F.Y2, s->dsub, s->st); mark(s, "diffuse_sources");
+  } else {
+    cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
+    for (int q = 1; q < s->diffSub; q += 2) {
+      cf::launch_diffuse(F, F.T, F.Y, F.Th, F.Yh, s->dsub, s->st); mark(s, "diffuse");
+      if (q + 2 < s->diffSub) {
+        cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
+      } else {
+        cf::launch_sources(F, true, F.Th, F.Yh, s->dsub, s->st); mark(s, "diffuse_sources");
+      }
+    }
+  }
+  enqueue_stage(s, 3);
+  enqueue_stage(s, 4);
+  enqueue_stage(s, 5);
+  enqueue_stage(s, 6);
+}
+
+static void drop_graph(cf_sim* s) {
+  if (s->gexec) hipGraphExecDestroy(s->gexec);
+  s->gexec = nullptr;
+  for (int q = 0; q < 6; q++) s->gkey[q] = -1;
+}
+
+static int ensure_graph(cf_sim* s) {
+  const int key[6] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev};
+  if (s->gexec && !memcmp(key, s->gkey, sizeof(key))) return CF_OK;
+  drop_graph(s);
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  enqueue_step(s);
+  HIPCHK(hipStreamEndCapture(s->st, &g));
+  hipError_t e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) { s->gexec = nullptr; return set_err(CF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+  memcpy(s->gkey, key, sizeof(key));
+  return CF_OK;
+}
+
+// ---- helpers -----------------------------------------------------------------
+static int param_index(const char* key) {
+  for (int q = 0; q < CF_NPARAMS; q++) if (!strcmp(PNAME[q], key)) return q;
+  return -1;
+}
+
+static int upload_params(cf_sim* s) {
+  const int slot = s->upP;
+  s->upP = (slot + 1) % UP_SLOTS;
+  if (s->evP_used[slot]) HIPCHK(hipEventSynchronize(s->evP[slot]));
+  s->hP[slot] = s->P;
+  HIPCHK(hipMemcpyAsync(s->dP, &s->hP[slot], sizeof(cf_params), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipEventRecord(s->evP[slot], s->st));
+  s->evP_used[slot] = true;
+  return CF_OK;
+}
+
+struct FieldRef {
+  void* ptr;
+  size_t bytes;
+};
+
+static bool lookup(cf_sim* s, const char* name, FieldRef& r) {
+  const size_t nC = s->nC, nU = s->nU, nV = s->nV;
+  Fields& F = s->F;
+  struct { const char* n; void* p; size_t b; } t[] = {
+      {"U", F.U, nU * 4}, {"Uh", F.Uh, nU * 4}, {"U2", F.U2, nU * 4},
+      {"V", F.V, nV * 4}, {"Vh", F.Vh, nV * 4}, {"V2", F.V2, nV * 4},
+      {"T", F.T, nC * 4}, {"Th", F.Th, nC * 4}, {"T2", F.T2, nC * 4},
+      {"Y", F.Y, nC * 16}, {"Yh", F.Yh, nC * 16}, {"Y2", F.Y2, nC * 16},
+      {"SQ", F.SQ, nC * 8}, {"cellG", F.cellG, nC * 8}, {"fx", F.fx, nU * 4}, {"fy", F.fy, nV * 4},
+      {"pieces", F.pieces, CF_MAX_PIECES * CF_PIECE_FLOATS * 4}, {"acc", F.acc, CF_ACC_LEN * 4},
+      {"rad", F.rad, CF_RAD_LEN * 16}, {"faceW", F.faceW, CF_MAX_PIECES * 4 * 4}};
+  for (auto& e : t)
+    if (!strcmp(e.n, name)) { r.ptr = e.p; r.bytes = e.b; return true; }
+  const char* pre[4] = {"rhs", "ax", "ay", "p"};
+  for (int a = 0; a < 4; a++) {
+    size_t n = strlen(pre[a]);
+    if (strncmp(name, pre[a], n)) continue;
+    const char* rest = name + n;
+    int l = 0;
+    if (*rest) {
+      char* e;
+      l = (int)strtol(rest, &e, 10);
+      if (*e) continue;
+    }
+    if (l < 0 || l >= s->nlev) return false;
+    const size_t lx = s->lnx[l], ly = s->lny[l];
+    Level& Lv = s->L[l];
+    if (a == 0) { r.ptr = Lv.rhs; r.bytes = lx * ly * 4; }
+    else if (a == 1) { r.ptr = Lv.ax; r.bytes = (lx + 1) * ly * 4; }
+    else if (a == 2) { r.ptr = Lv.ay; r.bytes = lx * (ly + 1) * 4; }
+    else { r.ptr = Lv.p; r.bytes = lx * ly * 4; }
+    return true;
+  }
+  return false;
+}
+
+// ---- C ABI -------------------------------------------------------------------
+extern "C" {
+
+const char* cf_last_error(void) { return g_err; }
+const char* cf_version(void) { return "campfire-hip 0.1 (gfx950)"; }
+const char* cf_param_name(int q) { return q >= 0 && q < CF_NPARAMS ? PNAME[q] : nullptr; }
+
+void cf_sim_destroy(cf_sim* s) {
+  if (!s) return;
+  if (s->st) hipStreamSynchronize(s->st);
+  drop_graph(s);
+  for (int q = 0; q < UP_SLOTS; q++) { hipEventDestroy(s->evP[q]); hipEventDestroy(s->evPc[q]); }
+  for (int q = 0; q < 3; q++) hipEventDestroy(s->ringEv[q]);
+  for (auto e : s->pev) hipEventDestroy(e);
+  if (s->hP) hipHostFree(s->hP);
+  if (s->hPieces) hipHostFree(s->hPieces);
+  if (s->ring) hipHostFree(s->ring);
+  if (s->block) hipFree(s->block);
+  if (s->st) hipStreamDestroy(s->st);
+  delete s;
+}
+
+int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
+  if (!out) return set_err(CF_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (nx < 2 || ny < 2 || nx > 16384 || ny > 16384 || !(dx > 0) || !(dt > 0))
+    return set_err(CF_EINVAL, "bad grid %dx%d dx=%g dt=%g", nx, ny, dx, dt);
+  cf_sim* s = new cf_sim();
+  s->nx = nx; s->ny = ny; s->dx = dx; s->dt = dt;
+  s->nC = nx * ny; s->nU = (nx + 1) * ny; s->nV = nx * (ny + 1);
+  // levels: halve while both dims are even and cells > 256 (js/fluid.js:71-91;
+  // the reference's 6-level cap is lifted so 1024^2 reaches a 16x16 coarsest level)
+  {
+    int lx = nx, ly = ny;
+    double h = dx;
+    for (int l = 0; l < MAXLEV; l++) {
+      s->lnx[l] = lx; s->lny[l] = ly; s->lh2[l] = (float)(h * h);
+      s->nlev = l + 1;
+      if (lx % 2 || ly % 2 || lx * ly <= 256) break;
+      lx /= 2; ly /= 2; h *= 2;
+    }
+    if (s->lnx[s->nlev - 1] * s->lny[s->nlev - 1] > 256) {
+      delete s;
+      return set_err(CF_EBADSIZE, "coarsest level too large (%dx%d)", s->lnx[s->nlev - 1], s->lny[s->nlev - 1]);
+    }
+  }
+  // defaults (js/fluid.js:35-45, js/config.js:12-47)
+  const double def[CF_NPARAMS] = {
+      (double)nx, (double)ny, dx, dt, PHYS_Tamb, phys_rhoAmb(), 0.232, 9.81,
+      1300, 0.7, 1, 1, 2.0e10, 15000, 15.0e6, 15.0e6 / 13.1e6,
+      6.0e6, 25000, 5.4e8, 19800,
+      1862, 0.8, 0.05, 0.8,
+      6.6, 5800, 0, 0,
+      0, 0, 0, 0, 0, 0, 0, 600,
+      1800, 5.670374e-8, 2.667, 0};
+  memcpy(s->pv, def, sizeof(def));
+  s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
+  const char* ng = getenv("CF_NO_GRAPH");
+  s->use_graph = !(ng && *ng && *ng != '0');
+
+  hipError_t e = hipGetDevice(&s->dev);
+  if (e != hipSuccess) { delete s; return set_err(CF_EDEVICE, "hipGetDevice: %s", hipGetErrorString(e)); }
+  e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { s->st = nullptr; delete s; return set_err(CF_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
+
+  // one allocation, 256-B aligned sub-buffers
+  std::vector<std::pair<void**, size_t>> plan;
+  Fields& F = s->F;
+  auto add = [&](void** p, size_t bytes) { plan.push_back({p, bytes}); };
+  const size_t nC = s->nC, nU = s->nU, nV = s->nV;
+  add((void**)&s->dP, sizeof(cf_params));
+  add((void**)&F.U, nU * 4); add((void**)&F.Uh, nU * 4); add((void**)&F.U2, nU * 4);
+  add((void**)&F.V, nV * 4); add((void**)&F.Vh, nV * 4); add((void**)&F.V2, nV * 4);
+  add((void**)&F.T, nC * 4); add((void**)&F.Th, nC * 4); add((void**)&F.T2, nC * 4);
+  add((void**)&F.Y, nC * 16); add((void**)&F.Yh, nC * 16); add((void**)&F.Y2, nC * 16);
+  add((void**)&F.SQ, nC * 8); add((void**)&F.cellG, nC * 8);
+  add((void**)&F.fx, nU * 4); add((void**)&F.fy, nV * 4);
+  add((void**)&F.pieces, CF_MAX_PIECES * CF_PIECE_FLOATS * 4);
+  add((void**)&F.acc, CF_ACC_LEN * 4);
+  add((void**)&F.rad, CF_RAD_LEN * 16);
+  add((void**)&F.faceW, CF_MAX_PIECES * 4 * 4);
+  for (int l = 0; l < s->nlev; l++) {
+    const size_t lx = s->lnx[l], ly = s->lny[l];
+    add((void**)&s->L[l].p, lx * ly * 4); add((void**)&s->L[l].rhs, lx * ly * 4);
+    add((void**)&s->L[l].ax, (lx + 1) * ly * 4); add((void**)&s->L[l].ay, lx * (ly + 1) * 4);
+  }
+  const unsigned nparts = (unsigned)(((nx + 31) / 32) * ((ny + 7) / 8));
+  add((void**)&s->dPart, (size_t)nparts * 8);
+  size_t total = 0;
+  for (auto& q : plan) total += (q.second + 255) & ~(size_t)255;
+  e = hipMalloc(&s->block, total);
+  if (e != hipSuccess) { s->block = nullptr; cf_sim_destroy(s); return set_err(CF_ENOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(e)); }
+  s->block_bytes = total;
+  {
+    size_t off = 0;
+    for (auto& q : plan) { *q.first = (char*)s->block + off; off += (q.second + 255) & ~(size_t)255; }
+  }
+  s->hPart.resize((size_t)nparts * 2);
+  F.P = s->dP; F.nx = nx; F.ny = ny;
+  F.p0 = s->L[0].p; F.rhs0 = s->L[0].rhs; F.ax0 = s->L[0].ax; F.ay0 = s->L[0].ay;
+  for (int l = 0; l < s->nlev; l++) { s->L[l].nx = s->lnx[l]; s->L[l].ny = s->lny[l]; s->L[l].h2 = s->lh2[l]; }
+  // device buffers start zeroed, as WebGPU buffers do
+  cf::launch_fill_u32(s->block, 0u, total / 4, s->st);
+
+  for (int q = 0; q < UP_SLOTS; q++) {
+    hipEventCreateWithFlags(&s->evP[q], hipEventDisableTiming);
+    hipEventCreateWithFlags(&s->evPc[q], hipEventDisableTiming);
+    s->evP_used[q] = s->evPc_used[q] = false;
+  }
+  for (int q = 0; q < 3; q++) { hipEventCreateWithFlags(&s->ringEv[q], hipEventDisableTiming); s->busy[q] = false; }
+  if (hipHostMalloc((void**)&s->hP, sizeof(cf_params) * UP_SLOTS, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->hPieces, sizeof(float) * CF_MAX_PIECES * CF_PIECE_FLOATS * UP_SLOTS, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->ring, RING_BYTES * 3, hipHostMallocDefault) != hipSuccess) {
+    cf_sim_destroy(s);
+    return set_err(CF_ENOMEM, "pinned host allocation failed");
+  }
+  int rc = cf_sim_write_params(s);
+  if (rc) { cf_sim_destroy(s); return rc; }
+  rc = cf_sim_reset(s);
+  if (rc) { cf_sim_destroy(s); return rc; }
+  e = hipStreamSynchronize(s->st);
+  if (e != hipSuccess) { cf_sim_destroy(s); return set_err(CF_EDEVICE, "init: %s", hipGetErrorString(e)); }
+  *out = s;
+  return CF_OK;
+}
+
+int cf_sim_reset(cf_sim* s) {  // js/fluid.js:162-176
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  Fields& F = s->F;
+  const float Ta = (float)PHYS_Tamb;
+  uint32_t tb;
+  memcpy(&tb, &Ta, 4);
+  cf::launch_fill_u32(F.T, tb, s->nC, s->st);
+  cf::launch_fill_u32(F.T2, tb, s->nC, s->st);
+  cf::launch_fill_u32(F.Th, tb, s->nC, s->st);
+  // Y = (0, YOamb, 0, 0): fill zero then set the O component through a strided view
+  std::vector<float> y((size_t)s->nC * 4, 0.f);
+  const float yo = (float)s->pv[6];
+  for (int k = 0; k < s->nC; k++) y[(size_t)k * 4 + 1] = yo;
+  HIPCHK(hipMemcpyAsync(F.Y, y.data(), y.size() * 4, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(F.Y2, y.data(), y.size() * 4, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(F.Yh, y.data(), y.size() * 4, hipMemcpyHostToDevice, s->st));
+  cf::launch_fill_u32(F.U, 0, s->nU, s->st); cf::launch_fill_u32(F.Uh, 0, s->nU, s->st); cf::launch_fill_u32(F.U2, 0, s->nU, s->st);
+  cf::launch_fill_u32(F.V, 0, s->nV, s->st); cf::launch_fill_u32(F.Vh, 0, s->nV, s->st); cf::launch_fill_u32(F.V2, 0, s->nV, s->st);
+  cf::launch_fill_u32(F.SQ, 0, (size_t)s->nC * 2, s->st);
+  cf::launch_fill_u32(F.acc, 0, CF_ACC_LEN, s->st);
+  cf::launch_fill_u32(F.rad, 0, CF_RAD_LEN * 4, s->st);
+  for (int l = 0; l < s->nlev; l++) cf::launch_fill_u32(s->L[l].p, 0, (size_t)s->lnx[l] * s->lny[l], s->st);
+  s->time = 0;
+  HIPCHK(hipStreamSynchronize(s->st));  // the staging vector goes out of scope
+  return CF_OK;
+}
+
+int cf_sim_set_param(cf_sim* s, const char* key, double v) {  // js/fluid.js:178
+  if (!s || !key) return set_err(CF_EINVAL, "NULL argument");
+  if (!strcmp(key, "time")) { s->time = v; return CF_OK; }
+  if (!strcmp(key, "frame")) { s->frame = (uint32_t)(int64_t)v; return CF_OK; }
+  const int q = param_index(key);
+  if (q < 0) return set_err(CF_EINVAL, "unknown parameter '%s'", key);
+  if (q <= 3) return set_err(CF_EINVAL, "'%s' is owned by the sim", key);
+  if (q == 26 && !(v >= 0 && v <= CF_MAX_PIECES)) return set_err(CF_EINVAL, "nPieces %g outside 0..%d", v, CF_MAX_PIECES);
+  s->pv[q] = v;
+  return CF_OK;
+}
+
+int cf_sim_get_param(const cf_sim* s, const char* key, double* v) {
+  if (!s || !key || !v) return set_err(CF_EINVAL, "NULL argument");
+  if (!strcmp(key, "time")) { *v = s->time; return CF_OK; }
+  if (!strcmp(key, "frame")) { *v = s->frame; return CF_OK; }
+  const int q = param_index(key);
+  if (q < 0) return set_err(CF_EINVAL, "unknown parameter '%s'", key);
+  *v = s->pv[q];
+  return CF_OK;
+}
+
+int cf_sim_write_params(cf_sim* s) {  // js/fluid.js:180-199
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  s->pv[0] = s->nx; s->pv[1] = s->ny; s->pv[2] = s->dx; s->pv[3] = s->dt;
+  s->pv[39] = s->time; s->pv[27] = (double)s->frame;
+  uint32_t* raw = (uint32_t*)&s->P;
+  for (int q = 0; q < CF_NPARAMS; q++) {
+    if (PKIND[q] == 'f') { float f = (float)s->pv[q]; memcpy(&raw[q], &f, 4); }
+    else if (PKIND[q] == 'i') { int32_t iv = (int32_t)s->pv[q]; memcpy(&raw[q], &iv, 4); }
+    else raw[q] = (uint32_t)(int64_t)s->pv[q];
+  }
+  // explicit diffusion stability: dt_sub <= dx^2 / (4.5 alpha(2200 K)), odd count <= 5
+  const double Tmax = 2200, mu = 1.716e-5 * pow(Tmax / 273.15, 1.5) * 383.55 / (Tmax + 110.4);
+  const double alpha = mu / 0.7 * s->pv[11] / (phys_rhoAmb() * PHYS_Tamb / Tmax);
+  int n = (int)ceil(s->dt / (s->dx * s->dx / (4.5 * alpha)));
+  n = n % 2 ? n : n + 1;
+  if (n > 5) n = 5;
+  s->diffSub = n;
+  s->dsub = (float)(s->dt / n);
+  return upload_params(s);
+}
+
+int cf_sim_get_packed_params(const cf_sim* s, cf_params* out) {
+  if (!s || !out) return set_err(CF_EINVAL, "NULL argument");
+  *out = s->P;
+  return CF_OK;
+}
+
+int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
+  if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
+  if (sc->nu < 0 || sc->vcycles < 1 || sc->coarse_iters < 1 || !(sc->tol >= 0))
+    return set_err(CF_EINVAL, "bad schedule");
+  if (sc->smoother_fp16) return set_err(CF_EINVAL, "FP16 smoother not available in this build");
+  s->sch = *sc;
+  return CF_OK;
+}
+
+int cf_sim_get_schedule(const cf_sim* s, cf_schedule* sc) {
+  if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
+  *sc = s->sch;
+  return CF_OK;
+}
+
+int cf_sim_upload_pieces(cf_sim* s, const float* data, int n) {  // js/fluid.js:201-204
+  if (!s || !data) return set_err(CF_EINVAL, "NULL argument");
+  if (n < 0 || n > CF_MAX_PIECES) return set_err(CF_EINVAL, "nPieces %d outside 0..%d", n, CF_MAX_PIECES);
+  s->pv[26] = n;
+  const int slot = s->upPc;
+  s->upPc = (slot + 1) % UP_SLOTS;
+  if (s->evPc_used[slot]) HIPCHK(hipEventSynchronize(s->evPc[slot]));
+  float* h = s->hPieces + (size_t)slot * CF_MAX_PIECES * CF_PIECE_FLOATS;
+  memcpy(h, data, sizeof(float) * CF_MAX_PIECES * CF_PIECE_FLOATS);
+  HIPCHK(hipMemcpyAsync(s->F.pieces, h, sizeof(float) * CF_MAX_PIECES * CF_PIECE_FLOATS, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipEventRecord(s->evPc[slot], s->st));
+  s->evPc_used[slot] = true;
+  return CF_OK;
+}
+
+int cf_sim_clear_geometry(cf_sim* s) {  // js/fluid.js:213
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  cf::launch_fill_u32(s->F.faceW, 0, CF_MAX_PIECES * 4, s->st);
+  HIPCHK(hipGetLastError());
+  return CF_OK;
+}
+
+int cf_sim_geometry(cf_sim* s) {  // js/fluid.js:215-218
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  cf::launch_geometry(s->F, s->st);
+  HIPCHK(hipGetLastError());
+  return CF_OK;
+}
+
+int cf_sim_radiation(cf_sim* s) {  // js/fluid.js:220-222
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  const int np = s->P.nPieces < CF_MAX_PIECES ? s->P.nPieces : CF_MAX_PIECES;
+  cf::launch_radiation(s->F, np * 4 + CF_N_FLOOR, s->st);
+  HIPCHK(hipGetLastError());
+  return CF_OK;
+}
+
+static int run_mg_to_tol(cf_sim* s);
+
+int cf_sim_step(cf_sim* s, int nsteps) {  // js/fluid.js:225-253
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  if (nsteps < 0) return set_err(CF_EINVAL, "nsteps < 0");
+  if (s->sch.tol > 0) {
+    for (int q = 0; q < nsteps; q++) {
+      for (int st = 0; st < 5; st++) enqueue_stage(s, st);
+      int rc = run_mg_to_tol(s);
+      enqueue_stage(s, 6);
+      s->time += s->dt;
+      if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+  }
+  if (s->use_graph) {
+    int rc = ensure_graph(s);
+    if (rc) return rc;
+    for (int q = 0; q < nsteps; q++) {
+      HIPCHK(hipGraphLaunch(s->gexec, s->st));
+      s->time += s->dt;
+    }
+  } else {
+    for (int q = 0; q < nsteps; q++) {
+      enqueue_step(s);
+      s->time += s->dt;
+    }
+    HIPCHK(hipGetLastError());
+  }
+  s->last_vcycles = s->sch.vcycles;
+  return CF_OK;
+}
+
+int cf_sim_frame(cf_sim* s, int nsteps, int with_radiation) {  // js/main.js:266-283
+  int rc;
+  if ((rc = cf_sim_write_params(s))) return rc;
+  if ((rc = cf_sim_clear_geometry(s))) return rc;
+  if ((rc = cf_sim_geometry(s))) return rc;
+  if ((rc = cf_sim_step(s, nsteps))) return rc;
+  if (with_radiation && (rc = cf_sim_radiation(s))) return rc;
+  return CF_OK;
+}
+
+int cf_sim_run_stage(cf_sim* s, int stage) {
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  int rc = enqueue_stage(s, stage);
+  if (rc) return rc;
+  HIPCHK(hipGetLastError());
+  return CF_OK;
+}
+
+// residual-controlled V-cycles (the schedule's tol > 0): residual norm after each cycle
+static int run_mg_to_tol(cf_sim* s) {
+  // norms are computed on the host from the level-0 arrays; a device-side
+  // norm replaces this when the mixed-precision solver lands
+  const int nx = s->nx, ny = s->ny;
+  std::vector<float> p((size_t)nx * ny), rhs((size_t)nx * ny), ax((size_t)(nx + 1) * ny), ay((size_t)nx * (ny + 1));
+  HIPCHK(hipMemcpyAsync(rhs.data(), s->L[0].rhs, rhs.size() * 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(ax.data(), s->L[0].ax, ax.size() * 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(ay.data(), s->L[0].ay, ay.size() * 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  double bb = 0;
+  for (float v : rhs) bb += (double)v * v;
+  const double h2 = s->lh2[0];
+  double r = 1e30;
+  int c = 0;
+  for (c = 0; c < s->sch.vcycles; c++) {
+    enqueue_vcycle(s, s->sch.nu);
+    HIPCHK(hipMemcpyAsync(p.data(), s->L[0].p, p.size() * 4, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    double rr = 0;
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const double w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], so = ay[j * nx + i], n = ay[(j + 1) * nx + i];
+        double off = 0, diag = 0;
+        if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else diag += 2 * w;
+        if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else diag += 2 * e;
+        if (j > 0) { off += so * p[(j - 1) * nx + i]; diag += so; }
+        if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else diag += 2 * n;
+        if (diag <= 0) continue;
+        const double res = rhs[j * nx + i] - (off - diag * p[j * nx + i]) / h2;
+        rr += res * res;
+      }
+    r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+    if (r < s->sch.tol) { c++; break; }
+  }
+  s->last_vcycles = c;
+  s->last_resid = (float)r;
+  if (r >= s->sch.tol) return set_err(CF_ENOCONV, "multigrid residual %.3g above %.3g after %d V-cycles", r, s->sch.tol, c);
+  return CF_OK;
+}
+
+int cf_sim_profile_steps(cf_sim* s, int nsteps, float* stage_ms) {
+  if (!s || !stage_ms) return set_err(CF_EINVAL, "NULL argument");
+  std::vector<hipEvent_t> ev((size_t)nsteps * 8);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  for (int q = 0; q < 7; q++) stage_ms[q] = 0;
+  for (int n = 0; n < nsteps; n++) {
+    hipEventRecord(ev[n * 8], s->st);
+    for (int st = 0; st < 7; st++) {
+      enqueue_stage(s, st);
+      hipEventRecord(ev[n * 8 + st + 1], s->st);
+    }
+    s->time += s->dt;
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (int n = 0; n < nsteps; n++)
+    for (int st = 0; st < 7; st++) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, ev[n * 8 + st], ev[n * 8 + st + 1]);
+      stage_ms[st] += ms;
+    }
+  for (auto& e : ev) hipEventDestroy(e);
+  return CF_OK;
+}
+
+// Per-kernel timing of the production step: an event after every launch site;
+// returns, per kernel name, the summed milliseconds and the launch count.
+int cf_sim_profile_kernels(cf_sim* s, int nsteps, int max, char* names, float* ms, int* launches, int* count) {
+  if (!s || !names || !ms || !launches || !count) return set_err(CF_EINVAL, "NULL argument");
+  for (auto e : s->pev) hipEventDestroy(e);
+  s->pev.clear(); s->ptag.clear(); s->tags.clear();
+  s->profiling = true;
+  hipEvent_t e0;
+  hipEventCreate(&e0);
+  hipEventRecord(e0, s->st);
+  for (int n = 0; n < nsteps; n++) { enqueue_step(s); s->time += s->dt; }
+  s->profiling = false;
+  HIPCHK(hipStreamSynchronize(s->st));
+  hipEvent_t prev = e0;
+  for (size_t q = 0; q < s->pev.size(); q++) {
+    float t = 0;
+    hipEventElapsedTime(&t, prev, s->pev[q]);
+    s->tags[s->ptag[q]].ms += t;
+    s->tags[s->ptag[q]].launches++;
+    prev = s->pev[q];
+  }
+  const int n = (int)s->tags.size() < max ? (int)s->tags.size() : max;
+  for (int q = 0; q < n; q++) {
+    snprintf(names + 32 * q, 32, "%s", s->tags[q].name);
+    ms[q] = (float)s->tags[q].ms;
+    launches[q] = s->tags[q].launches;
+  }
+  *count = n;
+  hipEventDestroy(e0);
+  for (auto e : s->pev) hipEventDestroy(e);
+  s->pev.clear(); s->ptag.clear();
+  return CF_OK;
+}
+
+int cf_sim_step_latencies(cf_sim* s, int nsteps, float* ms) {
+  if (!s || !ms || nsteps < 0) return set_err(CF_EINVAL, "bad argument");
+  if (s->sch.tol > 0) return set_err(CF_EINVAL, "latency timing needs a fixed V-cycle count");
+  std::vector<hipEvent_t> ev((size_t)nsteps + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  if (s->use_graph) {
+    int rc = ensure_graph(s);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(ev[0], s->st));
+  for (int q = 0; q < nsteps; q++) {
+    if (s->use_graph) HIPCHK(hipGraphLaunch(s->gexec, s->st));
+    else enqueue_step(s);
+    HIPCHK(hipEventRecord(ev[q + 1], s->st));
+    s->time += s->dt;
+  }
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (int q = 0; q < nsteps; q++) hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]);
+  for (auto& e : ev) hipEventDestroy(e);
+  return CF_OK;
+}
+
+int cf_sim_synchronize(cf_sim* s) {
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  HIPCHK(hipStreamSynchronize(s->st));
+  return CF_OK;
+}
+
+int cf_sim_encode_readback(cf_sim* s) {  // js/fluid.js:315-324
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  const int i = s->stageIdx;
+  if (s->busy[i]) return set_err(CF_EAGAIN, "readback ring slot %d busy", i);
+  uint8_t* h = s->ring + (size_t)i * RING_BYTES;
+  HIPCHK(hipMemcpyAsync(h, s->F.acc, CF_ACC_LEN * 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(h + CF_ACC_LEN * 4, s->F.rad, CF_RAD_LEN * 16, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(h + CF_ACC_LEN * 4 + CF_RAD_LEN * 16, s->F.faceW, CF_MAX_PIECES * 16, hipMemcpyDeviceToHost, s->st));
+  cf::launch_fill_u32(s->F.acc, 0, CF_ACC_LEN, s->st);
+  HIPCHK(hipEventRecord(s->ringEv[i], s->st));
+  s->busy[i] = true;
+  s->stageIdx = (i + 1) % 3;
+  return i;
+}
+
+int cf_sim_take_readback(cf_sim* s, int slot, int wait, int32_t* acc, float* rad, int32_t* faceW) {  // js/fluid.js:327-342
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  if (slot < 0 || slot > 2 || !s->busy[slot]) return set_err(CF_EINVAL, "readback slot %d not pending", slot);
+  if (wait) {
+    HIPCHK(hipEventSynchronize(s->ringEv[slot]));
+  } else {
+    hipError_t e = hipEventQuery(s->ringEv[slot]);
+    if (e == hipErrorNotReady) return set_err(CF_ENOTREADY, "readback slot %d not complete", slot);
+    if (e != hipSuccess) { s->busy[slot] = false; return set_err(CF_EDEVICE, "readback: %s", hipGetErrorString(e)); }
+  }
+  const uint8_t* h = s->ring + (size_t)slot * RING_BYTES;
+  if (acc) memcpy(acc, h, CF_ACC_LEN * 4);
+  if (rad) memcpy(rad, h + CF_ACC_LEN * 4, CF_RAD_LEN * 16);
+  if (faceW) memcpy(faceW, h + CF_ACC_LEN * 4 + CF_RAD_LEN * 16, CF_MAX_PIECES * 16);
+  s->busy[slot] = false;
+  return CF_OK;
+}
+
+int cf_sim_field_bytes(const cf_sim* s, const char* name, size_t* bytes) {
+  if (!s || !name || !bytes) return set_err(CF_EINVAL, "NULL argument");
+  FieldRef r;
+  if (!lookup(const_cast<cf_sim*>(s), name, r)) return set_err(CF_EINVAL, "no buffer '%s'", name);
+  *bytes = r.bytes;
+  return CF_OK;
+}
+
+int cf_sim_read_field(cf_sim* s, const char* name, void* dst, size_t bytes) {  // js/fluid.js:345-355
+  if (!s || !name || !dst) return set_err(CF_EINVAL, "NULL argument");
+  FieldRef r;
+  if (!lookup(s, name, r)) return set_err(CF_EINVAL, "no buffer '%s'", name);
+  if (bytes != r.bytes) return set_err(CF_EINVAL, "'%s' is %zu bytes, got %zu", name, r.bytes, bytes);
+  HIPCHK(hipMemcpyAsync(dst, r.ptr, bytes, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return CF_OK;
+}
+
+int cf_sim_write_field(cf_sim* s, const char* name, const void* src, size_t bytes) {
+  if (!s || !name || !src) return set_err(CF_EINVAL, "NULL argument");
+  FieldRef r;
+  if (!lookup(s, name, r)) return set_err(CF_EINVAL, "no buffer '%s'", name);
+  if (bytes != r.bytes) return set_err(CF_EINVAL, "'%s' is %zu bytes, got %zu", name, r.bytes, bytes);
+  HIPCHK(hipMemcpyAsync(r.ptr, src, bytes, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return CF_OK;
+}
+
+int cf_sim_get_info(const cf_sim* s, cf_sim_info* info) {
+  if (!s || !info) return set_err(CF_EINVAL, "NULL argument");
+  memset(info, 0, sizeof(*info));
+  info->nx = s->nx; info->ny = s->ny; info->dx = s->dx; info->dt = s->dt;
+  info->height = s->dx * s->ny; info->time = s->time; info->frame = s->frame;
+  info->diff_sub = s->diffSub; info->nlevels = s->nlev;
+  for (int l = 0; l < s->nlev; l++) { info->level_nx[l] = s->lnx[l]; info->level_ny[l] = s->lny[l]; }
+  info->last_vcycles = s->last_vcycles; info->last_resid = s->last_resid;
+  return CF_OK;
+}
+
+int cf_sim_div_error(cf_sim* s, float* max_err, float* mean_err) {
+  if (!s || !max_err || !mean_err) return set_err(CF_EINVAL, "NULL argument");
+  const unsigned nb = cf::div_error_blocks(s->F);
+  cf::launch_div_error(s->F, s->dPart, s->st);
+  HIPCHK(hipMemcpyAsync(s->hPart.data(), s->dPart, (size_t)nb * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  double mx = 0, sum = 0;
+  for (unsigned b = 0; b < nb; b++) { mx = fmax(mx, s->hPart[2 * b]); sum += s->hPart[2 * b + 1]; }
+  *max_err = (float)mx;
+  *mean_err = (float)(sum / s->nC);
+  return CF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,211 @@
+/*
+ * campfire_hip.h — C ABI of the MI355X gas solver (gfx950, HIP).
+ *
+ * This is the drop-in boundary for campfire's `FluidSim` (js/fluid.js:26-356).
+ * Every entry point below replaces one method or field of that class; the
+ * reference line each one stands in for is cited beside it.  The N-API addon
+ * (campfire_amd/js/addon.cc) and the Python ctypes mirror (campfire_amd/fluid.py)
+ * bind exactly these symbols.
+ *
+ * Conventions
+ *  - Every `int` entry point returns CF_OK (0) or a negative CF_E* code and never
+ *    throws across the ABI; cf_last_error() gives a thread-local message.
+ *  - The caller owns host buffers; a cf_sim owns all of its device memory.
+ *  - One HIP stream per cf_sim.  A cf_sim is not thread-safe.
+ *  - Work is enqueued asynchronously on the sim's stream, like WebGPU command
+ *    submission; functions that hand data back to the host synchronise.
+ *  - Units are SI; grid coordinates are cell units, cell (i,j) centred at
+ *    (i+0.5, j+0.5), U(i,j) at (i, j+0.5), V(i,j) at (i+0.5, j)
+ *    (shaders/common.wgsl:1-2).
+ */
+#ifndef CAMPFIRE_HIP_H
+#define CAMPFIRE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Exported constants (js/fluid.js:5-11). */
+#define CF_MAX_PIECES   64
+#define CF_PIECE_FLOATS 20
+#define CF_ACC_GLOBAL   512
+#define CF_ACC_LEN      (CF_ACC_GLOBAL + 16)
+#define CF_E_SCALE      1e6
+#define CF_M_SCALE      1e12
+#define CF_G_SCALE      1e3
+#define CF_N_FLOOR      64
+#define CF_RAD_LEN      (CF_MAX_PIECES * 4 + CF_N_FLOOR)
+#define CF_W_SCALE      1e8 /* faceW fixed point (shaders/geometry.wgsl:18) */
+#define CF_NPARAMS      40
+
+/* Status codes. */
+#define CF_OK          0
+#define CF_EINVAL     -1  /* bad argument / unknown name */
+#define CF_EBADSIZE   -2  /* 'coarsest level too large' (js/fluid.js:91) */
+#define CF_ENOMEM     -3  /* device allocation failed or over the HBM budget */
+#define CF_EDEVICE    -4  /* a HIP runtime call failed */
+#define CF_EAGAIN     -5  /* readback ring full / result not ready (js/fluid.js:317) */
+#define CF_ENOTREADY  -6  /* readback slot not complete yet */
+#define CF_ENOCONV    -7  /* multigrid did not reach its residual target */
+#define CF_ENONFINITE -8  /* NaN/Inf detected in the fields */
+
+/*
+ * The 160-byte parameter block, field order of PARAM_FIELDS (js/fluid.js:13-24)
+ * and of the WGSL `Params` uniform (shaders/common.wgsl:4-15).
+ */
+typedef struct cf_params {
+  int32_t nx, ny; float dx, dt;
+  float Tamb, rhoAmb, YOamb, grav;
+  float cp, Pr, viscMult, diffMult;
+  float reactB, reactTa, dHc, sO2;
+  float sootFormA, sootFormTa, sootOxA, sootOxTa;
+  float kappaSootC, kappaGas, exchAlpha, depth;
+  float charA, charTa; int32_t nPieces; uint32_t frame;
+  float heaterX0, heaterX1, heaterY1, heaterT;
+  float burnerX0, burnerX1, burnerFlux, burnerT;
+  float sootDensity, sigma, charO2, time;
+} cf_params;
+
+/*
+ * Solver schedule.  The reference runs exactly one V-cycle per step with
+ * nu = 2 red-black sweeps per level and 120 red-black sweeps on the coarsest
+ * level (js/fluid.js:250,294-312; shaders/pressure.wgsl:141), 8 chemistry
+ * substeps (shaders/sources.wgsl:110).  Those are the defaults.
+ */
+typedef struct cf_schedule {
+  int32_t nu;            /* smoother sweeps (each = red + black) per level, ref 2 */
+  int32_t vcycles;       /* V-cycles per step (max when tol > 0), ref 1 */
+  float   tol;           /* relative residual target ||r||/||rhs||; 0 = run exactly `vcycles` */
+  int32_t coarse_iters;  /* red-black sweeps on the coarsest level, ref 120;
+                            0 = direct FP64 solve of the coarsest level */
+  int32_t smoother_fp16; /* 1 = FP16 smoother, FP32 residual + defect correction */
+} cf_schedule;
+
+typedef struct cf_sim_info {
+  int32_t nx, ny;
+  double  dx, dt, height, time;
+  uint32_t frame;
+  int32_t diff_sub;      /* diffusion substeps (js/fluid.js:190-198) */
+  int32_t nlevels;       /* multigrid levels (js/fluid.js:69-91, without the 6-level cap) */
+  int32_t level_nx[16], level_ny[16];
+  int32_t last_vcycles;  /* V-cycles run by the last step */
+  float   last_resid;    /* relative residual after the last step's projection (if measured) */
+} cf_sim_info;
+
+typedef struct cf_sim cf_sim;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* new FluidSim(device, {nx, ny, dt}) + init()  (js/fluid.js:27-47,49-160).
+ * dx is explicit (the reference derives it as SCENE_WIDTH/nx, js/fluid.js:30).
+ * Levels halve while both dims are even and cells > 256; an odd coarsest
+ * level above 256 cells fails with CF_EBADSIZE as js/fluid.js:91 does.
+ * Allocates every buffer of js/fluid.js:50-91 on the current HIP device and
+ * calls cf_sim_reset(). */
+int  cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out);
+void cf_sim_destroy(cf_sim* sim);
+
+/* reset()  (js/fluid.js:162-176). */
+int  cf_sim_reset(cf_sim* sim);
+
+/* ---- parameters -------------------------------------------------------- */
+
+/* setParam(key, value)  (js/fluid.js:178).  Keys are the PARAM_FIELDS names.
+ * "time" and "frame" set the sim's own counters (the app does sim.frame++,
+ * js/main.js:282).  Takes effect at the next cf_sim_write_params(). */
+int  cf_sim_set_param(cf_sim* sim, const char* key, double value);
+int  cf_sim_get_param(const cf_sim* sim, const char* key, double* value);
+/* writeParams()  (js/fluid.js:180-199): packs the 160-B block (nx, ny, dx,
+ * dt, time, frame owned by the sim) and computes the diffusion substeps. */
+int  cf_sim_write_params(cf_sim* sim);
+/* The packed block as last written (for bit-level checks). */
+int  cf_sim_get_packed_params(const cf_sim* sim, cf_params* out);
+/* PARAM_FIELDS name of field `index` (0..39), or NULL. */
+const char* cf_param_name(int index);
+
+int  cf_sim_set_schedule(cf_sim* sim, const cf_schedule* s);
+int  cf_sim_get_schedule(const cf_sim* sim, cf_schedule* s);
+
+/* uploadPieces(Float32Array(64*20), n)  (js/fluid.js:201-204); also sets
+ * nPieces.  Layout per piece (shaders/common.wgsl:17-23):
+ * [x, y, angle, half, vx, vy, omega, alive, Ts[4], mflux[4], charCov[4]]. */
+int  cf_sim_upload_pieces(cf_sim* sim, const float* pieces, int n);
+
+/* ---- work -------------------------------------------------------------- */
+
+/* clearGeometry(encoder)  (js/fluid.js:213): faceW = 0. */
+int  cf_sim_clear_geometry(cf_sim* sim);
+/* encodeGeometry(pass)  (js/fluid.js:215-218): cellG, fx, fy, faceW. */
+int  cf_sim_geometry(cf_sim* sim);
+/* nsteps x encodeStep(pass)  (js/fluid.js:225-253); time += dt each step. */
+int  cf_sim_step(cf_sim* sim, int nsteps);
+/* encodeRadiation(pass)  (js/fluid.js:220-222). */
+int  cf_sim_radiation(cf_sim* sim);
+/* One app frame as js/main.js:266-283 encodes it: write_params, clear
+ * geometry, geometry, nsteps steps, radiation (if with_radiation). */
+int  cf_sim_frame(cf_sim* sim, int nsteps, int with_radiation);
+
+/* Run one of the seven profiled stages of encodeStepProfiled
+ * (js/fluid.js:256-292): 0 advect, 1 diffuse, 2 sources, 3 forces,
+ * 4 coef/rhs, 5 vcycle, 6 project.  Running 0..6 in order equals one step
+ * (time is not advanced). */
+int  cf_sim_run_stage(cf_sim* sim, int stage);
+/* encodeStepProfiled: runs nsteps steps with HIP events between the seven
+ * stages and returns the summed milliseconds per stage in stage_ms[7]. */
+int  cf_sim_profile_steps(cf_sim* sim, int nsteps, float* stage_ms);
+/* Per-kernel timing of nsteps production steps (events after every launch):
+ * up to `max` distinct kernel names (32 chars each, in `names`), summed
+ * milliseconds and launch counts; *count receives the number filled. */
+int  cf_sim_profile_kernels(cf_sim* sim, int nsteps, int max, char* names,
+                            float* ms, int* launches, int* count);
+
+/* Per-step latency: nsteps steps, each bracketed by HIP events on the sim's
+ * stream; ms[q] receives the device time of step q. */
+int  cf_sim_step_latencies(cf_sim* sim, int nsteps, float* ms);
+
+/* Wait for all enqueued work. */
+int  cf_sim_synchronize(cf_sim* sim);
+
+/* ---- readback ---------------------------------------------------------- */
+
+/* encodeReadback(encoder)  (js/fluid.js:315-324): enqueue a copy of acc
+ * (528 i32), rad (320 x vec4f) and faceW (256 i32) into the next slot of a
+ * 3-deep pinned ring and clear acc.  Returns the slot (0..2) or CF_EAGAIN
+ * when that slot is still held (the reference then skips the readback and
+ * acc keeps accumulating). */
+int  cf_sim_encode_readback(cf_sim* sim);
+/* afterSubmit's completion  (js/fluid.js:327-342): block until `slot` is
+ * complete (wait != 0) or test it (wait == 0, CF_ENOTREADY if pending), copy it
+ * out and release the slot.  Any output pointer may be NULL. */
+int  cf_sim_take_readback(cf_sim* sim, int slot, int wait,
+                          int32_t* acc, float* rad, int32_t* faceW);
+
+/* readBuffer(name)  (js/fluid.js:345-355) and the inverse (used by
+ * H.blob, dev/gas_test.html:112-120).  Names: U Uh U2 V Vh V2 T Th T2 Y Yh Y2
+ * SQ cellG fx fy pieces acc rad faceW and the level-0 MG arrays p rhs ax ay;
+ * "p1".."p15", "rhs1".. for coarse levels.  Y* are vec4 AoS (16 B/cell), SQ and
+ * cellG vec2 (8 B/cell), exactly the reference layouts.  `bytes` must equal the
+ * buffer size (cf_sim_field_bytes). */
+int  cf_sim_field_bytes(const cf_sim* sim, const char* name, size_t* bytes);
+int  cf_sim_read_field(cf_sim* sim, const char* name, void* dst, size_t bytes);
+int  cf_sim_write_field(cf_sim* sim, const char* name, const void* src, size_t bytes);
+
+/* Fields nx, ny, dx, dt, height, time, frame, diffSub, levels of FluidSim. */
+int  cf_sim_get_info(const cf_sim* sim, cf_sim_info* info);
+
+/* Max |div(f u) - S| and the mean, after a projection (H.divErr,
+ * dev/gas_test.html:101-111), computed on the device. */
+int  cf_sim_div_error(cf_sim* sim, float* max_err, float* mean_err);
+
+/* Thread-local text of the last error. */
+const char* cf_last_error(void);
+/* Library version string. */
+const char* cf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CAMPFIRE_HIP_H */

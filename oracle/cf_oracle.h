@@ -1,0 +1,23 @@
+/*
+ * cf_oracle.h — CPU oracle of campfire's gas step.  TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithm (the WGSL kernels under
+ * shaders/, the generated samplers of js/gpu.js:16-65, the schedule of
+ * js/fluid.js and tests/flame1d.mjs).  It is compiled twice from cf_oracle.c:
+ * with REAL=float (prefix cfo32_) for parity with the FP32 GPU path, and with
+ * REAL=double (prefix cfo64_) as the FP64 accuracy reference.  Only the test
+ * suite, the smoke check and the benchmark's CPU-baseline leg load it; the
+ * product library never links or calls it.
+ */
+#ifndef CF_ORACLE_H
+#define CF_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+#include "../include/campfire_hip.h"
+
+/* flame1d.c: tests/flame1d.mjs:13-71 restated (double, single thread). */
+typedef struct { double SL, Tb; int steps; } cfo_flame_result;
+cfo_flame_result cfo_flame_speed(double B, double Ta, double dx, double F,
+                                 double phi, double L, double tEnd);
+
+#endif

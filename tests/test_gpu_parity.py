@@ -1,0 +1,158 @@
+"""GPU parity: the HIP path against the FP32 oracle on identical seeded inputs.
+
+Bar: integer/index outputs bit-exact (acc up to +-1 per contributing cell or
+8x8 tile for the fixed-point sums); float fields within 1e-5 relative L2
+(the one-step tolerance of the north-star), with the observed maximum printed.
+"""
+import numpy as np
+import pytest
+
+from helpers import make_pair, put, rel_l2, sync_geometry
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _flush(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pair():
+    _flush("[parity] building 96x80 pair")
+    g, o, pieces, n = make_pair(96, 80, 0.004, 0.002, seed=3, npieces=5)
+    return g, o
+
+
+def test_params_packing_bitwise(pair):
+    g, o = pair
+    a = bytes(g.packed_params())
+    b = bytes(o.packed_params())
+    assert a == b
+    assert g.diff_sub == o.diff_sub
+
+
+def test_geometry_parity():
+    _flush("[parity] geometry 128x96")
+    g, o, pieces, n = make_pair(128, 96, 0.003, 0.002, seed=5, npieces=8, state=False)
+    sdf_g = g.read_field("cellG").reshape(-1, 2)
+    sdf_o = o.read("cellG").reshape(-1, 2)
+    assert np.array_equal(sdf_g[:, 1].view(np.uint32), sdf_o[:, 1].view(np.uint32))
+    assert np.max(np.abs(sdf_g[:, 0] - sdf_o[:, 0])) < 1e-6
+    assert np.max(np.abs(g.read_field("fx") - o.read("fx"))) < 1e-4
+    assert np.max(np.abs(g.read_field("fy") - o.read("fy"))) < 1e-4
+    fw_g, fw_o = g.read_field("faceW"), o.read("faceW")
+    # one rounding difference per band cell at most
+    assert np.max(np.abs(fw_g - fw_o)) <= 4, (fw_g[fw_g != fw_o], fw_o[fw_g != fw_o])
+
+
+@pytest.mark.parametrize("stage", [0, 1, 2, 3, 4, 5, 6])
+def test_stage_parity(stage):
+    _flush("[parity] stage", stage)
+    g, o, pieces, n = make_pair(96, 80, 0.004, 0.002, seed=11 + stage, npieces=5)
+    sync_geometry(g, o)
+    # coarse inputs of the later stages
+    if stage >= 3:
+        st = np.random.default_rng(stage).uniform(-50, 50, 96 * 80 * 2).astype(np.float32)
+        put(g, o, "SQ", st * 0.01)
+    if stage >= 5:
+        g.run_stage(4)
+        o.run_stage(4)
+    if stage >= 6:
+        g.run_stage(5)
+        o.run_stage(5)
+    g.run_stage(stage)
+    rc = o.run_stage(stage)
+    assert rc == 0
+    outs = {0: ["Th", "Yh", "T2", "Y2", "Uh", "Vh", "U2", "V2"], 1: ["T", "Y", "Th", "Yh"],
+            2: ["T", "Y", "SQ"], 3: ["U", "V"], 4: ["ax", "ay", "rhs", "ax1", "ay1"],
+            5: ["p", "p1"], 6: ["U", "V"]}[stage]
+    for name in outs:
+        a, b = g.read_field(name), o.read(name)
+        r = rel_l2(a, b)
+        _flush("  %s rel_l2=%.3g max|d|=%.3g" % (name, r, np.max(np.abs(a.astype(np.float64) - b))))
+        assert r <= TOL, name
+    if stage == 2:
+        ag, ao = g.read_field("acc"), o.read("acc")
+        d = np.abs(ag.astype(np.int64) - ao)
+        _flush("  acc max|d| =", d.max(), "global", ag[512:515], ao[512:515])
+        assert d[:512].max() <= 8
+        assert abs(int(ag[512]) - int(ao[512])) <= max(2, abs(int(ao[512])) * 1e-5)
+        assert ag[514] == ao[514]
+
+
+def test_one_step_parity():
+    _flush("[parity] one full step 128x96")
+    g, o, pieces, n = make_pair(128, 96, 0.003, 0.002, seed=21, npieces=6)
+    sync_geometry(g, o)
+    g.step(1)
+    o.step(1)
+    for name in ["T", "Y", "U", "V", "SQ", "p"]:
+        a, b = g.read_field(name), o.read(name)
+        r = rel_l2(a, b)
+        _flush("  %s rel_l2=%.3g" % (name, r))
+        assert r <= TOL, name
+
+
+def test_one_step_parity_diffsub3():
+    """dt large enough for 3 diffusion substeps: the fused last substep path."""
+    _flush("[parity] one step, 3 diffusion substeps")
+    g, o, pieces, n = make_pair(64, 64, 0.001, 0.004, seed=22, npieces=3)
+    assert g.diff_sub == o.diff_sub and g.diff_sub >= 3
+    sync_geometry(g, o)
+    g.step(1)
+    o.step(1)
+    for name in ["T", "Y", "U", "V", "SQ", "Th"]:
+        r = rel_l2(g.read_field(name), o.read(name))
+        _flush("  %s rel_l2=%.3g" % (name, r))
+        assert r <= TOL, name
+
+
+def test_radiation_parity():
+    _flush("[parity] radiation")
+    g, o, pieces, n = make_pair(96, 96, 0.004, 0.002, seed=31, npieces=6)
+    sync_geometry(g, o)
+    g.encode_radiation()
+    o.radiation()
+    a, b = g.read_field("rad"), o.read("rad")
+    r = rel_l2(a, b)
+    _flush("  rad rel_l2=%.3g" % r)
+    assert r <= 1e-4
+
+
+def test_readback_ring_and_reset():
+    from campfire_amd.fluid import FluidSim
+    _flush("[parity] readback ring")
+    with FluidSim({"nx": 64, "ny": 64, "dt": 0.002}, dx=0.005) as g:
+        g.write_field("acc", np.arange(528, dtype=np.int32))
+        got = []
+        for _ in range(3):
+            assert g.encode_readback() >= 0
+            g.after_submit(lambda r, tag: got.append(r["acc"].copy()), None)
+        assert g.encode_readback() == -1  # all three slots held: skipped (js/fluid.js:317)
+        g.flush()
+        assert np.array_equal(got[0], np.arange(528))
+        assert not got[1].any() and not got[2].any()  # acc cleared after the first copy
+        g.reset()
+        assert np.all(g.read_field("T") == np.float32(293.15))
+        Y = g.read_field("Y").reshape(-1, 4)
+        assert np.all(Y[:, 1] == np.float32(0.232)) and not Y[:, [0, 2, 3]].any()
+
+
+def test_frames_stay_finite():
+    """A few app frames (geometry, steps, radiation, readback) on a small scene."""
+    from campfire_amd.fluid import FluidSim
+    from campfire_amd import scenes
+    _flush("[parity] frames")
+    with FluidSim({"nx": 128, "ny": 128, "dt": 0.002}, dx=0.004) as g:
+        pieces, n = scenes.pile(6, 4, 128 * 0.004)
+        g.upload_pieces(pieces, n)
+        for f in range(5):
+            g.frame = f
+            g.frame_all(4, True)
+            r = g.readback_now()
+        for name in ("T", "Y", "U", "V", "p"):
+            assert np.all(np.isfinite(g.read_field(name))), name
+        assert r["acc"][514] > 800  # hot wood band
+        assert np.all(r["rad"][:n * 16:4] > 0)
