@@ -31,6 +31,15 @@ void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float*
 void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, hipStream_t s);
 void launch_forces(const Fields& F, hipStream_t s);
 void launch_coef_rhs(const Fields& F, hipStream_t s);
+// forces + coef + rhs0 + coefficient restriction of the first
+// forces_coef_cascade(nlev) coarse levels, in one pass
+int forces_coef_cascade(int nlev);
+void launch_forces_coef(const Fields& F, const Level* L, int nlev, hipStream_t s);
+// fused multigrid (cf_mg.hip)
+bool launch_mg_down(const Level& f, const Level& c, int nu, hipStream_t s);
+bool launch_mg_up(const Level& f, const Level& c, int nu, hipStream_t s);
+size_t mg_bottom_lds_bytes(const Level* L, int nlev);
+bool launch_mg_bottom(const Level* L, int nlev, int nu, int coarse_iters, hipStream_t s);
 void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s);
 void launch_relax(const Level& L, int color, hipStream_t s);
 void launch_restrict_res(const Level& f, const Level& c, hipStream_t s);

@@ -329,62 +329,174 @@ __device__ __forceinline__ float Tcl(const float* __restrict__ T, int nx, int ny
 __device__ __forceinline__ float exchangef(const cf_params& P, float spd, float Tf) {
   return expf(K(-2.0) * P.exchAlpha * spd * (Tf / P.Tamb) / P.depth * P.dt);
 }
+// new U at face (i, j), 0 <= i <= nx, 0 <= j < ny
+__device__ __forceinline__ float force_u(const cf_params& P, const Fields& F, int i, int j) {
+  const int nx = P.nx, ny = P.ny;
+  const float dt = P.dt;
+  const float idx2 = K(1) / (P.dx * P.dx);
+  const float* __restrict__ T = F.T;
+  const float* __restrict__ U2 = F.U2;
+  const float* __restrict__ V2 = F.V2;
+  const float* __restrict__ fx = F.fx;
+  auto u2 = [&](int a, int b) { const int q = b * (nx + 1) + a; return fx[q] > K(0) ? U2[q] : K(0); };
+  const int k = j * (nx + 1) + i;
+  if (!(fx[k] > K(0))) return K(0);
+  if (i == 0) return fminf(U2[j * (nx + 1) + 1], K(0));
+  if (i == nx) return fmaxf(U2[j * (nx + 1) + nx - 1], K(0));
+  const float u = U2[k];
+  const float TL = Tcl(T, nx, ny, i - 1, j), TR = Tcl(T, nx, ny, i, j);
+  const float rhoF = rhoOf(P, K(0.5) * (TL + TR));
+  const float muE = muOf(P, TR), muW = muOf(P, TL);
+  float uN = u, muN = muOf(P, K(0.5) * (TL + TR));
+  if (j < ny - 1) { uN = u2(i, j + 1); muN = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j + 1) + Tcl(T, nx, ny, i, j + 1))); }
+  float uS = -u, muS = muOf(P, K(0.5) * (TL + TR));
+  if (j > 0) { uS = u2(i, j - 1); muS = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i, j - 1))); }
+  const float visc = (muE * (u2(i + 1, j) - u) + muW * (u2(i - 1, j) - u) + muN * (uN - u) + muS * (uS - u)) * idx2 / rhoF;
+  const float vAvg = K(0.25) * (V2[j * nx + i - 1] + V2[j * nx + i] + V2[(j + 1) * nx + i - 1] + V2[(j + 1) * nx + i]);
+  return (u + dt * visc) * exchangef(P, sqrtf(u * u + vAvg * vAvg), K(0.5) * (TL + TR));
+}
+
+// new V at face (i, j), 0 <= i < nx, 0 <= j <= ny
+__device__ __forceinline__ float force_v(const cf_params& P, const Fields& F, int i, int j) {
+  const int nx = P.nx, ny = P.ny;
+  const float dt = P.dt;
+  const float idx2 = K(1) / (P.dx * P.dx);
+  const float* __restrict__ T = F.T;
+  const float* __restrict__ U2 = F.U2;
+  const float* __restrict__ V2 = F.V2;
+  const float* __restrict__ fy = F.fy;
+  auto v2 = [&](int a, int b) { const int q = b * nx + a; return fy[q] > K(0) ? V2[q] : K(0); };
+  const int k = j * nx + i;
+  if (j == 0 || !(fy[k] > K(0))) return K(0);
+  if (j == ny) return fmaxf(V2[(ny - 1) * nx + i], K(0));
+  const float v = V2[k];
+  const float TB = Tcl(T, nx, ny, i, j - 1), TT = Tcl(T, nx, ny, i, j);
+  const float Tf = K(0.5) * (TB + TT);
+  const float rhoF = rhoOf(P, Tf);
+  const float muN = muOf(P, TT), muS = muOf(P, TB);
+  float vE = v, muE = muOf(P, Tf);
+  if (i < nx - 1) { vE = v2(i + 1, j); muE = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i + 1, j - 1) + Tcl(T, nx, ny, i + 1, j))); }
+  float vW = v, muW = muOf(P, Tf);
+  if (i > 0) { vW = v2(i - 1, j); muW = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i - 1, j))); }
+  const float visc = (muN * (v2(i, j + 1) - v) + muS * (v2(i, j - 1) - v) + muE * (vE - v) + muW * (vW - v)) * idx2 / rhoF;
+  const float buoy = P.grav * (Tf / P.Tamb - K(1));
+  const float uAvg = K(0.25) * (U2[(j - 1) * (nx + 1) + i] + U2[(j - 1) * (nx + 1) + i + 1] + U2[j * (nx + 1) + i] + U2[j * (nx + 1) + i + 1]);
+  return (v + dt * (visc + buoy)) * exchangef(P, sqrtf(uAvg * uAvg + v * v), Tf);
+}
+
 __global__ __launch_bounds__(256) void k_forces(Fields F) {
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx + 1, ny + 1, i, j);
-  const float dt = P.dt, dx = P.dx;
-  const float idx2 = K(1) / (dx * dx);
-  const float* __restrict__ T = F.T;
-  const float* __restrict__ U2 = F.U2;
-  const float* __restrict__ V2 = F.V2;
-  const float* __restrict__ fx = F.fx;
-  const float* __restrict__ fy = F.fy;
-  auto u2 = [&](int a, int b) { const int q = b * (nx + 1) + a; return fx[q] > K(0) ? U2[q] : K(0); };
-  auto v2 = [&](int a, int b) { const int q = b * nx + a; return fy[q] > K(0) ? V2[q] : K(0); };
-  if (i <= nx && j < ny) {
-    const int k = j * (nx + 1) + i;
-    float out;
-    if (!(fx[k] > K(0))) out = K(0);
-    else if (i == 0) out = fminf(U2[j * (nx + 1) + 1], K(0));
-    else if (i == nx) out = fmaxf(U2[j * (nx + 1) + nx - 1], K(0));
-    else {
-      const float u = U2[k];
-      const float TL = Tcl(T, nx, ny, i - 1, j), TR = Tcl(T, nx, ny, i, j);
-      const float rhoF = rhoOf(P, K(0.5) * (TL + TR));
-      const float muE = muOf(P, TR), muW = muOf(P, TL);
-      float uN = u, muN = muOf(P, K(0.5) * (TL + TR));
-      if (j < ny - 1) { uN = u2(i, j + 1); muN = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j + 1) + Tcl(T, nx, ny, i, j + 1))); }
-      float uS = -u, muS = muOf(P, K(0.5) * (TL + TR));
-      if (j > 0) { uS = u2(i, j - 1); muS = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i, j - 1))); }
-      const float visc = (muE * (u2(i + 1, j) - u) + muW * (u2(i - 1, j) - u) + muN * (uN - u) + muS * (uS - u)) * idx2 / rhoF;
-      const float vAvg = K(0.25) * (V2[j * nx + i - 1] + V2[j * nx + i] + V2[(j + 1) * nx + i - 1] + V2[(j + 1) * nx + i]);
-      out = (u + dt * visc) * exchangef(P, sqrtf(u * u + vAvg * vAvg), K(0.5) * (TL + TR));
+  if (i <= nx && j < ny) F.U[j * (nx + 1) + i] = force_u(P, F, i, j);
+  if (i < nx && j <= ny) F.V[j * nx + i] = force_v(P, F, i, j);
+}
+
+// face coefficients a = f / rho (shaders/pressure.wgsl:30-43)
+__device__ __forceinline__ float coef_u(const cf_params& P, const Fields& F, int i, int j) {
+  const int nx = P.nx;
+  const float Tf = K(0.5) * (F.T[j * nx + iclamp(i - 1, 0, nx - 1)] + F.T[j * nx + min(i, nx - 1)]);
+  return F.fx[j * (nx + 1) + i] * invRho(P, Tf);
+}
+__device__ __forceinline__ float coef_v(const cf_params& P, const Fields& F, int i, int j) {
+  const int nx = P.nx, ny = P.ny;
+  const float Tf = K(0.5) * (F.T[iclamp(j - 1, 0, ny - 1) * nx + i] + F.T[min(j, ny - 1) * nx + i]);
+  return F.fy[j * nx + i] * invRho(P, Tf);
+}
+
+// Fused momentum update + projection set-up on 64x32-cell tiles:
+// forces (shaders/forces.wgsl:34-101), coef + rhs0 (shaders/pressure.wgsl:30-54)
+// and the coefficient restriction (:57-67) cascaded through as many coarse
+// levels as the tile halves into (nlev_fused levels past level 0).
+// Each tile recomputes the velocity and coefficient of the one face column and
+// row just past its edge (owned by the next tile) instead of reading them back.
+struct CoefLevels {
+  Level L[16];
+  int ncascade;  // coarse levels produced here (<= 5 for a 64x32 tile)
+};
+
+__global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL) {
+  constexpr int TX = 64, TY = 32;
+  __shared__ float su[TY][TX + 1], sv[TY + 1][TX];
+  __shared__ float sax[TY][TX + 1], say[TY + 1][TX];
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
+  const int t = xcd_tile(blockIdx.x, gx * gy);
+  const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
+  const int lx = threadIdx.x, ly0 = threadIdx.y;  // 64 x 4 threads
+  // ---- forces and coefficients of the tile's faces
+  for (int r = 0; r < TY / 4; r++) {
+    const int ly = ly0 + 4 * r;
+    const int i = x0 + lx, j = y0 + ly;
+    if (i < nx && j < ny) {
+      const float u = force_u(P, F, i, j), v = force_v(P, F, i, j);
+      su[ly][lx] = u; sv[ly][lx] = v;
+      F.U[j * (nx + 1) + i] = u; F.V[j * nx + i] = v;
+      const float a = coef_u(P, F, i, j), b = coef_v(P, F, i, j);
+      sax[ly][lx] = a; say[ly][lx] = b;
+      F.ax0[j * (nx + 1) + i] = a; F.ay0[j * nx + i] = b;
+      if (lx == TX - 1 || i == nx - 1) {  // the face just past the tile's right edge
+        const float u1 = force_u(P, F, i + 1, j), a1 = coef_u(P, F, i + 1, j);
+        su[ly][lx + 1] = u1; sax[ly][lx + 1] = a1;
+        if (i + 1 == nx) { F.U[j * (nx + 1) + nx] = u1; F.ax0[j * (nx + 1) + nx] = a1; }
+      }
+      if (ly == TY - 1 || j == ny - 1) {  // the face just past the tile's top edge
+        const float v1 = force_v(P, F, i, j + 1), b1 = coef_v(P, F, i, j + 1);
+        sv[ly + 1][lx] = v1; say[ly + 1][lx] = b1;
+        if (j + 1 == ny) { F.V[ny * nx + i] = v1; F.ay0[ny * nx + i] = b1; }
+      }
     }
-    F.U[k] = out;
   }
-  if (i < nx && j <= ny) {
-    const int k = j * nx + i;
-    float out;
-    if (j == 0 || !(fy[k] > K(0))) out = K(0);
-    else if (j == ny) out = fmaxf(V2[(ny - 1) * nx + i], K(0));
-    else {
-      const float v = V2[k];
-      const float TB = Tcl(T, nx, ny, i, j - 1), TT = Tcl(T, nx, ny, i, j);
-      const float Tf = K(0.5) * (TB + TT);
-      const float rhoF = rhoOf(P, Tf);
-      const float muN = muOf(P, TT), muS = muOf(P, TB);
-      float vE = v, muE = muOf(P, Tf);
-      if (i < nx - 1) { vE = v2(i + 1, j); muE = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i + 1, j - 1) + Tcl(T, nx, ny, i + 1, j))); }
-      float vW = v, muW = muOf(P, Tf);
-      if (i > 0) { vW = v2(i - 1, j); muW = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i - 1, j))); }
-      const float visc = (muN * (v2(i, j + 1) - v) + muS * (v2(i, j - 1) - v) + muE * (vE - v) + muW * (vW - v)) * idx2 / rhoF;
-      const float buoy = P.grav * (Tf / P.Tamb - K(1));
-      const float uAvg = K(0.25) * (U2[(j - 1) * (nx + 1) + i] + U2[(j - 1) * (nx + 1) + i + 1] + U2[j * (nx + 1) + i] + U2[j * (nx + 1) + i + 1]);
-      out = (v + dt * (visc + buoy)) * exchangef(P, sqrtf(uAvg * uAvg + v * v), Tf);
+  __syncthreads();
+  // ---- rhs0 (:46-54)
+  for (int r = 0; r < TY / 4; r++) {
+    const int ly = ly0 + 4 * r;
+    const int i = x0 + lx, j = y0 + ly;
+    if (i < nx && j < ny) {
+      const float fe = F.fx[j * (nx + 1) + i + 1], fw = F.fx[j * (nx + 1) + i];
+      const float fn = F.fy[(j + 1) * nx + i], fs = F.fy[j * nx + i];
+      const float div = (fe * su[ly][lx + 1] - fw * su[ly][lx] + fn * sv[ly + 1][lx] - fs * sv[ly][lx]) / P.dx;
+      const float open = fe + fw + fn + fs;
+      F.rhs0[j * nx + i] = open > K(0) ? (div - F.SQ[j * nx + i].x) / P.dt : K(0);
     }
-    F.V[k] = out;
+  }
+  // ---- cascade of coefficient restrictions (:57-67)
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  int tx_l = TX, ty_l = TY;
+  for (int l = 0; l < CL.ncascade; l++) {
+    const Level& C = CL.L[l + 1];
+    const int xc = x0 >> (l + 1), yc = y0 >> (l + 1);
+    const int cwx = tx_l / 2, cwy = ty_l / 2;  // coarse tile extent
+    const int nax = (cwx + 1) * cwy, nay = cwx * (cwy + 1);
+    float va[4], vb[4];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = tid + 256 * q;
+      va[q] = vb[q] = 0.f;
+      if (k < nax) { const int I = k % (cwx + 1), J = k / (cwx + 1); va[q] = K(0.5) * (sax[2 * J][2 * I] + sax[2 * J + 1][2 * I]); }
+      if (k < nay) { const int I = k % cwx, J = k / cwx; vb[q] = K(0.5) * (say[2 * J][2 * I] + say[2 * J][2 * I + 1]); }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = tid + 256 * q;
+      if (k < nax) {
+        const int I = k % (cwx + 1), J = k / (cwx + 1);
+        sax[J][I] = va[q];
+        const int gI = xc + I, gJ = yc + J;
+        if (gJ < C.ny && ((I < cwx && gI < C.nx) || gI == C.nx)) C.ax[gJ * (C.nx + 1) + gI] = va[q];
+      }
+      if (k < nay) {
+        const int I = k % cwx, J = k / cwx;
+        say[J][I] = vb[q];
+        const int gI = xc + I, gJ = yc + J;
+        if (gI < C.nx && ((J < cwy && gJ < C.ny) || gJ == C.ny)) C.ay[gJ * C.nx + gI] = vb[q];
+      }
+    }
+    tx_l = cwx; ty_l = cwy;
   }
 }
 
@@ -791,6 +903,17 @@ void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* 
 }
 void launch_forces(const Fields& F, hipStream_t s) {
   hipLaunchKernelGGL(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+}
+int forces_coef_cascade(int nlev) {
+  // a 64x32 tile halves into coarse tiles down to 2x1 cells: five levels
+  return nlev - 1 < 5 ? nlev - 1 : 5;
+}
+void launch_forces_coef(const Fields& F, const Level* L, int nlev, hipStream_t s) {
+  CoefLevels CL;
+  for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
+  CL.ncascade = forces_coef_cascade(nlev);
+  const unsigned nb = (unsigned)(((F.nx + 63) / 64) * ((F.ny + 31) / 32));
+  hipLaunchKernelGGL(k_forces_coef, dim3(nb), dim3(64, 4), 0, s, F, CL);
 }
 void launch_coef_rhs(const Fields& F, hipStream_t s) {
   hipLaunchKernelGGL(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);

@@ -101,6 +101,8 @@ struct cf_sim {
   hipGraphExec_t gexec = nullptr;
   int gkey[6] = {-1, -1, -1, -1, -1, -1};
   bool use_graph = true;
+  bool mg_simple = false;  // one launch per reference dispatch (debug comparisons)
+  int mg_bottom = 0;       // first level solved by the single-workgroup bottom kernel
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> pev;
@@ -129,6 +131,14 @@ static void mark(cf_sim* s, const char* tag) {
 // ---- step schedule ---------------------------------------------------------
 static void enqueue_vcycle(cf_sim* s, int nu) {  // js/fluid.js:294-312
   const int L = s->nlev;
+  if (!s->mg_simple) {
+    // fused: tiled down legs, one LDS-resident workgroup for the bottom levels, tiled up legs
+    const int b = s->mg_bottom;
+    for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, s->st); mark(s, "mg_down"); }
+    cf::launch_mg_bottom(&s->L[b], L - b, nu, s->sch.coarse_iters, s->st); mark(s, "mg_bottom");
+    for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, s->st); mark(s, "mg_up"); }
+    return;
+  }
   for (int l = 0; l < L - 1; l++) {
     for (int q = 0; q < nu; q++) {
       cf::launch_relax(s->L[l], 0, s->st); mark(s, "relax");
@@ -197,8 +207,15 @@ static void enqueue_step(cf_sim* s) {
       }
     }
   }
-  enqueue_stage(s, 3);
-  enqueue_stage(s, 4);
+  if (s->mg_simple) {
+    enqueue_stage(s, 3);
+    enqueue_stage(s, 4);
+  } else {
+    cf::launch_forces_coef(F, s->L, s->nlev, s->st); mark(s, "forces_coef");
+    for (int l = 1 + cf::forces_coef_cascade(s->nlev); l < s->nlev; l++) {
+      cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, "coef_restrict");
+    }
+  }
   enqueue_stage(s, 5);
   enqueue_stage(s, 6);
 }
@@ -341,6 +358,8 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
   s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
   const char* ng = getenv("CF_NO_GRAPH");
   s->use_graph = !(ng && *ng && *ng != '0');
+  const char* ms = getenv("CF_MG_SIMPLE");
+  s->mg_simple = ms && *ms && *ms != '0';
 
   hipError_t e = hipGetDevice(&s->dev);
   if (e != hipSuccess) { delete s; return set_err(CF_EDEVICE, "hipGetDevice: %s", hipGetErrorString(e)); }
@@ -383,6 +402,10 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
   F.P = s->dP; F.nx = nx; F.ny = ny;
   F.p0 = s->L[0].p; F.rhs0 = s->L[0].rhs; F.ax0 = s->L[0].ax; F.ay0 = s->L[0].ay;
   for (int l = 0; l < s->nlev; l++) { s->L[l].nx = s->lnx[l]; s->L[l].ny = s->lny[l]; s->L[l].h2 = s->lh2[l]; }
+  // bottom solver: the longest run of coarse levels whose arrays fit in 64 KB of LDS
+  s->mg_bottom = s->nlev - 1;
+  while (s->mg_bottom > 0 && cf::mg_bottom_lds_bytes(&s->L[s->mg_bottom - 1], s->nlev - s->mg_bottom + 1) <= 64 * 1024)
+    s->mg_bottom--;
   // device buffers start zeroed, as WebGPU buffers do
   cf::launch_fill_u32(s->block, 0u, total / 4, s->st);
 
@@ -486,8 +509,8 @@ int cf_sim_get_packed_params(const cf_sim* s, cf_params* out) {
 
 int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
   if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
-  if (sc->nu < 0 || sc->vcycles < 1 || sc->coarse_iters < 1 || !(sc->tol >= 0))
-    return set_err(CF_EINVAL, "bad schedule");
+  if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 1 || !(sc->tol >= 0))
+    return set_err(CF_EINVAL, "bad schedule (nu 1..3, vcycles >= 1, coarse_iters >= 1)");
   if (sc->smoother_fp16) return set_err(CF_EINVAL, "FP16 smoother not available in this build");
   s->sch = *sc;
   return CF_OK;
