@@ -24,13 +24,18 @@ __device__ __forceinline__ int toint(float x) {
 }
 __device__ __forceinline__ int iround(float x) { return (int)rintf(x); }
 
+// x^1.5 as x*sqrt(x): within ~1 ulp of the correctly rounded power (WGSL's pow
+// promises no better), a tenth of the cost of powf, which dominated the
+// stencils that evaluate Sutherland's law five times per face
+__device__ __forceinline__ float pow15(float x) { return x * sqrtf(x); }
+
 // gas closures (shaders/common.wgsl:33-42)
 __device__ __forceinline__ float rhoOf(const cf_params& P, float T) { return P.rhoAmb * P.Tamb / T; }
 __device__ __forceinline__ float muOf(const cf_params& P, float T) {
-  return K(1.716e-5) * powf(T / K(273.15), K(1.5)) * K(273.15 + 110.4) / (T + K(110.4)) * P.viscMult;
+  return K(1.716e-5) * pow15(T / K(273.15)) * K(273.15 + 110.4) / (T + K(110.4)) * P.viscMult;
 }
 __device__ __forceinline__ float rhoDOf(const cf_params& P, float T) {
-  return K(1.716e-5) * powf(T / K(273.15), K(1.5)) * K(273.15 + 110.4) / (T + K(110.4)) / P.Pr * P.diffMult;
+  return K(1.716e-5) * pow15(T / K(273.15)) * K(273.15 + 110.4) / (T + K(110.4)) / P.Pr * P.diffMult;
 }
 __device__ __forceinline__ float invRho(const cf_params& P, float T) { return T / (P.rhoAmb * P.Tamb); }
 

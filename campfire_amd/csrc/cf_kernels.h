@@ -28,7 +28,10 @@ struct Level {
 void launch_advect_fwd(const Fields& F, hipStream_t s);
 void launch_advect_corr(const Fields& F, hipStream_t s);
 void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s);
-void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, hipStream_t s);
+// sources (+ fused last diffusion substep) and the fold of its per-workgroup
+// accumulator partials (part: 4 ints per workgroup, sources_blocks() of them)
+unsigned sources_blocks(const Fields& F);
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, hipStream_t s);
 void launch_forces(const Fields& F, hipStream_t s);
 void launch_coef_rhs(const Fields& F, hipStream_t s);
 // forces + coef + rhs0 + coefficient restriction of the first

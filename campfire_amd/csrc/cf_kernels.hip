@@ -6,6 +6,8 @@
 // integer accumulators are reduced over 8x8 cell tiles exactly as the
 // reference's 8x8 workgroups do (shaders/sources.wgsl:160-172), so `acc` gets
 // the same set of integer contributions.
+#include <climits>
+
 #include "cf_kernels.h"
 
 namespace cf {
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(256) void k_diffuse(Fields F, const float* __restri
 // -------------------------------------------------------------------------
 __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F, int i, int j, float Tc, float4 Yc,
                                              float& Tout, float4& Yout, float2& SQout,
-                                             float& qrel, float& qrad, float& tpeak) {
+                                             float& qrel, float& qrad, float& tpeak, int* sacc) {
   const int nx = P.nx;
   const int k = j * nx + i;
   const float dx = P.dx, dt = P.dt, cp = P.cp, sO2 = P.sO2, dHc = P.dHc;
@@ -190,7 +192,7 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
     const float dTc = (Ts - Tc) * (K(1) - expf(-rate * dt));
     Tc += dTc;
     const float toWood = -rho * cp * dTc * cellA;
-    atomicAdd(&F.acc[(pi * 4u + fi) * 2u], iround(toWood * K(1e6)));
+    atomicAdd(&sacc[(pi * 4u + fi) * 2u], iround(toWood * K(1e6)));
     const float mf = pc[12 + fi] * w;
     if (mf > K(0)) {
       const float madd = mf * dt;
@@ -210,7 +212,7 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
                        (r * Yc.z + (mC + mO2)) / (r + mC), (r * Yc.w + K(0)) / (r + mC));
       Tc = (r * Tc + mC * Ts) / (r + mC);
       massSrc += mC / dt;
-      atomicAdd(&F.acc[(pi * 4u + fi) * 2u + 1u], iround(mC * cellA * K(1e12)));
+      atomicAdd(&sacc[(pi * 4u + fi) * 2u + 1u], iround(mC * cellA * K(1e12)));
     }
   }
   // test sources (:96-107)
@@ -277,10 +279,22 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
 
 // sources, optionally fused with the last diffusion substep (which then reads
 // Tin/Yin): a pointwise composition, identical to running the two in turn.
+//
+// The fixed-point accumulators receive the reference's integer contributions
+// (per band cell and per 8x8 tile, shaders/sources.wgsl:72,92,169-171).
+// Integer sums are order independent, so they are gathered per workgroup:
+// face entries in LDS and flushed with one global atomic each, the three
+// global sums as per-workgroup partials folded by k_acc_finish.  Same-address
+// atomics from every 8x8 tile would otherwise serialise on one L2 line.
 template <bool FUSE>
 __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restrict__ Tin,
-                                                 const float4* __restrict__ Yin, float dts) {
+                                                 const float4* __restrict__ Yin, float dts, int* __restrict__ part) {
   __shared__ float red[3][8][33];
+  __shared__ int sacc[CF_ACC_GLOBAL];
+  __shared__ int sglob[4][3];
+  const int tid = threadIdx.y * 32 + threadIdx.x;
+  for (int q = tid; q < CF_ACC_GLOBAL; q += 256) sacc[q] = 0;
+  __syncthreads();
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
   int i, j;
@@ -295,7 +309,7 @@ __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restri
     float To;
     float4 Yo;
     float2 sq;
-    sources_cell(P, F, i, j, Tc, Yc, To, Yo, sq, qrel, qrad, tpeak);
+    sources_cell(P, F, i, j, Tc, Yc, To, Yo, sq, qrel, qrad, tpeak, sacc);
     F.T[k] = To;
     F.Y[k] = Yo;
     F.SQ[k] = sq;
@@ -306,17 +320,53 @@ __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restri
   red[1][threadIdx.y][threadIdx.x] = qrad * dx * dx * dt;
   red[2][threadIdx.y][threadIdx.x] = tpeak;
   __syncthreads();
-  const int lin = threadIdx.y * 32 + threadIdx.x;
-  const int w = lin >> 6, lane = lin & 63;
+  const int w = tid >> 6, lane = tid & 63;
   const int lx = lane & 7, ly = lane >> 3;
   float vq = red[0][ly][w * 8 + lx], vr = red[1][ly][w * 8 + lx], vt = red[2][ly][w * 8 + lx];
   vq = wave_tree_sum(vq);
   vr = wave_tree_sum(vr);
   vt = wave_tree_max(vt);
   if (lane == 0) {
-    atomicAdd(&F.acc[CF_ACC_GLOBAL], iround(vq * K(1e3)));
-    atomicAdd(&F.acc[CF_ACC_GLOBAL + 1], iround(vr * K(1e3)));
-    atomicMax(&F.acc[CF_ACC_GLOBAL + 2], (int)vt);
+    sglob[w][0] = iround(vq * K(1e3));
+    sglob[w][1] = iround(vr * K(1e3));
+    sglob[w][2] = (int)vt;
+  }
+  __syncthreads();
+  for (int q = tid; q < CF_ACC_GLOBAL; q += 256)
+    if (sacc[q] != 0) atomicAdd(&F.acc[q], sacc[q]);
+  if (tid == 0) {
+    // two's-complement wrap-around, as repeated i32 atomicAdd gives
+    const unsigned sq_ = (unsigned)sglob[0][0] + (unsigned)sglob[1][0] + (unsigned)sglob[2][0] + (unsigned)sglob[3][0];
+    const unsigned sr_ = (unsigned)sglob[0][1] + (unsigned)sglob[1][1] + (unsigned)sglob[2][1] + (unsigned)sglob[3][1];
+    const int mt = max(max(sglob[0][2], sglob[1][2]), max(sglob[2][2], sglob[3][2]));
+    part[4 * blockIdx.x + 0] = (int)sq_;
+    part[4 * blockIdx.x + 1] = (int)sr_;
+    part[4 * blockIdx.x + 2] = mt;
+  }
+}
+
+// folds the per-workgroup partials of k_sources into acc[512..514]
+__global__ __launch_bounds__(1024) void k_acc_finish(const int* __restrict__ part, int nparts, int* __restrict__ acc) {
+  __shared__ unsigned sq[1024], sr[1024];
+  __shared__ int st[1024];
+  const int t = threadIdx.x;
+  unsigned a = 0, b = 0;
+  int m = INT_MIN;
+  for (int q = t; q < nparts; q += 1024) {
+    a += (unsigned)part[4 * q];
+    b += (unsigned)part[4 * q + 1];
+    m = max(m, part[4 * q + 2]);
+  }
+  sq[t] = a; sr[t] = b; st[t] = m;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (t < s) { sq[t] += sq[t + s]; sr[t] += sr[t + s]; st[t] = max(st[t], st[t + s]); }
+    __syncthreads();
+  }
+  if (t == 0) {
+    atomicAdd(&acc[CF_ACC_GLOBAL], (int)sq[0]);
+    atomicAdd(&acc[CF_ACC_GLOBAL + 1], (int)sr[0]);
+    atomicMax(&acc[CF_ACC_GLOBAL + 2], st[0]);
   }
 }
 
@@ -897,9 +947,12 @@ void launch_advect_corr(const Fields& F, hipStream_t s) {
 void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s) {
   hipLaunchKernelGGL(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout, dts);
 }
-void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, hipStream_t s) {
-  if (fuse) hipLaunchKernelGGL(k_sources<true>, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, dts);
-  else hipLaunchKernelGGL(k_sources<false>, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, dts);
+unsigned sources_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, hipStream_t s) {
+  const unsigned nb = nblk(F.nx, F.ny);
+  if (fuse) hipLaunchKernelGGL(k_sources<true>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
+  else hipLaunchKernelGGL(k_sources<false>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
+  hipLaunchKernelGGL(k_acc_finish, dim3(1), dim3(1024), 0, s, (const int*)part, (int)nb, F.acc);
 }
 void launch_forces(const Fields& F, hipStream_t s) {
   hipLaunchKernelGGL(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);

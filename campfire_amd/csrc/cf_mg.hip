@@ -184,22 +184,27 @@ struct BottomLevels {
   int nu, coarse_iters, first_is_level0;
 };
 
+// one half-sweep of a bottom level held in LDS; 16x16 threads walk rows and
+// same-colour columns without integer division
 __device__ __forceinline__ void b_relax(const Level& L, float* p, const float* rhs, const float* ax, const float* ay,
                                         int col) {
   const int nx = L.nx, ny = L.ny;
   const int half = (nx + 1) / 2;
-  for (int k = threadIdx.x; k < half * ny; k += blockDim.x) {
-    const int j = k / half;
-    const int i = 2 * (k % half) + ((j + col) & 1);
-    if (i >= nx) continue;
-    const float w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], s = ay[j * nx + i], n = ay[(j + 1) * nx + i];
-    float off = 0.f, diag = 0.f;
-    if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else { diag += K(2.0) * w; }
-    if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else { diag += K(2.0) * e; }
-    if (j > 0) { off += s * p[(j - 1) * nx + i]; diag += s; }
-    if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else { diag += K(2.0) * n; }
-    const int k2 = j * nx + i;
-    p[k2] = diag > K(0) ? (off - rhs[k2] * L.h2) / diag : K(0);
+  const float h2 = L.h2;
+  for (int j = threadIdx.y; j < ny; j += 16) {
+    const int par = (j + col) & 1;
+    for (int ii = threadIdx.x; ii < half; ii += 16) {
+      const int i = 2 * ii + par;
+      if (i >= nx) continue;
+      const float w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], s = ay[j * nx + i], n = ay[(j + 1) * nx + i];
+      float off = 0.f, diag = 0.f;
+      if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else { diag += K(2.0) * w; }
+      if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else { diag += K(2.0) * e; }
+      if (j > 0) { off += s * p[(j - 1) * nx + i]; diag += s; }
+      if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else { diag += K(2.0) * n; }
+      const int k2 = j * nx + i;
+      p[k2] = diag > K(0) ? (off - rhs[k2] * h2) / diag : K(0);
+    }
   }
 }
 
@@ -218,17 +223,18 @@ __device__ __forceinline__ float b_residual(const Level& L, const float* p, cons
 
 __global__ __launch_bounds__(256) void k_mg_bottom(BottomLevels B) {
   extern __shared__ float lds[];
-  const int T = blockDim.x;
+  const int T = 256;  // 16 x 16 threads
+  const int tid = threadIdx.y * 16 + threadIdx.x;
   // load coefficients of every level and p, rhs of the first
   for (int l = 0; l < B.nlev; l++) {
     const Level& L = B.L[l];
     const int n = L.nx * L.ny, nax = (L.nx + 1) * L.ny, nay = L.nx * (L.ny + 1);
     float* base = lds + B.off[l];
     float *p = base, *rhs = base + n, *ax = rhs + n, *ay = ax + nax;
-    for (int k = threadIdx.x; k < nax; k += T) ax[k] = L.ax[k];
-    for (int k = threadIdx.x; k < nay; k += T) ay[k] = L.ay[k];
+    for (int k = tid; k < nax; k += T) ax[k] = L.ax[k];
+    for (int k = tid; k < nay; k += T) ay[k] = L.ay[k];
     if (l == 0)
-      for (int k = threadIdx.x; k < n; k += T) { p[k] = L.p[k]; rhs[k] = L.rhs[k]; }
+      for (int k = tid; k < n; k += T) { p[k] = L.p[k]; rhs[k] = L.rhs[k]; }
   }
   __syncthreads();
   auto P = [&](int l) { return lds + B.off[l]; };
@@ -245,7 +251,7 @@ __global__ __launch_bounds__(256) void k_mg_bottom(BottomLevels B) {
     }
     const Level& C = B.L[l + 1];
     float *pc = P(l + 1), *rc = R(l + 1);
-    for (int k = threadIdx.x; k < C.nx * C.ny; k += T) {
+    for (int k = tid; k < C.nx * C.ny; k += T) {
       const int I = k % C.nx, J = k / C.nx;
       const float r = b_residual(L, P(l), R(l), AX(l), AY(l), 2 * I, 2 * J) +
                       b_residual(L, P(l), R(l), AX(l), AY(l), 2 * I + 1, 2 * J) +
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(256) void k_mg_bottom(BottomLevels B) {
   {
     const Level& L = B.L[last];
     float* p = P(last);
-    for (int k = threadIdx.x; k < L.nx * L.ny; k += T) p[k] = K(0);
+    for (int k = tid; k < L.nx * L.ny; k += T) p[k] = K(0);
     __syncthreads();
     for (int it = 0; it < B.coarse_iters; it++) {
       b_relax(L, p, R(last), AX(last), AY(last), 0); __syncthreads();
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(256) void k_mg_bottom(BottomLevels B) {
     const Level& C = B.L[l + 1];
     float *p = P(l), *pc = P(l + 1);
     const float *ax = AX(l), *ay = AY(l);
-    for (int k = threadIdx.x; k < L.nx * L.ny; k += T) {
+    for (int k = tid; k < L.nx * L.ny; k += T) {
       const int i = k % L.nx, j = k / L.nx;
       const int I = i / 2, J = j / 2;
       const int I2 = iclamp(I + ((i & 1) == 1 ? 1 : -1), 0, C.nx - 1);
@@ -298,7 +304,7 @@ __global__ __launch_bounds__(256) void k_mg_bottom(BottomLevels B) {
   for (int l = 0; l < B.nlev; l++) {
     const Level& L = B.L[l];
     const int n = L.nx * L.ny;
-    for (int k = threadIdx.x; k < n; k += T) {
+    for (int k = tid; k < n; k += T) {
       L.p[k] = P(l)[k];
       if (l > 0) L.rhs[k] = R(l)[k];
     }
@@ -350,7 +356,7 @@ bool launch_mg_bottom(const Level* L, int nlev, int nu, int coarse_iters, hipStr
   B.nlev = nlev; B.nu = nu; B.coarse_iters = coarse_iters; B.first_is_level0 = 0;
   const size_t bytes = (size_t)off * 4;
   if (bytes > 64 * 1024) return false;
-  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(256), bytes, s, B);
+  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(16, 16), bytes, s, B);
   return true;
 }
 

@@ -97,6 +97,7 @@ struct cf_sim {
   // diagnostics
   float* dPart = nullptr;
   std::vector<float> hPart;
+  int* dAccPart = nullptr;  // per-workgroup partial sums of the sources kernel
   // graph of one step
   hipGraphExec_t gexec = nullptr;
   int gkey[6] = {-1, -1, -1, -1, -1, -1};
@@ -178,14 +179,20 @@ static int enqueue_stage(cf_sim* s, int stage) {
         cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
       }
       break;
-    case 2: cf::launch_sources(F, false, nullptr, nullptr, 0.f, s->st); mark(s, "sources"); break;
+    case 2: cf::launch_sources(F, false, nullptr, nullptr, 0.f, s->dAccPart, s->st); mark(s, "sources"); break;
     case 3: cf::launch_forces(F, s->st); mark(s, "forces"); break;
     case 4: enqueue_coef(s); break;
     case 5:
       for (int c = 0; c < s->sch.vcycles; c++) enqueue_vcycle(s, s->sch.nu);
       break;
     case 6: cf::launch_project(F, s->st); mark(s, "project"); break;
-    default: return set_err(CF_EINVAL, "stage %d out of range 0..6", stage);
+    case 7:
+      cf::launch_forces_coef(F, s->L, s->nlev, s->st); mark(s, "forces_coef");
+      for (int l = 1 + cf::forces_coef_cascade(s->nlev); l < s->nlev; l++) {
+        cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, "coef_restrict");
+      }
+      break;
+    default: return set_err(CF_EINVAL, "stage %d out of range 0..7", stage);
   }
   return CF_OK;
 }
@@ -195,7 +202,7 @@ static void enqueue_step(cf_sim* s) {
   Fields& F = s->F;
   enqueue_stage(s, 0);
   if (s->diffSub == 1) {
-    cf::launch_sources(F, true, F.T2, F.Y2, s->dsub, s->st); mark(s, "diffuse_sources");
+    cf::launch_sources(F, true, F.T2, F.Y2, s->dsub, s->dAccPart, s->st); mark(s, "diffuse_sources");
   } else {
     cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
     for (int q = 1; q < s->diffSub; q += 2) {
@@ -203,7 +210,7 @@ static void enqueue_step(cf_sim* s) {
       if (q + 2 < s->diffSub) {
         cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
       } else {
-        cf::launch_sources(F, true, F.Th, F.Yh, s->dsub, s->st); mark(s, "diffuse_sources");
+        cf::launch_sources(F, true, F.Th, F.Yh, s->dsub, s->dAccPart, s->st); mark(s, "diffuse_sources");
       }
     }
   }
@@ -211,10 +218,7 @@ static void enqueue_step(cf_sim* s) {
     enqueue_stage(s, 3);
     enqueue_stage(s, 4);
   } else {
-    cf::launch_forces_coef(F, s->L, s->nlev, s->st); mark(s, "forces_coef");
-    for (int l = 1 + cf::forces_coef_cascade(s->nlev); l < s->nlev; l++) {
-      cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, "coef_restrict");
-    }
+    enqueue_stage(s, 7);
   }
   enqueue_stage(s, 5);
   enqueue_stage(s, 6);
@@ -389,6 +393,7 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
   }
   const unsigned nparts = (unsigned)(((nx + 31) / 32) * ((ny + 7) / 8));
   add((void**)&s->dPart, (size_t)nparts * 8);
+  add((void**)&s->dAccPart, (size_t)nparts * 16);
   size_t total = 0;
   for (auto& q : plan) total += (q.second + 255) & ~(size_t)255;
   e = hipMalloc(&s->block, total);

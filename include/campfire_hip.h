@@ -151,7 +151,8 @@ int  cf_sim_frame(cf_sim* sim, int nsteps, int with_radiation);
 /* Run one of the seven profiled stages of encodeStepProfiled
  * (js/fluid.js:256-292): 0 advect, 1 diffuse, 2 sources, 3 forces,
  * 4 coef/rhs, 5 vcycle, 6 project.  Running 0..6 in order equals one step
- * (time is not advanced). */
+ * (time is not advanced).  Stage 7 runs stages 3 and 4 as the production step
+ * does, in one fused pass. */
 int  cf_sim_run_stage(cf_sim* sim, int stage);
 /* encodeStepProfiled: runs nsteps steps with HIP events between the seven
  * stages and returns the summed milliseconds per stage in stage_ms[7]. */
