@@ -31,18 +31,22 @@ void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float*
 // sources (+ fused last diffusion substep) and the fold of its per-workgroup
 // accumulator partials (part: 4 ints per workgroup, sources_blocks() of them)
 unsigned sources_blocks(const Fields& F);
-void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, hipStream_t s);
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, bool finish,
+                    hipStream_t s);
 void launch_forces(const Fields& F, hipStream_t s);
 void launch_coef_rhs(const Fields& F, hipStream_t s);
 // forces + coef + rhs0 + coefficient restriction of the first
-// forces_coef_cascade(nlev) coarse levels, in one pass
+// forces_coef_cascade(nlev) coarse levels, in one pass; its first workgroup
+// also folds nparts sources partials into acc (nparts = 0: none pending)
 int forces_coef_cascade(int nlev);
-void launch_forces_coef(const Fields& F, const Level* L, int nlev, hipStream_t s);
+void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* part, int nparts, hipStream_t s);
 // fused multigrid (cf_mg.hip)
 bool launch_mg_down(const Level& f, const Level& c, int nu, hipStream_t s);
 bool launch_mg_up(const Level& f, const Level& c, int nu, hipStream_t s);
-size_t mg_bottom_lds_bytes(const Level* L, int nlev);
-bool launch_mg_bottom(const Level* L, int nlev, int nu, int coarse_iters, hipStream_t s);
+// first level of the single-workgroup bottom solver (the last 1-3 levels)
+int mg_bottom_first(const Level* L, int nlev);
+// coarse_coef: restrict the coarsest level's coefficients inside the kernel
+bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, hipStream_t s);
 void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s);
 void launch_relax(const Level& L, int color, hipStream_t s);
 void launch_restrict_res(const Level& f, const Level& c, hipStream_t s);

@@ -345,21 +345,24 @@ __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restri
   }
 }
 
-// folds the per-workgroup partials of k_sources into acc[512..514]
-__global__ __launch_bounds__(1024) void k_acc_finish(const int* __restrict__ part, int nparts, int* __restrict__ acc) {
-  __shared__ unsigned sq[1024], sr[1024];
-  __shared__ int st[1024];
-  const int t = threadIdx.x;
+// Folds the per-workgroup partials of k_sources into acc[512..514]: integer
+// sums with two's-complement wrap-around and a max, so the order is irrelevant.
+// Called by 256 threads; `scratch` holds 3 x 256 ints of LDS.
+__device__ __forceinline__ void acc_fold(const int* __restrict__ part, int nparts, int* __restrict__ acc, int* scratch) {
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  unsigned* sq = (unsigned*)scratch;
+  unsigned* sr = sq + 256;
+  int* st = (int*)(sr + 256);
   unsigned a = 0, b = 0;
   int m = INT_MIN;
-  for (int q = t; q < nparts; q += 1024) {
+  for (int q = t; q < nparts; q += 256) {
     a += (unsigned)part[4 * q];
     b += (unsigned)part[4 * q + 1];
     m = max(m, part[4 * q + 2]);
   }
   sq[t] = a; sr[t] = b; st[t] = m;
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
+  for (int s = 128; s > 0; s >>= 1) {
     if (t < s) { sq[t] += sq[t + s]; sr[t] += sr[t + s]; st[t] = max(st[t], st[t + s]); }
     __syncthreads();
   }
@@ -368,6 +371,12 @@ __global__ __launch_bounds__(1024) void k_acc_finish(const int* __restrict__ par
     atomicAdd(&acc[CF_ACC_GLOBAL + 1], (int)sr[0]);
     atomicMax(&acc[CF_ACC_GLOBAL + 2], st[0]);
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_acc_finish(const int* __restrict__ part, int nparts, int* __restrict__ acc) {
+  __shared__ int scratch[3 * 256];
+  acc_fold(part, nparts, acc, scratch);
 }
 
 // -------------------------------------------------------------------------
@@ -466,17 +475,20 @@ struct CoefLevels {
   int ncascade;  // coarse levels produced here (<= 5 for a 64x32 tile)
 };
 
-__global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL) {
+__global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL, const int* __restrict__ part, int nparts) {
   constexpr int TX = 64, TY = 32;
   __shared__ float su[TY][TX + 1], sv[TY + 1][TX];
   __shared__ float sax[TY][TX + 1], say[TY + 1][TX];
   const cf_params P = *F.P;
+  if (blockIdx.x == 0 && nparts > 0) acc_fold(part, nparts, F.acc, (int*)&su[0][0]);
   const int nx = P.nx, ny = P.ny;
   const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
   const int t = xcd_tile(blockIdx.x, gx * gy);
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
   const int lx = threadIdx.x, ly0 = threadIdx.y;  // 64 x 4 threads
-  // ---- forces and coefficients of the tile's faces
+  // ---- forces and coefficients of the tile's faces (rolled loop: the force
+  // code is long, and an unrolled copy per row overflows the instruction cache)
+#pragma unroll 1
   for (int r = 0; r < TY / 4; r++) {
     const int ly = ly0 + 4 * r;
     const int i = x0 + lx, j = y0 + ly;
@@ -501,6 +513,7 @@ __global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL) {
   }
   __syncthreads();
   // ---- rhs0 (:46-54)
+#pragma unroll 1
   for (int r = 0; r < TY / 4; r++) {
     const int ly = ly0 + 4 * r;
     const int i = x0 + lx, j = y0 + ly;
@@ -948,11 +961,12 @@ void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float*
   hipLaunchKernelGGL(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout, dts);
 }
 unsigned sources_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
-void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, hipStream_t s) {
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, bool finish,
+                    hipStream_t s) {
   const unsigned nb = nblk(F.nx, F.ny);
   if (fuse) hipLaunchKernelGGL(k_sources<true>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
   else hipLaunchKernelGGL(k_sources<false>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
-  hipLaunchKernelGGL(k_acc_finish, dim3(1), dim3(1024), 0, s, (const int*)part, (int)nb, F.acc);
+  if (finish) hipLaunchKernelGGL(k_acc_finish, dim3(1), dim3(256), 0, s, (const int*)part, (int)nb, F.acc);
 }
 void launch_forces(const Fields& F, hipStream_t s) {
   hipLaunchKernelGGL(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
@@ -961,12 +975,12 @@ int forces_coef_cascade(int nlev) {
   // a 64x32 tile halves into coarse tiles down to 2x1 cells: five levels
   return nlev - 1 < 5 ? nlev - 1 : 5;
 }
-void launch_forces_coef(const Fields& F, const Level* L, int nlev, hipStream_t s) {
+void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* part, int nparts, hipStream_t s) {
   CoefLevels CL;
   for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
   CL.ncascade = forces_coef_cascade(nlev);
   const unsigned nb = (unsigned)(((F.nx + 63) / 64) * ((F.ny + 31) / 32));
-  hipLaunchKernelGGL(k_forces_coef, dim3(nb), dim3(64, 4), 0, s, F, CL);
+  hipLaunchKernelGGL(k_forces_coef, dim3(nb), dim3(64, 4), 0, s, F, CL, part, nparts);
 }
 void launch_coef_rhs(const Fields& F, hipStream_t s) {
   hipLaunchKernelGGL(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);

@@ -15,13 +15,82 @@
 
 namespace cf {
 
-// ---- tiled down leg ------------------------------------------------------------
-// Tile TX x TY fine cells per workgroup (TX, TY even), region = tile + halo.
+// ---- tiled legs ----------------------------------------------------------------
+// A tile of TX x TY fine cells plus a halo of H cells lives in LDS.  Each
+// thread owns horizontal pairs of region cells (one red, one black), keeps
+// their coefficients in registers and, per half-sweep, updates the pair's cell
+// of that colour: a branch-free update whose missing neighbours (domain edge)
+// read the cell itself with a zero coefficient.  Cells on the region edge
+// whose outward neighbour is a real cell are never updated; that error moves
+// inward one ring per half-sweep and stays out of the tile interior.
+struct PairCell {
+  float w, e, s, n, diag, rhs;
+  int nb;   // bit 0..3: neighbour west, east, south, north exists
+  bool upd, in;
+};
+
+__device__ __forceinline__ PairCell pair_cell(const Level& f, int x, int y, int rx, int ry, int RX, int RY) {
+  PairCell c;
+  const int nx = f.nx, ny = f.ny;
+  c.in = x >= 0 && x < nx && y >= 0 && y < ny;
+  c.upd = c.in && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
+          !(ry == RY - 1 && y < ny - 1);
+  c.w = c.e = c.s = c.n = c.diag = c.rhs = 0.f;
+  c.nb = 0;
+  if (c.in) {
+    const float w = f.ax[y * (nx + 1) + x], e = f.ax[y * (nx + 1) + x + 1];
+    const float s = f.ay[y * nx + x], n = f.ay[(y + 1) * nx + x];
+    float diag = 0.f;  // shaders/pressure.wgsl:77-85, same accumulation order
+    if (x > 0) diag += w; else diag += K(2.0) * w;
+    if (x < nx - 1) diag += e; else diag += K(2.0) * e;
+    if (y > 0) diag += s;
+    if (y < ny - 1) diag += n; else diag += K(2.0) * n;
+    c.diag = diag;
+    c.w = x > 0 ? w : 0.f;
+    c.e = x < nx - 1 ? e : 0.f;
+    c.s = y > 0 ? s : 0.f;
+    c.n = y < ny - 1 ? n : 0.f;
+    c.nb = (x > 0 ? 1 : 0) | (x < nx - 1 ? 2 : 0) | (y > 0 ? 4 : 0) | (y < ny - 1 ? 8 : 0);
+    c.rhs = f.rhs[y * nx + x];
+  }
+  return c;
+}
+
+__device__ __forceinline__ PairCell pair_none() {
+  PairCell c;
+  c.w = c.e = c.s = c.n = c.diag = c.rhs = 0.f;
+  c.nb = 0;
+  c.upd = c.in = false;
+  return c;
+}
+
+__device__ __forceinline__ float pair_off(const PairCell& c, const float* sp, int ci, int RX) {
+  const int iw = ci - (c.nb & 1), ie = ci + ((c.nb >> 1) & 1);
+  const int is = ci - RX * ((c.nb >> 2) & 1), in = ci + RX * ((c.nb >> 3) & 1);
+  return c.w * sp[iw] + c.e * sp[ie] + c.s * sp[is] + c.n * sp[in];
+}
+
+// one half-sweep of colour col over the thread's pairs
+template <int PQ, int RX, int NP>
+__device__ __forceinline__ void pairs_relax(const PairCell (&c)[PQ][2], const int (&base)[PQ], const int (&par)[PQ],
+                                            float* sp, float h2, int col) {
+#pragma unroll
+  for (int q = 0; q < PQ; q++) {
+    const int sub = par[q] == col ? 0 : 1;  // which cell of the pair has colour col
+    const PairCell& a = sub ? c[q][1] : c[q][0];
+    const int ci = base[q] + sub;
+    const float off = pair_off(a, sp, ci, RX);
+    const float v = a.diag > K(0) ? (off - a.rhs * h2) / a.diag : K(0);
+    if (a.upd) sp[ci] = v;
+  }
+}
+
 template <int TX, int TY, int NU>
 __global__ __launch_bounds__(256) void k_mg_down(Level f, Level c) {
   constexpr int H = 2 * NU + 1;
-  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY;
-  constexpr int Q = (NR + 255) / 256;
+  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
+  constexpr int PQ = (NP + 255) / 256;
+  static_assert(RX % 2 == 0, "pairs need an even region width");
   __shared__ float sp[NR];
   __shared__ float sres[TX * TY];
   const int nx = f.nx, ny = f.ny;
@@ -30,62 +99,44 @@ __global__ __launch_bounds__(256) void k_mg_down(Level f, Level c) {
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
   const int tid = threadIdx.x;
   const float h2 = f.h2;
-  float cw[Q], ce[Q], cs[Q], cn[Q], cr[Q];
-  int xs[Q], ys[Q];
-  bool in[Q], upd[Q];
+  PairCell cell[PQ][2];
+  int base[PQ], par[PQ];
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int cidx = tid + 256 * q;
-    const int rx = cidx % RX, ry = cidx / RX;
+  for (int q = 0; q < PQ; q++) {
+    const int pi = tid + 256 * q;
+    const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
     const int x = x0 - H + rx, y = y0 - H + ry;
-    xs[q] = x; ys[q] = y;
-    in[q] = cidx < NR && x >= 0 && x < nx && y >= 0 && y < ny;
-    // a cell on the region edge whose outward neighbour is a real cell cannot be updated
-    upd[q] = in[q] && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
-             !(ry == RY - 1 && y < ny - 1);
-    cw[q] = ce[q] = cs[q] = cn[q] = cr[q] = 0.f;
-    if (in[q]) {
-      cw[q] = f.ax[y * (nx + 1) + x]; ce[q] = f.ax[y * (nx + 1) + x + 1];
-      cs[q] = f.ay[y * nx + x]; cn[q] = f.ay[(y + 1) * nx + x];
-      cr[q] = f.rhs[y * nx + x];
-      sp[cidx] = f.p[y * nx + x];
-    } else if (cidx < NR) {
-      sp[cidx] = 0.f;
+    base[q] = ry * RX + rx;
+    par[q] = (x + y) & 1;
+    if (pi < NP) {
+      cell[q][0] = pair_cell(f, x, y, rx, ry, RX, RY);
+      cell[q][1] = pair_cell(f, x + 1, y, rx + 1, ry, RX, RY);
+      sp[base[q]] = cell[q][0].in ? f.p[y * nx + x] : 0.f;
+      sp[base[q] + 1] = cell[q][1].in ? f.p[y * nx + x + 1] : 0.f;
+    } else {
+      cell[q][0] = cell[q][1] = pair_none();
+      base[q] = 0;
     }
   }
   __syncthreads();
   for (int hs = 0; hs < 2 * NU; hs++) {
-    const int col = hs & 1;  // red (0) then black (1), js/fluid.js:298-301
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int x = xs[q], y = ys[q];
-      if (upd[q] && ((x + y) & 1) == col) {
-        const int cidx = tid + 256 * q;
-        float off = 0.f, diag = 0.f;
-        if (x > 0) { off += cw[q] * sp[cidx - 1]; diag += cw[q]; } else { diag += K(2.0) * cw[q]; }
-        if (x < nx - 1) { off += ce[q] * sp[cidx + 1]; diag += ce[q]; } else { diag += K(2.0) * ce[q]; }
-        if (y > 0) { off += cs[q] * sp[cidx - RX]; diag += cs[q]; }
-        if (y < ny - 1) { off += cn[q] * sp[cidx + RX]; diag += cn[q]; } else { diag += K(2.0) * cn[q]; }
-        sp[cidx] = diag > K(0) ? (off - cr[q] * h2) / diag : K(0);
-      }
-    }
+    pairs_relax<PQ, RX, NP>(cell, base, par, sp, h2, hs & 1);  // red (0) then black (1), js/fluid.js:298-301
     __syncthreads();
   }
   // interior: write p, residual into sres (shaders/pressure.wgsl:96-101)
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int cidx = tid + 256 * q;
-    const int rx = cidx % RX, ry = cidx / RX;
-    if (in[q] && rx >= H && rx < H + TX && ry >= H && ry < H + TY) {
-      const int x = xs[q], y = ys[q];
-      float off = 0.f, diag = 0.f;
-      if (x > 0) { off += cw[q] * sp[cidx - 1]; diag += cw[q]; } else { diag += K(2.0) * cw[q]; }
-      if (x < nx - 1) { off += ce[q] * sp[cidx + 1]; diag += ce[q]; } else { diag += K(2.0) * ce[q]; }
-      if (y > 0) { off += cs[q] * sp[cidx - RX]; diag += cs[q]; }
-      if (y < ny - 1) { off += cn[q] * sp[cidx + RX]; diag += cn[q]; } else { diag += K(2.0) * cn[q]; }
-      const float pk = sp[cidx];
-      sres[(ry - H) * TX + (rx - H)] = diag <= K(0) ? K(0) : cr[q] - (off - diag * pk) / h2;
-      f.p[y * nx + x] = pk;
+  for (int q = 0; q < PQ; q++) {
+#pragma unroll
+    for (int sub = 0; sub < 2; sub++) {
+      const PairCell& a = cell[q][sub];
+      const int ci = base[q] + sub;
+      const int rx = ci % RX, ry = ci / RX;
+      if (a.in && rx >= H && rx < H + TX && ry >= H && ry < H + TY) {
+        const float pk = sp[ci];
+        const float off = pair_off(a, sp, ci, RX);
+        sres[(ry - H) * TX + (rx - H)] = a.diag <= K(0) ? K(0) : a.rhs - (off - a.diag * pk) / h2;
+        f.p[(y0 - H + ry) * nx + (x0 - H + rx)] = pk;
+      }
     }
   }
   __syncthreads();
@@ -102,12 +153,12 @@ __global__ __launch_bounds__(256) void k_mg_down(Level f, Level c) {
   }
 }
 
-// ---- tiled up leg ----------------------------------------------------------------
 template <int TX, int TY, int NU>
 __global__ __launch_bounds__(256) void k_mg_up(Level f, Level c) {
   constexpr int H = 2 * NU;
-  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY;
-  constexpr int Q = (NR + 255) / 256;
+  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
+  constexpr int PQ = (NP + 255) / 256;
+  static_assert(RX % 2 == 0, "pairs need an even region width");
   __shared__ float sp[NR];
   const int nx = f.nx, ny = f.ny;
   const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
@@ -116,197 +167,264 @@ __global__ __launch_bounds__(256) void k_mg_up(Level f, Level c) {
   const int tid = threadIdx.x;
   const float h2 = f.h2;
   const int nxc = c.nx, nyc = c.ny;
-  float cw[Q], ce[Q], cs[Q], cn[Q], cr[Q];
-  int xs[Q], ys[Q];
-  bool in[Q], upd[Q];
+  PairCell cell[PQ][2];
+  int base[PQ], par[PQ];
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int cidx = tid + 256 * q;
-    const int rx = cidx % RX, ry = cidx / RX;
-    const int x = x0 - H + rx, y = y0 - H + ry;
-    xs[q] = x; ys[q] = y;
-    in[q] = cidx < NR && x >= 0 && x < nx && y >= 0 && y < ny;
-    upd[q] = in[q] && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
-             !(ry == RY - 1 && y < ny - 1);
-    cw[q] = ce[q] = cs[q] = cn[q] = cr[q] = 0.f;
-    if (in[q]) {
-      cw[q] = f.ax[y * (nx + 1) + x]; ce[q] = f.ax[y * (nx + 1) + x + 1];
-      cs[q] = f.ay[y * nx + x]; cn[q] = f.ay[(y + 1) * nx + x];
-      cr[q] = f.rhs[y * nx + x];
-      // bilinear coarse correction, skipped in closed cells (:116-129)
-      const int I = x / 2, J = y / 2;
-      const int I2 = iclamp(I + ((x & 1) == 1 ? 1 : -1), 0, nxc - 1);
-      const int J2 = iclamp(J + ((y & 1) == 1 ? 1 : -1), 0, nyc - 1);
-      const float cc = K(0.5625) * c.p[J * nxc + I] + K(0.1875) * (c.p[J * nxc + I2] + c.p[J2 * nxc + I]) +
-                       K(0.0625) * c.p[J2 * nxc + I2];
-      float diag = 0.f;
-      if (x > 0) diag += cw[q]; else diag += K(2.0) * cw[q];
-      if (x < nx - 1) diag += ce[q]; else diag += K(2.0) * ce[q];
-      if (y > 0) diag += cs[q];
-      if (y < ny - 1) diag += cn[q]; else diag += K(2.0) * cn[q];
-      sp[cidx] = f.p[y * nx + x] + (diag > K(0) ? cc : K(0));
-    } else if (cidx < NR) {
-      sp[cidx] = 0.f;
+  for (int q = 0; q < PQ; q++) {
+    const int pi = tid + 256 * q;
+    const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
+    const int y = y0 - H + ry;
+    base[q] = ry * RX + rx;
+    par[q] = (x0 - H + rx + y) & 1;
+    if (pi < NP) {
+#pragma unroll
+      for (int sub = 0; sub < 2; sub++) {
+        const int x = x0 - H + rx + sub;
+        cell[q][sub] = pair_cell(f, x, y, rx + sub, ry, RX, RY);
+        float v = 0.f;
+        if (cell[q][sub].in) {
+          // bilinear coarse correction, skipped in closed cells (:116-129)
+          const int I = x / 2, J = y / 2;
+          const int I2 = iclamp(I + ((x & 1) == 1 ? 1 : -1), 0, nxc - 1);
+          const int J2 = iclamp(J + ((y & 1) == 1 ? 1 : -1), 0, nyc - 1);
+          const float cc = K(0.5625) * c.p[J * nxc + I] + K(0.1875) * (c.p[J * nxc + I2] + c.p[J2 * nxc + I]) +
+                           K(0.0625) * c.p[J2 * nxc + I2];
+          v = f.p[y * nx + x] + (cell[q][sub].diag > K(0) ? cc : K(0));
+        }
+        sp[base[q] + sub] = v;
+      }
+    } else {
+      cell[q][0] = cell[q][1] = pair_none();
+      base[q] = 0;
     }
   }
   __syncthreads();
   for (int hs = 0; hs < 2 * NU; hs++) {
-    const int col = (hs & 1) ? 0 : 1;  // black (1) then red (0), js/fluid.js:307-310
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int x = xs[q], y = ys[q];
-      if (upd[q] && ((x + y) & 1) == col) {
-        const int cidx = tid + 256 * q;
-        float off = 0.f, diag = 0.f;
-        if (x > 0) { off += cw[q] * sp[cidx - 1]; diag += cw[q]; } else { diag += K(2.0) * cw[q]; }
-        if (x < nx - 1) { off += ce[q] * sp[cidx + 1]; diag += ce[q]; } else { diag += K(2.0) * ce[q]; }
-        if (y > 0) { off += cs[q] * sp[cidx - RX]; diag += cs[q]; }
-        if (y < ny - 1) { off += cn[q] * sp[cidx + RX]; diag += cn[q]; } else { diag += K(2.0) * cn[q]; }
-        sp[cidx] = diag > K(0) ? (off - cr[q] * h2) / diag : K(0);
-      }
-    }
+    pairs_relax<PQ, RX, NP>(cell, base, par, sp, h2, (hs & 1) ? 0 : 1);  // black then red, js/fluid.js:307-310
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int cidx = tid + 256 * q;
-    const int rx = cidx % RX, ry = cidx / RX;
-    if (in[q] && rx >= H && rx < H + TX && ry >= H && ry < H + TY) f.p[ys[q] * nx + xs[q]] = sp[cidx];
+  for (int q = 0; q < PQ; q++) {
+#pragma unroll
+    for (int sub = 0; sub < 2; sub++) {
+      const int ci = base[q] + sub;
+      const int rx = ci % RX, ry = ci / RX;
+      if (cell[q][sub].in && rx >= H && rx < H + TX && ry >= H && ry < H + TY)
+        f.p[(y0 - H + ry) * nx + (x0 - H + rx)] = sp[ci];
+    }
   }
 }
 
 // ---- bottom levels in one workgroup ------------------------------------------------
-// LDS layout per level: p[n], rhs[n], ax[(nx+1)ny], ay[nx(ny+1)].
-struct BottomLevels {
-  Level L[4];
-  int off[4];  // float offset of each level's block in LDS
-  int nlev;    // levels handled here (the last one is the coarsest)
-  int nu, coarse_iters, first_is_level0;
+// The last one to three levels of the hierarchy hold <= 4096, <= 1024 and
+// <= 256 cells (each level above the coarsest has more than 256 cells and
+// four times the cells of the next).  One workgroup of 1024 threads keeps p
+// and rhs of all of them in LDS and each thread's cells' coefficients in
+// registers (4, 1 and 1 cells per thread), so a red-black half-sweep is one
+// pass of branch-free updates between two barriers: neighbours missing at the
+// domain edge are read at the cell itself with a zero coefficient.  The
+// coarsest level's face coefficients are restricted here from the level above
+// when the caller has not produced them.
+constexpr int BT = 1024;
+
+struct Bottom {
+  Level L[3];       // finest first; the last is the coarsest level
+  int n;            // 1..3
+  int coarse_coef;  // 1: restrict the coarsest level's ax/ay from L[n-2] here
+  int nu, coarse_iters;
 };
 
-// one half-sweep of a bottom level held in LDS; 16x16 threads walk rows and
-// same-colour columns without integer division
-__device__ __forceinline__ void b_relax(const Level& L, float* p, const float* rhs, const float* ax, const float* ay,
-                                        int col) {
-  const int nx = L.nx, ny = L.ny;
-  const int half = (nx + 1) / 2;
-  const float h2 = L.h2;
-  for (int j = threadIdx.y; j < ny; j += 16) {
-    const int par = (j + col) & 1;
-    for (int ii = threadIdx.x; ii < half; ii += 16) {
-      const int i = 2 * ii + par;
-      if (i >= nx) continue;
+template <int CPT>
+struct CellSet {
+  int k[CPT];               // linear cell index (or -1)
+  int iw[CPT], ie[CPT], is[CPT], in[CPT];
+  float w[CPT], e[CPT], s[CPT], n[CPT], diag[CPT];
+  int col[CPT];
+};
+
+// cell data of level L for thread t: cells t + BT*q; coefficient arrays may be
+// in global memory (ax, ay) or in LDS
+template <int CPT>
+__device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, const float* ax, const float* ay) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int k = t + BT * q;
+    C.k[q] = -1;
+    C.w[q] = C.e[q] = C.s[q] = C.n[q] = C.diag[q] = 0.f;
+    C.iw[q] = C.ie[q] = C.is[q] = C.in[q] = 0;
+    C.col[q] = 2;
+    if (k < nx * ny) {
+      const int i = k % nx, j = k / nx;
       const float w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], s = ay[j * nx + i], n = ay[(j + 1) * nx + i];
-      float off = 0.f, diag = 0.f;
-      if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else { diag += K(2.0) * w; }
-      if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else { diag += K(2.0) * e; }
-      if (j > 0) { off += s * p[(j - 1) * nx + i]; diag += s; }
-      if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else { diag += K(2.0) * n; }
-      const int k2 = j * nx + i;
-      p[k2] = diag > K(0) ? (off - rhs[k2] * h2) / diag : K(0);
+      float diag = 0.f;  // the diagonal of shaders/pressure.wgsl:77-85, same accumulation order
+      if (i > 0) diag += w; else diag += K(2.0) * w;
+      if (i < nx - 1) diag += e; else diag += K(2.0) * e;
+      if (j > 0) diag += s;
+      if (j < ny - 1) diag += n; else diag += K(2.0) * n;
+      C.k[q] = k;
+      C.diag[q] = diag;
+      C.w[q] = i > 0 ? w : 0.f;      C.iw[q] = i > 0 ? k - 1 : k;
+      C.e[q] = i < nx - 1 ? e : 0.f; C.ie[q] = i < nx - 1 ? k + 1 : k;
+      C.s[q] = j > 0 ? s : 0.f;      C.is[q] = j > 0 ? k - nx : k;
+      C.n[q] = j < ny - 1 ? n : 0.f; C.in[q] = j < ny - 1 ? k + nx : k;
+      C.col[q] = (i + j) & 1;
     }
   }
 }
 
-__device__ __forceinline__ float b_residual(const Level& L, const float* p, const float* rhs, const float* ax,
-                                            const float* ay, int i, int j) {
-  const int nx = L.nx, ny = L.ny;
-  const float w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], s = ay[j * nx + i], n = ay[(j + 1) * nx + i];
-  float off = 0.f, diag = 0.f;
-  if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else { diag += K(2.0) * w; }
-  if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else { diag += K(2.0) * e; }
-  if (j > 0) { off += s * p[(j - 1) * nx + i]; diag += s; }
-  if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else { diag += K(2.0) * n; }
-  if (diag <= K(0)) return K(0);
-  return rhs[j * nx + i] - (off - diag * p[j * nx + i]) / L.h2;
+template <int CPT>
+__device__ __forceinline__ void cells_relax(const CellSet<CPT>& C, float* p, const float* rhs, float h2, int col) {
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    if (C.col[q] == col) {
+      const int k = C.k[q];
+      const float off = C.w[q] * p[C.iw[q]] + C.e[q] * p[C.ie[q]] + C.s[q] * p[C.is[q]] + C.n[q] * p[C.in[q]];
+      p[k] = C.diag[q] > K(0) ? (off - rhs[k] * h2) / C.diag[q] : K(0);
+    }
+  }
 }
 
-__global__ __launch_bounds__(256) void k_mg_bottom(BottomLevels B) {
-  extern __shared__ float lds[];
-  const int T = 256;  // 16 x 16 threads
-  const int tid = threadIdx.y * 16 + threadIdx.x;
-  // load coefficients of every level and p, rhs of the first
-  for (int l = 0; l < B.nlev; l++) {
-    const Level& L = B.L[l];
-    const int n = L.nx * L.ny, nax = (L.nx + 1) * L.ny, nay = L.nx * (L.ny + 1);
-    float* base = lds + B.off[l];
-    float *p = base, *rhs = base + n, *ax = rhs + n, *ay = ax + nax;
-    for (int k = tid; k < nax; k += T) ax[k] = L.ax[k];
-    for (int k = tid; k < nay; k += T) ay[k] = L.ay[k];
-    if (l == 0)
-      for (int k = tid; k < n; k += T) { p[k] = L.p[k]; rhs[k] = L.rhs[k]; }
+template <int CPT>
+__device__ __forceinline__ void cells_residual(const CellSet<CPT>& C, const float* p, const float* rhs, float h2,
+                                               float* res) {
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int k = C.k[q];
+    if (k >= 0) {
+      const float off = C.w[q] * p[C.iw[q]] + C.e[q] * p[C.ie[q]] + C.s[q] * p[C.is[q]] + C.n[q] * p[C.in[q]];
+      res[k] = C.diag[q] <= K(0) ? K(0) : rhs[k] - (off - C.diag[q] * p[k]) / h2;
+    }
+  }
+}
+
+// coarse rhs = 1/4 of the 4 fine residuals (shaders/pressure.wgsl:104-113)
+__device__ __forceinline__ void restrict_to(const float* res, int nxf, float* rhsc, float* pc, int nxc, int nyc) {
+  for (int k = threadIdx.x; k < nxc * nyc; k += BT) {
+    const int I = k % nxc, J = k / nxc;
+    const float* r0 = res + (2 * J) * nxf + 2 * I;
+    rhsc[k] = K(0.25) * (r0[0] + r0[1] + r0[nxf] + r0[nxf + 1]);
+    pc[k] = K(0);
+  }
+}
+
+// bilinear coarse correction (shaders/pressure.wgsl:116-129)
+template <int CPT>
+__device__ __forceinline__ void cells_prolong(const CellSet<CPT>& C, int nx, float* p, const float* pc, int nxc, int nyc) {
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int k = C.k[q];
+    if (k >= 0) {
+      const int i = k % nx, j = k / nx;
+      const int I = i / 2, J = j / 2;
+      const int I2 = iclamp(I + ((i & 1) == 1 ? 1 : -1), 0, nxc - 1);
+      const int J2 = iclamp(J + ((j & 1) == 1 ? 1 : -1), 0, nyc - 1);
+      const float cc = K(0.5625) * pc[J * nxc + I] + K(0.1875) * (pc[J * nxc + I2] + pc[J2 * nxc + I]) +
+                       K(0.0625) * pc[J2 * nxc + I2];
+      p[k] += C.diag[q] > K(0) ? cc : K(0);
+    }
+  }
+}
+
+// LDS of the bottom solver: p and rhs of the three slots, then the residual scratch
+constexpr int BOT_P0 = 0, BOT_R0 = 4096, BOT_P1 = 8192, BOT_R1 = 9216, BOT_P2 = 10240, BOT_R2 = 10496,
+              BOT_RES = 10752, BOT_TOTAL = 14848;
+__device__ __forceinline__ int bot_p(int s) { return s == 0 ? BOT_P0 : (s == 1 ? BOT_P1 : BOT_P2); }
+__device__ __forceinline__ int bot_r(int s) { return s == 0 ? BOT_R0 : (s == 1 ? BOT_R1 : BOT_R2); }
+
+__global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
+  __shared__ float sm[BOT_TOTAL];
+  float* sp2 = sm + BOT_P2;
+  float* sr2 = sm + BOT_R2;
+  float* sres = sm + BOT_RES;  // fine residuals; first holds the coarsest level's coefficients
+  auto P = [&](int s) { return sm + bot_p(s); };
+  auto R = [&](int s) { return sm + bot_r(s); };
+  const int nb = B.n, t = threadIdx.x;
+  const Level& Lc = B.L[nb - 1];
+  // coarsest-level coefficients: from global memory, or restricted here (:57-67)
+  const float *axc = Lc.ax, *ayc = Lc.ay;
+  if (B.coarse_coef) {
+    const Level& Lf = B.L[nb - 2];
+    float* ax = sres;
+    float* ay = sres + (Lc.nx + 1) * Lc.ny;
+    for (int k = t; k < (Lc.nx + 1) * Lc.ny; k += BT) {
+      const int I = k % (Lc.nx + 1), J = k / (Lc.nx + 1);
+      const float v = K(0.5) * (Lf.ax[(2 * J) * (Lf.nx + 1) + 2 * I] + Lf.ax[(2 * J + 1) * (Lf.nx + 1) + 2 * I]);
+      ax[k] = v; Lc.ax[k] = v;
+    }
+    for (int k = t; k < Lc.nx * (Lc.ny + 1); k += BT) {
+      const int I = k % Lc.nx, J = k / Lc.nx;
+      const float v = K(0.5) * (Lf.ay[(2 * J) * Lf.nx + 2 * I] + Lf.ay[(2 * J) * Lf.nx + 2 * I + 1]);
+      ay[k] = v; Lc.ay[k] = v;
+    }
+    __syncthreads();
+    axc = ax; ayc = ay;
+  }
+  // per-thread cell data; level 0 of this kernel may hold up to 4096 cells
+  CellSet<4> C0;
+  CellSet<1> C1, C2;
+  {
+    const Level& L0 = B.L[0];
+    if (nb == 1) cells_load<1>(C2, L0.nx, L0.ny, axc, ayc);
+    else if (nb == 2) { cells_load<1>(C1, L0.nx, L0.ny, L0.ax, L0.ay); cells_load<1>(C2, Lc.nx, Lc.ny, axc, ayc); }
+    else {
+      cells_load<4>(C0, L0.nx, L0.ny, L0.ax, L0.ay);
+      cells_load<1>(C1, B.L[1].nx, B.L[1].ny, B.L[1].ax, B.L[1].ay);
+      cells_load<1>(C2, Lc.nx, Lc.ny, axc, ayc);
+    }
+    // p (warm start or zero) and rhs of the first level; slot = 3 - nb
+    const int s0 = 3 - nb;
+    for (int k = t; k < L0.nx * L0.ny; k += BT) { P(s0)[k] = L0.p[k]; R(s0)[k] = L0.rhs[k]; }
   }
   __syncthreads();
-  auto P = [&](int l) { return lds + B.off[l]; };
-  auto R = [&](int l) { return lds + B.off[l] + B.L[l].nx * B.L[l].ny; };
-  auto AX = [&](int l) { return R(l) + B.L[l].nx * B.L[l].ny; };
-  auto AY = [&](int l) { return AX(l) + (B.L[l].nx + 1) * B.L[l].ny; };
-  const int last = B.nlev - 1;
+  // slot s holds level B.L[s - (3 - nb)]
+  const int s0 = 3 - nb;
+  auto lev = [&](int s) -> const Level& { return B.L[s - s0]; };
   // down (js/fluid.js:297-303)
-  for (int l = 0; l < last; l++) {
-    const Level& L = B.L[l];
+  for (int s = s0; s < 2; s++) {
+    const Level& L = lev(s);
     for (int q = 0; q < B.nu; q++) {
-      b_relax(L, P(l), R(l), AX(l), AY(l), 0); __syncthreads();
-      b_relax(L, P(l), R(l), AX(l), AY(l), 1); __syncthreads();
+      for (int col = 0; col < 2; col++) {
+        if (s == 0) cells_relax<4>(C0, P(0), R(0), L.h2, col); else cells_relax<1>(C1, P(1), R(1), L.h2, col);
+        __syncthreads();
+      }
     }
-    const Level& C = B.L[l + 1];
-    float *pc = P(l + 1), *rc = R(l + 1);
-    for (int k = tid; k < C.nx * C.ny; k += T) {
-      const int I = k % C.nx, J = k / C.nx;
-      const float r = b_residual(L, P(l), R(l), AX(l), AY(l), 2 * I, 2 * J) +
-                      b_residual(L, P(l), R(l), AX(l), AY(l), 2 * I + 1, 2 * J) +
-                      b_residual(L, P(l), R(l), AX(l), AY(l), 2 * I, 2 * J + 1) +
-                      b_residual(L, P(l), R(l), AX(l), AY(l), 2 * I + 1, 2 * J + 1);
-      rc[k] = K(0.25) * r;
-      pc[k] = K(0);
-    }
+    if (s == 0) cells_residual<4>(C0, P(0), R(0), L.h2, sres); else cells_residual<1>(C1, P(1), R(1), L.h2, sres);
+    __syncthreads();
+    const Level& Cn = lev(s + 1);
+    restrict_to(sres, L.nx, R(s + 1), P(s + 1), Cn.nx, Cn.ny);
     __syncthreads();
   }
-  // coarsest: sweeps from zero (shaders/pressure.wgsl:131-156)
-  {
-    const Level& L = B.L[last];
-    float* p = P(last);
-    for (int k = tid; k < L.nx * L.ny; k += T) p[k] = K(0);
+  // coarsest: red-black sweeps from zero (shaders/pressure.wgsl:131-156)
+  for (int k = t; k < Lc.nx * Lc.ny; k += BT) sp2[k] = K(0);
+  __syncthreads();
+  for (int it = 0; it < B.coarse_iters; it++) {
+    cells_relax<1>(C2, sp2, sr2, Lc.h2, 0);
     __syncthreads();
-    for (int it = 0; it < B.coarse_iters; it++) {
-      b_relax(L, p, R(last), AX(last), AY(last), 0); __syncthreads();
-      b_relax(L, p, R(last), AX(last), AY(last), 1); __syncthreads();
-    }
+    cells_relax<1>(C2, sp2, sr2, Lc.h2, 1);
+    __syncthreads();
   }
   // up (js/fluid.js:305-311)
-  for (int l = last - 1; l >= 0; l--) {
-    const Level& L = B.L[l];
-    const Level& C = B.L[l + 1];
-    float *p = P(l), *pc = P(l + 1);
-    const float *ax = AX(l), *ay = AY(l);
-    for (int k = tid; k < L.nx * L.ny; k += T) {
-      const int i = k % L.nx, j = k / L.nx;
-      const int I = i / 2, J = j / 2;
-      const int I2 = iclamp(I + ((i & 1) == 1 ? 1 : -1), 0, C.nx - 1);
-      const int J2 = iclamp(J + ((j & 1) == 1 ? 1 : -1), 0, C.ny - 1);
-      const float cc = K(0.5625) * pc[J * C.nx + I] + K(0.1875) * (pc[J * C.nx + I2] + pc[J2 * C.nx + I]) +
-                       K(0.0625) * pc[J2 * C.nx + I2];
-      float diag = 0.f;
-      const int nx = L.nx;
-      if (i > 0) diag += ax[j * (nx + 1) + i]; else diag += K(2.0) * ax[j * (nx + 1) + i];
-      if (i < nx - 1) diag += ax[j * (nx + 1) + i + 1]; else diag += K(2.0) * ax[j * (nx + 1) + i + 1];
-      if (j > 0) diag += ay[j * nx + i];
-      if (j < L.ny - 1) diag += ay[(j + 1) * nx + i]; else diag += K(2.0) * ay[(j + 1) * nx + i];
-      p[k] += diag > K(0) ? cc : K(0);
-    }
+  for (int s = 1; s >= s0; s--) {
+    const Level& L = lev(s);
+    const Level& Cn = lev(s + 1);
+    if (s == 0) cells_prolong<4>(C0, L.nx, P(0), P(1), Cn.nx, Cn.ny);
+    else cells_prolong<1>(C1, L.nx, P(1), P(2), Cn.nx, Cn.ny);
     __syncthreads();
     for (int q = 0; q < B.nu; q++) {
-      b_relax(L, p, R(l), ax, ay, 1); __syncthreads();
-      b_relax(L, p, R(l), ax, ay, 0); __syncthreads();
+      for (int c = 0; c < 2; c++) {
+        const int col = 1 - c;
+        if (s == 0) cells_relax<4>(C0, P(0), R(0), L.h2, col); else cells_relax<1>(C1, P(1), R(1), L.h2, col);
+        __syncthreads();
+      }
     }
   }
-  // write back p and rhs of every bottom level (the first level's rhs is unchanged)
-  for (int l = 0; l < B.nlev; l++) {
-    const Level& L = B.L[l];
-    const int n = L.nx * L.ny;
-    for (int k = tid; k < n; k += T) {
-      L.p[k] = P(l)[k];
-      if (l > 0) L.rhs[k] = R(l)[k];
+  // write back p of every level and rhs of the levels below the first
+  for (int s = s0; s < 3; s++) {
+    const Level& L = lev(s);
+    for (int k = t; k < L.nx * L.ny; k += BT) {
+      L.p[k] = P(s)[k];
+      if (s > s0) L.rhs[k] = R(s)[k];
     }
   }
 }
@@ -338,25 +456,25 @@ bool launch_mg_up(const Level& f, const Level& c, int nu, hipStream_t s) {
   }
 }
 
-size_t mg_bottom_lds_bytes(const Level* L, int nlev) {
-  size_t fl = 0;
-  for (int l = 0; l < nlev; l++) fl += 2 * (size_t)L[l].nx * L[l].ny + (size_t)(L[l].nx + 1) * L[l].ny + (size_t)L[l].nx * (L[l].ny + 1);
-  return fl * 4;
+// the bottom kernel takes the last min(3, nlev) levels when each fits its slot
+int mg_bottom_first(const Level* L, int nlev) {
+  const int cap[3] = {256, 1024, 4096};
+  int b = nlev - 1;
+  for (int d = 1; d < 3 && b - 1 >= 0; d++) {
+    if (L[b - 1].nx * L[b - 1].ny > cap[d]) break;
+    b--;
+  }
+  return b;
 }
 
-bool launch_mg_bottom(const Level* L, int nlev, int nu, int coarse_iters, hipStream_t s) {
-  if (nlev < 1 || nlev > 4) return false;
-  BottomLevels B;
-  int off = 0;
-  for (int l = 0; l < nlev; l++) {
-    B.L[l] = L[l];
-    B.off[l] = off;
-    off += 2 * L[l].nx * L[l].ny + (L[l].nx + 1) * L[l].ny + L[l].nx * (L[l].ny + 1);
-  }
-  B.nlev = nlev; B.nu = nu; B.coarse_iters = coarse_iters; B.first_is_level0 = 0;
-  const size_t bytes = (size_t)off * 4;
-  if (bytes > 64 * 1024) return false;
-  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(16, 16), bytes, s, B);
+bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, hipStream_t s) {
+  const int n = nlev - first;
+  if (n < 1 || n > 3 || L[nlev - 1].nx * L[nlev - 1].ny > 256) return false;
+  if (coarse_coef && n < 2) return false;
+  Bottom B;
+  for (int q = 0; q < n; q++) B.L[q] = L[first + q];
+  B.n = n; B.coarse_coef = coarse_coef ? 1 : 0; B.nu = nu; B.coarse_iters = coarse_iters;
+  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(BT), 0, s, B);
   return true;
 }
 

@@ -99,7 +99,8 @@ struct cf_sim {
   std::vector<float> hPart;
   int* dAccPart = nullptr;  // per-workgroup partial sums of the sources kernel
   // graph of one step
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;   // one step
+  hipGraphExec_t gexec8 = nullptr;  // eight steps: one replay per 8 steps amortises the launch
   int gkey[6] = {-1, -1, -1, -1, -1, -1};
   bool use_graph = true;
   bool mg_simple = false;  // one launch per reference dispatch (debug comparisons)
@@ -136,7 +137,7 @@ static void enqueue_vcycle(cf_sim* s, int nu) {  // js/fluid.js:294-312
     // fused: tiled down legs, one LDS-resident workgroup for the bottom levels, tiled up legs
     const int b = s->mg_bottom;
     for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, s->st); mark(s, "mg_down"); }
-    cf::launch_mg_bottom(&s->L[b], L - b, nu, s->sch.coarse_iters, s->st); mark(s, "mg_bottom");
+    cf::launch_mg_bottom(s->L, L, b, false, nu, s->sch.coarse_iters, s->st); mark(s, "mg_bottom");
     for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, s->st); mark(s, "mg_up"); }
     return;
   }
@@ -179,7 +180,7 @@ static int enqueue_stage(cf_sim* s, int stage) {
         cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
       }
       break;
-    case 2: cf::launch_sources(F, false, nullptr, nullptr, 0.f, s->dAccPart, s->st); mark(s, "sources"); break;
+    case 2: cf::launch_sources(F, false, nullptr, nullptr, 0.f, s->dAccPart, true, s->st); mark(s, "sources"); break;
     case 3: cf::launch_forces(F, s->st); mark(s, "forces"); break;
     case 4: enqueue_coef(s); break;
     case 5:
@@ -187,7 +188,7 @@ static int enqueue_stage(cf_sim* s, int stage) {
       break;
     case 6: cf::launch_project(F, s->st); mark(s, "project"); break;
     case 7:
-      cf::launch_forces_coef(F, s->L, s->nlev, s->st); mark(s, "forces_coef");
+      cf::launch_forces_coef(F, s->L, s->nlev, nullptr, 0, s->st); mark(s, "forces_coef");
       for (int l = 1 + cf::forces_coef_cascade(s->nlev); l < s->nlev; l++) {
         cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, "coef_restrict");
       }
@@ -197,50 +198,74 @@ static int enqueue_stage(cf_sim* s, int stage) {
   return CF_OK;
 }
 
-// one step (js/fluid.js:225-253); the last diffusion substep runs inside sources
+// one step (js/fluid.js:225-253); the last diffusion substep runs inside
+// sources, forces through the coefficient cascade run as one pass which also
+// folds the accumulator partials, and the coarsest level's coefficients are
+// restricted inside the bottom solver
 static void enqueue_step(cf_sim* s) {
   Fields& F = s->F;
+  const bool fused = !s->mg_simple;
   enqueue_stage(s, 0);
-  if (s->diffSub == 1) {
-    cf::launch_sources(F, true, F.T2, F.Y2, s->dsub, s->dAccPart, s->st); mark(s, "diffuse_sources");
-  } else {
+  const float* Tin = F.T2;
+  const float4* Yin = F.Y2;
+  if (s->diffSub > 1) {
     cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
     for (int q = 1; q < s->diffSub; q += 2) {
       cf::launch_diffuse(F, F.T, F.Y, F.Th, F.Yh, s->dsub, s->st); mark(s, "diffuse");
-      if (q + 2 < s->diffSub) {
-        cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
-      } else {
-        cf::launch_sources(F, true, F.Th, F.Yh, s->dsub, s->dAccPart, s->st); mark(s, "diffuse_sources");
-      }
+      if (q + 2 < s->diffSub) { cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse"); }
     }
+    Tin = F.Th;
+    Yin = F.Yh;
   }
-  if (s->mg_simple) {
+  cf::launch_sources(F, true, Tin, Yin, s->dsub, s->dAccPart, !fused, s->st); mark(s, "diffuse_sources");
+  if (!fused) {
     enqueue_stage(s, 3);
     enqueue_stage(s, 4);
+    enqueue_stage(s, 5);
   } else {
-    enqueue_stage(s, 7);
+    cf::launch_forces_coef(F, s->L, s->nlev, s->dAccPart, (int)cf::sources_blocks(F), s->st); mark(s, "forces_coef");
+    const int last_coef = s->nlev - 1 - (s->mg_bottom < s->nlev - 1 ? 1 : 0);
+    for (int l = 1 + cf::forces_coef_cascade(s->nlev); l <= last_coef; l++) {
+      cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, "coef_restrict");
+    }
+    const bool coarse_coef = last_coef < s->nlev - 1 && s->nlev - 1 > cf::forces_coef_cascade(s->nlev);
+    for (int c = 0; c < s->sch.vcycles; c++) {
+      const int b = s->mg_bottom;
+      for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, "mg_down"); }
+      // the coarsest coefficients are restricted once, in the first V-cycle's bottom solve
+      cf::launch_mg_bottom(s->L, s->nlev, b, coarse_coef && c == 0, s->sch.nu, s->sch.coarse_iters, s->st);
+      mark(s, "mg_bottom");
+      for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, "mg_up"); }
+    }
   }
-  enqueue_stage(s, 5);
   enqueue_stage(s, 6);
 }
 
 static void drop_graph(cf_sim* s) {
   if (s->gexec) hipGraphExecDestroy(s->gexec);
-  s->gexec = nullptr;
+  if (s->gexec8) hipGraphExecDestroy(s->gexec8);
+  s->gexec = s->gexec8 = nullptr;
   for (int q = 0; q < 6; q++) s->gkey[q] = -1;
+}
+
+static int capture(cf_sim* s, int nsteps, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  for (int q = 0; q < nsteps; q++) enqueue_step(s);
+  HIPCHK(hipStreamEndCapture(s->st, &g));
+  hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) { *out = nullptr; return set_err(CF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+  return CF_OK;
 }
 
 static int ensure_graph(cf_sim* s) {
   const int key[6] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev};
-  if (s->gexec && !memcmp(key, s->gkey, sizeof(key))) return CF_OK;
+  if (s->gexec && s->gexec8 && !memcmp(key, s->gkey, sizeof(key))) return CF_OK;
   drop_graph(s);
-  hipGraph_t g = nullptr;
-  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-  enqueue_step(s);
-  HIPCHK(hipStreamEndCapture(s->st, &g));
-  hipError_t e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
-  if (e != hipSuccess) { s->gexec = nullptr; return set_err(CF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+  int rc = capture(s, 1, &s->gexec);
+  if (!rc) rc = capture(s, 8, &s->gexec8);
+  if (rc) { drop_graph(s); return rc; }
   memcpy(s->gkey, key, sizeof(key));
   return CF_OK;
 }
@@ -407,10 +432,8 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
   F.P = s->dP; F.nx = nx; F.ny = ny;
   F.p0 = s->L[0].p; F.rhs0 = s->L[0].rhs; F.ax0 = s->L[0].ax; F.ay0 = s->L[0].ay;
   for (int l = 0; l < s->nlev; l++) { s->L[l].nx = s->lnx[l]; s->L[l].ny = s->lny[l]; s->L[l].h2 = s->lh2[l]; }
-  // bottom solver: the longest run of coarse levels whose arrays fit in 64 KB of LDS
-  s->mg_bottom = s->nlev - 1;
-  while (s->mg_bottom > 0 && cf::mg_bottom_lds_bytes(&s->L[s->mg_bottom - 1], s->nlev - s->mg_bottom + 1) <= 64 * 1024)
-    s->mg_bottom--;
+  // bottom solver: the last 1-3 levels (<= 4096, 1024, 256 cells) in one workgroup
+  s->mg_bottom = cf::mg_bottom_first(s->L, s->nlev);
   // device buffers start zeroed, as WebGPU buffers do
   cf::launch_fill_u32(s->block, 0u, total / 4, s->st);
 
@@ -583,10 +606,10 @@ int cf_sim_step(cf_sim* s, int nsteps) {  // js/fluid.js:225-253
   if (s->use_graph) {
     int rc = ensure_graph(s);
     if (rc) return rc;
-    for (int q = 0; q < nsteps; q++) {
-      HIPCHK(hipGraphLaunch(s->gexec, s->st));
-      s->time += s->dt;
-    }
+    int q = 0;
+    for (; q + 8 <= nsteps; q += 8) HIPCHK(hipGraphLaunch(s->gexec8, s->st));
+    for (; q < nsteps; q++) HIPCHK(hipGraphLaunch(s->gexec, s->st));
+    s->time += s->dt * nsteps;
   } else {
     for (int q = 0; q < nsteps; q++) {
       enqueue_step(s);

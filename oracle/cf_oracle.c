@@ -132,11 +132,14 @@ static inline v4 getY(const REAL* Y, int k) { return V4(Y[4 * k], Y[4 * k + 1], 
 static inline void putY(REAL* Y, int k, v4 v) { Y[4 * k] = v.x; Y[4 * k + 1] = v.y; Y[4 * k + 2] = v.z; Y[4 * k + 3] = v.w; }
 
 static inline REAL rhoOf(const S* s, REAL T) { return PP(rhoAmb) * PP(Tamb) / T; } /* :33 */
+/* pow(x, 1.5) written x*sqrt(x): the same power, within an ulp of the
+ * correctly rounded value (WGSL's pow is looser), and what the kernels use */
+static inline REAL pow15(REAL x) { return x * SQRT(x); }
 static inline REAL muOf(const S* s, REAL T) {                                     /* :36-38 */
-  return K(1.716e-5) * POW(T / K(273.15), K(1.5)) * K(273.15 + 110.4) / (T + K(110.4)) * PP(viscMult);
+  return K(1.716e-5) * pow15(T / K(273.15)) * K(273.15 + 110.4) / (T + K(110.4)) * PP(viscMult);
 }
 static inline REAL rhoDOf(const S* s, REAL T) {                                   /* :40-42 */
-  return K(1.716e-5) * POW(T / K(273.15), K(1.5)) * K(273.15 + 110.4) / (T + K(110.4)) / PP(Pr) * PP(diffMult);
+  return K(1.716e-5) * pow15(T / K(273.15)) * K(273.15 + 110.4) / (T + K(110.4)) / PP(Pr) * PP(diffMult);
 }
 static inline v4 ambY(const S* s) { return V4(K(0), PP(YOamb), K(0), K(0)); }   /* :44 */
 
