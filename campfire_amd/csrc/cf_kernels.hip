@@ -388,59 +388,80 @@ __device__ __forceinline__ float Tcl(const float* __restrict__ T, int nx, int ny
 __device__ __forceinline__ float exchangef(const cf_params& P, float spd, float Tf) {
   return expf(K(-2.0) * P.exchAlpha * spd * (Tf / P.Tamb) / P.depth * P.dt);
 }
-// new U at face (i, j), 0 <= i <= nx, 0 <= j < ny
-__device__ __forceinline__ float force_u(const cf_params& P, const Fields& F, int i, int j) {
+// Momentum updates, written against an accessor so the same arithmetic runs
+// from global memory (k_forces) or from an LDS tile (k_forces_coef).
+//   T(i, j)   cell temperature at clamped indices (forces.wgsl:17-19)
+//   U2/V2     advected face velocities, raw
+//   fx/fy     open fractions
+struct GlobalAcc {
+  const float *T, *U2, *V2, *fx, *fy;
+  int nx, ny;
+  __device__ float Tc(int i, int j) const { return T[iclamp(j, 0, ny - 1) * nx + iclamp(i, 0, nx - 1)]; }
+  __device__ float U(int i, int j) const { return U2[j * (nx + 1) + i]; }
+  __device__ float V(int i, int j) const { return V2[j * nx + i]; }
+  __device__ float FX(int i, int j) const { return fx[j * (nx + 1) + i]; }
+  __device__ float FY(int i, int j) const { return fy[j * nx + i]; }
+};
+
+// new U at face (i, j), 0 <= i <= nx, 0 <= j < ny (shaders/forces.wgsl:44-71)
+template <class A>
+__device__ __forceinline__ float force_u(const cf_params& P, const A& a, int i, int j) {
   const int nx = P.nx, ny = P.ny;
   const float dt = P.dt;
   const float idx2 = K(1) / (P.dx * P.dx);
-  const float* __restrict__ T = F.T;
-  const float* __restrict__ U2 = F.U2;
-  const float* __restrict__ V2 = F.V2;
-  const float* __restrict__ fx = F.fx;
-  auto u2 = [&](int a, int b) { const int q = b * (nx + 1) + a; return fx[q] > K(0) ? U2[q] : K(0); };
-  const int k = j * (nx + 1) + i;
-  if (!(fx[k] > K(0))) return K(0);
-  if (i == 0) return fminf(U2[j * (nx + 1) + 1], K(0));
-  if (i == nx) return fmaxf(U2[j * (nx + 1) + nx - 1], K(0));
-  const float u = U2[k];
-  const float TL = Tcl(T, nx, ny, i - 1, j), TR = Tcl(T, nx, ny, i, j);
+  auto u2 = [&](int x, int y) { return a.FX(x, y) > K(0) ? a.U(x, y) : K(0); };
+  if (!(a.FX(i, j) > K(0))) return K(0);
+  if (i == 0) return fminf(a.U(1, j), K(0));
+  if (i == nx) return fmaxf(a.U(nx - 1, j), K(0));
+  const float u = a.U(i, j);
+  const float TL = a.Tc(i - 1, j), TR = a.Tc(i, j);
   const float rhoF = rhoOf(P, K(0.5) * (TL + TR));
   const float muE = muOf(P, TR), muW = muOf(P, TL);
   float uN = u, muN = muOf(P, K(0.5) * (TL + TR));
-  if (j < ny - 1) { uN = u2(i, j + 1); muN = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j + 1) + Tcl(T, nx, ny, i, j + 1))); }
+  if (j < ny - 1) { uN = u2(i, j + 1); muN = muOf(P, K(0.25) * (TL + TR + a.Tc(i - 1, j + 1) + a.Tc(i, j + 1))); }
   float uS = -u, muS = muOf(P, K(0.5) * (TL + TR));
-  if (j > 0) { uS = u2(i, j - 1); muS = muOf(P, K(0.25) * (TL + TR + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i, j - 1))); }
+  if (j > 0) { uS = u2(i, j - 1); muS = muOf(P, K(0.25) * (TL + TR + a.Tc(i - 1, j - 1) + a.Tc(i, j - 1))); }
   const float visc = (muE * (u2(i + 1, j) - u) + muW * (u2(i - 1, j) - u) + muN * (uN - u) + muS * (uS - u)) * idx2 / rhoF;
-  const float vAvg = K(0.25) * (V2[j * nx + i - 1] + V2[j * nx + i] + V2[(j + 1) * nx + i - 1] + V2[(j + 1) * nx + i]);
+  const float vAvg = K(0.25) * (a.V(i - 1, j) + a.V(i, j) + a.V(i - 1, j + 1) + a.V(i, j + 1));
   return (u + dt * visc) * exchangef(P, sqrtf(u * u + vAvg * vAvg), K(0.5) * (TL + TR));
 }
 
-// new V at face (i, j), 0 <= i < nx, 0 <= j <= ny
-__device__ __forceinline__ float force_v(const cf_params& P, const Fields& F, int i, int j) {
+// new V at face (i, j), 0 <= i < nx, 0 <= j <= ny (shaders/forces.wgsl:72-100)
+template <class A>
+__device__ __forceinline__ float force_v(const cf_params& P, const A& a, int i, int j) {
   const int nx = P.nx, ny = P.ny;
   const float dt = P.dt;
   const float idx2 = K(1) / (P.dx * P.dx);
-  const float* __restrict__ T = F.T;
-  const float* __restrict__ U2 = F.U2;
-  const float* __restrict__ V2 = F.V2;
-  const float* __restrict__ fy = F.fy;
-  auto v2 = [&](int a, int b) { const int q = b * nx + a; return fy[q] > K(0) ? V2[q] : K(0); };
-  const int k = j * nx + i;
-  if (j == 0 || !(fy[k] > K(0))) return K(0);
-  if (j == ny) return fmaxf(V2[(ny - 1) * nx + i], K(0));
-  const float v = V2[k];
-  const float TB = Tcl(T, nx, ny, i, j - 1), TT = Tcl(T, nx, ny, i, j);
+  auto v2 = [&](int x, int y) { return a.FY(x, y) > K(0) ? a.V(x, y) : K(0); };
+  if (j == 0 || !(a.FY(i, j) > K(0))) return K(0);
+  if (j == ny) return fmaxf(a.V(i, ny - 1), K(0));
+  const float v = a.V(i, j);
+  const float TB = a.Tc(i, j - 1), TT = a.Tc(i, j);
   const float Tf = K(0.5) * (TB + TT);
   const float rhoF = rhoOf(P, Tf);
   const float muN = muOf(P, TT), muS = muOf(P, TB);
   float vE = v, muE = muOf(P, Tf);
-  if (i < nx - 1) { vE = v2(i + 1, j); muE = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i + 1, j - 1) + Tcl(T, nx, ny, i + 1, j))); }
+  if (i < nx - 1) { vE = v2(i + 1, j); muE = muOf(P, K(0.25) * (TB + TT + a.Tc(i + 1, j - 1) + a.Tc(i + 1, j))); }
   float vW = v, muW = muOf(P, Tf);
-  if (i > 0) { vW = v2(i - 1, j); muW = muOf(P, K(0.25) * (TB + TT + Tcl(T, nx, ny, i - 1, j - 1) + Tcl(T, nx, ny, i - 1, j))); }
+  if (i > 0) { vW = v2(i - 1, j); muW = muOf(P, K(0.25) * (TB + TT + a.Tc(i - 1, j - 1) + a.Tc(i - 1, j))); }
   const float visc = (muN * (v2(i, j + 1) - v) + muS * (v2(i, j - 1) - v) + muE * (vE - v) + muW * (vW - v)) * idx2 / rhoF;
   const float buoy = P.grav * (Tf / P.Tamb - K(1));
-  const float uAvg = K(0.25) * (U2[(j - 1) * (nx + 1) + i] + U2[(j - 1) * (nx + 1) + i + 1] + U2[j * (nx + 1) + i] + U2[j * (nx + 1) + i + 1]);
+  const float uAvg = K(0.25) * (a.U(i, j - 1) + a.U(i + 1, j - 1) + a.U(i, j) + a.U(i + 1, j));
   return (v + dt * (visc + buoy)) * exchangef(P, sqrtf(uAvg * uAvg + v * v), Tf);
+}
+
+// face coefficients a = f / rho (shaders/pressure.wgsl:30-43)
+template <class A>
+__device__ __forceinline__ float coef_u(const cf_params& P, const A& a, int i, int j) {
+  const int nx = P.nx;
+  const float Tf = K(0.5) * (a.Tc(iclamp(i - 1, 0, nx - 1), j) + a.Tc(min(i, nx - 1), j));
+  return a.FX(i, j) * invRho(P, Tf);
+}
+template <class A>
+__device__ __forceinline__ float coef_v(const cf_params& P, const A& a, int i, int j) {
+  const int ny = P.ny;
+  const float Tf = K(0.5) * (a.Tc(i, iclamp(j - 1, 0, ny - 1)) + a.Tc(i, min(j, ny - 1)));
+  return a.FY(i, j) * invRho(P, Tf);
 }
 
 __global__ __launch_bounds__(256) void k_forces(Fields F) {
@@ -448,112 +469,135 @@ __global__ __launch_bounds__(256) void k_forces(Fields F) {
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx + 1, ny + 1, i, j);
-  if (i <= nx && j < ny) F.U[j * (nx + 1) + i] = force_u(P, F, i, j);
-  if (i < nx && j <= ny) F.V[j * nx + i] = force_v(P, F, i, j);
+  const GlobalAcc a{F.T, F.U2, F.V2, F.fx, F.fy, nx, ny};
+  if (i <= nx && j < ny) F.U[j * (nx + 1) + i] = force_u(P, a, i, j);
+  if (i < nx && j <= ny) F.V[j * nx + i] = force_v(P, a, i, j);
 }
 
-// face coefficients a = f / rho (shaders/pressure.wgsl:30-43)
-__device__ __forceinline__ float coef_u(const cf_params& P, const Fields& F, int i, int j) {
-  const int nx = P.nx;
-  const float Tf = K(0.5) * (F.T[j * nx + iclamp(i - 1, 0, nx - 1)] + F.T[j * nx + min(i, nx - 1)]);
-  return F.fx[j * (nx + 1) + i] * invRho(P, Tf);
-}
-__device__ __forceinline__ float coef_v(const cf_params& P, const Fields& F, int i, int j) {
-  const int nx = P.nx, ny = P.ny;
-  const float Tf = K(0.5) * (F.T[iclamp(j - 1, 0, ny - 1) * nx + i] + F.T[min(j, ny - 1) * nx + i]);
-  return F.fy[j * nx + i] * invRho(P, Tf);
-}
-
-// Fused momentum update + projection set-up on 64x32-cell tiles:
+// Fused momentum update + projection set-up on 32x32-cell tiles:
 // forces (shaders/forces.wgsl:34-101), coef + rhs0 (shaders/pressure.wgsl:30-54)
-// and the coefficient restriction (:57-67) cascaded through as many coarse
-// levels as the tile halves into (nlev_fused levels past level 0).
-// Each tile recomputes the velocity and coefficient of the one face column and
-// row just past its edge (owned by the next tile) instead of reading them back.
+// and the coefficient restriction (:57-67) cascaded through the coarse levels
+// the tile halves into (5 levels: 16^2 .. 1^2 coarse cells per tile).
+// T, U2, V2, fx and fy of the tile and a one-cell halo are staged in LDS with
+// coalesced loads; every face update then reads LDS.  Each tile also computes
+// the face column and row just past its edge (owned by the next tile) for its
+// own rhs0 and coarse coefficients, instead of reading them back.
 struct CoefLevels {
   Level L[16];
-  int ncascade;  // coarse levels produced here (<= 5 for a 64x32 tile)
+  int ncascade;  // coarse levels produced here (<= 5)
+};
+
+constexpr int FC_T = 32;                 // tile edge in cells
+constexpr int FC_R = FC_T + 2;           // T region: [x0-1, x0+32] x [y0-1, y0+32]
+constexpr int FC_UX = FC_T + 3;          // U faces [x0-1, x0+33] x rows [y0-1, y0+32]
+constexpr int FC_VY = FC_T + 3;          // V faces cols [x0-1, x0+32] x rows [y0-1, y0+33]
+
+struct TileAcc {
+  const float *T, *U2, *V2, *fx, *fy;    // LDS, region-relative
+  int x0, y0;
+  __device__ float Tc(int i, int j) const { return T[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
+  __device__ float U(int i, int j) const { return U2[(j - y0 + 1) * FC_UX + (i - x0 + 1)]; }
+  __device__ float FX(int i, int j) const { return fx[(j - y0 + 1) * FC_UX + (i - x0 + 1)]; }
+  __device__ float V(int i, int j) const { return V2[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
+  __device__ float FY(int i, int j) const { return fy[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
 };
 
 __global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL, const int* __restrict__ part, int nparts) {
-  constexpr int TX = 64, TY = 32;
+  constexpr int TX = FC_T, TY = FC_T;
+  __shared__ float sT[FC_R * FC_R];
+  __shared__ float sU[FC_R * FC_UX], sfx[FC_R * FC_UX];
+  __shared__ float sV[FC_VY * FC_R], sfy[FC_VY * FC_R];
   __shared__ float su[TY][TX + 1], sv[TY + 1][TX];
   __shared__ float sax[TY][TX + 1], say[TY + 1][TX];
   const cf_params P = *F.P;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;  // 32 x 8 threads
   if (blockIdx.x == 0 && nparts > 0) acc_fold(part, nparts, F.acc, (int*)&su[0][0]);
   const int nx = P.nx, ny = P.ny;
   const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
   const int t = xcd_tile(blockIdx.x, gx * gy);
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
-  const int lx = threadIdx.x, ly0 = threadIdx.y;  // 64 x 4 threads
-  // ---- forces and coefficients of the tile's faces (rolled loop: the force
-  // code is long, and an unrolled copy per row overflows the instruction cache)
-#pragma unroll 1
-  for (int r = 0; r < TY / 4; r++) {
-    const int ly = ly0 + 4 * r;
-    const int i = x0 + lx, j = y0 + ly;
-    if (i < nx && j < ny) {
-      const float u = force_u(P, F, i, j), v = force_v(P, F, i, j);
-      su[ly][lx] = u; sv[ly][lx] = v;
-      F.U[j * (nx + 1) + i] = u; F.V[j * nx + i] = v;
-      const float a = coef_u(P, F, i, j), b = coef_v(P, F, i, j);
-      sax[ly][lx] = a; say[ly][lx] = b;
-      F.ax0[j * (nx + 1) + i] = a; F.ay0[j * nx + i] = b;
-      if (lx == TX - 1 || i == nx - 1) {  // the face just past the tile's right edge
-        const float u1 = force_u(P, F, i + 1, j), a1 = coef_u(P, F, i + 1, j);
-        su[ly][lx + 1] = u1; sax[ly][lx + 1] = a1;
-        if (i + 1 == nx) { F.U[j * (nx + 1) + nx] = u1; F.ax0[j * (nx + 1) + nx] = a1; }
-      }
-      if (ly == TY - 1 || j == ny - 1) {  // the face just past the tile's top edge
-        const float v1 = force_v(P, F, i, j + 1), b1 = coef_v(P, F, i, j + 1);
-        sv[ly + 1][lx] = v1; say[ly + 1][lx] = b1;
-        if (j + 1 == ny) { F.V[ny * nx + i] = v1; F.ay0[ny * nx + i] = b1; }
-      }
+  // ---- stage inputs (cells at clamped indices; faces outside the domain read 0, never used)
+  for (int q = tid; q < FC_R * FC_R; q += 256) {
+    const int a = q % FC_R, b = q / FC_R;
+    sT[q] = F.T[iclamp(y0 - 1 + b, 0, ny - 1) * nx + iclamp(x0 - 1 + a, 0, nx - 1)];
+  }
+  for (int q = tid; q < FC_R * FC_UX; q += 256) {
+    const int a = q % FC_UX, b = q / FC_UX;
+    const int i = x0 - 1 + a, j = y0 - 1 + b;
+    const bool in = i >= 0 && i <= nx && j >= 0 && j < ny;
+    sU[q] = in ? F.U2[j * (nx + 1) + i] : 0.f;
+    sfx[q] = in ? F.fx[j * (nx + 1) + i] : 0.f;
+  }
+  for (int q = tid; q < FC_VY * FC_R; q += 256) {
+    const int a = q % FC_R, b = q / FC_R;
+    const int i = x0 - 1 + a, j = y0 - 1 + b;
+    const bool in = i >= 0 && i < nx && j >= 0 && j <= ny;
+    sV[q] = in ? F.V2[j * nx + i] : 0.f;
+    sfy[q] = in ? F.fy[j * nx + i] : 0.f;
+  }
+  __syncthreads();
+  const TileAcc acc{sT, sU, sV, sfx, sfy, x0, y0};
+  // ---- U faces [x0, x0+32] x [y0, y0+32): forces and coefficients
+  for (int q = tid; q < (TX + 1) * TY; q += 256) {
+    const int li = q % (TX + 1), lj = q / (TX + 1);
+    const int i = x0 + li, j = y0 + lj;
+    if (i <= nx && j < ny) {
+      const float u = force_u(P, acc, i, j), a = coef_u(P, acc, i, j);
+      su[lj][li] = u; sax[lj][li] = a;
+      if (li < TX || i == nx) { F.U[j * (nx + 1) + i] = u; F.ax0[j * (nx + 1) + i] = a; }
+    }
+  }
+  // ---- V faces [x0, x0+32) x [y0, y0+32]
+  for (int q = tid; q < TX * (TY + 1); q += 256) {
+    const int li = q % TX, lj = q / TX;
+    const int i = x0 + li, j = y0 + lj;
+    if (i < nx && j <= ny) {
+      const float v = force_v(P, acc, i, j), b = coef_v(P, acc, i, j);
+      sv[lj][li] = v; say[lj][li] = b;
+      if (lj < TY || j == ny) { F.V[j * nx + i] = v; F.ay0[j * nx + i] = b; }
     }
   }
   __syncthreads();
   // ---- rhs0 (:46-54)
-#pragma unroll 1
-  for (int r = 0; r < TY / 4; r++) {
-    const int ly = ly0 + 4 * r;
-    const int i = x0 + lx, j = y0 + ly;
+  for (int q = tid; q < TX * TY; q += 256) {
+    const int li = q % TX, lj = q / TX;
+    const int i = x0 + li, j = y0 + lj;
     if (i < nx && j < ny) {
-      const float fe = F.fx[j * (nx + 1) + i + 1], fw = F.fx[j * (nx + 1) + i];
-      const float fn = F.fy[(j + 1) * nx + i], fs = F.fy[j * nx + i];
-      const float div = (fe * su[ly][lx + 1] - fw * su[ly][lx] + fn * sv[ly + 1][lx] - fs * sv[ly][lx]) / P.dx;
+      const float fe = acc.FX(i + 1, j), fw = acc.FX(i, j);
+      const float fn = acc.FY(i, j + 1), fs = acc.FY(i, j);
+      const float div = (fe * su[lj][li + 1] - fw * su[lj][li] + fn * sv[lj + 1][li] - fs * sv[lj][li]) / P.dx;
       const float open = fe + fw + fn + fs;
       F.rhs0[j * nx + i] = open > K(0) ? (div - F.SQ[j * nx + i].x) / P.dt : K(0);
     }
   }
   // ---- cascade of coefficient restrictions (:57-67)
-  const int tid = threadIdx.y * TX + threadIdx.x;
   int tx_l = TX, ty_l = TY;
   for (int l = 0; l < CL.ncascade; l++) {
     const Level& C = CL.L[l + 1];
     const int xc = x0 >> (l + 1), yc = y0 >> (l + 1);
     const int cwx = tx_l / 2, cwy = ty_l / 2;  // coarse tile extent
     const int nax = (cwx + 1) * cwy, nay = cwx * (cwy + 1);
-    float va[4], vb[4];
+    float va[2], vb[2];
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = tid + 256 * q;
+    for (int q = 0; q < 2; q++) {
+      const int kk = tid + 256 * q;
       va[q] = vb[q] = 0.f;
-      if (k < nax) { const int I = k % (cwx + 1), J = k / (cwx + 1); va[q] = K(0.5) * (sax[2 * J][2 * I] + sax[2 * J + 1][2 * I]); }
-      if (k < nay) { const int I = k % cwx, J = k / cwx; vb[q] = K(0.5) * (say[2 * J][2 * I] + say[2 * J][2 * I + 1]); }
+      if (kk < nax) { const int I = kk % (cwx + 1), J = kk / (cwx + 1); va[q] = K(0.5) * (sax[2 * J][2 * I] + sax[2 * J + 1][2 * I]); }
+      if (kk < nay) { const int I = kk % cwx, J = kk / cwx; vb[q] = K(0.5) * (say[2 * J][2 * I] + say[2 * J][2 * I + 1]); }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = tid + 256 * q;
-      if (k < nax) {
-        const int I = k % (cwx + 1), J = k / (cwx + 1);
+    for (int q = 0; q < 2; q++) {
+      const int kk = tid + 256 * q;
+      if (kk < nax) {
+        const int I = kk % (cwx + 1), J = kk / (cwx + 1);
         sax[J][I] = va[q];
         const int gI = xc + I, gJ = yc + J;
         if (gJ < C.ny && ((I < cwx && gI < C.nx) || gI == C.nx)) C.ax[gJ * (C.nx + 1) + gI] = va[q];
       }
-      if (k < nay) {
-        const int I = k % cwx, J = k / cwx;
+      if (kk < nay) {
+        const int I = kk % cwx, J = kk / cwx;
         say[J][I] = vb[q];
         const int gI = xc + I, gJ = yc + J;
         if (gI < C.nx && ((J < cwy && gJ < C.ny) || gJ == C.ny)) C.ay[gJ * C.nx + gI] = vb[q];
@@ -563,9 +607,6 @@ __global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL, co
   }
 }
 
-// -------------------------------------------------------------------------
-// projection set-up: coef + rhs0 in one pass (shaders/pressure.wgsl:30-54)
-// -------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_coef_rhs(Fields F) {
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
@@ -972,15 +1013,15 @@ void launch_forces(const Fields& F, hipStream_t s) {
   hipLaunchKernelGGL(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 int forces_coef_cascade(int nlev) {
-  // a 64x32 tile halves into coarse tiles down to 2x1 cells: five levels
+  // a 32x32 tile halves into coarse tiles down to 1x1 cell: five levels
   return nlev - 1 < 5 ? nlev - 1 : 5;
 }
 void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* part, int nparts, hipStream_t s) {
   CoefLevels CL;
   for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
   CL.ncascade = forces_coef_cascade(nlev);
-  const unsigned nb = (unsigned)(((F.nx + 63) / 64) * ((F.ny + 31) / 32));
-  hipLaunchKernelGGL(k_forces_coef, dim3(nb), dim3(64, 4), 0, s, F, CL, part, nparts);
+  const unsigned nb = (unsigned)(((F.nx + FC_T - 1) / FC_T) * ((F.ny + FC_T - 1) / FC_T));
+  hipLaunchKernelGGL(k_forces_coef, dim3(nb), dim3(32, 8), 0, s, F, CL, part, nparts);
 }
 void launch_coef_rhs(const Fields& F, hipStream_t s) {
   hipLaunchKernelGGL(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);

@@ -236,25 +236,23 @@ struct Bottom {
 
 template <int CPT>
 struct CellSet {
-  int k[CPT];               // linear cell index (or -1)
-  int iw[CPT], ie[CPT], is[CPT], in[CPT];
+  int k[CPT];   // linear cell index (or -1)
+  int nb[CPT];  // bits 0..3: west, east, south, north neighbour exists; bit 4: colour
   float w[CPT], e[CPT], s[CPT], n[CPT], diag[CPT];
-  int col[CPT];
 };
 
-// cell data of level L for thread t: cells t + BT*q; coefficient arrays may be
-// in global memory (ax, ay) or in LDS
+// cell data of a level for the cells kidx[q] (ignored when outside the level);
+// coefficient arrays may be in global memory (ax, ay) or in LDS
 template <int CPT>
-__device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, const float* ax, const float* ay) {
-  const int t = threadIdx.x;
+__device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, const float* ax, const float* ay,
+                                           const int (&kidx)[CPT]) {
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
-    const int k = t + BT * q;
+    const int k = kidx[q];
     C.k[q] = -1;
     C.w[q] = C.e[q] = C.s[q] = C.n[q] = C.diag[q] = 0.f;
-    C.iw[q] = C.ie[q] = C.is[q] = C.in[q] = 0;
-    C.col[q] = 2;
-    if (k < nx * ny) {
+    C.nb[q] = 0;
+    if (k >= 0 && k < nx * ny) {
       const int i = k % nx, j = k / nx;
       const float w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], s = ay[j * nx + i], n = ay[(j + 1) * nx + i];
       float diag = 0.f;  // the diagonal of shaders/pressure.wgsl:77-85, same accumulation order
@@ -264,38 +262,61 @@ __device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, cons
       if (j < ny - 1) diag += n; else diag += K(2.0) * n;
       C.k[q] = k;
       C.diag[q] = diag;
-      C.w[q] = i > 0 ? w : 0.f;      C.iw[q] = i > 0 ? k - 1 : k;
-      C.e[q] = i < nx - 1 ? e : 0.f; C.ie[q] = i < nx - 1 ? k + 1 : k;
-      C.s[q] = j > 0 ? s : 0.f;      C.is[q] = j > 0 ? k - nx : k;
-      C.n[q] = j < ny - 1 ? n : 0.f; C.in[q] = j < ny - 1 ? k + nx : k;
-      C.col[q] = (i + j) & 1;
+      C.w[q] = i > 0 ? w : 0.f;
+      C.e[q] = i < nx - 1 ? e : 0.f;
+      C.s[q] = j > 0 ? s : 0.f;
+      C.n[q] = j < ny - 1 ? n : 0.f;
+      C.nb[q] = (i > 0 ? 1 : 0) | (i < nx - 1 ? 2 : 0) | (j > 0 ? 4 : 0) | (j < ny - 1 ? 8 : 0) | (((i + j) & 1) << 4);
     }
   }
 }
 
 template <int CPT>
-__device__ __forceinline__ void cells_relax(const CellSet<CPT>& C, float* p, const float* rhs, float h2, int col) {
+__device__ __forceinline__ void cells_load_strided(CellSet<CPT>& C, int nx, int ny, const float* ax, const float* ay) {
+  int kidx[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; q++) kidx[q] = threadIdx.x + BT * q;
+  cells_load<CPT>(C, nx, ny, ax, ay, kidx);
+}
+
+template <int CPT>
+__device__ __forceinline__ float cells_off(const CellSet<CPT>& C, int q, const float* p, int nx) {
+  const int k = C.k[q], b = C.nb[q];
+  return C.w[q] * p[k - (b & 1)] + C.e[q] * p[k + ((b >> 1) & 1)] + C.s[q] * p[k - nx * ((b >> 2) & 1)] +
+         C.n[q] * p[k + nx * ((b >> 3) & 1)];
+}
+
+template <int CPT>
+__device__ __forceinline__ void cells_relax(const CellSet<CPT>& C, int nx, float* p, const float* rhs, float h2,
+                                            int col) {
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
-    if (C.col[q] == col) {
+    if (C.k[q] >= 0 && ((C.nb[q] >> 4) & 1) == col) {
       const int k = C.k[q];
-      const float off = C.w[q] * p[C.iw[q]] + C.e[q] * p[C.ie[q]] + C.s[q] * p[C.is[q]] + C.n[q] * p[C.in[q]];
+      const float off = cells_off<CPT>(C, q, p, nx);
       p[k] = C.diag[q] > K(0) ? (off - rhs[k] * h2) / C.diag[q] : K(0);
     }
   }
 }
 
 template <int CPT>
-__device__ __forceinline__ void cells_residual(const CellSet<CPT>& C, const float* p, const float* rhs, float h2,
-                                               float* res) {
+__device__ __forceinline__ void cells_residual(const CellSet<CPT>& C, int nx, const float* p, const float* rhs,
+                                               float h2, float* res) {
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
     const int k = C.k[q];
     if (k >= 0) {
-      const float off = C.w[q] * p[C.iw[q]] + C.e[q] * p[C.ie[q]] + C.s[q] * p[C.is[q]] + C.n[q] * p[C.in[q]];
+      const float off = cells_off<CPT>(C, q, p, nx);
       res[k] = C.diag[q] <= K(0) ? K(0) : rhs[k] - (off - C.diag[q] * p[k]) / h2;
     }
   }
+}
+
+// orders one wave's LDS traffic between half-sweeps without a workgroup barrier
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // coarse rhs = 1/4 of the 4 fine residuals (shaders/pressure.wgsl:104-113)
@@ -360,17 +381,19 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
     __syncthreads();
     axc = ax; ayc = ay;
   }
-  // per-thread cell data; level 0 of this kernel may hold up to 4096 cells
-  CellSet<4> C0;
-  CellSet<1> C1, C2;
+  // per-thread cell data; level 0 of this kernel may hold up to 4096 cells.
+  // The coarsest level is swept by the first wave alone: lane l owns cells
+  // 2l, 2l+1, 2l+128, 2l+129 (two of each colour).
+  CellSet<4> C0, C2;
+  CellSet<1> C1;
   {
     const Level& L0 = B.L[0];
-    if (nb == 1) cells_load<1>(C2, L0.nx, L0.ny, axc, ayc);
-    else if (nb == 2) { cells_load<1>(C1, L0.nx, L0.ny, L0.ax, L0.ay); cells_load<1>(C2, Lc.nx, Lc.ny, axc, ayc); }
-    else {
-      cells_load<4>(C0, L0.nx, L0.ny, L0.ax, L0.ay);
-      cells_load<1>(C1, B.L[1].nx, B.L[1].ny, B.L[1].ax, B.L[1].ay);
-      cells_load<1>(C2, Lc.nx, Lc.ny, axc, ayc);
+    const int kw[4] = {t < 64 ? 2 * t : -1, t < 64 ? 2 * t + 1 : -1, t < 64 ? 2 * t + 128 : -1, t < 64 ? 2 * t + 129 : -1};
+    cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
+    if (nb == 2) cells_load_strided<1>(C1, L0.nx, L0.ny, L0.ax, L0.ay);
+    if (nb == 3) {
+      cells_load_strided<4>(C0, L0.nx, L0.ny, L0.ax, L0.ay);
+      cells_load_strided<1>(C1, B.L[1].nx, B.L[1].ny, B.L[1].ax, B.L[1].ay);
     }
     // p (warm start or zero) and rhs of the first level; slot = 3 - nb
     const int s0 = 3 - nb;
@@ -385,25 +408,29 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
     const Level& L = lev(s);
     for (int q = 0; q < B.nu; q++) {
       for (int col = 0; col < 2; col++) {
-        if (s == 0) cells_relax<4>(C0, P(0), R(0), L.h2, col); else cells_relax<1>(C1, P(1), R(1), L.h2, col);
+        if (s == 0) cells_relax<4>(C0, L.nx, P(0), R(0), L.h2, col); else cells_relax<1>(C1, L.nx, P(1), R(1), L.h2, col);
         __syncthreads();
       }
     }
-    if (s == 0) cells_residual<4>(C0, P(0), R(0), L.h2, sres); else cells_residual<1>(C1, P(1), R(1), L.h2, sres);
+    if (s == 0) cells_residual<4>(C0, L.nx, P(0), R(0), L.h2, sres); else cells_residual<1>(C1, L.nx, P(1), R(1), L.h2, sres);
     __syncthreads();
     const Level& Cn = lev(s + 1);
     restrict_to(sres, L.nx, R(s + 1), P(s + 1), Cn.nx, Cn.ny);
     __syncthreads();
   }
-  // coarsest: red-black sweeps from zero (shaders/pressure.wgsl:131-156)
+  // coarsest: red-black sweeps from zero (shaders/pressure.wgsl:131-156) by
+  // the first wave; the other waves wait at the next workgroup barrier
   for (int k = t; k < Lc.nx * Lc.ny; k += BT) sp2[k] = K(0);
   __syncthreads();
-  for (int it = 0; it < B.coarse_iters; it++) {
-    cells_relax<1>(C2, sp2, sr2, Lc.h2, 0);
-    __syncthreads();
-    cells_relax<1>(C2, sp2, sr2, Lc.h2, 1);
-    __syncthreads();
+  if (t < 64) {
+    for (int it = 0; it < B.coarse_iters; it++) {
+      cells_relax<4>(C2, Lc.nx, sp2, sr2, Lc.h2, 0);
+      wave_sync();
+      cells_relax<4>(C2, Lc.nx, sp2, sr2, Lc.h2, 1);
+      wave_sync();
+    }
   }
+  __syncthreads();
   // up (js/fluid.js:305-311)
   for (int s = 1; s >= s0; s--) {
     const Level& L = lev(s);
@@ -414,7 +441,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
     for (int q = 0; q < B.nu; q++) {
       for (int c = 0; c < 2; c++) {
         const int col = 1 - c;
-        if (s == 0) cells_relax<4>(C0, P(0), R(0), L.h2, col); else cells_relax<1>(C1, P(1), R(1), L.h2, col);
+        if (s == 0) cells_relax<4>(C0, L.nx, P(0), R(0), L.h2, col); else cells_relax<1>(C1, L.nx, P(1), R(1), L.h2, col);
         __syncthreads();
       }
     }

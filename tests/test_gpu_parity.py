@@ -203,18 +203,38 @@ def test_fused_forces_coef_matches_stages_3_and_4():
         o.close()
 
 
-def test_steps_512_one_step_tight_then_statistics():
-    """One step at 512^2 within 1e-5; two more steps keep the heat release and
-    the field statistics (short-horizon statistics of the north-star)."""
-    _flush("[parity] 512x512: 1 step, then 2 more")
+def test_steps_512_stagewise_then_statistics():
+    """512^2 random-field step compared stage by stage (1e-5 relative L2 per
+    stage output), then two more steps for the short-horizon statistics.
+
+    After the projection the bar for U, V is relative to the pre-projection
+    velocity: on random fields one V-cycle leaves a pressure correction as
+    large as u* itself, so the projected velocity is a difference of two
+    nearly equal fields and its own norm is the wrong scale for round-off."""
+    _flush("[parity] 512x512 stagewise")
     g, o, pieces, n = make_pair(512, 512, 0.0025, 0.0005, seed=51, npieces=8)
     sync_geometry(g, o)
-    g.step(1)
-    o.step(1)
-    for name in ["T", "Y", "U", "V", "p", "SQ"]:
-        r = rel_l2(g.read_field(name), o.read(name))
-        _flush("  step 1 %s rel_l2=%.3g" % (name, r))
-        assert r <= 1e-5, name
+
+    def cmp(stage, names, tol=1e-5, scale=None):
+        for name in names:
+            a, b = g.read_field(name), o.read(name)
+            r = rel_l2(a, b) if scale is None else \
+                np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(scale[name])
+            _flush("  after %-9s %-3s rel_l2=%.3g" % (stage, name, r))
+            assert r <= tol, (stage, name, r)
+
+    g.run_stage(0); o.run_stage(0)
+    cmp("advect", ["T2", "Y2", "U2", "V2"])
+    g.run_stage(1); o.run_stage(1)
+    g.run_stage(2); o.run_stage(2)
+    cmp("sources", ["T", "Y", "SQ"])
+    g.run_stage(7); o.run_stage(3); o.run_stage(4)
+    cmp("forces", ["U", "V", "rhs", "ax", "ay"])
+    ustar = {"U": o.read("U").astype(np.float64), "V": o.read("V").astype(np.float64)}
+    g.run_stage(5); o.run_stage(5)
+    cmp("vcycle", ["p"])
+    g.run_stage(6); o.run_stage(6)
+    cmp("project", ["U", "V"], scale=ustar)
     g.step(2)
     o.step(2)
     for name in ["T", "Y"]:
