@@ -14,10 +14,13 @@ steps replayed from the captured step graph, bracketed by stream syncs.
 Besides the throughput line it measures, live on the sim's stream with HIP
 events: per-kernel average durations (roofline of the dominant kernel
 against the 8 TB/s HBM peak with its algorithmic bytes per launch), and per-
-step latency p50/p99.  The CPU baseline is the FP32 oracle (OpenMP) on a
-bounded sample of the same workload, timed on this host.
+step latency p50/p99.  Before any of that (the process must not have touched
+the GPU when it starts a profiler), two rocprofv3 --pmc runs of a short
+no-graph copy of the workload give the HBM bytes each kernel fetches and
+writes, calibrated on a 256 MiB device copy.  The CPU baseline is the FP32
+oracle (OpenMP) on a bounded sample of the same workload, timed on this host.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-traffic]
 """
 import argparse
 import json
@@ -38,30 +41,169 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(tag, nx, ny, levels):
-    """Algorithmic HBM bytes of one step's launches of `tag` (FP32; each array
-    read once and written once per launch; faces counted as cells)."""
+def kernel_bytes(tag, nx, ny, levels, launches):
+    """Algorithmic HBM bytes of one step's `launches` launches of `tag`: each
+    array the kernel needs read once and each output written once (FP32;
+    faces counted as cells; float4 Y = 16 B, float2 SQ = 8 B per cell).
+    Halo re-reads, LDS staging and partial sums are not counted."""
     nC = nx * ny
-    per_cell = {"advect_fwd": 56, "advect_corr": 84, "diffuse": 48, "diffuse_sources": 72,
-                "sources": 68, "forces": 28, "coef_rhs": 40, "project": 32}
+    per_cell = {
+        "advect": 56,            # T, Y, U, V read; T2, Y2, U2, V2 written
+        "advect_fwd": 56,        # T, Y, U, V read; Th, Yh, Uh, Vh written
+        "advect_corr": 84,       # T, Y, U, V, Th, Yh, Uh, Vh read; T2, Y2, U2, V2 written
+        "diffuse": 48,           # T, Y, fx, fy read; T, Y written
+        "diffuse_sources": 72,   # diffusion + sources: T, Y, fx, fy, cellG read; T, Y, SQ written
+        "sources": 68,
+        "forces": 28,
+        "coef_rhs": 40,
+        "project": 32,           # U, V, p (x2 neighbours), fx, fy read; U, V written
+    }
     if tag in per_cell:
         return per_cell[tag] * nC
     fine = [a * b for a, b in levels]
-    if tag == "relax":       # p all (4) + rhs half (2) + ax, ay (8) + p half written (2), per half-sweep
-        return None
-    if tag == "coef_restrict":
-        return sum(10 * f for f in fine[:-1])
+    base, _, lev = tag.partition(".")
+    lev = int(lev) if lev else None
+    if tag == "forces_coef":
+        # T, U2, V2, fx, fy (4 B), SQ (8 B) read; U, V, ax0, ay0, rhs0 written;
+        # ax, ay of min(L-1, 5) coarse levels written
+        casc = min(len(levels) - 1, 5)
+        return 48 * nC + sum(8 * f for f in fine[1:1 + casc])
+    if base == "mg_down" and lev is not None:   # p, rhs, ax, ay read; p, coarse rhs, coarse p written
+        return 22 * fine[lev]
+    if base == "mg_up" and lev is not None:     # p, coarse p, rhs, ax, ay read; p written
+        return 21 * fine[lev]
+    if base == "coef_restrict" and lev is not None:   # fine ax, ay read; coarse ax, ay written
+        return 8 * fine[lev - 1] + 8 * fine[lev]
     if tag == "restrict_res":
         return sum(18 * f for f in fine[:-1])
     if tag == "prolong":
         return sum(17 * f for f in fine[:-1])
-    return None
+    return None     # mg_bottom: one workgroup on <= 5.4k cells, latency-bound; relax: see relax_bytes
 
 
 def relax_bytes(levels, nu, vcycles):
     fine = [a * b for a, b in levels]
     # per level: 2 nu sweeps down + 2 nu up, each sweep = 2 half-sweeps of 16 B/cell
     return vcycles * sum(16 * f * 4 * nu for f in fine[:-1])
+
+
+# kernel symbol (base name, template argument) behind each profile tag
+KERNEL_SYMBOL = {"diffuse_sources": ("k_sources", "true"), "sources": ("k_sources", "false")}
+COPY_BYTES = 256 << 20   # the calibration copy: 256 MiB read, 256 MiB written
+
+
+def _symbol(kname):
+    import re
+    m = re.search(r"cf::(k_\w+)(?:<([^>]*)>)?", kname)
+    if not m:
+        return None
+    return m.group(1), (m.group(2) or "").split(",")[0].strip()
+
+
+def pmc_child(args):
+    """Body of a counter run: a plain 256 MiB elementwise device copy (the
+    calibration kernel), then a few steps of the workload dispatched one kernel
+    at a time."""
+    os.environ["CF_NO_GRAPH"] = "1"
+    import torch
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)      # an elementwise copy: plain vector loads and stores
+    torch.cuda.synchronize()
+    del a, b
+    from campfire_amd import scenes
+    from campfire_amd.fluid import FluidSim
+    cfg = bench_config(args)
+    sim = FluidSim({"nx": cfg["nx"], "ny": cfg["ny"], "dt": cfg["dt"]}, dx=cfg["dx"])
+    scenes.apply(sim, cfg)
+    sim.frame_all(8, True)
+    sim.readback_now()
+    sim.step(4)
+    sim.synchronize()
+    sim.close()
+    log("[pmc-child] done")
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of every gas-step kernel from two rocprofv3 --pmc
+    runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by the
+    ratio of the known bytes of a plain 256 MiB copy to what the same counter
+    reads for it.  Runs before this process touches the GPU."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="cf_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__), "--pmc-child", "--size", str(args.size),
+               "--seed", str(args.seed)]
+        log("[bench] counter run %s" % counter)
+        env = dict(os.environ, TMPDIR="/tmp")
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, start_new_session=True,
+                             stdout=subprocess.DEVNULL, stderr=sys.stderr)
+        try:
+            rc = p.wait(timeout=240)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            return None, "%s counter run timed out" % counter
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None, "%s counter run wrote no csv (rc %d)" % (counter, rc)
+        rows = []
+        for f in files:
+            with open(f, newline="") as fh:
+                rows += [r for r in csv.DictReader(fh) if r.get("Counter_Name") == counter]
+        shutil.rmtree(d, ignore_errors=True)
+        per = {}
+        copy = []
+        for r in rows:
+            v = float(r["Counter_Value"])
+            sym = _symbol(r["Kernel_Name"])
+            if sym is None:
+                copy.append((v, r["Kernel_Name"]))
+            else:
+                per.setdefault(sym + (int(r.get("Grid_Size") or 0),), []).append(v)
+        out[counter] = (per, copy)
+    # calibration: the copy kernel is the non-gas kernel that moves the most
+    fetch_copy = max(out["FETCH_SIZE"][1], default=(0, ""))
+    name = fetch_copy[1]
+    write_copy = [v for v, n in out["WRITE_SIZE"][1] if n == name]
+    if fetch_copy[0] <= 0 or not write_copy or max(write_copy) <= 0:
+        return None, "copy calibration kernel not found"
+    f_scale = COPY_BYTES / (fetch_copy[0] * 1024.0)
+    w_scale = COPY_BYTES / (float(np.median(write_copy)) * 1024.0)
+    per = {}
+    for sym in set(out["FETCH_SIZE"][0]) | set(out["WRITE_SIZE"][0]):
+        fv = out["FETCH_SIZE"][0].get(sym)
+        wv = out["WRITE_SIZE"][0].get(sym)
+        if fv and wv:
+            per[sym] = {"fetch": float(np.mean(fv)) * 1024.0 * f_scale, "write": float(np.mean(wv)) * 1024.0 * w_scale,
+                        "dispatches": len(fv)}
+    cal = {"copy_kernel": name[:80], "copy_bytes": COPY_BYTES, "fetch_factor": round(f_scale, 4),
+           "write_factor": round(w_scale, 4)}
+    for k in sorted(per, key=lambda k: -(per[k]["fetch"] + per[k]["write"])):
+        log("[bench]   traffic %-16s grid %9d  fetch %10.0f  write %10.0f  B/launch" %
+            (k[0] + ("<%s>" % k[1] if k[1] else ""), k[2], per[k]["fetch"], per[k]["write"]))
+    return (per, cal), None
+
+
+def bench_config(args):
+    from campfire_amd import scenes
+    cfg = scenes.config_pile_1024(args.seed)
+    if args.size != 1024:
+        h = cfg["dx"] * 1024 / args.size
+        cfg.update(nx=args.size, ny=args.size, dx=h, dt=0.5 * h / 2.5)
+        cfg["pieces"], cfg["n"] = scenes.pile(12, args.seed, args.size * h)
+    return cfg
 
 
 def cpu_baseline(cfg, budget_s=12.0, max_steps=30):
@@ -96,21 +238,26 @@ def main():
     ap.add_argument("--burnin-frames", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs one scene per GPU on a single GPU; --gpus %d is not supported "
               "(scenes shard across GPUs with no communication; see the design notes)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
+    traffic, traffic_err = (None, "disabled") if args.no_traffic else measure_traffic(args)
+    if traffic_err:
+        log("[bench] traffic: %s" % traffic_err)
+
     from campfire_amd import scenes
     from campfire_amd.fluid import FluidSim
 
-    cfg = scenes.config_pile_1024(args.seed)
-    if args.size != 1024:
-        h = cfg["dx"] * 1024 / args.size
-        cfg.update(nx=args.size, ny=args.size, dx=h, dt=0.5 * h / 2.5)
-        cfg["pieces"], cfg["n"] = scenes.pile(12, args.seed, args.size * h)
+    cfg = bench_config(args)
     nx, ny = cfg["nx"], cfg["ny"]
     log("[bench] %dx%d h=%.3g mm dt=%.3g ms" % (nx, ny, cfg["dx"] * 1e3, cfg["dt"] * 1e3))
     sim = FluidSim({"nx": nx, "ny": ny, "dt": cfg["dt"]}, dx=cfg["dx"])
@@ -146,7 +293,7 @@ def main():
     sched = {"nu": 2, "vcycles": 1}
     rows = []
     for tag, (ms, launches) in prof.items():
-        b = kernel_bytes(tag, nx, ny, levels)
+        b = kernel_bytes(tag, nx, ny, levels, launches // args.profile_steps)
         if tag == "relax":
             b = relax_bytes(levels, sched["nu"], sched["vcycles"])
         ms_per_step = ms / args.profile_steps
@@ -156,13 +303,29 @@ def main():
     for r in rows:
         log("[bench]   %-16s %8.4f ms/step  %3d launches  %s" % (r[0], r[1], r[2],
             ("%.0f GB/s" % r[4]) if r[4] else ""))
+    # the roofline kernel: the HBM-streaming launch site with the most time per step
     dom = next(r for r in rows if r[3])
     per_launch_ms = dom[1] / max(dom[2], 1)
     bytes_per_launch = dom[3] / max(dom[2], 1)
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    tr = None
+    if traffic:
+        base, _, lev = dom[0].partition(".")
+        sym = KERNEL_SYMBOL.get(base, ("k_" + base, None))
+        hits = {k: v for k, v in traffic[0].items() if k[0] == sym[0] and (sym[1] is None or k[1] == sym[1])}
+        grids = sorted({k[2] for k in hits}, reverse=True)
+        if grids:
+            g = grids[int(lev)] if lev and int(lev) < len(grids) else grids[0]
+            h = [v for k, v in hits.items() if k[2] == g]
+            tr = int(sum((v["fetch"] + v["write"]) * v["dispatches"] for v in h) / sum(v["dispatches"] for v in h))
     roof = {"kernel": dom[0], "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr,
             "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(per_launch_ms * 1e3, 2)}
+    if traffic:
+        roof["traffic_calibration"] = traffic[1]
+    elif traffic_err:
+        roof["traffic_note"] = traffic_err
+    lat_bound = {r[0]: round(r[1] / max(r[2], 1) * 1e3, 2) for r in rows if not r[3]}
     stage_sum = sum(r[1] for r in rows)
 
     out = {
@@ -176,7 +339,9 @@ def main():
                    "grid": [nx, ny], "levels": len(levels), "diff_sub": sim.diff_sub, "seed": args.seed},
         "roofline": roof,
         "kernels_ms_per_step": {r[0]: round(r[1], 5) for r in rows},
+        "kernels_gbs": {r[0]: round(r[4], 1) for r in rows if r[4]},
         "profiled_ms_per_step": round(stage_sum, 5),
+        "latency_bound_us_per_launch": lat_bound,
     }
     if lat is not None:
         out["step_latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 5),

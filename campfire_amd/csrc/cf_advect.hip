@@ -1,0 +1,349 @@
+// cf_advect.hip — MacCormack advection of one step in a single pass.
+//
+// The reference runs it as two dispatch pairs (shaders/advect.wgsl):
+// scalForward + velForward write Th, Yh, Uh, Vh; scalCorrect + velCorrect
+// read them back at the backward-traced points and write T2, Y2, U2, V2.
+// Here one workgroup owns a 32x16 tile of cells and keeps everything of the
+// tile's neighbourhood in LDS:
+//   A. U, V, T, Y over the tile + 5 cells, as the values the boundary-aware
+//      samplers of js/gpu.js:16-65 return for each index (ambient outside the
+//      sides and top, floor clamp, no-slip ghost for U), so sampling LDS is
+//      sampling the field;
+//   B. the forward pass (Th, Yh, Uh, Vh) over the tile + 2 cells, by the
+//      same rules;
+//   C. the correction and limiter for the tile's cells and faces.
+// A bilinear footprint that leaves a window (a displacement of more than
+// about one cell per step) reads the global field, or evaluates the forward
+// pass at that index, with the same arithmetic, so the result never depends
+// on the window sizes.  Th, Yh, Uh, Vh are not written to memory.
+#include "cf_kernels.h"
+
+namespace cf {
+
+namespace {
+
+constexpr int AX = 32, AY = 16;  // output tile (cells)
+constexpr int AH = 2;            // halo of the forward-pass window
+constexpr int AR = 5;            // halo of the staged input window
+constexpr int A_CX = AX + 2 * AR, A_CY = AY + 2 * AR;  // T, Y
+constexpr int A_UX = A_CX + 1, A_UY = A_CY;            // U faces
+constexpr int A_VX = A_CX, A_VY = A_CY + 1;            // V faces
+constexpr int B_CX = AX + 2 * AH, B_CY = AY + 2 * AH;  // Th, Yh
+constexpr int B_UX = B_CX + 1, B_UY = B_CY;            // Uh
+constexpr int B_VX = B_CX, B_VY = B_CY + 1;            // Vh
+
+__device__ __forceinline__ float mixv(float a, float b, float t) { return mixf(a, b, t); }
+__device__ __forceinline__ float4 mixv(float4 a, float4 b, float t) { return mix4(a, b, t); }
+
+// A window of an index space held in LDS, element (i, j) at
+// s[(j - oy) * W + (i - ox)].  Reads flag `ok` false when a bilinear
+// footprint leaves the window; the caller then redoes the point from global
+// memory.
+template <class T, int W, int H>
+struct Win {
+  const T* s;
+  int ox, oy;
+  __device__ __forceinline__ const T* at(int i, int j, bool& ok) const {
+    const int a = i - ox, b = j - oy;
+    const bool in = a >= 0 && a < W - 1 && b >= 0 && b < H - 1;
+    ok = ok && in;
+    return s + (in ? b * W + a : 0);
+  }
+  __device__ __forceinline__ T own(int i, int j) const { return s[(j - oy) * W + (i - ox)]; }
+  // bilinear sample at (fx, fy) = grid position minus the field's stagger
+  __device__ __forceinline__ T bil(float fx, float fy, bool& ok) const {
+    const float bx = floorf(fx), by = floorf(fy);
+    const float tx = fx - bx, ty = fy - by;
+    const T* p = at(toint(bx), toint(by), ok);
+    return mixv(mixv(p[0], p[1], tx), mixv(p[W], p[W + 1], tx), ty);
+  }
+};
+
+__device__ __forceinline__ float2 mm4v(float a, float c, float d, float e) {
+  return make_float2(fminf(fminf(a, c), fminf(d, e)), fmaxf(fmaxf(a, c), fmaxf(d, e)));
+}
+
+// what a tile's passes read
+struct Tile {
+  Win<float, A_CX, A_CY> T;
+  Win<float4, A_CX, A_CY> Y;
+  Win<float, A_UX, A_UY> U;
+  Win<float, A_VX, A_VY> V;
+  Win<float, B_CX, B_CY> Th;
+  Win<float4, B_CX, B_CY> Yh;
+  Win<float, B_UX, B_UY> Uh;
+  Win<float, B_VX, B_VY> Vh;
+  float dx;
+};
+
+// RK2 departure point (shaders/advect.wgsl:28-34) from the velocity windows
+__device__ __forceinline__ float2 trace_l(const Tile& t, float gx, float gy, float dt, bool& ok) {
+  const float vx = t.U.bil(gx - K(0), gy - K(0.5), ok) / t.dx;
+  const float vy = t.V.bil(gx - K(0.5), gy - K(0), ok) / t.dx;
+  const float hdt = K(0.5) * dt;
+  const float mx_ = gx - hdt * vx, my_ = gy - hdt * vy;
+  const float wx = t.U.bil(mx_ - K(0), my_ - K(0.5), ok) / t.dx;
+  const float wy = t.V.bil(mx_ - K(0.5), my_ - K(0), ok) / t.dx;
+  return make_float2(gx - dt * wx, gy - dt * wy);
+}
+
+// ---- global-memory forms (points whose footprints leave the windows) ----
+// forward pass at any index, as the corrector's samplers see Th/Yh, Uh, Vh
+__device__ __forceinline__ void g_fwd_cell(const Fields& F, const cf_params& P, int i, int j, float& th, float4& yh) {
+  const int nx = P.nx, ny = P.ny;
+  if (i < 0 || i >= nx || j >= ny) { th = P.Tamb; yh = make_float4(0.f, P.YOamb, 0.f, 0.f); return; }
+  j = max(j, 0);
+  const float2 d = trace(F.U, F.V, nx, ny, P.dx, (float)i + K(0.5), (float)j + K(0.5), P.dt);
+  th = sC(F.T, nx, ny, P.Tamb, d.x, d.y);
+  yh = sC4(F.Y, nx, ny, make_float4(0.f, P.YOamb, 0.f, 0.f), d.x, d.y);
+}
+__device__ __forceinline__ float g_fwd_u(const Fields& F, const cf_params& P, int i, int j) {
+  const int nx = P.nx, ny = P.ny;
+  if (i < 0 || i > nx || j >= ny) return K(0);
+  const int jj = max(j, 0);
+  const float2 d = trace(F.U, F.V, nx, ny, P.dx, (float)i, (float)jj + K(0.5), P.dt);
+  const float u = sU(F.U, nx, ny, d.x, d.y);
+  return j < 0 ? -u : u;
+}
+__device__ __forceinline__ float g_fwd_v(const Fields& F, const cf_params& P, int i, int j) {
+  const int nx = P.nx, ny = P.ny;
+  if (i < 0 || i >= nx || j > ny) return K(0);
+  j = max(j, 0);
+  const float2 d = trace(F.U, F.V, nx, ny, P.dx, (float)i + K(0.5), (float)j, P.dt);
+  return sV(F.V, nx, ny, d.x, d.y);
+}
+
+// the corrector of one cell / U face / V face from global memory only
+// (shaders/advect.wgsl:49-66, 78-103)
+__device__ __forceinline__ void g_corr_cell(const Fields& F, const cf_params& P, int i, int j, float& To, float4& Yo) {
+  const int nx = P.nx, ny = P.ny;
+  const float4 aY = make_float4(0.f, P.YOamb, 0.f, 0.f);
+  const float gx = (float)i + K(0.5), gy = (float)j + K(0.5);
+  const float2 d = trace(F.U, F.V, nx, ny, P.dx, gx, gy, P.dt);
+  const float2 f = trace(F.U, F.V, nx, ny, P.dx, gx, gy, -P.dt);
+  const float2 mT = mmC(F.T, nx, ny, P.Tamb, d.x, d.y);
+  float4 lo, hi;
+  mmC4(F.Y, nx, ny, aY, d.x, d.y, lo, hi);
+  float Thk;
+  float4 Yhk;
+  g_fwd_cell(F, P, i, j, Thk, Yhk);
+  const float4 Yk = F.Y[j * nx + i];
+  // sC / sC4 of the forward fields at f
+  const float fx = f.x - K(0.5), fy = f.y - K(0.5);
+  const float bx = floorf(fx), by = floorf(fy);
+  const float tx = fx - bx, ty = fy - by;
+  const int bi = toint(bx), bj = toint(by);
+  float t00, t10, t01, t11;
+  float4 y00, y10, y01, y11;
+  g_fwd_cell(F, P, bi, bj, t00, y00);
+  g_fwd_cell(F, P, bi + 1, bj, t10, y10);
+  g_fwd_cell(F, P, bi, bj + 1, t01, y01);
+  g_fwd_cell(F, P, bi + 1, bj + 1, t11, y11);
+  const float sbT = mixf(mixf(t00, t10, tx), mixf(t01, t11, tx), ty);
+  const float4 sb = mix4(mix4(y00, y10, tx), mix4(y01, y11, tx), ty);
+  const float Tc = Thk + K(0.5) * (F.T[j * nx + i] - sbT);
+  const float4 Yc = make_float4(Yhk.x + K(0.5) * (Yk.x - sb.x), Yhk.y + K(0.5) * (Yk.y - sb.y),
+                                Yhk.z + K(0.5) * (Yk.z - sb.z), Yhk.w + K(0.5) * (Yk.w - sb.w));
+  const bool inT = Tc >= mT.x && Tc <= mT.y;
+  const bool inY = Yc.x >= lo.x && Yc.y >= lo.y && Yc.z >= lo.z && Yc.w >= lo.w &&
+                   Yc.x <= hi.x && Yc.y <= hi.y && Yc.z <= hi.z && Yc.w <= hi.w;
+  const float mixC = fminf(Yc.x, Yc.y / P.sO2);
+  const float mixL = fminf(Yhk.x, Yhk.y / P.sO2);
+  const bool take = inT && inY && mixC <= mixL + K(1e-5);
+  To = take ? Tc : Thk;
+  Yo = take ? Yc : Yhk;
+}
+__device__ __forceinline__ float g_corr_u(const Fields& F, const cf_params& P, int i, int j) {
+  const int nx = P.nx, ny = P.ny;
+  const float gx = (float)i, gy = (float)j + K(0.5);
+  const float2 b = trace(F.U, F.V, nx, ny, P.dx, gx, gy, -P.dt);
+  const float fx = b.x - K(0), fy = b.y - K(0.5);
+  const float bx = floorf(fx), by = floorf(fy);
+  const float tx = fx - bx, ty = fy - by;
+  const int bi = toint(bx), bj = toint(by);
+  const float back = mixf(mixf(g_fwd_u(F, P, bi, bj), g_fwd_u(F, P, bi + 1, bj), tx),
+                          mixf(g_fwd_u(F, P, bi, bj + 1), g_fwd_u(F, P, bi + 1, bj + 1), tx), ty);
+  const float2 d = trace(F.U, F.V, nx, ny, P.dx, gx, gy, P.dt);
+  const float2 m = mmU(F.U, nx, ny, d.x, d.y);
+  return clampf(g_fwd_u(F, P, i, j) + K(0.5) * (F.U[j * (nx + 1) + i] - back), m.x, m.y);
+}
+__device__ __forceinline__ float g_corr_v(const Fields& F, const cf_params& P, int i, int j) {
+  const int nx = P.nx, ny = P.ny;
+  const float gx = (float)i + K(0.5), gy = (float)j;
+  const float2 b = trace(F.U, F.V, nx, ny, P.dx, gx, gy, -P.dt);
+  const float fx = b.x - K(0.5), fy = b.y - K(0);
+  const float bx = floorf(fx), by = floorf(fy);
+  const float tx = fx - bx, ty = fy - by;
+  const int bi = toint(bx), bj = toint(by);
+  const float back = mixf(mixf(g_fwd_v(F, P, bi, bj), g_fwd_v(F, P, bi + 1, bj), tx),
+                          mixf(g_fwd_v(F, P, bi, bj + 1), g_fwd_v(F, P, bi + 1, bj + 1), tx), ty);
+  const float2 d = trace(F.U, F.V, nx, ny, P.dx, gx, gy, P.dt);
+  const float2 m = mmV(F.V, nx, ny, d.x, d.y);
+  return clampf(g_fwd_v(F, P, i, j) + K(0.5) * (F.V[j * nx + i] - back), m.x, m.y);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_advect(Fields F) {
+  __shared__ float sT[A_CX * A_CY];
+  __shared__ float4 sY[A_CX * A_CY];
+  __shared__ float sU[A_UX * A_UY];
+  __shared__ float sV[A_VX * A_VY];
+  __shared__ float bT[B_CX * B_CY];
+  __shared__ float4 bY[B_CX * B_CY];
+  __shared__ float bU[B_UX * B_UY];
+  __shared__ float bV[B_VX * B_VY];
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  const int gx = (nx + AX - 1) / AX, gy = (ny + AY - 1) / AY;
+  const int tl = xcd_tile(blockIdx.x, gx * gy);
+  const int x0 = (tl % gx) * AX, y0 = (tl / gx) * AY;
+  const int tid = threadIdx.x;
+  const float dt = P.dt;
+  const float4 aY = make_float4(0.f, P.YOamb, 0.f, 0.f);
+  const int ax0 = x0 - AR, ay0 = y0 - AR, bx0 = x0 - AH, by0 = y0 - AH;
+  Tile t;
+  t.T = {sT, ax0, ay0}; t.Y = {sY, ax0, ay0}; t.U = {sU, ax0, ay0}; t.V = {sV, ax0, ay0};
+  t.Th = {bT, bx0, by0}; t.Yh = {bY, bx0, by0}; t.Uh = {bU, bx0, by0}; t.Vh = {bV, bx0, by0};
+  t.dx = P.dx;
+
+  // A. inputs as the samplers see them
+  for (int q = tid; q < A_CX * A_CY; q += 256) {
+    const int i = ax0 + q % A_CX, j = ay0 + q / A_CX;
+    sT[q] = fC(F.T, nx, ny, P.Tamb, i, j);
+    sY[q] = fC4(F.Y, nx, ny, aY, i, j);
+  }
+  for (int q = tid; q < A_UX * A_UY; q += 256) sU[q] = fU(F.U, nx, ny, ax0 + q % A_UX, ay0 + q / A_UX);
+  for (int q = tid; q < A_VX * A_VY; q += 256) sV[q] = fV(F.V, nx, ny, ax0 + q % A_VX, ay0 + q / A_VX);
+  __syncthreads();
+
+  // B. forward pass over the tile + AH, as the corrector's samplers see it
+  for (int q = tid; q < B_CX * B_CY; q += 256) {
+    const int i = bx0 + q % B_CX, j = by0 + q / B_CX;
+    float th = P.Tamb;
+    float4 yh = aY;
+    if (!(i < 0 || i >= nx || j >= ny)) {
+      const int jj = max(j, 0);
+      bool ok = true;
+      const float2 d = trace_l(t, (float)i + K(0.5), (float)jj + K(0.5), dt, ok);
+      th = t.T.bil(d.x - K(0.5), d.y - K(0.5), ok);
+      yh = t.Y.bil(d.x - K(0.5), d.y - K(0.5), ok);
+      if (!ok) g_fwd_cell(F, P, i, jj, th, yh);
+    }
+    bT[q] = th;
+    bY[q] = yh;
+  }
+  for (int q = tid; q < B_UX * B_UY; q += 256) {
+    const int i = bx0 + q % B_UX, j = by0 + q / B_UX;
+    float u = K(0);
+    if (!(i < 0 || i > nx || j >= ny)) {
+      const int jj = max(j, 0);
+      bool ok = true;
+      const float2 d = trace_l(t, (float)i, (float)jj + K(0.5), dt, ok);
+      u = t.U.bil(d.x - K(0), d.y - K(0.5), ok);
+      if (!ok) u = g_fwd_u(F, P, i, jj);
+      if (j < 0) u = -u;
+    }
+    bU[q] = u;
+  }
+  for (int q = tid; q < B_VX * B_VY; q += 256) {
+    const int i = bx0 + q % B_VX, j = by0 + q / B_VX;
+    float v = K(0);
+    if (!(i < 0 || i >= nx || j > ny)) {
+      const int jj = max(j, 0);
+      bool ok = true;
+      const float2 d = trace_l(t, (float)i + K(0.5), (float)jj, dt, ok);
+      v = t.V.bil(d.x - K(0.5), d.y - K(0), ok);
+      if (!ok) v = g_fwd_v(F, P, i, jj);
+    }
+    bV[q] = v;
+  }
+  __syncthreads();
+
+  // C. correction of the tile's cells and faces; the tile at the right (top)
+  // domain edge also owns the U (V) faces on that edge
+  const bool ex = x0 + AX == nx, ey = y0 + AY == ny;
+  for (int q = tid; q < AX * AY + AY + AX; q += 256) {
+    int i, j;
+    if (q < AX * AY) { i = x0 + q % AX; j = y0 + q / AX; }
+    else if (q < AX * AY + AY) { if (!ex) continue; i = nx; j = y0 + (q - AX * AY); }
+    else { if (!ey) continue; i = x0 + (q - AX * AY - AY); j = ny; }
+    if (i < nx && j < ny) {  // scalCorrect (:49-66)
+      const int k = j * nx + i;
+      const float gxc = (float)i + K(0.5), gyc = (float)j + K(0.5);
+      bool ok = true;
+      const float2 d = trace_l(t, gxc, gyc, dt, ok);
+      const float2 f = trace_l(t, gxc, gyc, -dt, ok);
+      const float* pT = t.T.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
+      const float4* pY = t.Y.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
+      const float sbT = t.Th.bil(f.x - K(0.5), f.y - K(0.5), ok);
+      const float4 sb = t.Yh.bil(f.x - K(0.5), f.y - K(0.5), ok);
+      float To;
+      float4 Yo;
+      if (ok) {
+        const float2 mT = mm4v(pT[0], pT[1], pT[A_CX], pT[A_CX + 1]);
+        const float4 a = pY[0], c = pY[1], dd = pY[A_CX], e = pY[A_CX + 1];
+        const float4 lo = make_float4(fminf(fminf(a.x, c.x), fminf(dd.x, e.x)), fminf(fminf(a.y, c.y), fminf(dd.y, e.y)),
+                                      fminf(fminf(a.z, c.z), fminf(dd.z, e.z)), fminf(fminf(a.w, c.w), fminf(dd.w, e.w)));
+        const float4 hi = make_float4(fmaxf(fmaxf(a.x, c.x), fmaxf(dd.x, e.x)), fmaxf(fmaxf(a.y, c.y), fmaxf(dd.y, e.y)),
+                                      fmaxf(fmaxf(a.z, c.z), fmaxf(dd.z, e.z)), fmaxf(fmaxf(a.w, c.w), fmaxf(dd.w, e.w)));
+        const float Thk = t.Th.own(i, j);
+        const float4 Yhk = t.Yh.own(i, j), Yk = t.Y.own(i, j);
+        const float Tc = Thk + K(0.5) * (t.T.own(i, j) - sbT);
+        const float4 Yc = make_float4(Yhk.x + K(0.5) * (Yk.x - sb.x), Yhk.y + K(0.5) * (Yk.y - sb.y),
+                                      Yhk.z + K(0.5) * (Yk.z - sb.z), Yhk.w + K(0.5) * (Yk.w - sb.w));
+        const bool inT = Tc >= mT.x && Tc <= mT.y;
+        const bool inY = Yc.x >= lo.x && Yc.y >= lo.y && Yc.z >= lo.z && Yc.w >= lo.w &&
+                         Yc.x <= hi.x && Yc.y <= hi.y && Yc.z <= hi.z && Yc.w <= hi.w;
+        const float mixC = fminf(Yc.x, Yc.y / P.sO2);
+        const float mixL = fminf(Yhk.x, Yhk.y / P.sO2);
+        const bool take = inT && inY && mixC <= mixL + K(1e-5);
+        To = take ? Tc : Thk;
+        Yo = take ? Yc : Yhk;
+      } else {
+        g_corr_cell(F, P, i, j, To, Yo);
+      }
+      F.T2[k] = To;
+      F.Y2[k] = Yo;
+    }
+    if (i <= nx && j < ny) {  // velCorrect, U (:78-90)
+      const float gxu = (float)i, gyu = (float)j + K(0.5);
+      bool ok = true;
+      const float2 b = trace_l(t, gxu, gyu, -dt, ok);
+      const float back = t.Uh.bil(b.x - K(0), b.y - K(0.5), ok);
+      const float2 d = trace_l(t, gxu, gyu, dt, ok);
+      const float* pU = t.U.at(toint(floorf(d.x - K(0))), toint(floorf(d.y - K(0.5))), ok);
+      float u;
+      if (ok) {
+        const float2 m = mm4v(pU[0], pU[1], pU[A_UX], pU[A_UX + 1]);
+        u = clampf(t.Uh.own(i, j) + K(0.5) * (t.U.own(i, j) - back), m.x, m.y);
+      } else {
+        u = g_corr_u(F, P, i, j);
+      }
+      F.U2[j * (nx + 1) + i] = u;
+    }
+    if (i < nx && j <= ny) {  // velCorrect, V (:91-103)
+      const float gxv = (float)i + K(0.5), gyv = (float)j;
+      bool ok = true;
+      const float2 b = trace_l(t, gxv, gyv, -dt, ok);
+      const float back = t.Vh.bil(b.x - K(0.5), b.y - K(0), ok);
+      const float2 d = trace_l(t, gxv, gyv, dt, ok);
+      const float* pV = t.V.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0))), ok);
+      float v;
+      if (ok) {
+        const float2 m = mm4v(pV[0], pV[1], pV[A_VX], pV[A_VX + 1]);
+        v = clampf(t.Vh.own(i, j) + K(0.5) * (t.V.own(i, j) - back), m.x, m.y);
+      } else {
+        v = g_corr_v(F, P, i, j);
+      }
+      F.V2[j * nx + i] = v;
+    }
+  }
+}
+
+void launch_advect(const Fields& F, hipStream_t s) {
+  const unsigned nb = (unsigned)(((F.nx + AX - 1) / AX) * ((F.ny + AY - 1) / AY));
+  launch(k_advect, dim3(nb), dim3(256), 0, s, F);
+}
+
+}  // namespace cf

@@ -1,8 +1,32 @@
 // cf_kernels.h — device buffers of one scene and the kernel launchers.
 #pragma once
+#include <hip/hip_ext.h>
+#include <vector>
 #include "cf_device.h"
 
 namespace cf {
+
+// Start/stop events of one launch, recorded by the runtime around the
+// dispatch packet itself, so their interval is the kernel's execution alone.
+struct LaunchEvents {
+  hipEvent_t start, stop;
+};
+// While set (a profiling pass of the host library, on its calling thread),
+// every launch appends its events here.
+extern thread_local std::vector<LaunchEvents>* prof_sink;
+
+template <typename... KArgs, typename... Args>
+inline void launch(void (*k)(KArgs...), dim3 grid, dim3 block, unsigned shmem, hipStream_t s, Args... args) {
+  if (prof_sink) {
+    LaunchEvents e{nullptr, nullptr};
+    if (hipEventCreate(&e.start) == hipSuccess && hipEventCreate(&e.stop) == hipSuccess) {
+      hipExtLaunchKernelGGL(k, grid, block, shmem, s, e.start, e.stop, 0, args...);
+      prof_sink->push_back(e);
+      return;
+    }
+  }
+  hipLaunchKernelGGL(k, grid, block, shmem, s, args...);
+}
 
 // Every device buffer of one scene (the `buf` dictionary of js/fluid.js:52-68
 // plus the level-0 multigrid arrays).  Passed by value as a kernel argument.
@@ -25,6 +49,9 @@ struct Level {
   float *p, *rhs, *ax, *ay;
 };
 
+// advection in one pass (cf_advect.hip); the two launches below are the
+// reference's dispatch pairs and also write the forward fields Th, Yh, Uh, Vh
+void launch_advect(const Fields& F, hipStream_t s);
 void launch_advect_fwd(const Fields& F, hipStream_t s);
 void launch_advect_corr(const Fields& F, hipStream_t s);
 void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s);

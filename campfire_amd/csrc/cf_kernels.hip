@@ -993,24 +993,24 @@ static inline unsigned nblk(int ex, int ey) { return (unsigned)(((ex + 31) / 32)
 static const dim3 B328(32, 8);
 
 void launch_advect_fwd(const Fields& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_advect_fwd, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+  launch(k_advect_fwd, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_advect_corr(const Fields& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_advect_corr, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+  launch(k_advect_corr, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s) {
-  hipLaunchKernelGGL(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout, dts);
+  launch(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout, dts);
 }
 unsigned sources_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
 void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, bool finish,
                     hipStream_t s) {
   const unsigned nb = nblk(F.nx, F.ny);
-  if (fuse) hipLaunchKernelGGL(k_sources<true>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
-  else hipLaunchKernelGGL(k_sources<false>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
-  if (finish) hipLaunchKernelGGL(k_acc_finish, dim3(1), dim3(256), 0, s, (const int*)part, (int)nb, F.acc);
+  if (fuse) launch(k_sources<true>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
+  else launch(k_sources<false>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
+  if (finish) launch(k_acc_finish, dim3(1), dim3(256), 0, s, (const int*)part, (int)nb, F.acc);
 }
 void launch_forces(const Fields& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+  launch(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 int forces_coef_cascade(int nlev) {
   // a 32x32 tile halves into coarse tiles down to 1x1 cell: five levels
@@ -1021,44 +1021,44 @@ void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* pa
   for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
   CL.ncascade = forces_coef_cascade(nlev);
   const unsigned nb = (unsigned)(((F.nx + FC_T - 1) / FC_T) * ((F.ny + FC_T - 1) / FC_T));
-  hipLaunchKernelGGL(k_forces_coef, dim3(nb), dim3(32, 8), 0, s, F, CL, part, nparts);
+  launch(k_forces_coef, dim3(nb), dim3(32, 8), 0, s, F, CL, part, nparts);
 }
 void launch_coef_rhs(const Fields& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+  launch(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_coef_restrict, dim3(nblk(c.nx + 1, c.ny + 1)), B328, 0, s, f, c);
+  launch(k_coef_restrict, dim3(nblk(c.nx + 1, c.ny + 1)), B328, 0, s, f, c);
 }
 void launch_relax(const Level& L, int color, hipStream_t s) {
-  hipLaunchKernelGGL(k_relax, dim3(nblk((L.nx + 1) / 2, L.ny)), B328, 0, s, L, color);
+  launch(k_relax, dim3(nblk((L.nx + 1) / 2, L.ny)), B328, 0, s, L, color);
 }
 void launch_restrict_res(const Level& f, const Level& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_restrict_res, dim3(nblk(c.nx, c.ny)), B328, 0, s, f, c);
+  launch(k_restrict_res, dim3(nblk(c.nx, c.ny)), B328, 0, s, f, c);
 }
 void launch_prolong(const Level& f, const Level& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_prolong, dim3(nblk(f.nx, f.ny)), B328, 0, s, f, c);
+  launch(k_prolong, dim3(nblk(f.nx, f.ny)), B328, 0, s, f, c);
 }
 void launch_coarsest(const Level& L, int iters, hipStream_t s) {
-  hipLaunchKernelGGL(k_coarsest, dim3(1), dim3(256), 0, s, L, iters);
+  launch(k_coarsest, dim3(1), dim3(256), 0, s, L, iters);
 }
 void launch_project(const Fields& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_project, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+  launch(k_project, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_geometry(const Fields& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_geometry, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+  launch(k_geometry, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s) {
-  if (ntargets > 0) hipLaunchKernelGGL(k_radiation, dim3(ntargets), dim3(64), 0, s, F);
+  if (ntargets > 0) launch(k_radiation, dim3(ntargets), dim3(64), 0, s, F);
 }
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s) {
   if (n == 0) return;
   unsigned b = (unsigned)((n + 255) / 256);
   if (b > 4096) b = 4096;
-  hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(256), 0, s, (uint32_t*)a, v, n);
+  launch(k_fill_u32, dim3(b), dim3(256), 0, s, (uint32_t*)a, v, n);
 }
 unsigned div_error_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
 void launch_div_error(const Fields& F, float* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_div_error, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, part);
+  launch(k_div_error, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, part);
 }
 
 }  // namespace cf

@@ -312,6 +312,30 @@ __device__ __forceinline__ void cells_residual(const CellSet<CPT>& C, int nx, co
   }
 }
 
+// One half-sweep of colour col over a wave's cell pairs: slots 2q and 2q+1
+// hold linear cells 2m and 2m+1, which always differ in colour (same row when
+// nx is even; (nx-1, j) and (0, j+1) when nx is odd), so each pair updates
+// exactly one cell, chosen by select rather than a divergent branch.  rh holds
+// rhs*h2 of every slot.
+template <int NPAIR>
+__device__ __forceinline__ void cells_relax_pairs(const CellSet<2 * NPAIR>& C, const float (&rh)[2 * NPAIR], int nx,
+                                                  float* p, int col) {
+#pragma unroll
+  for (int q = 0; q < NPAIR; q++) {
+    const int a = 2 * q, b = 2 * q + 1;
+    const bool sa = ((C.nb[a] >> 4) & 1) == col;
+    const int k = sa ? C.k[a] : C.k[b], nb = sa ? C.nb[a] : C.nb[b];
+    const float w = sa ? C.w[a] : C.w[b], e = sa ? C.e[a] : C.e[b];
+    const float s = sa ? C.s[a] : C.s[b], n = sa ? C.n[a] : C.n[b];
+    const float diag = sa ? C.diag[a] : C.diag[b], r = sa ? rh[a] : rh[b];
+    if (k >= 0) {
+      const float off = w * p[k - (nb & 1)] + e * p[k + ((nb >> 1) & 1)] + s * p[k - nx * ((nb >> 2) & 1)] +
+                        n * p[k + nx * ((nb >> 3) & 1)];
+      p[k] = diag > K(0) ? (off - r) / diag : K(0);
+    }
+  }
+}
+
 // orders one wave's LDS traffic between half-sweeps without a workgroup barrier
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -423,10 +447,13 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
   for (int k = t; k < Lc.nx * Lc.ny; k += BT) sp2[k] = K(0);
   __syncthreads();
   if (t < 64) {
+    float rh[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) rh[q] = C2.k[q] >= 0 ? sr2[C2.k[q]] * Lc.h2 : K(0);
     for (int it = 0; it < B.coarse_iters; it++) {
-      cells_relax<4>(C2, Lc.nx, sp2, sr2, Lc.h2, 0);
+      cells_relax_pairs<2>(C2, rh, Lc.nx, sp2, 0);
       wave_sync();
-      cells_relax<4>(C2, Lc.nx, sp2, sr2, Lc.h2, 1);
+      cells_relax_pairs<2>(C2, rh, Lc.nx, sp2, 1);
       wave_sync();
     }
   }
@@ -466,9 +493,9 @@ static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
 bool launch_mg_down(const Level& f, const Level& c, int nu, hipStream_t s) {
   const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
   switch (nu) {
-    case 1: hipLaunchKernelGGL((k_mg_down<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 2: hipLaunchKernelGGL((k_mg_down<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 3: hipLaunchKernelGGL((k_mg_down<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 1: launch((k_mg_down<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 2: launch((k_mg_down<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 3: launch((k_mg_down<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c); return true;
     default: return false;
   }
 }
@@ -476,9 +503,9 @@ bool launch_mg_down(const Level& f, const Level& c, int nu, hipStream_t s) {
 bool launch_mg_up(const Level& f, const Level& c, int nu, hipStream_t s) {
   const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
   switch (nu) {
-    case 1: hipLaunchKernelGGL((k_mg_up<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 2: hipLaunchKernelGGL((k_mg_up<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 3: hipLaunchKernelGGL((k_mg_up<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 1: launch((k_mg_up<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 2: launch((k_mg_up<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 3: launch((k_mg_up<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c); return true;
     default: return false;
   }
 }
@@ -501,7 +528,7 @@ bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int
   Bottom B;
   for (int q = 0; q < n; q++) B.L[q] = L[first + q];
   B.n = n; B.coarse_coef = coarse_coef ? 1 : 0; B.nu = nu; B.coarse_iters = coarse_iters;
-  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(BT), 0, s, B);
+  launch(k_mg_bottom, dim3(1), dim3(BT), 0, s, B);
   return true;
 }
 

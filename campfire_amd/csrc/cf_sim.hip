@@ -103,12 +103,12 @@ struct cf_sim {
   hipGraphExec_t gexec8 = nullptr;  // eight steps: one replay per 8 steps amortises the launch
   int gkey[6] = {-1, -1, -1, -1, -1, -1};
   bool use_graph = true;
-  bool mg_simple = false;  // one launch per reference dispatch (debug comparisons)
+  bool mg_simple = false;  // one launch per reference dispatch pair (debug comparisons)
   int mg_bottom = 0;       // first level solved by the single-workgroup bottom kernel
   // profiling
   bool profiling = false;
-  std::vector<hipEvent_t> pev;
-  std::vector<int> ptag;
+  std::vector<cf::LaunchEvents> plaunch;  // one per launch of a profiling pass
+  std::vector<int> ptag;                  // its tag
   std::vector<ProfTag> tags;
   int last_vcycles = 0;
   float last_resid = -1;
@@ -117,17 +117,30 @@ struct cf_sim {
 static const size_t RING_BYTES = CF_ACC_LEN * 4 + CF_RAD_LEN * 16 + CF_MAX_PIECES * 4 * 4;
 
 // ---- profiling of launch sites -------------------------------------------------
+namespace cf {
+thread_local std::vector<LaunchEvents>* prof_sink = nullptr;
+}
+
+// names of per-level launch sites ("mg_down.0" ...), static for the tag table
+static const char* level_tag(const char* base, int l) {
+  static char names[4][16][24];
+  static const char* bases[4] = {"mg_down", "mg_up", "coef_restrict", "relax"};
+  for (int b = 0; b < 4; b++)
+    if (!strcmp(base, bases[b]) && l >= 0 && l < 16) {
+      if (!names[b][l][0]) snprintf(names[b][l], sizeof(names[b][l]), "%s.%d", base, l);
+      return names[b][l];
+    }
+  return base;
+}
+
+// attributes every launch since the previous mark to `tag`
 static void mark(cf_sim* s, const char* tag) {
   if (!s->profiling) return;
   int id = -1;
   for (size_t q = 0; q < s->tags.size(); q++)
     if (!strcmp(s->tags[q].name, tag)) { id = (int)q; break; }
   if (id < 0) { s->tags.push_back({tag, 0.0, 0}); id = (int)s->tags.size() - 1; }
-  hipEvent_t e;
-  hipEventCreate(&e);
-  hipEventRecord(e, s->st);
-  s->pev.push_back(e);
-  s->ptag.push_back(id);
+  while (s->ptag.size() < s->plaunch.size()) s->ptag.push_back(id);
 }
 
 // ---- step schedule ---------------------------------------------------------
@@ -136,9 +149,9 @@ static void enqueue_vcycle(cf_sim* s, int nu) {  // js/fluid.js:294-312
   if (!s->mg_simple) {
     // fused: tiled down legs, one LDS-resident workgroup for the bottom levels, tiled up legs
     const int b = s->mg_bottom;
-    for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, s->st); mark(s, "mg_down"); }
+    for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, s->st); mark(s, level_tag("mg_down", l)); }
     cf::launch_mg_bottom(s->L, L, b, false, nu, s->sch.coarse_iters, s->st); mark(s, "mg_bottom");
-    for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, s->st); mark(s, "mg_up"); }
+    for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, s->st); mark(s, level_tag("mg_up", l)); }
     return;
   }
   for (int l = 0; l < L - 1; l++) {
@@ -170,6 +183,10 @@ static int enqueue_stage(cf_sim* s, int stage) {
   Fields& F = s->F;
   switch (stage) {
     case 0:
+      if (s->mg_simple) return enqueue_stage(s, 8);
+      cf::launch_advect(F, s->st); mark(s, "advect");
+      break;
+    case 8:
       cf::launch_advect_fwd(F, s->st); mark(s, "advect_fwd");
       cf::launch_advect_corr(F, s->st); mark(s, "advect_corr");
       break;
@@ -190,10 +207,10 @@ static int enqueue_stage(cf_sim* s, int stage) {
     case 7:
       cf::launch_forces_coef(F, s->L, s->nlev, nullptr, 0, s->st); mark(s, "forces_coef");
       for (int l = 1 + cf::forces_coef_cascade(s->nlev); l < s->nlev; l++) {
-        cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, "coef_restrict");
+        cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, level_tag("coef_restrict", l));
       }
       break;
-    default: return set_err(CF_EINVAL, "stage %d out of range 0..7", stage);
+    default: return set_err(CF_EINVAL, "stage %d out of range 0..8", stage);
   }
   return CF_OK;
 }
@@ -226,16 +243,16 @@ static void enqueue_step(cf_sim* s) {
     cf::launch_forces_coef(F, s->L, s->nlev, s->dAccPart, (int)cf::sources_blocks(F), s->st); mark(s, "forces_coef");
     const int last_coef = s->nlev - 1 - (s->mg_bottom < s->nlev - 1 ? 1 : 0);
     for (int l = 1 + cf::forces_coef_cascade(s->nlev); l <= last_coef; l++) {
-      cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, "coef_restrict");
+      cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, level_tag("coef_restrict", l));
     }
     const bool coarse_coef = last_coef < s->nlev - 1 && s->nlev - 1 > cf::forces_coef_cascade(s->nlev);
     for (int c = 0; c < s->sch.vcycles; c++) {
       const int b = s->mg_bottom;
-      for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, "mg_down"); }
+      for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, level_tag("mg_down", l)); }
       // the coarsest coefficients are restricted once, in the first V-cycle's bottom solve
       cf::launch_mg_bottom(s->L, s->nlev, b, coarse_coef && c == 0, s->sch.nu, s->sch.coarse_iters, s->st);
       mark(s, "mg_bottom");
-      for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, "mg_up"); }
+      for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, level_tag("mg_up", l)); }
     }
   }
   enqueue_stage(s, 6);
@@ -341,7 +358,6 @@ void cf_sim_destroy(cf_sim* s) {
   drop_graph(s);
   for (int q = 0; q < UP_SLOTS; q++) { hipEventDestroy(s->evP[q]); hipEventDestroy(s->evPc[q]); }
   for (int q = 0; q < 3; q++) hipEventDestroy(s->ringEv[q]);
-  for (auto e : s->pev) hipEventDestroy(e);
   if (s->hP) hipHostFree(s->hP);
   if (s->hPieces) hipHostFree(s->hPieces);
   if (s->ring) hipHostFree(s->ring);
@@ -708,33 +724,33 @@ int cf_sim_profile_steps(cf_sim* s, int nsteps, float* stage_ms) {
 // returns, per kernel name, the summed milliseconds and the launch count.
 int cf_sim_profile_kernels(cf_sim* s, int nsteps, int max, char* names, float* ms, int* launches, int* count) {
   if (!s || !names || !ms || !launches || !count) return set_err(CF_EINVAL, "NULL argument");
-  for (auto e : s->pev) hipEventDestroy(e);
-  s->pev.clear(); s->ptag.clear(); s->tags.clear();
+  s->plaunch.clear(); s->ptag.clear(); s->tags.clear();
   s->profiling = true;
-  hipEvent_t e0;
-  hipEventCreate(&e0);
-  hipEventRecord(e0, s->st);
+  cf::prof_sink = &s->plaunch;
   for (int n = 0; n < nsteps; n++) { enqueue_step(s); s->time += s->dt; }
+  cf::prof_sink = nullptr;
   s->profiling = false;
-  HIPCHK(hipStreamSynchronize(s->st));
-  hipEvent_t prev = e0;
-  for (size_t q = 0; q < s->pev.size(); q++) {
+  mark(s, "other");  // launches after the last mark (none in a step)
+  hipError_t e = hipStreamSynchronize(s->st);
+  for (size_t q = 0; q < s->plaunch.size(); q++) {
     float t = 0;
-    hipEventElapsedTime(&t, prev, s->pev[q]);
-    s->tags[s->ptag[q]].ms += t;
-    s->tags[s->ptag[q]].launches++;
-    prev = s->pev[q];
+    if (e == hipSuccess && hipEventElapsedTime(&t, s->plaunch[q].start, s->plaunch[q].stop) == hipSuccess) {
+      s->tags[s->ptag[q]].ms += t;
+      s->tags[s->ptag[q]].launches++;
+    }
   }
-  const int n = (int)s->tags.size() < max ? (int)s->tags.size() : max;
-  for (int q = 0; q < n; q++) {
-    snprintf(names + 32 * q, 32, "%s", s->tags[q].name);
-    ms[q] = (float)s->tags[q].ms;
-    launches[q] = s->tags[q].launches;
+  for (auto& ev : s->plaunch) { hipEventDestroy(ev.start); hipEventDestroy(ev.stop); }
+  s->plaunch.clear(); s->ptag.clear();
+  if (e != hipSuccess) return set_err(CF_EDEVICE, "profile_kernels: %s", hipGetErrorString(e));
+  int n = 0;
+  for (size_t q = 0; q < s->tags.size() && n < max; q++) {
+    if (!s->tags[q].launches) continue;
+    snprintf(names + 32 * n, 32, "%s", s->tags[q].name);
+    ms[n] = (float)s->tags[q].ms;
+    launches[n] = s->tags[q].launches;
+    n++;
   }
   *count = n;
-  hipEventDestroy(e0);
-  for (auto e : s->pev) hipEventDestroy(e);
-  s->pev.clear(); s->ptag.clear();
   return CF_OK;
 }
 

@@ -152,7 +152,9 @@ int  cf_sim_frame(cf_sim* sim, int nsteps, int with_radiation);
  * (js/fluid.js:256-292): 0 advect, 1 diffuse, 2 sources, 3 forces,
  * 4 coef/rhs, 5 vcycle, 6 project.  Running 0..6 in order equals one step
  * (time is not advanced).  Stage 7 runs stages 3 and 4 as the production step
- * does, in one fused pass. */
+ * does, in one fused pass.  Stage 0 advects in one pass and leaves the
+ * forward fields Th, Yh, Uh, Vh unwritten; stage 8 runs the reference's two
+ * dispatch pairs instead and writes them. */
 int  cf_sim_run_stage(cf_sim* sim, int stage);
 /* encodeStepProfiled: runs nsteps steps with HIP events between the seven
  * stages and returns the summed milliseconds per stage in stage_ms[7]. */

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from helpers import make_pair, put, rel_l2, sync_geometry
+from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -47,8 +48,11 @@ def test_geometry_parity():
     assert np.max(np.abs(fw_g - fw_o)) <= 4, (fw_g[fw_g != fw_o], fw_o[fw_g != fw_o])
 
 
-@pytest.mark.parametrize("stage", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("stage", [0, 1, 2, 3, 4, 5, 6, 8])
 def test_stage_parity(stage):
+    """Each profiled stage against the oracle's on identical inputs.  Stage 0
+    is the single-pass advection (T2, Y2, U2, V2); stage 8 is the reference's
+    two dispatch pairs, which also leave the forward fields Th, Yh, Uh, Vh."""
     _flush("[parity] stage", stage)
     g, o, pieces, n = make_pair(96, 80, 0.004, 0.002, seed=11 + stage, npieces=5)
     sync_geometry(g, o)
@@ -63,9 +67,10 @@ def test_stage_parity(stage):
         g.run_stage(5)
         o.run_stage(5)
     g.run_stage(stage)
-    rc = o.run_stage(stage)
+    rc = o.run_stage(0 if stage == 8 else stage)
     assert rc == 0
-    outs = {0: ["Th", "Yh", "T2", "Y2", "Uh", "Vh", "U2", "V2"], 1: ["T", "Y", "Th", "Yh"],
+    outs = {0: ["T2", "Y2", "U2", "V2"], 8: ["Th", "Yh", "T2", "Y2", "Uh", "Vh", "U2", "V2"],
+            1: ["T", "Y", "Th", "Yh"],
             2: ["T", "Y", "SQ"], 3: ["U", "V"], 4: ["ax", "ay", "rhs", "ax1", "ay1"],
             5: ["p", "p1"], 6: ["U", "V"]}[stage]
     for name in outs:
@@ -80,6 +85,25 @@ def test_stage_parity(stage):
         assert d[:512].max() <= 8
         assert abs(int(ag[512]) - int(ao[512])) <= max(2, abs(int(ao[512])) * 1e-5)
         assert ag[514] == ao[514]
+
+
+@pytest.mark.parametrize("dt", [0.002, 0.008, 0.02])
+def test_advect_large_displacement(dt):
+    """Single-pass advection where departure points travel 1.5, 6 and 15 cells:
+    footprints leave the LDS windows and the points are redone from global
+    memory (including the forward pass evaluated at the footprint corners)."""
+    _flush("[parity] advect dt", dt)
+    g, o, pieces, n = make_pair(160, 96, 0.004, dt, seed=17, npieces=4)
+    sync_geometry(g, o)
+    g.run_stage(0)
+    o.run_stage(0)
+    for name in ["T2", "Y2", "U2", "V2"]:
+        a, b = g.read_field(name), o.read(name)
+        r = rel_l2(a, b)
+        _flush("  %s rel_l2=%.3g max|d|=%.3g" % (name, r, np.max(np.abs(a.astype(np.float64) - b))))
+        assert r <= 1e-6, name
+    g.close()
+    o.close()
 
 
 def test_one_step_parity():
@@ -204,22 +228,31 @@ def test_fused_forces_coef_matches_stages_3_and_4():
 
 
 def test_steps_512_stagewise_then_statistics():
-    """512^2 random-field step compared stage by stage (1e-5 relative L2 per
-    stage output), then two more steps for the short-horizon statistics.
+    """512^2 random-field step compared stage by stage, each stage fed the same
+    inputs on both sides (1e-5 relative L2 per stage output, 1e-6 for the
+    projection), then two more steps checked against the FP32 oracle's own
+    sensitivity.
 
-    After the projection the bar for U, V is relative to the pre-projection
-    velocity: on random fields one V-cycle leaves a pressure correction as
-    large as u* itself, so the projected velocity is a difference of two
-    nearly equal fields and its own norm is the wrong scale for round-off."""
+    On random fields the pressure reaches ~6e4 and the projection turns
+    |u*| ~ 1 into |u| ~ 70: u = u* - dt/rho grad p amplifies a 1e-7 relative
+    change of p into ~2e-5 of u.  The oracle shows the same amplification
+    against itself (rhs perturbed by 1e-7), so after the first projection the
+    fields are compared to that floor rather than to a fixed 1e-5."""
     _flush("[parity] 512x512 stagewise")
     g, o, pieces, n = make_pair(512, 512, 0.0025, 0.0005, seed=51, npieces=8)
     sync_geometry(g, o)
+    # a second oracle on the same state, for the FP32 noise floor
+    o2 = Oracle(512, 512, 0.0025, 0.0005)
+    o2.reset()
+    o2.upload_pieces(pieces, n)
+    o2.write_params()
+    for name in ("cellG", "fx", "fy", "faceW", "T", "T2", "Th", "Y", "Y2", "Yh",
+                 "U", "U2", "Uh", "V", "V2", "Vh", "p"):
+        o2.write(name, o.read(name))
 
-    def cmp(stage, names, tol=1e-5, scale=None):
+    def cmp(stage, names, tol=1e-5):
         for name in names:
-            a, b = g.read_field(name), o.read(name)
-            r = rel_l2(a, b) if scale is None else \
-                np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(scale[name])
+            r = rel_l2(g.read_field(name), o.read(name))
             _flush("  after %-9s %-3s rel_l2=%.3g" % (stage, name, r))
             assert r <= tol, (stage, name, r)
 
@@ -230,17 +263,29 @@ def test_steps_512_stagewise_then_statistics():
     cmp("sources", ["T", "Y", "SQ"])
     g.run_stage(7); o.run_stage(3); o.run_stage(4)
     cmp("forces", ["U", "V", "rhs", "ax", "ay"])
-    ustar = {"U": o.read("U").astype(np.float64), "V": o.read("V").astype(np.float64)}
     g.run_stage(5); o.run_stage(5)
-    cmp("vcycle", ["p"])
+    cmp("vcycle", ["p"], tol=1e-6)
+    # the projection on identical p
+    o.write("p", g.read_field("p"))
     g.run_stage(6); o.run_stage(6)
-    cmp("project", ["U", "V"], scale=ustar)
+    cmp("project", ["U", "V"], tol=1e-6)
+
     g.step(2)
     o.step(2)
-    for name in ["T", "Y"]:
+    # the oracle against itself with the first rhs perturbed by 1e-7 relative
+    for st in range(5):
+        o2.run_stage(st)
+    rhs = o2.read("rhs")
+    rng = np.random.default_rng(3)
+    o2.write("rhs", (rhs * (1 + 1e-7 * rng.standard_normal(rhs.size))).astype(np.float32))
+    o2.run_stage(5)
+    o2.run_stage(6)
+    o2.step(2)
+    for name in ["T", "Y", "U", "V"]:
         r = rel_l2(g.read_field(name), o.read(name))
-        _flush("  step 3 %s rel_l2=%.3g" % (name, r))
-        assert r <= 1e-2, name
+        floor = rel_l2(o2.read(name), o.read(name))
+        _flush("  step 3 %s rel_l2=%.3g  oracle floor=%.3g" % (name, r, floor))
+        assert r <= max(10 * floor, 1e-5), (name, r, floor)
     ag, ao = g.read_field("acc"), o.read("acc")
     _flush("  HRR", ag[512], ao[512])
     assert abs(int(ag[512]) - int(ao[512])) <= 0.01 * abs(int(ao[512]))
