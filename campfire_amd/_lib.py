@@ -47,6 +47,16 @@ class CfSimInfo(C.Structure):
                 ("last_vcycles", C.c_int32), ("last_resid", C.c_float)]
 
 
+class CfFlameCase(C.Structure):
+    _fields_ = [("B", C.c_double), ("Ta", C.c_double), ("dx", C.c_double), ("F", C.c_double),
+                ("phi", C.c_double), ("L", C.c_double), ("tEnd", C.c_double),
+                ("Tamb", C.c_double), ("Tign", C.c_double), ("max_steps", C.c_int32), ("pad_", C.c_int32)]
+
+
+class CfFlameResult(C.Structure):
+    _fields_ = [("SL", C.c_double), ("Tb", C.c_double), ("steps", C.c_int32), ("n", C.c_int32)]
+
+
 class CampfireError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%s)" % (msg, ERR_NAMES.get(code, code)))
@@ -95,6 +105,7 @@ def lib():
         "cf_sim_write_field": ([p, cp, vp, sz], i),
         "cf_sim_get_info": ([p, C.POINTER(CfSimInfo)], i),
         "cf_sim_div_error": ([p, C.POINTER(f), C.POINTER(f)], i),
+        "cf_flame1d_run": ([C.POINTER(CfFlameCase), i, i, C.POINTER(CfFlameResult)], i),
         "cf_last_error": ([], cp),
         "cf_version": ([], cp),
     }

@@ -23,7 +23,7 @@ ARCH = os.environ.get("CF_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-command-line-argument"]
-SOURCES = ["cf_kernels.hip", "cf_advect.hip", "cf_mg.hip", "cf_sim.hip"]
+SOURCES = ["cf_kernels.hip", "cf_advect.hip", "cf_mg.hip", "cf_flame1d.hip", "cf_sim.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -83,6 +83,8 @@ void launch_project(const Fields& F, hipStream_t s);
 void launch_geometry(const Fields& F, hipStream_t s);
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s);
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s);
+// tests/flame1d.mjs on the GPU (cf_flame1d.hip): one workgroup per case
+void launch_flame1d(const cf_flame_case* cases, int ncases, bool fp64, cf_flame_result* out, hipStream_t s);
 unsigned div_error_blocks(const Fields& F);
 void launch_div_error(const Fields& F, float* part, hipStream_t s);
 

@@ -867,4 +867,34 @@ int cf_sim_div_error(cf_sim* s, float* max_err, float* mean_err) {
   return CF_OK;
 }
 
+int cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_result* out) {
+  if (!cases || !out || ncases < 1) return set_err(CF_EINVAL, "bad argument");
+  std::vector<cf_flame_case> c(cases, cases + ncases);
+  for (auto& k : c) {
+    if (!(k.dx > 0) || !(k.L > 0) || !(k.tEnd >= 0) || !(k.F > 0)) return set_err(CF_EINVAL, "flame case: bad dx, L, tEnd or F");
+    const double n = floor(k.L / k.dx + 0.5);  // Math.round (tests/flame1d.mjs:15)
+    if (n < 2 || n > CF_FLAME_MAXN) return set_err(CF_EBADSIZE, "flame case: %g cells (2..%d)", n, CF_FLAME_MAXN);
+    if (k.max_steps <= 0) k.max_steps = 10000000;
+  }
+  cf_flame_case* dc = nullptr;
+  cf_flame_result* dr = nullptr;
+  hipStream_t st = nullptr;
+  int rc = CF_OK;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void**)&dc, sizeof(cf_flame_case) * ncases);
+  if (e == hipSuccess) e = hipMalloc((void**)&dr, sizeof(cf_flame_result) * ncases);
+  if (e == hipSuccess) e = hipMemcpyAsync(dc, c.data(), sizeof(cf_flame_case) * ncases, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    cf::launch_flame1d(dc, ncases, fp64 != 0, dr, st);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dr, sizeof(cf_flame_result) * ncases, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) rc = set_err(CF_EDEVICE, "flame1d: %s", hipGetErrorString(e));
+  if (dc) hipFree(dc);
+  if (dr) hipFree(dr);
+  if (st) hipStreamDestroy(st);
+  return rc;
+}
+
 }  // extern "C"

@@ -203,6 +203,26 @@ int  cf_sim_get_info(const cf_sim* sim, cf_sim_info* info);
  * dev/gas_test.html:101-111), computed on the device. */
 int  cf_sim_div_error(cf_sim* sim, float* max_err, float* mean_err);
 
+/* ---- 1D premixed flame (tests/flame1d.mjs:13-71) ---------------------------
+ * flameSpeed({B, Ta, dx, F, phi, L, tEnd}) on the GPU, one workgroup per case,
+ * all cases in one launch.  The strip has n = Math.round(L/dx) cells (at most
+ * CF_FLAME_MAXN), the first Math.round(0.01/dx) ignited at Tign. */
+#define CF_FLAME_MAXN 1536
+typedef struct cf_flame_case {
+  double B, Ta, dx, F, phi, L, tEnd;  /* the script's flameSpeed arguments (:13) */
+  double Tamb, Tign;                  /* 293.15 K and 2000 K in the script (:6, :21) */
+  int32_t max_steps;                  /* cap on the device time loop; 0 = 10 million */
+  int32_t pad_;
+} cf_flame_case;
+typedef struct cf_flame_result {
+  double SL, Tb;      /* { SL, Tb } of flameSpeed (:68-70) */
+  int32_t steps, n;   /* time steps taken, cells */
+} cf_flame_result;
+/* fp64 = 1 runs the script's double arithmetic, 0 the single-precision form.
+ * Synchronous; returns CF_EBADSIZE when a case needs more than CF_FLAME_MAXN
+ * or fewer than 2 cells. */
+int  cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_result* out);
+
 /* Thread-local text of the last error. */
 const char* cf_last_error(void);
 /* Library version string. */

@@ -19,5 +19,11 @@
 typedef struct { double SL, Tb; int steps; } cfo_flame_result;
 cfo_flame_result cfo_flame_speed(double B, double Ta, double dx, double F,
                                  double phi, double L, double tEnd);
+/* The same with the ambient and ignition temperatures as arguments (293.15 K
+ * and 2000 K in the script), in double and in single precision. */
+cfo_flame_result cfo_flame_speed_ex(double B, double Ta, double dx, double F, double phi, double L, double tEnd,
+                                    double Tamb, double Tign);
+cfo_flame_result cfo_flame_speed32_ex(double B, double Ta, double dx, double F, double phi, double L, double tEnd,
+                                      double Tamb, double Tign);
 
 #endif

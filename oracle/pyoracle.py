@@ -79,13 +79,22 @@ def lib():
             getattr(L, p + "closure").restype = C.c_double
         L.cfo_flame_speed.restype = FlameResult
         L.cfo_flame_speed.argtypes = [C.c_double] * 7
+        for n in ("cfo_flame_speed_ex", "cfo_flame_speed32_ex"):
+            getattr(L, n).restype = FlameResult
+            getattr(L, n).argtypes = [C.c_double] * 9
         _LIB = L
     return _LIB
 
 
-def flame_speed(B, Ta, dx, F=1.0, phi=1.0, L=0.12, tEnd=0.25):
-    """tests/flame1d.mjs flameSpeed() restated in C (oracle/flame1d.c)."""
-    r = lib().cfo_flame_speed(B, Ta, dx, F, phi, L, tEnd)
+def flame_speed(B, Ta, dx, F=1.0, phi=1.0, L=0.12, tEnd=0.25, Tamb=None, Tign=None, fp32=False):
+    """tests/flame1d.mjs flameSpeed() restated in C (oracle/flame1d.c).  Tamb
+    and Tign default to the script's 293.15 K and 2000 K; fp32 runs the
+    single-precision build of the same algorithm."""
+    if Tamb is None and Tign is None and not fp32:
+        r = lib().cfo_flame_speed(B, Ta, dx, F, phi, L, tEnd)
+    else:
+        fn = lib().cfo_flame_speed32_ex if fp32 else lib().cfo_flame_speed_ex
+        r = fn(B, Ta, dx, F, phi, L, tEnd, 293.15 if Tamb is None else Tamb, 2000.0 if Tign is None else Tign)
     return {"SL": r.SL, "Tb": r.Tb, "steps": r.steps}
 
 

@@ -75,3 +75,14 @@ def test_reference_mode_hierarchies():
     # the reference's 6-level cap is lifted: 1024^2 reaches 16x16 in 7 levels
     o = Oracle(1024, 1024, 0.00125, 0.00025)
     assert o.nlevels == 7
+
+
+def test_flame1d_rejects_bad_sizes_without_device_work():
+    import ctypes as C
+    from campfire_amd import _lib as L
+    c = (L.CfFlameCase * 1)()
+    c[0].B, c[0].Ta, c[0].dx, c[0].F, c[0].phi, c[0].L, c[0].tEnd = 2e10, 15000, 1e-5, 1, 1, 0.03, 1
+    out = (L.CfFlameResult * 1)()
+    assert L.lib().cf_flame1d_run(c, 1, 1, out) == L.CF_EBADSIZE   # 3000 cells > CF_FLAME_MAXN
+    c[0].dx = 0.0
+    assert L.lib().cf_flame1d_run(c, 1, 1, out) == L.CF_EINVAL
