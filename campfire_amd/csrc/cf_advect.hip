@@ -184,7 +184,14 @@ __device__ __forceinline__ float g_corr_v(const Fields& F, const cf_params& P, i
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_advect(Fields F) {
+struct AdvectOp {
+  using Args = Fields;
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Fields& F, int tile);
+  static int tiles(int nx, int ny) { return ((nx + AX - 1) / AX) * ((ny + AY - 1) / AY); }
+};
+
+__device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   __shared__ float sT[A_CX * A_CY];
   __shared__ float4 sY[A_CX * A_CY];
   __shared__ float sU[A_UX * A_UY];
@@ -195,9 +202,8 @@ __global__ __launch_bounds__(256) void k_advect(Fields F) {
   __shared__ float bV[B_VX * B_VY];
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
-  const int gx = (nx + AX - 1) / AX, gy = (ny + AY - 1) / AY;
-  const int tl = xcd_tile(blockIdx.x, gx * gy);
-  const int x0 = (tl % gx) * AX, y0 = (tl / gx) * AY;
+  const int gx = (nx + AX - 1) / AX;
+  const int x0 = (tile % gx) * AX, y0 = (tile / gx) * AY;
   const int tid = threadIdx.x;
   const float dt = P.dt;
   const float4 aY = make_float4(0.f, P.YOamb, 0.f, 0.f);
@@ -341,9 +347,14 @@ __global__ __launch_bounds__(256) void k_advect(Fields F) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_advect(Fields F) { AdvectOp::run(F, xcd_tile(blockIdx.x, gridDim.x)); }
+
 void launch_advect(const Fields& F, hipStream_t s) {
-  const unsigned nb = (unsigned)(((F.nx + AX - 1) / AX) * ((F.ny + AY - 1) / AY));
-  launch(k_advect, dim3(nb), dim3(256), 0, s, F);
+  launch(k_advect, dim3(AdvectOp::tiles(F.nx, F.ny)), dim3(256), 0, s, F);
+}
+int advect_tiles(int nx, int ny) { return AdvectOp::tiles(nx, ny); }
+void launch_advect_batch(const Fields* args, const Item* items, int n, hipStream_t s) {
+  if (n > 0) launch(k_batch<AdvectOp>, dim3(n), dim3(256), 0, s, args, items, n);
 }
 
 }  // namespace cf

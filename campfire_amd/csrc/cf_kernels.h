@@ -15,6 +15,24 @@ struct LaunchEvents {
 // every launch appends its events here.
 extern thread_local std::vector<LaunchEvents>* prof_sink;
 
+// ---- batched launches ------------------------------------------------------------
+// A kernel's work is written once as Op::run(args, tile): what one workgroup
+// does for tile `tile` of one scene.  The single-scene kernels call it with
+// their (XCD-remapped) block index; k_batch runs many scenes of any sizes in
+// one launch from a table of work items, each naming an argument block (one
+// per scene) and a tile of it.  A scene's tiles and their arithmetic are the
+// same either way, so its results do not depend on what it is batched with.
+struct Item {
+  int arg, tile;
+};
+
+template <class Op>
+__global__ __launch_bounds__(Op::kThreads) void k_batch(const typename Op::Args* __restrict__ args,
+                                                       const Item* __restrict__ items, int n) {
+  const Item it = items[xcd_tile(blockIdx.x, n)];
+  Op::run(args[it.arg], it.tile);
+}
+
 template <typename... KArgs, typename... Args>
 inline void launch(void (*k)(KArgs...), dim3 grid, dim3 block, unsigned shmem, hipStream_t s, Args... args) {
   if (prof_sink) {
@@ -28,10 +46,27 @@ inline void launch(void (*k)(KArgs...), dim3 grid, dim3 block, unsigned shmem, h
   hipLaunchKernelGGL(k, grid, block, shmem, s, args...);
 }
 
+// Per-scene control values in device memory, right after the parameter block.
+// dsub and dt arrive with every parameter upload (fixed dt) or are written by
+// the CFL kernel at the start of a step; the rest is scratch of the control
+// kernels.
+struct Dyn {
+  float dsub;          // diffusion substep dt (DiffParams.dt, js/fluid.js:196-198)
+  float dt;            // this step's dt (mirrors the parameter block's dt)
+  uint32_t umax_bits;  // CFL: max |u| over the faces, as float bits (atomicMax)
+  uint32_t cfl_done;   // CFL: workgroups finished
+  uint32_t rn_done;    // residual norm: workgroups finished
+  int32_t converged;   // V-cycle loop: relative residual below tol this step
+  int32_t cycles;      // V-cycles run this step
+  float resid;         // relative residual after the last V-cycle
+  double time;         // simulated time accumulated by the CFL kernel
+};
+
 // Every device buffer of one scene (the `buf` dictionary of js/fluid.js:52-68
 // plus the level-0 multigrid arrays).  Passed by value as a kernel argument.
 struct Fields {
   const cf_params* P;  // the 160-B parameter block in device memory
+  Dyn* dyn;            // control values right after it
   int nx, ny;
   float *U, *Uh, *U2, *V, *Vh, *V2, *T, *Th, *T2;
   float4 *Y, *Yh, *Y2;
@@ -54,11 +89,12 @@ struct Level {
 void launch_advect(const Fields& F, hipStream_t s);
 void launch_advect_fwd(const Fields& F, hipStream_t s);
 void launch_advect_corr(const Fields& F, hipStream_t s);
-void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s);
+// one diffusion substep of length dyn->dsub
+void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, hipStream_t s);
 // sources (+ fused last diffusion substep) and the fold of its per-workgroup
 // accumulator partials (part: 4 ints per workgroup, sources_blocks() of them)
 unsigned sources_blocks(const Fields& F);
-void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, bool finish,
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, int* part, bool finish,
                     hipStream_t s);
 void launch_forces(const Fields& F, hipStream_t s);
 void launch_coef_rhs(const Fields& F, hipStream_t s);
@@ -67,19 +103,36 @@ void launch_coef_rhs(const Fields& F, hipStream_t s);
 // also folds nparts sources partials into acc (nparts = 0: none pending)
 int forces_coef_cascade(int nlev);
 void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* part, int nparts, hipStream_t s);
-// fused multigrid (cf_mg.hip)
-bool launch_mg_down(const Level& f, const Level& c, int nu, hipStream_t s);
-bool launch_mg_up(const Level& f, const Level& c, int nu, hipStream_t s);
+// fused multigrid (cf_mg.hip); skip (may be null): the launch does nothing
+// once *skip is non-zero (the residual target of the step has been met)
+bool launch_mg_down(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s);
+bool launch_mg_up(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s);
 // first level of the single-workgroup bottom solver (the last 1-3 levels)
 int mg_bottom_first(const Level* L, int nlev);
 // coarse_coef: restrict the coarsest level's coefficients inside the kernel
-bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, hipStream_t s);
+bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, const int* skip,
+                      hipStream_t s);
+// step control (cf_control.hip)
+// dt = cfl * dx / max(max|u|, umin) into the parameter block and dyn, dsub =
+// dt / nsub, time += dt
+void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s);
+// ||r|| / ||rhs|| of level 0 after a V-cycle into dyn->resid (FP64 sums);
+// sets dyn->converged when below tol.  part: 2 doubles per workgroup
+unsigned resnorm_blocks(const Level& L0);
+// defect correction at level 0: r = b - A p64 (FP64) into L0.rhs, L0.p = 0,
+// ||r||/||b|| into dyn; then, after a V-cycle on the correction, p64 += L0.p
+void launch_dc_defect(const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol, bool first,
+                      hipStream_t s);
+void launch_dc_accum(const Level& L0, double* p64, const int* skip, hipStream_t s);
+void launch_p32_to_64(const float* p, double* p64, int n, hipStream_t s);
+void launch_resnorm(const Level& L0, Dyn* dyn, double* part, float tol, hipStream_t s);
 void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s);
 void launch_relax(const Level& L, int color, hipStream_t s);
 void launch_restrict_res(const Level& f, const Level& c, hipStream_t s);
 void launch_prolong(const Level& f, const Level& c, hipStream_t s);
 void launch_coarsest(const Level& L, int iters, hipStream_t s);
-void launch_project(const Fields& F, hipStream_t s);
+// p64 (may be null): gradients from the FP64 pressure of defect correction
+void launch_project(const Fields& F, const double* p64, hipStream_t s);
 void launch_geometry(const Fields& F, hipStream_t s);
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s);
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s);

@@ -140,8 +140,9 @@ __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny,
 }
 
 __global__ __launch_bounds__(256) void k_diffuse(Fields F, const float* __restrict__ Tin, const float4* __restrict__ Yin,
-                                                 float* __restrict__ Tout, float4* __restrict__ Yout, float dts) {
+                                                 float* __restrict__ Tout, float4* __restrict__ Yout) {
   const cf_params P = *F.P;
+  const float dts = F.dyn->dsub;
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx, ny, i, j);
@@ -288,7 +289,7 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
 // atomics from every 8x8 tile would otherwise serialise on one L2 line.
 template <bool FUSE>
 __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restrict__ Tin,
-                                                 const float4* __restrict__ Yin, float dts, int* __restrict__ part) {
+                                                 const float4* __restrict__ Yin, int* __restrict__ part) {
   __shared__ float red[3][8][33];
   __shared__ int sacc[CF_ACC_GLOBAL];
   __shared__ int sglob[4][3];
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restri
     const int k = j * nx + i;
     float Tc;
     float4 Yc;
-    if (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, dts, i, j, Tc, Yc);
+    if (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, Tc, Yc);
     else { Tc = F.T[k]; Yc = F.Y[k]; }
     float To;
     float4 Yo;
@@ -684,6 +685,192 @@ __global__ __launch_bounds__(256) void k_restrict_res(Level f, Level c) {
   c.p[J * c.nx + I] = K(0);
 }
 
+// ---- step control ------------------------------------------------------------
+// The last workgroup to finish (a ticket from an atomic counter) completes
+// the reduction, so each control value costs one launch and the step stays a
+// fixed sequence of launches that a graph can replay.
+
+// CFL time step: dt = cfl * dx / max(max |u|, umin) over every U and V face.
+// Max is order independent: per-workgroup maxima meet in one atomicMax on
+// the float bits (non-negative floats order like their bit patterns).
+__global__ __launch_bounds__(256) void k_cfl(Fields F, float cfl, float umin, int nsub) {
+  __shared__ float wm[4];
+  __shared__ int last;
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  const int nU = (nx + 1) * ny, nV = nx * (ny + 1), n = nU > nV ? nU : nV;
+  const int tid = threadIdx.x;
+  float m = K(0);
+  for (int k = blockIdx.x * 256 + tid; k < n; k += gridDim.x * 256) {
+    if (k < nU) m = fmaxf(m, fabsf(F.U[k]));
+    if (k < nV) m = fmaxf(m, fabsf(F.V[k]));
+  }
+  m = wave_tree_max(m);
+  if ((tid & 63) == 0) wm[tid >> 6] = m;
+  __syncthreads();
+  if (tid == 0) {
+    const float b = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+    atomicMax(&F.dyn->umax_bits, __float_as_uint(b));
+    __threadfence();
+    last = atomicAdd(&F.dyn->cfl_done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && tid == 0) {
+    __threadfence();
+    Dyn* d = F.dyn;
+    const float umax = __uint_as_float(atomicOr(&d->umax_bits, 0u));
+    const float dt = cfl * P.dx / fmaxf(umax, umin);
+    ((cf_params*)F.P)->dt = dt;
+    d->dt = dt;
+    d->dsub = (float)((double)dt / (double)nsub);
+    d->time += (double)dt;
+    atomicExch(&d->umax_bits, 0u);
+    atomicExch(&d->cfl_done, 0u);
+  }
+}
+
+// ||r|| / ||rhs|| over level 0 (sums of squares in FP64), after a V-cycle:
+// per-workgroup partials, then the last workgroup adds them in a fixed order,
+// so the value does not depend on scheduling.  Below tol it sets
+// dyn->converged, and the remaining V-cycles of the step are skipped.
+__global__ __launch_bounds__(256) void k_resnorm(Level L, Dyn* d, double* __restrict__ part, float tol) {
+  __shared__ double wr[4], wb[4];
+  __shared__ int last;
+  if (d->converged) return;
+  const int tid = threadIdx.x, n = L.nx * L.ny;
+  double rr = 0, bb = 0;
+  for (int k = blockIdx.x * 256 + tid; k < n; k += gridDim.x * 256) {
+    const float r = residual(L, k % L.nx, k / L.nx);
+    const double b = (double)L.rhs[k];
+    rr += (double)r * (double)r;
+    bb += b * b;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_down(rr, o, 64);
+    bb += __shfl_down(bb, o, 64);
+  }
+  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
+  __syncthreads();
+  if (tid == 0) {
+    part[2 * blockIdx.x] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
+    part[2 * blockIdx.x + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
+    __threadfence();
+    last = atomicAdd(&d->rn_done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  rr = 0; bb = 0;
+  for (int b = tid; b < (int)gridDim.x; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_down(rr, o, 64);
+    bb += __shfl_down(bb, o, 64);
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
+  __syncthreads();
+  if (tid == 0) {
+    rr = ((wr[0] + wr[1]) + wr[2]) + wr[3];
+    bb = ((wb[0] + wb[1]) + wb[2]) + wb[3];
+    const double r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);  // relative unless rhs = 0
+    d->resid = (float)r;
+    d->cycles += 1;
+    if (r < (double)tol) d->converged = 1;
+    atomicExch(&d->rn_done, 0u);
+  }
+}
+
+// ---- defect correction at level 0 (schedule refine = 1) ----------------------
+// The FP32 pressure cannot hold a solution whose residual is below ~1e-5 of
+// ||rhs|| on a burning scene (rounding of p alone leaves that much), so the
+// outer iteration keeps p in FP64 and solves A e = b - A p with FP32 V-cycles.
+
+// the residual b - A p of one cell with p in FP64 (the stencil of
+// shaders/pressure.wgsl:77-94 with the coefficients promoted)
+__device__ __forceinline__ double residual64(const Level& L, const double* __restrict__ p, int i, int j, double b) {
+  const int nx = L.nx, ny = L.ny;
+  const double w = L.ax[j * (nx + 1) + i], e = L.ax[j * (nx + 1) + i + 1];
+  const double s = L.ay[j * nx + i], n = L.ay[(j + 1) * nx + i];
+  double off = 0, diag = 0;
+  if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else { diag += 2.0 * w; }
+  if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else { diag += 2.0 * e; }
+  if (j > 0) { off += s * p[(j - 1) * nx + i]; diag += s; }
+  if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else { diag += 2.0 * n; }
+  if (diag <= 0) return 0;
+  return b - (off - diag * p[j * nx + i]) / (double)L.h2;
+}
+
+// r = b - A p64 into the level's rhs (the correction equation) with e = 0 as
+// its starting guess; ||r|| / ||b|| into dyn (FP64 sums, fixed-order fold by
+// the last workgroup), converged when below tol.  first: b is the level's
+// rhs as the forces pass wrote it, saved to b0 for the later iterations.
+__global__ __launch_bounds__(256) void k_dc_defect(Level L, double* __restrict__ p64, float* __restrict__ b0, Dyn* d,
+                                                   double* __restrict__ part, float tol, int first) {
+  __shared__ double wr[4], wb[4];
+  __shared__ int last;
+  if (d->converged) return;
+  const int tid = threadIdx.x, n = L.nx * L.ny;
+  double rr = 0, bb = 0;
+  for (int k = blockIdx.x * 256 + tid; k < n; k += gridDim.x * 256) {
+    float bf;
+    if (first) { bf = L.rhs[k]; b0[k] = bf; } else bf = b0[k];
+    const double b = (double)bf;
+    const double r = residual64(L, p64, k % L.nx, k / L.nx, b);
+    L.rhs[k] = (float)r;
+    L.p[k] = K(0);
+    rr += r * r;
+    bb += b * b;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_down(rr, o, 64);
+    bb += __shfl_down(bb, o, 64);
+  }
+  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
+  __syncthreads();
+  if (tid == 0) {
+    part[2 * blockIdx.x] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
+    part[2 * blockIdx.x + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
+    __threadfence();
+    last = atomicAdd(&d->rn_done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  rr = 0; bb = 0;
+  for (int b = tid; b < (int)gridDim.x; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_down(rr, o, 64);
+    bb += __shfl_down(bb, o, 64);
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
+  __syncthreads();
+  if (tid == 0) {
+    rr = ((wr[0] + wr[1]) + wr[2]) + wr[3];
+    bb = ((wb[0] + wb[1]) + wb[2]) + wb[3];
+    const double r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+    d->resid = (float)r;
+    if (!first) d->cycles += 1;
+    if (r < (double)tol) d->converged = 1;
+    atomicExch(&d->rn_done, 0u);
+  }
+}
+
+// p64 += e after a correction V-cycle
+__global__ __launch_bounds__(256) void k_dc_accum(Level L, double* __restrict__ p64, const int* __restrict__ skip) {
+  if (*skip) return;
+  const int n = L.nx * L.ny;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) p64[k] += (double)L.p[k];
+}
+
+__global__ __launch_bounds__(256) void k_p32_to_64(const float* __restrict__ p, double* __restrict__ p64, int n) {
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) p64[k] = (double)p[k];
+}
+
 // bilinear coarse correction (:116-129)
 __global__ __launch_bounds__(256) void k_prolong(Level f, Level c) {
   int i, j;
@@ -741,7 +928,11 @@ __global__ __launch_bounds__(256) void k_coarsest(Level L, int iters) {
 }
 
 // velocity correction u -= dt (1/rho) grad p (:159-180)
-__global__ __launch_bounds__(256) void k_project(Fields F) {
+// P64: the pressure differences come from the FP64 pressure of defect
+// correction (rounded to FP32 after the subtraction), and the cells' FP32 p
+// is refreshed from it
+template <bool P64>
+__global__ __launch_bounds__(256) void k_project(Fields F, const double* __restrict__ p64) {
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
   int i, j;
@@ -752,21 +943,38 @@ __global__ __launch_bounds__(256) void k_project(Fields F) {
   if (i <= nx && j < ny) {
     const int k = j * (nx + 1) + i;
     if (F.fx[k] > K(0)) {
-      const float pl = i > 0 ? p[j * nx + max(i - 1, 0)] : -p[j * nx];
-      const float pr = i < nx ? p[j * nx + min(i, nx - 1)] : -p[j * nx + nx - 1];
+      float dp;
+      if (P64) {
+        const double pl = i > 0 ? p64[j * nx + max(i - 1, 0)] : -p64[j * nx];
+        const double pr = i < nx ? p64[j * nx + min(i, nx - 1)] : -p64[j * nx + nx - 1];
+        dp = (float)(pr - pl);
+      } else {
+        const float pl = i > 0 ? p[j * nx + max(i - 1, 0)] : -p[j * nx];
+        const float pr = i < nx ? p[j * nx + min(i, nx - 1)] : -p[j * nx + nx - 1];
+        dp = pr - pl;
+      }
       const float Tf = K(0.5) * (T[j * nx + iclamp(i - 1, 0, nx - 1)] + T[j * nx + min(i, nx - 1)]);
-      F.U[k] -= dt * invRho(P, Tf) * (pr - pl) / dx;
+      F.U[k] -= dt * invRho(P, Tf) * dp / dx;
     } else F.U[k] = K(0);
   }
   if (i < nx && j <= ny) {
     const int k = j * nx + i;
     if (j > 0 && F.fy[k] > K(0)) {
-      const float pb = p[(j - 1) * nx + i];
-      const float pt = j < ny ? p[min(j, ny - 1) * nx + i] : -p[(ny - 1) * nx + i];
+      float dp;
+      if (P64) {
+        const double pb = p64[(j - 1) * nx + i];
+        const double pt = j < ny ? p64[min(j, ny - 1) * nx + i] : -p64[(ny - 1) * nx + i];
+        dp = (float)(pt - pb);
+      } else {
+        const float pb = p[(j - 1) * nx + i];
+        const float pt = j < ny ? p[min(j, ny - 1) * nx + i] : -p[(ny - 1) * nx + i];
+        dp = pt - pb;
+      }
       const float Tf = K(0.5) * (T[(j - 1) * nx + i] + T[min(j, ny - 1) * nx + i]);
-      F.V[k] -= dt * invRho(P, Tf) * (pt - pb) / dx;
+      F.V[k] -= dt * invRho(P, Tf) * dp / dx;
     } else F.V[k] = K(0);
   }
+  if (P64 && i < nx && j < ny) F.p0[j * nx + i] = (float)p64[j * nx + i];
 }
 
 // -------------------------------------------------------------------------
@@ -998,15 +1206,15 @@ void launch_advect_fwd(const Fields& F, hipStream_t s) {
 void launch_advect_corr(const Fields& F, hipStream_t s) {
   launch(k_advect_corr, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
-void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, float dts, hipStream_t s) {
-  launch(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout, dts);
+void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, hipStream_t s) {
+  launch(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout);
 }
 unsigned sources_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
-void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, float dts, int* part, bool finish,
+void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, int* part, bool finish,
                     hipStream_t s) {
   const unsigned nb = nblk(F.nx, F.ny);
-  if (fuse) launch(k_sources<true>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
-  else launch(k_sources<false>, dim3(nb), B328, 0, s, F, Tin, Yin, dts, part);
+  if (fuse) launch(k_sources<true>, dim3(nb), B328, 0, s, F, Tin, Yin, part);
+  else launch(k_sources<false>, dim3(nb), B328, 0, s, F, Tin, Yin, part);
   if (finish) launch(k_acc_finish, dim3(1), dim3(256), 0, s, (const int*)part, (int)nb, F.acc);
 }
 void launch_forces(const Fields& F, hipStream_t s) {
@@ -1041,14 +1249,40 @@ void launch_prolong(const Level& f, const Level& c, hipStream_t s) {
 void launch_coarsest(const Level& L, int iters, hipStream_t s) {
   launch(k_coarsest, dim3(1), dim3(256), 0, s, L, iters);
 }
-void launch_project(const Fields& F, hipStream_t s) {
-  launch(k_project, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+void launch_project(const Fields& F, const double* p64, hipStream_t s) {
+  if (p64) launch(k_project<true>, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F, p64);
+  else launch(k_project<false>, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F, p64);
 }
 void launch_geometry(const Fields& F, hipStream_t s) {
   launch(k_geometry, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s) {
   if (ntargets > 0) launch(k_radiation, dim3(ntargets), dim3(64), 0, s, F);
+}
+void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s) {
+  const int n = (F.nx + 1) * F.ny > F.nx * (F.ny + 1) ? (F.nx + 1) * F.ny : F.nx * (F.ny + 1);
+  unsigned b = (unsigned)((n + 1023) / 1024);
+  if (b > 1024) b = 1024;
+  launch(k_cfl, dim3(b), dim3(256), 0, s, F, cfl, umin, nsub);
+}
+void launch_dc_defect(const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol, bool first,
+                      hipStream_t s) {
+  launch(k_dc_defect, dim3(resnorm_blocks(L0)), dim3(256), 0, s, L0, p64, b0, dyn, part, tol, first ? 1 : 0);
+}
+void launch_dc_accum(const Level& L0, double* p64, const int* skip, hipStream_t s) {
+  unsigned b = (unsigned)((L0.nx * L0.ny + 1023) / 1024);
+  launch(k_dc_accum, dim3(b > 2048 ? 2048 : b), dim3(256), 0, s, L0, p64, skip);
+}
+void launch_p32_to_64(const float* p, double* p64, int n, hipStream_t s) {
+  unsigned b = (unsigned)((n + 1023) / 1024);
+  launch(k_p32_to_64, dim3(b > 2048 ? 2048 : b), dim3(256), 0, s, p, p64, n);
+}
+unsigned resnorm_blocks(const Level& L0) {
+  unsigned b = (unsigned)((L0.nx * L0.ny + 1023) / 1024);
+  return b > 1024 ? 1024 : b;
+}
+void launch_resnorm(const Level& L0, Dyn* dyn, double* part, float tol, hipStream_t s) {
+  launch(k_resnorm, dim3(resnorm_blocks(L0)), dim3(256), 0, s, L0, dyn, part, tol);
 }
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s) {
   if (n == 0) return;

@@ -86,7 +86,8 @@ __device__ __forceinline__ void pairs_relax(const PairCell (&c)[PQ][2], const in
 }
 
 template <int TX, int TY, int NU>
-__global__ __launch_bounds__(256) void k_mg_down(Level f, Level c) {
+__global__ __launch_bounds__(256) void k_mg_down(Level f, Level c, const int* __restrict__ skip) {
+  if (skip && *skip) return;  // the step's residual target is met
   constexpr int H = 2 * NU + 1;
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
   constexpr int PQ = (NP + 255) / 256;
@@ -154,7 +155,8 @@ __global__ __launch_bounds__(256) void k_mg_down(Level f, Level c) {
 }
 
 template <int TX, int TY, int NU>
-__global__ __launch_bounds__(256) void k_mg_up(Level f, Level c) {
+__global__ __launch_bounds__(256) void k_mg_up(Level f, Level c, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int H = 2 * NU;
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
   constexpr int PQ = (NP + 255) / 256;
@@ -229,6 +231,7 @@ constexpr int BT = 1024;
 
 struct Bottom {
   Level L[3];       // finest first; the last is the coarsest level
+  const int* skip;  // may be null: nothing to do once *skip is non-zero
   int n;            // 1..3
   int coarse_coef;  // 1: restrict the coarsest level's ax/ay from L[n-2] here
   int nu, coarse_iters;
@@ -378,6 +381,7 @@ __device__ __forceinline__ int bot_p(int s) { return s == 0 ? BOT_P0 : (s == 1 ?
 __device__ __forceinline__ int bot_r(int s) { return s == 0 ? BOT_R0 : (s == 1 ? BOT_R1 : BOT_R2); }
 
 __global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
+  if (B.skip && *B.skip) return;
   __shared__ float sm[BOT_TOTAL];
   float* sp2 = sm + BOT_P2;
   float* sr2 = sm + BOT_R2;
@@ -490,22 +494,22 @@ static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
   return (unsigned)(((nx + tx - 1) / tx) * ((ny + ty - 1) / ty));
 }
 
-bool launch_mg_down(const Level& f, const Level& c, int nu, hipStream_t s) {
+bool launch_mg_down(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s) {
   const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
   switch (nu) {
-    case 1: launch((k_mg_down<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 2: launch((k_mg_down<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 3: launch((k_mg_down<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 1: launch((k_mg_down<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
+    case 2: launch((k_mg_down<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
+    case 3: launch((k_mg_down<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
     default: return false;
   }
 }
 
-bool launch_mg_up(const Level& f, const Level& c, int nu, hipStream_t s) {
+bool launch_mg_up(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s) {
   const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
   switch (nu) {
-    case 1: launch((k_mg_up<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 2: launch((k_mg_up<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c); return true;
-    case 3: launch((k_mg_up<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c); return true;
+    case 1: launch((k_mg_up<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
+    case 2: launch((k_mg_up<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
+    case 3: launch((k_mg_up<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
     default: return false;
   }
 }
@@ -521,13 +525,14 @@ int mg_bottom_first(const Level* L, int nlev) {
   return b;
 }
 
-bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, hipStream_t s) {
+bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, const int* skip,
+                      hipStream_t s) {
   const int n = nlev - first;
   if (n < 1 || n > 3 || L[nlev - 1].nx * L[nlev - 1].ny > 256) return false;
   if (coarse_coef && n < 2) return false;
   Bottom B;
   for (int q = 0; q < n; q++) B.L[q] = L[first + q];
-  B.n = n; B.coarse_coef = coarse_coef ? 1 : 0; B.nu = nu; B.coarse_iters = coarse_iters;
+  B.n = n; B.coarse_coef = coarse_coef ? 1 : 0; B.nu = nu; B.coarse_iters = coarse_iters; B.skip = skip;
   launch(k_mg_bottom, dim3(1), dim3(BT), 0, s, B);
   return true;
 }

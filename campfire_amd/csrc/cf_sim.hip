@@ -65,6 +65,12 @@ struct ProfTag {
   int launches;
 };
 
+// one parameter upload: the 160-B block and the first two fields of Dyn
+struct ParamUpload {
+  cf_params P;
+  float dsub, dt;
+};
+
 struct cf_sim {
   int dev = 0;
   hipStream_t st = nullptr;
@@ -83,8 +89,12 @@ struct cf_sim {
   Fields F;
   Level L[MAXLEV];
   cf_params* dP = nullptr;
+  cf::Dyn* dDyn = nullptr;   // control values, right after the parameter block
+  double* dNorm = nullptr;   // residual-norm partials
+  double* dP64 = nullptr;    // level-0 pressure in FP64 (defect correction)
+  float* dB0 = nullptr;      // level-0 right-hand side kept across correction cycles
   // upload rings
-  cf_params* hP = nullptr;
+  ParamUpload* hP = nullptr;
   float* hPieces = nullptr;
   hipEvent_t evP[UP_SLOTS], evPc[UP_SLOTS];
   bool evP_used[UP_SLOTS], evPc_used[UP_SLOTS];
@@ -101,7 +111,7 @@ struct cf_sim {
   // graph of one step
   hipGraphExec_t gexec = nullptr;   // one step
   hipGraphExec_t gexec8 = nullptr;  // eight steps: one replay per 8 steps amortises the launch
-  int gkey[6] = {-1, -1, -1, -1, -1, -1};
+  int gkey[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   bool use_graph = true;
   bool mg_simple = false;  // one launch per reference dispatch pair (debug comparisons)
   int mg_bottom = 0;       // first level solved by the single-workgroup bottom kernel
@@ -144,14 +154,16 @@ static void mark(cf_sim* s, const char* tag) {
 }
 
 // ---- step schedule ---------------------------------------------------------
-static void enqueue_vcycle(cf_sim* s, int nu) {  // js/fluid.js:294-312
+// one V-cycle (js/fluid.js:294-312); coarse_coef: the bottom kernel also
+// restricts the coarsest level's coefficients; skip: see launch_mg_down
+static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef) {
   const int L = s->nlev;
   if (!s->mg_simple) {
     // fused: tiled down legs, one LDS-resident workgroup for the bottom levels, tiled up legs
     const int b = s->mg_bottom;
-    for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, s->st); mark(s, level_tag("mg_down", l)); }
-    cf::launch_mg_bottom(s->L, L, b, false, nu, s->sch.coarse_iters, s->st); mark(s, "mg_bottom");
-    for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, s->st); mark(s, level_tag("mg_up", l)); }
+    for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, skip, s->st); mark(s, level_tag("mg_down", l)); }
+    cf::launch_mg_bottom(s->L, L, b, coarse_coef, nu, s->sch.coarse_iters, skip, s->st); mark(s, "mg_bottom");
+    for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, skip, s->st); mark(s, level_tag("mg_up", l)); }
     return;
   }
   for (int l = 0; l < L - 1; l++) {
@@ -167,6 +179,28 @@ static void enqueue_vcycle(cf_sim* s, int nu) {  // js/fluid.js:294-312
     for (int q = 0; q < nu; q++) {
       cf::launch_relax(s->L[l], 1, s->st); mark(s, "relax");
       cf::launch_relax(s->L[l], 0, s->st); mark(s, "relax");
+    }
+  }
+}
+
+// The V-cycles of one step (js/fluid.js:250).  With a residual target
+// (tol > 0) up to `vcycles` cycles are enqueued, each followed by the
+// relative residual norm; once the target is met the remaining launches
+// return at once, so the sequence is fixed and a graph replays it.  (The
+// per-dispatch path of CF_MG_SIMPLE has no skip and runs every cycle.)
+static void enqueue_mg(cf_sim* s, bool coarse_coef) {
+  const bool tol = s->sch.tol > 0;
+  const bool dc = tol && s->sch.refine;  // defect correction with the FP64 pressure
+  const int* skip = tol ? &s->dDyn->converged : nullptr;
+  if (tol) { cf::launch_fill_u32(&s->dDyn->converged, 0u, 2, s->st); mark(s, "mg_control"); }  // converged, cycles
+  if (dc) { cf::launch_dc_defect(s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, true, s->st); mark(s, "dc_defect"); }
+  for (int c = 0; c < s->sch.vcycles; c++) {
+    enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0);
+    if (dc) {
+      cf::launch_dc_accum(s->L[0], s->dP64, skip, s->st); mark(s, "dc_accum");
+      cf::launch_dc_defect(s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, false, s->st); mark(s, "dc_defect");
+    } else if (tol) {
+      cf::launch_resnorm(s->L[0], s->dDyn, s->dNorm, s->sch.tol, s->st); mark(s, "resnorm");
     }
   }
 }
@@ -191,19 +225,21 @@ static int enqueue_stage(cf_sim* s, int stage) {
       cf::launch_advect_corr(F, s->st); mark(s, "advect_corr");
       break;
     case 1:
-      cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
+      cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->st); mark(s, "diffuse");
       for (int q = 1; q < s->diffSub; q += 2) {
-        cf::launch_diffuse(F, F.T, F.Y, F.Th, F.Yh, s->dsub, s->st); mark(s, "diffuse");
-        cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
+        cf::launch_diffuse(F, F.T, F.Y, F.Th, F.Yh, s->st); mark(s, "diffuse");
+        cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->st); mark(s, "diffuse");
       }
       break;
-    case 2: cf::launch_sources(F, false, nullptr, nullptr, 0.f, s->dAccPart, true, s->st); mark(s, "sources"); break;
+    case 2: cf::launch_sources(F, false, nullptr, nullptr, s->dAccPart, true, s->st); mark(s, "sources"); break;
     case 3: cf::launch_forces(F, s->st); mark(s, "forces"); break;
     case 4: enqueue_coef(s); break;
     case 5:
-      for (int c = 0; c < s->sch.vcycles; c++) enqueue_vcycle(s, s->sch.nu);
+      enqueue_mg(s, false);
       break;
-    case 6: cf::launch_project(F, s->st); mark(s, "project"); break;
+    case 6:
+      cf::launch_project(F, s->sch.tol > 0 && s->sch.refine ? s->dP64 : nullptr, s->st); mark(s, "project");
+      break;
     case 7:
       cf::launch_forces_coef(F, s->L, s->nlev, nullptr, 0, s->st); mark(s, "forces_coef");
       for (int l = 1 + cf::forces_coef_cascade(s->nlev); l < s->nlev; l++) {
@@ -222,19 +258,21 @@ static int enqueue_stage(cf_sim* s, int stage) {
 static void enqueue_step(cf_sim* s) {
   Fields& F = s->F;
   const bool fused = !s->mg_simple;
+  // CFL time step from the current velocities, on the device
+  if (s->sch.cfl > 0) { cf::launch_cfl(F, s->sch.cfl, s->sch.umin, s->diffSub, s->st); mark(s, "cfl"); }
   enqueue_stage(s, 0);
   const float* Tin = F.T2;
   const float4* Yin = F.Y2;
   if (s->diffSub > 1) {
-    cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse");
+    cf::launch_diffuse(F, F.T2, F.Y2, F.T, F.Y, s->st); mark(s, "diffuse");
     for (int q = 1; q < s->diffSub; q += 2) {
-      cf::launch_diffuse(F, F.T, F.Y, F.Th, F.Yh, s->dsub, s->st); mark(s, "diffuse");
-      if (q + 2 < s->diffSub) { cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->dsub, s->st); mark(s, "diffuse"); }
+      cf::launch_diffuse(F, F.T, F.Y, F.Th, F.Yh, s->st); mark(s, "diffuse");
+      if (q + 2 < s->diffSub) { cf::launch_diffuse(F, F.Th, F.Yh, F.T, F.Y, s->st); mark(s, "diffuse"); }
     }
     Tin = F.Th;
     Yin = F.Yh;
   }
-  cf::launch_sources(F, true, Tin, Yin, s->dsub, s->dAccPart, !fused, s->st); mark(s, "diffuse_sources");
+  cf::launch_sources(F, true, Tin, Yin, s->dAccPart, !fused, s->st); mark(s, "diffuse_sources");
   if (!fused) {
     enqueue_stage(s, 3);
     enqueue_stage(s, 4);
@@ -246,14 +284,8 @@ static void enqueue_step(cf_sim* s) {
       cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, level_tag("coef_restrict", l));
     }
     const bool coarse_coef = last_coef < s->nlev - 1 && s->nlev - 1 > cf::forces_coef_cascade(s->nlev);
-    for (int c = 0; c < s->sch.vcycles; c++) {
-      const int b = s->mg_bottom;
-      for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, level_tag("mg_down", l)); }
-      // the coarsest coefficients are restricted once, in the first V-cycle's bottom solve
-      cf::launch_mg_bottom(s->L, s->nlev, b, coarse_coef && c == 0, s->sch.nu, s->sch.coarse_iters, s->st);
-      mark(s, "mg_bottom");
-      for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], s->sch.nu, s->st); mark(s, level_tag("mg_up", l)); }
-    }
+    // the coarsest coefficients are restricted once, in the first V-cycle's bottom solve
+    enqueue_mg(s, coarse_coef);
   }
   enqueue_stage(s, 6);
 }
@@ -277,7 +309,12 @@ static int capture(cf_sim* s, int nsteps, hipGraphExec_t* out) {
 }
 
 static int ensure_graph(cf_sim* s) {
-  const int key[6] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev};
+  // everything a captured step bakes in as launch arguments
+  int key[10] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev, 0, 0, 0,
+                 s->sch.refine};
+  memcpy(&key[6], &s->sch.tol, 4);
+  memcpy(&key[7], &s->sch.cfl, 4);
+  memcpy(&key[8], &s->sch.umin, 4);
   if (s->gexec && s->gexec8 && !memcmp(key, s->gkey, sizeof(key))) return CF_OK;
   drop_graph(s);
   int rc = capture(s, 1, &s->gexec);
@@ -297,8 +334,10 @@ static int upload_params(cf_sim* s) {
   const int slot = s->upP;
   s->upP = (slot + 1) % UP_SLOTS;
   if (s->evP_used[slot]) HIPCHK(hipEventSynchronize(s->evP[slot]));
-  s->hP[slot] = s->P;
-  HIPCHK(hipMemcpyAsync(s->dP, &s->hP[slot], sizeof(cf_params), hipMemcpyHostToDevice, s->st));
+  s->hP[slot].P = s->P;
+  s->hP[slot].dsub = s->dsub;
+  s->hP[slot].dt = (float)s->dt;
+  HIPCHK(hipMemcpyAsync(s->dP, &s->hP[slot], sizeof(ParamUpload), hipMemcpyHostToDevice, s->st));
   HIPCHK(hipEventRecord(s->evP[slot], s->st));
   s->evP_used[slot] = true;
   return CF_OK;
@@ -401,6 +440,7 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
       1800, 5.670374e-8, 2.667, 0};
   memcpy(s->pv, def, sizeof(def));
   s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
+  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0;
   const char* ng = getenv("CF_NO_GRAPH");
   s->use_graph = !(ng && *ng && *ng != '0');
   const char* ms = getenv("CF_MG_SIMPLE");
@@ -416,7 +456,8 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
   Fields& F = s->F;
   auto add = [&](void** p, size_t bytes) { plan.push_back({p, bytes}); };
   const size_t nC = s->nC, nU = s->nU, nV = s->nV;
-  add((void**)&s->dP, sizeof(cf_params));
+  static_assert(sizeof(cf_params) % 8 == 0 && offsetof(ParamUpload, dsub) == sizeof(cf_params), "upload layout");
+  add((void**)&s->dP, sizeof(cf_params) + sizeof(cf::Dyn));
   add((void**)&F.U, nU * 4); add((void**)&F.Uh, nU * 4); add((void**)&F.U2, nU * 4);
   add((void**)&F.V, nV * 4); add((void**)&F.Vh, nV * 4); add((void**)&F.V2, nV * 4);
   add((void**)&F.T, nC * 4); add((void**)&F.Th, nC * 4); add((void**)&F.T2, nC * 4);
@@ -435,6 +476,9 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
   const unsigned nparts = (unsigned)(((nx + 31) / 32) * ((ny + 7) / 8));
   add((void**)&s->dPart, (size_t)nparts * 8);
   add((void**)&s->dAccPart, (size_t)nparts * 16);
+  add((void**)&s->dNorm, 2 * sizeof(double) * 1024);
+  add((void**)&s->dP64, nC * sizeof(double));
+  add((void**)&s->dB0, nC * sizeof(float));
   size_t total = 0;
   for (auto& q : plan) total += (q.second + 255) & ~(size_t)255;
   e = hipMalloc(&s->block, total);
@@ -446,6 +490,8 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
   }
   s->hPart.resize((size_t)nparts * 2);
   F.P = s->dP; F.nx = nx; F.ny = ny;
+  s->dDyn = (cf::Dyn*)(s->dP + 1);
+  F.dyn = s->dDyn;
   F.p0 = s->L[0].p; F.rhs0 = s->L[0].rhs; F.ax0 = s->L[0].ax; F.ay0 = s->L[0].ay;
   for (int l = 0; l < s->nlev; l++) { s->L[l].nx = s->lnx[l]; s->L[l].ny = s->lny[l]; s->L[l].h2 = s->lh2[l]; }
   // bottom solver: the last 1-3 levels (<= 4096, 1024, 256 cells) in one workgroup
@@ -459,7 +505,7 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
     s->evP_used[q] = s->evPc_used[q] = false;
   }
   for (int q = 0; q < 3; q++) { hipEventCreateWithFlags(&s->ringEv[q], hipEventDisableTiming); s->busy[q] = false; }
-  if (hipHostMalloc((void**)&s->hP, sizeof(cf_params) * UP_SLOTS, hipHostMallocDefault) != hipSuccess ||
+  if (hipHostMalloc((void**)&s->hP, sizeof(ParamUpload) * UP_SLOTS, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&s->hPieces, sizeof(float) * CF_MAX_PIECES * CF_PIECE_FLOATS * UP_SLOTS, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&s->ring, RING_BYTES * 3, hipHostMallocDefault) != hipSuccess) {
     cf_sim_destroy(s);
@@ -497,7 +543,9 @@ int cf_sim_reset(cf_sim* s) {  // js/fluid.js:162-176
   cf::launch_fill_u32(F.acc, 0, CF_ACC_LEN, s->st);
   cf::launch_fill_u32(F.rad, 0, CF_RAD_LEN * 4, s->st);
   for (int l = 0; l < s->nlev; l++) cf::launch_fill_u32(s->L[l].p, 0, (size_t)s->lnx[l] * s->lny[l], s->st);
+  cf::launch_fill_u32(s->dP64, 0, (size_t)s->nC * 2, s->st);
   s->time = 0;
+  HIPCHK(hipMemsetAsync(&s->dDyn->time, 0, sizeof(double), s->st));
   HIPCHK(hipStreamSynchronize(s->st));  // the staging vector goes out of scope
   return CF_OK;
 }
@@ -537,7 +585,9 @@ int cf_sim_write_params(cf_sim* s) {  // js/fluid.js:180-199
   // explicit diffusion stability: dt_sub <= dx^2 / (4.5 alpha(2200 K)), odd count <= 5
   const double Tmax = 2200, mu = 1.716e-5 * pow(Tmax / 273.15, 1.5) * 383.55 / (Tmax + 110.4);
   const double alpha = mu / 0.7 * s->pv[11] / (phys_rhoAmb() * PHYS_Tamb / Tmax);
-  int n = (int)ceil(s->dt / (s->dx * s->dx / (4.5 * alpha)));
+  // with a CFL time step the count is fixed by the largest dt the rule allows
+  const double dtmax = s->sch.cfl > 0 ? (double)s->sch.cfl * s->dx / (double)s->sch.umin : s->dt;
+  int n = (int)ceil(dtmax / (s->dx * s->dx / (4.5 * alpha)));
   n = n % 2 ? n : n + 1;
   if (n > 5) n = 5;
   s->diffSub = n;
@@ -555,8 +605,16 @@ int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
   if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
   if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 1 || !(sc->tol >= 0))
     return set_err(CF_EINVAL, "bad schedule (nu 1..3, vcycles >= 1, coarse_iters >= 1)");
+  if (!(sc->cfl >= 0) || (sc->cfl > 0 && !(sc->umin > 0)))
+    return set_err(CF_EINVAL, "bad schedule (cfl >= 0; umin > 0 with a CFL step)");
+  if (sc->refine && !(sc->tol > 0)) return set_err(CF_EINVAL, "bad schedule (refine needs tol > 0)");
+  // switching defect correction on: the FP64 pressure starts from the FP32 one
+  if (sc->refine && !s->sch.refine) cf::launch_p32_to_64(s->L[0].p, s->dP64, s->nC, s->st);
   if (sc->smoother_fp16) return set_err(CF_EINVAL, "FP16 smoother not available in this build");
+  const bool cfl_changed = sc->cfl != s->sch.cfl || sc->umin != s->sch.umin;
   s->sch = *sc;
+  // the diffusion substep count follows the largest dt of a CFL rule
+  if (cfl_changed) return cf_sim_write_params(s);
   return CF_OK;
 }
 
@@ -603,36 +661,22 @@ int cf_sim_radiation(cf_sim* s) {  // js/fluid.js:220-222
   return CF_OK;
 }
 
-static int run_mg_to_tol(cf_sim* s);
 
 int cf_sim_step(cf_sim* s, int nsteps) {  // js/fluid.js:225-253
   if (!s) return set_err(CF_EINVAL, "sim is NULL");
   if (nsteps < 0) return set_err(CF_EINVAL, "nsteps < 0");
-  if (s->sch.tol > 0) {
-    for (int q = 0; q < nsteps; q++) {
-      for (int st = 0; st < 5; st++) enqueue_stage(s, st);
-      int rc = run_mg_to_tol(s);
-      enqueue_stage(s, 6);
-      s->time += s->dt;
-      if (rc) return rc;
-    }
-    HIPCHK(hipGetLastError());
-    return CF_OK;
-  }
   if (s->use_graph) {
     int rc = ensure_graph(s);
     if (rc) return rc;
     int q = 0;
     for (; q + 8 <= nsteps; q += 8) HIPCHK(hipGraphLaunch(s->gexec8, s->st));
     for (; q < nsteps; q++) HIPCHK(hipGraphLaunch(s->gexec, s->st));
-    s->time += s->dt * nsteps;
   } else {
-    for (int q = 0; q < nsteps; q++) {
-      enqueue_step(s);
-      s->time += s->dt;
-    }
+    for (int q = 0; q < nsteps; q++) enqueue_step(s);
     HIPCHK(hipGetLastError());
   }
+  // with a CFL step the simulated time lives on the device (cf_sim_get_info)
+  if (s->sch.cfl <= 0) s->time += s->dt * nsteps;
   s->last_vcycles = s->sch.vcycles;
   return CF_OK;
 }
@@ -655,46 +699,6 @@ int cf_sim_run_stage(cf_sim* s, int stage) {
   return CF_OK;
 }
 
-// residual-controlled V-cycles (the schedule's tol > 0): residual norm after each cycle
-static int run_mg_to_tol(cf_sim* s) {
-  // norms are computed on the host from the level-0 arrays; a device-side
-  // norm replaces this when the mixed-precision solver lands
-  const int nx = s->nx, ny = s->ny;
-  std::vector<float> p((size_t)nx * ny), rhs((size_t)nx * ny), ax((size_t)(nx + 1) * ny), ay((size_t)nx * (ny + 1));
-  HIPCHK(hipMemcpyAsync(rhs.data(), s->L[0].rhs, rhs.size() * 4, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipMemcpyAsync(ax.data(), s->L[0].ax, ax.size() * 4, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipMemcpyAsync(ay.data(), s->L[0].ay, ay.size() * 4, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
-  double bb = 0;
-  for (float v : rhs) bb += (double)v * v;
-  const double h2 = s->lh2[0];
-  double r = 1e30;
-  int c = 0;
-  for (c = 0; c < s->sch.vcycles; c++) {
-    enqueue_vcycle(s, s->sch.nu);
-    HIPCHK(hipMemcpyAsync(p.data(), s->L[0].p, p.size() * 4, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
-    double rr = 0;
-    for (int j = 0; j < ny; j++)
-      for (int i = 0; i < nx; i++) {
-        const double w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], so = ay[j * nx + i], n = ay[(j + 1) * nx + i];
-        double off = 0, diag = 0;
-        if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else diag += 2 * w;
-        if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else diag += 2 * e;
-        if (j > 0) { off += so * p[(j - 1) * nx + i]; diag += so; }
-        if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else diag += 2 * n;
-        if (diag <= 0) continue;
-        const double res = rhs[j * nx + i] - (off - diag * p[j * nx + i]) / h2;
-        rr += res * res;
-      }
-    r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
-    if (r < s->sch.tol) { c++; break; }
-  }
-  s->last_vcycles = c;
-  s->last_resid = (float)r;
-  if (r >= s->sch.tol) return set_err(CF_ENOCONV, "multigrid residual %.3g above %.3g after %d V-cycles", r, s->sch.tol, c);
-  return CF_OK;
-}
 
 int cf_sim_profile_steps(cf_sim* s, int nsteps, float* stage_ms) {
   if (!s || !stage_ms) return set_err(CF_EINVAL, "NULL argument");
@@ -851,6 +855,14 @@ int cf_sim_get_info(const cf_sim* s, cf_sim_info* info) {
   info->diff_sub = s->diffSub; info->nlevels = s->nlev;
   for (int l = 0; l < s->nlev; l++) { info->level_nx[l] = s->lnx[l]; info->level_ny[l] = s->lny[l]; }
   info->last_vcycles = s->last_vcycles; info->last_resid = s->last_resid;
+  if (s->sch.cfl > 0 || s->sch.tol > 0) {
+    // device-side control values of the last step
+    cf::Dyn d;
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipMemcpy(&d, s->dDyn, sizeof(d), hipMemcpyDeviceToHost));
+    if (s->sch.cfl > 0) { info->time = d.time; info->dt = d.dt; }
+    if (s->sch.tol > 0) { info->last_vcycles = d.cycles; info->last_resid = d.resid; }
+  }
   return CF_OK;
 }
 

@@ -121,8 +121,12 @@ class FluidSim:
     def write_params(self):  # js/fluid.js:180-199
         check(L.lib().cf_sim_write_params(self._h), "writeParams")
 
-    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0):
-        s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16)
+    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0):
+        """Multigrid schedule (reference: nu 2, 1 V-cycle, 120 coarsest sweeps).
+        tol > 0 runs up to `vcycles` V-cycles until ||r||/||rhs|| < tol; cfl > 0
+        takes dt = cfl*dx/max(max|u|, umin) each step, on the device; refine = 1
+        keeps the level-0 pressure and residual in FP64 (defect correction)."""
+        s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine)
         check(L.lib().cf_sim_set_schedule(self._h, C.byref(s)), "set_schedule")
 
     def upload_pieces(self, data, n):  # js/fluid.js:201-204

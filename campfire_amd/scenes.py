@@ -139,13 +139,18 @@ def air_params(Tamb=300.0, YO=0.233):
 # ---- the benchmark workloads ------------------------------------------------------
 def config_plain_256(seed=1):
     """256x256, h = 5 mm, 300 K air with Y_O = 0.233, three 5 cm logs stacked on
-    the floor at a seeded x, a 1200 K ignition disc of r = 3 cm at the pile."""
+    the floor at a seeded x, a 1200 K ignition disc of r = 3 cm at the pile.
+    Each step takes dt = 0.5 h / max(max|u|, 1 m/s) and runs V-cycles until
+    ||r|| / ||rhs|| < 1e-6 (at most 60), with the level-0 pressure and
+    residual in FP64 (defect correction): an FP32 pressure stalls near 1e-5."""
     nx = ny = 256
     h = 0.005
     pieces, n = stacked_logs(seed, nx * h)
     cx = float(pieces[0] + pieces[20]) / 2
-    return {"nx": nx, "ny": ny, "dx": h, "dt": 0.5 * h / 2.5, "pieces": pieces, "n": n,
-            "params": air_params(), "disc": (cx, 0.05, 0.03, 1200.0)}
+    return {"nx": nx, "ny": ny, "dx": h, "dt": 0.5 * h / 1.0, "pieces": pieces, "n": n,
+            "params": air_params(), "disc": (cx, 0.05, 0.03, 1200.0),
+            "schedule": {"nu": 2, "vcycles": 60, "tol": 1e-6, "coarse_iters": 120, "cfl": 0.5, "umin": 1.0,
+                         "refine": 1}}
 
 
 def config_pile_1024(seed=1):
@@ -163,6 +168,8 @@ def apply(sim, cfg):
     """Load a workload into a FluidSim (or an oracle with the same methods)."""
     for k, v in cfg["params"].items():
         sim.set_param(k, v)
+    if "schedule" in cfg:
+        sim.set_schedule(**cfg["schedule"])
     sim.reset()
     Tamb = cfg["params"]["Tamb"]
     cx, cy, r, Tin = cfg["disc"]

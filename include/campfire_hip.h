@@ -82,6 +82,13 @@ typedef struct cf_schedule {
   int32_t coarse_iters;  /* red-black sweeps on the coarsest level, ref 120;
                             0 = direct FP64 solve of the coarsest level */
   int32_t smoother_fp16; /* 1 = FP16 smoother, FP32 residual + defect correction */
+  float   cfl;           /* > 0: each step takes dt = cfl * dx / max(max|u|, umin),
+                            computed on the device from the current U, V; 0 = fixed dt */
+  float   umin;          /* velocity floor of the CFL rule (m/s) */
+  int32_t refine;        /* 1 = defect correction at level 0: the pressure and the
+                            residual b - A p in FP64, each V-cycle (FP32) solving
+                            for the correction; the projection takes its gradient
+                            from the FP64 pressure.  Needs tol > 0. */
 } cf_schedule;
 
 typedef struct cf_sim_info {

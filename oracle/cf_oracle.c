@@ -103,6 +103,8 @@ typedef struct {
   int32_t acc[CF_ACC_LEN], faceW[CF_MAX_PIECES * 4];
   REAL rad[CF_RAD_LEN * 4];
   REAL *p[MAXLEV], *rhs[MAXLEV], *ax[MAXLEV], *ay[MAXLEV];
+  double* p64;            /* defect correction: level-0 pressure in FP64 */
+  REAL* b0;               /* defect correction: level-0 right-hand side */
   cf_schedule sch;
   int last_vcycles;
   double last_resid;
@@ -726,6 +728,36 @@ static double resid_norm(const S* s) {
   return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
 }
 
+/* defect correction (the build's schedule option refine = 1): r = b - A p64
+ * in FP64 into rhs[0] with p[0] = 0 as the correction's starting guess;
+ * returns ||r|| / ||b|| */
+static double dc_defect(S* s, int first) {
+  const int nx = s->nx, ny = s->ny;
+  const REAL* ax = s->ax[0];
+  const REAL* ay = s->ay[0];
+  const double* p = s->p64;
+  const double h2 = (double)s->lh2[0];
+  double rr = 0, bb = 0;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int k = j * nx + i;
+      if (first) s->b0[k] = s->rhs[0][k];
+      const double b = (double)s->b0[k];
+      const double w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1];
+      const double so = ay[j * nx + i], n = ay[(j + 1) * nx + i];
+      double off = 0, diag = 0;
+      if (i > 0) { off += w * p[k - 1]; diag += w; } else diag += 2.0 * w;
+      if (i < nx - 1) { off += e * p[k + 1]; diag += e; } else diag += 2.0 * e;
+      if (j > 0) { off += so * p[k - nx]; diag += so; }
+      if (j < ny - 1) { off += n * p[k + nx]; diag += n; } else diag += 2.0 * n;
+      const double r = diag <= 0 ? 0.0 : b - (off - diag * p[k]) / h2;
+      s->rhs[0][k] = (REAL)r;
+      s->p[0][k] = 0;
+      rr += r * r; bb += b * b;
+    }
+  return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+}
+
 /* vcycle (js/fluid.js:294-312) */
 static void vcycle(S* s, int nu) {
   const int L = s->nlev;
@@ -744,28 +776,47 @@ static void k_project(S* s) {                                                   
   const int nx = s->nx, ny = s->ny;
   const REAL dx = PP(dx), dt = PP(dt);
   const REAL* p = s->p[0];
+  /* defect correction: differences of the FP64 pressure, rounded after the subtraction */
+  const double* q = (s->sch.tol > 0 && s->sch.refine) ? s->p64 : NULL;
 #pragma omp parallel for schedule(static)
   for (int j = 0; j <= ny; j++)
     for (int i = 0; i <= nx; i++) {
       if (i <= nx && j < ny) {
         int k = idxU(s, i, j);
         if (s->fx[k] > K(0)) {
-          REAL pl = i > 0 ? p[idxC(s, imax(i - 1, 0), j)] : -p[idxC(s, 0, j)];
-          REAL pr = i < nx ? p[idxC(s, imin(i, nx - 1), j)] : -p[idxC(s, nx - 1, j)];
+          REAL dp;
+          if (q) {
+            double pl = i > 0 ? q[idxC(s, imax(i - 1, 0), j)] : -q[idxC(s, 0, j)];
+            double pr = i < nx ? q[idxC(s, imin(i, nx - 1), j)] : -q[idxC(s, nx - 1, j)];
+            dp = (REAL)(pr - pl);
+          } else {
+            REAL pl = i > 0 ? p[idxC(s, imax(i - 1, 0), j)] : -p[idxC(s, 0, j)];
+            REAL pr = i < nx ? p[idxC(s, imin(i, nx - 1), j)] : -p[idxC(s, nx - 1, j)];
+            dp = pr - pl;
+          }
           REAL Tf = K(0.5) * (s->T[idxC(s, iclamp(i - 1, 0, nx - 1), j)] + s->T[idxC(s, imin(i, nx - 1), j)]);
-          s->U[k] -= dt * invRho(s, Tf) * (pr - pl) / dx;
+          s->U[k] -= dt * invRho(s, Tf) * dp / dx;
         } else s->U[k] = K(0);
       }
       if (i < nx && j <= ny) {
         int k = idxV(s, i, j);
         if (j > 0 && s->fy[k] > K(0)) {
-          REAL pb = p[idxC(s, i, j - 1)];
-          REAL pt = j < ny ? p[idxC(s, i, imin(j, ny - 1))] : -p[idxC(s, i, ny - 1)];
+          REAL dp;
+          if (q) {
+            double pb = q[idxC(s, i, j - 1)];
+            double pt = j < ny ? q[idxC(s, i, imin(j, ny - 1))] : -q[idxC(s, i, ny - 1)];
+            dp = (REAL)(pt - pb);
+          } else {
+            REAL pb = p[idxC(s, i, j - 1)];
+            REAL pt = j < ny ? p[idxC(s, i, imin(j, ny - 1))] : -p[idxC(s, i, ny - 1)];
+            dp = pt - pb;
+          }
           REAL Tf = K(0.5) * (s->T[idxC(s, i, j - 1)] + s->T[idxC(s, i, imin(j, ny - 1))]);
-          s->V[k] -= dt * invRho(s, Tf) * (pt - pb) / dx;
+          s->V[k] -= dt * invRho(s, Tf) * dp / dx;
         } else s->V[k] = K(0);
       }
     }
+  if (q) for (int k = 0; k < s->nC; k++) s->p[0][k] = (REAL)q[k];
 }
 
 /* ---------------- shaders/geometry.wgsl ---------------- */
@@ -980,6 +1031,7 @@ void* PFX(create)(int nx, int ny, double dx, double dt, int* err) {             
   s->U = calloc(nU, sizeof(REAL)); s->Uh = calloc(nU, sizeof(REAL)); s->U2 = calloc(nU, sizeof(REAL));
   s->V = calloc(nV, sizeof(REAL)); s->Vh = calloc(nV, sizeof(REAL)); s->V2 = calloc(nV, sizeof(REAL));
   s->T = calloc(nC, sizeof(REAL)); s->Th = calloc(nC, sizeof(REAL)); s->T2 = calloc(nC, sizeof(REAL));
+  s->p64 = calloc(nC, sizeof(double)); s->b0 = calloc(nC, sizeof(REAL));
   s->Y = calloc(4 * nC, sizeof(REAL)); s->Yh = calloc(4 * nC, sizeof(REAL)); s->Y2 = calloc(4 * nC, sizeof(REAL));
   s->SQ = calloc(2 * nC, sizeof(REAL));
   s->sdf = calloc(nC, sizeof(REAL)); s->sid = calloc(nC, sizeof(uint32_t));
@@ -1003,6 +1055,7 @@ void* PFX(create)(int nx, int ny, double dx, double dt, int* err) {             
     1800, 5.670374e-8, 2.667, 0};
   memcpy(s->pv, def, sizeof(def));
   s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
+  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0;
   s->diffSub = 1;
   return s;
 }
@@ -1012,6 +1065,7 @@ void PFX(destroy)(void* h) {
   if (!s) return;
   free(s->U); free(s->Uh); free(s->U2); free(s->V); free(s->Vh); free(s->V2);
   free(s->T); free(s->Th); free(s->T2); free(s->Y); free(s->Yh); free(s->Y2);
+  free(s->p64); free(s->b0);
   free(s->SQ); free(s->sdf); free(s->sid); free(s->fx); free(s->fy);
   for (int l = 0; l < s->nlev; l++) { free(s->p[l]); free(s->rhs[l]); free(s->ax[l]); free(s->ay[l]); }
   free(s);
@@ -1052,7 +1106,9 @@ int PFX(write_params)(void* h) {                                                
   double Tmax = 2200, mu = 1.716e-5 * pow(Tmax / 273.15, 1.5) * 383.55 / (Tmax + 110.4);
   double rhoAmbP = 101325.0 * 0.0289 / (8.314462 * 293.15);
   double alpha = mu / 0.7 * s->pv[11] / (rhoAmbP * 293.15 / Tmax);
-  int n = (int)ceil(s->dt / (s->dx * s->dx / (4.5 * alpha)));
+  /* with a CFL step the count is fixed by the largest dt the rule allows */
+  const double dtmax = s->sch.cfl > 0 ? (double)s->sch.cfl * s->dx / (double)s->sch.umin : s->dt;
+  int n = (int)ceil(dtmax / (s->dx * s->dx / (4.5 * alpha)));
   n = n % 2 ? n : n + 1;
   if (n > 5) n = 5;
   s->diffSub = n;
@@ -1068,7 +1124,12 @@ double PFX(time)(void* h) { return ((S*)h)->time; }
 int PFX(set_schedule)(void* h, const cf_schedule* sc) {
   S* s = (S*)h;
   if (sc->nu < 0 || sc->vcycles < 1 || sc->coarse_iters < 0) return CF_EINVAL;
+  if (!(sc->cfl >= 0) || (sc->cfl > 0 && !(sc->umin > 0))) return CF_EINVAL;
+  if (sc->refine && !(sc->tol > 0)) return CF_EINVAL;
+  if (sc->refine && !s->sch.refine) for (int k = 0; k < s->nC; k++) s->p64[k] = (double)s->p[0][k];
+  const int cfl_changed = sc->cfl != s->sch.cfl || sc->umin != s->sch.umin;
   s->sch = *sc;
+  if (cfl_changed) PFX(write_params)(h);
   return 0;
 }
 int PFX(last_vcycles)(void* h) { return ((S*)h)->last_vcycles; }
@@ -1088,6 +1149,7 @@ int PFX(reset)(void* h) {                                                       
   memset(s->acc, 0, sizeof(s->acc));
   memset(s->rad, 0, sizeof(s->rad));
   for (int l = 0; l < s->nlev; l++) memset(s->p[l], 0, sizeof(REAL) * s->lnx[l] * s->lny[l]);
+  memset(s->p64, 0, sizeof(double) * s->nC);
   s->time = 0;
   return 0;
 }
@@ -1125,6 +1187,18 @@ int PFX(run_stage)(void* h, int stage) {
     case 5: {
       int nv = 0;
       double r = -1;
+      if (s->sch.tol > 0 && s->sch.refine) {
+        r = dc_defect(s, 1);
+        for (int c = 0; c < s->sch.vcycles && !(r < s->sch.tol); c++) {
+          vcycle(s, s->sch.nu);
+          nv++;
+          for (int k = 0; k < s->nC; k++) s->p64[k] += (double)s->p[0][k];
+          r = dc_defect(s, 0);
+        }
+        s->last_vcycles = nv; s->last_resid = r;
+        if (r >= s->sch.tol) return CF_ENOCONV;
+        break;
+      }
       for (int c = 0; c < s->sch.vcycles; c++) {
         vcycle(s, s->sch.nu);
         nv++;
@@ -1164,9 +1238,23 @@ int PFX(run_kernel)(void* h, const char* name, int arg) {
   return 0;
 }
 
+/* CFL time step (the build's schedule option, not in the reference):
+ * dt = cfl * dx / max(max |u| over U and V faces, umin) in the field
+ * precision, written into the parameter block; dsub = dt / diffSub. */
+static void cfl_step(S* s) {
+  REAL m = 0;
+  for (size_t k = 0; k < s->nU; k++) m = FMAX(m, FABS(s->U[k]));
+  for (size_t k = 0; k < s->nV; k++) m = FMAX(m, FABS(s->V[k]));
+  const REAL dt = (REAL)s->sch.cfl * PP(dx) / FMAX(m, (REAL)s->sch.umin);
+  s->P.dt = (float)dt;
+  s->dt = (double)(float)dt;
+  s->dsub = (float)((double)(float)dt / (double)s->diffSub);
+}
+
 int PFX(step)(void* h, int nsteps) {                                              /* js/fluid.js:225-253 */
   S* s = (S*)h;
   for (int q = 0; q < nsteps; q++) {
+    if (s->sch.cfl > 0) cfl_step(s);
     for (int st = 0; st < 7; st++) {
       int rc = PFX(run_stage)(h, st);
       if (rc && rc != CF_ENOCONV) return rc;

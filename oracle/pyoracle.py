@@ -31,7 +31,7 @@ class CfParams(C.Structure):
 
 class CfSchedule(C.Structure):
     _fields_ = [("nu", C.c_int32), ("vcycles", C.c_int32), ("tol", C.c_float),
-                ("coarse_iters", C.c_int32), ("smoother_fp16", C.c_int32)]
+                ("coarse_iters", C.c_int32), ("smoother_fp16", C.c_int32), ("cfl", C.c_float), ("umin", C.c_float), ("refine", C.c_int32)]
 
 
 class FlameResult(C.Structure):
@@ -166,8 +166,17 @@ class Oracle:
     def last_resid(self):
         return self._f("last_resid")(self.h)
 
-    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0):
-        s = CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16)
+    @property
+    def time(self):
+        return self._f("time")(self.h)
+
+    @property
+    def dt_now(self):
+        """dt of the parameter block (the CFL step writes it each step)."""
+        return float(self.packed_params().dt)
+
+    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0):
+        s = CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine)
         self._chk(self._f("set_schedule")(self.h, C.byref(s)), "set_schedule")
 
     def reset(self):

@@ -1,0 +1,120 @@
+"""The 256^2 plain-method workload: CFL time step (dt = 0.5 h / max(|u|, 1 m/s))
+and V-cycles to ||r||/||rhs|| < 1e-6 (FP64 defect correction at level 0), both
+decided on the device, against the FP32 oracle running the same rule; then
+1000 steps on the GPU."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from campfire_amd import scenes
+from helpers import rel_l2
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(seed=1):
+    from campfire_amd.fluid import FluidSim
+    cfg = scenes.config_plain_256(seed)
+    g = FluidSim({"nx": cfg["nx"], "ny": cfg["ny"], "dt": cfg["dt"]}, dx=cfg["dx"])
+    o = Oracle(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dt"])
+    scenes.apply(g, cfg)
+    scenes.apply(o, cfg)
+    g.clear_geometry(); g.encode_geometry()
+    o.clear_geometry(); o.geometry()
+    for name in ("cellG", "fx", "fy", "faceW"):
+        o.write(name, g.read_field(name))
+    return cfg, g, o
+
+
+def test_cfl_and_tolerance_match_the_oracle():
+    _flush("[plain256] CFL + tol parity")
+    cfg, g, o = _pair()
+    assert g.diff_sub == o.diff_sub == 1
+    for s in range(4):
+        g.step(1)
+        o.step(1)
+        gi = g.info()
+        _flush("  step %d: dt GPU %.7g oracle %.7g | cycles %d / %d | resid %.3g / %.3g | t %.6f / %.6f"
+               % (s, gi.dt, o.dt_now, gi.last_vcycles, o.last_vcycles, gi.last_resid, o.last_resid, gi.time, o.time))
+        assert abs(gi.dt - o.dt_now) <= 1e-5 * o.dt_now
+        assert abs(gi.time - o.time) <= 1e-5 * o.time
+        # both converge, or both stop at the cycle cap; near the target the
+        # count can differ by a few cycles
+        assert (gi.last_resid < 1e-6) == (o.last_resid < 1e-6)
+        assert abs(gi.last_vcycles - o.last_vcycles) <= 3
+        for name in ["T", "Y", "U", "V"]:
+            r = rel_l2(g.read_field(name), o.read(name))
+            _flush("    %s rel_l2=%.3g" % (name, r))
+            assert r <= 1e-4, (s, name, r)
+    g.close()
+    o.close()
+
+
+def test_1000_steps_converge_and_stay_finite():
+    from campfire_amd.fluid import FluidSim
+    cfg = scenes.config_plain_256(1)
+    g = FluidSim({"nx": cfg["nx"], "ny": cfg["ny"], "dt": cfg["dt"]}, dx=cfg["dx"])
+    scenes.apply(g, cfg)
+    g.clear_geometry(); g.encode_geometry()
+    _flush("[plain256] 1000 steps")
+    g.step(1)
+    g.synchronize()
+    cycles, resid, dts = [], [], []
+    t0 = time.perf_counter()
+    for s in range(999):
+        g.step(1)
+        if s % 50 == 0:
+            i = g.info()
+            cycles.append(i.last_vcycles); resid.append(i.last_resid); dts.append(i.dt)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    i = g.info()
+    _flush("  %.3f s for 999 steps (%.1f Mcell-steps/s incl. %d info syncs); t = %.4f s"
+           % (wall, 256 * 256 * 999 / wall / 1e6, len(cycles), i.time))
+    _flush("  V-cycles every 50 steps:", cycles)
+    _flush("  relative residual:", ["%.2g" % r for r in resid])
+    _flush("  dt (ms):", ["%.3f" % (d * 1e3) for d in dts])
+    assert all(r < 1e-6 for r in resid[1:]), resid
+    assert all(0 < d <= 0.5 * cfg["dx"] / 1.0 + 1e-12 for d in dts)
+    for name in ["T", "Y", "U", "V", "p"]:
+        a = g.read_field(name)
+        assert np.all(np.isfinite(a)), name
+    T = g.read_field("T")
+    _flush("  T range %.1f .. %.1f K" % (T.min(), T.max()))
+    assert T.max() > 600  # the fire is burning
+    g.close()
+
+
+def test_short_horizon_heat_release_and_centerline_temperature():
+    """100 steps: the heat release read back every 10 steps and the final
+    centerline temperature profile, GPU against the FP32 oracle."""
+    _flush("[plain256] 100-step statistics")
+    cfg, g, o = _pair(2)
+    hg, ho = [], []
+    for f in range(10):
+        g.step(10)
+        o.step(10)
+        ag = g.readback_now()["acc"]
+        ao = o.readback()[0]
+        hg.append(int(ag[512])); ho.append(int(ao[512]))
+    _flush("  HRR sums GPU  ", hg)
+    _flush("  HRR sums oracle", ho)
+    hg, ho = np.array(hg, float), np.array(ho, float)
+    assert np.abs(hg - ho).sum() <= 0.02 * np.abs(ho).sum()
+    nx = cfg["nx"]
+    cx = int(float(cfg["pieces"][0] + cfg["pieces"][20]) / 2 / cfg["dx"])
+    Tg = g.read_field("T").reshape(-1, nx)[:, cx]
+    To = o.read("T").reshape(-1, nx)[:, cx]
+    r = rel_l2(Tg, To)
+    _flush("  centerline T rel_l2 = %.3g (max %.1f K / %.1f K)" % (r, Tg.max(), To.max()))
+    assert r <= 1e-2
+    g.close()
+    o.close()
