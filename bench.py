@@ -196,6 +196,60 @@ def measure_traffic(args):
     return (per, cal), None
 
 
+def batched(args):
+    """The mixed-size batch: args.batch scenes (sizes drawn from 256^2, 512^2,
+    1024^2) stepped together by one ScenePool, against the same scenes stepped
+    one after another with their own step graphs (the per-scene form).  Both
+    give bitwise the same fields."""
+    from campfire_amd import scenes
+    from campfire_amd.pool import ScenePool
+    cfgs = scenes.config_batch(args.seed, args.batch)
+    pool = ScenePool(200 << 30)
+    sims = []
+    for c in cfgs:
+        slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
+        scenes.apply(sim, c)
+        sims.append(sim)
+    pool.frame(8, True)          # geometry and a first burning frame
+    pool.step(args.warmup)
+    pool.synchronize()
+    info = pool.info()
+    cells = int(info.cells)
+    K = args.batch_steps
+    log("[bench] batch: %d scenes, %.1f M cells, %.2f GB, %d launches per step"
+        % (len(cfgs), cells / 1e6, info.used / 1e9, info.launches_per_step))
+    t0 = time.perf_counter()
+    pool.step(K)
+    pool.synchronize()
+    tb = time.perf_counter() - t0
+    lat = pool.step_latencies(K)
+    # per-scene form: each scene alone, one after another
+    for sim in sims:
+        sim.step(2)              # capture its own graph
+    pool.synchronize()
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.step(K)
+    pool.synchronize()
+    ts = time.perf_counter() - t0
+    pool.close()
+    n = len(cfgs)
+    res = {"scenes": n, "sizes": {str(s): sum(1 for c in cfgs if c["nx"] == s) for s in (256, 512, 1024)},
+           "cells": cells, "hbm_bytes": int(info.used), "steps": K,
+           "batched_mcell_steps_per_s": round(cells * K / tb / 1e6, 1),
+           "batched_scene_steps_per_s": round(n * K / tb, 1),
+           "batched_ms_per_step": round(tb * 1e3 / K, 4),
+           "per_scene_mcell_steps_per_s": round(cells * K / ts / 1e6, 1),
+           "speedup_vs_per_scene": round(ts / tb, 3),
+           "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                               "p99": round(float(np.percentile(lat, 99)), 4)},
+           "launches_per_step": int(info.launches_per_step)}
+    log("[bench] batch: %.1f Mcell-steps/s batched, %.1f per scene (x%.2f); %.1f scene-steps/s; p99 %.3f ms"
+        % (res["batched_mcell_steps_per_s"], res["per_scene_mcell_steps_per_s"], res["speedup_vs_per_scene"],
+           res["batched_scene_steps_per_s"], res["step_latency_ms"]["p99"]))
+    return res
+
+
 def bench_config(args):
     from campfire_amd import scenes
     cfg = scenes.config_pile_1024(args.seed)
@@ -239,6 +293,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
+    ap.add_argument("--batch", type=int, default=64, help="scenes of the batched measurement (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -347,6 +403,8 @@ def main():
         out["step_latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 5),
                                   "p99": round(float(np.percentile(lat, 99)), 5)}
     sim.close()
+    if args.batch > 0:
+        out["batched"] = batched(args)
     if not args.no_cpu:
         log("[bench] CPU baseline (FP32 oracle)")
         out["cpu_baseline"] = cpu_baseline(cfg)

@@ -57,6 +57,15 @@ class CfFlameResult(C.Structure):
     _fields_ = [("SL", C.c_double), ("Tb", C.c_double), ("steps", C.c_int32), ("n", C.c_int32)]
 
 
+class CfSceneDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("dx", C.c_double), ("dt", C.c_double)]
+
+
+class CfPoolInfo(C.Structure):
+    _fields_ = [("nscenes", C.c_int32), ("nslots", C.c_int32), ("budget", C.c_uint64), ("used", C.c_uint64),
+                ("cells", C.c_uint64), ("steps", C.c_int64), ("launches_per_step", C.c_int32), ("pad_", C.c_int32)]
+
+
 class CampfireError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%s)" % (msg, ERR_NAMES.get(code, code)))
@@ -105,7 +114,20 @@ def lib():
         "cf_sim_write_field": ([p, cp, vp, sz], i),
         "cf_sim_get_info": ([p, C.POINTER(CfSimInfo)], i),
         "cf_sim_div_error": ([p, C.POINTER(f), C.POINTER(f)], i),
+        "cf_sim_compat_sums": ([p, C.POINTER(d), C.POINTER(d)], i),
         "cf_flame1d_run": ([C.POINTER(CfFlameCase), i, i, C.POINTER(CfFlameResult)], i),
+        "cf_scene_bytes": ([i, i, C.c_double], C.c_uint64),
+        "cf_pool_create": ([C.c_uint64, C.POINTER(p)], i),
+        "cf_pool_destroy": ([p], None),
+        "cf_pool_set_schedule": ([p, C.POINTER(CfSchedule)], i),
+        "cf_pool_admit": ([p, C.POINTER(CfSceneDesc), C.POINTER(i)], i),
+        "cf_pool_scene": ([p, i, C.POINTER(p)], i),
+        "cf_pool_release": ([p, i], i),
+        "cf_pool_step": ([p, i], i),
+        "cf_pool_frame": ([p, i, i], i),
+        "cf_pool_synchronize": ([p], i),
+        "cf_pool_get_info": ([p, C.POINTER(CfPoolInfo)], i),
+        "cf_pool_step_latencies": ([p, i, vp], i),
         "cf_last_error": ([], cp),
         "cf_version": ([], cp),
     }

@@ -352,9 +352,8 @@ __global__ __launch_bounds__(256) void k_advect(Fields F) { AdvectOp::run(F, xcd
 void launch_advect(const Fields& F, hipStream_t s) {
   launch(k_advect, dim3(AdvectOp::tiles(F.nx, F.ny)), dim3(256), 0, s, F);
 }
-int advect_tiles(int nx, int ny) { return AdvectOp::tiles(nx, ny); }
-void launch_advect_batch(const Fields* args, const Item* items, int n, hipStream_t s) {
-  if (n > 0) launch(k_batch<AdvectOp>, dim3(n), dim3(256), 0, s, args, items, n);
+void phase_advect(BatchPhase& ph, const Fields& F) {
+  phase_add<AdvectOp>(ph, F, AdvectOp::tiles(F.nx, F.ny), dim3(256), "advect");
 }
 
 }  // namespace cf

@@ -1,6 +1,7 @@
 // cf_kernels.h — device buffers of one scene and the kernel launchers.
 #pragma once
 #include <hip/hip_ext.h>
+#include <string.h>
 #include <vector>
 #include "cf_device.h"
 
@@ -31,6 +32,67 @@ __global__ __launch_bounds__(Op::kThreads) void k_batch(const typename Op::Args*
                                                        const Item* __restrict__ items, int n) {
   const Item it = items[xcd_tile(blockIdx.x, n)];
   Op::run(args[it.arg], it.tile);
+}
+
+template <typename... KArgs, typename... Args>
+inline void launch(void (*k)(KArgs...), dim3 grid, dim3 block, unsigned shmem, hipStream_t s, Args... args);
+
+// One batched launch: every scene's argument block and its work items,
+// collected on the host (add) and copied to the device once (upload); run()
+// then replays the launch, also under graph capture.
+struct BatchPhase {
+  void (*fn)(const BatchPhase&, hipStream_t) = nullptr;
+  dim3 block;
+  size_t asize = 0;
+  int nargs = 0;
+  std::vector<unsigned char> hargs;
+  std::vector<Item> hitems;
+  void* dargs = nullptr;
+  Item* ditems = nullptr;
+  const char* tag = "";
+
+  template <class A>
+  void add(const A& a, int ntiles) {
+    asize = sizeof(A);
+    const size_t off = hargs.size();
+    hargs.resize(off + sizeof(A));
+    memcpy(&hargs[off], &a, sizeof(A));
+    for (int t = 0; t < ntiles; t++) hitems.push_back(Item{nargs, t});
+    nargs++;
+  }
+  bool empty() const { return hitems.empty(); }
+  hipError_t upload(hipStream_t s) {
+    if (empty()) return hipSuccess;
+    hipError_t e = hipMalloc(&dargs, hargs.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&ditems, hitems.size() * sizeof(Item));
+    if (e == hipSuccess) e = hipMemcpyAsync(dargs, hargs.data(), hargs.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ditems, hitems.data(), hitems.size() * sizeof(Item), hipMemcpyHostToDevice, s);
+    return e;
+  }
+  void release() {
+    if (dargs) hipFree(dargs);
+    if (ditems) hipFree(ditems);
+    dargs = nullptr;
+    ditems = nullptr;
+  }
+  void run(hipStream_t s) const {
+    if (fn && !empty()) fn(*this, s);
+  }
+};
+
+template <class Op>
+void run_batch(const BatchPhase& ph, hipStream_t s) {
+  launch(k_batch<Op>, dim3((unsigned)ph.hitems.size()), ph.block, 0, s, (const typename Op::Args*)ph.dargs, ph.ditems,
+         (int)ph.hitems.size());
+}
+
+// start a phase of Op (or check that it is one) and add one scene
+template <class Op>
+void phase_add(BatchPhase& ph, const typename Op::Args& a, int ntiles, dim3 block, const char* tag) {
+  ph.fn = &run_batch<Op>;
+  ph.block = block;
+  ph.tag = tag;
+  ph.add(a, ntiles);
 }
 
 template <typename... KArgs, typename... Args>
@@ -110,8 +172,10 @@ bool launch_mg_up(const Level& f, const Level& c, int nu, const int* skip, hipSt
 // first level of the single-workgroup bottom solver (the last 1-3 levels)
 int mg_bottom_first(const Level* L, int nlev);
 // coarse_coef: restrict the coarsest level's coefficients inside the kernel
+// coarse_iters = 0: direct FP64 solve of the coarsest level (<= 32 cells wide)
+// in `band` (scratch of (256 * 33 + 256) doubles)
 bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, const int* skip,
-                      hipStream_t s);
+                      double* band, hipStream_t s);
 // step control (cf_control.hip)
 // dt = cfl * dx / max(max|u|, umin) into the parameter block and dyn, dsub =
 // dt / nsub, time += dt
@@ -139,6 +203,30 @@ void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s);
 // tests/flame1d.mjs on the GPU (cf_flame1d.hip): one workgroup per case
 void launch_flame1d(const cf_flame_case* cases, int ncases, bool fp64, cf_flame_result* out, hipStream_t s);
 unsigned div_error_blocks(const Fields& F);
+// FP64 net source and open-boundary outflow: 2 doubles per workgroup
+unsigned compat_blocks(const Fields& F);
+void launch_compat(const Fields& F, double* part, hipStream_t s);
 void launch_div_error(const Fields& F, float* part, hipStream_t s);
+
+// ---- batched phases: add one scene's launch of each kernel to a phase ----
+// (the arguments mean what they mean for the single-scene launchers above)
+void phase_advect(BatchPhase& ph, const Fields& F);
+void phase_diffuse(BatchPhase& ph, const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout);
+void phase_sources(BatchPhase& ph, const Fields& F, const float* Tin, const float4* Yin, int* part);
+void phase_forces_coef(BatchPhase& ph, const Fields& F, const Level* L, int nlev, const int* part, int nparts);
+void phase_coef_restrict(BatchPhase& ph, const Level& f, const Level& c);
+void phase_cfl(BatchPhase& ph, const Fields& F, float cfl, float umin, int nsub);
+void phase_resnorm(BatchPhase& ph, const Level& L0, Dyn* dyn, double* part, float tol);
+void phase_dc_defect(BatchPhase& ph, const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
+                     bool first);
+void phase_dc_accum(BatchPhase& ph, const Level& L0, double* p64, const int* skip);
+void phase_project(BatchPhase& ph, const Fields& F, const double* p64);
+void phase_geometry(BatchPhase& ph, const Fields& F);
+void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets);
+void phase_fill_u32(BatchPhase& ph, void* a, uint32_t v, size_t n);
+bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip);
+bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip);
+bool phase_mg_bottom(BatchPhase& ph, const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters,
+                     const int* skip, double* band);
 
 }  // namespace cf

@@ -15,6 +15,14 @@ namespace cf {
 // -------------------------------------------------------------------------
 // tile decode: TX x TY threads per workgroup over an (ex x ey) index domain
 // -------------------------------------------------------------------------
+// (i, j) of this thread in tile t of a TX x TY tiling of an ex-wide domain
+template <int TX, int TY>
+__device__ __forceinline__ bool tile_ij(int t, int ex, int ey, int& i, int& j) {
+  const int gx = (ex + TX - 1) / TX;
+  i = (t % gx) * TX + threadIdx.x;
+  j = (t / gx) * TY + threadIdx.y;
+  return true;
+}
 template <int TX, int TY>
 __device__ __forceinline__ bool tile_ij(int ex, int ey, int& i, int& j) {
   const int gx = (ex + TX - 1) / TX, gy = (ey + TY - 1) / TY;
@@ -139,19 +147,39 @@ __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny,
                    fmaxf(Y0.z + c * sY.z * K(1), K(0)), fmaxf(Y0.w + c * sY.w * K(0), K(0)));
 }
 
-__global__ __launch_bounds__(256) void k_diffuse(Fields F, const float* __restrict__ Tin, const float4* __restrict__ Yin,
-                                                 float* __restrict__ Tout, float4* __restrict__ Yout) {
+struct DiffuseOp {
+  struct Args {
+    Fields F;
+    const float* Tin;
+    const float4* Yin;
+    float* Tout;
+    float4* Yout;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+    const float* __restrict__ Tin = a_.Tin;
+    const float4* __restrict__ Yin = a_.Yin;
+    float* __restrict__ Tout = a_.Tout;
+    float4* __restrict__ Yout = a_.Yout;
+
   const cf_params P = *F.P;
   const float dts = F.dyn->dsub;
   const int nx = P.nx, ny = P.ny;
   int i, j;
-  tile_ij<32, 8>(nx, ny, i, j);
+  tile_ij<32, 8>(vb, nx, ny, i, j);
   if (i >= nx || j >= ny) return;
   float To;
   float4 Yo;
   diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, dts, i, j, To, Yo);
   Tout[j * nx + i] = To;
   Yout[j * nx + i] = Yo;
+
+  }
+};
+__global__ __launch_bounds__(256) void k_diffuse(const DiffuseOp::Args a_) {
+  DiffuseOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // -------------------------------------------------------------------------
@@ -288,8 +316,21 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
 // global sums as per-workgroup partials folded by k_acc_finish.  Same-address
 // atomics from every 8x8 tile would otherwise serialise on one L2 line.
 template <bool FUSE>
-__global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restrict__ Tin,
-                                                 const float4* __restrict__ Yin, int* __restrict__ part) {
+struct SourcesOp {
+  struct Args {
+    Fields F;
+    const float* Tin;
+    const float4* Yin;
+    int* part;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+    const float* __restrict__ Tin = a_.Tin;
+    const float4* __restrict__ Yin = a_.Yin;
+    int* __restrict__ part = a_.part;
+
   __shared__ float red[3][8][33];
   __shared__ int sacc[CF_ACC_GLOBAL];
   __shared__ int sglob[4][3];
@@ -299,7 +340,7 @@ __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restri
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
   int i, j;
-  tile_ij<32, 8>(nx, ny, i, j);
+  tile_ij<32, 8>(vb, nx, ny, i, j);
   float qrel = K(0), qrad = K(0), tpeak = K(0);
   if (i < nx && j < ny) {
     const int k = j * nx + i;
@@ -340,10 +381,16 @@ __global__ __launch_bounds__(256) void k_sources(Fields F, const float* __restri
     const unsigned sq_ = (unsigned)sglob[0][0] + (unsigned)sglob[1][0] + (unsigned)sglob[2][0] + (unsigned)sglob[3][0];
     const unsigned sr_ = (unsigned)sglob[0][1] + (unsigned)sglob[1][1] + (unsigned)sglob[2][1] + (unsigned)sglob[3][1];
     const int mt = max(max(sglob[0][2], sglob[1][2]), max(sglob[2][2], sglob[3][2]));
-    part[4 * blockIdx.x + 0] = (int)sq_;
-    part[4 * blockIdx.x + 1] = (int)sr_;
-    part[4 * blockIdx.x + 2] = mt;
+    part[4 * vb + 0] = (int)sq_;
+    part[4 * vb + 1] = (int)sr_;
+    part[4 * vb + 2] = mt;
   }
+
+  }
+};
+template <bool FUSE>
+__global__ __launch_bounds__(256) void k_sources(const typename SourcesOp<FUSE>::Args a_) {
+  SourcesOp<FUSE>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Folds the per-workgroup partials of k_sources into acc[512..514]: integer
@@ -503,7 +550,21 @@ struct TileAcc {
   __device__ float FY(int i, int j) const { return fy[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
 };
 
-__global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL, const int* __restrict__ part, int nparts) {
+struct ForcesCoefOp {
+  struct Args {
+    Fields F;
+    CoefLevels CL;
+    const int* part;
+    int nparts;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+    const CoefLevels& CL = a_.CL;
+    const int* __restrict__ part = a_.part;
+    int nparts = a_.nparts;
+
   constexpr int TX = FC_T, TY = FC_T;
   __shared__ float sT[FC_R * FC_R];
   __shared__ float sU[FC_R * FC_UX], sfx[FC_R * FC_UX];
@@ -512,10 +573,10 @@ __global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL, co
   __shared__ float sax[TY][TX + 1], say[TY + 1][TX];
   const cf_params P = *F.P;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;  // 32 x 8 threads
-  if (blockIdx.x == 0 && nparts > 0) acc_fold(part, nparts, F.acc, (int*)&su[0][0]);
+  if (vb == 0 && nparts > 0) acc_fold(part, nparts, F.acc, (int*)&su[0][0]);
   const int nx = P.nx, ny = P.ny;
   const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
-  const int t = xcd_tile(blockIdx.x, gx * gy);
+  const int t = vb;
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
   // ---- stage inputs (cells at clamped indices; faces outside the domain read 0, never used)
   for (int q = tid; q < FC_R * FC_R; q += 256) {
@@ -606,6 +667,11 @@ __global__ __launch_bounds__(256) void k_forces_coef(Fields F, CoefLevels CL, co
     }
     tx_l = cwx; ty_l = cwy;
   }
+
+  }
+};
+__global__ __launch_bounds__(256) void k_forces_coef(const ForcesCoefOp::Args a_) {
+  ForcesCoefOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 __global__ __launch_bounds__(256) void k_coef_rhs(Fields F) {
@@ -635,12 +701,27 @@ __global__ __launch_bounds__(256) void k_coef_rhs(Fields F) {
 }
 
 // coefficient restriction to the next coarser level (:57-67)
-__global__ __launch_bounds__(256) void k_coef_restrict(Level f, Level c) {
+struct CoefRestrictOp {
+  struct Args {
+    Level f;
+    Level c;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& f = a_.f;
+    const Level& c = a_.c;
+
   int I, J;
-  tile_ij<32, 8>(c.nx + 1, c.ny + 1, I, J);
+  tile_ij<32, 8>(vb, c.nx + 1, c.ny + 1, I, J);
   const int nx = f.nx, nxc = c.nx, nyc = c.ny;
   if (I <= nxc && J < nyc) c.ax[J * (nxc + 1) + I] = K(0.5) * (f.ax[(2 * J) * (nx + 1) + 2 * I] + f.ax[(2 * J + 1) * (nx + 1) + 2 * I]);
   if (I < nxc && J <= nyc) c.ay[J * nxc + I] = K(0.5) * (f.ay[(2 * J) * nx + 2 * I] + f.ay[(2 * J) * nx + 2 * I + 1]);
+
+  }
+};
+__global__ __launch_bounds__(256) void k_coef_restrict(const CoefRestrictOp::Args a_) {
+  CoefRestrictOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // 5-point stencil: sum a*p_nb and the diagonal (:77-85)
@@ -693,7 +774,21 @@ __global__ __launch_bounds__(256) void k_restrict_res(Level f, Level c) {
 // CFL time step: dt = cfl * dx / max(max |u|, umin) over every U and V face.
 // Max is order independent: per-workgroup maxima meet in one atomicMax on
 // the float bits (non-negative floats order like their bit patterns).
-__global__ __launch_bounds__(256) void k_cfl(Fields F, float cfl, float umin, int nsub) {
+struct CflOp {
+  struct Args {
+    Fields F;
+    float cfl;
+    float umin;
+    int nsub;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+    float cfl = a_.cfl;
+    float umin = a_.umin;
+    int nsub = a_.nsub;
+
   __shared__ float wm[4];
   __shared__ int last;
   const cf_params P = *F.P;
@@ -701,7 +796,7 @@ __global__ __launch_bounds__(256) void k_cfl(Fields F, float cfl, float umin, in
   const int nU = (nx + 1) * ny, nV = nx * (ny + 1), n = nU > nV ? nU : nV;
   const int tid = threadIdx.x;
   float m = K(0);
-  for (int k = blockIdx.x * 256 + tid; k < n; k += gridDim.x * 256) {
+  for (int k = vb * 256 + tid; k < n; k += a_.vg * 256) {
     if (k < nU) m = fmaxf(m, fabsf(F.U[k]));
     if (k < nV) m = fmaxf(m, fabsf(F.V[k]));
   }
@@ -712,7 +807,7 @@ __global__ __launch_bounds__(256) void k_cfl(Fields F, float cfl, float umin, in
     const float b = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
     atomicMax(&F.dyn->umax_bits, __float_as_uint(b));
     __threadfence();
-    last = atomicAdd(&F.dyn->cfl_done, 1u) == gridDim.x - 1;
+    last = atomicAdd(&F.dyn->cfl_done, 1u) == a_.vg - 1;
   }
   __syncthreads();
   if (last && tid == 0) {
@@ -727,19 +822,38 @@ __global__ __launch_bounds__(256) void k_cfl(Fields F, float cfl, float umin, in
     atomicExch(&d->umax_bits, 0u);
     atomicExch(&d->cfl_done, 0u);
   }
+
+  }
+};
+__global__ __launch_bounds__(256) void k_cfl(const CflOp::Args a_) {
+  CflOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ||r|| / ||rhs|| over level 0 (sums of squares in FP64), after a V-cycle:
 // per-workgroup partials, then the last workgroup adds them in a fixed order,
 // so the value does not depend on scheduling.  Below tol it sets
 // dyn->converged, and the remaining V-cycles of the step are skipped.
-__global__ __launch_bounds__(256) void k_resnorm(Level L, Dyn* d, double* __restrict__ part, float tol) {
+struct ResnormOp {
+  struct Args {
+    Level L;
+    Dyn* d;
+    double* part;
+    float tol;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& L = a_.L;
+    Dyn* d = a_.d;
+    double* __restrict__ part = a_.part;
+    float tol = a_.tol;
+
   __shared__ double wr[4], wb[4];
   __shared__ int last;
   if (d->converged) return;
   const int tid = threadIdx.x, n = L.nx * L.ny;
   double rr = 0, bb = 0;
-  for (int k = blockIdx.x * 256 + tid; k < n; k += gridDim.x * 256) {
+  for (int k = vb * 256 + tid; k < n; k += a_.vg * 256) {
     const float r = residual(L, k % L.nx, k / L.nx);
     const double b = (double)L.rhs[k];
     rr += (double)r * (double)r;
@@ -753,16 +867,16 @@ __global__ __launch_bounds__(256) void k_resnorm(Level L, Dyn* d, double* __rest
   if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
   __syncthreads();
   if (tid == 0) {
-    part[2 * blockIdx.x] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
-    part[2 * blockIdx.x + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
+    part[2 * vb] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
+    part[2 * vb + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
     __threadfence();
-    last = atomicAdd(&d->rn_done, 1u) == gridDim.x - 1;
+    last = atomicAdd(&d->rn_done, 1u) == a_.vg - 1;
   }
   __syncthreads();
   if (!last) return;
   __threadfence();
   rr = 0; bb = 0;
-  for (int b = tid; b < (int)gridDim.x; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
+  for (int b = tid; b < (int)a_.vg; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     rr += __shfl_down(rr, o, 64);
@@ -780,6 +894,11 @@ __global__ __launch_bounds__(256) void k_resnorm(Level L, Dyn* d, double* __rest
     if (r < (double)tol) d->converged = 1;
     atomicExch(&d->rn_done, 0u);
   }
+
+  }
+};
+__global__ __launch_bounds__(256) void k_resnorm(const ResnormOp::Args a_) {
+  ResnormOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---- defect correction at level 0 (schedule refine = 1) ----------------------
@@ -806,14 +925,33 @@ __device__ __forceinline__ double residual64(const Level& L, const double* __res
 // its starting guess; ||r|| / ||b|| into dyn (FP64 sums, fixed-order fold by
 // the last workgroup), converged when below tol.  first: b is the level's
 // rhs as the forces pass wrote it, saved to b0 for the later iterations.
-__global__ __launch_bounds__(256) void k_dc_defect(Level L, double* __restrict__ p64, float* __restrict__ b0, Dyn* d,
-                                                   double* __restrict__ part, float tol, int first) {
+struct DcDefectOp {
+  struct Args {
+    Level L;
+    double* p64;
+    float* b0;
+    Dyn* d;
+    double* part;
+    float tol;
+    int first;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& L = a_.L;
+    double* __restrict__ p64 = a_.p64;
+    float* __restrict__ b0 = a_.b0;
+    Dyn* d = a_.d;
+    double* __restrict__ part = a_.part;
+    float tol = a_.tol;
+    int first = a_.first;
+
   __shared__ double wr[4], wb[4];
   __shared__ int last;
   if (d->converged) return;
   const int tid = threadIdx.x, n = L.nx * L.ny;
   double rr = 0, bb = 0;
-  for (int k = blockIdx.x * 256 + tid; k < n; k += gridDim.x * 256) {
+  for (int k = vb * 256 + tid; k < n; k += a_.vg * 256) {
     float bf;
     if (first) { bf = L.rhs[k]; b0[k] = bf; } else bf = b0[k];
     const double b = (double)bf;
@@ -831,16 +969,16 @@ __global__ __launch_bounds__(256) void k_dc_defect(Level L, double* __restrict__
   if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
   __syncthreads();
   if (tid == 0) {
-    part[2 * blockIdx.x] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
-    part[2 * blockIdx.x + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
+    part[2 * vb] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
+    part[2 * vb + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
     __threadfence();
-    last = atomicAdd(&d->rn_done, 1u) == gridDim.x - 1;
+    last = atomicAdd(&d->rn_done, 1u) == a_.vg - 1;
   }
   __syncthreads();
   if (!last) return;
   __threadfence();
   rr = 0; bb = 0;
-  for (int b = tid; b < (int)gridDim.x; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
+  for (int b = tid; b < (int)a_.vg; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     rr += __shfl_down(rr, o, 64);
@@ -858,13 +996,35 @@ __global__ __launch_bounds__(256) void k_dc_defect(Level L, double* __restrict__
     if (r < (double)tol) d->converged = 1;
     atomicExch(&d->rn_done, 0u);
   }
+
+  }
+};
+__global__ __launch_bounds__(256) void k_dc_defect(const DcDefectOp::Args a_) {
+  DcDefectOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // p64 += e after a correction V-cycle
-__global__ __launch_bounds__(256) void k_dc_accum(Level L, double* __restrict__ p64, const int* __restrict__ skip) {
+struct DcAccumOp {
+  struct Args {
+    Level L;
+    double* p64;
+    const int* skip;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& L = a_.L;
+    double* __restrict__ p64 = a_.p64;
+    const int* __restrict__ skip = a_.skip;
+
   if (*skip) return;
   const int n = L.nx * L.ny;
-  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) p64[k] += (double)L.p[k];
+  for (int k = vb * 256 + threadIdx.x; k < n; k += a_.vg * 256) p64[k] += (double)L.p[k];
+
+  }
+};
+__global__ __launch_bounds__(256) void k_dc_accum(const DcAccumOp::Args a_) {
+  DcAccumOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 __global__ __launch_bounds__(256) void k_p32_to_64(const float* __restrict__ p, double* __restrict__ p64, int n) {
@@ -932,11 +1092,21 @@ __global__ __launch_bounds__(256) void k_coarsest(Level L, int iters) {
 // correction (rounded to FP32 after the subtraction), and the cells' FP32 p
 // is refreshed from it
 template <bool P64>
-__global__ __launch_bounds__(256) void k_project(Fields F, const double* __restrict__ p64) {
+struct ProjectOp {
+  struct Args {
+    Fields F;
+    const double* p64;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+    const double* __restrict__ p64 = a_.p64;
+
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
   int i, j;
-  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  tile_ij<32, 8>(vb, nx + 1, ny + 1, i, j);
   const float dt = P.dt, dx = P.dx;
   const float* __restrict__ p = F.p0;
   const float* __restrict__ T = F.T;
@@ -975,6 +1145,12 @@ __global__ __launch_bounds__(256) void k_project(Fields F, const double* __restr
     } else F.V[k] = K(0);
   }
   if (P64 && i < nx && j < ny) F.p0[j * nx + i] = (float)p64[j * nx + i];
+
+  }
+};
+template <bool P64>
+__global__ __launch_bounds__(256) void k_project(const typename ProjectOp<P64>::Args a_) {
+  ProjectOp<P64>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // -------------------------------------------------------------------------
@@ -1004,11 +1180,19 @@ __device__ __forceinline__ float nearest(const cf_params& P, const float* __rest
   }
   return best;
 }
-__global__ __launch_bounds__(256) void k_geometry(Fields F) {
+struct GeometryOp {
+  struct Args {
+    Fields F;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
   int i, j;
-  tile_ij<32, 8>(nx + 1, ny + 1, i, j);
+  tile_ij<32, 8>(vb, nx + 1, ny + 1, i, j);
   const float dx = P.dx;
   uint32_t id;
   if (i < nx && j < ny) {
@@ -1027,6 +1211,11 @@ __global__ __launch_bounds__(256) void k_geometry(Fields F) {
     const float d = nearest(P, F.pieces, ((float)i + K(0.5)) * dx, (float)j * dx, id);
     F.fy[j * nx + i] = j == 0 ? K(0) : clampf(K(0.5) + d / dx, K(0), K(1));
   }
+
+  }
+};
+__global__ __launch_bounds__(256) void k_geometry(const GeometryOp::Args a_) {
+  GeometryOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // -------------------------------------------------------------------------
@@ -1071,9 +1260,17 @@ __device__ __forceinline__ void gasAt(const cf_params& P, const float* __restric
   g0 = kap * P.sigma * (T2 * T2 - powf(P.Tamb, K(4.0))) / CF_PI;
   g1 = K(6.3) * fv / K(580e-9) * planck580(Tg);
 }
-__global__ __launch_bounds__(64) void k_radiation(Fields F) {
+struct RadiationOp {
+  struct Args {
+    Fields F;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 64;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+
   const cf_params P = *F.P;
-  const int tgt = blockIdx.x;
+  const int tgt = vb;
   const int li = threadIdx.x;
   const int nFaces = min(P.nPieces, CF_MAX_PIECES) * 4;
   float ox = 0, oy = 0, nrx = 0, nry = 1;
@@ -1161,15 +1358,38 @@ __global__ __launch_bounds__(64) void k_radiation(Fields F) {
     const float a = old.x <= K(0) ? K(1.0) : K(0.35);
     F.rad[tgt] = make_float4(mixf(old.x, dq, a), mixf(old.y, dv, a), K(0), K(0));
   }
+
+  }
+};
+__global__ __launch_bounds__(64) void k_radiation(const RadiationOp::Args a_) {
+  RadiationOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // -------------------------------------------------------------------------
 // utilities
 // -------------------------------------------------------------------------
-__global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
+struct FillOp {
+  struct Args {
+    uint32_t* a;
+    uint32_t v;
+    size_t n;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    uint32_t* __restrict__ a = a_.a;
+    uint32_t v = a_.v;
+    size_t n = a_.n;
+
+  size_t q = (size_t)vb * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)a_.vg * blockDim.x;
   for (; q < n; q += stride) a[q] = v;
+
+  }
+};
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
+  const FillOp::Args a_{a, v, n, (int)gridDim.x};
+  FillOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // |div(f u) - S| after projection (the H.divErr diagnostic, dev/gas_test.html:101-111):
@@ -1196,6 +1416,48 @@ __global__ __launch_bounds__(256) void k_div_error(Fields F, float* __restrict__
   if (t == 0) { part[2 * blockIdx.x] = smax[0]; part[2 * blockIdx.x + 1] = ssum[0]; }
 }
 
+// Divergence compatibility in FP64: the net source of the open cells
+// (sum S dx^2) and the net outflow through the open domain faces
+// (dx sum f u.n: -U at i = 0, +U at i = nx, -V at j = 0, +V at j = ny).  After
+// a converged projection div(f u) = S in every open cell, so the two agree.
+__global__ __launch_bounds__(256) void k_compat(Fields F, double* __restrict__ part) {
+  __shared__ double ws[4], wo[4];
+  const cf_params P = *F.P;
+  const int nx = P.nx, ny = P.ny;
+  const double dx = (double)P.dx;
+  double src = 0, out = 0;
+  const int n = (nx + 1) * (ny + 1);
+  for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+    const int i = q % (nx + 1), j = q / (nx + 1);
+    if (i < nx && j < ny) {
+      const int iu = j * (nx + 1) + i, iv = j * nx + i;
+      const float open = F.fx[iu] + F.fx[iu + 1] + F.fy[iv] + F.fy[iv + nx];
+      if (open > K(0)) src += (double)F.SQ[iv].x * dx * dx;
+    }
+    if (j < ny && (i == 0 || i == nx)) {
+      const int iu = j * (nx + 1) + i;
+      const double f = (double)F.fx[iu] * (double)F.U[iu] * dx;
+      out += i == 0 ? -f : f;
+    }
+    if (i < nx && (j == 0 || j == ny)) {
+      const int iv = j * nx + i;
+      const double f = (double)F.fy[iv] * (double)F.V[iv] * dx;
+      out += j == 0 ? -f : f;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    src += __shfl_down(src, o, 64);
+    out += __shfl_down(out, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { ws[threadIdx.x >> 6] = src; wo[threadIdx.x >> 6] = out; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    part[2 * blockIdx.x + 1] = ((wo[0] + wo[1]) + wo[2]) + wo[3];
+  }
+}
+
 // ---- launchers ----------------------------------------------------------
 static inline unsigned nblk(int ex, int ey) { return (unsigned)(((ex + 31) / 32) * ((ey + 7) / 8)); }
 static const dim3 B328(32, 8);
@@ -1207,14 +1469,15 @@ void launch_advect_corr(const Fields& F, hipStream_t s) {
   launch(k_advect_corr, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_diffuse(const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout, hipStream_t s) {
-  launch(k_diffuse, dim3(nblk(F.nx, F.ny)), B328, 0, s, F, Tin, Yin, Tout, Yout);
+  const int nb = (int)nblk(F.nx, F.ny);
+  launch(k_diffuse, dim3(nb), B328, 0, s, DiffuseOp::Args{F, Tin, Yin, Tout, Yout, nb});
 }
 unsigned sources_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
 void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* Yin, int* part, bool finish,
                     hipStream_t s) {
   const unsigned nb = nblk(F.nx, F.ny);
-  if (fuse) launch(k_sources<true>, dim3(nb), B328, 0, s, F, Tin, Yin, part);
-  else launch(k_sources<false>, dim3(nb), B328, 0, s, F, Tin, Yin, part);
+  if (fuse) launch(k_sources<true>, dim3(nb), B328, 0, s, SourcesOp<true>::Args{F, Tin, Yin, part, (int)nb});
+  else launch(k_sources<false>, dim3(nb), B328, 0, s, SourcesOp<false>::Args{F, Tin, Yin, part, (int)nb});
   if (finish) launch(k_acc_finish, dim3(1), dim3(256), 0, s, (const int*)part, (int)nb, F.acc);
 }
 void launch_forces(const Fields& F, hipStream_t s) {
@@ -1229,13 +1492,14 @@ void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* pa
   for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
   CL.ncascade = forces_coef_cascade(nlev);
   const unsigned nb = (unsigned)(((F.nx + FC_T - 1) / FC_T) * ((F.ny + FC_T - 1) / FC_T));
-  launch(k_forces_coef, dim3(nb), dim3(32, 8), 0, s, F, CL, part, nparts);
+  launch(k_forces_coef, dim3(nb), dim3(32, 8), 0, s, ForcesCoefOp::Args{F, CL, part, nparts, (int)nb});
 }
 void launch_coef_rhs(const Fields& F, hipStream_t s) {
   launch(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
 }
 void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s) {
-  launch(k_coef_restrict, dim3(nblk(c.nx + 1, c.ny + 1)), B328, 0, s, f, c);
+  const int nb = (int)nblk(c.nx + 1, c.ny + 1);
+  launch(k_coef_restrict, dim3(nb), B328, 0, s, CoefRestrictOp::Args{f, c, nb});
 }
 void launch_relax(const Level& L, int color, hipStream_t s) {
   launch(k_relax, dim3(nblk((L.nx + 1) / 2, L.ny)), B328, 0, s, L, color);
@@ -1250,28 +1514,32 @@ void launch_coarsest(const Level& L, int iters, hipStream_t s) {
   launch(k_coarsest, dim3(1), dim3(256), 0, s, L, iters);
 }
 void launch_project(const Fields& F, const double* p64, hipStream_t s) {
-  if (p64) launch(k_project<true>, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F, p64);
-  else launch(k_project<false>, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F, p64);
+  const int nb = (int)nblk(F.nx + 1, F.ny + 1);
+  if (p64) launch(k_project<true>, dim3(nb), B328, 0, s, ProjectOp<true>::Args{F, p64, nb});
+  else launch(k_project<false>, dim3(nb), B328, 0, s, ProjectOp<false>::Args{F, p64, nb});
 }
 void launch_geometry(const Fields& F, hipStream_t s) {
-  launch(k_geometry, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
+  const int nb = (int)nblk(F.nx + 1, F.ny + 1);
+  launch(k_geometry, dim3(nb), B328, 0, s, GeometryOp::Args{F, nb});
 }
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s) {
-  if (ntargets > 0) launch(k_radiation, dim3(ntargets), dim3(64), 0, s, F);
+  if (ntargets > 0) launch(k_radiation, dim3(ntargets), dim3(64), 0, s, RadiationOp::Args{F, ntargets});
 }
 void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s) {
   const int n = (F.nx + 1) * F.ny > F.nx * (F.ny + 1) ? (F.nx + 1) * F.ny : F.nx * (F.ny + 1);
   unsigned b = (unsigned)((n + 1023) / 1024);
   if (b > 1024) b = 1024;
-  launch(k_cfl, dim3(b), dim3(256), 0, s, F, cfl, umin, nsub);
+  launch(k_cfl, dim3(b), dim3(256), 0, s, CflOp::Args{F, cfl, umin, nsub, (int)b});
 }
 void launch_dc_defect(const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol, bool first,
                       hipStream_t s) {
-  launch(k_dc_defect, dim3(resnorm_blocks(L0)), dim3(256), 0, s, L0, p64, b0, dyn, part, tol, first ? 1 : 0);
+  const int nb = (int)resnorm_blocks(L0);
+  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, p64, b0, dyn, part, tol, first ? 1 : 0, nb});
 }
 void launch_dc_accum(const Level& L0, double* p64, const int* skip, hipStream_t s) {
   unsigned b = (unsigned)((L0.nx * L0.ny + 1023) / 1024);
-  launch(k_dc_accum, dim3(b > 2048 ? 2048 : b), dim3(256), 0, s, L0, p64, skip);
+  if (b > 2048) b = 2048;
+  launch(k_dc_accum, dim3(b), dim3(256), 0, s, DcAccumOp::Args{L0, p64, skip, (int)b});
 }
 void launch_p32_to_64(const float* p, double* p64, int n, hipStream_t s) {
   unsigned b = (unsigned)((n + 1023) / 1024);
@@ -1282,13 +1550,80 @@ unsigned resnorm_blocks(const Level& L0) {
   return b > 1024 ? 1024 : b;
 }
 void launch_resnorm(const Level& L0, Dyn* dyn, double* part, float tol, hipStream_t s) {
-  launch(k_resnorm, dim3(resnorm_blocks(L0)), dim3(256), 0, s, L0, dyn, part, tol);
+  const int nb = (int)resnorm_blocks(L0);
+  launch(k_resnorm, dim3(nb), dim3(256), 0, s, ResnormOp::Args{L0, dyn, part, tol, nb});
 }
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s) {
   if (n == 0) return;
   unsigned b = (unsigned)((n + 255) / 256);
   if (b > 4096) b = 4096;
   launch(k_fill_u32, dim3(b), dim3(256), 0, s, (uint32_t*)a, v, n);
+}
+// ---- batched phases (one scene's contribution each) ----
+void phase_diffuse(BatchPhase& ph, const Fields& F, const float* Tin, const float4* Yin, float* Tout, float4* Yout) {
+  const int nb = (int)nblk(F.nx, F.ny);
+  phase_add<DiffuseOp>(ph, DiffuseOp::Args{F, Tin, Yin, Tout, Yout, nb}, nb, B328, "diffuse");
+}
+void phase_sources(BatchPhase& ph, const Fields& F, const float* Tin, const float4* Yin, int* part) {
+  const int nb = (int)nblk(F.nx, F.ny);
+  phase_add<SourcesOp<true>>(ph, SourcesOp<true>::Args{F, Tin, Yin, part, nb}, nb, B328, "diffuse_sources");
+}
+void phase_forces_coef(BatchPhase& ph, const Fields& F, const Level* L, int nlev, const int* part, int nparts) {
+  CoefLevels CL;
+  for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
+  CL.ncascade = forces_coef_cascade(nlev);
+  const int nb = ((F.nx + FC_T - 1) / FC_T) * ((F.ny + FC_T - 1) / FC_T);
+  phase_add<ForcesCoefOp>(ph, ForcesCoefOp::Args{F, CL, part, nparts, nb}, nb, dim3(32, 8), "forces_coef");
+}
+void phase_coef_restrict(BatchPhase& ph, const Level& f, const Level& c) {
+  const int nb = (int)nblk(c.nx + 1, c.ny + 1);
+  phase_add<CoefRestrictOp>(ph, CoefRestrictOp::Args{f, c, nb}, nb, B328, "coef_restrict");
+}
+void phase_cfl(BatchPhase& ph, const Fields& F, float cfl, float umin, int nsub) {
+  const int n = (F.nx + 1) * F.ny > F.nx * (F.ny + 1) ? (F.nx + 1) * F.ny : F.nx * (F.ny + 1);
+  int b = (n + 1023) / 1024;
+  if (b > 1024) b = 1024;
+  phase_add<CflOp>(ph, CflOp::Args{F, cfl, umin, nsub, b}, b, dim3(256), "cfl");
+}
+void phase_resnorm(BatchPhase& ph, const Level& L0, Dyn* dyn, double* part, float tol) {
+  const int nb = (int)resnorm_blocks(L0);
+  phase_add<ResnormOp>(ph, ResnormOp::Args{L0, dyn, part, tol, nb}, nb, dim3(256), "resnorm");
+}
+void phase_dc_defect(BatchPhase& ph, const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
+                     bool first) {
+  const int nb = (int)resnorm_blocks(L0);
+  phase_add<DcDefectOp>(ph, DcDefectOp::Args{L0, p64, b0, dyn, part, tol, first ? 1 : 0, nb}, nb, dim3(256),
+                        "dc_defect");
+}
+void phase_dc_accum(BatchPhase& ph, const Level& L0, double* p64, const int* skip) {
+  int b = (L0.nx * L0.ny + 1023) / 1024;
+  if (b > 2048) b = 2048;
+  phase_add<DcAccumOp>(ph, DcAccumOp::Args{L0, p64, skip, b}, b, dim3(256), "dc_accum");
+}
+void phase_project(BatchPhase& ph, const Fields& F, const double* p64) {
+  const int nb = (int)nblk(F.nx + 1, F.ny + 1);
+  if (p64) phase_add<ProjectOp<true>>(ph, ProjectOp<true>::Args{F, p64, nb}, nb, B328, "project");
+  else phase_add<ProjectOp<false>>(ph, ProjectOp<false>::Args{F, p64, nb}, nb, B328, "project");
+}
+void phase_geometry(BatchPhase& ph, const Fields& F) {
+  const int nb = (int)nblk(F.nx + 1, F.ny + 1);
+  phase_add<GeometryOp>(ph, GeometryOp::Args{F, nb}, nb, B328, "geometry");
+}
+void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets) {
+  if (ntargets > 0) phase_add<RadiationOp>(ph, RadiationOp::Args{F, ntargets}, ntargets, dim3(64), "radiation");
+}
+void phase_fill_u32(BatchPhase& ph, void* a, uint32_t v, size_t n) {
+  int b = (int)((n + 255) / 256);
+  if (b > 4096) b = 4096;
+  if (n > 0) phase_add<FillOp>(ph, FillOp::Args{(uint32_t*)a, v, n, b}, b, dim3(256), "fill");
+}
+
+unsigned compat_blocks(const Fields& F) {
+  const unsigned b = (unsigned)(((F.nx + 1) * (F.ny + 1) + 2047) / 2048);
+  return b > 1024 ? 1024 : b;
+}
+void launch_compat(const Fields& F, double* part, hipStream_t s) {
+  launch(k_compat, dim3(compat_blocks(F)), dim3(256), 0, s, F, part);
 }
 unsigned div_error_blocks(const Fields& F) { return nblk(F.nx, F.ny); }
 void launch_div_error(const Fields& F, float* part, hipStream_t s) {

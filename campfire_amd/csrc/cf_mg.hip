@@ -86,7 +86,19 @@ __device__ __forceinline__ void pairs_relax(const PairCell (&c)[PQ][2], const in
 }
 
 template <int TX, int TY, int NU>
-__global__ __launch_bounds__(256) void k_mg_down(Level f, Level c, const int* __restrict__ skip) {
+struct MgDownOp {
+  struct Args {
+    Level f;
+    Level c;
+    const int* skip;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& f = a_.f;
+    const Level& c = a_.c;
+    const int* __restrict__ skip = a_.skip;
+
   if (skip && *skip) return;  // the step's residual target is met
   constexpr int H = 2 * NU + 1;
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
@@ -96,7 +108,7 @@ __global__ __launch_bounds__(256) void k_mg_down(Level f, Level c, const int* __
   __shared__ float sres[TX * TY];
   const int nx = f.nx, ny = f.ny;
   const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
-  const int t = xcd_tile(blockIdx.x, gx * gy);
+  const int t = vb;
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
   const int tid = threadIdx.x;
   const float h2 = f.h2;
@@ -152,10 +164,28 @@ __global__ __launch_bounds__(256) void k_mg_down(Level f, Level c, const int* __
       c.p[J * c.nx + I] = K(0);
     }
   }
+
+  }
+};
+template <int TX, int TY, int NU>
+__global__ __launch_bounds__(256) void k_mg_down(const typename MgDownOp<TX, TY, NU>::Args a_) {
+  MgDownOp<TX, TY, NU>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <int TX, int TY, int NU>
-__global__ __launch_bounds__(256) void k_mg_up(Level f, Level c, const int* __restrict__ skip) {
+struct MgUpOp {
+  struct Args {
+    Level f;
+    Level c;
+    const int* skip;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& f = a_.f;
+    const Level& c = a_.c;
+    const int* __restrict__ skip = a_.skip;
+
   if (skip && *skip) return;
   constexpr int H = 2 * NU;
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
@@ -164,7 +194,7 @@ __global__ __launch_bounds__(256) void k_mg_up(Level f, Level c, const int* __re
   __shared__ float sp[NR];
   const int nx = f.nx, ny = f.ny;
   const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
-  const int t = xcd_tile(blockIdx.x, gx * gy);
+  const int t = vb;
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
   const int tid = threadIdx.x;
   const float h2 = f.h2;
@@ -215,6 +245,12 @@ __global__ __launch_bounds__(256) void k_mg_up(Level f, Level c, const int* __re
         f.p[(y0 - H + ry) * nx + (x0 - H + rx)] = sp[ci];
     }
   }
+
+  }
+};
+template <int TX, int TY, int NU>
+__global__ __launch_bounds__(256) void k_mg_up(const typename MgUpOp<TX, TY, NU>::Args a_) {
+  MgUpOp<TX, TY, NU>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---- bottom levels in one workgroup ------------------------------------------------
@@ -232,9 +268,10 @@ constexpr int BT = 1024;
 struct Bottom {
   Level L[3];       // finest first; the last is the coarsest level
   const int* skip;  // may be null: nothing to do once *skip is non-zero
+  double* band;     // coarse_iters = 0: scratch of the direct solve
   int n;            // 1..3
   int coarse_coef;  // 1: restrict the coarsest level's ax/ay from L[n-2] here
-  int nu, coarse_iters;
+  int nu, coarse_iters;  // coarse_iters = 0: direct FP64 solve of the coarsest level
 };
 
 template <int CPT>
@@ -346,6 +383,78 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Direct FP64 solve of the coarsest level (schedule coarse_iters = 0):
+// A p = -rhs h2, A the relaxation's operator (diag on the diagonal, minus the
+// face coefficients to existing neighbours; a closed cell is an identity row),
+// symmetric positive definite.  It is factored as L L^T inside its band of
+// half-width ncx, band[k * W + d] = A[k][k - d] (W = ncx + 1, ncx <= 32), in
+// the scene's scratch memory: one column per pair of workgroup barriers.  The
+// first wave then solves L z = y and L^T p = z, each lane holding one of the
+// last ncx unknowns, the products summed across the wave.
+template <int CPT>
+__device__ void coarse_direct(const CellSet<CPT>& C2, int ncx, int ncy, float h2, const float* sr2, float* sp2,
+                              double* band, double* z) {
+  const int t = threadIdx.x, N = ncx * ncy, W = ncx + 1;
+  for (int q = t; q < N * W; q += BT) band[q] = 0.0;
+  __syncthreads();
+  if (t < 64) {
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int k = C2.k[q];
+      if (k < 0) continue;
+      if (C2.diag[q] > K(0)) {
+        band[k * W] = (double)C2.diag[q];
+        if (C2.nb[q] & 1) band[k * W + 1] = -(double)C2.w[q];
+        if (C2.nb[q] & 4) band[k * W + ncx] = -(double)C2.s[q];
+        z[k] = -(double)sr2[k] * (double)h2;
+      } else {
+        band[k * W] = 1.0;
+        z[k] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < N; j++) {
+    const double ljj = sqrt(band[j * W]);
+    if (t >= 1 && t <= ncx && j + t < N) band[(j + t) * W + t] /= ljj;
+    __syncthreads();
+    if (t == 0) band[j * W] = ljj;
+    if (t < ncx * ncx) {
+      const int a = t / ncx + 1, c = t % ncx + 1;  // row j + a, column j + c
+      if (c <= a && j + a < N) band[(j + a) * W + (a - c)] -= band[(j + a) * W + a] * band[(j + c) * W + c];
+    }
+    __syncthreads();
+  }
+  if (t < 64) {
+    const int l = t;  // lane l holds the unknown l rows back (forward) or ahead (backward)
+    double win = 0.0;
+    for (int j = 0; j < N; j++) {  // L z = y, y in z (LDS, read by every lane before lane 0 overwrites it)
+      const double part = (l >= 1 && l <= ncx && j - l >= 0) ? band[j * W + l] * win : 0.0;
+      const double zj = (z[j] - wave_sum64(part)) / band[j * W];
+      const double up = __shfl_up(win, 1, 64);
+      win = l == 1 ? zj : up;
+      wave_sync();
+      if (l == 0) z[j] = zj;
+    }
+    wave_sync();
+    win = 0.0;
+    for (int j = N - 1; j >= 0; j--) {  // L^T p = z
+      const double part = (l >= 1 && l <= ncx && j + l < N) ? band[(j + l) * W + l] * win : 0.0;
+      const double pj = (z[j] - wave_sum64(part)) / band[j * W];
+      const double up = __shfl_up(win, 1, 64);
+      win = l == 1 ? pj : up;
+      if (l == 0) sp2[j] = (float)pj;
+    }
+  }
+}
+
+
 // coarse rhs = 1/4 of the 4 fine residuals (shaders/pressure.wgsl:104-113)
 __device__ __forceinline__ void restrict_to(const float* res, int nxf, float* rhsc, float* pc, int nxc, int nyc) {
   for (int k = threadIdx.x; k < nxc * nyc; k += BT) {
@@ -380,7 +489,15 @@ constexpr int BOT_P0 = 0, BOT_R0 = 4096, BOT_P1 = 8192, BOT_R1 = 9216, BOT_P2 = 
 __device__ __forceinline__ int bot_p(int s) { return s == 0 ? BOT_P0 : (s == 1 ? BOT_P1 : BOT_P2); }
 __device__ __forceinline__ int bot_r(int s) { return s == 0 ? BOT_R0 : (s == 1 ? BOT_R1 : BOT_R2); }
 
-__global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
+struct MgBottomOp {
+  struct Args {
+    Bottom B;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 1024;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Bottom& B = a_.B;
+
   if (B.skip && *B.skip) return;
   __shared__ float sm[BOT_TOTAL];
   float* sp2 = sm + BOT_P2;
@@ -450,7 +567,9 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
   // the first wave; the other waves wait at the next workgroup barrier
   for (int k = t; k < Lc.nx * Lc.ny; k += BT) sp2[k] = K(0);
   __syncthreads();
-  if (t < 64) {
+  if (B.coarse_iters <= 0) {
+    coarse_direct<4>(C2, Lc.nx, Lc.ny, Lc.h2, sr2, sp2, B.band, (double*)sres);
+  } else if (t < 64) {
     float rh[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) rh[q] = C2.k[q] >= 0 ? sr2[C2.k[q]] * Lc.h2 : K(0);
@@ -485,6 +604,11 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(Bottom B) {
       if (s > s0) L.rhs[k] = R(s)[k];
     }
   }
+
+  }
+};
+__global__ __launch_bounds__(1024) void k_mg_bottom(const MgBottomOp::Args a_) {
+  MgBottomOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---- launchers -------------------------------------------------------------------
@@ -497,9 +621,9 @@ static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
 bool launch_mg_down(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s) {
   const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
   switch (nu) {
-    case 1: launch((k_mg_down<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
-    case 2: launch((k_mg_down<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
-    case 3: launch((k_mg_down<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
+    case 1: launch((k_mg_down<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, MgDownOp<MG_TX, MG_TY, 1>::Args{f, c, skip, (int)nb}); return true;
+    case 2: launch((k_mg_down<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, MgDownOp<MG_TX, MG_TY, 2>::Args{f, c, skip, (int)nb}); return true;
+    case 3: launch((k_mg_down<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, MgDownOp<MG_TX, MG_TY, 3>::Args{f, c, skip, (int)nb}); return true;
     default: return false;
   }
 }
@@ -507,9 +631,29 @@ bool launch_mg_down(const Level& f, const Level& c, int nu, const int* skip, hip
 bool launch_mg_up(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s) {
   const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
   switch (nu) {
-    case 1: launch((k_mg_up<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
-    case 2: launch((k_mg_up<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
-    case 3: launch((k_mg_up<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, f, c, skip); return true;
+    case 1: launch((k_mg_up<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, MgUpOp<MG_TX, MG_TY, 1>::Args{f, c, skip, (int)nb}); return true;
+    case 2: launch((k_mg_up<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, MgUpOp<MG_TX, MG_TY, 2>::Args{f, c, skip, (int)nb}); return true;
+    case 3: launch((k_mg_up<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, MgUpOp<MG_TX, MG_TY, 3>::Args{f, c, skip, (int)nb}); return true;
+    default: return false;
+  }
+}
+
+bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip) {
+  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
+  switch (nu) {
+    case 1: phase_add<MgDownOp<MG_TX, MG_TY, 1>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_down"); return true;
+    case 2: phase_add<MgDownOp<MG_TX, MG_TY, 2>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_down"); return true;
+    case 3: phase_add<MgDownOp<MG_TX, MG_TY, 3>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_down"); return true;
+    default: return false;
+  }
+}
+
+bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip) {
+  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
+  switch (nu) {
+    case 1: phase_add<MgUpOp<MG_TX, MG_TY, 1>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_up"); return true;
+    case 2: phase_add<MgUpOp<MG_TX, MG_TY, 2>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_up"); return true;
+    case 3: phase_add<MgUpOp<MG_TX, MG_TY, 3>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_up"); return true;
     default: return false;
   }
 }
@@ -525,15 +669,31 @@ int mg_bottom_first(const Level* L, int nlev) {
   return b;
 }
 
-bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, const int* skip,
-                      hipStream_t s) {
+static bool bottom_args(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, const int* skip,
+                        double* band, Bottom& B) {
   const int n = nlev - first;
   if (n < 1 || n > 3 || L[nlev - 1].nx * L[nlev - 1].ny > 256) return false;
   if (coarse_coef && n < 2) return false;
-  Bottom B;
+  if (coarse_iters <= 0 && (!band || L[nlev - 1].nx > 32)) return false;
   for (int q = 0; q < n; q++) B.L[q] = L[first + q];
   B.n = n; B.coarse_coef = coarse_coef ? 1 : 0; B.nu = nu; B.coarse_iters = coarse_iters; B.skip = skip;
-  launch(k_mg_bottom, dim3(1), dim3(BT), 0, s, B);
+  B.band = band;
+  return true;
+}
+
+bool phase_mg_bottom(BatchPhase& ph, const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters,
+                     const int* skip, double* band) {
+  Bottom B;
+  if (!bottom_args(L, nlev, first, coarse_coef, nu, coarse_iters, skip, band, B)) return false;
+  phase_add<MgBottomOp>(ph, MgBottomOp::Args{B, 1}, 1, dim3(BT), "mg_bottom");
+  return true;
+}
+
+bool launch_mg_bottom(const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters, const int* skip,
+                      double* band, hipStream_t s) {
+  Bottom B;
+  if (!bottom_args(L, nlev, first, coarse_coef, nu, coarse_iters, skip, band, B)) return false;
+  launch(k_mg_bottom, dim3(1), dim3(BT), 0, s, MgBottomOp::Args{B, 1});
   return true;
 }
 

@@ -12,6 +12,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "cf_kernels.h"
@@ -74,6 +75,8 @@ struct ParamUpload {
 struct cf_sim {
   int dev = 0;
   hipStream_t st = nullptr;
+  bool own_stream = true;  // false for the scenes of a pool (the pool's stream)
+  bool in_pool = false;
   int nx = 0, ny = 0, nC = 0, nU = 0, nV = 0;
   double dx = 0, dt = 0, time = 0;
   uint32_t frame = 0;
@@ -93,6 +96,7 @@ struct cf_sim {
   double* dNorm = nullptr;   // residual-norm partials
   double* dP64 = nullptr;    // level-0 pressure in FP64 (defect correction)
   float* dB0 = nullptr;      // level-0 right-hand side kept across correction cycles
+  double* dBand = nullptr;   // direct coarsest solve: Cholesky band
   // upload rings
   ParamUpload* hP = nullptr;
   float* hPieces = nullptr;
@@ -162,7 +166,7 @@ static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef)
     // fused: tiled down legs, one LDS-resident workgroup for the bottom levels, tiled up legs
     const int b = s->mg_bottom;
     for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, skip, s->st); mark(s, level_tag("mg_down", l)); }
-    cf::launch_mg_bottom(s->L, L, b, coarse_coef, nu, s->sch.coarse_iters, skip, s->st); mark(s, "mg_bottom");
+    cf::launch_mg_bottom(s->L, L, b, coarse_coef, nu, s->sch.coarse_iters, skip, s->dBand, s->st); mark(s, "mg_bottom");
     for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, skip, s->st); mark(s, level_tag("mg_up", l)); }
     return;
   }
@@ -385,6 +389,196 @@ static bool lookup(cf_sim* s, const char* name, FieldRef& r) {
 }
 
 // ---- C ABI -------------------------------------------------------------------
+struct cf_pool {
+  hipStream_t st = nullptr;
+  uint64_t budget = 0, used = 0;
+  std::vector<cf_sim*> slots;  // null: free slot
+  cf_schedule sch;
+  std::vector<cf::BatchPhase> step, frame_pre, frame_post;
+  bool dirty = true;
+  uint64_t key = 0, fkey = 0;
+  hipGraphExec_t g1 = nullptr, g8 = nullptr;
+  bool use_graph = true;
+  int nlaunch = 0;
+  int64_t steps = 0;
+};
+
+// ---- scene pool: many scenes stepped together ---------------------------------
+// Every phase of a step is one launch over the tiles of all live scenes
+// (k_batch), so scenes of any mix of sizes advance together and the small
+// scenes' latency-bound launches overlap with the large ones'.  A scene's own
+// work (tiles, reductions, V-cycle levels) is exactly what it does alone.
+
+// the order of a scene's extra diffusion substeps before sources, as
+// enqueue_step issues them: (in, out) pairs and the fields sources reads
+static void diffusion_chain(cf_sim* s, std::vector<std::array<void*, 4>>& chain, const float** Tin, const float4** Yin) {
+  Fields& F = s->F;
+  *Tin = F.T2;
+  *Yin = F.Y2;
+  if (s->diffSub > 1) {
+    chain.push_back({(void*)F.T2, (void*)F.Y2, (void*)F.T, (void*)F.Y});
+    for (int q = 1; q < s->diffSub; q += 2) {
+      chain.push_back({(void*)F.T, (void*)F.Y, (void*)F.Th, (void*)F.Yh});
+      if (q + 2 < s->diffSub) chain.push_back({(void*)F.Th, (void*)F.Yh, (void*)F.T, (void*)F.Y});
+    }
+    *Tin = F.Th;
+    *Yin = F.Yh;
+  }
+}
+
+static void pool_release_phases(std::vector<cf::BatchPhase>& v) {
+  for (auto& p : v) p.release();
+  v.clear();
+}
+
+static void pool_drop_graph(cf_pool* pl) {
+  if (pl->g1) hipGraphExecDestroy(pl->g1);
+  if (pl->g8) hipGraphExecDestroy(pl->g8);
+  pl->g1 = pl->g8 = nullptr;
+}
+
+// the batched step of the live scenes, in the order of enqueue_step
+static int pool_build_step(cf_pool* pl) {
+  pool_drop_graph(pl);
+  if (pl->st) hipStreamSynchronize(pl->st);  // the old tables may still be read
+  pool_release_phases(pl->step);
+  std::vector<cf_sim*> sc;
+  for (auto* s : pl->slots) if (s) sc.push_back(s);
+  std::vector<cf::BatchPhase>& ph = pl->step;
+  ph.reserve(64 + 16 * (size_t)pl->sch.vcycles);
+  auto P = [&]() -> cf::BatchPhase& { ph.emplace_back(); return ph.back(); };
+  const cf_schedule& k = pl->sch;
+  if (k.cfl > 0) {
+    cf::BatchPhase& a = P();
+    for (auto* s : sc) cf::phase_cfl(a, s->F, k.cfl, k.umin, s->diffSub);
+  }
+  { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_advect(a, s->F); }
+  // extra diffusion substeps: phase d holds every scene's d-th
+  std::vector<std::vector<std::array<void*, 4>>> chains(sc.size());
+  std::vector<const float*> tin(sc.size());
+  std::vector<const float4*> yin(sc.size());
+  size_t maxc = 0;
+  for (size_t q = 0; q < sc.size(); q++) {
+    diffusion_chain(sc[q], chains[q], &tin[q], &yin[q]);
+    maxc = chains[q].size() > maxc ? chains[q].size() : maxc;
+  }
+  for (size_t d = 0; d < maxc; d++) {
+    cf::BatchPhase& a = P();
+    for (size_t q = 0; q < sc.size(); q++)
+      if (d < chains[q].size())
+        cf::phase_diffuse(a, sc[q]->F, (const float*)chains[q][d][0], (const float4*)chains[q][d][1],
+                          (float*)chains[q][d][2], (float4*)chains[q][d][3]);
+  }
+  {
+    cf::BatchPhase& a = P();
+    for (size_t q = 0; q < sc.size(); q++) cf::phase_sources(a, sc[q]->F, tin[q], yin[q], sc[q]->dAccPart);
+  }
+  {
+    cf::BatchPhase& a = P();
+    for (auto* s : sc) cf::phase_forces_coef(a, s->F, s->L, s->nlev, s->dAccPart, (int)cf::sources_blocks(s->F));
+  }
+  // coefficient restriction beyond the forces pass's cascade, by level
+  std::vector<char> coarse_coef(sc.size());
+  for (size_t q = 0; q < sc.size(); q++) {
+    cf_sim* s = sc[q];
+    const int last_coef = s->nlev - 1 - (s->mg_bottom < s->nlev - 1 ? 1 : 0);
+    coarse_coef[q] = last_coef < s->nlev - 1 && s->nlev - 1 > cf::forces_coef_cascade(s->nlev);
+  }
+  for (int l = 1; l < MAXLEV; l++) {
+    cf::BatchPhase a;
+    for (auto* s : sc) {
+      const int last_coef = s->nlev - 1 - (s->mg_bottom < s->nlev - 1 ? 1 : 0);
+      if (l >= 1 + cf::forces_coef_cascade(s->nlev) && l <= last_coef) cf::phase_coef_restrict(a, s->L[l - 1], s->L[l]);
+    }
+    if (!a.empty()) ph.push_back(a);
+  }
+  // V-cycles; down leg d and up leg d hold level d of every scene that has one
+  const bool tol = k.tol > 0, dc = tol && k.refine;
+  if (tol) { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_fill_u32(a, &s->dDyn->converged, 0u, 2); }
+  if (dc) {
+    cf::BatchPhase& a = P();
+    for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, true);
+  }
+  int maxb = 0;
+  for (auto* s : sc) maxb = s->mg_bottom > maxb ? s->mg_bottom : maxb;
+  for (int c = 0; c < k.vcycles; c++) {
+    for (int d = 0; d < maxb; d++) {
+      cf::BatchPhase& a = P();
+      for (auto* s : sc)
+        if (d < s->mg_bottom) cf::phase_mg_down(a, s->L[d], s->L[d + 1], k.nu, tol ? &s->dDyn->converged : nullptr);
+    }
+    {
+      cf::BatchPhase& a = P();
+      for (size_t q = 0; q < sc.size(); q++) {
+        cf_sim* s = sc[q];
+        if (!cf::phase_mg_bottom(a, s->L, s->nlev, s->mg_bottom, coarse_coef[q] && c == 0, k.nu, k.coarse_iters,
+                                 tol ? &s->dDyn->converged : nullptr, s->dBand))
+          return set_err(CF_EINVAL, "pool: bottom solver does not fit a %dx%d scene", s->nx, s->ny);
+      }
+    }
+    for (int d = maxb - 1; d >= 0; d--) {
+      cf::BatchPhase& a = P();
+      for (auto* s : sc)
+        if (d < s->mg_bottom) cf::phase_mg_up(a, s->L[d], s->L[d + 1], k.nu, tol ? &s->dDyn->converged : nullptr);
+    }
+    if (dc) {
+      { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_dc_accum(a, s->L[0], s->dP64, &s->dDyn->converged); }
+      {
+        cf::BatchPhase& a = P();
+        for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, false);
+      }
+    } else if (tol) {
+      cf::BatchPhase& a = P();
+      for (auto* s : sc) cf::phase_resnorm(a, s->L[0], s->dDyn, s->dNorm, k.tol);
+    }
+  }
+  { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_project(a, s->F, dc ? s->dP64 : nullptr); }
+  for (auto& a : ph) {
+    hipError_t e = a.upload(pl->st);
+    if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables: %s", hipGetErrorString(e));
+  }
+  pl->nlaunch = 0;
+  for (auto& a : ph) pl->nlaunch += a.empty() ? 0 : 1;
+  pl->dirty = false;
+  return CF_OK;
+}
+
+static void pool_enqueue_step(cf_pool* pl) {
+  for (auto& a : pl->step) a.run(pl->st);
+}
+
+static int pool_capture(cf_pool* pl, int n, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(pl->st, hipStreamCaptureModeThreadLocal));
+  for (int q = 0; q < n; q++) pool_enqueue_step(pl);
+  HIPCHK(hipStreamEndCapture(pl->st, &g));
+  hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return set_err(CF_EDEVICE, "pool graph: %s", hipGetErrorString(e));
+  return CF_OK;
+}
+
+// the batched step for the current scenes, rebuilt when scenes come or go
+// or a scene's substep count changes
+static int pool_ready(cf_pool* pl) {
+  uint64_t key = 1469598103934665603ull;
+  for (auto* s : pl->slots) {
+    key = (key ^ (uint64_t)(uintptr_t)s) * 1099511628211ull;
+    if (s) key = (key ^ (uint64_t)s->diffSub) * 1099511628211ull;
+  }
+  if (key != pl->key) { pl->dirty = true; pl->key = key; }
+  if (pl->dirty) {
+    int rc = pool_build_step(pl);
+    if (rc) return rc;
+  }
+  if (pl->use_graph && !pl->g1) {
+    int rc = pool_capture(pl, 1, &pl->g1);
+    if (!rc) rc = pool_capture(pl, 8, &pl->g8);
+    if (rc) return rc;
+  }
+  return CF_OK;
+}
+
 extern "C" {
 
 const char* cf_last_error(void) { return g_err; }
@@ -401,58 +595,33 @@ void cf_sim_destroy(cf_sim* s) {
   if (s->hPieces) hipHostFree(s->hPieces);
   if (s->ring) hipHostFree(s->ring);
   if (s->block) hipFree(s->block);
-  if (s->st) hipStreamDestroy(s->st);
+  if (s->st && s->own_stream) hipStreamDestroy(s->st);
   delete s;
 }
 
-int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
-  if (!out) return set_err(CF_EINVAL, "out is NULL");
-  *out = nullptr;
+// grid sizes and the multigrid levels of a new scene; levels halve while both
+// dims are even and cells > 256 (js/fluid.js:71-91; the reference's 6-level
+// cap is lifted so 1024^2 reaches a 16x16 coarsest level)
+static int sim_levels(cf_sim* s, int nx, int ny, double dx, double dt) {
   if (nx < 2 || ny < 2 || nx > 16384 || ny > 16384 || !(dx > 0) || !(dt > 0))
     return set_err(CF_EINVAL, "bad grid %dx%d dx=%g dt=%g", nx, ny, dx, dt);
-  cf_sim* s = new cf_sim();
   s->nx = nx; s->ny = ny; s->dx = dx; s->dt = dt;
   s->nC = nx * ny; s->nU = (nx + 1) * ny; s->nV = nx * (ny + 1);
-  // levels: halve while both dims are even and cells > 256 (js/fluid.js:71-91;
-  // the reference's 6-level cap is lifted so 1024^2 reaches a 16x16 coarsest level)
-  {
-    int lx = nx, ly = ny;
-    double h = dx;
-    for (int l = 0; l < MAXLEV; l++) {
-      s->lnx[l] = lx; s->lny[l] = ly; s->lh2[l] = (float)(h * h);
-      s->nlev = l + 1;
-      if (lx % 2 || ly % 2 || lx * ly <= 256) break;
-      lx /= 2; ly /= 2; h *= 2;
-    }
-    if (s->lnx[s->nlev - 1] * s->lny[s->nlev - 1] > 256) {
-      delete s;
-      return set_err(CF_EBADSIZE, "coarsest level too large (%dx%d)", s->lnx[s->nlev - 1], s->lny[s->nlev - 1]);
-    }
+  int lx = nx, ly = ny;
+  double h = dx;
+  for (int l = 0; l < MAXLEV; l++) {
+    s->lnx[l] = lx; s->lny[l] = ly; s->lh2[l] = (float)(h * h);
+    s->nlev = l + 1;
+    if (lx % 2 || ly % 2 || lx * ly <= 256) break;
+    lx /= 2; ly /= 2; h *= 2;
   }
-  // defaults (js/fluid.js:35-45, js/config.js:12-47)
-  const double def[CF_NPARAMS] = {
-      (double)nx, (double)ny, dx, dt, PHYS_Tamb, phys_rhoAmb(), 0.232, 9.81,
-      1300, 0.7, 1, 1, 2.0e10, 15000, 15.0e6, 15.0e6 / 13.1e6,
-      6.0e6, 25000, 5.4e8, 19800,
-      1862, 0.8, 0.05, 0.8,
-      6.6, 5800, 0, 0,
-      0, 0, 0, 0, 0, 0, 0, 600,
-      1800, 5.670374e-8, 2.667, 0};
-  memcpy(s->pv, def, sizeof(def));
-  s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
-  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0;
-  const char* ng = getenv("CF_NO_GRAPH");
-  s->use_graph = !(ng && *ng && *ng != '0');
-  const char* ms = getenv("CF_MG_SIMPLE");
-  s->mg_simple = ms && *ms && *ms != '0';
+  if (s->lnx[s->nlev - 1] * s->lny[s->nlev - 1] > 256)
+    return set_err(CF_EBADSIZE, "coarsest level too large (%dx%d)", s->lnx[s->nlev - 1], s->lny[s->nlev - 1]);
+  return CF_OK;
+}
 
-  hipError_t e = hipGetDevice(&s->dev);
-  if (e != hipSuccess) { delete s; return set_err(CF_EDEVICE, "hipGetDevice: %s", hipGetErrorString(e)); }
-  e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
-  if (e != hipSuccess) { s->st = nullptr; delete s; return set_err(CF_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
-
-  // one allocation, 256-B aligned sub-buffers
-  std::vector<std::pair<void**, size_t>> plan;
+// the scene's device buffers: one allocation of 256-B aligned sub-buffers
+static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
   Fields& F = s->F;
   auto add = [&](void** p, size_t bytes) { plan.push_back({p, bytes}); };
   const size_t nC = s->nC, nU = s->nU, nV = s->nV;
@@ -473,14 +642,68 @@ int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
     add((void**)&s->L[l].p, lx * ly * 4); add((void**)&s->L[l].rhs, lx * ly * 4);
     add((void**)&s->L[l].ax, (lx + 1) * ly * 4); add((void**)&s->L[l].ay, lx * (ly + 1) * 4);
   }
-  const unsigned nparts = (unsigned)(((nx + 31) / 32) * ((ny + 7) / 8));
+  const unsigned nparts = (unsigned)(((s->nx + 31) / 32) * ((s->ny + 7) / 8));
   add((void**)&s->dPart, (size_t)nparts * 8);
   add((void**)&s->dAccPart, (size_t)nparts * 16);
   add((void**)&s->dNorm, 2 * sizeof(double) * 1024);
   add((void**)&s->dP64, nC * sizeof(double));
   add((void**)&s->dB0, nC * sizeof(float));
+  add((void**)&s->dBand, (256 * 33 + 256) * sizeof(double));
+}
+
+static size_t plan_total(const std::vector<std::pair<void**, size_t>>& plan) {
   size_t total = 0;
   for (auto& q : plan) total += (q.second + 255) & ~(size_t)255;
+  return total;
+}
+
+static int sim_create(int nx, int ny, double dx, double dt, hipStream_t shared, cf_sim** out);
+
+int cf_sim_create(int nx, int ny, double dx, double dt, cf_sim** out) {
+  return sim_create(nx, ny, dx, dt, nullptr, out);
+}
+
+// shared: the stream of a pool (borrowed), or null for a stream of its own
+static int sim_create(int nx, int ny, double dx, double dt, hipStream_t shared, cf_sim** out) {
+  if (!out) return set_err(CF_EINVAL, "out is NULL");
+  *out = nullptr;
+  cf_sim* s = new cf_sim();
+  {
+    const int rc = sim_levels(s, nx, ny, dx, dt);
+    if (rc) { delete s; return rc; }
+  }
+  // defaults (js/fluid.js:35-45, js/config.js:12-47)
+  const double def[CF_NPARAMS] = {
+      (double)nx, (double)ny, dx, dt, PHYS_Tamb, phys_rhoAmb(), 0.232, 9.81,
+      1300, 0.7, 1, 1, 2.0e10, 15000, 15.0e6, 15.0e6 / 13.1e6,
+      6.0e6, 25000, 5.4e8, 19800,
+      1862, 0.8, 0.05, 0.8,
+      6.6, 5800, 0, 0,
+      0, 0, 0, 0, 0, 0, 0, 600,
+      1800, 5.670374e-8, 2.667, 0};
+  memcpy(s->pv, def, sizeof(def));
+  s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
+  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0;
+  const char* ng = getenv("CF_NO_GRAPH");
+  s->use_graph = !(ng && *ng && *ng != '0');
+  const char* ms = getenv("CF_MG_SIMPLE");
+  s->mg_simple = ms && *ms && *ms != '0';
+
+  hipError_t e = hipGetDevice(&s->dev);
+  if (e != hipSuccess) { delete s; return set_err(CF_EDEVICE, "hipGetDevice: %s", hipGetErrorString(e)); }
+  if (shared) {
+    s->st = shared;
+    s->own_stream = false;
+  } else {
+    e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+    if (e != hipSuccess) { s->st = nullptr; delete s; return set_err(CF_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  }
+
+  std::vector<std::pair<void**, size_t>> plan;
+  sim_plan(s, plan);
+  Fields& F = s->F;
+  const size_t total = plan_total(plan);
+  const unsigned nparts = (unsigned)(((nx + 31) / 32) * ((ny + 7) / 8));
   e = hipMalloc(&s->block, total);
   if (e != hipSuccess) { s->block = nullptr; cf_sim_destroy(s); return set_err(CF_ENOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(e)); }
   s->block_bytes = total;
@@ -603,8 +826,10 @@ int cf_sim_get_packed_params(const cf_sim* s, cf_params* out) {
 
 int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
   if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
-  if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 1 || !(sc->tol >= 0))
-    return set_err(CF_EINVAL, "bad schedule (nu 1..3, vcycles >= 1, coarse_iters >= 1)");
+  if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 0 || !(sc->tol >= 0))
+    return set_err(CF_EINVAL, "bad schedule (nu 1..3, vcycles >= 1, coarse_iters >= 0)");
+  if (sc->coarse_iters == 0 && (s->lnx[s->nlev - 1] > 32 || s->mg_simple))
+    return set_err(CF_EINVAL, "direct coarsest solve needs a coarsest level <= 32 cells wide (fused path)");
   if (!(sc->cfl >= 0) || (sc->cfl > 0 && !(sc->umin > 0)))
     return set_err(CF_EINVAL, "bad schedule (cfl >= 0; umin > 0 with a CFL step)");
   if (sc->refine && !(sc->tol > 0)) return set_err(CF_EINVAL, "bad schedule (refine needs tol > 0)");
@@ -879,6 +1104,20 @@ int cf_sim_div_error(cf_sim* s, float* max_err, float* mean_err) {
   return CF_OK;
 }
 
+int cf_sim_compat_sums(cf_sim* s, double* source, double* outflow) {
+  if (!s || !source || !outflow) return set_err(CF_EINVAL, "NULL argument");
+  const unsigned nb = cf::compat_blocks(s->F);
+  std::vector<double> h((size_t)nb * 2);
+  cf::launch_compat(s->F, s->dNorm, s->st);
+  HIPCHK(hipMemcpyAsync(h.data(), s->dNorm, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  double a = 0, b = 0;
+  for (unsigned q = 0; q < nb; q++) { a += h[2 * q]; b += h[2 * q + 1]; }
+  *source = a;
+  *outflow = b;
+  return CF_OK;
+}
+
 int cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_result* out) {
   if (!cases || !out || ncases < 1) return set_err(CF_EINVAL, "bad argument");
   std::vector<cf_flame_case> c(cases, cases + ncases);
@@ -907,6 +1146,203 @@ int cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_re
   if (dr) hipFree(dr);
   if (st) hipStreamDestroy(st);
   return rc;
+}
+
+uint64_t cf_scene_bytes(int nx, int ny, double dx) {
+  cf_sim* s = new cf_sim();
+  uint64_t total = 0;
+  if (sim_levels(s, nx, ny, dx, 1.0) == CF_OK) {
+    std::vector<std::pair<void**, size_t>> plan;
+    sim_plan(s, plan);
+    total = plan_total(plan);
+  }
+  delete s;
+  return total;
+}
+
+int cf_pool_create(uint64_t hbm_budget, cf_pool** out) {
+  if (!out) return set_err(CF_EINVAL, "out is NULL");
+  *out = nullptr;
+  cf_pool* pl = new cf_pool();
+  pl->budget = hbm_budget;
+  pl->sch.nu = 2; pl->sch.vcycles = 1; pl->sch.tol = 0; pl->sch.coarse_iters = 120; pl->sch.smoother_fp16 = 0;
+  pl->sch.cfl = 0; pl->sch.umin = 1.0f; pl->sch.refine = 0;
+  const char* ng = getenv("CF_NO_GRAPH");
+  pl->use_graph = !(ng && *ng && *ng != '0');
+  hipError_t e = hipStreamCreateWithFlags(&pl->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete pl; return set_err(CF_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  *out = pl;
+  return CF_OK;
+}
+
+void cf_pool_destroy(cf_pool* pl) {
+  if (!pl) return;
+  hipStreamSynchronize(pl->st);
+  pool_drop_graph(pl);
+  pool_release_phases(pl->step);
+  pool_release_phases(pl->frame_pre);
+  pool_release_phases(pl->frame_post);
+  for (auto* s : pl->slots) if (s) cf_sim_destroy(s);
+  hipStreamDestroy(pl->st);
+  delete pl;
+}
+
+int cf_pool_set_schedule(cf_pool* pl, const cf_schedule* sc) {
+  if (!pl || !sc) return set_err(CF_EINVAL, "NULL argument");
+  for (auto* s : pl->slots)
+    if (s) { int rc = cf_sim_set_schedule(s, sc); if (rc) return rc; }
+  // validate with a scene's rules even when the pool is empty
+  if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 0 || !(sc->tol >= 0) || !(sc->cfl >= 0) ||
+      (sc->refine && !(sc->tol > 0)) || sc->smoother_fp16)
+    return set_err(CF_EINVAL, "bad schedule");
+  pl->sch = *sc;
+  pl->dirty = true;
+  return CF_OK;
+}
+
+int cf_pool_admit(cf_pool* pl, const cf_scene_desc* d, int* slot) {
+  if (!pl || !d || !slot) return set_err(CF_EINVAL, "NULL argument");
+  const uint64_t need = cf_scene_bytes(d->nx, d->ny, d->dx);
+  if (!need) return set_err(CF_EBADSIZE, "pool: unsupported grid %dx%d", d->nx, d->ny);
+  if (pl->used + need > pl->budget)
+    return set_err(CF_EAGAIN, "pool: %llu bytes needed, %llu of %llu in use", (unsigned long long)need,
+                   (unsigned long long)pl->used, (unsigned long long)pl->budget);
+  cf_sim* s = nullptr;
+  int rc = sim_create(d->nx, d->ny, d->dx, d->dt, pl->st, &s);
+  if (rc) return rc;
+  s->in_pool = true;
+  rc = cf_sim_set_schedule(s, &pl->sch);
+  if (rc) { cf_sim_destroy(s); return rc; }
+  int q = 0;
+  while (q < (int)pl->slots.size() && pl->slots[q]) q++;
+  if (q == (int)pl->slots.size()) pl->slots.push_back(nullptr);
+  pl->slots[q] = s;
+  pl->used += s->block_bytes;
+  pl->dirty = true;
+  *slot = q;
+  return CF_OK;
+}
+
+int cf_pool_scene(cf_pool* pl, int slot, cf_sim** sim) {
+  if (!pl || !sim) return set_err(CF_EINVAL, "NULL argument");
+  if (slot < 0 || slot >= (int)pl->slots.size() || !pl->slots[slot]) return set_err(CF_EINVAL, "pool: no scene in slot %d", slot);
+  *sim = pl->slots[slot];
+  return CF_OK;
+}
+
+int cf_pool_release(cf_pool* pl, int slot) {
+  if (!pl) return set_err(CF_EINVAL, "pool is NULL");
+  if (slot < 0 || slot >= (int)pl->slots.size() || !pl->slots[slot]) return set_err(CF_EINVAL, "pool: no scene in slot %d", slot);
+  HIPCHK(hipStreamSynchronize(pl->st));
+  pl->used -= pl->slots[slot]->block_bytes;
+  cf_sim_destroy(pl->slots[slot]);
+  pl->slots[slot] = nullptr;
+  pl->dirty = true;
+  return CF_OK;
+}
+
+int cf_pool_step(cf_pool* pl, int nsteps) {
+  if (!pl) return set_err(CF_EINVAL, "pool is NULL");
+  if (nsteps < 0) return set_err(CF_EINVAL, "nsteps < 0");
+  int live = 0;
+  for (auto* s : pl->slots) live += s ? 1 : 0;
+  if (!live || !nsteps) return CF_OK;
+  int rc = pool_ready(pl);
+  if (rc) return rc;
+  int q = 0;
+  if (pl->use_graph) {
+    for (; q + 8 <= nsteps; q += 8) HIPCHK(hipGraphLaunch(pl->g8, pl->st));
+    for (; q < nsteps; q++) HIPCHK(hipGraphLaunch(pl->g1, pl->st));
+  } else {
+    for (; q < nsteps; q++) pool_enqueue_step(pl);
+    HIPCHK(hipGetLastError());
+  }
+  for (auto* s : pl->slots)
+    if (s && s->sch.cfl <= 0) s->time += s->dt * nsteps;
+  pl->steps += nsteps;
+  return CF_OK;
+}
+
+// one app frame for every scene (js/main.js:266-283): parameters, geometry,
+// nsteps steps, radiation; geometry and radiation are batched too.  As with
+// cf_sim_frame, the caller advances each scene's frame counter.
+int cf_pool_frame(cf_pool* pl, int nsteps, int with_radiation) {
+  if (!pl) return set_err(CF_EINVAL, "pool is NULL");
+  std::vector<cf_sim*> sc;
+  for (auto* s : pl->slots) if (s) sc.push_back(s);
+  if (sc.empty()) return CF_OK;
+  for (auto* s : sc) { int rc = cf_sim_write_params(s); if (rc) return rc; }
+  // per-frame tables: rebuilt when the piece counts change
+  uint64_t key = 1469598103934665603ull;
+  for (auto* s : pl->slots) {
+    key = (key ^ (uint64_t)(uintptr_t)s) * 1099511628211ull;
+    if (s) key = (key ^ (uint64_t)s->P.nPieces) * 1099511628211ull;
+  }
+  if (key != pl->fkey || pl->frame_pre.empty()) {
+    HIPCHK(hipStreamSynchronize(pl->st));
+    pool_release_phases(pl->frame_pre);
+    pool_release_phases(pl->frame_post);
+    pl->frame_pre.resize(2);
+    pl->frame_post.resize(1);
+    for (auto* s : sc) {
+      cf::phase_fill_u32(pl->frame_pre[0], s->F.faceW, 0u, CF_MAX_PIECES * 4);
+      cf::phase_geometry(pl->frame_pre[1], s->F);
+      const int np = s->P.nPieces < CF_MAX_PIECES ? s->P.nPieces : CF_MAX_PIECES;
+      cf::phase_radiation(pl->frame_post[0], s->F, np * 4 + CF_N_FLOOR);
+    }
+    for (auto& a : pl->frame_pre) { hipError_t e = a.upload(pl->st); if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables"); }
+    for (auto& a : pl->frame_post) { hipError_t e = a.upload(pl->st); if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables"); }
+    pl->fkey = key;
+  }
+  for (auto& a : pl->frame_pre) a.run(pl->st);
+  int rc = cf_pool_step(pl, nsteps);
+  if (rc) return rc;
+  if (with_radiation) for (auto& a : pl->frame_post) a.run(pl->st);
+  HIPCHK(hipGetLastError());
+  return CF_OK;
+}
+
+int cf_pool_synchronize(cf_pool* pl) {
+  if (!pl) return set_err(CF_EINVAL, "pool is NULL");
+  HIPCHK(hipStreamSynchronize(pl->st));
+  return CF_OK;
+}
+
+int cf_pool_get_info(const cf_pool* pl, cf_pool_info* info) {
+  if (!pl || !info) return set_err(CF_EINVAL, "NULL argument");
+  memset(info, 0, sizeof(*info));
+  for (auto* s : pl->slots) {
+    if (!s) continue;
+    info->nscenes++;
+    info->cells += (uint64_t)s->nC;
+  }
+  info->nslots = (int32_t)pl->slots.size();
+  info->budget = pl->budget;
+  info->used = pl->used;
+  info->steps = pl->steps;
+  info->launches_per_step = pl->nlaunch;
+  return CF_OK;
+}
+
+int cf_pool_step_latencies(cf_pool* pl, int nsteps, float* ms) {
+  if (!pl || !ms || nsteps < 0) return set_err(CF_EINVAL, "bad argument");
+  int rc = pool_ready(pl);
+  if (rc) return rc;
+  std::vector<hipEvent_t> ev((size_t)nsteps + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], pl->st));
+  for (int q = 0; q < nsteps; q++) {
+    if (pl->use_graph) HIPCHK(hipGraphLaunch(pl->g1, pl->st));
+    else pool_enqueue_step(pl);
+    HIPCHK(hipEventRecord(ev[q + 1], pl->st));
+  }
+  HIPCHK(hipStreamSynchronize(pl->st));
+  for (int q = 0; q < nsteps; q++) hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]);
+  for (auto& e : ev) hipEventDestroy(e);
+  for (auto* s : pl->slots)
+    if (s && s->sch.cfl <= 0) s->time += s->dt * nsteps;
+  pl->steps += nsteps;
+  return CF_OK;
 }
 
 }  // extern "C"

@@ -33,9 +33,19 @@ class FluidSim:
         self.dx = float(dx) if dx is not None else SCENE_WIDTH / self.nx
         self.height = self.dx * self.ny
         self._h = None
+        self._owned = True
         self._pending = []
         if init:
             self.init()
+
+    @classmethod
+    def borrow(cls, handle, mode, dx):
+        """A view of a scene owned elsewhere (a pool slot): closing it does not
+        free the scene."""
+        self = cls(mode, dx, init=False)
+        self._h = handle
+        self._owned = False
+        return self
 
     # ---- lifetime ----------------------------------------------------------
     def init(self):
@@ -48,9 +58,9 @@ class FluidSim:
         return self
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             L.lib().cf_sim_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -262,6 +272,12 @@ class FluidSim:
         if a.nbytes != nb:
             raise ValueError("%s: %d bytes expected, got %d" % (name, nb, a.nbytes))
         check(L.lib().cf_sim_write_field(self._h, name.encode(), a.ctypes.data, a.nbytes), "write " + name)
+
+    def compat_sums(self):
+        """(net source sum S dx^2, net open-boundary outflow), FP64 sums on the device."""
+        a, b = C.c_double(), C.c_double()
+        check(L.lib().cf_sim_compat_sums(self._h, C.byref(a), C.byref(b)), "compat_sums")
+        return a.value, b.value
 
     def div_error(self):
         """(max, mean) of |div(f u) - S| (H.divErr, dev/gas_test.html:101-111)."""

@@ -182,3 +182,21 @@ def apply(sim, cfg):
     sim.upload_pieces(cfg["pieces"], cfg["n"])
     sim.write_params()
     return sim
+
+
+def config_batch(seed=1, n=64, width=1.28):
+    """n scenes with sizes drawn by seed from {256^2, 512^2, 1024^2} over the
+    same width (h = 5, 2.5, 1.25 mm), 2-12 seeded pieces each, a 1200 K
+    ignition disc at each pile; dt = 0.5 h / 2.5 m/s."""
+    g = PCG32(seed)
+    out = []
+    for q in range(n):
+        nx = (256, 512, 1024)[min(int(g.uniform(0, 3)), 2)]
+        h = width / nx
+        npc = 2 + min(int(g.uniform(0, 11)), 10)
+        pieces, k = pile(npc, seed * 1000 + q, width)
+        cx = float(np.mean(pieces.reshape(64, 20)[:k, 0]))
+        out.append({"nx": nx, "ny": nx, "dx": h, "dt": 0.5 * h / 2.5, "pieces": pieces, "n": k,
+                    "params": air_params(), "disc": (cx, 0.06, 0.03, 1200.0)})
+    return out
+

@@ -210,6 +210,48 @@ int  cf_sim_get_info(const cf_sim* sim, cf_sim_info* info);
  * dev/gas_test.html:101-111), computed on the device. */
 int  cf_sim_div_error(cf_sim* sim, float* max_err, float* mean_err);
 
+/* ---- scene pool: many scenes stepped together ------------------------------
+ * No counterpart in the reference (one FluidSim per page).  Scenes of any
+ * sizes live in one pool; each phase of a step is ONE launch over the tiles of
+ * every live scene, captured once as a graph.  A scene's results are the same
+ * alone and in any mix.  Scenes are admitted while their device memory fits
+ * the pool's HBM budget; each is a cf_sim (borrowed through cf_pool_scene) for
+ * parameters, pieces, fields and readback, on the pool's stream.  All scenes
+ * use the pool's schedule.  A pool is not thread-safe. */
+typedef struct cf_pool cf_pool;
+typedef struct cf_scene_desc {
+  int32_t nx, ny;
+  double dx, dt;
+} cf_scene_desc;
+typedef struct cf_pool_info {
+  int32_t nscenes, nslots;
+  uint64_t budget, used;   /* HBM bytes */
+  uint64_t cells;          /* cells of the live scenes */
+  int64_t steps;           /* pool steps taken */
+  int32_t launches_per_step;
+  int32_t pad_;
+} cf_pool_info;
+/* Device bytes a scene of this grid needs (0: not a supported grid). */
+uint64_t cf_scene_bytes(int nx, int ny, double dx);
+int  cf_pool_create(uint64_t hbm_budget, cf_pool** out);
+void cf_pool_destroy(cf_pool* pool);
+/* The schedule of every scene (cf_sim_set_schedule rules). */
+int  cf_pool_set_schedule(cf_pool* pool, const cf_schedule* sc);
+/* A new scene in the first free slot; CF_EAGAIN when it does not fit the budget. */
+int  cf_pool_admit(cf_pool* pool, const cf_scene_desc* desc, int* slot);
+/* The scene of a slot (owned by the pool: do not cf_sim_destroy it). */
+int  cf_pool_scene(cf_pool* pool, int slot, cf_sim** sim);
+/* Frees a slot and its device memory. */
+int  cf_pool_release(cf_pool* pool, int slot);
+/* Every live scene advances nsteps steps (asynchronous). */
+int  cf_pool_step(cf_pool* pool, int nsteps);
+/* One app frame for every scene: parameters, geometry, nsteps steps, radiation. */
+int  cf_pool_frame(cf_pool* pool, int nsteps, int with_radiation);
+int  cf_pool_synchronize(cf_pool* pool);
+int  cf_pool_get_info(const cf_pool* pool, cf_pool_info* info);
+/* Device time of each of nsteps pool steps (ms), events around every step. */
+int  cf_pool_step_latencies(cf_pool* pool, int nsteps, float* ms);
+
 /* ---- 1D premixed flame (tests/flame1d.mjs:13-71) ---------------------------
  * flameSpeed({B, Ta, dx, F, phi, L, tEnd}) on the GPU, one workgroup per case,
  * all cases in one launch.  The strip has n = Math.round(L/dx) cells (at most
@@ -229,6 +271,12 @@ typedef struct cf_flame_result {
  * Synchronous; returns CF_EBADSIZE when a case needs more than CF_FLAME_MAXN
  * or fewer than 2 cells. */
 int  cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_result* out);
+
+/* Divergence compatibility, summed on the device in FP64: the net source of
+ * the open cells (sum S dx^2, m^2/s) and the net outflow through the open
+ * domain faces (dx sum f u.n).  Equal after a converged projection.  No
+ * counterpart in the reference. */
+int  cf_sim_compat_sums(cf_sim* sim, double* source, double* outflow);
 
 /* Thread-local text of the last error. */
 const char* cf_last_error(void);

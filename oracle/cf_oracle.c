@@ -716,6 +716,57 @@ static void k_coarsest(S* s, int l, int iters) {
     for (int c = 0; c < 2; c++) k_relax(s, l, c);
 }
 
+/* the build's schedule option coarse_iters = 0: a direct FP64 solve of the
+ * coarsest level, A p = -rhs h2 with A the relaxation's operator (diag on
+ * the diagonal, minus the coefficients to existing neighbours, a closed cell
+ * an identity row), by dense Cholesky */
+static void coarse_direct(S* s, int l) {
+  const int nx = s->lnx[l], ny = s->lny[l], N = nx * ny;
+  const REAL *ax = s->ax[l], *ay = s->ay[l];
+  double* A = calloc((size_t)N * N, sizeof(double));
+  double* y = calloc((size_t)N, sizeof(double));
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int k = j * nx + i;
+      REAL off, diag;
+      stencil(s, l, s->p[l], i, j, &off, &diag);
+      if (diag > K(0)) {
+        A[(size_t)k * N + k] = (double)diag;
+        if (i > 0) A[(size_t)k * N + k - 1] = -(double)ax[j * (nx + 1) + i];
+        if (i < nx - 1) A[(size_t)k * N + k + 1] = -(double)ax[j * (nx + 1) + i + 1];
+        if (j > 0) A[(size_t)k * N + k - nx] = -(double)ay[j * nx + i];
+        if (j < ny - 1) A[(size_t)k * N + k + nx] = -(double)ay[(j + 1) * nx + i];
+        y[k] = -(double)s->rhs[l][k] * (double)s->lh2[l];
+      } else {
+        A[(size_t)k * N + k] = 1.0;
+      }
+    }
+  for (int c = 0; c < N; c++) {            /* A = L L^T in the lower triangle */
+    double d = A[(size_t)c * N + c];
+    for (int k = 0; k < c; k++) d -= A[(size_t)c * N + k] * A[(size_t)c * N + k];
+    d = sqrt(d);
+    A[(size_t)c * N + c] = d;
+    for (int r = c + 1; r < N; r++) {
+      double v = A[(size_t)r * N + c];
+      for (int k = 0; k < c; k++) v -= A[(size_t)r * N + k] * A[(size_t)c * N + k];
+      A[(size_t)r * N + c] = v / d;
+    }
+  }
+  for (int r = 0; r < N; r++) {            /* L z = y */
+    double v = y[r];
+    for (int k = 0; k < r; k++) v -= A[(size_t)r * N + k] * y[k];
+    y[r] = v / A[(size_t)r * N + r];
+  }
+  for (int r = N - 1; r >= 0; r--) {       /* L^T x = z */
+    double v = y[r];
+    for (int k = r + 1; k < N; k++) v -= A[(size_t)k * N + r] * y[k];
+    y[r] = v / A[(size_t)r * N + r];
+  }
+  for (int k = 0; k < N; k++) s->p[l][k] = (REAL)y[k];
+  free(A);
+  free(y);
+}
+
 /* level-0 relative residual ||r||2 / ||rhs||2 (convergence test of the
  * converged-multigrid schedule; the reference runs a fixed single V-cycle) */
 static double resid_norm(const S* s) {
@@ -765,7 +816,8 @@ static void vcycle(S* s, int nu) {
     for (int q = 0; q < nu; q++) { k_relax(s, l, 0); k_relax(s, l, 1); }
     k_restrictRes(s, l);
   }
-  k_coarsest(s, L - 1, s->sch.coarse_iters > 0 ? s->sch.coarse_iters : 4000);
+  if (s->sch.coarse_iters > 0) k_coarsest(s, L - 1, s->sch.coarse_iters);
+  else coarse_direct(s, L - 1);
   for (int l = L - 2; l >= 0; l--) {
     k_prolong(s, l);
     for (int q = 0; q < nu; q++) { k_relax(s, l, 1); k_relax(s, l, 0); }

@@ -1,0 +1,126 @@
+"""Precision study of one 1024^2 step: each multigrid schedule against an
+FP64 oracle step from the same state.
+
+The scene (the seeded 12-piece pile) is warmed up for `warm` steps on the GPU
+with the reference schedule; its state is then copied into the FP64 oracle,
+which takes one step with V-cycles to ||r||/||rhs|| < 1e-10 and an exact
+coarsest solve.  From the same state the GPU takes one step with each
+schedule, and the per-field relative L2 error against the FP64 step is
+reported for U, V, T, the four Y species, p and the dilatation S.
+
+    python tests/precision_study.py --size 1024 --warm 2000
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from campfire_amd import scenes  # noqa: E402
+
+STATE = ["T", "T2", "Th", "Y", "Y2", "Yh", "U", "U2", "Uh", "V", "V2", "Vh", "SQ", "cellG", "fx", "fy", "faceW",
+         "p", "acc", "rad"]
+
+SCHEDULES = {
+    "fp32_reference": {"nu": 2, "vcycles": 1, "tol": 0.0, "coarse_iters": 120},
+    "fp32_dc_1e-6": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 120, "refine": 1},
+    "fp32_dc_1e-6_direct_coarse": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 0, "refine": 1},
+}
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+
+
+def split(fields):
+    """The compared quantities from a field dict (Y as 4 species, S from SQ)."""
+    Y = fields["Y"].reshape(-1, 4)
+    return {"U": fields["U"], "V": fields["V"], "T": fields["T"], "Y_F": Y[:, 0], "Y_O": Y[:, 1], "Y_P": Y[:, 2],
+            "Y_S": Y[:, 3], "p": fields["p"], "S": fields["SQ"].reshape(-1, 2)[:, 0]}
+
+
+def study(size=1024, warm=2000, seed=1, log=None):
+    from oracle.pyoracle import Oracle
+    from campfire_amd.fluid import FluidSim
+    log = log or (lambda *a: None)
+    cfg = scenes.config_pile_1024(seed)
+    if size != 1024:
+        h = cfg["dx"] * 1024 / size
+        cfg.update(nx=size, ny=size, dx=h, dt=0.5 * h / 2.5)
+        cfg["pieces"], cfg["n"] = scenes.pile(12, seed, size * h)
+    g = FluidSim({"nx": cfg["nx"], "ny": cfg["ny"], "dt": cfg["dt"]}, dx=cfg["dx"])
+    scenes.apply(g, cfg)
+    t0 = time.perf_counter()
+    for f in range(warm // 8):
+        g.frame = f
+        g.frame_all(8, True)
+        g.readback_now()
+    g.write_params()
+    g.synchronize()
+    log("[precision] %dx%d warmed %d steps in %.1f s" % (cfg["nx"], cfg["ny"], warm, time.perf_counter() - t0))
+    snap = {k: g.read_field(k) for k in STATE}
+    out = {"grid": [cfg["nx"], cfg["ny"]], "warm_steps": warm, "errors": {}, "vcycles": {}, "resid": {},
+           "compat": {}}
+    # the FP64 step
+    o = Oracle(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dt"], fp64=True)
+    scenes.apply(o, cfg)
+    for k, v in snap.items():
+        o.write64(k, v.astype(np.float64)) if k not in ("faceW", "acc") else o.write(k, v)
+    o.write_params()
+    o.set_schedule(nu=2, vcycles=200, tol=1e-10, coarse_iters=0)
+    t0 = time.perf_counter()
+    o.step(1)
+    ref = {k: o.read64(k) for k in ("T", "Y", "U", "V", "p", "SQ")}
+    log("[precision] FP64 oracle step: %d V-cycles, residual %.2e, %.1f s"
+        % (o.last_vcycles, o.last_resid, time.perf_counter() - t0))
+    R = split(ref)
+    # the FP32 oracle with the reference schedule: FP32 arithmetic alone
+    o32 = Oracle(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dt"])
+    scenes.apply(o32, cfg)
+    for k, v in snap.items():
+        o32.write(k, v)
+    o32.write_params()
+    o32.step(1)
+    F = split({k: o32.read(k) for k in ("T", "Y", "U", "V", "p", "SQ")})
+    out["errors"]["oracle_fp32_reference"] = {k: rel(F[k], R[k]) for k in R}
+    # the GPU schedules
+    for name, sch in SCHEDULES.items():
+        for k, v in snap.items():
+            g.write_field(k, v)
+        g.set_schedule(**sch)
+        g.write_params()
+        g.step(1)
+        info = g.info()
+        G = split({k: g.read_field(k) for k in ("T", "Y", "U", "V", "p", "SQ")})
+        out["errors"][name] = {k: rel(G[k], R[k]) for k in R}
+        out["vcycles"][name] = int(info.last_vcycles)
+        out["resid"][name] = float(info.last_resid) if sch.get("tol", 0) > 0 else None
+        src, outf = g.compat_sums()
+        out["compat"][name] = {"source": src, "outflow": outf, "rel_gap": abs(src - outf) / max(abs(src), 1e-300)}
+        log("[precision] %-28s %s" % (name, " ".join("%s %.1e" % (k, v) for k, v in out["errors"][name].items())))
+    g.set_schedule()
+    g.close()
+    o.close()
+    o32.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--warm", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args()
+    res = study(a.size, a.warm, a.seed, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

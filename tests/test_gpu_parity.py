@@ -289,3 +289,23 @@ def test_steps_512_stagewise_then_statistics():
     ag, ao = g.read_field("acc"), o.read("acc")
     _flush("  HRR", ag[512], ao[512])
     assert abs(int(ag[512]) - int(ao[512])) <= 0.01 * abs(int(ao[512]))
+
+
+@pytest.mark.parametrize("nx,ny", [(96, 80), (256, 256)])
+def test_direct_coarsest_vcycle_matches_oracle(nx, ny):
+    """One V-cycle with the FP64 direct coarsest solve (coarse_iters = 0)."""
+    _flush("[parity] direct coarsest", nx, ny)
+    g, o, pieces, n = make_pair(nx, ny, 0.004, 0.002, seed=23, npieces=4)
+    sync_geometry(g, o)
+    for q in (g, o):
+        q.set_schedule(nu=2, vcycles=1, coarse_iters=0)
+    g.run_stage(4)
+    o.run_stage(4)
+    g.run_stage(5)
+    o.run_stage(5)
+    for name in ["p", "p1"]:
+        r = rel_l2(g.read_field(name), o.read(name))
+        _flush("  %s rel_l2=%.3g" % (name, r))
+        assert r <= 1e-6, name
+    g.close()
+    o.close()

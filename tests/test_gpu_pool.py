@@ -1,0 +1,129 @@
+"""Scenes stepped together in a ScenePool: each scene's fields, accumulators
+and radiation are bitwise what the same scene gives alone; admission follows
+the HBM budget."""
+import sys
+
+import numpy as np
+import pytest
+
+from campfire_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["T", "Y", "U", "V", "p", "SQ", "rad", "faceW"]
+
+
+def _flush(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _setup(sim, nx, ny, dx, seed, npieces):
+    pieces, n = scenes.pile(npieces, seed, nx * dx, spread=nx * dx / 3)
+    sim.reset()
+    sim.upload_pieces(pieces, n)
+    st = scenes.random_state(nx, ny, seed, vmax=1.5)
+    for name in ("T", "T2", "Th"):
+        sim.write_field(name, st["T"])
+    for name in ("Y", "Y2", "Yh"):
+        sim.write_field(name, st["Y"])
+    for name in ("U", "U2", "Uh"):
+        sim.write_field(name, st["U"])
+    for name in ("V", "V2", "Vh"):
+        sim.write_field(name, st["V"])
+    sim.write_params()
+
+
+SPECS = [(96, 80, 0.004, 0.0016, 3, 3), (256, 256, 0.003, 0.0006, 4, 6), (160, 224, 0.0035, 0.0007, 5, 4)]
+
+
+def _compare_with_alone(schedule, frames, steps):
+    from campfire_amd.fluid import FluidSim
+    from campfire_amd.pool import ScenePool
+    pool = ScenePool(8 << 30)
+    if schedule:
+        pool.set_schedule(**schedule)
+    views = []
+    for (nx, ny, dx, dt, seed, npc) in SPECS:
+        slot, sim = pool.admit(nx, ny, dx, dt)
+        _setup(sim, nx, ny, dx, seed, npc)
+        views.append(sim)
+    for f in range(frames):
+        pool.frame(steps, True)
+    pool.synchronize()
+    _flush("[pool] %d scenes, %d launches per step" % (pool.info().nscenes, pool.info().launches_per_step))
+    for (nx, ny, dx, dt, seed, npc), v in zip(SPECS, views):
+        a = FluidSim({"nx": nx, "ny": ny, "dt": dt}, dx=dx)
+        if schedule:
+            a.set_schedule(**schedule)
+        _setup(a, nx, ny, dx, seed, npc)
+        for f in range(frames):
+            a.frame_all(steps, True)
+        a.synchronize()
+        for name in FIELDS:
+            x, y = v.read_field(name), a.read_field(name)
+            same = np.array_equal(x, y)
+            if not same:
+                _flush("  %dx%d %s differs: max|d| %.3g" % (nx, ny, name, np.max(np.abs(x.astype(np.float64) - y))))
+            assert same, (nx, ny, name)
+        ra, rv = a.readback_now(), v.readback_now()
+        assert np.array_equal(ra["acc"], rv["acc"]), (nx, ny)
+        _flush("  %dx%d: identical to the scene alone (T max %.1f K)" % (nx, ny, a.read_field("T").max()))
+        a.close()
+    pool.close()
+
+
+def test_batched_equals_alone_reference_schedule():
+    _compare_with_alone(None, frames=2, steps=8)
+
+
+def test_batched_equals_alone_cfl_tolerance_schedule():
+    _compare_with_alone({"nu": 2, "vcycles": 30, "tol": 1e-6, "coarse_iters": 120, "cfl": 0.5, "umin": 1.0,
+                         "refine": 1}, frames=1, steps=4)
+
+
+def test_admission_follows_budget():
+    from campfire_amd.pool import ScenePool, scene_bytes
+    b256, b512 = scene_bytes(256, 256, 0.005), scene_bytes(512, 512, 0.0025)
+    _flush("[pool] scene bytes 256^2 %d, 512^2 %d" % (b256, b512))
+    assert 0 < b256 < b512
+    pool = ScenePool(b512 + b256 + 1024)
+    s1 = pool.admit(512, 512, 0.0025, 5e-4)
+    s2 = pool.admit(256, 256, 0.005, 1e-3)
+    assert s1 is not None and s2 is not None
+    assert pool.admit(256, 256, 0.005, 1e-3) is None          # over budget
+    assert pool.info().used == b512 + b256
+    pool.release(s1[0])
+    s3 = pool.admit(256, 256, 0.005, 1e-3)
+    assert s3 is not None and s3[0] == s1[0]                  # the freed slot is reused
+    pool.step(2)
+    pool.synchronize()
+    assert pool.info().nscenes == 2
+    pool.close()
+
+
+def test_mixed_batch_of_64_runs():
+    from campfire_amd.pool import ScenePool
+    pool = ScenePool(64 << 30)
+    g = scenes.PCG32(7)
+    sizes = []
+    for q in range(64):
+        n = [256, 512, 1024][int(g.uniform(0, 3)) % 3]
+        h = 1.28 / n
+        slot, sim = pool.admit(n, n, h, 0.5 * h / 2.5)
+        npc = 2 + int(g.uniform(0, 11)) % 11
+        pieces, k = scenes.pile(npc, 100 + q, 1.28)
+        sim.reset()
+        sim.upload_pieces(pieces, k)
+        sim.write_params()
+        sizes.append(n)
+    pool.frame(4, True)
+    pool.synchronize()
+    info = pool.info()
+    _flush("[pool] 64 scenes (%d x 256^2, %d x 512^2, %d x 1024^2), %.1f M cells, %.2f GB, %d launches per step"
+           % (sizes.count(256), sizes.count(512), sizes.count(1024), info.cells / 1e6, info.used / 1e9,
+              info.launches_per_step))
+    for slot in (0, 31, 63):
+        T = pool.scene(slot).read_field("T")
+        assert np.all(np.isfinite(T))
+    pool.close()

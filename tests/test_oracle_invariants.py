@@ -6,6 +6,7 @@ import math
 import numpy as np
 import pytest
 
+from campfire_amd import scenes
 from oracle.pyoracle import Oracle
 
 TAMB = 293.15
@@ -570,3 +571,27 @@ def test_radiation_temporal_smoothing():
     o3.radiation()
     fresh = o3.read64("rad").reshape(-1, 4)[:64, 0]
     assert np.allclose(q2, 0.65 * q1 + 0.35 * fresh, rtol=1e-12)   # mix(old, new, 0.35)
+
+
+def test_direct_coarsest_solve_is_exact():
+    """coarse_iters = 0 solves the coarsest level by Cholesky in FP64: on a
+    16x16 grid the coarsest level is the whole problem, so one V-cycle leaves
+    a residual at round-off."""
+    o = Oracle(16, 16, 0.01, 0.002, fp64=True)
+    o.reset()
+    pieces, n = scenes.pile(2, 9, 0.16, spread=0.05)
+    o.upload_pieces(pieces, n)
+    o.write_params()
+    o.clear_geometry()
+    o.geometry()
+    st = scenes.random_state(16, 16, 9)
+    for name in ("T", "Y", "U", "V"):
+        o.write(name, st[name])
+    o.run_stage(4)
+    o.set_schedule(nu=2, vcycles=1, tol=1e-30, coarse_iters=0)
+    o.run_stage(5)
+    assert o.last_resid < 1e-12, o.last_resid
+    o.set_schedule(nu=2, vcycles=1, tol=1e-30, coarse_iters=120)
+    o.write("p", np.zeros(256))
+    o.run_stage(5)
+    assert o.last_resid > 1e-6   # 120 sweeps alone are not exact
