@@ -114,6 +114,7 @@ def lib():
         "cf_sim_write_field": ([p, cp, vp, sz], i),
         "cf_sim_get_info": ([p, C.POINTER(CfSimInfo)], i),
         "cf_sim_div_error": ([p, C.POINTER(f), C.POINTER(f)], i),
+        "cf_sim_copy_state": ([p, p], i),
         "cf_sim_compat_sums": ([p, C.POINTER(d), C.POINTER(d)], i),
         "cf_flame1d_run": ([C.POINTER(CfFlameCase), i, i, C.POINTER(CfFlameResult)], i),
         "cf_scene_bytes": ([i, i, C.c_double], C.c_uint64),

@@ -1,0 +1,84 @@
+"""Autotuner of the multigrid schedule for residual-controlled stepping.
+
+For a mix of scene sizes it builds a probe pool (one seeded burning scene
+per size), warms it up, snapshots every scene on the device, and then, from
+the same snapshot, times a few batched steps under each candidate schedule
+(smoother sweeps nu, coarsest solve: direct FP64 or a number of red-black
+sweeps).  The fastest schedule whose every step reaches the tolerance wins.
+The choice depends on the hardware and the mix, which is why it is measured
+rather than fixed.
+
+    best, table = tune(sizes=(256, 512, 1024), tol=1e-6)
+    pool.set_schedule(**best)
+"""
+import time
+
+import numpy as np
+
+from . import scenes
+from .fluid import FluidSim
+from .pool import ScenePool, scene_bytes
+
+CANDIDATES = [{"nu": nu, "coarse_iters": ci} for nu in (1, 2, 3) for ci in (0, 24, 120)]
+
+
+def probe_config(n, seed, width=1.28):
+    h = width / n
+    pieces, k = scenes.pile(8, seed, width)
+    cx = float(np.mean(pieces.reshape(64, 20)[:k, 0]))
+    return {"nx": n, "ny": n, "dx": h, "dt": 0.5 * h / 2.5, "pieces": pieces, "n": k,
+            "params": scenes.air_params(), "disc": (cx, 0.06, 0.03, 1200.0)}
+
+
+def tune(sizes=(256, 512, 1024), tol=1e-6, refine=1, cfl=0.5, umin=1.0, vcycles=80, warm=24, steps=6, seed=1,
+         candidates=None, log=None):
+    """Returns (best schedule dict, [(schedule, ms per step, converged, V-cycles per scene)])."""
+    log = log or (lambda *a: None)
+    candidates = candidates or CANDIDATES
+    base = {"vcycles": vcycles, "tol": tol, "refine": refine, "cfl": cfl, "umin": umin}
+    cfgs = [probe_config(n, seed + q) for q, n in enumerate(sizes)]
+    pool = ScenePool(sum(2 * scene_bytes(c["nx"], c["ny"], c["dx"]) for c in cfgs) + (64 << 20))
+    pool.set_schedule(nu=2, coarse_iters=120, **base)
+    probes, twins = [], []
+    for c in cfgs:
+        slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
+        scenes.apply(sim, c)
+        probes.append(sim)
+    pool.frame(warm, True)
+    pool.synchronize()
+    for c, sim in zip(cfgs, probes):
+        t = FluidSim({"nx": c["nx"], "ny": c["ny"], "dt": c["dt"]}, dx=c["dx"])
+        t.copy_state_from(sim)
+        twins.append(t)
+    table = []
+    for cand in candidates:
+        sched = dict(base, **cand)
+        pool.set_schedule(**sched)
+        for sim, t in zip(probes, twins):
+            sim.copy_state_from(t)
+        pool.step(1)                       # builds and captures the batched step
+        for sim, t in zip(probes, twins):
+            sim.copy_state_from(t)
+        ok = True
+        cycles = []
+        t0 = time.perf_counter()
+        elapsed = 0.0
+        for q in range(steps):
+            t0 = time.perf_counter()
+            pool.step(1)
+            pool.synchronize()
+            elapsed += time.perf_counter() - t0
+            infos = [sim.info() for sim in probes]
+            ok = ok and all(i.last_resid < tol for i in infos)
+            cycles.append([int(i.last_vcycles) for i in infos])
+        ms = elapsed * 1e3 / steps
+        table.append((sched, ms, ok, cycles))
+        log("[autotune] nu=%d coarse=%-3s %.3f ms/step %s cycles %s"
+            % (sched["nu"], sched["coarse_iters"] if sched["coarse_iters"] else "LU", ms, "ok" if ok else "NOT CONVERGED",
+               cycles[-1]))
+    for t in twins:
+        t.close()
+    pool.close()
+    good = [r for r in table if r[2]]
+    best = min(good, key=lambda r: r[1])[0] if good else None
+    return best, table

@@ -1104,6 +1104,24 @@ int cf_sim_div_error(cf_sim* s, float* max_err, float* mean_err) {
   return CF_OK;
 }
 
+int cf_sim_copy_state(cf_sim* dst, const cf_sim* src) {
+  if (!dst || !src) return set_err(CF_EINVAL, "NULL argument");
+  if (dst->nx != src->nx || dst->ny != src->ny || dst->block_bytes != src->block_bytes)
+    return set_err(CF_EINVAL, "copy_state: grids differ (%dx%d, %dx%d)", src->nx, src->ny, dst->nx, dst->ny);
+  HIPCHK(hipStreamSynchronize(src->st));
+  HIPCHK(hipMemcpyAsync(dst->block, src->block, src->block_bytes, hipMemcpyDeviceToDevice, dst->st));
+  // the host mirror of what the copied parameter block holds
+  memcpy(dst->pv, src->pv, sizeof(dst->pv));
+  dst->P = src->P;
+  dst->dsub = src->dsub;
+  dst->diffSub = src->diffSub;
+  dst->dt = src->dt;
+  dst->time = src->time;
+  dst->frame = src->frame;
+  HIPCHK(hipStreamSynchronize(dst->st));
+  return CF_OK;
+}
+
 int cf_sim_compat_sums(cf_sim* s, double* source, double* outflow) {
   if (!s || !source || !outflow) return set_err(CF_EINVAL, "NULL argument");
   const unsigned nb = cf::compat_blocks(s->F);

@@ -273,6 +273,10 @@ class FluidSim:
             raise ValueError("%s: %d bytes expected, got %d" % (name, nb, a.nbytes))
         check(L.lib().cf_sim_write_field(self._h, name.encode(), a.ctypes.data, a.nbytes), "write " + name)
 
+    def copy_state_from(self, other):
+        """Every device buffer and parameter of `other` (same grid) into this scene."""
+        check(L.lib().cf_sim_copy_state(self._h, other._h), "copy_state")
+
     def compat_sums(self):
         """(net source sum S dx^2, net open-boundary outflow), FP64 sums on the device."""
         a, b = C.c_double(), C.c_double()

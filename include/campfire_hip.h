@@ -278,6 +278,10 @@ int  cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_r
  * counterpart in the reference. */
 int  cf_sim_compat_sums(cf_sim* sim, double* source, double* outflow);
 
+/* Copies every device buffer and parameter of src into dst (same grid): a
+ * snapshot, or a restore of one.  No counterpart in the reference. */
+int  cf_sim_copy_state(cf_sim* dst, const cf_sim* src);
+
 /* Thread-local text of the last error. */
 const char* cf_last_error(void);
 /* Library version string. */
