@@ -1,6 +1,7 @@
 // cf_kernels.h — device buffers of one scene and the kernel launchers.
 #pragma once
 #include <hip/hip_ext.h>
+#include <hip/hip_fp16.h>
 #include <string.h>
 #include <vector>
 #include "cf_device.h"
@@ -121,6 +122,7 @@ struct Dyn {
   int32_t converged;   // V-cycle loop: relative residual below tol this step
   int32_t cycles;      // V-cycles run this step
   float resid;         // relative residual after the last V-cycle
+  float sig, isig;     // FP16 smoother: p is stored as p * sig (a power of two), read as h * isig
   double time;         // simulated time accumulated by the CFL kernel
 };
 
@@ -144,6 +146,28 @@ struct Level {
   int nx, ny;
   float h2;
   float *p, *rhs, *ax, *ay;
+  // levels smoothed by tiled legs: p between the down and the up leg.  A
+  // tiled leg reads p over its tile plus a halo that neighbouring tiles own,
+  // so it must not write p in place: a neighbour that starts late would read
+  // smoothed values in its halo.  The down leg writes q, the up leg reads q
+  // and writes p.
+  float* q;
+};
+
+// FP16 storage of a level for the FP16 smoother (schedule smoother_fp16 = 1):
+// p as p * dyn->sig, the face coefficients as they are.  The level's FP32
+// arrays stay: rhs is always FP32, and the FP32 coefficients feed the
+// coefficient restriction and the FP64 defect.
+struct LevelH {
+  __half *p, *ax, *ay;
+  __half* q;  // as Level::q
+};
+
+// storage of the two levels a tiled multigrid leg touches
+struct MgStore {
+  int mode;       // 0: both FP32; 1: fine FP16, coarse FP32; 3: both FP16
+  LevelH f, c;    // FP16 arrays of the fine and the coarse level (mode bits 0 and 1)
+  const Dyn* dyn; // sig, isig (mode != 0)
 };
 
 // advection in one pass (cf_advect.hip); the two launches below are the
@@ -167,8 +191,10 @@ int forces_coef_cascade(int nlev);
 void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* part, int nparts, hipStream_t s);
 // fused multigrid (cf_mg.hip); skip (may be null): the launch does nothing
 // once *skip is non-zero (the residual target of the step has been met)
-bool launch_mg_down(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s);
-bool launch_mg_up(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s);
+bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
+bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
+// FP16 copies of a level's face coefficients (once per step, FP16 smoother)
+void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s);
 // first level of the single-workgroup bottom solver (the last 1-3 levels)
 int mg_bottom_first(const Level* L, int nlev);
 // coarse_coef: restrict the coarsest level's coefficients inside the kernel
@@ -183,11 +209,12 @@ void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s)
 // ||r|| / ||rhs|| of level 0 after a V-cycle into dyn->resid (FP64 sums);
 // sets dyn->converged when below tol.  part: 2 doubles per workgroup
 unsigned resnorm_blocks(const Level& L0);
-// defect correction at level 0: r = b - A p64 (FP64) into L0.rhs, L0.p = 0,
-// ||r||/||b|| into dyn; then, after a V-cycle on the correction, p64 += L0.p
-void launch_dc_defect(const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol, bool first,
-                      hipStream_t s);
-void launch_dc_accum(const Level& L0, double* p64, const int* skip, hipStream_t s);
+// defect correction at level 0: r = b - A p64 (FP64) into L0.rhs, L0.p = 0
+// (ph0 non-null: its FP16 p instead), ||r||/||b|| and the FP16 scale into
+// dyn; then, after a V-cycle on the correction, p64 += L0.p (or ph0 * isig)
+void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
+                      bool first, hipStream_t s);
+void launch_dc_accum(const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip, hipStream_t s);
 void launch_p32_to_64(const float* p, double* p64, int n, hipStream_t s);
 void launch_resnorm(const Level& L0, Dyn* dyn, double* part, float tol, hipStream_t s);
 void launch_coef_restrict(const Level& f, const Level& c, hipStream_t s);
@@ -217,15 +244,16 @@ void phase_forces_coef(BatchPhase& ph, const Fields& F, const Level* L, int nlev
 void phase_coef_restrict(BatchPhase& ph, const Level& f, const Level& c);
 void phase_cfl(BatchPhase& ph, const Fields& F, float cfl, float umin, int nsub);
 void phase_resnorm(BatchPhase& ph, const Level& L0, Dyn* dyn, double* part, float tol);
-void phase_dc_defect(BatchPhase& ph, const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
-                     bool first);
-void phase_dc_accum(BatchPhase& ph, const Level& L0, double* p64, const int* skip);
+void phase_dc_defect(BatchPhase& ph, const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part,
+                     float tol, bool first);
+void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip);
+void phase_coef_half(BatchPhase& ph, const Level& L, const LevelH& H);
 void phase_project(BatchPhase& ph, const Fields& F, const double* p64);
 void phase_geometry(BatchPhase& ph, const Fields& F);
 void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets);
 void phase_fill_u32(BatchPhase& ph, void* a, uint32_t v, size_t n);
-bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip);
-bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip);
+bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip);
+bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip);
 bool phase_mg_bottom(BatchPhase& ph, const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters,
                      const int* skip, double* band);
 

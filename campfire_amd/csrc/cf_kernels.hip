@@ -724,6 +724,27 @@ __global__ __launch_bounds__(256) void k_coef_restrict(const CoefRestrictOp::Arg
   CoefRestrictOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// FP16 copies of one level's face coefficients for the FP16 smoother
+// (round to nearest even; a face stays closed, 0, exactly)
+struct CoefHalfOp {
+  struct Args {
+    Level L;
+    LevelH H;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& L = a_.L;
+    int i, j;
+    tile_ij<32, 8>(vb, L.nx + 1, L.ny + 1, i, j);
+    if (i <= L.nx && j < L.ny) a_.H.ax[j * (L.nx + 1) + i] = __float2half(L.ax[j * (L.nx + 1) + i]);
+    if (i < L.nx && j <= L.ny) a_.H.ay[j * L.nx + i] = __float2half(L.ay[j * L.nx + i]);
+  }
+};
+__global__ __launch_bounds__(256) void k_coef_half(const CoefHalfOp::Args a_) {
+  CoefHalfOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // 5-point stencil: sum a*p_nb and the diagonal (:77-85)
 __device__ __forceinline__ void stencil(const Level& L, const float* __restrict__ p, int i, int j, float& off, float& diag) {
   const int nx = L.nx, ny = L.ny;
@@ -928,6 +949,7 @@ __device__ __forceinline__ double residual64(const Level& L, const double* __res
 struct DcDefectOp {
   struct Args {
     Level L;
+    __half* ph0;  // FP16 smoother: the correction's start e = 0 goes here instead of L.p
     double* p64;
     float* b0;
     Dyn* d;
@@ -957,7 +979,8 @@ struct DcDefectOp {
     const double b = (double)bf;
     const double r = residual64(L, p64, k % L.nx, k / L.nx, b);
     L.rhs[k] = (float)r;
-    L.p[k] = K(0);
+    if (a_.ph0) a_.ph0[k] = __float2half(0.f);
+    else L.p[k] = K(0);
     rr += r * r;
     bb += b * b;
   }
@@ -991,6 +1014,17 @@ struct DcDefectOp {
     rr = ((wr[0] + wr[1]) + wr[2]) + wr[3];
     bb = ((wb[0] + wb[1]) + wb[2]) + wb[3];
     const double r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+    // FP16 smoother: p is stored scaled by the power of two that brings the
+    // rms of this residual into [0.5, 1), so the correction's values sit in
+    // the normal range of FP16 (the scaling itself is exact)
+    int ex = 0;
+    const double rms = sqrt(rr / ((double)L.nx * (double)L.ny));
+    if (rms > 0) {
+      frexp(rms, &ex);
+      ex = ex < -60 ? -60 : (ex > 60 ? 60 : ex);
+    }
+    d->sig = (float)ldexp(1.0, -ex);
+    d->isig = (float)ldexp(1.0, ex);
     d->resid = (float)r;
     if (!first) d->cycles += 1;
     if (r < (double)tol) d->converged = 1;
@@ -1007,6 +1041,8 @@ __global__ __launch_bounds__(256) void k_dc_defect(const DcDefectOp::Args a_) {
 struct DcAccumOp {
   struct Args {
     Level L;
+    const __half* ph0;  // FP16 smoother: e is ph0 * isig
+    const Dyn* dyn;
     double* p64;
     const int* skip;
     int vg;  // tiles of this scene's launch
@@ -1019,7 +1055,12 @@ struct DcAccumOp {
 
   if (*skip) return;
   const int n = L.nx * L.ny;
-  for (int k = vb * 256 + threadIdx.x; k < n; k += a_.vg * 256) p64[k] += (double)L.p[k];
+  if (a_.ph0) {
+    const float is = a_.dyn->isig;
+    for (int k = vb * 256 + threadIdx.x; k < n; k += a_.vg * 256) p64[k] += (double)(__half2float(a_.ph0[k]) * is);
+  } else {
+    for (int k = vb * 256 + threadIdx.x; k < n; k += a_.vg * 256) p64[k] += (double)L.p[k];
+  }
 
   }
 };
@@ -1531,15 +1572,21 @@ void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s)
   if (b > 1024) b = 1024;
   launch(k_cfl, dim3(b), dim3(256), 0, s, CflOp::Args{F, cfl, umin, nsub, (int)b});
 }
-void launch_dc_defect(const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol, bool first,
-                      hipStream_t s) {
+void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
+                      bool first, hipStream_t s) {
   const int nb = (int)resnorm_blocks(L0);
-  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, p64, b0, dyn, part, tol, first ? 1 : 0, nb});
+  launch(k_dc_defect, dim3(nb), dim3(256), 0, s,
+         DcDefectOp::Args{L0, ph0, p64, b0, dyn, part, tol, first ? 1 : 0, nb});
 }
-void launch_dc_accum(const Level& L0, double* p64, const int* skip, hipStream_t s) {
+void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s) {
+  const int nb = (int)nblk(L.nx + 1, L.ny + 1);
+  launch(k_coef_half, dim3(nb), B328, 0, s, CoefHalfOp::Args{L, H, nb});
+}
+void launch_dc_accum(const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip,
+                     hipStream_t s) {
   unsigned b = (unsigned)((L0.nx * L0.ny + 1023) / 1024);
   if (b > 2048) b = 2048;
-  launch(k_dc_accum, dim3(b), dim3(256), 0, s, DcAccumOp::Args{L0, p64, skip, (int)b});
+  launch(k_dc_accum, dim3(b), dim3(256), 0, s, DcAccumOp::Args{L0, ph0, dyn, p64, skip, (int)b});
 }
 void launch_p32_to_64(const float* p, double* p64, int n, hipStream_t s) {
   unsigned b = (unsigned)((n + 1023) / 1024);
@@ -1589,16 +1636,20 @@ void phase_resnorm(BatchPhase& ph, const Level& L0, Dyn* dyn, double* part, floa
   const int nb = (int)resnorm_blocks(L0);
   phase_add<ResnormOp>(ph, ResnormOp::Args{L0, dyn, part, tol, nb}, nb, dim3(256), "resnorm");
 }
-void phase_dc_defect(BatchPhase& ph, const Level& L0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
-                     bool first) {
+void phase_dc_defect(BatchPhase& ph, const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part,
+                     float tol, bool first) {
   const int nb = (int)resnorm_blocks(L0);
-  phase_add<DcDefectOp>(ph, DcDefectOp::Args{L0, p64, b0, dyn, part, tol, first ? 1 : 0, nb}, nb, dim3(256),
+  phase_add<DcDefectOp>(ph, DcDefectOp::Args{L0, ph0, p64, b0, dyn, part, tol, first ? 1 : 0, nb}, nb, dim3(256),
                         "dc_defect");
 }
-void phase_dc_accum(BatchPhase& ph, const Level& L0, double* p64, const int* skip) {
+void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip) {
   int b = (L0.nx * L0.ny + 1023) / 1024;
   if (b > 2048) b = 2048;
-  phase_add<DcAccumOp>(ph, DcAccumOp::Args{L0, p64, skip, b}, b, dim3(256), "dc_accum");
+  phase_add<DcAccumOp>(ph, DcAccumOp::Args{L0, ph0, dyn, p64, skip, b}, b, dim3(256), "dc_accum");
+}
+void phase_coef_half(BatchPhase& ph, const Level& L, const LevelH& H) {
+  const int nb = (int)nblk(L.nx + 1, L.ny + 1);
+  phase_add<CoefHalfOp>(ph, CoefHalfOp::Args{L, H, nb}, nb, B328, "coef_half");
 }
 void phase_project(BatchPhase& ph, const Fields& F, const double* p64) {
   const int nb = (int)nblk(F.nx + 1, F.ny + 1);

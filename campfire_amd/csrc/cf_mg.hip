@@ -29,17 +29,23 @@ struct PairCell {
   bool upd, in;
 };
 
-__device__ __forceinline__ PairCell pair_cell(const Level& f, int x, int y, int rx, int ry, int RX, int RY) {
+__device__ __forceinline__ float ldc(const float* a, int k) { return a[k]; }
+__device__ __forceinline__ float ldc(const __half* a, int k) { return __half2float(a[k]); }
+
+// CT: float, or __half for the FP16 smoother's coefficient copies
+template <class CT>
+__device__ __forceinline__ PairCell pair_cell(int nx, int ny, const CT* __restrict__ ax, const CT* __restrict__ ay,
+                                              const float* __restrict__ rhs, int x, int y, int rx, int ry, int RX,
+                                              int RY) {
   PairCell c;
-  const int nx = f.nx, ny = f.ny;
   c.in = x >= 0 && x < nx && y >= 0 && y < ny;
   c.upd = c.in && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
           !(ry == RY - 1 && y < ny - 1);
   c.w = c.e = c.s = c.n = c.diag = c.rhs = 0.f;
   c.nb = 0;
   if (c.in) {
-    const float w = f.ax[y * (nx + 1) + x], e = f.ax[y * (nx + 1) + x + 1];
-    const float s = f.ay[y * nx + x], n = f.ay[(y + 1) * nx + x];
+    const float w = ldc(ax, y * (nx + 1) + x), e = ldc(ax, y * (nx + 1) + x + 1);
+    const float s = ldc(ay, y * nx + x), n = ldc(ay, (y + 1) * nx + x);
     float diag = 0.f;  // shaders/pressure.wgsl:77-85, same accumulation order
     if (x > 0) diag += w; else diag += K(2.0) * w;
     if (x < nx - 1) diag += e; else diag += K(2.0) * e;
@@ -51,9 +57,29 @@ __device__ __forceinline__ PairCell pair_cell(const Level& f, int x, int y, int 
     c.s = y > 0 ? s : 0.f;
     c.n = y < ny - 1 ? n : 0.f;
     c.nb = (x > 0 ? 1 : 0) | (x < nx - 1 ? 2 : 0) | (y > 0 ? 4 : 0) | (y < ny - 1 ? 8 : 0);
-    c.rhs = f.rhs[y * nx + x];
+    c.rhs = rhs[y * nx + x];
   }
   return c;
+}
+
+// the fine level's cell (x, y) from the FP32 or the FP16 coefficients
+template <int M>
+__device__ __forceinline__ PairCell pair_cell_m(const Level& f, const MgStore& st, int x, int y, int rx, int ry, int RX,
+                                                int RY) {
+  if constexpr ((M & 1) != 0) return pair_cell(f.nx, f.ny, st.f.ax, st.f.ay, f.rhs, x, y, rx, ry, RX, RY);
+  else return pair_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, x, y, rx, ry, RX, RY);
+}
+
+// p of a level: FP32, or FP16 holding p * sig (read back as h * isig, exact)
+template <bool H>
+__device__ __forceinline__ float ldp(const float* p, const __half* ph, int k, float isig) {
+  if constexpr (H) return __half2float(ph[k]) * isig;
+  else return p[k];
+}
+template <bool H>
+__device__ __forceinline__ void stp(float* p, __half* ph, int k, float v, float sig) {
+  if constexpr (H) ph[k] = __float2half(v * sig);
+  else p[k] = v;
 }
 
 __device__ __forceinline__ PairCell pair_none() {
@@ -85,11 +111,15 @@ __device__ __forceinline__ void pairs_relax(const PairCell (&c)[PQ][2], const in
   }
 }
 
-template <int TX, int TY, int NU>
+// M (storage, MgStore::mode): bit 0, the fine level's p and coefficients in
+// FP16; bit 1, the coarse level's p in FP16.  Sweeps, residual and
+// restriction are FP32 arithmetic either way.
+template <int TX, int TY, int NU, int M>
 struct MgDownOp {
   struct Args {
     Level f;
     Level c;
+    MgStore st;
     const int* skip;
     int vg;  // tiles of this scene's launch
   };
@@ -112,6 +142,7 @@ struct MgDownOp {
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
   const int tid = threadIdx.x;
   const float h2 = f.h2;
+  const float sig = M ? a_.st.dyn->sig : 1.f, isig = M ? a_.st.dyn->isig : 1.f;
   PairCell cell[PQ][2];
   int base[PQ], par[PQ];
 #pragma unroll
@@ -122,10 +153,10 @@ struct MgDownOp {
     base[q] = ry * RX + rx;
     par[q] = (x + y) & 1;
     if (pi < NP) {
-      cell[q][0] = pair_cell(f, x, y, rx, ry, RX, RY);
-      cell[q][1] = pair_cell(f, x + 1, y, rx + 1, ry, RX, RY);
-      sp[base[q]] = cell[q][0].in ? f.p[y * nx + x] : 0.f;
-      sp[base[q] + 1] = cell[q][1].in ? f.p[y * nx + x + 1] : 0.f;
+      cell[q][0] = pair_cell_m<M>(f, a_.st, x, y, rx, ry, RX, RY);
+      cell[q][1] = pair_cell_m<M>(f, a_.st, x + 1, y, rx + 1, ry, RX, RY);
+      sp[base[q]] = cell[q][0].in ? ldp<(M & 1) != 0>(f.p, a_.st.f.p, y * nx + x, isig) : 0.f;
+      sp[base[q] + 1] = cell[q][1].in ? ldp<(M & 1) != 0>(f.p, a_.st.f.p, y * nx + x + 1, isig) : 0.f;
     } else {
       cell[q][0] = cell[q][1] = pair_none();
       base[q] = 0;
@@ -148,7 +179,7 @@ struct MgDownOp {
         const float pk = sp[ci];
         const float off = pair_off(a, sp, ci, RX);
         sres[(ry - H) * TX + (rx - H)] = a.diag <= K(0) ? K(0) : a.rhs - (off - a.diag * pk) / h2;
-        f.p[(y0 - H + ry) * nx + (x0 - H + rx)] = pk;
+        stp<(M & 1) != 0>(f.q, a_.st.f.q, (y0 - H + ry) * nx + (x0 - H + rx), pk, sig);
       }
     }
   }
@@ -161,22 +192,23 @@ struct MgDownOp {
       const float* r0 = sres + (2 * lj) * TX + 2 * li;
       const float r = r0[0] + r0[1] + r0[TX] + r0[TX + 1];
       c.rhs[J * c.nx + I] = K(0.25) * r;
-      c.p[J * c.nx + I] = K(0);
+      stp<(M & 2) != 0>(c.p, a_.st.c.p, J * c.nx + I, K(0), sig);
     }
   }
 
   }
 };
-template <int TX, int TY, int NU>
-__global__ __launch_bounds__(256) void k_mg_down(const typename MgDownOp<TX, TY, NU>::Args a_) {
-  MgDownOp<TX, TY, NU>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+template <int TX, int TY, int NU, int M>
+__global__ __launch_bounds__(256) void k_mg_down(const typename MgDownOp<TX, TY, NU, M>::Args a_) {
+  MgDownOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-template <int TX, int TY, int NU>
+template <int TX, int TY, int NU, int M>
 struct MgUpOp {
   struct Args {
     Level f;
     Level c;
+    MgStore st;
     const int* skip;
     int vg;  // tiles of this scene's launch
   };
@@ -199,6 +231,8 @@ struct MgUpOp {
   const int tid = threadIdx.x;
   const float h2 = f.h2;
   const int nxc = c.nx, nyc = c.ny;
+  const float sig = M ? a_.st.dyn->sig : 1.f, isig = M ? a_.st.dyn->isig : 1.f;
+  constexpr bool FH = (M & 1) != 0, CH = (M & 2) != 0;
   PairCell cell[PQ][2];
   int base[PQ], par[PQ];
 #pragma unroll
@@ -212,16 +246,19 @@ struct MgUpOp {
 #pragma unroll
       for (int sub = 0; sub < 2; sub++) {
         const int x = x0 - H + rx + sub;
-        cell[q][sub] = pair_cell(f, x, y, rx + sub, ry, RX, RY);
+        cell[q][sub] = pair_cell_m<M>(f, a_.st, x, y, rx + sub, ry, RX, RY);
         float v = 0.f;
         if (cell[q][sub].in) {
           // bilinear coarse correction, skipped in closed cells (:116-129)
           const int I = x / 2, J = y / 2;
           const int I2 = iclamp(I + ((x & 1) == 1 ? 1 : -1), 0, nxc - 1);
           const int J2 = iclamp(J + ((y & 1) == 1 ? 1 : -1), 0, nyc - 1);
-          const float cc = K(0.5625) * c.p[J * nxc + I] + K(0.1875) * (c.p[J * nxc + I2] + c.p[J2 * nxc + I]) +
-                           K(0.0625) * c.p[J2 * nxc + I2];
-          v = f.p[y * nx + x] + (cell[q][sub].diag > K(0) ? cc : K(0));
+          const float c00 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I, isig);
+          const float c10 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I2, isig);
+          const float c01 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I, isig);
+          const float c11 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I2, isig);
+          const float cc = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
+          v = ldp<FH>(f.q, a_.st.f.q, y * nx + x, isig) + (cell[q][sub].diag > K(0) ? cc : K(0));
         }
         sp[base[q] + sub] = v;
       }
@@ -242,15 +279,15 @@ struct MgUpOp {
       const int ci = base[q] + sub;
       const int rx = ci % RX, ry = ci / RX;
       if (cell[q][sub].in && rx >= H && rx < H + TX && ry >= H && ry < H + TY)
-        f.p[(y0 - H + ry) * nx + (x0 - H + rx)] = sp[ci];
+        stp<FH>(f.p, a_.st.f.p, (y0 - H + ry) * nx + (x0 - H + rx), sp[ci], sig);
     }
   }
 
   }
 };
-template <int TX, int TY, int NU>
-__global__ __launch_bounds__(256) void k_mg_up(const typename MgUpOp<TX, TY, NU>::Args a_) {
-  MgUpOp<TX, TY, NU>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+template <int TX, int TY, int NU, int M>
+__global__ __launch_bounds__(256) void k_mg_up(const typename MgUpOp<TX, TY, NU, M>::Args a_) {
+  MgUpOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---- bottom levels in one workgroup ------------------------------------------------
@@ -614,48 +651,57 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const MgBottomOp::Args a_) {
 // ---- launchers -------------------------------------------------------------------
 static constexpr int MG_TX = 32, MG_TY = 32;
 
+template <int TX, int TY, int NU, int M>
+static auto k_mg_down_of(MgDownOp<TX, TY, NU, M>) { return k_mg_down<TX, TY, NU, M>; }
+template <int TX, int TY, int NU, int M>
+static auto k_mg_up_of(MgUpOp<TX, TY, NU, M>) { return k_mg_up<TX, TY, NU, M>; }
+
 static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
   return (unsigned)(((nx + tx - 1) / tx) * ((ny + ty - 1) / ty));
 }
 
-bool launch_mg_down(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s) {
-  const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  switch (nu) {
-    case 1: launch((k_mg_down<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, MgDownOp<MG_TX, MG_TY, 1>::Args{f, c, skip, (int)nb}); return true;
-    case 2: launch((k_mg_down<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, MgDownOp<MG_TX, MG_TY, 2>::Args{f, c, skip, (int)nb}); return true;
-    case 3: launch((k_mg_down<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, MgDownOp<MG_TX, MG_TY, 3>::Args{f, c, skip, (int)nb}); return true;
-    default: return false;
-  }
+// dispatch on (nu, storage mode) to the instantiation
+template <template <int, int, int, int> class Op, class F>
+static bool mg_dispatch(int nu, int mode, F&& f) {
+#define CF_MG_CASE(NU, M) \
+  if (nu == NU && mode == M) { f(Op<MG_TX, MG_TY, NU, M>{}); return true; }
+  CF_MG_CASE(1, 0) CF_MG_CASE(2, 0) CF_MG_CASE(3, 0)
+  CF_MG_CASE(1, 1) CF_MG_CASE(2, 1) CF_MG_CASE(3, 1)
+  CF_MG_CASE(1, 3) CF_MG_CASE(2, 3) CF_MG_CASE(3, 3)
+#undef CF_MG_CASE
+  return false;
 }
 
-bool launch_mg_up(const Level& f, const Level& c, int nu, const int* skip, hipStream_t s) {
-  const unsigned nb = ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  switch (nu) {
-    case 1: launch((k_mg_up<MG_TX, MG_TY, 1>), dim3(nb), dim3(256), 0, s, MgUpOp<MG_TX, MG_TY, 1>::Args{f, c, skip, (int)nb}); return true;
-    case 2: launch((k_mg_up<MG_TX, MG_TY, 2>), dim3(nb), dim3(256), 0, s, MgUpOp<MG_TX, MG_TY, 2>::Args{f, c, skip, (int)nb}); return true;
-    case 3: launch((k_mg_up<MG_TX, MG_TY, 3>), dim3(nb), dim3(256), 0, s, MgUpOp<MG_TX, MG_TY, 3>::Args{f, c, skip, (int)nb}); return true;
-    default: return false;
-  }
-}
-
-bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip) {
+bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s) {
   const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  switch (nu) {
-    case 1: phase_add<MgDownOp<MG_TX, MG_TY, 1>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_down"); return true;
-    case 2: phase_add<MgDownOp<MG_TX, MG_TY, 2>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_down"); return true;
-    case 3: phase_add<MgDownOp<MG_TX, MG_TY, 3>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_down"); return true;
-    default: return false;
-  }
+  return mg_dispatch<MgDownOp>(nu, st.mode, [&](auto op) {
+    using Op = decltype(op);
+    launch(k_mg_down_of(op), dim3(nb), dim3(256), 0, s, typename Op::Args{f, c, st, skip, nb});
+  });
 }
 
-bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, int nu, const int* skip) {
+bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s) {
   const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  switch (nu) {
-    case 1: phase_add<MgUpOp<MG_TX, MG_TY, 1>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_up"); return true;
-    case 2: phase_add<MgUpOp<MG_TX, MG_TY, 2>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_up"); return true;
-    case 3: phase_add<MgUpOp<MG_TX, MG_TY, 3>>(ph, {f, c, skip, nb}, nb, dim3(256), "mg_up"); return true;
-    default: return false;
-  }
+  return mg_dispatch<MgUpOp>(nu, st.mode, [&](auto op) {
+    using Op = decltype(op);
+    launch(k_mg_up_of(op), dim3(nb), dim3(256), 0, s, typename Op::Args{f, c, st, skip, nb});
+  });
+}
+
+bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip) {
+  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
+  return mg_dispatch<MgDownOp>(nu, st.mode, [&](auto op) {
+    using Op = decltype(op);
+    phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(256), "mg_down");
+  });
+}
+
+bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip) {
+  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
+  return mg_dispatch<MgUpOp>(nu, st.mode, [&](auto op) {
+    using Op = decltype(op);
+    phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(256), "mg_up");
+  });
 }
 
 // the bottom kernel takes the last min(3, nlev) levels when each fits its slot

@@ -91,6 +91,7 @@ struct cf_sim {
   size_t block_bytes = 0;
   Fields F;
   Level L[MAXLEV];
+  cf::LevelH LH[MAXLEV] = {};  // FP16 smoother: p, ax, ay of the tiled levels (below mg_bottom)
   cf_params* dP = nullptr;
   cf::Dyn* dDyn = nullptr;   // control values, right after the parameter block
   double* dNorm = nullptr;   // residual-norm partials
@@ -160,14 +161,32 @@ static void mark(cf_sim* s, const char* tag) {
 // ---- step schedule ---------------------------------------------------------
 // one V-cycle (js/fluid.js:294-312); coarse_coef: the bottom kernel also
 // restricts the coarsest level's coefficients; skip: see launch_mg_down
+// storage of the levels a tiled leg at level l touches: FP32, or with the
+// FP16 smoother FP16 p and coefficients at every tiled level (the bottom
+// levels stay FP32)
+static cf::MgStore mg_store(const cf_sim* s, int l) {
+  cf::MgStore m{};
+  if (!s->sch.smoother_fp16) return m;
+  const bool ch = l + 1 < s->mg_bottom;
+  m.mode = 1 | (ch ? 2 : 0);
+  m.f = s->LH[l];
+  if (ch) m.c = s->LH[l + 1];
+  m.dyn = s->dDyn;
+  return m;
+}
+
 static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef) {
   const int L = s->nlev;
   if (!s->mg_simple) {
     // fused: tiled down legs, one LDS-resident workgroup for the bottom levels, tiled up legs
     const int b = s->mg_bottom;
-    for (int l = 0; l < b; l++) { cf::launch_mg_down(s->L[l], s->L[l + 1], nu, skip, s->st); mark(s, level_tag("mg_down", l)); }
+    for (int l = 0; l < b; l++) {
+      cf::launch_mg_down(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_down", l));
+    }
     cf::launch_mg_bottom(s->L, L, b, coarse_coef, nu, s->sch.coarse_iters, skip, s->dBand, s->st); mark(s, "mg_bottom");
-    for (int l = b - 1; l >= 0; l--) { cf::launch_mg_up(s->L[l], s->L[l + 1], nu, skip, s->st); mark(s, level_tag("mg_up", l)); }
+    for (int l = b - 1; l >= 0; l--) {
+      cf::launch_mg_up(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_up", l));
+    }
     return;
   }
   for (int l = 0; l < L - 1; l++) {
@@ -196,13 +215,20 @@ static void enqueue_mg(cf_sim* s, bool coarse_coef) {
   const bool tol = s->sch.tol > 0;
   const bool dc = tol && s->sch.refine;  // defect correction with the FP64 pressure
   const int* skip = tol ? &s->dDyn->converged : nullptr;
+  // the FP16 smoother (defect correction only) reads FP16 copies of the coefficients
+  __half* ph0 = s->sch.smoother_fp16 ? s->LH[0].p : nullptr;
+  if (ph0) for (int l = 0; l < s->mg_bottom; l++) { cf::launch_coef_half(s->L[l], s->LH[l], s->st); mark(s, "coef_half"); }
   if (tol) { cf::launch_fill_u32(&s->dDyn->converged, 0u, 2, s->st); mark(s, "mg_control"); }  // converged, cycles
-  if (dc) { cf::launch_dc_defect(s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, true, s->st); mark(s, "dc_defect"); }
+  if (dc) {
+    cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, true, s->st);
+    mark(s, "dc_defect");
+  }
   for (int c = 0; c < s->sch.vcycles; c++) {
     enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0);
     if (dc) {
-      cf::launch_dc_accum(s->L[0], s->dP64, skip, s->st); mark(s, "dc_accum");
-      cf::launch_dc_defect(s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, false, s->st); mark(s, "dc_defect");
+      cf::launch_dc_accum(s->L[0], ph0, s->dDyn, s->dP64, skip, s->st); mark(s, "dc_accum");
+      cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, false, s->st);
+      mark(s, "dc_defect");
     } else if (tol) {
       cf::launch_resnorm(s->L[0], s->dDyn, s->dNorm, s->sch.tol, s->st); mark(s, "resnorm");
     }
@@ -298,7 +324,7 @@ static void drop_graph(cf_sim* s) {
   if (s->gexec) hipGraphExecDestroy(s->gexec);
   if (s->gexec8) hipGraphExecDestroy(s->gexec8);
   s->gexec = s->gexec8 = nullptr;
-  for (int q = 0; q < 6; q++) s->gkey[q] = -1;
+  for (int q = 0; q < 10; q++) s->gkey[q] = -1;
 }
 
 static int capture(cf_sim* s, int nsteps, hipGraphExec_t* out) {
@@ -362,7 +388,8 @@ static bool lookup(cf_sim* s, const char* name, FieldRef& r) {
       {"Y", F.Y, nC * 16}, {"Yh", F.Yh, nC * 16}, {"Y2", F.Y2, nC * 16},
       {"SQ", F.SQ, nC * 8}, {"cellG", F.cellG, nC * 8}, {"fx", F.fx, nU * 4}, {"fy", F.fy, nV * 4},
       {"pieces", F.pieces, CF_MAX_PIECES * CF_PIECE_FLOATS * 4}, {"acc", F.acc, CF_ACC_LEN * 4},
-      {"rad", F.rad, CF_RAD_LEN * 16}, {"faceW", F.faceW, CF_MAX_PIECES * 4 * 4}};
+      {"rad", F.rad, CF_RAD_LEN * 16}, {"faceW", F.faceW, CF_MAX_PIECES * 4 * 4},
+      {"p64", s->dP64, nC * 8}};
   for (auto& e : t)
     if (!strcmp(e.n, name)) { r.ptr = e.p; r.bytes = e.b; return true; }
   const char* pre[4] = {"rhs", "ax", "ay", "p"};
@@ -494,10 +521,19 @@ static int pool_build_step(cf_pool* pl) {
   }
   // V-cycles; down leg d and up leg d hold level d of every scene that has one
   const bool tol = k.tol > 0, dc = tol && k.refine;
+  auto ph0 = [&](cf_sim* s) { return k.smoother_fp16 ? s->LH[0].p : (__half*)nullptr; };
+  if (k.smoother_fp16) {
+    for (int l = 0; l < MAXLEV; l++) {
+      cf::BatchPhase a;
+      for (auto* s : sc)
+        if (l < s->mg_bottom) cf::phase_coef_half(a, s->L[l], s->LH[l]);
+      if (!a.empty()) ph.push_back(a);
+    }
+  }
   if (tol) { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_fill_u32(a, &s->dDyn->converged, 0u, 2); }
   if (dc) {
     cf::BatchPhase& a = P();
-    for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, true);
+    for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], ph0(s), s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, true);
   }
   int maxb = 0;
   for (auto* s : sc) maxb = s->mg_bottom > maxb ? s->mg_bottom : maxb;
@@ -505,7 +541,8 @@ static int pool_build_step(cf_pool* pl) {
     for (int d = 0; d < maxb; d++) {
       cf::BatchPhase& a = P();
       for (auto* s : sc)
-        if (d < s->mg_bottom) cf::phase_mg_down(a, s->L[d], s->L[d + 1], k.nu, tol ? &s->dDyn->converged : nullptr);
+        if (d < s->mg_bottom)
+          cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
     }
     {
       cf::BatchPhase& a = P();
@@ -519,13 +556,17 @@ static int pool_build_step(cf_pool* pl) {
     for (int d = maxb - 1; d >= 0; d--) {
       cf::BatchPhase& a = P();
       for (auto* s : sc)
-        if (d < s->mg_bottom) cf::phase_mg_up(a, s->L[d], s->L[d + 1], k.nu, tol ? &s->dDyn->converged : nullptr);
+        if (d < s->mg_bottom)
+          cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
     }
     if (dc) {
-      { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_dc_accum(a, s->L[0], s->dP64, &s->dDyn->converged); }
       {
         cf::BatchPhase& a = P();
-        for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, false);
+        for (auto* s : sc) cf::phase_dc_accum(a, s->L[0], ph0(s), s->dDyn, s->dP64, &s->dDyn->converged);
+      }
+      {
+        cf::BatchPhase& a = P();
+        for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], ph0(s), s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, false);
       }
     } else if (tol) {
       cf::BatchPhase& a = P();
@@ -617,6 +658,9 @@ static int sim_levels(cf_sim* s, int nx, int ny, double dx, double dt) {
   }
   if (s->lnx[s->nlev - 1] * s->lny[s->nlev - 1] > 256)
     return set_err(CF_EBADSIZE, "coarsest level too large (%dx%d)", s->lnx[s->nlev - 1], s->lny[s->nlev - 1]);
+  for (int l = 0; l < s->nlev; l++) { s->L[l].nx = s->lnx[l]; s->L[l].ny = s->lny[l]; s->L[l].h2 = s->lh2[l]; }
+  // bottom solver: the last 1-3 levels (<= 4096, 1024, 256 cells) in one workgroup
+  s->mg_bottom = cf::mg_bottom_first(s->L, s->nlev);
   return CF_OK;
 }
 
@@ -641,6 +685,12 @@ static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
     const size_t lx = s->lnx[l], ly = s->lny[l];
     add((void**)&s->L[l].p, lx * ly * 4); add((void**)&s->L[l].rhs, lx * ly * 4);
     add((void**)&s->L[l].ax, (lx + 1) * ly * 4); add((void**)&s->L[l].ay, lx * (ly + 1) * 4);
+  }
+  for (int l = 0; l < s->mg_bottom; l++) {  // leg buffers and FP16 smoother storage of the tiled levels
+    const size_t lx = s->lnx[l], ly = s->lny[l];
+    add((void**)&s->L[l].q, lx * ly * 4);
+    add((void**)&s->LH[l].p, lx * ly * 2); add((void**)&s->LH[l].q, lx * ly * 2);
+    add((void**)&s->LH[l].ax, (lx + 1) * ly * 2); add((void**)&s->LH[l].ay, lx * (ly + 1) * 2);
   }
   const unsigned nparts = (unsigned)(((s->nx + 31) / 32) * ((s->ny + 7) / 8));
   add((void**)&s->dPart, (size_t)nparts * 8);
@@ -716,9 +766,6 @@ static int sim_create(int nx, int ny, double dx, double dt, hipStream_t shared, 
   s->dDyn = (cf::Dyn*)(s->dP + 1);
   F.dyn = s->dDyn;
   F.p0 = s->L[0].p; F.rhs0 = s->L[0].rhs; F.ax0 = s->L[0].ax; F.ay0 = s->L[0].ay;
-  for (int l = 0; l < s->nlev; l++) { s->L[l].nx = s->lnx[l]; s->L[l].ny = s->lny[l]; s->L[l].h2 = s->lh2[l]; }
-  // bottom solver: the last 1-3 levels (<= 4096, 1024, 256 cells) in one workgroup
-  s->mg_bottom = cf::mg_bottom_first(s->L, s->nlev);
   // device buffers start zeroed, as WebGPU buffers do
   cf::launch_fill_u32(s->block, 0u, total / 4, s->st);
 
@@ -834,8 +881,9 @@ int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
     return set_err(CF_EINVAL, "bad schedule (cfl >= 0; umin > 0 with a CFL step)");
   if (sc->refine && !(sc->tol > 0)) return set_err(CF_EINVAL, "bad schedule (refine needs tol > 0)");
   // switching defect correction on: the FP64 pressure starts from the FP32 one
+  if (sc->smoother_fp16 && (!sc->refine || s->mg_simple))
+    return set_err(CF_EINVAL, "FP16 smoother needs defect correction (refine = 1, tol > 0) and the fused path");
   if (sc->refine && !s->sch.refine) cf::launch_p32_to_64(s->L[0].p, s->dP64, s->nC, s->st);
-  if (sc->smoother_fp16) return set_err(CF_EINVAL, "FP16 smoother not available in this build");
   const bool cfl_changed = sc->cfl != s->sch.cfl || sc->umin != s->sch.umin;
   s->sch = *sc;
   // the diffusion substep count follows the largest dt of a CFL rule
@@ -985,7 +1033,6 @@ int cf_sim_profile_kernels(cf_sim* s, int nsteps, int max, char* names, float* m
 
 int cf_sim_step_latencies(cf_sim* s, int nsteps, float* ms) {
   if (!s || !ms || nsteps < 0) return set_err(CF_EINVAL, "bad argument");
-  if (s->sch.tol > 0) return set_err(CF_EINVAL, "latency timing needs a fixed V-cycle count");
   std::vector<hipEvent_t> ev((size_t)nsteps + 1);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   if (s->use_graph) {
@@ -997,8 +1044,9 @@ int cf_sim_step_latencies(cf_sim* s, int nsteps, float* ms) {
     if (s->use_graph) HIPCHK(hipGraphLaunch(s->gexec, s->st));
     else enqueue_step(s);
     HIPCHK(hipEventRecord(ev[q + 1], s->st));
-    s->time += s->dt;
+    if (s->sch.cfl <= 0) s->time += s->dt;
   }
+  s->last_vcycles = s->sch.vcycles;
   HIPCHK(hipStreamSynchronize(s->st));
   for (int q = 0; q < nsteps; q++) hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]);
   for (auto& e : ev) hipEventDestroy(e);
@@ -1207,12 +1255,12 @@ void cf_pool_destroy(cf_pool* pl) {
 
 int cf_pool_set_schedule(cf_pool* pl, const cf_schedule* sc) {
   if (!pl || !sc) return set_err(CF_EINVAL, "NULL argument");
-  for (auto* s : pl->slots)
-    if (s) { int rc = cf_sim_set_schedule(s, sc); if (rc) return rc; }
   // validate with a scene's rules even when the pool is empty
   if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 0 || !(sc->tol >= 0) || !(sc->cfl >= 0) ||
-      (sc->refine && !(sc->tol > 0)) || sc->smoother_fp16)
+      (sc->refine && !(sc->tol > 0)) || (sc->smoother_fp16 && !sc->refine))
     return set_err(CF_EINVAL, "bad schedule");
+  for (auto* s : pl->slots)
+    if (s) { int rc = cf_sim_set_schedule(s, sc); if (rc) return rc; }
   pl->sch = *sc;
   pl->dirty = true;
   return CF_OK;

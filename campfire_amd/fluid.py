@@ -135,7 +135,9 @@ class FluidSim:
         """Multigrid schedule (reference: nu 2, 1 V-cycle, 120 coarsest sweeps).
         tol > 0 runs up to `vcycles` V-cycles until ||r||/||rhs|| < tol; cfl > 0
         takes dt = cfl*dx/max(max|u|, umin) each step, on the device; refine = 1
-        keeps the level-0 pressure and residual in FP64 (defect correction)."""
+        keeps the level-0 pressure and residual in FP64 (defect correction);
+        smoother_fp16 = 1 (with refine) stores p and the face coefficients of
+        the tiled multigrid levels in FP16, arithmetic staying FP32."""
         s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine)
         check(L.lib().cf_sim_set_schedule(self._h, C.byref(s)), "set_schedule")
 
@@ -264,6 +266,12 @@ class FluidSim:
         return self.read_field(name).view(np.float32)
 
     readBuffer = read_buffer
+
+    def read_p64(self):
+        """The FP64 level-0 pressure of defect correction (refine = 1)."""
+        a = np.empty(self.nx * self.ny, np.float64)
+        check(L.lib().cf_sim_read_field(self._h, b"p64", a.ctypes.data, a.nbytes), "read p64")
+        return a
 
     def write_field(self, name, arr):
         nb = self.field_bytes(name)

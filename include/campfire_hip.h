@@ -81,7 +81,11 @@ typedef struct cf_schedule {
   float   tol;           /* relative residual target ||r||/||rhs||; 0 = run exactly `vcycles` */
   int32_t coarse_iters;  /* red-black sweeps on the coarsest level, ref 120;
                             0 = direct FP64 solve of the coarsest level */
-  int32_t smoother_fp16; /* 1 = FP16 smoother, FP32 residual + defect correction */
+  int32_t smoother_fp16; /* 1 = FP16 smoother: the multigrid levels above the
+                            single-workgroup bottom solver keep p (scaled by a
+                            power of two from the defect's rms) and the face
+                            coefficients in FP16; sweeps, residual and
+                            restriction compute in FP32.  Needs refine = 1. */
   float   cfl;           /* > 0: each step takes dt = cfl * dx / max(max|u|, umin),
                             computed on the device from the current U, V; 0 = fixed dt */
   float   umin;          /* velocity floor of the CFL rule (m/s) */
@@ -196,9 +200,10 @@ int  cf_sim_take_readback(cf_sim* sim, int slot, int wait,
 /* readBuffer(name)  (js/fluid.js:345-355) and the inverse (used by
  * H.blob, dev/gas_test.html:112-120).  Names: U Uh U2 V Vh V2 T Th T2 Y Yh Y2
  * SQ cellG fx fy pieces acc rad faceW and the level-0 MG arrays p rhs ax ay;
- * "p1".."p15", "rhs1".. for coarse levels.  Y* are vec4 AoS (16 B/cell), SQ and
- * cellG vec2 (8 B/cell), exactly the reference layouts.  `bytes` must equal the
- * buffer size (cf_sim_field_bytes). */
+ * "p1".."p15", "rhs1".. for coarse levels, and "p64", the FP64 level-0
+ * pressure of defect correction (8 B/cell).  Y* are vec4 AoS (16 B/cell), SQ
+ * and cellG vec2 (8 B/cell), exactly the reference layouts.  `bytes` must
+ * equal the buffer size (cf_sim_field_bytes). */
 int  cf_sim_field_bytes(const cf_sim* sim, const char* name, size_t* bytes);
 int  cf_sim_read_field(cf_sim* sim, const char* name, void* dst, size_t bytes);
 int  cf_sim_write_field(cf_sim* sim, const char* name, const void* src, size_t bytes);

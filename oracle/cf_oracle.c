@@ -27,6 +27,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "cf_oracle.h"
+#include "half.h"
 
 #ifdef CFO_DOUBLE
 typedef double REAL;
@@ -105,6 +106,7 @@ typedef struct {
   REAL *p[MAXLEV], *rhs[MAXLEV], *ax[MAXLEV], *ay[MAXLEV];
   double* p64;            /* defect correction: level-0 pressure in FP64 */
   REAL* b0;               /* defect correction: level-0 right-hand side */
+  float sig, isig;        /* FP16 smoother: p stored as p * sig, a power of two */
   cf_schedule sch;
   int last_vcycles;
   double last_resid;
@@ -806,11 +808,94 @@ static double dc_defect(S* s, int first) {
       s->p[0][k] = 0;
       rr += r * r; bb += b * b;
     }
+  {
+    /* FP16 smoother: the power of two bringing the residual's rms into [0.5, 1) */
+    int ex = 0;
+    const double rms = sqrt(rr / ((double)nx * (double)ny));
+    if (rms > 0) {
+      frexp(rms, &ex);
+      ex = ex < -60 ? -60 : (ex > 60 ? 60 : ex);
+    }
+    s->sig = (float)ldexp(1.0, -ex);
+    s->isig = (float)ldexp(1.0, ex);
+  }
   return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
 }
 
-/* vcycle (js/fluid.js:294-312) */
+#ifndef CFO_DOUBLE
+/* the first of the levels that the GPU's single-workgroup bottom solver
+ * takes: the coarsest (<= 256 cells) and, when they fit, the one or two above
+ * it (<= 1024 and <= 4096 cells).  The levels above it are smoothed by tiled
+ * legs; with the FP16 smoother those keep p and the coefficients in FP16. */
+static int mg_bottom_first(const S* s) {
+  const int cap[3] = {256, 1024, 4096};
+  int b = s->nlev - 1;
+  for (int d = 1; d < 3 && b - 1 >= 0; d++) {
+    if (s->lnx[b - 1] * s->lny[b - 1] > cap[d]) break;
+    b--;
+  }
+  return b;
+}
+
+/* p of level l through its FP16 storage: p * sig rounded to FP16, read back
+ * as h * isig */
+static void p_through_half(S* s, int l) {
+  const int n = s->lnx[l] * s->lny[l];
+  REAL* p = s->p[l];
+  for (int k = 0; k < n; k++) p[k] = cfo_h2f1(cfo_f2h1(p[k] * s->sig)) * s->isig;
+}
+
+static REAL* half_copy(const REAL* a, int n) {
+  REAL* c = malloc(sizeof(REAL) * (size_t)n);
+  for (int k = 0; k < n; k++) c[k] = cfo_h2f1(cfo_f2h1(a[k]));
+  return c;
+}
+#endif
+
+static void vcycle_ref(S* s, int nu);
+
+/* the build's schedule option smoother_fp16 (with defect correction): the
+ * V-cycle of js/fluid.js:294-312 where the levels above the bottom solver's
+ * keep p and the face coefficients in FP16 between legs.  Arithmetic stays
+ * FP32: each down leg sweeps, forms the residual and restricts it from the
+ * FP32 values, then stores p; each up leg reads p and the coarse p, adds the
+ * correction, sweeps and stores p. */
 static void vcycle(S* s, int nu) {
+#ifndef CFO_DOUBLE
+  const int b = s->sch.smoother_fp16 ? mg_bottom_first(s) : 0;
+  if (b > 0) {
+    REAL *ax[MAXLEV], *ay[MAXLEV];
+    for (int l = 0; l < b; l++) {
+      const int nx = s->lnx[l], ny = s->lny[l];
+      ax[l] = s->ax[l]; ay[l] = s->ay[l];
+      s->ax[l] = half_copy(ax[l], (nx + 1) * ny);
+      s->ay[l] = half_copy(ay[l], nx * (ny + 1));
+    }
+    const int L = s->nlev;
+    for (int l = 0; l < L - 1; l++) {
+      for (int q = 0; q < nu; q++) { k_relax(s, l, 0); k_relax(s, l, 1); }
+      k_restrictRes(s, l);
+      if (l < b) p_through_half(s, l);
+    }
+    if (s->sch.coarse_iters > 0) k_coarsest(s, L - 1, s->sch.coarse_iters);
+    else coarse_direct(s, L - 1);
+    for (int l = L - 2; l >= 0; l--) {
+      k_prolong(s, l);
+      for (int q = 0; q < nu; q++) { k_relax(s, l, 1); k_relax(s, l, 0); }
+      if (l < b) p_through_half(s, l);
+    }
+    for (int l = 0; l < b; l++) {
+      free(s->ax[l]); free(s->ay[l]);
+      s->ax[l] = ax[l]; s->ay[l] = ay[l];
+    }
+    return;
+  }
+#endif
+  vcycle_ref(s, nu);
+}
+
+/* vcycle (js/fluid.js:294-312) */
+static void vcycle_ref(S* s, int nu) {
   const int L = s->nlev;
   for (int l = 0; l < L - 1; l++) {
     for (int q = 0; q < nu; q++) { k_relax(s, l, 0); k_relax(s, l, 1); }
@@ -1107,6 +1192,7 @@ void* PFX(create)(int nx, int ny, double dx, double dt, int* err) {             
     1800, 5.670374e-8, 2.667, 0};
   memcpy(s->pv, def, sizeof(def));
   s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
+  s->sig = s->isig = 1.0f;
   s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0;
   s->diffSub = 1;
   return s;
@@ -1178,6 +1264,11 @@ int PFX(set_schedule)(void* h, const cf_schedule* sc) {
   if (sc->nu < 0 || sc->vcycles < 1 || sc->coarse_iters < 0) return CF_EINVAL;
   if (!(sc->cfl >= 0) || (sc->cfl > 0 && !(sc->umin > 0))) return CF_EINVAL;
   if (sc->refine && !(sc->tol > 0)) return CF_EINVAL;
+#ifdef CFO_DOUBLE
+  if (sc->smoother_fp16) return CF_EINVAL; /* an FP32-arithmetic option */
+#else
+  if (sc->smoother_fp16 && !sc->refine) return CF_EINVAL;
+#endif
   if (sc->refine && !s->sch.refine) for (int k = 0; k < s->nC; k++) s->p64[k] = (double)s->p[0][k];
   const int cfl_changed = sc->cfl != s->sch.cfl || sc->umin != s->sch.umin;
   s->sch = *sc;
@@ -1185,6 +1276,10 @@ int PFX(set_schedule)(void* h, const cf_schedule* sc) {
   return 0;
 }
 int PFX(last_vcycles)(void* h) { return ((S*)h)->last_vcycles; }
+#ifndef CFO_DOUBLE
+void cfo_f2h(const float* a, uint16_t* h, int n) { for (int k = 0; k < n; k++) h[k] = cfo_f2h1(a[k]); }
+void cfo_h2f(const uint16_t* h, float* a, int n) { for (int k = 0; k < n; k++) a[k] = cfo_h2f1(h[k]); }
+#endif
 double PFX(last_resid)(void* h) { return ((S*)h)->last_resid; }
 
 int PFX(reset)(void* h) {                                                         /* js/fluid.js:162-176 */
@@ -1386,6 +1481,7 @@ int PFX(read_field64)(void* h, const char* name, double* dst, size_t count) {
   S* s = (S*)h;
   if (!strcmp(name, "sdf")) { if (count != (size_t)s->nC) return CF_EINVAL; for (int k = 0; k < s->nC; k++) dst[k] = s->sdf[k]; return 0; }
   if (!strcmp(name, "rad")) { if (count != CF_RAD_LEN * 4) return CF_EINVAL; for (int q = 0; q < CF_RAD_LEN * 4; q++) dst[q] = s->rad[q]; return 0; }
+  if (!strcmp(name, "p64")) { if (count != (size_t)s->nC) return CF_EINVAL; memcpy(dst, s->p64, sizeof(double) * s->nC); return 0; }
   size_t c; int lev;
   REAL* f = field(s, name, &c, &lev);
   if (!f || count != c) return CF_EINVAL;

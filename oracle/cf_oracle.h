@@ -23,6 +23,9 @@ cfo_flame_result cfo_flame_speed(double B, double Ta, double dx, double F,
  * and 2000 K in the script), in double and in single precision. */
 cfo_flame_result cfo_flame_speed_ex(double B, double Ta, double dx, double F, double phi, double L, double tEnd,
                                     double Tamb, double Tign);
+/* IEEE binary16 conversions of the oracle (round to nearest even, subnormals kept) */
+void cfo_f2h(const float* a, uint16_t* h, int n);
+void cfo_h2f(const uint16_t* h, float* a, int n);
 cfo_flame_result cfo_flame_speed32_ex(double B, double Ta, double dx, double F, double phi, double L, double tEnd,
                                       double Tamb, double Tign);
 

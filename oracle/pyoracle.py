@@ -77,6 +77,10 @@ def lib():
                 getattr(L, p + n).argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]
             getattr(L, p + "closure").argtypes = [C.c_void_p, C.c_char_p, C.c_double]
             getattr(L, p + "closure").restype = C.c_double
+        L.cfo_f2h.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.cfo_f2h.restype = None
+        L.cfo_h2f.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.cfo_h2f.restype = None
         L.cfo_flame_speed.restype = FlameResult
         L.cfo_flame_speed.argtypes = [C.c_double] * 7
         for n in ("cfo_flame_speed_ex", "cfo_flame_speed32_ex"):
@@ -84,6 +88,22 @@ def lib():
             getattr(L, n).argtypes = [C.c_double] * 9
         _LIB = L
     return _LIB
+
+
+def f2h(a):
+    """float32 -> IEEE binary16 bits (uint16) by the oracle's conversion."""
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    h = np.zeros(a.size, np.uint16)
+    lib().cfo_f2h(a.ctypes.data, h.ctypes.data, a.size)
+    return h
+
+
+def h2f(h):
+    """IEEE binary16 bits (uint16) -> float32 by the oracle's conversion."""
+    h = np.ascontiguousarray(h, dtype=np.uint16).reshape(-1)
+    a = np.zeros(h.size, np.float32)
+    lib().cfo_h2f(h.ctypes.data, a.ctypes.data, h.size)
+    return a
 
 
 def flame_speed(B, Ta, dx, F=1.0, phi=1.0, L=0.12, tEnd=0.25, Tamb=None, Tign=None, fp32=False):
@@ -217,7 +237,7 @@ class Oracle:
         nC, nU, nV = nx * ny, (nx + 1) * ny, nx * (ny + 1)
         tab = {"U": nU, "Uh": nU, "U2": nU, "V": nV, "Vh": nV, "V2": nV, "T": nC, "Th": nC, "T2": nC,
                "Y": 4 * nC, "Yh": 4 * nC, "Y2": 4 * nC, "SQ": 2 * nC, "cellG": 2 * nC, "fx": nU, "fy": nV,
-               "acc": 528, "faceW": 256, "pieces": 1280, "rad": 1280}
+               "acc": 528, "faceW": 256, "pieces": 1280, "rad": 1280, "p64": nC}
         if name in tab:
             return tab[name]
         lx, ly = nx, ny

@@ -33,6 +33,8 @@ SCHEDULES = {
     "fp32_reference": {"nu": 2, "vcycles": 1, "tol": 0.0, "coarse_iters": 120},
     "fp32_dc_120sweeps": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 120, "refine": 1},
     "fp32_dc_direct_coarse": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 0, "refine": 1},
+    "fp16_dc_direct_coarse": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 0, "refine": 1,
+                              "smoother_fp16": 1},
 }
 
 
@@ -71,7 +73,7 @@ def study(size=1024, warm=2000, seed=1, log=None):
     log("[precision] %dx%d warmed %d steps in %.1f s" % (cfg["nx"], cfg["ny"], warm, time.perf_counter() - t0))
     snap = {k: g.read_field(k) for k in STATE}
     out = {"grid": [cfg["nx"], cfg["ny"]], "warm_steps": warm, "errors": {}, "vcycles": {}, "resid": {},
-           "compat": {}}
+           "compat": {}, "ms_per_step": {}}
     # the FP64 step
     o = Oracle(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dt"], fp64=True)
     scenes.apply(o, cfg)
@@ -98,11 +100,16 @@ def study(size=1024, warm=2000, seed=1, log=None):
     F = split({k: o32.read(k) for k in ("T", "Y", "U", "V", "p", "SQ")})
     out["errors"]["oracle_fp32_reference"] = {k: rel(F[k], R[k]) for k in R}
     # the GPU schedules
-    for name, sch in SCHEDULES.items():
+    def restore(sch):
+        # the fields, and (through a schedule without refine) p64 from p
         for k, v in snap.items():
             g.write_field(k, v)
+        g.set_schedule()
         g.set_schedule(**sch)
         g.write_params()
+
+    for name, sch in SCHEDULES.items():
+        restore(sch)
         g.step(1)
         info = g.info()
         G = split({k: g.read_field(k) for k in ("T", "Y", "U", "V", "p", "SQ")})
@@ -111,7 +118,15 @@ def study(size=1024, warm=2000, seed=1, log=None):
         out["resid"][name] = float(info.last_resid) if sch.get("tol", 0) > 0 else None
         src, outf = g.compat_sums()
         out["compat"][name] = {"source": src, "outflow": outf, "rel_gap": abs(src - outf) / max(abs(src), 1e-300)}
-        log("[precision] %-28s %s" % (name, " ".join("%s %.1e" % (k, v) for k, v in out["errors"][name].items())))
+        # device time of the same step from the same state (median of 5 tries)
+        lat = []
+        for _ in range(5):
+            restore(sch)
+            lat.append(float(g.step_latencies(1)[0]))
+        out["ms_per_step"][name] = float(np.median(lat))
+        log("[precision] %-28s %s  %d cycles  %.3f ms" % (
+            name, " ".join("%s %.1e" % (k, v) for k, v in out["errors"][name].items()), out["vcycles"][name],
+            out["ms_per_step"][name]))
     g.set_schedule()
     g.close()
     o.close()

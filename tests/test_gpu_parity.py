@@ -182,11 +182,13 @@ def test_frames_stay_finite():
         assert np.all(r["rad"][:n * 16:4] > 0)
 
 
-@pytest.mark.parametrize("shape", [(160, 224), (96, 80), (72, 48), (256, 256)])
+@pytest.mark.parametrize("shape", [(160, 224), (96, 80), (72, 48), (256, 256), (1024, 1024)])
 def test_fused_vcycle_matches_reference_dispatches(shape):
     """The fused V-cycle (tiled down/up legs + LDS bottom solver) against the
     oracle's one-dispatch-per-half-sweep V-cycle, on shapes with partial tiles
-    and 4-6 levels."""
+    and 4-7 levels.  1024^2 has four tiled levels and 1024 tiles at level 0,
+    so workgroups start well after their neighbours finish: a leg that read
+    its halo from an array its neighbours write would show here."""
     nx, ny = shape
     _flush("[parity] fused V-cycle", shape)
     g, o, pieces, n = make_pair(nx, ny, 0.7 / nx, 0.002, seed=41, npieces=4)
@@ -307,5 +309,46 @@ def test_direct_coarsest_vcycle_matches_oracle(nx, ny):
         r = rel_l2(g.read_field(name), o.read(name))
         _flush("  %s rel_l2=%.3g" % (name, r))
         assert r <= 1e-6, name
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("nx,ny,ntiled", [(128, 128, 1), (256, 192, 2), (512, 512, 3)])
+def test_fp16_smoother_vcycle_matches_oracle(nx, ny, ntiled):
+    """The FP16 smoother (p and coefficients of the tiled levels in FP16, FP32
+    sweeps, FP64 defect correction) against the oracle's restatement: the
+    correction of one V-cycle, then a solve to 1e-6.  One, two and three
+    tiled levels above the bottom solver's."""
+    _flush("[parity] fp16 smoother", nx, ny)
+    g, o, pieces, n = make_pair(nx, ny, 0.7 / nx, 0.002, seed=29, npieces=5)
+    sync_geometry(g, o)
+    put(g, o, "SQ", np.random.default_rng(3).uniform(-0.5, 0.5, nx * ny * 2).astype(np.float32))
+    g.run_stage(4)
+    o.run_stage(4)
+    levels = g.levels
+    assert sum(1 for (lx, ly) in levels if lx * ly > 4096) == ntiled
+    p0 = np.random.default_rng(4).uniform(-1, 1, nx * ny).astype(np.float32)
+    put(g, o, "p", p0)
+    for q in (g, o):  # switching refine on starts p64 from p
+        q.set_schedule(nu=2, vcycles=1, tol=1e-30, coarse_iters=0, refine=1, smoother_fp16=1)
+    g.run_stage(5)
+    o.run_stage(5)
+    e_g = g.read_p64() - p0
+    e_o = o.read64("p64") - p0
+    r = rel_l2(e_g, e_o)
+    _flush("  one cycle: correction rel_l2=%.3g, resid %.3g / %.3g" % (r, g.info().last_resid, o.last_resid))
+    assert r <= 1e-5
+    assert abs(g.info().last_resid - o.last_resid) <= 1e-4 * o.last_resid
+    # the rest of the solve to 1e-6 (both start from their p64 after one cycle)
+    for q in (g, o):
+        q.set_schedule(nu=2, vcycles=40, tol=1e-6, coarse_iters=0, refine=1, smoother_fp16=1)
+    g.run_stage(5)
+    o.run_stage(5)
+    ig = g.info()
+    _flush("  solve: cycles %d / %d, resid %.3g / %.3g" % (ig.last_vcycles, o.last_vcycles, ig.last_resid,
+                                                        o.last_resid))
+    assert ig.last_resid < 1e-6 and o.last_resid < 1e-6
+    assert abs(ig.last_vcycles - o.last_vcycles) <= 1
+    assert rel_l2(g.read_p64(), o.read64("p64")) <= 1e-6
     g.close()
     o.close()

@@ -29,3 +29,9 @@ def test_precision_study_1024():
     # the GPU's FP32 reference step is the FP32 oracle's to round-off
     for k in ("T", "Y_F", "Y_O", "U", "V"):
         assert abs(e["fp32_reference"][k] - e["oracle_fp32_reference"][k]) <= 0.1 * e["oracle_fp32_reference"][k] + 1e-7
+    # the FP16 smoother with FP64 defect correction meets the same target:
+    # fields as close to the FP64 step as the FP32-smoother variant's
+    assert res["resid"]["fp16_dc_direct_coarse"] < 1e-6
+    for k in ("U", "V", "T", "Y_F", "Y_O", "Y_P", "Y_S", "p", "S"):
+        assert e["fp16_dc_direct_coarse"][k] <= 2 * e["fp32_dc_direct_coarse"][k] + 1e-6, k
+    assert res["compat"]["fp16_dc_direct_coarse"]["rel_gap"] < 1e-6

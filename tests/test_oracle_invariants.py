@@ -383,9 +383,9 @@ def _poisson_direct(o, l=0):
     return spl.spsolve(A.tocsr(), b).reshape(ny, nx)
 
 
-def _projection_case(nx=32, ny=32, seed=5):
+def _projection_case(nx=32, ny=32, seed=5, fp64=True):
     dx, dt = 0.004, 1e-3
-    o = Oracle(nx, ny, dx, dt, fp64=True)
+    o = Oracle(nx, ny, dx, dt, fp64=fp64)
     o.reset()
     o.upload_pieces(_square(0.064, 0.03, 0.012, angle=0.3), 1)
     o.write_params()
@@ -595,3 +595,71 @@ def test_direct_coarsest_solve_is_exact():
     o.write("p", np.zeros(256))
     o.run_stage(5)
     assert o.last_resid > 1e-6   # 120 sweeps alone are not exact
+
+
+# ---------------- FP16 storage (the FP16 smoother) ----------------
+def test_half_conversion_matches_numpy_float16():
+    """The oracle's binary16 conversion (round to nearest even, subnormals,
+    overflow to inf) against numpy's: every half value back to float, and
+    random float bit patterns, boundary values and every tie between
+    adjacent halves forward."""
+    from oracle import pyoracle as po
+    h = np.arange(65536, dtype=np.uint16)
+    ref = h.view(np.float16).astype(np.float32)
+    got = po.h2f(h)
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), ref[~nan].view(np.uint32))
+    rng = np.random.default_rng(1)
+    f = rng.integers(0, 2 ** 32, 500_000, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    hv = np.arange(0, 0x7bff, dtype=np.uint16)
+    mid = ((hv.view(np.float16).astype(np.float64) + (hv + 1).view(np.float16).astype(np.float64)) / 2)
+    edge = np.array([65504, 65519.99, 65520, 65536, 6.1035156e-05, 5.96e-08, 2.98e-08, 2.99e-08, 1e-9,
+                     0.0, -0.0, np.inf, -np.inf], np.float32)
+    f = np.concatenate([f, edge, mid.astype(np.float32), -mid.astype(np.float32)])
+    with np.errstate(over="ignore", invalid="ignore"):
+        exp = f.astype(np.float16).view(np.uint16)
+    got = po.f2h(f)
+    fin = ~np.isnan(f)
+    assert np.array_equal(got[fin], exp[fin])
+    assert np.all((got[~fin] & 0x7c00) == 0x7c00) and np.all((got[~fin] & 0x3ff) != 0)
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_fp16_smoother_defect_correction_reaches_the_direct_solve(n):
+    """FP16 storage of p and the coefficients at the tiled levels (one at
+    128^2, two at 256^2), FP32 sweeps, FP64 defect correction: the FP64
+    pressure converges to the sparse direct solve of the level-0 system, the
+    coefficients come back unchanged, and the FP16 V-cycle's correction
+    differs from the FP32 V-cycle's by FP16 rounding, not more or less."""
+    o = _projection_case(n, n, fp64=False)
+    o.run_stage(4)
+    ref = _poisson_direct(o)
+    ax0, ay0 = o.read("ax").copy(), o.read("ay").copy()
+    p_start = o.read("p").copy()
+    o.set_schedule(nu=2, vcycles=60, tol=1e-9, coarse_iters=0, refine=1, smoother_fp16=1)
+    assert o.run_stage(5) == 0
+    cyc16 = o.last_vcycles
+    p64 = o.read64("p64").reshape(n, n)
+    assert np.max(np.abs(p64 - ref)) < 1e-6 * np.max(np.abs(ref))
+    assert np.array_equal(o.read("ax"), ax0) and np.array_equal(o.read("ay"), ay0)
+    # the same solve with the FP32 smoother: as many cycles, give or take two
+    o2 = _projection_case(n, n, fp64=False)
+    o2.run_stage(4)
+    o2.set_schedule(nu=2, vcycles=60, tol=1e-9, coarse_iters=0, refine=1)
+    assert o2.run_stage(5) == 0
+    assert o2.last_vcycles - 2 <= cyc16 <= o2.last_vcycles + 2, (o2.last_vcycles, cyc16)
+    # one correction V-cycle each from the same start: about 1 % apart (the
+    # FP16 rounding of p at 5e-4 relative, spread by the sweeps that follow);
+    # both cut the residual by well over 10x
+    outs = []
+    for fp16 in (0, 1):
+        o3 = _projection_case(n, n, fp64=False)
+        o3.run_stage(4)
+        o3.write("p", p_start)
+        o3.set_schedule(nu=2, vcycles=1, tol=1e-30, coarse_iters=0, refine=1, smoother_fp16=fp16)
+        o3.run_stage(5)
+        outs.append(o3.read64("p64") - p_start.astype(np.float64))
+        assert o3.last_resid < 0.05, (fp16, o3.last_resid)
+    d = np.linalg.norm(outs[1] - outs[0]) / np.linalg.norm(outs[0])
+    assert 1e-4 < d < 5e-2, d
