@@ -51,6 +51,9 @@ struct BatchPhase {
   void* dargs = nullptr;
   Item* ditems = nullptr;
   const char* tag = "";
+  // set when a scene of another kernel was offered: one launch runs one
+  // kernel over every item, so such a phase must not run
+  bool mixed = false;
 
   template <class A>
   void add(const A& a, int ntiles) {
@@ -77,7 +80,7 @@ struct BatchPhase {
     ditems = nullptr;
   }
   void run(hipStream_t s) const {
-    if (fn && !empty()) fn(*this, s);
+    if (fn && !empty() && !mixed) fn(*this, s);
   }
 };
 
@@ -90,7 +93,12 @@ void run_batch(const BatchPhase& ph, hipStream_t s) {
 // start a phase of Op (or check that it is one) and add one scene
 template <class Op>
 void phase_add(BatchPhase& ph, const typename Op::Args& a, int ntiles, dim3 block, const char* tag) {
-  ph.fn = &run_batch<Op>;
+  void (*fn)(const BatchPhase&, hipStream_t) = &run_batch<Op>;
+  if (ph.fn && ph.fn != fn) {
+    ph.mixed = true;
+    return;
+  }
+  ph.fn = fn;
   ph.block = block;
   ph.tag = tag;
   ph.add(a, ntiles);

@@ -537,12 +537,19 @@ static int pool_build_step(cf_pool* pl) {
   }
   int maxb = 0;
   for (auto* s : sc) maxb = s->mg_bottom > maxb ? s->mg_bottom : maxb;
+  // a leg's kernel depends on the storage of its two levels (FP16 smoother:
+  // whether the coarse level is tiled too), so each level's legs form one
+  // phase per storage mode
+  const int modes[3] = {0, 1, 3};
   for (int c = 0; c < k.vcycles; c++) {
     for (int d = 0; d < maxb; d++) {
-      cf::BatchPhase& a = P();
-      for (auto* s : sc)
-        if (d < s->mg_bottom)
-          cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+      for (int m : modes) {
+        cf::BatchPhase a;
+        for (auto* s : sc)
+          if (d < s->mg_bottom && mg_store(s, d).mode == m)
+            cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+        if (!a.empty()) ph.push_back(a);
+      }
     }
     {
       cf::BatchPhase& a = P();
@@ -554,10 +561,13 @@ static int pool_build_step(cf_pool* pl) {
       }
     }
     for (int d = maxb - 1; d >= 0; d--) {
-      cf::BatchPhase& a = P();
-      for (auto* s : sc)
-        if (d < s->mg_bottom)
-          cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+      for (int m : modes) {
+        cf::BatchPhase a;
+        for (auto* s : sc)
+          if (d < s->mg_bottom && mg_store(s, d).mode == m)
+            cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+        if (!a.empty()) ph.push_back(a);
+      }
     }
     if (dc) {
       {
@@ -574,6 +584,8 @@ static int pool_build_step(cf_pool* pl) {
     }
   }
   { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_project(a, s->F, dc ? s->dP64 : nullptr); }
+  for (auto& a : ph)
+    if (a.mixed) return set_err(CF_EINVAL, "pool: phase '%s' would run one kernel over scenes of another", a.tag);
   for (auto& a : ph) {
     hipError_t e = a.upload(pl->st);
     if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables: %s", hipGetErrorString(e));
