@@ -24,6 +24,20 @@ __device__ __forceinline__ int toint(float x) {
 }
 __device__ __forceinline__ int iround(float x) { return (int)rintf(x); }
 
+// x / d correctly rounded, given y = 1/d correctly rounded: Markstein's
+// correction step (P. Markstein, IBM J. Res.
+// Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point Arithmetic,
+// the theorem on division with a correctly rounded reciprocal).  q = x*y is
+// within one ulp of x/d, the FMA residual x - q*d is exact, and one more FMA
+// rounds to the nearest quotient.  Valid without underflow or overflow (the
+// sweeps' pressures and coefficients are far from both); three instructions
+// where a division takes eleven, for divisors reused many times.
+__device__ __forceinline__ float div_rcp(float x, float d, float y) {
+  const float q = x * y;
+  const float r = fmaf(-q, d, x);
+  return fmaf(r, y, q);
+}
+
 // x^1.5 as x*sqrt(x): within ~1 ulp of the correctly rounded power (WGSL's pow
 // promises no better), a tenth of the cost of powf, which dominated the
 // stencils that evaluate Sutherland's law five times per face

@@ -25,6 +25,7 @@ namespace cf {
 // inward one ring per half-sweep and stays out of the tile interior.
 struct PairCell {
   float w, e, s, n, diag, rhs;
+  float y;  // 1 / diag (correctly rounded; 0 for a closed cell)
   int nb;   // bit 0..3: neighbour west, east, south, north exists
   bool upd, in;
 };
@@ -41,7 +42,7 @@ __device__ __forceinline__ PairCell pair_cell(int nx, int ny, const CT* __restri
   c.in = x >= 0 && x < nx && y >= 0 && y < ny;
   c.upd = c.in && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
           !(ry == RY - 1 && y < ny - 1);
-  c.w = c.e = c.s = c.n = c.diag = c.rhs = 0.f;
+  c.w = c.e = c.s = c.n = c.diag = c.rhs = c.y = 0.f;
   c.nb = 0;
   if (c.in) {
     const float w = ldc(ax, y * (nx + 1) + x), e = ldc(ax, y * (nx + 1) + x + 1);
@@ -52,6 +53,7 @@ __device__ __forceinline__ PairCell pair_cell(int nx, int ny, const CT* __restri
     if (y > 0) diag += s;
     if (y < ny - 1) diag += n; else diag += K(2.0) * n;
     c.diag = diag;
+    c.y = diag > K(0) ? K(1) / diag : K(0);
     c.w = x > 0 ? w : 0.f;
     c.e = x < nx - 1 ? e : 0.f;
     c.s = y > 0 ? s : 0.f;
@@ -84,7 +86,7 @@ __device__ __forceinline__ void stp(float* p, __half* ph, int k, float v, float 
 
 __device__ __forceinline__ PairCell pair_none() {
   PairCell c;
-  c.w = c.e = c.s = c.n = c.diag = c.rhs = 0.f;
+  c.w = c.e = c.s = c.n = c.diag = c.rhs = c.y = 0.f;
   c.nb = 0;
   c.upd = c.in = false;
   return c;
@@ -106,7 +108,7 @@ __device__ __forceinline__ void pairs_relax(const PairCell (&c)[PQ][2], const in
     const PairCell& a = sub ? c[q][1] : c[q][0];
     const int ci = base[q] + sub;
     const float off = pair_off(a, sp, ci, RX);
-    const float v = a.diag > K(0) ? (off - a.rhs * h2) / a.diag : K(0);
+    const float v = a.diag > K(0) ? div_rcp(off - a.rhs * h2, a.diag, a.y) : K(0);
     if (a.upd) sp[ci] = v;
   }
 }
@@ -316,6 +318,7 @@ struct CellSet {
   int k[CPT];   // linear cell index (or -1)
   int nb[CPT];  // bits 0..3: west, east, south, north neighbour exists; bit 4: colour
   float w[CPT], e[CPT], s[CPT], n[CPT], diag[CPT];
+  float y[CPT];  // 1 / diag (0 for a closed cell)
 };
 
 // cell data of a level for the cells kidx[q] (ignored when outside the level);
@@ -327,7 +330,7 @@ __device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, cons
   for (int q = 0; q < CPT; q++) {
     const int k = kidx[q];
     C.k[q] = -1;
-    C.w[q] = C.e[q] = C.s[q] = C.n[q] = C.diag[q] = 0.f;
+    C.w[q] = C.e[q] = C.s[q] = C.n[q] = C.diag[q] = C.y[q] = 0.f;
     C.nb[q] = 0;
     if (k >= 0 && k < nx * ny) {
       const int i = k % nx, j = k / nx;
@@ -339,6 +342,7 @@ __device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, cons
       if (j < ny - 1) diag += n; else diag += K(2.0) * n;
       C.k[q] = k;
       C.diag[q] = diag;
+      C.y[q] = diag > K(0) ? K(1) / diag : K(0);
       C.w[q] = i > 0 ? w : 0.f;
       C.e[q] = i < nx - 1 ? e : 0.f;
       C.s[q] = j > 0 ? s : 0.f;
@@ -346,14 +350,6 @@ __device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, cons
       C.nb[q] = (i > 0 ? 1 : 0) | (i < nx - 1 ? 2 : 0) | (j > 0 ? 4 : 0) | (j < ny - 1 ? 8 : 0) | (((i + j) & 1) << 4);
     }
   }
-}
-
-template <int CPT>
-__device__ __forceinline__ void cells_load_strided(CellSet<CPT>& C, int nx, int ny, const float* ax, const float* ay) {
-  int kidx[CPT];
-#pragma unroll
-  for (int q = 0; q < CPT; q++) kidx[q] = threadIdx.x + BT * q;
-  cells_load<CPT>(C, nx, ny, ax, ay, kidx);
 }
 
 template <int CPT>
@@ -404,13 +400,62 @@ __device__ __forceinline__ void cells_relax_pairs(const CellSet<2 * NPAIR>& C, c
     const int k = sa ? C.k[a] : C.k[b], nb = sa ? C.nb[a] : C.nb[b];
     const float w = sa ? C.w[a] : C.w[b], e = sa ? C.e[a] : C.e[b];
     const float s = sa ? C.s[a] : C.s[b], n = sa ? C.n[a] : C.n[b];
-    const float diag = sa ? C.diag[a] : C.diag[b], r = sa ? rh[a] : rh[b];
+    const float diag = sa ? C.diag[a] : C.diag[b], r = sa ? rh[a] : rh[b], y = sa ? C.y[a] : C.y[b];
     if (k >= 0) {
       const float off = w * p[k - (nb & 1)] + e * p[k + ((nb >> 1) & 1)] + s * p[k - nx * ((nb >> 2) & 1)] +
                         n * p[k + nx * ((nb >> 3) & 1)];
-      p[k] = diag > K(0) ? (off - r) / diag : K(0);
+      p[k] = diag > K(0) ? div_rcp(off - r, diag, y) : K(0);
     }
   }
+}
+
+// One cell of the coarsest level's sweeps with everything in registers: the
+// LDS indices of its four neighbours (the cell itself where one is missing,
+// with a zero coefficient), the coefficients, the diagonal, its reciprocal
+// and rhs * h2.  A lane slot without a cell reads and writes index `dummy`
+// (LDS that the sweeps do not otherwise use) with zero coefficients, so the
+// update needs no branch.
+struct SweepCell {
+  int k, iw, ie, is, in;
+  float w, e, s, n, d, y, r;
+};
+
+template <int CPT>
+__device__ __forceinline__ SweepCell sweep_cell(const CellSet<CPT>& C, const float (&rh)[CPT], int q, int nx,
+                                                int dummy) {
+  SweepCell c;
+  const int k = C.k[q], b = C.nb[q];
+  const bool has = k >= 0;
+  c.k = has ? k : dummy;
+  c.iw = has ? k - (b & 1) : dummy;
+  c.ie = has ? k + ((b >> 1) & 1) : dummy;
+  c.is = has ? k - nx * ((b >> 2) & 1) : dummy;
+  c.in = has ? k + nx * ((b >> 3) & 1) : dummy;
+  c.w = C.w[q]; c.e = C.e[q]; c.s = C.s[q]; c.n = C.n[q];  // zero without a cell
+  c.d = C.diag[q]; c.y = C.y[q]; c.r = has ? rh[q] : K(0);
+  return c;
+}
+
+__device__ __forceinline__ SweepCell sweep_sel(bool a, const SweepCell& x, const SweepCell& z) {
+  SweepCell c;
+  c.k = a ? x.k : z.k; c.iw = a ? x.iw : z.iw; c.ie = a ? x.ie : z.ie; c.is = a ? x.is : z.is; c.in = a ? x.in : z.in;
+  c.w = a ? x.w : z.w; c.e = a ? x.e : z.e; c.s = a ? x.s : z.s; c.n = a ? x.n : z.n;
+  c.d = a ? x.d : z.d; c.y = a ? x.y : z.y; c.r = a ? x.r : z.r;
+  return c;
+}
+
+// the red-black update of shaders/pressure.wgsl:87-94 (terms in its order)
+// for two cells of one colour, which never neighbour each other: all eight
+// reads are issued before either write.  A closed cell (d = 0, y = 0) gives
+// a finite quotient that the select replaces by 0.
+__device__ __forceinline__ void sweep_relax2(const SweepCell& a, const SweepCell& b, float* p) {
+  const float a0 = p[a.iw], a1 = p[a.ie], a2 = p[a.is], a3 = p[a.in];
+  const float b0 = p[b.iw], b1 = p[b.ie], b2 = p[b.is], b3 = p[b.in];
+  const float offa = a.w * a0 + a.e * a1 + a.s * a2 + a.n * a3;
+  const float offb = b.w * b0 + b.e * b1 + b.s * b2 + b.n * b3;
+  const float va = div_rcp(offa - a.r, a.d, a.y), vb = div_rcp(offb - b.r, b.d, b.y);
+  p[a.k] = a.d > K(0) ? va : K(0);
+  p[b.k] = b.d > K(0) ? vb : K(0);
 }
 
 // orders one wave's LDS traffic between half-sweeps without a workgroup barrier
@@ -521,8 +566,9 @@ __device__ __forceinline__ void cells_prolong(const CellSet<CPT>& C, int nx, flo
 }
 
 // LDS of the bottom solver: p and rhs of the three slots, then the residual scratch
+// and the coarsest level's face coefficients (at most (nx+1)ny + nx(ny+1) <= 800)
 constexpr int BOT_P0 = 0, BOT_R0 = 4096, BOT_P1 = 8192, BOT_R1 = 9216, BOT_P2 = 10240, BOT_R2 = 10496,
-              BOT_RES = 10752, BOT_TOTAL = 14848;
+              BOT_RES = 10752, BOT_CAX = 14848, BOT_TOTAL = 15648;
 __device__ __forceinline__ int bot_p(int s) { return s == 0 ? BOT_P0 : (s == 1 ? BOT_P1 : BOT_P2); }
 __device__ __forceinline__ int bot_r(int s) { return s == 0 ? BOT_R0 : (s == 1 ? BOT_R1 : BOT_R2); }
 
@@ -539,44 +585,51 @@ struct MgBottomOp {
   __shared__ float sm[BOT_TOTAL];
   float* sp2 = sm + BOT_P2;
   float* sr2 = sm + BOT_R2;
-  float* sres = sm + BOT_RES;  // fine residuals; first holds the coarsest level's coefficients
+  float* sres = sm + BOT_RES;  // fine residuals
   auto P = [&](int s) { return sm + bot_p(s); };
   auto R = [&](int s) { return sm + bot_r(s); };
   const int nb = B.n, t = threadIdx.x;
   const Level& Lc = B.L[nb - 1];
-  // coarsest-level coefficients: from global memory, or restricted here (:57-67)
-  const float *axc = Lc.ax, *ayc = Lc.ay;
+  // coarsest-level coefficients into LDS: copied, or restricted here (:57-67)
+  float* axc = sm + BOT_CAX;
+  float* ayc = axc + (Lc.nx + 1) * Lc.ny;
   if (B.coarse_coef) {
     const Level& Lf = B.L[nb - 2];
-    float* ax = sres;
-    float* ay = sres + (Lc.nx + 1) * Lc.ny;
     for (int k = t; k < (Lc.nx + 1) * Lc.ny; k += BT) {
       const int I = k % (Lc.nx + 1), J = k / (Lc.nx + 1);
       const float v = K(0.5) * (Lf.ax[(2 * J) * (Lf.nx + 1) + 2 * I] + Lf.ax[(2 * J + 1) * (Lf.nx + 1) + 2 * I]);
-      ax[k] = v; Lc.ax[k] = v;
+      axc[k] = v; Lc.ax[k] = v;
     }
     for (int k = t; k < Lc.nx * (Lc.ny + 1); k += BT) {
       const int I = k % Lc.nx, J = k / Lc.nx;
       const float v = K(0.5) * (Lf.ay[(2 * J) * Lf.nx + 2 * I] + Lf.ay[(2 * J) * Lf.nx + 2 * I + 1]);
-      ay[k] = v; Lc.ay[k] = v;
+      ayc[k] = v; Lc.ay[k] = v;
     }
-    __syncthreads();
-    axc = ax; ayc = ay;
+  } else {
+    for (int k = t; k < (Lc.nx + 1) * Lc.ny; k += BT) axc[k] = Lc.ax[k];
+    for (int k = t; k < Lc.nx * (Lc.ny + 1); k += BT) ayc[k] = Lc.ay[k];
   }
-  // per-thread cell data; level 0 of this kernel may hold up to 4096 cells.
-  // The coarsest level is swept by the first wave alone: lane l owns cells
-  // 2l, 2l+1, 2l+128, 2l+129 (two of each colour).
+  // per-thread cell data: slot 0 (up to 4096 cells) cells t + 1024q, slot
+  // 1 (up to 1024) cell t.  The coarsest level is swept by the first wave
+  // alone: lane l owns cells 2l, 2l+1, 2l+128, 2l+129 (two of each colour),
+  // loaded just before its sweeps.
   CellSet<4> C0, C2;
   CellSet<1> C1;
+  // (re)loaded after the coarsest sweeps rather than kept through them, which
+  // leaves the coarsest level's per-colour cells room in registers
+  auto load_levels = [&]() {
+    const Level& L0 = B.L[0];
+    const int k0[4] = {t, t + 1024, t + 2048, t + 3072};
+    const int k1[1] = {t};
+    if (nb == 2) cells_load<1>(C1, L0.nx, L0.ny, L0.ax, L0.ay, k1);
+    if (nb == 3) {
+      cells_load<4>(C0, L0.nx, L0.ny, L0.ax, L0.ay, k0);
+      cells_load<1>(C1, B.L[1].nx, B.L[1].ny, B.L[1].ax, B.L[1].ay, k1);
+    }
+  };
   {
     const Level& L0 = B.L[0];
-    const int kw[4] = {t < 64 ? 2 * t : -1, t < 64 ? 2 * t + 1 : -1, t < 64 ? 2 * t + 128 : -1, t < 64 ? 2 * t + 129 : -1};
-    cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
-    if (nb == 2) cells_load_strided<1>(C1, L0.nx, L0.ny, L0.ax, L0.ay);
-    if (nb == 3) {
-      cells_load_strided<4>(C0, L0.nx, L0.ny, L0.ax, L0.ay);
-      cells_load_strided<1>(C1, B.L[1].nx, B.L[1].ny, B.L[1].ax, B.L[1].ay);
-    }
+    load_levels();
     // p (warm start or zero) and rhs of the first level; slot = 3 - nb
     const int s0 = 3 - nb;
     for (int k = t; k < L0.nx * L0.ny; k += BT) { P(s0)[k] = L0.p[k]; R(s0)[k] = L0.rhs[k]; }
@@ -603,6 +656,10 @@ struct MgBottomOp {
   // coarsest: red-black sweeps from zero (shaders/pressure.wgsl:131-156) by
   // the first wave; the other waves wait at the next workgroup barrier
   for (int k = t; k < Lc.nx * Lc.ny; k += BT) sp2[k] = K(0);
+  {
+    const int kw[4] = {t < 64 ? 2 * t : -1, t < 64 ? 2 * t + 1 : -1, t < 64 ? 2 * t + 128 : -1, t < 64 ? 2 * t + 129 : -1};
+    cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
+  }
   __syncthreads();
   if (B.coarse_iters <= 0) {
     coarse_direct<4>(C2, Lc.nx, Lc.ny, Lc.h2, sr2, sp2, B.band, (double*)sres);
@@ -610,14 +667,25 @@ struct MgBottomOp {
     float rh[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) rh[q] = C2.k[q] >= 0 ? sr2[C2.k[q]] * Lc.h2 : K(0);
+    // each lane's red and black cell of its two pairs; no cell: sres[0]
+    SweepCell X[2][2];
+    const int dummy = BOT_RES - BOT_P2;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const SweepCell c0 = sweep_cell<4>(C2, rh, 2 * a, Lc.nx, dummy), c1 = sweep_cell<4>(C2, rh, 2 * a + 1, Lc.nx, dummy);
+      const bool first_red = ((C2.nb[2 * a] >> 4) & 1) == 0;
+      X[0][a] = sweep_sel(first_red, c0, c1);
+      X[1][a] = sweep_sel(first_red, c1, c0);
+    }
     for (int it = 0; it < B.coarse_iters; it++) {
-      cells_relax_pairs<2>(C2, rh, Lc.nx, sp2, 0);
+      sweep_relax2(X[0][0], X[0][1], sp2);
       wave_sync();
-      cells_relax_pairs<2>(C2, rh, Lc.nx, sp2, 1);
+      sweep_relax2(X[1][0], X[1][1], sp2);
       wave_sync();
     }
   }
   __syncthreads();
+  load_levels();
   // up (js/fluid.js:305-311)
   for (int s = 1; s >= s0; s--) {
     const Level& L = lev(s);
