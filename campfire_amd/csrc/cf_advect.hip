@@ -63,7 +63,7 @@ __device__ __forceinline__ float2 mm4v(float a, float c, float d, float e) {
   return make_float2(fminf(fminf(a, c), fminf(d, e)), fmaxf(fmaxf(a, c), fmaxf(d, e)));
 }
 
-// what a tile's passes read
+// what a tile's passes read (idx: 1/dx correctly rounded, for div_rcp)
 struct Tile {
   Win<float, A_CX, A_CY> T;
   Win<float4, A_CX, A_CY> Y;
@@ -73,17 +73,18 @@ struct Tile {
   Win<float4, B_CX, B_CY> Yh;
   Win<float, B_UX, B_UY> Uh;
   Win<float, B_VX, B_VY> Vh;
-  float dx;
+  float dx, idx;
 };
 
-// RK2 departure point (shaders/advect.wgsl:28-34) from the velocity windows
+// RK2 departure point (shaders/advect.wgsl:28-34) from the velocity windows;
+// the divisions by dx are the correctly rounded quotients (div_rcp_fast)
 __device__ __forceinline__ float2 trace_l(const Tile& t, float gx, float gy, float dt, bool& ok) {
-  const float vx = t.U.bil(gx - K(0), gy - K(0.5), ok) / t.dx;
-  const float vy = t.V.bil(gx - K(0.5), gy - K(0), ok) / t.dx;
+  const float vx = div_rcp_fast(t.U.bil(gx - K(0), gy - K(0.5), ok), t.dx, t.idx);
+  const float vy = div_rcp_fast(t.V.bil(gx - K(0.5), gy - K(0), ok), t.dx, t.idx);
   const float hdt = K(0.5) * dt;
   const float mx_ = gx - hdt * vx, my_ = gy - hdt * vy;
-  const float wx = t.U.bil(mx_ - K(0), my_ - K(0.5), ok) / t.dx;
-  const float wy = t.V.bil(mx_ - K(0.5), my_ - K(0), ok) / t.dx;
+  const float wx = div_rcp_fast(t.U.bil(mx_ - K(0), my_ - K(0.5), ok), t.dx, t.idx);
+  const float wy = div_rcp_fast(t.V.bil(mx_ - K(0.5), my_ - K(0), ok), t.dx, t.idx);
   return make_float2(gx - dt * wx, gy - dt * wy);
 }
 
@@ -212,6 +213,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   t.T = {sT, ax0, ay0}; t.Y = {sY, ax0, ay0}; t.U = {sU, ax0, ay0}; t.V = {sV, ax0, ay0};
   t.Th = {bT, bx0, by0}; t.Yh = {bY, bx0, by0}; t.Uh = {bU, bx0, by0}; t.Vh = {bV, bx0, by0};
   t.dx = P.dx;
+  t.idx = K(1) / P.dx;
 
   // A. inputs as the samplers see them
   for (int q = tid; q < A_CX * A_CY; q += 256) {
@@ -223,62 +225,140 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   for (int q = tid; q < A_VX * A_VY; q += 256) sV[q] = fV(F.V, nx, ny, ax0 + q % A_VX, ay0 + q / A_VX);
   __syncthreads();
 
-  // B. forward pass over the tile + AH, as the corrector's samplers see it
-  for (int q = tid; q < B_CX * B_CY; q += 256) {
-    const int i = bx0 + q % B_CX, j = by0 + q / B_CX;
+  // B. forward pass over the tile + AH, as the corrector's samplers see it.
+  // The corrector's departure trace of a cell or face (+dt) is the forward
+  // pass's trace of the same point, so each thread first does the forward
+  // pass of the points it will correct in C and keeps their traces in
+  // registers (B1); the rest of the window follows (B2).
+  const float sO2i = K(1) / P.sO2;
+  const bool ex = x0 + AX == nx, ey = y0 + AY == ny;
+  constexpr int NQ = AX * AY + AY + AX;  // C's work items: the tile's cells (+ faces), then edge faces
+  constexpr int NS = (NQ + 255) / 256;
+  // the points C corrects: item q -> (i, j) and which of its cell / U / V it owns
+  auto item = [&](int q, int& i, int& j, bool& oc, bool& ou, bool& ov) {
+    const bool tp = q < AX * AY, eu = q >= AX * AY && q < AX * AY + AY, ev = q >= AX * AY + AY && q < NQ;
+    i = tp ? x0 + q % AX : (eu ? nx : x0 + (q - AX * AY - AY));
+    j = tp ? y0 + q / AX : (eu ? y0 + (q - AX * AY) : ny);
+    oc = tp && i < nx && j < ny;
+    ou = (tp && i <= nx && j < ny) || (eu && ex && j < ny);
+    ov = (tp && i < nx && j <= ny) || (ev && ey && i < nx);
+  };
+  // forward values of one point (i, j >= 0 inside the domain); d, okd: its trace
+  auto fwd_cell = [&](int i, int j, float& th, float4& yh, float2& d, bool& okd) {
+    okd = true;
+    d = trace_l(t, (float)i + K(0.5), (float)j + K(0.5), dt, okd);
+    bool ok = okd;
+    th = t.T.bil(d.x - K(0.5), d.y - K(0.5), ok);
+    yh = t.Y.bil(d.x - K(0.5), d.y - K(0.5), ok);
+    if (!ok) g_fwd_cell(F, P, i, j, th, yh);
+  };
+  auto fwd_u = [&](int i, int j, float2& d, bool& okd) {
+    okd = true;
+    d = trace_l(t, (float)i, (float)j + K(0.5), dt, okd);
+    bool ok = okd;
+    float u = t.U.bil(d.x - K(0), d.y - K(0.5), ok);
+    if (!ok) u = g_fwd_u(F, P, i, j);
+    return u;
+  };
+  auto fwd_v = [&](int i, int j, float2& d, bool& okd) {
+    okd = true;
+    d = trace_l(t, (float)i + K(0.5), (float)j, dt, okd);
+    bool ok = okd;
+    float v = t.V.bil(d.x - K(0.5), d.y - K(0), ok);
+    if (!ok) v = g_fwd_v(F, P, i, j);
+    return v;
+  };
+  // the traces of one work item's cell and faces, kept from B1 for C
+  struct Slot {
+    float2 dC, dU, dV;
+    bool okC, okU, okV;
+  };
+  static_assert(NS == 3 && AX * AY == 512, "two tile slots and one edge slot per thread");
+  Slot S0, S1;
+  // B1: the tile rectangle's points of work items tid and tid + 256: the
+  // forward pass of the points C corrects (traces kept), the outside values
+  // for the rest (beyond the domain's right or top edge)
+  auto b1 = [&](int q, Slot& S) {
+    int i, j;
+    bool oc, ou, ov;
+    item(q, i, j, oc, ou, ov);
+    S.dC = S.dU = S.dV = make_float2(0.f, 0.f);
+    S.okC = S.okU = S.okV = false;
+    float th = P.Tamb;
+    float4 yh = aY;
+    if (oc) fwd_cell(i, j, th, yh, S.dC, S.okC);
+    const int qb = (j - by0) * B_CX + (i - bx0);
+    bT[qb] = th;
+    bY[qb] = yh;
+    bU[(j - by0) * B_UX + (i - bx0)] = ou ? fwd_u(i, j, S.dU, S.okU) : K(0);
+    bV[(j - by0) * B_VX + (i - bx0)] = ov ? fwd_v(i, j, S.dV, S.okV) : K(0);
+  };
+  b1(tid, S0);
+  b1(tid + 256, S1);
+  // B2: the ring of the window around the tile rectangle, enumerated without
+  // gaps: r -> window (a, b) for a W x H window whose rectangle starts at
+  // (AH, AH) with size AX x AY
+  auto ring = [&](int r, int W, int& a, int& b) {
+    const int top = AH * W, mid = AY * (W - AX);
+    if (r < top) { a = r % W; b = r / W; }
+    else if (r < top + mid) {
+      const int c = (r - top) % (W - AX);
+      a = c < AH ? c : c + AX;
+      b = AH + (r - top) / (W - AX);
+    } else { a = (r - top - mid) % W; b = AH + AY + (r - top - mid) / W; }
+  };
+  for (int r = tid; r < B_CX * B_CY - AX * AY; r += 256) {
+    int a, b;
+    ring(r, B_CX, a, b);
+    const int i = bx0 + a, j = by0 + b;
     float th = P.Tamb;
     float4 yh = aY;
     if (!(i < 0 || i >= nx || j >= ny)) {
-      const int jj = max(j, 0);
-      bool ok = true;
-      const float2 d = trace_l(t, (float)i + K(0.5), (float)jj + K(0.5), dt, ok);
-      th = t.T.bil(d.x - K(0.5), d.y - K(0.5), ok);
-      yh = t.Y.bil(d.x - K(0.5), d.y - K(0.5), ok);
-      if (!ok) g_fwd_cell(F, P, i, jj, th, yh);
+      float2 d;
+      bool okd;
+      fwd_cell(i, max(j, 0), th, yh, d, okd);
     }
-    bT[q] = th;
-    bY[q] = yh;
+    bT[b * B_CX + a] = th;
+    bY[b * B_CX + a] = yh;
   }
-  for (int q = tid; q < B_UX * B_UY; q += 256) {
-    const int i = bx0 + q % B_UX, j = by0 + q / B_UX;
+  for (int r = tid; r < B_UX * B_UY - AX * AY; r += 256) {
+    int a, b;
+    ring(r, B_UX, a, b);
+    const int i = bx0 + a, j = by0 + b;
     float u = K(0);
     if (!(i < 0 || i > nx || j >= ny)) {
-      const int jj = max(j, 0);
-      bool ok = true;
-      const float2 d = trace_l(t, (float)i, (float)jj + K(0.5), dt, ok);
-      u = t.U.bil(d.x - K(0), d.y - K(0.5), ok);
-      if (!ok) u = g_fwd_u(F, P, i, jj);
+      float2 d;
+      bool okd;
+      u = fwd_u(i, max(j, 0), d, okd);
       if (j < 0) u = -u;
     }
-    bU[q] = u;
+    bU[b * B_UX + a] = u;
   }
-  for (int q = tid; q < B_VX * B_VY; q += 256) {
-    const int i = bx0 + q % B_VX, j = by0 + q / B_VX;
+  for (int r = tid; r < B_VX * B_VY - AX * AY; r += 256) {
+    int a, b;
+    ring(r, B_VX, a, b);
+    const int i = bx0 + a, j = by0 + b;
     float v = K(0);
     if (!(i < 0 || i >= nx || j > ny)) {
-      const int jj = max(j, 0);
-      bool ok = true;
-      const float2 d = trace_l(t, (float)i + K(0.5), (float)jj, dt, ok);
-      v = t.V.bil(d.x - K(0.5), d.y - K(0), ok);
-      if (!ok) v = g_fwd_v(F, P, i, jj);
+      float2 d;
+      bool okd;
+      v = fwd_v(i, max(j, 0), d, okd);
     }
-    bV[q] = v;
+    bV[b * B_VX + a] = v;
   }
   __syncthreads();
 
   // C. correction of the tile's cells and faces; the tile at the right (top)
   // domain edge also owns the U (V) faces on that edge
-  const bool ex = x0 + AX == nx, ey = y0 + AY == ny;
-  for (int q = tid; q < AX * AY + AY + AX; q += 256) {
+  auto c1 = [&](int q, const Slot& S) {
     int i, j;
-    if (q < AX * AY) { i = x0 + q % AX; j = y0 + q / AX; }
-    else if (q < AX * AY + AY) { if (!ex) continue; i = nx; j = y0 + (q - AX * AY); }
-    else { if (!ey) continue; i = x0 + (q - AX * AY - AY); j = ny; }
-    if (i < nx && j < ny) {  // scalCorrect (:49-66)
+    bool oc, ou, ov;
+    item(q, i, j, oc, ou, ov);
+    if (oc) {  // scalCorrect (:49-66)
       const int k = j * nx + i;
       const float gxc = (float)i + K(0.5), gyc = (float)j + K(0.5);
-      bool ok = true;
-      const float2 d = trace_l(t, gxc, gyc, dt, ok);
+      bool ok = S.okC;
+      const float2 d = S.dC;
       const float2 f = trace_l(t, gxc, gyc, -dt, ok);
       const float* pT = t.T.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
       const float4* pY = t.Y.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
@@ -301,8 +381,8 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
         const bool inT = Tc >= mT.x && Tc <= mT.y;
         const bool inY = Yc.x >= lo.x && Yc.y >= lo.y && Yc.z >= lo.z && Yc.w >= lo.w &&
                          Yc.x <= hi.x && Yc.y <= hi.y && Yc.z <= hi.z && Yc.w <= hi.w;
-        const float mixC = fminf(Yc.x, Yc.y / P.sO2);
-        const float mixL = fminf(Yhk.x, Yhk.y / P.sO2);
+        const float mixC = fminf(Yc.x, div_rcp(Yc.y, P.sO2, sO2i));
+        const float mixL = fminf(Yhk.x, div_rcp(Yhk.y, P.sO2, sO2i));
         const bool take = inT && inY && mixC <= mixL + K(1e-5);
         To = take ? Tc : Thk;
         Yo = take ? Yc : Yhk;
@@ -312,38 +392,51 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       F.T2[k] = To;
       F.Y2[k] = Yo;
     }
-    if (i <= nx && j < ny) {  // velCorrect, U (:78-90)
+    if (ou) {  // velCorrect, U (:78-90)
       const float gxu = (float)i, gyu = (float)j + K(0.5);
-      bool ok = true;
+      bool ok = S.okU;
       const float2 b = trace_l(t, gxu, gyu, -dt, ok);
       const float back = t.Uh.bil(b.x - K(0), b.y - K(0.5), ok);
-      const float2 d = trace_l(t, gxu, gyu, dt, ok);
+      const float2 d = S.dU;
       const float* pU = t.U.at(toint(floorf(d.x - K(0))), toint(floorf(d.y - K(0.5))), ok);
       float u;
       if (ok) {
-        const float2 m = mm4v(pU[0], pU[1], pU[A_UX], pU[A_UX + 1]);
-        u = clampf(t.Uh.own(i, j) + K(0.5) * (t.U.own(i, j) - back), m.x, m.y);
+        const float2 mm = mm4v(pU[0], pU[1], pU[A_UX], pU[A_UX + 1]);
+        u = clampf(t.Uh.own(i, j) + K(0.5) * (t.U.own(i, j) - back), mm.x, mm.y);
       } else {
         u = g_corr_u(F, P, i, j);
       }
       F.U2[j * (nx + 1) + i] = u;
     }
-    if (i < nx && j <= ny) {  // velCorrect, V (:91-103)
+    if (ov) {  // velCorrect, V (:91-103)
       const float gxv = (float)i + K(0.5), gyv = (float)j;
-      bool ok = true;
+      bool ok = S.okV;
       const float2 b = trace_l(t, gxv, gyv, -dt, ok);
       const float back = t.Vh.bil(b.x - K(0.5), b.y - K(0), ok);
-      const float2 d = trace_l(t, gxv, gyv, dt, ok);
+      const float2 d = S.dV;
       const float* pV = t.V.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0))), ok);
       float v;
       if (ok) {
-        const float2 m = mm4v(pV[0], pV[1], pV[A_VX], pV[A_VX + 1]);
-        v = clampf(t.Vh.own(i, j) + K(0.5) * (t.V.own(i, j) - back), m.x, m.y);
+        const float2 mm = mm4v(pV[0], pV[1], pV[A_VX], pV[A_VX + 1]);
+        v = clampf(t.Vh.own(i, j) + K(0.5) * (t.V.own(i, j) - back), mm.x, mm.y);
       } else {
         v = g_corr_v(F, P, i, j);
       }
       F.V2[j * nx + i] = v;
     }
+  };
+  c1(tid, S0);
+  c1(tid + 256, S1);
+  {  // the domain-edge faces: traced here
+    int i, j;
+    bool oc, ou, ov;
+    item(tid + 512, i, j, oc, ou, ov);
+    Slot S2;
+    S2.dC = S2.dU = S2.dV = make_float2(0.f, 0.f);
+    S2.okC = S2.okU = S2.okV = false;
+    if (ou) { S2.okU = true; S2.dU = trace_l(t, (float)i, (float)j + K(0.5), dt, S2.okU); }
+    if (ov) { S2.okV = true; S2.dV = trace_l(t, (float)i + K(0.5), (float)j, dt, S2.okV); }
+    c1(tid + 512, S2);
   }
 }
 

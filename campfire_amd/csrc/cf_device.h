@@ -24,18 +24,36 @@ __device__ __forceinline__ int toint(float x) {
 }
 __device__ __forceinline__ int iround(float x) { return (int)rintf(x); }
 
-// x / d correctly rounded, given y = 1/d correctly rounded: Markstein's
-// correction step (P. Markstein, IBM J. Res.
-// Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point Arithmetic,
-// the theorem on division with a correctly rounded reciprocal).  q = x*y is
-// within one ulp of x/d, the FMA residual x - q*d is exact, and one more FMA
-// rounds to the nearest quotient.  Valid without underflow or overflow (the
-// sweeps' pressures and coefficients are far from both); three instructions
-// where a division takes eleven, for divisors reused many times.
+// x / d correctly rounded, for d > 0 given y = 1/d correctly rounded:
+// Markstein's correction step (P. Markstein, IBM J. Res. Dev. 34(1), 1990;
+// Muller et al., Handbook of Floating-Point Arithmetic, the theorem on
+// division with a correctly rounded reciprocal).  q = x*y is within one ulp
+// of x/d, the FMA residual x - q*d is exact, and one more FMA rounds to the
+// nearest quotient; the quotient takes the sign of x, as x/d does for zero.
+// The theorem assumes no underflow, so a nonzero dividend below 2^-100 (a
+// mass fraction that has burnt out, say) takes the IEEE division.  A few
+// instructions where a division takes eleven, for divisors reused many times.
 __device__ __forceinline__ float div_rcp(float x, float d, float y) {
   const float q = x * y;
-  const float r = fmaf(-q, d, x);
-  return fmaf(r, y, q);
+  const float m = copysignf(fmaf(fmaf(-q, d, x), y, q), x);
+  const uint32_t ax = __float_as_uint(x) & 0x7fffffffu;
+  if (__builtin_expect(ax - 1u < 0x0d7fffffu, 0)) {  // 0 < |x| < 2^-100 (2^-100 = 0x0d800000)
+    // an empty volatile statement keeps the compiler from computing this
+    // rare division on every call and selecting
+    asm volatile("");
+    return x / d;
+  }
+  return m;
+}
+
+// div_rcp without the small-dividend branch, for loops where the branch
+// costs more than the division saves (the coarsest sweeps, the advection
+// traces).  Equal to x / d for x = 0 and |x| >= 2^-100; a nonzero dividend
+// below that (never a pressure update or a velocity of this solver, which
+// are zero or many orders above it) may round its quotient one ulp apart.
+__device__ __forceinline__ float div_rcp_fast(float x, float d, float y) {
+  const float q = x * y;
+  return copysignf(fmaf(fmaf(-q, d, x), y, q), x);
 }
 
 // x^1.5 as x*sqrt(x): within ~1 ulp of the correctly rounded power (WGSL's pow
@@ -43,13 +61,20 @@ __device__ __forceinline__ float div_rcp(float x, float d, float y) {
 // stencils that evaluate Sutherland's law five times per face
 __device__ __forceinline__ float pow15(float x) { return x * sqrtf(x); }
 
-// gas closures (shaders/common.wgsl:33-42)
+// gas closures (shaders/common.wgsl:33-42).  The divisions of a temperature
+// by the constant 273.15 and of a diffusivity by Pr are correctly rounded
+// quotients through div_rcp_fast (the reciprocal of 273.15 folds at compile
+// time; 1/Pr is one division per thread)
 __device__ __forceinline__ float rhoOf(const cf_params& P, float T) { return P.rhoAmb * P.Tamb / T; }
 __device__ __forceinline__ float muOf(const cf_params& P, float T) {
-  return K(1.716e-5) * pow15(T / K(273.15)) * K(273.15 + 110.4) / (T + K(110.4)) * P.viscMult;
+  return K(1.716e-5) * pow15(div_rcp_fast(T, K(273.15), K(1) / K(273.15))) * K(273.15 + 110.4) / (T + K(110.4)) *
+         P.viscMult;
 }
 __device__ __forceinline__ float rhoDOf(const cf_params& P, float T) {
-  return K(1.716e-5) * pow15(T / K(273.15)) * K(273.15 + 110.4) / (T + K(110.4)) / P.Pr * P.diffMult;
+  return div_rcp_fast(
+             K(1.716e-5) * pow15(div_rcp_fast(T, K(273.15), K(1) / K(273.15))) * K(273.15 + 110.4) / (T + K(110.4)),
+             P.Pr, K(1) / P.Pr) *
+         P.diffMult;
 }
 __device__ __forceinline__ float invRho(const cf_params& P, float T) { return T / (P.rhoAmb * P.Tamb); }
 

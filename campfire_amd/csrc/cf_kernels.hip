@@ -191,6 +191,8 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
   const int nx = P.nx;
   const int k = j * nx + i;
   const float dx = P.dx, dt = P.dt, cp = P.cp, sO2 = P.sO2, dHc = P.dHc;
+  // reciprocals for the correctly rounded quotients of div_rcp
+  const float isO2 = K(1) / sO2, icp = K(1) / cp;
   const float2 g = F.cellG[k];
   const float sdf = g.x;
   // ids come from the geometry pass (piece*4 + face) or are SOLID_NONE;
@@ -263,22 +265,22 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
 #pragma unroll 1
   for (int s = 0; s < 8; s++) {
     const float wr = P.reactB * rho0 * Yc.x * Yc.y * expf(-P.reactTa / Tc);
-    const float dF = fminf(wr * h, fminf(Yc.x, Yc.y / sO2));
+    const float dF = fminf(wr * h, fminf(Yc.x, div_rcp(Yc.y, sO2, isO2)));
     Yc.x += -dF; Yc.y += -sO2 * dF; Yc.z += (K(1) + sO2) * dF; Yc.w += K(0);
-    Tc += dF * dHc / cp;
+    Tc += div_rcp(dF * dHc, cp, icp);
     burned += dF;
   }
   // soot formation and oxidation (:122-127)
   const float dSf = fminf(P.sootFormA * expf(-P.sootFormTa / Tc) * Yc.x * dt, Yc.x);
   Yc.x += -dSf; Yc.y += K(0); Yc.z += K(0); Yc.w += dSf;
-  const float dSo = fminf(P.sootOxA * Yc.w * Yc.y * expf(-P.sootOxTa / Tc) * dt, fminf(Yc.w, Yc.y / sO2));
+  const float dSo = fminf(P.sootOxA * Yc.w * Yc.y * expf(-P.sootOxTa / Tc) * dt, fminf(Yc.w, div_rcp(Yc.y, sO2, isO2)));
   Yc.x += K(0); Yc.y += -sO2 * dSo; Yc.z += (K(1) + sO2) * dSo; Yc.w += -dSo;
-  Tc += dSo * dHc / cp;
+  Tc += div_rcp(dSo * dHc, cp, icp);
   burned += dSo;
   qrel = rho0 * burned * dHc / dt;
   // optically thin loss, linearised implicit (:130-136)
   const float r1 = rhoOf(P, Tc);
-  const float fv = r1 * Yc.w / P.sootDensity;
+  const float fv = div_rcp(r1 * Yc.w, P.sootDensity, K(1) / P.sootDensity);
   const float kap = P.kappaSootC * fv * Tc + P.kappaGas * Yc.z;
   const float T3 = Tc * Tc * Tc;
   qrad = K(4.0) * kap * P.sigma * (T3 * Tc - powf(P.Tamb, K(4.0)));
@@ -291,9 +293,10 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
   if (me > K(0)) {
     const float madd = me * dt;
     const float r = rhoOf(P, Tc);
-    Yc = make_float4((r * Yc.x + madd * K(0)) / (r + madd), (r * Yc.y + madd * P.YOamb) / (r + madd),
-                     (r * Yc.z + madd * K(0)) / (r + madd), (r * Yc.w + madd * K(0)) / (r + madd));
-    Tc = (r * Tc + madd * P.Tamb) / (r + madd);
+    const float den = r + madd, iden = K(1) / den;  // five quotients by one denominator
+    Yc = make_float4(div_rcp(r * Yc.x + madd * K(0), den, iden), div_rcp(r * Yc.y + madd * P.YOamb, den, iden),
+                     div_rcp(r * Yc.z + madd * K(0), den, iden), div_rcp(r * Yc.w + madd * K(0), den, iden));
+    Tc = div_rcp(r * Tc + madd * P.Tamb, den, iden);
     massSrc += me;
   }
   // write (:150-156)

@@ -108,7 +108,7 @@ __device__ __forceinline__ void pairs_relax(const PairCell (&c)[PQ][2], const in
     const PairCell& a = sub ? c[q][1] : c[q][0];
     const int ci = base[q] + sub;
     const float off = pair_off(a, sp, ci, RX);
-    const float v = a.diag > K(0) ? div_rcp(off - a.rhs * h2, a.diag, a.y) : K(0);
+    const float v = a.diag > K(0) ? div_rcp_fast(off - a.rhs * h2, a.diag, a.y) : K(0);
     if (a.upd) sp[ci] = v;
   }
 }
@@ -404,7 +404,7 @@ __device__ __forceinline__ void cells_relax_pairs(const CellSet<2 * NPAIR>& C, c
     if (k >= 0) {
       const float off = w * p[k - (nb & 1)] + e * p[k + ((nb >> 1) & 1)] + s * p[k - nx * ((nb >> 2) & 1)] +
                         n * p[k + nx * ((nb >> 3) & 1)];
-      p[k] = diag > K(0) ? div_rcp(off - r, diag, y) : K(0);
+      p[k] = diag > K(0) ? div_rcp_fast(off - r, diag, y) : K(0);
     }
   }
 }
@@ -453,7 +453,7 @@ __device__ __forceinline__ void sweep_relax2(const SweepCell& a, const SweepCell
   const float b0 = p[b.iw], b1 = p[b.ie], b2 = p[b.is], b3 = p[b.in];
   const float offa = a.w * a0 + a.e * a1 + a.s * a2 + a.n * a3;
   const float offb = b.w * b0 + b.e * b1 + b.s * b2 + b.n * b3;
-  const float va = div_rcp(offa - a.r, a.d, a.y), vb = div_rcp(offb - b.r, b.d, b.y);
+  const float va = div_rcp_fast(offa - a.r, a.d, a.y), vb = div_rcp_fast(offb - b.r, b.d, b.y);
   p[a.k] = a.d > K(0) ? va : K(0);
   p[b.k] = b.d > K(0) ? vb : K(0);
 }

@@ -4,8 +4,9 @@ campfire_amd/csrc/cf_device.h): y = RN(1/d), q = RN(x*y),
 r = fma(-q, d, x), result fma(r, y, q).  The kernels' bitwise parity with
 the oracle's true divisions rests on this being the correctly rounded x/d.
 Checked here in C (gcc, libm's fmaf) on every significand of x against
-divisors with long runs of ones and zeros, and on random pairs over the
-exponent range the sweeps see."""
+divisors with long runs of ones and zeros, on random pairs over the exponent
+range the sweeps see, and on dividends down through the subnormal range (the
+IEEE-division branch) and signed zeros."""
 import os
 import subprocess
 import tempfile
@@ -16,9 +17,19 @@ SRC = r"""
 #include <stdio.h>
 #include <string.h>
 static float bits(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+/* div_rcp of campfire_amd/csrc/cf_device.h */
+static float div_rcp(float x, float d, float y) {
+  const float q = x * y;
+  const float m = copysignf(fmaf(fmaf(-q, d, x), y, q), x);
+  uint32_t ax;
+  memcpy(&ax, &x, 4);
+  ax &= 0x7fffffffu;
+  if (ax - 1u < 0x0d7fffffu) return x / d;
+  return m;
+}
 static int check(float x, float d) {
   volatile float yv = 1.0f / d;
-  const float y = yv, q = x * y, r = fmaf(-q, d, x), v = fmaf(r, y, q), ref = x / d;
+  const float v = div_rcp(x, d, yv), ref = x / d;
   return memcmp(&v, &ref, 4) != 0;
 }
 int main(void) {
@@ -39,6 +50,15 @@ int main(void) {
     const float d = bits((ed << 23) | ((uint32_t)(s >> 40) & 0x7fffffu));
     bad += check(x, d);
   }
+  /* dividends down to the subnormal range and signed zeros, divisors 2^-20..2^20 */
+  for (long n = 0; n < 4000000; n++) {
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    const uint32_t ex = (uint32_t)(s % 40), ed = 127u - 20u + (uint32_t)((s >> 8) % 41);
+    const float x = bits((ex << 23) | ((uint32_t)(s >> 16) & 0x7fffffu)) * ((s >> 63) ? -1.f : 1.f);
+    const float d = bits((ed << 23) | ((uint32_t)(s >> 40) & 0x7fffffu));
+    bad += check(x, d);
+  }
+  bad += check(0.0f, 3.0f) + check(-0.0f, 3.0f) + check(0x1p-149f, 1.5f) + check(-0x1p-127f, 0.7f);
   printf("%ld\n", bad);
   return 0;
 }
