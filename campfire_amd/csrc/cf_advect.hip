@@ -76,16 +76,46 @@ struct Tile {
   float dx, idx;
 };
 
-// RK2 departure point (shaders/advect.wgsl:28-34) from the velocity windows;
-// the divisions by dx are the correctly rounded quotients (div_rcp_fast)
-__device__ __forceinline__ float2 trace_l(const Tile& t, float gx, float gy, float dt, bool& ok) {
-  const float vx = div_rcp_fast(t.U.bil(gx - K(0), gy - K(0.5), ok), t.dx, t.idx);
-  const float vy = div_rcp_fast(t.V.bil(gx - K(0.5), gy - K(0), ok), t.dx, t.idx);
+// RK2 departure point (shaders/advect.wgsl:28-34) from the velocity windows,
+// the divisions by dx as correctly rounded quotients (div_rcp_fast).  Traces
+// start at a cell centre or a face, whose first velocity samples have
+// integer footprints and weights 0.5 or 0: they are read at fixed offsets
+// and mixed with constant weights, the arithmetic of bil() on those points.
+// Every B and C point's first footprint lies inside the A windows (their
+// halo is 5, the points are within 2 of the tile, the footprint reaches 1
+// more), so `ok` is unchanged by these samples.
+__device__ __forceinline__ float2 trace_rest(const Tile& t, float gx, float gy, float u, float v, float dt, bool& ok) {
+  const float vx = div_rcp_fast(u, t.dx, t.idx);
+  const float vy = div_rcp_fast(v, t.dx, t.idx);
   const float hdt = K(0.5) * dt;
   const float mx_ = gx - hdt * vx, my_ = gy - hdt * vy;
   const float wx = div_rcp_fast(t.U.bil(mx_ - K(0), my_ - K(0.5), ok), t.dx, t.idx);
   const float wy = div_rcp_fast(t.V.bil(mx_ - K(0.5), my_ - K(0), ok), t.dx, t.idx);
   return make_float2(gx - dt * wx, gy - dt * wy);
+}
+// from cell centre (i + 1/2, j + 1/2): U footprint (i, j) with (0.5, 0), V (i, j) with (0, 0.5)
+__device__ __forceinline__ float2 trace_cell(const Tile& t, int i, int j, float dt, bool& ok) {
+  const float u = mixf(mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0.5)), mixf(t.U.own(i, j + 1), t.U.own(i + 1, j + 1), K(0.5)),
+                       K(0));
+  const float v = mixf(mixf(t.V.own(i, j), t.V.own(i + 1, j), K(0)), mixf(t.V.own(i, j + 1), t.V.own(i + 1, j + 1), K(0)),
+                       K(0.5));
+  return trace_rest(t, (float)i + K(0.5), (float)j + K(0.5), u, v, dt, ok);
+}
+// from U face (i, j + 1/2): U footprint (i, j) with (0, 0), V (i - 1, j) with (0.5, 0.5)
+__device__ __forceinline__ float2 trace_uface(const Tile& t, int i, int j, float dt, bool& ok) {
+  const float u = mixf(mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0)), mixf(t.U.own(i, j + 1), t.U.own(i + 1, j + 1), K(0)),
+                       K(0));
+  const float v = mixf(mixf(t.V.own(i - 1, j), t.V.own(i, j), K(0.5)), mixf(t.V.own(i - 1, j + 1), t.V.own(i, j + 1), K(0.5)),
+                       K(0.5));
+  return trace_rest(t, (float)i, (float)j + K(0.5), u, v, dt, ok);
+}
+// from V face (i + 1/2, j): U footprint (i, j - 1) with (0.5, 0.5), V (i, j) with (0, 0)
+__device__ __forceinline__ float2 trace_vface(const Tile& t, int i, int j, float dt, bool& ok) {
+  const float u = mixf(mixf(t.U.own(i, j - 1), t.U.own(i + 1, j - 1), K(0.5)), mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0.5)),
+                       K(0.5));
+  const float v = mixf(mixf(t.V.own(i, j), t.V.own(i + 1, j), K(0)), mixf(t.V.own(i, j + 1), t.V.own(i + 1, j + 1), K(0)),
+                       K(0));
+  return trace_rest(t, (float)i + K(0.5), (float)j, u, v, dt, ok);
 }
 
 // ---- global-memory forms (points whose footprints leave the windows) ----
@@ -246,7 +276,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   // forward values of one point (i, j >= 0 inside the domain); d, okd: its trace
   auto fwd_cell = [&](int i, int j, float& th, float4& yh, float2& d, bool& okd) {
     okd = true;
-    d = trace_l(t, (float)i + K(0.5), (float)j + K(0.5), dt, okd);
+    d = trace_cell(t, i, j, dt, okd);
     bool ok = okd;
     th = t.T.bil(d.x - K(0.5), d.y - K(0.5), ok);
     yh = t.Y.bil(d.x - K(0.5), d.y - K(0.5), ok);
@@ -254,7 +284,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   };
   auto fwd_u = [&](int i, int j, float2& d, bool& okd) {
     okd = true;
-    d = trace_l(t, (float)i, (float)j + K(0.5), dt, okd);
+    d = trace_uface(t, i, j, dt, okd);
     bool ok = okd;
     float u = t.U.bil(d.x - K(0), d.y - K(0.5), ok);
     if (!ok) u = g_fwd_u(F, P, i, j);
@@ -262,7 +292,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   };
   auto fwd_v = [&](int i, int j, float2& d, bool& okd) {
     okd = true;
-    d = trace_l(t, (float)i + K(0.5), (float)j, dt, okd);
+    d = trace_vface(t, i, j, dt, okd);
     bool ok = okd;
     float v = t.V.bil(d.x - K(0.5), d.y - K(0), ok);
     if (!ok) v = g_fwd_v(F, P, i, j);
@@ -356,10 +386,9 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
     item(q, i, j, oc, ou, ov);
     if (oc) {  // scalCorrect (:49-66)
       const int k = j * nx + i;
-      const float gxc = (float)i + K(0.5), gyc = (float)j + K(0.5);
       bool ok = S.okC;
       const float2 d = S.dC;
-      const float2 f = trace_l(t, gxc, gyc, -dt, ok);
+      const float2 f = trace_cell(t, i, j, -dt, ok);
       const float* pT = t.T.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
       const float4* pY = t.Y.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
       const float sbT = t.Th.bil(f.x - K(0.5), f.y - K(0.5), ok);
@@ -393,9 +422,8 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       F.Y2[k] = Yo;
     }
     if (ou) {  // velCorrect, U (:78-90)
-      const float gxu = (float)i, gyu = (float)j + K(0.5);
       bool ok = S.okU;
-      const float2 b = trace_l(t, gxu, gyu, -dt, ok);
+      const float2 b = trace_uface(t, i, j, -dt, ok);
       const float back = t.Uh.bil(b.x - K(0), b.y - K(0.5), ok);
       const float2 d = S.dU;
       const float* pU = t.U.at(toint(floorf(d.x - K(0))), toint(floorf(d.y - K(0.5))), ok);
@@ -409,9 +437,8 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       F.U2[j * (nx + 1) + i] = u;
     }
     if (ov) {  // velCorrect, V (:91-103)
-      const float gxv = (float)i + K(0.5), gyv = (float)j;
       bool ok = S.okV;
-      const float2 b = trace_l(t, gxv, gyv, -dt, ok);
+      const float2 b = trace_vface(t, i, j, -dt, ok);
       const float back = t.Vh.bil(b.x - K(0.5), b.y - K(0), ok);
       const float2 d = S.dV;
       const float* pV = t.V.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0))), ok);
@@ -434,8 +461,8 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
     Slot S2;
     S2.dC = S2.dU = S2.dV = make_float2(0.f, 0.f);
     S2.okC = S2.okU = S2.okV = false;
-    if (ou) { S2.okU = true; S2.dU = trace_l(t, (float)i, (float)j + K(0.5), dt, S2.okU); }
-    if (ov) { S2.okV = true; S2.dV = trace_l(t, (float)i + K(0.5), (float)j, dt, S2.okV); }
+    if (ou) { S2.okU = true; S2.dU = trace_uface(t, i, j, dt, S2.okU); }
+    if (ov) { S2.okV = true; S2.dV = trace_vface(t, i, j, dt, S2.okV); }
     c1(tid + 512, S2);
   }
 }

@@ -70,12 +70,14 @@ __device__ __forceinline__ float muOf(const cf_params& P, float T) {
   return K(1.716e-5) * pow15(div_rcp_fast(T, K(273.15), K(1) / K(273.15))) * K(273.15 + 110.4) / (T + K(110.4)) *
          P.viscMult;
 }
-__device__ __forceinline__ float rhoDOf(const cf_params& P, float T) {
+// iPr = 1/P.Pr, computed once by the caller
+__device__ __forceinline__ float rhoDOf(const cf_params& P, float T, float iPr) {
   return div_rcp_fast(
              K(1.716e-5) * pow15(div_rcp_fast(T, K(273.15), K(1) / K(273.15))) * K(273.15 + 110.4) / (T + K(110.4)),
-             P.Pr, K(1) / P.Pr) *
+             P.Pr, iPr) *
          P.diffMult;
 }
+__device__ __forceinline__ float rhoDOf(const cf_params& P, float T) { return rhoDOf(P, T, K(1) / P.Pr); }
 __device__ __forceinline__ float invRho(const cf_params& P, float T) { return T / (P.rhoAmb * P.Tamb); }
 
 // Hardware places workgroup b on XCD (b mod 8).  Give each XCD a contiguous

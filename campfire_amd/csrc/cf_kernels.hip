@@ -122,6 +122,7 @@ __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny,
   const int k = j * nx + i;
   const float T0 = Tin[k];
   const float4 Y0 = Yin[k];
+  const float iPr = K(1) / P.Pr;
   float sT = K(0);
   float4 sY = make_float4(0.f, 0.f, 0.f, 0.f);
 #define CF_DIFF_NB(cond, fidx, farr, n)                                       \
@@ -130,7 +131,7 @@ __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny,
     if (f > K(0)) {                                                           \
       const float Tn = Tin[n];                                                \
       const float4 Yn = Yin[n];                                               \
-      const float a = f * rhoDOf(P, K(0.5) * (T0 + Tn));                      \
+      const float a = f * rhoDOf(P, K(0.5) * (T0 + Tn), iPr);                 \
       sT += a * (Tn - T0);                                                    \
       sY.x += a * (Yn.x - Y0.x); sY.y += a * (Yn.y - Y0.y);                   \
       sY.z += a * (Yn.z - Y0.z); sY.w += a * (Yn.w - Y0.w);                   \
@@ -185,9 +186,10 @@ __global__ __launch_bounds__(256) void k_diffuse(const DiffuseOp::Args a_) {
 // -------------------------------------------------------------------------
 // pointwise sources of one cell (shaders/sources.wgsl:41-157)
 // -------------------------------------------------------------------------
+// tamb4 = powf(P.Tamb, 4), evaluated once per workgroup by the caller
 __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F, int i, int j, float Tc, float4 Yc,
                                              float& Tout, float4& Yout, float2& SQout,
-                                             float& qrel, float& qrad, float& tpeak, int* sacc) {
+                                             float& qrel, float& qrad, float& tpeak, int* sacc, float tamb4) {
   const int nx = P.nx;
   const int k = j * nx + i;
   const float dx = P.dx, dt = P.dt, cp = P.cp, sO2 = P.sO2, dHc = P.dHc;
@@ -283,8 +285,9 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
   const float fv = div_rcp(r1 * Yc.w, P.sootDensity, K(1) / P.sootDensity);
   const float kap = P.kappaSootC * fv * Tc + P.kappaGas * Yc.z;
   const float T3 = Tc * Tc * Tc;
-  qrad = K(4.0) * kap * P.sigma * (T3 * Tc - powf(P.Tamb, K(4.0)));
-  const float dTr = qrad * dt / (r1 * cp) / (K(1) + K(16.0) * kap * P.sigma * T3 * dt / (r1 * cp));
+  qrad = K(4.0) * kap * P.sigma * (T3 * Tc - tamb4);
+  const float rc = r1 * cp, irc = K(1) / rc;  // two quotients by r1 * cp
+  const float dTr = div_rcp(qrad * dt, rc, irc) / (K(1) + div_rcp(K(16.0) * kap * P.sigma * T3 * dt, rc, irc));
   Tc -= dTr;
   // entrainment through the front/back faces (:140-148)
   const float ucx = K(0.5) * (F.U[j * (nx + 1) + i] + F.U[j * (nx + 1) + i + 1]);
@@ -337,10 +340,13 @@ struct SourcesOp {
   __shared__ float red[3][8][33];
   __shared__ int sacc[CF_ACC_GLOBAL];
   __shared__ int sglob[4][3];
+  __shared__ float stamb4;
   const int tid = threadIdx.y * 32 + threadIdx.x;
-  for (int q = tid; q < CF_ACC_GLOBAL; q += 256) sacc[q] = 0;
-  __syncthreads();
   const cf_params P = *F.P;
+  for (int q = tid; q < CF_ACC_GLOBAL; q += 256) sacc[q] = 0;
+  if (tid == 0) stamb4 = powf(P.Tamb, K(4.0));  // one wave evaluates the power
+  __syncthreads();
+  const float tamb4 = stamb4;
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(vb, nx, ny, i, j);
@@ -354,7 +360,7 @@ struct SourcesOp {
     float To;
     float4 Yo;
     float2 sq;
-    sources_cell(P, F, i, j, Tc, Yc, To, Yo, sq, qrel, qrad, tpeak, sacc);
+    sources_cell(P, F, i, j, Tc, Yc, To, Yo, sq, qrel, qrad, tpeak, sacc, tamb4);
     F.T[k] = To;
     F.Y[k] = Yo;
     F.SQ[k] = sq;
