@@ -250,6 +250,88 @@ def batched(args):
     return res
 
 
+def converged_batch(args):
+    """The same mixed batch at equal accuracy: every scene's V-cycles run to
+    ||r||/||rhs|| < tol (capped at `cap` cycles per step) in
+      (a) the plain per-scene FP32 form: one FluidSim per scene, stepped one
+          scene after another, FP32 V-cycles with the reference smoother
+          settings (nu 2, 120 coarsest sweeps);
+      (b) the batched mixed-precision form: one ScenePool stepping every
+          scene per launch, FP16 smoother with FP64 defect correction, nu 3
+          and 24 coarsest sweeps (the autotuner's pick for these scenes).
+    Steps are timed one at a time; each scene's V-cycles and achieved
+    residual after every step are recorded for both, so the accuracies are
+    compared, not assumed."""
+    from campfire_amd import scenes
+    from campfire_amd.fluid import FluidSim
+    from campfire_amd.pool import ScenePool
+    tol, cap, K = 1e-4, 40, args.conv_steps
+    cfgs = scenes.config_batch(args.seed, args.batch)
+    cells = sum(c["nx"] * c["ny"] for c in cfgs)
+    # (b) batched mixed precision
+    pool = ScenePool(200 << 30)
+    pool.set_schedule(nu=3, vcycles=cap, tol=tol, coarse_iters=24, refine=1, smoother_fp16=1)
+    views = []
+    for c in cfgs:
+        slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
+        scenes.apply(sim, c)
+        views.append(sim)
+    pool.frame(8, True)
+    pool.step(2)
+    pool.synchronize()
+    tb, rb, cb = 0.0, [], []
+    for q in range(K):
+        t0 = time.perf_counter()
+        pool.step(1)
+        pool.synchronize()
+        tb += time.perf_counter() - t0
+        rb += [float(v.info().last_resid) for v in views]
+        cb += [int(v.info().last_vcycles) for v in views]
+    lat = pool.step_latencies(K)
+    pool.close()
+    # (a) plain per-scene FP32
+    sims = []
+    for c in cfgs:
+        sim = FluidSim({"nx": c["nx"], "ny": c["ny"], "dt": c["dt"]}, dx=c["dx"])
+        scenes.apply(sim, c)
+        sim.set_schedule(nu=2, vcycles=cap, tol=tol, coarse_iters=120)
+        sim.frame_all(8, True)
+        sim.step(2)
+        sims.append(sim)
+    for sim in sims:
+        sim.synchronize()
+    ts, ra, ca = 0.0, [], []
+    for q in range(K):
+        for sim in sims:
+            t0 = time.perf_counter()
+            sim.step(1)
+            sim.synchronize()
+            ts += time.perf_counter() - t0
+            ra.append(float(sim.info().last_resid))
+            ca.append(int(sim.info().last_vcycles))
+    for sim in sims:
+        sim.close()
+    res = {"tol": tol, "cycle_cap": cap, "steps": K, "scenes": len(cfgs), "cells": cells,
+           "batched_mixed": {"mcell_steps_per_s": round(cells * K / tb / 1e6, 1),
+                             "schedule": "nu 3, 24 coarsest sweeps, FP16 smoother, FP64 defect correction",
+                             "resid_max": max(rb), "cycles_mean": round(float(np.mean(cb)), 1),
+                             "steps_at_cap": int(sum(1 for c in cb if c >= cap)),
+                             "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                                 "p99": round(float(np.percentile(lat, 99)), 4)}},
+           "per_scene_fp32": {"mcell_steps_per_s": round(cells * K / ts / 1e6, 1),
+                              "schedule": "nu 2, 120 coarsest sweeps, FP32",
+                              "resid_max": max(ra), "cycles_mean": round(float(np.mean(ca)), 1),
+                              "steps_at_cap": int(sum(1 for c in ca if c >= cap))},
+           "speedup": round(ts / tb, 3)}
+    log("[bench] equal accuracy (tol %.0e): batched mixed %.1f Mcell-steps/s (resid max %.2e, %.1f cycles, "
+        "%d scene-steps at the cap), per-scene FP32 %.1f (resid max %.2e, %.1f cycles, %d at the cap): x%.2f; "
+        "batched p99 %.2f ms"
+        % (tol, res["batched_mixed"]["mcell_steps_per_s"], max(rb), np.mean(cb), res["batched_mixed"]["steps_at_cap"],
+           res["per_scene_fp32"]["mcell_steps_per_s"], max(ra), np.mean(ca), res["per_scene_fp32"]["steps_at_cap"],
+           res["speedup"], res["batched_mixed"]["step_latency_ms"]["p99"]))
+    return res
+
+
 def bench_config(args):
     from campfire_amd import scenes
     cfg = scenes.config_pile_1024(args.seed)
@@ -295,6 +377,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
     ap.add_argument("--batch", type=int, default=64, help="scenes of the batched measurement (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=40)
+    ap.add_argument("--conv-steps", type=int, default=6,
+                    help="steps of the equal-accuracy batch comparison (0: skip)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -405,6 +489,8 @@ def main():
     sim.close()
     if args.batch > 0:
         out["batched"] = batched(args)
+        if args.conv_steps > 0:
+            out["converged_batch"] = converged_batch(args)
     if not args.no_cpu:
         log("[bench] CPU baseline (FP32 oracle)")
         out["cpu_baseline"] = cpu_baseline(cfg)

@@ -203,6 +203,8 @@ bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, c
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
 // FP16 copies of a level's face coefficients (once per step, FP16 smoother)
 void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s);
+// tile edge (32 or 16) of a level's legs; a batched phase holds one tile size
+int mg_tile(const Level& f);
 // first level of the single-workgroup bottom solver (the last 1-3 levels)
 int mg_bottom_first(const Level* L, int nlev);
 // coarse_coef: restrict the coarsest level's coefficients inside the kernel

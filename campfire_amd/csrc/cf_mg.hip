@@ -16,35 +16,39 @@
 namespace cf {
 
 // ---- tiled legs ----------------------------------------------------------------
-// A tile of TX x TY fine cells plus a halo of H cells lives in LDS.  Each
-// thread owns horizontal pairs of region cells (one red, one black), keeps
-// their coefficients in registers and, per half-sweep, updates the pair's cell
-// of that colour: a branch-free update whose missing neighbours (domain edge)
-// read the cell itself with a zero coefficient.  Cells on the region edge
-// whose outward neighbour is a real cell are never updated; that error moves
-// inward one ring per half-sweep and stays out of the tile interior.
-struct PairCell {
-  float w, e, s, n, diag, rhs;
-  float y;  // 1 / diag (correctly rounded; 0 for a closed cell)
-  int nb;   // bit 0..3: neighbour west, east, south, north exists
-  bool upd, in;
+// A tile of TX x TY fine cells plus a halo of H cells (the region) lives in
+// LDS inside one more ring of zeros, so every cell's four neighbours sit at
+// fixed offsets.  Each thread owns horizontal pairs of region cells, one of
+// each colour, and keeps per cell its LDS index and flags, the four face
+// coefficients (zero towards a missing neighbour, whose value is then taken
+// times 0), the diagonal, its reciprocal and rhs*h2 in registers; the red
+// and the black cells of its pairs are sorted apart at load, so a half-sweep
+// updates one set without selects.  Cells on the region edge whose outward
+// neighbour is a real cell are never updated; that error moves inward one
+// ring per half-sweep and stays out of the tile interior.
+struct LegCell {
+  int ci;  // padded LDS index | LEG_UPD | LEG_IN
+  float w, e, s, n, d, y, rh;
 };
+constexpr int LEG_UPD = 1 << 16, LEG_IN = 1 << 17, LEG_IDX = 0xffff;
 
 __device__ __forceinline__ float ldc(const float* a, int k) { return a[k]; }
 __device__ __forceinline__ float ldc(const __half* a, int k) { return __half2float(a[k]); }
 
-// CT: float, or __half for the FP16 smoother's coefficient copies
+// cell (x, y) at region position (rx, ry); CT: float, or __half for the FP16
+// smoother's coefficient copies.  rhs_out: its rhs (0 outside the domain).
 template <class CT>
-__device__ __forceinline__ PairCell pair_cell(int nx, int ny, const CT* __restrict__ ax, const CT* __restrict__ ay,
-                                              const float* __restrict__ rhs, int x, int y, int rx, int ry, int RX,
-                                              int RY) {
-  PairCell c;
-  c.in = x >= 0 && x < nx && y >= 0 && y < ny;
-  c.upd = c.in && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
-          !(ry == RY - 1 && y < ny - 1);
-  c.w = c.e = c.s = c.n = c.diag = c.rhs = c.y = 0.f;
-  c.nb = 0;
-  if (c.in) {
+__device__ __forceinline__ LegCell leg_cell(int nx, int ny, const CT* __restrict__ ax, const CT* __restrict__ ay,
+                                            const float* __restrict__ rhs, float h2, int x, int y, int rx, int ry,
+                                            int RX, int RY, int PX, float& rhs_out) {
+  LegCell c;
+  const bool in = x >= 0 && x < nx && y >= 0 && y < ny;
+  const bool upd = in && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
+                   !(ry == RY - 1 && y < ny - 1);
+  c.ci = ((ry + 1) * PX + rx + 1) | (upd ? LEG_UPD : 0) | (in ? LEG_IN : 0);
+  c.w = c.e = c.s = c.n = c.d = c.y = c.rh = 0.f;
+  rhs_out = 0.f;
+  if (in) {
     const float w = ldc(ax, y * (nx + 1) + x), e = ldc(ax, y * (nx + 1) + x + 1);
     const float s = ldc(ay, y * nx + x), n = ldc(ay, (y + 1) * nx + x);
     float diag = 0.f;  // shaders/pressure.wgsl:77-85, same accumulation order
@@ -52,24 +56,41 @@ __device__ __forceinline__ PairCell pair_cell(int nx, int ny, const CT* __restri
     if (x < nx - 1) diag += e; else diag += K(2.0) * e;
     if (y > 0) diag += s;
     if (y < ny - 1) diag += n; else diag += K(2.0) * n;
-    c.diag = diag;
+    c.d = diag;
     c.y = diag > K(0) ? K(1) / diag : K(0);
     c.w = x > 0 ? w : 0.f;
     c.e = x < nx - 1 ? e : 0.f;
     c.s = y > 0 ? s : 0.f;
     c.n = y < ny - 1 ? n : 0.f;
-    c.nb = (x > 0 ? 1 : 0) | (x < nx - 1 ? 2 : 0) | (y > 0 ? 4 : 0) | (y < ny - 1 ? 8 : 0);
-    c.rhs = rhs[y * nx + x];
+    rhs_out = rhs[y * nx + x];
+    c.rh = rhs_out * h2;
   }
   return c;
 }
 
-// the fine level's cell (x, y) from the FP32 or the FP16 coefficients
 template <int M>
-__device__ __forceinline__ PairCell pair_cell_m(const Level& f, const MgStore& st, int x, int y, int rx, int ry, int RX,
-                                                int RY) {
-  if constexpr ((M & 1) != 0) return pair_cell(f.nx, f.ny, st.f.ax, st.f.ay, f.rhs, x, y, rx, ry, RX, RY);
-  else return pair_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, x, y, rx, ry, RX, RY);
+__device__ __forceinline__ LegCell leg_cell_m(const Level& f, const MgStore& st, int x, int y, int rx, int ry, int RX,
+                                              int RY, int PX, float& rhs_out) {
+  if constexpr ((M & 1) != 0)
+    return leg_cell(f.nx, f.ny, st.f.ax, st.f.ay, f.rhs, f.h2, x, y, rx, ry, RX, RY, PX, rhs_out);
+  else return leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx, ry, RX, RY, PX, rhs_out);
+}
+
+// a slot without a cell: reads around the first region cell, never writes
+__device__ __forceinline__ LegCell leg_none(int PX) {
+  LegCell c;
+  c.ci = PX + 1;
+  c.w = c.e = c.s = c.n = c.d = c.y = c.rh = 0.f;
+  return c;
+}
+
+// the red-black update of shaders/pressure.wgsl:87-94 (terms in its order)
+template <int PX>
+__device__ __forceinline__ void leg_relax(const LegCell& c, float* sp) {
+  const int ci = c.ci & LEG_IDX;
+  const float off = c.w * sp[ci - 1] + c.e * sp[ci + 1] + c.s * sp[ci - PX] + c.n * sp[ci + PX];
+  const float v = c.d > K(0) ? div_rcp_fast(off - c.rh, c.d, c.y) : K(0);
+  if (c.ci & LEG_UPD) sp[ci] = v;
 }
 
 // p of a level: FP32, or FP16 holding p * sig (read back as h * isig, exact)
@@ -84,38 +105,10 @@ __device__ __forceinline__ void stp(float* p, __half* ph, int k, float v, float 
   else p[k] = v;
 }
 
-__device__ __forceinline__ PairCell pair_none() {
-  PairCell c;
-  c.w = c.e = c.s = c.n = c.diag = c.rhs = c.y = 0.f;
-  c.nb = 0;
-  c.upd = c.in = false;
-  return c;
-}
-
-__device__ __forceinline__ float pair_off(const PairCell& c, const float* sp, int ci, int RX) {
-  const int iw = ci - (c.nb & 1), ie = ci + ((c.nb >> 1) & 1);
-  const int is = ci - RX * ((c.nb >> 2) & 1), in = ci + RX * ((c.nb >> 3) & 1);
-  return c.w * sp[iw] + c.e * sp[ie] + c.s * sp[is] + c.n * sp[in];
-}
-
-// one half-sweep of colour col over the thread's pairs
-template <int PQ, int RX, int NP>
-__device__ __forceinline__ void pairs_relax(const PairCell (&c)[PQ][2], const int (&base)[PQ], const int (&par)[PQ],
-                                            float* sp, float h2, int col) {
-#pragma unroll
-  for (int q = 0; q < PQ; q++) {
-    const int sub = par[q] == col ? 0 : 1;  // which cell of the pair has colour col
-    const PairCell& a = sub ? c[q][1] : c[q][0];
-    const int ci = base[q] + sub;
-    const float off = pair_off(a, sp, ci, RX);
-    const float v = a.diag > K(0) ? div_rcp_fast(off - a.rhs * h2, a.diag, a.y) : K(0);
-    if (a.upd) sp[ci] = v;
-  }
-}
-
 // M (storage, MgStore::mode): bit 0, the fine level's p and coefficients in
 // FP16; bit 1, the coarse level's p in FP16.  Sweeps, residual and
-// restriction are FP32 arithmetic either way.
+// restriction are FP32 arithmetic either way.  The fine p is read from p and
+// written to the leg buffer q.
 template <int TX, int TY, int NU, int M>
 struct MgDownOp {
   struct Args {
@@ -126,78 +119,88 @@ struct MgDownOp {
     int vg;  // tiles of this scene's launch
   };
   static constexpr int kThreads = 256;
+  static constexpr int H = 2 * NU + 1;
+  static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, PX = RX + 2, PY = RY + 2, NP = RX * RY / 2;
+  static constexpr int PQ = (NP + 255) / 256;
+  static_assert(RX % 2 == 0, "pairs need an even region width");
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Level& f = a_.f;
     const Level& c = a_.c;
-    const int* __restrict__ skip = a_.skip;
-
-  if (skip && *skip) return;  // the step's residual target is met
-  constexpr int H = 2 * NU + 1;
-  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
-  constexpr int PQ = (NP + 255) / 256;
-  static_assert(RX % 2 == 0, "pairs need an even region width");
-  __shared__ float sp[NR];
-  __shared__ float sres[TX * TY];
-  const int nx = f.nx, ny = f.ny;
-  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
-  const int t = vb;
-  const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
-  const int tid = threadIdx.x;
-  const float h2 = f.h2;
-  const float sig = M ? a_.st.dyn->sig : 1.f, isig = M ? a_.st.dyn->isig : 1.f;
-  PairCell cell[PQ][2];
-  int base[PQ], par[PQ];
-#pragma unroll
-  for (int q = 0; q < PQ; q++) {
-    const int pi = tid + 256 * q;
-    const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
-    const int x = x0 - H + rx, y = y0 - H + ry;
-    base[q] = ry * RX + rx;
-    par[q] = (x + y) & 1;
-    if (pi < NP) {
-      cell[q][0] = pair_cell_m<M>(f, a_.st, x, y, rx, ry, RX, RY);
-      cell[q][1] = pair_cell_m<M>(f, a_.st, x + 1, y, rx + 1, ry, RX, RY);
-      sp[base[q]] = cell[q][0].in ? ldp<(M & 1) != 0>(f.p, a_.st.f.p, y * nx + x, isig) : 0.f;
-      sp[base[q] + 1] = cell[q][1].in ? ldp<(M & 1) != 0>(f.p, a_.st.f.p, y * nx + x + 1, isig) : 0.f;
-    } else {
-      cell[q][0] = cell[q][1] = pair_none();
-      base[q] = 0;
+    if (a_.skip && *a_.skip) return;  // the step's residual target is met
+    __shared__ float sp[PX * PY];
+    __shared__ float sres[TX * TY];  // the interior's rhs, then its residual
+    constexpr bool FH = (M & 1) != 0;
+    const int nx = f.nx;
+    const int gx = (nx + TX - 1) / TX;
+    const int x0 = (vb % gx) * TX, y0 = (vb / gx) * TY;
+    const int tid = threadIdx.x;
+    const float h2 = f.h2;
+    const float sig = M ? a_.st.dyn->sig : 1.f, isig = M ? a_.st.dyn->isig : 1.f;
+    for (int k = tid; k < 2 * PX + 2 * RY; k += 256) {  // the ring of zeros
+      const int idx = k < PX ? k : (k < 2 * PX ? (PY - 1) * PX + (k - PX) : ((k - 2 * PX) / 2 + 1) * PX +
+                                                                               ((k & 1) ? PX - 1 : 0));
+      sp[idx] = 0.f;
     }
-  }
-  __syncthreads();
-  for (int hs = 0; hs < 2 * NU; hs++) {
-    pairs_relax<PQ, RX, NP>(cell, base, par, sp, h2, hs & 1);  // red (0) then black (1), js/fluid.js:298-301
-    __syncthreads();
-  }
-  // interior: write p, residual into sres (shaders/pressure.wgsl:96-101)
+    LegCell cR[PQ], cB[PQ];
 #pragma unroll
-  for (int q = 0; q < PQ; q++) {
-#pragma unroll
-    for (int sub = 0; sub < 2; sub++) {
-      const PairCell& a = cell[q][sub];
-      const int ci = base[q] + sub;
-      const int rx = ci % RX, ry = ci / RX;
-      if (a.in && rx >= H && rx < H + TX && ry >= H && ry < H + TY) {
-        const float pk = sp[ci];
-        const float off = pair_off(a, sp, ci, RX);
-        sres[(ry - H) * TX + (rx - H)] = a.diag <= K(0) ? K(0) : a.rhs - (off - a.diag * pk) / h2;
-        stp<(M & 1) != 0>(f.q, a_.st.f.q, (y0 - H + ry) * nx + (x0 - H + rx), pk, sig);
+    for (int q = 0; q < PQ; q++) {
+      const int pi = tid + 256 * q;
+      cR[q] = cB[q] = leg_none(PX);
+      if (pi < NP) {
+        const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
+        const int x = x0 - H + rx, y = y0 - H + ry;
+        float r0, r1;
+        const LegCell c0 = leg_cell_m<M>(f, a_.st, x, y, rx, ry, RX, RY, PX, r0);
+        const LegCell c1 = leg_cell_m<M>(f, a_.st, x + 1, y, rx + 1, ry, RX, RY, PX, r1);
+        sp[c0.ci & LEG_IDX] = (c0.ci & LEG_IN) ? ldp<FH>(f.p, a_.st.f.p, y * nx + x, isig) : 0.f;
+        sp[c1.ci & LEG_IDX] = (c1.ci & LEG_IN) ? ldp<FH>(f.p, a_.st.f.p, y * nx + x + 1, isig) : 0.f;
+        if (ry >= H && ry < H + TY) {
+          if (rx >= H && rx < H + TX) sres[(ry - H) * TX + (rx - H)] = r0;
+          if (rx + 1 >= H && rx + 1 < H + TX) sres[(ry - H) * TX + (rx + 1 - H)] = r1;
+        }
+        const bool red0 = ((x + y) & 1) == 0;
+        cR[q] = red0 ? c0 : c1;
+        cB[q] = red0 ? c1 : c0;
       }
     }
-  }
-  __syncthreads();
-  // restriction to the coarse rhs, coarse p = 0 (:104-113)
-  for (int k = tid; k < (TX / 2) * (TY / 2); k += 256) {
-    const int li = k % (TX / 2), lj = k / (TX / 2);
-    const int I = x0 / 2 + li, J = y0 / 2 + lj;
-    if (I < c.nx && J < c.ny) {
-      const float* r0 = sres + (2 * lj) * TX + 2 * li;
-      const float r = r0[0] + r0[1] + r0[TX] + r0[TX + 1];
-      c.rhs[J * c.nx + I] = K(0.25) * r;
-      stp<(M & 2) != 0>(c.p, a_.st.c.p, J * c.nx + I, K(0), sig);
+    __syncthreads();
+    for (int s = 0; s < NU; s++) {  // red then black (js/fluid.js:298-301)
+#pragma unroll
+      for (int q = 0; q < PQ; q++) leg_relax<PX>(cR[q], sp);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PQ; q++) leg_relax<PX>(cB[q], sp);
+      __syncthreads();
     }
-  }
-
+    // interior: p into the leg buffer, residual into sres (shaders/pressure.wgsl:96-101)
+    auto finish = [&](const LegCell& a) {
+      const int ci = a.ci & LEG_IDX;
+      const int rx = ci % PX - 1, ry = ci / PX - 1;
+      if ((a.ci & LEG_IN) && rx >= H && rx < H + TX && ry >= H && ry < H + TY) {
+        const float pk = sp[ci];
+        const float off = a.w * sp[ci - 1] + a.e * sp[ci + 1] + a.s * sp[ci - PX] + a.n * sp[ci + PX];
+        const int r = (ry - H) * TX + (rx - H);
+        sres[r] = a.d <= K(0) ? K(0) : sres[r] - (off - a.d * pk) / h2;
+        stp<FH>(f.q, a_.st.f.q, (y0 - H + ry) * nx + (x0 - H + rx), pk, sig);
+      }
+    };
+#pragma unroll
+    for (int q = 0; q < PQ; q++) {
+      finish(cR[q]);
+      finish(cB[q]);
+    }
+    __syncthreads();
+    // restriction to the coarse rhs, coarse p = 0 (:104-113)
+    for (int k = tid; k < (TX / 2) * (TY / 2); k += 256) {
+      const int li = k % (TX / 2), lj = k / (TX / 2);
+      const int I = x0 / 2 + li, J = y0 / 2 + lj;
+      if (I < c.nx && J < c.ny) {
+        const float* r0 = sres + (2 * lj) * TX + 2 * li;
+        const float r = r0[0] + r0[1] + r0[TX] + r0[TX + 1];
+        c.rhs[J * c.nx + I] = K(0.25) * r;
+        stp<(M & 2) != 0>(c.p, a_.st.c.p, J * c.nx + I, K(0), sig);
+      }
+    }
   }
 };
 template <int TX, int TY, int NU, int M>
@@ -205,6 +208,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const typename MgDownOp<TX, TY,
   MgDownOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// the fine p is read from the leg buffer q and written to p
 template <int TX, int TY, int NU, int M>
 struct MgUpOp {
   struct Args {
@@ -215,76 +219,81 @@ struct MgUpOp {
     int vg;  // tiles of this scene's launch
   };
   static constexpr int kThreads = 256;
+  static constexpr int H = 2 * NU;
+  static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, PX = RX + 2, PY = RY + 2, NP = RX * RY / 2;
+  static constexpr int PQ = (NP + 255) / 256;
+  static_assert(RX % 2 == 0, "pairs need an even region width");
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Level& f = a_.f;
     const Level& c = a_.c;
-    const int* __restrict__ skip = a_.skip;
-
-  if (skip && *skip) return;
-  constexpr int H = 2 * NU;
-  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, NR = RX * RY, NP = NR / 2;
-  constexpr int PQ = (NP + 255) / 256;
-  static_assert(RX % 2 == 0, "pairs need an even region width");
-  __shared__ float sp[NR];
-  const int nx = f.nx, ny = f.ny;
-  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
-  const int t = vb;
-  const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
-  const int tid = threadIdx.x;
-  const float h2 = f.h2;
-  const int nxc = c.nx, nyc = c.ny;
-  const float sig = M ? a_.st.dyn->sig : 1.f, isig = M ? a_.st.dyn->isig : 1.f;
-  constexpr bool FH = (M & 1) != 0, CH = (M & 2) != 0;
-  PairCell cell[PQ][2];
-  int base[PQ], par[PQ];
+    if (a_.skip && *a_.skip) return;
+    __shared__ float sp[PX * PY];
+    constexpr bool FH = (M & 1) != 0, CH = (M & 2) != 0;
+    const int nx = f.nx;
+    const int gx = (nx + TX - 1) / TX;
+    const int x0 = (vb % gx) * TX, y0 = (vb / gx) * TY;
+    const int tid = threadIdx.x;
+    const int nxc = c.nx, nyc = c.ny;
+    const float sig = M ? a_.st.dyn->sig : 1.f, isig = M ? a_.st.dyn->isig : 1.f;
+    for (int k = tid; k < 2 * PX + 2 * RY; k += 256) {  // the ring of zeros
+      const int idx = k < PX ? k : (k < 2 * PX ? (PY - 1) * PX + (k - PX) : ((k - 2 * PX) / 2 + 1) * PX +
+                                                                               ((k & 1) ? PX - 1 : 0));
+      sp[idx] = 0.f;
+    }
+    LegCell cR[PQ], cB[PQ];
 #pragma unroll
-  for (int q = 0; q < PQ; q++) {
-    const int pi = tid + 256 * q;
-    const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
-    const int y = y0 - H + ry;
-    base[q] = ry * RX + rx;
-    par[q] = (x0 - H + rx + y) & 1;
-    if (pi < NP) {
+    for (int q = 0; q < PQ; q++) {
+      const int pi = tid + 256 * q;
+      cR[q] = cB[q] = leg_none(PX);
+      if (pi < NP) {
+        const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
+        const int y = y0 - H + ry;
+        LegCell cc[2];
 #pragma unroll
-      for (int sub = 0; sub < 2; sub++) {
-        const int x = x0 - H + rx + sub;
-        cell[q][sub] = pair_cell_m<M>(f, a_.st, x, y, rx + sub, ry, RX, RY);
-        float v = 0.f;
-        if (cell[q][sub].in) {
-          // bilinear coarse correction, skipped in closed cells (:116-129)
-          const int I = x / 2, J = y / 2;
-          const int I2 = iclamp(I + ((x & 1) == 1 ? 1 : -1), 0, nxc - 1);
-          const int J2 = iclamp(J + ((y & 1) == 1 ? 1 : -1), 0, nyc - 1);
-          const float c00 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I, isig);
-          const float c10 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I2, isig);
-          const float c01 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I, isig);
-          const float c11 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I2, isig);
-          const float cc = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
-          v = ldp<FH>(f.q, a_.st.f.q, y * nx + x, isig) + (cell[q][sub].diag > K(0) ? cc : K(0));
+        for (int sub = 0; sub < 2; sub++) {
+          const int x = x0 - H + rx + sub;
+          float rr;
+          cc[sub] = leg_cell_m<M>(f, a_.st, x, y, rx + sub, ry, RX, RY, PX, rr);
+          float v = 0.f;
+          if (cc[sub].ci & LEG_IN) {
+            // bilinear coarse correction, skipped in closed cells (:116-129)
+            const int I = x / 2, J = y / 2;
+            const int I2 = iclamp(I + ((x & 1) == 1 ? 1 : -1), 0, nxc - 1);
+            const int J2 = iclamp(J + ((y & 1) == 1 ? 1 : -1), 0, nyc - 1);
+            const float c00 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I, isig);
+            const float c10 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I2, isig);
+            const float c01 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I, isig);
+            const float c11 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I2, isig);
+            const float ccr = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
+            v = ldp<FH>(f.q, a_.st.f.q, y * nx + x, isig) + (cc[sub].d > K(0) ? ccr : K(0));
+          }
+          sp[cc[sub].ci & LEG_IDX] = v;
         }
-        sp[base[q] + sub] = v;
+        const bool red0 = ((x0 - H + rx + y) & 1) == 0;
+        cR[q] = red0 ? cc[0] : cc[1];
+        cB[q] = red0 ? cc[1] : cc[0];
       }
-    } else {
-      cell[q][0] = cell[q][1] = pair_none();
-      base[q] = 0;
     }
-  }
-  __syncthreads();
-  for (int hs = 0; hs < 2 * NU; hs++) {
-    pairs_relax<PQ, RX, NP>(cell, base, par, sp, h2, (hs & 1) ? 0 : 1);  // black then red, js/fluid.js:307-310
     __syncthreads();
-  }
+    for (int s = 0; s < NU; s++) {  // black then red (js/fluid.js:307-310)
 #pragma unroll
-  for (int q = 0; q < PQ; q++) {
+      for (int q = 0; q < PQ; q++) leg_relax<PX>(cB[q], sp);
+      __syncthreads();
 #pragma unroll
-    for (int sub = 0; sub < 2; sub++) {
-      const int ci = base[q] + sub;
-      const int rx = ci % RX, ry = ci / RX;
-      if (cell[q][sub].in && rx >= H && rx < H + TX && ry >= H && ry < H + TY)
-        stp<FH>(f.p, a_.st.f.p, (y0 - H + ry) * nx + (x0 - H + rx), sp[ci], sig);
+      for (int q = 0; q < PQ; q++) leg_relax<PX>(cR[q], sp);
+      __syncthreads();
     }
-  }
-
+    auto finish = [&](const LegCell& a) {
+      const int ci = a.ci & LEG_IDX;
+      const int rx = ci % PX - 1, ry = ci / PX - 1;
+      if ((a.ci & LEG_IN) && rx >= H && rx < H + TX && ry >= H && ry < H + TY)
+        stp<FH>(f.p, a_.st.f.p, (y0 - H + ry) * nx + (x0 - H + rx), sp[ci], sig);
+    };
+#pragma unroll
+    for (int q = 0; q < PQ; q++) {
+      finish(cR[q]);
+      finish(cB[q]);
+    }
   }
 };
 template <int TX, int TY, int NU, int M>
@@ -717,8 +726,6 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const MgBottomOp::Args a_) {
 }
 
 // ---- launchers -------------------------------------------------------------------
-static constexpr int MG_TX = 32, MG_TY = 32;
-
 template <int TX, int TY, int NU, int M>
 static auto k_mg_down_of(MgDownOp<TX, TY, NU, M>) { return k_mg_down<TX, TY, NU, M>; }
 template <int TX, int TY, int NU, int M>
@@ -728,45 +735,54 @@ static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
   return (unsigned)(((nx + tx - 1) / tx) * ((ny + ty - 1) / ty));
 }
 
-// dispatch on (nu, storage mode) to the instantiation
+// 32x32 tiles where a level has at least 1024 of them, 16x16 below: a small
+// level then has four times the workgroups, each with half the per-thread
+// chain, and its legs are a shorter latency chain
+int mg_tile(const Level& f) { return ntiles(f.nx, f.ny, 32, 32) >= 1024 ? 32 : 16; }
+
+// dispatch on (tile, nu, storage mode) to the instantiation
 template <template <int, int, int, int> class Op, class F>
-static bool mg_dispatch(int nu, int mode, F&& f) {
-#define CF_MG_CASE(NU, M) \
-  if (nu == NU && mode == M) { f(Op<MG_TX, MG_TY, NU, M>{}); return true; }
-  CF_MG_CASE(1, 0) CF_MG_CASE(2, 0) CF_MG_CASE(3, 0)
-  CF_MG_CASE(1, 1) CF_MG_CASE(2, 1) CF_MG_CASE(3, 1)
-  CF_MG_CASE(1, 3) CF_MG_CASE(2, 3) CF_MG_CASE(3, 3)
+static bool mg_dispatch(int tile, int nu, int mode, F&& f) {
+#define CF_MG_CASE(T, NU, M) \
+  if (tile == T && nu == NU && mode == M) { f(Op<T, T, NU, M>{}); return true; }
+#define CF_MG_TILE(T) \
+  CF_MG_CASE(T, 1, 0) CF_MG_CASE(T, 2, 0) CF_MG_CASE(T, 3, 0) \
+  CF_MG_CASE(T, 1, 1) CF_MG_CASE(T, 2, 1) CF_MG_CASE(T, 3, 1) \
+  CF_MG_CASE(T, 1, 3) CF_MG_CASE(T, 2, 3) CF_MG_CASE(T, 3, 3)
+  CF_MG_TILE(32)
+  CF_MG_TILE(16)
+#undef CF_MG_TILE
 #undef CF_MG_CASE
   return false;
 }
 
 bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s) {
-  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  return mg_dispatch<MgDownOp>(nu, st.mode, [&](auto op) {
+  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+  return mg_dispatch<MgDownOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
     launch(k_mg_down_of(op), dim3(nb), dim3(256), 0, s, typename Op::Args{f, c, st, skip, nb});
   });
 }
 
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s) {
-  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  return mg_dispatch<MgUpOp>(nu, st.mode, [&](auto op) {
+  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+  return mg_dispatch<MgUpOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
     launch(k_mg_up_of(op), dim3(nb), dim3(256), 0, s, typename Op::Args{f, c, st, skip, nb});
   });
 }
 
 bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip) {
-  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  return mg_dispatch<MgDownOp>(nu, st.mode, [&](auto op) {
+  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+  return mg_dispatch<MgDownOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
     phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(256), "mg_down");
   });
 }
 
 bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip) {
-  const int nb = (int)ntiles(f.nx, f.ny, MG_TX, MG_TY);
-  return mg_dispatch<MgUpOp>(nu, st.mode, [&](auto op) {
+  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+  return mg_dispatch<MgUpOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
     phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(256), "mg_up");
   });

@@ -537,19 +537,23 @@ static int pool_build_step(cf_pool* pl) {
   }
   int maxb = 0;
   for (auto* s : sc) maxb = s->mg_bottom > maxb ? s->mg_bottom : maxb;
-  // a leg's kernel depends on the storage of its two levels (FP16 smoother:
-  // whether the coarse level is tiled too), so each level's legs form one
-  // phase per storage mode
-  const int modes[3] = {0, 1, 3};
+  // a leg's kernel depends on its tile size (from the level's size) and on
+  // the storage of its two levels (FP16 smoother: whether the coarse level is
+  // tiled too), so each level's legs form one phase per (tile, storage)
+  const int modes[3] = {0, 1, 3}, tiles[2] = {32, 16};
+  auto leg_ok = [&](cf_sim* s, int d, int t, int m) {
+    return d < s->mg_bottom && cf::mg_tile(s->L[d]) == t && mg_store(s, d).mode == m;
+  };
   for (int c = 0; c < k.vcycles; c++) {
     for (int d = 0; d < maxb; d++) {
-      for (int m : modes) {
-        cf::BatchPhase a;
-        for (auto* s : sc)
-          if (d < s->mg_bottom && mg_store(s, d).mode == m)
-            cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
-        if (!a.empty()) ph.push_back(a);
-      }
+      for (int t : tiles)
+        for (int m : modes) {
+          cf::BatchPhase a;
+          for (auto* s : sc)
+            if (leg_ok(s, d, t, m))
+              cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+          if (!a.empty()) ph.push_back(a);
+        }
     }
     {
       cf::BatchPhase& a = P();
@@ -561,13 +565,14 @@ static int pool_build_step(cf_pool* pl) {
       }
     }
     for (int d = maxb - 1; d >= 0; d--) {
-      for (int m : modes) {
-        cf::BatchPhase a;
-        for (auto* s : sc)
-          if (d < s->mg_bottom && mg_store(s, d).mode == m)
-            cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
-        if (!a.empty()) ph.push_back(a);
-      }
+      for (int t : tiles)
+        for (int m : modes) {
+          cf::BatchPhase a;
+          for (auto* s : sc)
+            if (leg_ok(s, d, t, m))
+              cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+          if (!a.empty()) ph.push_back(a);
+        }
     }
     if (dc) {
       {
