@@ -235,6 +235,7 @@ struct MgUpOp {
     const int tid = threadIdx.x;
     const int nxc = c.nx, nyc = c.ny;
     const float sig = M ? a_.st.dyn->sig : 1.f, isig = M ? a_.st.dyn->isig : 1.f;
+
     for (int k = tid; k < 2 * PX + 2 * RY; k += 256) {  // the ring of zeros
       const int idx = k < PX ? k : (k < 2 * PX ? (PY - 1) * PX + (k - PX) : ((k - 2 * PX) / 2 + 1) * PX +
                                                                                ((k & 1) ? PX - 1 : 0));
@@ -390,30 +391,6 @@ __device__ __forceinline__ void cells_residual(const CellSet<CPT>& C, int nx, co
     if (k >= 0) {
       const float off = cells_off<CPT>(C, q, p, nx);
       res[k] = C.diag[q] <= K(0) ? K(0) : rhs[k] - (off - C.diag[q] * p[k]) / h2;
-    }
-  }
-}
-
-// One half-sweep of colour col over a wave's cell pairs: slots 2q and 2q+1
-// hold linear cells 2m and 2m+1, which always differ in colour (same row when
-// nx is even; (nx-1, j) and (0, j+1) when nx is odd), so each pair updates
-// exactly one cell, chosen by select rather than a divergent branch.  rh holds
-// rhs*h2 of every slot.
-template <int NPAIR>
-__device__ __forceinline__ void cells_relax_pairs(const CellSet<2 * NPAIR>& C, const float (&rh)[2 * NPAIR], int nx,
-                                                  float* p, int col) {
-#pragma unroll
-  for (int q = 0; q < NPAIR; q++) {
-    const int a = 2 * q, b = 2 * q + 1;
-    const bool sa = ((C.nb[a] >> 4) & 1) == col;
-    const int k = sa ? C.k[a] : C.k[b], nb = sa ? C.nb[a] : C.nb[b];
-    const float w = sa ? C.w[a] : C.w[b], e = sa ? C.e[a] : C.e[b];
-    const float s = sa ? C.s[a] : C.s[b], n = sa ? C.n[a] : C.n[b];
-    const float diag = sa ? C.diag[a] : C.diag[b], r = sa ? rh[a] : rh[b], y = sa ? C.y[a] : C.y[b];
-    if (k >= 0) {
-      const float off = w * p[k - (nb & 1)] + e * p[k + ((nb >> 1) & 1)] + s * p[k - nx * ((nb >> 2) & 1)] +
-                        n * p[k + nx * ((nb >> 3) & 1)];
-      p[k] = diag > K(0) ? div_rcp_fast(off - r, diag, y) : K(0);
     }
   }
 }
