@@ -258,37 +258,52 @@ def converged_batch(args):
           settings (nu 2, 120 coarsest sweeps);
       (b) the batched mixed-precision form: one ScenePool stepping every
           scene per launch, FP16 smoother with FP64 defect correction, nu 3
-          and 24 coarsest sweeps (the autotuner's pick for these scenes).
+          and 24 coarsest sweeps;
+      (c) as (b) with the V-cycle preconditioning flexible conjugate
+          gradients at level 0 (FP64 pressure, residual and direction), at
+          the autotuner's pick for these scenes.
     Steps are timed one at a time; each scene's V-cycles and achieved
-    residual after every step are recorded for both, so the accuracies are
-    compared, not assumed."""
+    residual after every step are recorded for every form, so the accuracies
+    are compared, not assumed."""
     from campfire_amd import scenes
     from campfire_amd.fluid import FluidSim
     from campfire_amd.pool import ScenePool
     tol, cap, K = 1e-4, 40, args.conv_steps
     cfgs = scenes.config_batch(args.seed, args.batch)
     cells = sum(c["nx"] * c["ny"] for c in cfgs)
-    # (b) batched mixed precision
-    pool = ScenePool(200 << 30)
-    pool.set_schedule(nu=3, vcycles=cap, tol=tol, coarse_iters=24, refine=1, smoother_fp16=1)
-    views = []
-    for c in cfgs:
-        slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
-        scenes.apply(sim, c)
-        views.append(sim)
-    pool.frame(8, True)
-    pool.step(2)
-    pool.synchronize()
-    tb, rb, cb = 0.0, [], []
-    for q in range(K):
-        t0 = time.perf_counter()
-        pool.step(1)
+
+    def batched(sched):
+        pool = ScenePool(200 << 30)
+        pool.set_schedule(vcycles=cap, tol=tol, refine=1, smoother_fp16=1, **sched)
+        views = []
+        for c in cfgs:
+            slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
+            scenes.apply(sim, c)
+            views.append(sim)
+        pool.frame(8, True)
+        pool.step(2)
         pool.synchronize()
-        tb += time.perf_counter() - t0
-        rb += [float(v.info().last_resid) for v in views]
-        cb += [int(v.info().last_vcycles) for v in views]
-    lat = pool.step_latencies(K)
-    pool.close()
+        t, rb, cb = 0.0, [], []
+        for q in range(K):
+            t0 = time.perf_counter()
+            pool.step(1)
+            pool.synchronize()
+            t += time.perf_counter() - t0
+            rb += [float(v.info().last_resid) for v in views]
+            cb += [int(v.info().last_vcycles) for v in views]
+        lat = pool.step_latencies(K)
+        pool.close()
+        return {"mcell_steps_per_s": round(cells * K / t / 1e6, 1),
+                "schedule": "nu %d, %s coarsest, FP16 smoother, FP64 %s"
+                            % (sched["nu"], "%d-sweep" % sched["coarse_iters"] if sched["coarse_iters"] else "direct",
+                               "flexible CG" if sched.get("krylov") else "defect correction"),
+                "resid_max": max(rb), "cycles_mean": round(float(np.mean(cb)), 2),
+                "steps_at_cap": int(sum(1 for c in cb if c >= cap)),
+                "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                    "p99": round(float(np.percentile(lat, 99)), 4)}}, t
+
+    dcb, tdc = batched({"nu": 3, "coarse_iters": 24})
+    kry, tkr = batched({"nu": args.fcg_nu, "coarse_iters": args.fcg_coarse, "krylov": 1})
     # (a) plain per-scene FP32
     sims = []
     for c in cfgs:
@@ -312,23 +327,17 @@ def converged_batch(args):
     for sim in sims:
         sim.close()
     res = {"tol": tol, "cycle_cap": cap, "steps": K, "scenes": len(cfgs), "cells": cells,
-           "batched_mixed": {"mcell_steps_per_s": round(cells * K / tb / 1e6, 1),
-                             "schedule": "nu 3, 24 coarsest sweeps, FP16 smoother, FP64 defect correction",
-                             "resid_max": max(rb), "cycles_mean": round(float(np.mean(cb)), 1),
-                             "steps_at_cap": int(sum(1 for c in cb if c >= cap)),
-                             "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
-                                                 "p99": round(float(np.percentile(lat, 99)), 4)}},
+           "batched_mixed_fcg": kry, "batched_mixed": dcb,
            "per_scene_fp32": {"mcell_steps_per_s": round(cells * K / ts / 1e6, 1),
                               "schedule": "nu 2, 120 coarsest sweeps, FP32",
-                              "resid_max": max(ra), "cycles_mean": round(float(np.mean(ca)), 1),
+                              "resid_max": max(ra), "cycles_mean": round(float(np.mean(ca)), 2),
                               "steps_at_cap": int(sum(1 for c in ca if c >= cap))},
-           "speedup": round(ts / tb, 3)}
-    log("[bench] equal accuracy (tol %.0e): batched mixed %.1f Mcell-steps/s (resid max %.2e, %.1f cycles, "
-        "%d scene-steps at the cap), per-scene FP32 %.1f (resid max %.2e, %.1f cycles, %d at the cap): x%.2f; "
-        "batched p99 %.2f ms"
-        % (tol, res["batched_mixed"]["mcell_steps_per_s"], max(rb), np.mean(cb), res["batched_mixed"]["steps_at_cap"],
-           res["per_scene_fp32"]["mcell_steps_per_s"], max(ra), np.mean(ca), res["per_scene_fp32"]["steps_at_cap"],
-           res["speedup"], res["batched_mixed"]["step_latency_ms"]["p99"]))
+           "speedup": round(ts / tkr, 3), "speedup_defect_correction": round(ts / tdc, 3)}
+    for name, r in (("batched mixed + FCG", kry), ("batched mixed", dcb), ("per-scene FP32", res["per_scene_fp32"])):
+        log("[bench] equal accuracy (tol %.0e): %s %.1f Mcell-steps/s (resid max %.2e, %.2f cycles, %d scene-steps "
+            "at the cap)" % (tol, name, r["mcell_steps_per_s"], r["resid_max"], r["cycles_mean"], r["steps_at_cap"]))
+    log("[bench] equal accuracy: x%.2f (FCG), x%.2f (defect correction); batched FCG p99 %.2f ms"
+        % (res["speedup"], res["speedup_defect_correction"], kry["step_latency_ms"]["p99"]))
     return res
 
 
@@ -377,6 +386,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
     ap.add_argument("--batch", type=int, default=64, help="scenes of the batched measurement (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=40)
+    ap.add_argument("--fcg-nu", type=int, default=2, help="smoother sweeps of the batched FCG form")
+    ap.add_argument("--fcg-coarse", type=int, default=24, help="coarsest sweeps of the batched FCG form (0: direct)")
     ap.add_argument("--conv-steps", type=int, default=6,
                     help="steps of the equal-accuracy batch comparison (0: skip)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)

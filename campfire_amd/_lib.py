@@ -36,7 +36,7 @@ class CfParams(C.Structure):
 
 class CfSchedule(C.Structure):
     _fields_ = [("nu", C.c_int32), ("vcycles", C.c_int32), ("tol", C.c_float),
-                ("coarse_iters", C.c_int32), ("smoother_fp16", C.c_int32), ("cfl", C.c_float), ("umin", C.c_float), ("refine", C.c_int32)]
+                ("coarse_iters", C.c_int32), ("smoother_fp16", C.c_int32), ("cfl", C.c_float), ("umin", C.c_float), ("refine", C.c_int32), ("krylov", C.c_int32)]
 
 
 class CfSimInfo(C.Structure):

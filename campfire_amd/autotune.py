@@ -4,7 +4,8 @@ For a mix of scene sizes it builds a probe pool (one seeded burning scene
 per size), warms it up, snapshots every scene on the device, and then, from
 the same snapshot, times a few batched steps under each candidate schedule
 (smoother sweeps nu, coarsest solve: direct FP64 or a number of red-black
-sweeps).  The fastest schedule whose every step reaches the tolerance wins.
+sweeps, the V-cycle alone on the defect or as the preconditioner of
+flexible conjugate gradients).  The fastest schedule whose every step reaches the tolerance wins.
 The choice depends on the hardware and the mix, which is why it is measured
 rather than fixed.
 
@@ -19,7 +20,7 @@ from . import scenes
 from .fluid import FluidSim
 from .pool import ScenePool, scene_bytes
 
-CANDIDATES = [{"nu": nu, "coarse_iters": ci} for nu in (1, 2, 3) for ci in (0, 24, 120)]
+CANDIDATES = [{"nu": nu, "coarse_iters": ci, "krylov": kr} for kr in (0, 1) for nu in (1, 2, 3) for ci in (0, 24, 120)]
 
 
 def probe_config(n, seed, width=1.28):
@@ -31,14 +32,14 @@ def probe_config(n, seed, width=1.28):
 
 
 def tune(sizes=(256, 512, 1024), tol=1e-6, refine=1, cfl=0.5, umin=1.0, vcycles=80, warm=24, steps=6, seed=1,
-         candidates=None, log=None):
+         candidates=None, smoother_fp16=0, log=None):
     """Returns (best schedule dict, [(schedule, ms per step, converged, V-cycles per scene)])."""
     log = log or (lambda *a: None)
     candidates = candidates or CANDIDATES
-    base = {"vcycles": vcycles, "tol": tol, "refine": refine, "cfl": cfl, "umin": umin}
+    base = {"vcycles": vcycles, "tol": tol, "refine": refine, "cfl": cfl, "umin": umin, "smoother_fp16": smoother_fp16}
     cfgs = [probe_config(n, seed + q) for q, n in enumerate(sizes)]
     pool = ScenePool(sum(2 * scene_bytes(c["nx"], c["ny"], c["dx"]) for c in cfgs) + (64 << 20))
-    pool.set_schedule(nu=2, coarse_iters=120, **base)
+    pool.set_schedule(nu=2, coarse_iters=120, **dict(base, smoother_fp16=0))
     probes, twins = [], []
     for c in cfgs:
         slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
@@ -73,9 +74,9 @@ def tune(sizes=(256, 512, 1024), tol=1e-6, refine=1, cfl=0.5, umin=1.0, vcycles=
             cycles.append([int(i.last_vcycles) for i in infos])
         ms = elapsed * 1e3 / steps
         table.append((sched, ms, ok, cycles))
-        log("[autotune] nu=%d coarse=%-3s %.3f ms/step %s cycles %s"
-            % (sched["nu"], sched["coarse_iters"] if sched["coarse_iters"] else "LU", ms, "ok" if ok else "NOT CONVERGED",
-               cycles[-1]))
+        log("[autotune] nu=%d coarse=%-3s %-6s %.3f ms/step %s cycles %s"
+            % (sched["nu"], sched["coarse_iters"] if sched["coarse_iters"] else "LU",
+               "FCG" if sched.get("krylov") else "defect", ms, "ok" if ok else "NOT CONVERGED", cycles[-1]))
     for t in twins:
         t.close()
     pool.close()

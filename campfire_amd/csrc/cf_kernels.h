@@ -126,12 +126,16 @@ struct Dyn {
   float dt;            // this step's dt (mirrors the parameter block's dt)
   uint32_t umax_bits;  // CFL: max |u| over the faces, as float bits (atomicMax)
   uint32_t cfl_done;   // CFL: workgroups finished
-  uint32_t rn_done;    // residual norm: workgroups finished
   int32_t converged;   // V-cycle loop: relative residual below tol this step
   int32_t cycles;      // V-cycles run this step
   float resid;         // relative residual after the last V-cycle
   float sig, isig;     // FP16 smoother: p is stored as p * sig (a power of two), read as h * isig
   double time;         // simulated time accumulated by the CFL kernel
+  // Krylov (schedule krylov = 1): ||b||^2; (d, B d) of the stored direction
+  // (0: none yet); this iteration's step and direction coefficients
+  double kbb, kdq, kalpha, kbeta;
+  double kbest;        // Krylov: lowest relative residual of this solve
+  int32_t kfall;       // Krylov: fallen back to defect correction for this solve
 };
 
 // Every device buffer of one scene (the `buf` dictionary of js/fluid.js:52-68
@@ -222,8 +226,30 @@ unsigned resnorm_blocks(const Level& L0);
 // defect correction at level 0: r = b - A p64 (FP64) into L0.rhs, L0.p = 0
 // (ph0 non-null: its FP16 p instead), ||r||/||b|| and the FP16 scale into
 // dyn; then, after a V-cycle on the correction, p64 += L0.p (or ph0 * isig)
-void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
-                      bool first, hipStream_t s);
+// Krylov iteration (schedule krylov = 1), after a V-cycle left z in p (or
+// ph0 * isig): kry_dots forms w = B z and the dot products (z, q), (r, z),
+// (z, w) and from them the step coefficients; kry_step updates the direction
+// d, q = B d, the pressure and the residual, and sets up the next V-cycle's
+// right-hand side and the convergence flag.  part holds 3 doubles per
+// workgroup.  A residual above twice the lowest of the solve so far switches
+// the rest of the solve to defect correction (kry_dots: p64 += z; kry_step:
+// r = b - B p64), which stalls where CG diverges: at a right-hand side
+// outside the range of a singular system.
+struct KryBufs {
+  double *p64, *r64, *d64, *q64, *w64;
+  float* b0;  // the defect correction's saved right-hand side (fallback)
+};
+void launch_kry_dots(const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k, double* part, hipStream_t s);
+void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
+                     hipStream_t s);
+// fold: the phase of the one-workgroup-per-scene kernel that follows
+void phase_kry_dots(BatchPhase& ph, BatchPhase& fold, const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k,
+                    double* part);
+void phase_kry_step(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k,
+                    double* part, float tol);
+// r64 (Krylov, first call only; else null): the FP64 residual kept too
+void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, double* r64, Dyn* dyn, double* part,
+                      float tol, bool first, hipStream_t s);
 void launch_dc_accum(const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip, hipStream_t s);
 void launch_p32_to_64(const float* p, double* p64, int n, hipStream_t s);
 void launch_resnorm(const Level& L0, Dyn* dyn, double* part, float tol, hipStream_t s);
@@ -253,9 +279,9 @@ void phase_sources(BatchPhase& ph, const Fields& F, const float* Tin, const floa
 void phase_forces_coef(BatchPhase& ph, const Fields& F, const Level* L, int nlev, const int* part, int nparts);
 void phase_coef_restrict(BatchPhase& ph, const Level& f, const Level& c);
 void phase_cfl(BatchPhase& ph, const Fields& F, float cfl, float umin, int nsub);
-void phase_resnorm(BatchPhase& ph, const Level& L0, Dyn* dyn, double* part, float tol);
-void phase_dc_defect(BatchPhase& ph, const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part,
-                     float tol, bool first);
+void phase_resnorm(BatchPhase& ph, BatchPhase& fold, const Level& L0, Dyn* dyn, double* part, float tol);
+void phase_dc_defect(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, double* p64, float* b0,
+                     double* r64, Dyn* dyn, double* part, float tol, bool first);
 void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip);
 void phase_coef_half(BatchPhase& ph, const Level& L, const LevelH& H);
 void phase_project(BatchPhase& ph, const Fields& F, const double* p64);

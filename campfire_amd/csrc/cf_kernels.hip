@@ -859,72 +859,97 @@ __global__ __launch_bounds__(256) void k_cfl(const CflOp::Args a_) {
   CflOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// Fixed-order sums of NV per-thread values over a launch of vg workgroups
+// of 256 threads, in two kernels: each wave folds its lanes as a tree (lane
+// i += lane i + o, o = 32 ... 1) and each workgroup stores ((w0 + w1) + w2) +
+// w3 of its waves in part[NV * vb + q] (store_partials); after the kernel
+// boundary one workgroup folds the partials the same way, thread t adding
+// partials t, t + 256, ... (fold_partials, totals in thread 0).  A last-
+// workgroup ticket would save the second launch but needs a device-scope
+// fence in every workgroup, which on this GPU writes back and invalidates
+// the XCD's L2 each time: in kernels that stream FP64 vectors that costs
+// more than the data movement.
+template <int NV>
+__device__ __forceinline__ void wave_block_sum(double (&v)[NV]) {
+  __shared__ double w[NV][4];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+    if ((tid & 63) == 0) w[q][tid >> 6] = v[q];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = ((w[q][0] + w[q][1]) + w[q][2]) + w[q][3];
+  }
+}
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double* __restrict__ part, int vb) {
+  wave_block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) part[NV * vb + q] = v[q];
+  }
+}
+template <int NV>
+__device__ __forceinline__ void fold_partials(double (&v)[NV], const double* __restrict__ part, int vg) {
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double a = 0;
+    for (int b = threadIdx.x; b < vg; b += 256) a += part[NV * b + q];
+    v[q] = a;
+  }
+  wave_block_sum<NV>(v);
+}
+
 // ||r|| / ||rhs|| over level 0 (sums of squares in FP64), after a V-cycle:
-// per-workgroup partials, then the last workgroup adds them in a fixed order,
-// so the value does not depend on scheduling.  Below tol it sets
+// per-workgroup partials here, then resid_fold adds them in a fixed order, so
+// the value does not depend on scheduling.  Below tol it sets
 // dyn->converged, and the remaining V-cycles of the step are skipped.
 struct ResnormOp {
   struct Args {
     Level L;
     Dyn* d;
     double* part;
-    float tol;
     int vg;  // tiles of this scene's launch
   };
   static constexpr int kThreads = 256;
+  // residual(L, i, j) without branches: the neighbours are loaded from
+  // clamped indices first, then the same FP32 terms in the same order
+  static __device__ __forceinline__ float residual_bf(const Level& L, int i, int j) {
+    const int nx = L.nx, ny = L.ny, k = j * nx + i;
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    const float w = L.ax[j * (nx + 1) + i], e = L.ax[j * (nx + 1) + i + 1];
+    const float s = L.ay[k], n = L.ay[k + nx];
+    const float pw = L.p[hw ? k - 1 : k], pe = L.p[he ? k + 1 : k];
+    const float ps = L.p[hs ? k - nx : k], pn = L.p[hn ? k + nx : k], pc = L.p[k];
+    float off = K(0), diag = K(0);
+    off = hw ? off + w * pw : off;
+    diag = hw ? diag + w : diag + K(2.0) * w;
+    off = he ? off + e * pe : off;
+    diag = he ? diag + e : diag + K(2.0) * e;
+    off = hs ? off + s * ps : off;
+    diag = hs ? diag + s : diag;
+    off = hn ? off + n * pn : off;
+    diag = hn ? diag + n : diag + K(2.0) * n;
+    const float r = L.rhs[k] - (off - diag * pc) / L.h2;
+    return diag <= K(0) ? K(0) : r;
+  }
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Level& L = a_.L;
-    Dyn* d = a_.d;
-    double* __restrict__ part = a_.part;
-    float tol = a_.tol;
-
-  __shared__ double wr[4], wb[4];
-  __shared__ int last;
-  if (d->converged) return;
-  const int tid = threadIdx.x, n = L.nx * L.ny;
-  double rr = 0, bb = 0;
-  for (int k = vb * 256 + tid; k < n; k += a_.vg * 256) {
-    const float r = residual(L, k % L.nx, k / L.nx);
-    const double b = (double)L.rhs[k];
-    rr += (double)r * (double)r;
-    bb += b * b;
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    rr += __shfl_down(rr, o, 64);
-    bb += __shfl_down(bb, o, 64);
-  }
-  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
-  __syncthreads();
-  if (tid == 0) {
-    part[2 * vb] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
-    part[2 * vb + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
-    __threadfence();
-    last = atomicAdd(&d->rn_done, 1u) == a_.vg - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  rr = 0; bb = 0;
-  for (int b = tid; b < (int)a_.vg; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    rr += __shfl_down(rr, o, 64);
-    bb += __shfl_down(bb, o, 64);
-  }
-  __syncthreads();
-  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
-  __syncthreads();
-  if (tid == 0) {
-    rr = ((wr[0] + wr[1]) + wr[2]) + wr[3];
-    bb = ((wb[0] + wb[1]) + wb[2]) + wb[3];
-    const double r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);  // relative unless rhs = 0
-    d->resid = (float)r;
-    d->cycles += 1;
-    if (r < (double)tol) d->converged = 1;
-    atomicExch(&d->rn_done, 0u);
-  }
-
+    if (a_.d->converged) return;
+    const int n = L.nx * L.ny;
+    double v[2] = {0, 0};  // ||r||^2, ||rhs||^2
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      const int j = k / L.nx, i = k - j * L.nx;
+      const float r = residual_bf(L, i, j);
+      const double b = (double)L.rhs[k];
+      v[0] += (double)r * (double)r;
+      v[1] += b * b;
+    }
+    store_partials<2>(v, a_.part, vb);
   }
 };
 __global__ __launch_bounds__(256) void k_resnorm(const ResnormOp::Args a_) {
@@ -936,115 +961,161 @@ __global__ __launch_bounds__(256) void k_resnorm(const ResnormOp::Args a_) {
 // ||rhs|| on a burning scene (rounding of p alone leaves that much), so the
 // outer iteration keeps p in FP64 and solves A e = b - A p with FP32 V-cycles.
 
-// the residual b - A p of one cell with p in FP64 (the stencil of
-// shaders/pressure.wgsl:77-94 with the coefficients promoted)
-__device__ __forceinline__ double residual64(const Level& L, const double* __restrict__ p, int i, int j, double b) {
-  const int nx = L.nx, ny = L.ny;
+// A v at cell (i, j) of level 0 in FP64: bv = (sum a v_nb - diag v) / h2
+// with the stencil of shaders/pressure.wgsl:77-94 (twice the face coefficient
+// at the open sides and the top, none at the floor), the coefficients
+// promoted; returns whether the cell is open (diag > 0).  Branch-free: the
+// four neighbours are loaded from clamped indices before any arithmetic, so a
+// thread's loads are in flight together, and the terms are then added in the
+// fixed order w, e, s, n.  val(k) gives v at cell k.
+template <class V>
+__device__ __forceinline__ bool apply64(const Level& L, const V& val, int i, int j, double vc, double& bv) {
+  const int nx = L.nx, ny = L.ny, k = j * nx + i;
+  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
   const double w = L.ax[j * (nx + 1) + i], e = L.ax[j * (nx + 1) + i + 1];
-  const double s = L.ay[j * nx + i], n = L.ay[(j + 1) * nx + i];
+  const double s = L.ay[k], n = L.ay[k + nx];
+  const double vw = val(hw ? k - 1 : k), ve = val(he ? k + 1 : k);
+  const double vs = val(hs ? k - nx : k), vn = val(hn ? k + nx : k);
   double off = 0, diag = 0;
-  if (i > 0) { off += w * p[j * nx + i - 1]; diag += w; } else { diag += 2.0 * w; }
-  if (i < nx - 1) { off += e * p[j * nx + i + 1]; diag += e; } else { diag += 2.0 * e; }
-  if (j > 0) { off += s * p[(j - 1) * nx + i]; diag += s; }
-  if (j < ny - 1) { off += n * p[(j + 1) * nx + i]; diag += n; } else { diag += 2.0 * n; }
-  if (diag <= 0) return 0;
-  return b - (off - diag * p[j * nx + i]) / (double)L.h2;
+  off = hw ? off + w * vw : off;
+  diag = hw ? diag + w : diag + 2.0 * w;
+  off = he ? off + e * ve : off;
+  diag = he ? diag + e : diag + 2.0 * e;
+  off = hs ? off + s * vs : off;
+  diag = hs ? diag + s : diag;
+  off = hn ? off + n * vn : off;
+  diag = hn ? diag + n : diag + 2.0 * n;
+  bv = (off - diag * vc) / (double)L.h2;
+  return diag > 0;
+}
+
+// the residual b - A p of one cell with p in FP64, 0 at a closed cell
+__device__ __forceinline__ double residual64(const Level& L, const double* __restrict__ p, int i, int j, double b) {
+  double bv;
+  const int k = j * L.nx + i;
+  const bool open = apply64(L, [&](int q) { return p[q]; }, i, j, p[k], bv);
+  return open ? b - bv : 0.0;
+}
+
+// end of a residual evaluation, in the fold's thread 0: ||r|| / ||b|| into
+// dyn, converged when below tol; the FP16 smoother's scale for the next
+// correction: p is stored scaled by the power of two that brings the rms of
+// this residual into [0.5, 1), so the correction's values sit in the normal
+// range of FP16 (the scaling itself is exact)
+__device__ __forceinline__ double resid_done(Dyn* d, double rr, double bb, int ncell, float tol, bool count) {
+  const double r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+  int ex = 0;
+  const double rms = sqrt(rr / (double)ncell);
+  if (rms > 0) {
+    frexp(rms, &ex);
+    ex = ex < -60 ? -60 : (ex > 60 ? 60 : ex);
+  }
+  d->sig = (float)ldexp(1.0, -ex);
+  d->isig = (float)ldexp(1.0, ex);
+  d->resid = (float)r;
+  if (count) d->cycles += 1;
+  if (r < (double)tol) d->converged = 1;
+  return r;
 }
 
 // r = b - A p64 into the level's rhs (the correction equation) with e = 0 as
-// its starting guess; ||r|| / ||b|| into dyn (FP64 sums, fixed-order fold by
-// the last workgroup), converged when below tol.  first: b is the level's
-// rhs as the forces pass wrote it, saved to b0 for the later iterations.
+// its starting guess, and per-workgroup partials of ||r||^2 and ||b||^2
+// (FP64) for resid_fold.  first: b is the level's rhs as the forces pass
+// wrote it, saved to b0 for the later iterations.
 struct DcDefectOp {
   struct Args {
     Level L;
     __half* ph0;  // FP16 smoother: the correction's start e = 0 goes here instead of L.p
     double* p64;
     float* b0;
+    double* r64;  // Krylov start: the FP64 residual is kept too
     Dyn* d;
     double* part;
-    float tol;
     int first;
     int vg;  // tiles of this scene's launch
   };
   static constexpr int kThreads = 256;
-  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+  // FIRST: b from the level's rhs (saved to b0), else from b0; H: the
+  // correction's start e = 0 goes to ph0 (FP16 smoother) instead of L.p
+  template <bool FIRST, bool H>
+  static __device__ __forceinline__ void sweep(const Args& a_, int vb, double (&v)[2]) {
     const Level& L = a_.L;
-    double* __restrict__ p64 = a_.p64;
+    const double* __restrict__ p64 = a_.p64;
     float* __restrict__ b0 = a_.b0;
-    Dyn* d = a_.d;
-    double* __restrict__ part = a_.part;
-    float tol = a_.tol;
-    int first = a_.first;
-
-  __shared__ double wr[4], wb[4];
-  __shared__ int last;
-  if (d->converged) return;
-  const int tid = threadIdx.x, n = L.nx * L.ny;
-  double rr = 0, bb = 0;
-  for (int k = vb * 256 + tid; k < n; k += a_.vg * 256) {
-    float bf;
-    if (first) { bf = L.rhs[k]; b0[k] = bf; } else bf = b0[k];
-    const double b = (double)bf;
-    const double r = residual64(L, p64, k % L.nx, k / L.nx, b);
-    L.rhs[k] = (float)r;
-    if (a_.ph0) a_.ph0[k] = __float2half(0.f);
-    else L.p[k] = K(0);
-    rr += r * r;
-    bb += b * b;
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    rr += __shfl_down(rr, o, 64);
-    bb += __shfl_down(bb, o, 64);
-  }
-  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
-  __syncthreads();
-  if (tid == 0) {
-    part[2 * vb] = ((wr[0] + wr[1]) + wr[2]) + wr[3];
-    part[2 * vb + 1] = ((wb[0] + wb[1]) + wb[2]) + wb[3];
-    __threadfence();
-    last = atomicAdd(&d->rn_done, 1u) == a_.vg - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  rr = 0; bb = 0;
-  for (int b = tid; b < (int)a_.vg; b += 256) { rr += part[2 * b]; bb += part[2 * b + 1]; }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    rr += __shfl_down(rr, o, 64);
-    bb += __shfl_down(bb, o, 64);
-  }
-  __syncthreads();
-  if ((tid & 63) == 0) { wr[tid >> 6] = rr; wb[tid >> 6] = bb; }
-  __syncthreads();
-  if (tid == 0) {
-    rr = ((wr[0] + wr[1]) + wr[2]) + wr[3];
-    bb = ((wb[0] + wb[1]) + wb[2]) + wb[3];
-    const double r = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
-    // FP16 smoother: p is stored scaled by the power of two that brings the
-    // rms of this residual into [0.5, 1), so the correction's values sit in
-    // the normal range of FP16 (the scaling itself is exact)
-    int ex = 0;
-    const double rms = sqrt(rr / ((double)L.nx * (double)L.ny));
-    if (rms > 0) {
-      frexp(rms, &ex);
-      ex = ex < -60 ? -60 : (ex > 60 ? 60 : ex);
+    double* __restrict__ r64 = a_.r64;
+    const int n = L.nx * L.ny;
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      const float bf = FIRST ? L.rhs[k] : b0[k];
+      if (FIRST) b0[k] = bf;
+      const double b = (double)bf;
+      const int j = k / L.nx, i = k - j * L.nx;
+      const double r = residual64(L, p64, i, j, b);
+      L.rhs[k] = (float)r;
+      if (FIRST && r64) r64[k] = r;
+      if (H) a_.ph0[k] = __float2half(0.f);
+      else L.p[k] = K(0);
+      v[0] += r * r;
+      v[1] += b * b;
     }
-    d->sig = (float)ldexp(1.0, -ex);
-    d->isig = (float)ldexp(1.0, ex);
-    d->resid = (float)r;
-    if (!first) d->cycles += 1;
-    if (r < (double)tol) d->converged = 1;
-    atomicExch(&d->rn_done, 0u);
   }
-
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    if (a_.d->converged) return;
+    double v[2] = {0, 0};  // ||r||^2, ||b||^2
+    if (a_.first) {
+      if (a_.ph0) sweep<true, true>(a_, vb, v);
+      else sweep<true, false>(a_, vb, v);
+    } else {
+      if (a_.ph0) sweep<false, true>(a_, vb, v);
+      else sweep<false, false>(a_, vb, v);
+    }
+    store_partials<2>(v, a_.part, vb);
   }
 };
 __global__ __launch_bounds__(256) void k_dc_defect(const DcDefectOp::Args a_) {
   DcDefectOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
+
+// One workgroup per scene after dc_defect or kry_step: folds their partials
+// (||r||^2, ||b||^2) and ends the residual evaluation: ||r|| / ||b||, the
+// convergence flag, the FP16 smoother's scale and the cycle count.  Krylov
+// (krylov = 1): the first evaluation also starts the iteration (||b||^2, the
+// residual as the lowest so far, no stored direction); a later one that
+// follows a Krylov step divides by the saved ||b||^2 and, when the residual
+// is above twice the lowest of the solve so far, switches the rest of the
+// solve to defect correction.
+struct ResidFoldOp {
+  struct Args {
+    Dyn* d;
+    const double* part;
+    int vg;     // workgroups of the partials' launch
+    int ncell;  // level-0 cells
+    float tol;
+    int first;
+    int krylov;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int) {
+    Dyn* d = a_.d;
+    if (d->converged) return;
+    double v[2];
+    fold_partials<2>(v, a_.part, a_.vg);
+    if (threadIdx.x != 0) return;
+    if (a_.krylov && !a_.first && !d->kfall) {
+      const double r = resid_done(d, v[0], d->kbb, a_.ncell, a_.tol, true);
+      if (r > 2.0 * d->kbest) d->kfall = 1;
+      if (r < d->kbest) d->kbest = r;
+      return;
+    }
+    const double r = resid_done(d, v[0], v[1], a_.ncell, a_.tol, !a_.first);
+    if (a_.first && a_.krylov) {
+      d->kbb = v[1];
+      d->kbest = r;
+      d->kdq = 0;
+      d->kfall = 0;
+    }
+  }
+};
+__global__ __launch_bounds__(256) void k_resid_fold(const ResidFoldOp::Args a_) { ResidFoldOp::run(a_, 0); }
 
 // p64 += e after a correction V-cycle
 struct DcAccumOp {
@@ -1075,6 +1146,144 @@ struct DcAccumOp {
 };
 __global__ __launch_bounds__(256) void k_dc_accum(const DcAccumOp::Args a_) {
   DcAccumOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// z of the last V-cycle at cell k: p (FP32 smoother) or, H, ph0 * isig
+template <bool H>
+__device__ __forceinline__ double kry_z(const Level& L, const __half* __restrict__ ph0, float is, int k) {
+  return H ? (double)(__half2float(ph0[k]) * is) : (double)L.p[k];
+}
+
+// Flexible conjugate gradients with one stored direction (Notay's FCG(1)),
+// the V-cycle's z = M r as the preconditioned residual:
+//   beta = (z, B d) / (d, B d),  d <- z - beta d,  q <- B z - beta q (= B d),
+//   alpha = (r, z) / (d, B d),   p64 += alpha d,   r -= alpha q,
+// where (d, B d) of the new direction expands to (z, B z) - beta (z, q_old),
+// so one pass over z gives all three dot products: kry_dots forms w = B z
+// and their partials, kry_coef folds them into alpha and beta, kry_step
+// updates the vectors and leaves the partials of ||r||^2 for resid_fold.
+// After a fallback to defect correction kry_dots adds e to p64 and kry_step
+// forms r = b - A p64 instead.
+struct KryDotsOp {
+  struct Args {
+    Level L;
+    const __half* ph0;
+    Dyn* d;
+    KryBufs k;
+    double* part;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  template <bool H>
+  static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double (&v)[3]) {
+    const Level& L = a_.L;
+    const double* __restrict__ q64 = a_.k.q64;
+    const double* __restrict__ r64 = a_.k.r64;
+    double* __restrict__ w64 = a_.k.w64;
+    const auto zat = [&](int q) { return kry_z<H>(L, a_.ph0, is, q); };
+    const int n = L.nx * L.ny;
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      const int j = k / L.nx, i = k - j * L.nx;
+      const double z = zat(k), q = q64[k], r = r64[k];
+      double bz;
+      if (!apply64(L, zat, i, j, z, bz)) bz = 0.0;
+      w64[k] = bz;
+      v[0] += z * q;
+      v[1] += r * z;
+      v[2] += z * bz;
+    }
+  }
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    Dyn* d = a_.d;
+    if (d->converged) return;
+    if (d->kfall) {  // defect correction: p64 += z
+      DcAccumOp::run(DcAccumOp::Args{a_.L, a_.ph0, d, a_.k.p64, &d->converged, a_.vg}, vb);
+      return;
+    }
+    double v[3] = {0, 0, 0};  // (z, q), (r, z), (z, B z)
+    if (a_.ph0) sweep<true>(a_, vb, d->isig, v);
+    else sweep<false>(a_, vb, 1.f, v);
+    store_partials<3>(v, a_.part, vb);
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_dots(const KryDotsOp::Args a_) {
+  KryDotsOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// one workgroup per scene: alpha, beta and (d, B d) from kry_dots' partials
+struct KryCoefOp {
+  struct Args {
+    Dyn* d;
+    const double* part;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int) {
+    Dyn* d = a_.d;
+    if (d->converged || d->kfall) return;
+    double v[3];
+    fold_partials<3>(v, a_.part, a_.vg);
+    if (threadIdx.x != 0) return;
+    const double beta = d->kdq != 0 ? v[0] / d->kdq : 0.0;
+    const double dq = v[2] - beta * v[0];
+    d->kalpha = dq != 0 ? v[1] / dq : 0.0;
+    d->kbeta = beta;
+    d->kdq = dq;
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_coef(const KryCoefOp::Args a_) { KryCoefOp::run(a_, 0); }
+
+struct KryStepOp {
+  struct Args {
+    Level L;
+    __half* ph0;
+    Dyn* d;
+    KryBufs k;
+    double* part;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  template <bool H>
+  static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double alpha, double beta,
+                                               double (&v)[2]) {
+    const Level& L = a_.L;
+    double* __restrict__ p64 = a_.k.p64;
+    double* __restrict__ r64 = a_.k.r64;
+    double* __restrict__ d64 = a_.k.d64;
+    double* __restrict__ q64 = a_.k.q64;
+    const double* __restrict__ w64 = a_.k.w64;
+    const int n = L.nx * L.ny;
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      const double z = kry_z<H>(L, a_.ph0, is, k);
+      const double dn = z - beta * d64[k];
+      const double qn = w64[k] - beta * q64[k];
+      const double x = p64[k] + alpha * dn;
+      const double r = r64[k] - alpha * qn;
+      p64[k] = x;
+      d64[k] = dn;
+      q64[k] = qn;
+      r64[k] = r;
+      L.rhs[k] = (float)r;
+      if (H) a_.ph0[k] = __float2half(0.f);
+      else L.p[k] = K(0);
+      v[0] += r * r;
+    }
+  }
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    Dyn* d = a_.d;
+    if (d->converged) return;
+    if (d->kfall) {  // defect correction: r = b - A p64
+      DcDefectOp::run(DcDefectOp::Args{a_.L, a_.ph0, a_.k.p64, a_.k.b0, nullptr, d, a_.part, 0, a_.vg}, vb);
+      return;
+    }
+    double v[2] = {0, 0};  // ||r||^2 (the fold divides by the saved ||b||^2)
+    if (a_.ph0) sweep<true>(a_, vb, d->isig, d->kalpha, d->kbeta, v);
+    else sweep<false>(a_, vb, 1.f, d->kalpha, d->kbeta, v);
+    store_partials<2>(v, a_.part, vb);
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_step(const KryStepOp::Args a_) {
+  KryStepOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 __global__ __launch_bounds__(256) void k_p32_to_64(const float* __restrict__ p, double* __restrict__ p64, int n) {
@@ -1581,11 +1790,23 @@ void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s)
   if (b > 1024) b = 1024;
   launch(k_cfl, dim3(b), dim3(256), 0, s, CflOp::Args{F, cfl, umin, nsub, (int)b});
 }
-void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
-                      bool first, hipStream_t s) {
+void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, double* r64, Dyn* dyn, double* part,
+                      float tol, bool first, hipStream_t s) {
   const int nb = (int)resnorm_blocks(L0);
-  launch(k_dc_defect, dim3(nb), dim3(256), 0, s,
-         DcDefectOp::Args{L0, ph0, p64, b0, dyn, part, tol, first ? 1 : 0, nb});
+  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, p64, b0, r64, dyn, part, first ? 1 : 0, nb});
+  launch(k_resid_fold, dim3(1), dim3(256), 0, s,
+         ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, first ? 1 : 0, r64 ? 1 : 0});
+}
+void launch_kry_dots(const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k, double* part, hipStream_t s) {
+  const int nb = (int)resnorm_blocks(L0);
+  launch(k_kry_dots, dim3(nb), dim3(256), 0, s, KryDotsOp::Args{L0, ph0, dyn, k, part, nb});
+  launch(k_kry_coef, dim3(1), dim3(256), 0, s, KryCoefOp::Args{dyn, part, nb});
+}
+void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
+                     hipStream_t s) {
+  const int nb = (int)resnorm_blocks(L0);
+  launch(k_kry_step, dim3(nb), dim3(256), 0, s, KryStepOp::Args{L0, ph0, dyn, k, part, nb});
+  launch(k_resid_fold, dim3(1), dim3(256), 0, s, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 1});
 }
 void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s) {
   const int nb = (int)nblk(L.nx + 1, L.ny + 1);
@@ -1607,7 +1828,8 @@ unsigned resnorm_blocks(const Level& L0) {
 }
 void launch_resnorm(const Level& L0, Dyn* dyn, double* part, float tol, hipStream_t s) {
   const int nb = (int)resnorm_blocks(L0);
-  launch(k_resnorm, dim3(nb), dim3(256), 0, s, ResnormOp::Args{L0, dyn, part, tol, nb});
+  launch(k_resnorm, dim3(nb), dim3(256), 0, s, ResnormOp::Args{L0, dyn, part, nb});
+  launch(k_resid_fold, dim3(1), dim3(256), 0, s, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 0});
 }
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s) {
   if (n == 0) return;
@@ -1641,15 +1863,32 @@ void phase_cfl(BatchPhase& ph, const Fields& F, float cfl, float umin, int nsub)
   if (b > 1024) b = 1024;
   phase_add<CflOp>(ph, CflOp::Args{F, cfl, umin, nsub, b}, b, dim3(256), "cfl");
 }
-void phase_resnorm(BatchPhase& ph, const Level& L0, Dyn* dyn, double* part, float tol) {
+void phase_resnorm(BatchPhase& ph, BatchPhase& fold, const Level& L0, Dyn* dyn, double* part, float tol) {
   const int nb = (int)resnorm_blocks(L0);
-  phase_add<ResnormOp>(ph, ResnormOp::Args{L0, dyn, part, tol, nb}, nb, dim3(256), "resnorm");
+  phase_add<ResnormOp>(ph, ResnormOp::Args{L0, dyn, part, nb}, nb, dim3(256), "resnorm");
+  phase_add<ResidFoldOp>(fold, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 0}, 1, dim3(256),
+                         "resid_fold");
 }
-void phase_dc_defect(BatchPhase& ph, const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part,
-                     float tol, bool first) {
+void phase_dc_defect(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, double* p64, float* b0,
+                     double* r64, Dyn* dyn, double* part, float tol, bool first) {
   const int nb = (int)resnorm_blocks(L0);
-  phase_add<DcDefectOp>(ph, DcDefectOp::Args{L0, ph0, p64, b0, dyn, part, tol, first ? 1 : 0, nb}, nb, dim3(256),
+  phase_add<DcDefectOp>(ph, DcDefectOp::Args{L0, ph0, p64, b0, r64, dyn, part, first ? 1 : 0, nb}, nb, dim3(256),
                         "dc_defect");
+  phase_add<ResidFoldOp>(fold, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, first ? 1 : 0, r64 ? 1 : 0}, 1,
+                         dim3(256), "resid_fold");
+}
+void phase_kry_dots(BatchPhase& ph, BatchPhase& fold, const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k,
+                    double* part) {
+  const int nb = (int)resnorm_blocks(L0);
+  phase_add<KryDotsOp>(ph, KryDotsOp::Args{L0, ph0, dyn, k, part, nb}, nb, dim3(256), "kry_dots");
+  phase_add<KryCoefOp>(fold, KryCoefOp::Args{dyn, part, nb}, 1, dim3(256), "kry_coef");
+}
+void phase_kry_step(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k,
+                    double* part, float tol) {
+  const int nb = (int)resnorm_blocks(L0);
+  phase_add<KryStepOp>(ph, KryStepOp::Args{L0, ph0, dyn, k, part, nb}, nb, dim3(256), "kry_step");
+  phase_add<ResidFoldOp>(fold, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 1}, 1, dim3(256),
+                         "resid_fold");
 }
 void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip) {
   int b = (L0.nx * L0.ny + 1023) / 1024;

@@ -97,6 +97,7 @@ struct cf_sim {
   double* dNorm = nullptr;   // residual-norm partials
   double* dP64 = nullptr;    // level-0 pressure in FP64 (defect correction)
   float* dB0 = nullptr;      // level-0 right-hand side kept across correction cycles
+  double *dR64 = nullptr, *dD64 = nullptr, *dQ64 = nullptr, *dW64 = nullptr;  // Krylov: r, d, B d, B z
   double* dBand = nullptr;   // direct coarsest solve: Cholesky band
   // upload rings
   ParamUpload* hP = nullptr;
@@ -116,7 +117,7 @@ struct cf_sim {
   // graph of one step
   hipGraphExec_t gexec = nullptr;   // one step
   hipGraphExec_t gexec8 = nullptr;  // eight steps: one replay per 8 steps amortises the launch
-  int gkey[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int gkey[11] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   bool use_graph = true;
   bool mg_simple = false;  // one launch per reference dispatch pair (debug comparisons)
   int mg_bottom = 0;       // first level solved by the single-workgroup bottom kernel
@@ -219,15 +220,21 @@ static void enqueue_mg(cf_sim* s, bool coarse_coef) {
   __half* ph0 = s->sch.smoother_fp16 ? s->LH[0].p : nullptr;
   if (ph0) for (int l = 0; l < s->mg_bottom; l++) { cf::launch_coef_half(s->L[l], s->LH[l], s->st); mark(s, "coef_half"); }
   if (tol) { cf::launch_fill_u32(&s->dDyn->converged, 0u, 2, s->st); mark(s, "mg_control"); }  // converged, cycles
+  const bool kry = dc && s->sch.krylov;
+  const cf::KryBufs kb{s->dP64, s->dR64, s->dD64, s->dQ64, s->dW64, s->dB0};
   if (dc) {
-    cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, true, s->st);
+    cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, kry ? s->dR64 : nullptr, s->dDyn, s->dNorm, s->sch.tol, true,
+                         s->st);
     mark(s, "dc_defect");
   }
   for (int c = 0; c < s->sch.vcycles; c++) {
     enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0);
-    if (dc) {
+    if (kry) {
+      cf::launch_kry_dots(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->st); mark(s, "kry_dots");
+      cf::launch_kry_step(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st); mark(s, "kry_step");
+    } else if (dc) {
       cf::launch_dc_accum(s->L[0], ph0, s->dDyn, s->dP64, skip, s->st); mark(s, "dc_accum");
-      cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, false, s->st);
+      cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, nullptr, s->dDyn, s->dNorm, s->sch.tol, false, s->st);
       mark(s, "dc_defect");
     } else if (tol) {
       cf::launch_resnorm(s->L[0], s->dDyn, s->dNorm, s->sch.tol, s->st); mark(s, "resnorm");
@@ -324,7 +331,7 @@ static void drop_graph(cf_sim* s) {
   if (s->gexec) hipGraphExecDestroy(s->gexec);
   if (s->gexec8) hipGraphExecDestroy(s->gexec8);
   s->gexec = s->gexec8 = nullptr;
-  for (int q = 0; q < 10; q++) s->gkey[q] = -1;
+  for (int q = 0; q < 11; q++) s->gkey[q] = -1;
 }
 
 static int capture(cf_sim* s, int nsteps, hipGraphExec_t* out) {
@@ -340,8 +347,8 @@ static int capture(cf_sim* s, int nsteps, hipGraphExec_t* out) {
 
 static int ensure_graph(cf_sim* s) {
   // everything a captured step bakes in as launch arguments
-  int key[10] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev, 0, 0, 0,
-                 s->sch.refine};
+  int key[11] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev, 0, 0, 0,
+                 s->sch.refine, s->sch.krylov};
   memcpy(&key[6], &s->sch.tol, 4);
   memcpy(&key[7], &s->sch.cfl, 4);
   memcpy(&key[8], &s->sch.umin, 4);
@@ -389,7 +396,7 @@ static bool lookup(cf_sim* s, const char* name, FieldRef& r) {
       {"SQ", F.SQ, nC * 8}, {"cellG", F.cellG, nC * 8}, {"fx", F.fx, nU * 4}, {"fy", F.fy, nV * 4},
       {"pieces", F.pieces, CF_MAX_PIECES * CF_PIECE_FLOATS * 4}, {"acc", F.acc, CF_ACC_LEN * 4},
       {"rad", F.rad, CF_RAD_LEN * 16}, {"faceW", F.faceW, CF_MAX_PIECES * 4 * 4},
-      {"p64", s->dP64, nC * 8}};
+      {"p64", s->dP64, nC * 8}, {"r64", s->dR64, nC * 8}, {"d64", s->dD64, nC * 8}, {"q64", s->dQ64, nC * 8}};
   for (auto& e : t)
     if (!strcmp(e.n, name)) { r.ptr = e.p; r.bytes = e.b; return true; }
   const char* pre[4] = {"rhs", "ax", "ay", "p"};
@@ -520,7 +527,8 @@ static int pool_build_step(cf_pool* pl) {
     if (!a.empty()) ph.push_back(a);
   }
   // V-cycles; down leg d and up leg d hold level d of every scene that has one
-  const bool tol = k.tol > 0, dc = tol && k.refine;
+  const bool tol = k.tol > 0, dc = tol && k.refine, kry = dc && k.krylov;
+  auto kb = [](cf_sim* s) { return cf::KryBufs{s->dP64, s->dR64, s->dD64, s->dQ64, s->dW64, s->dB0}; };
   auto ph0 = [&](cf_sim* s) { return k.smoother_fp16 ? s->LH[0].p : (__half*)nullptr; };
   if (k.smoother_fp16) {
     for (int l = 0; l < MAXLEV; l++) {
@@ -531,9 +539,18 @@ static int pool_build_step(cf_pool* pl) {
     }
   }
   if (tol) { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_fill_u32(a, &s->dDyn->converged, 0u, 2); }
+  // a kernel and the fold that follows it: two phases, built as locals (a
+  // reference from P() would not survive the second one's growth of ph)
+  auto pair = [&](cf::BatchPhase& a, cf::BatchPhase& f) {
+    ph.push_back(std::move(a));
+    ph.push_back(std::move(f));
+  };
   if (dc) {
-    cf::BatchPhase& a = P();
-    for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], ph0(s), s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, true);
+    cf::BatchPhase a, f;
+    for (auto* s : sc)
+      cf::phase_dc_defect(a, f, s->L[0], ph0(s), s->dP64, s->dB0, kry ? s->dR64 : nullptr, s->dDyn, s->dNorm, k.tol,
+                          true);
+    pair(a, f);
   }
   int maxb = 0;
   for (auto* s : sc) maxb = s->mg_bottom > maxb ? s->mg_bottom : maxb;
@@ -574,18 +591,32 @@ static int pool_build_step(cf_pool* pl) {
           if (!a.empty()) ph.push_back(a);
         }
     }
-    if (dc) {
+    if (kry) {
+      {
+        cf::BatchPhase a, f;
+        for (auto* s : sc) cf::phase_kry_dots(a, f, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm);
+        pair(a, f);
+      }
+      {
+        cf::BatchPhase a, f;
+        for (auto* s : sc) cf::phase_kry_step(a, f, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm, k.tol);
+        pair(a, f);
+      }
+    } else if (dc) {
       {
         cf::BatchPhase& a = P();
         for (auto* s : sc) cf::phase_dc_accum(a, s->L[0], ph0(s), s->dDyn, s->dP64, &s->dDyn->converged);
       }
       {
-        cf::BatchPhase& a = P();
-        for (auto* s : sc) cf::phase_dc_defect(a, s->L[0], ph0(s), s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, false);
+        cf::BatchPhase a, f;
+        for (auto* s : sc)
+          cf::phase_dc_defect(a, f, s->L[0], ph0(s), s->dP64, s->dB0, nullptr, s->dDyn, s->dNorm, k.tol, false);
+        pair(a, f);
       }
     } else if (tol) {
-      cf::BatchPhase& a = P();
-      for (auto* s : sc) cf::phase_resnorm(a, s->L[0], s->dDyn, s->dNorm, k.tol);
+      cf::BatchPhase a, f;
+      for (auto* s : sc) cf::phase_resnorm(a, f, s->L[0], s->dDyn, s->dNorm, k.tol);
+      pair(a, f);
     }
   }
   { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_project(a, s->F, dc ? s->dP64 : nullptr); }
@@ -712,8 +743,10 @@ static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
   const unsigned nparts = (unsigned)(((s->nx + 31) / 32) * ((s->ny + 7) / 8));
   add((void**)&s->dPart, (size_t)nparts * 8);
   add((void**)&s->dAccPart, (size_t)nparts * 16);
-  add((void**)&s->dNorm, 2 * sizeof(double) * 1024);
+  add((void**)&s->dNorm, 4 * sizeof(double) * 1024);
   add((void**)&s->dP64, nC * sizeof(double));
+  add((void**)&s->dR64, nC * sizeof(double)); add((void**)&s->dD64, nC * sizeof(double));
+  add((void**)&s->dQ64, nC * sizeof(double)); add((void**)&s->dW64, nC * sizeof(double));
   add((void**)&s->dB0, nC * sizeof(float));
   add((void**)&s->dBand, (256 * 33 + 256) * sizeof(double));
 }
@@ -750,7 +783,7 @@ static int sim_create(int nx, int ny, double dx, double dt, hipStream_t shared, 
       1800, 5.670374e-8, 2.667, 0};
   memcpy(s->pv, def, sizeof(def));
   s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
-  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0;
+  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0; s->sch.krylov = 0;
   const char* ng = getenv("CF_NO_GRAPH");
   s->use_graph = !(ng && *ng && *ng != '0');
   const char* ms = getenv("CF_MG_SIMPLE");
@@ -897,6 +930,7 @@ int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
   if (!(sc->cfl >= 0) || (sc->cfl > 0 && !(sc->umin > 0)))
     return set_err(CF_EINVAL, "bad schedule (cfl >= 0; umin > 0 with a CFL step)");
   if (sc->refine && !(sc->tol > 0)) return set_err(CF_EINVAL, "bad schedule (refine needs tol > 0)");
+  if (sc->krylov && !sc->refine) return set_err(CF_EINVAL, "bad schedule (krylov needs refine = 1)");
   // switching defect correction on: the FP64 pressure starts from the FP32 one
   if (sc->smoother_fp16 && (!sc->refine || s->mg_simple))
     return set_err(CF_EINVAL, "FP16 smoother needs defect correction (refine = 1, tol > 0) and the fused path");
@@ -1249,7 +1283,7 @@ int cf_pool_create(uint64_t hbm_budget, cf_pool** out) {
   cf_pool* pl = new cf_pool();
   pl->budget = hbm_budget;
   pl->sch.nu = 2; pl->sch.vcycles = 1; pl->sch.tol = 0; pl->sch.coarse_iters = 120; pl->sch.smoother_fp16 = 0;
-  pl->sch.cfl = 0; pl->sch.umin = 1.0f; pl->sch.refine = 0;
+  pl->sch.cfl = 0; pl->sch.umin = 1.0f; pl->sch.refine = 0; pl->sch.krylov = 0;
   const char* ng = getenv("CF_NO_GRAPH");
   pl->use_graph = !(ng && *ng && *ng != '0');
   hipError_t e = hipStreamCreateWithFlags(&pl->st, hipStreamNonBlocking);
@@ -1274,7 +1308,7 @@ int cf_pool_set_schedule(cf_pool* pl, const cf_schedule* sc) {
   if (!pl || !sc) return set_err(CF_EINVAL, "NULL argument");
   // validate with a scene's rules even when the pool is empty
   if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 0 || !(sc->tol >= 0) || !(sc->cfl >= 0) ||
-      (sc->refine && !(sc->tol > 0)) || (sc->smoother_fp16 && !sc->refine))
+      (sc->refine && !(sc->tol > 0)) || (sc->smoother_fp16 && !sc->refine) || (sc->krylov && !sc->refine))
     return set_err(CF_EINVAL, "bad schedule");
   for (auto* s : pl->slots)
     if (s) { int rc = cf_sim_set_schedule(s, sc); if (rc) return rc; }

@@ -131,14 +131,17 @@ class FluidSim:
     def write_params(self):  # js/fluid.js:180-199
         check(L.lib().cf_sim_write_params(self._h), "writeParams")
 
-    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0):
+    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0,
+                     krylov=0):
         """Multigrid schedule (reference: nu 2, 1 V-cycle, 120 coarsest sweeps).
         tol > 0 runs up to `vcycles` V-cycles until ||r||/||rhs|| < tol; cfl > 0
         takes dt = cfl*dx/max(max|u|, umin) each step, on the device; refine = 1
         keeps the level-0 pressure and residual in FP64 (defect correction);
         smoother_fp16 = 1 (with refine) stores p and the face coefficients of
-        the tiled multigrid levels in FP16, arithmetic staying FP32."""
-        s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine)
+        the tiled multigrid levels in FP16, arithmetic staying FP32; krylov = 1
+        (with refine) runs flexible conjugate gradients at level 0, each
+        V-cycle preconditioning one iteration."""
+        s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine, krylov)
         check(L.lib().cf_sim_set_schedule(self._h, C.byref(s)), "set_schedule")
 
     def upload_pieces(self, data, n):  # js/fluid.js:201-204
@@ -267,10 +270,14 @@ class FluidSim:
 
     readBuffer = read_buffer
 
-    def read_p64(self):
-        """The FP64 level-0 pressure of defect correction (refine = 1)."""
+    def read_p64(self, name="p64"):
+        """The FP64 level-0 pressure of defect correction (refine = 1), or with
+        name "r64", "d64", "q64" the Krylov residual, direction and B times the
+        direction (krylov = 1)."""
+        if name not in ("p64", "r64", "d64", "q64"):
+            raise KeyError(name)
         a = np.empty(self.nx * self.ny, np.float64)
-        check(L.lib().cf_sim_read_field(self._h, b"p64", a.ctypes.data, a.nbytes), "read p64")
+        check(L.lib().cf_sim_read_field(self._h, name.encode(), a.ctypes.data, a.nbytes), "read " + name)
         return a
 
     def write_field(self, name, arr):

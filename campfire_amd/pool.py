@@ -51,8 +51,9 @@ class ScenePool:
     def __exit__(self, *a):
         self.close()
 
-    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0):
-        s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine)
+    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0,
+                     krylov=0):
+        s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine, krylov)
         L.check(L.lib().cf_pool_set_schedule(self._h, C.byref(s)), "pool schedule")
 
     def admit(self, nx, ny, dx, dt):

@@ -93,6 +93,10 @@ typedef struct cf_schedule {
                             residual b - A p in FP64, each V-cycle (FP32) solving
                             for the correction; the projection takes its gradient
                             from the FP64 pressure.  Needs tol > 0. */
+  int32_t krylov;        /* 1 = flexible conjugate gradients at level 0 (FCG with
+                            one stored direction), each V-cycle preconditioning
+                            one iteration: the pressure, residual, direction and
+                            its image under A in FP64.  Needs refine = 1. */
 } cf_schedule;
 
 typedef struct cf_sim_info {

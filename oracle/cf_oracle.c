@@ -106,6 +106,10 @@ typedef struct {
   REAL *p[MAXLEV], *rhs[MAXLEV], *ax[MAXLEV], *ay[MAXLEV];
   double* p64;            /* defect correction: level-0 pressure in FP64 */
   REAL* b0;               /* defect correction: level-0 right-hand side */
+  double *r64, *d64, *q64; /* Krylov: residual, direction, A times the direction */
+  double kbb, kdq;        /* Krylov: ||b||^2, (d, A d) of the stored direction (0: none) */
+  double kbest;           /* Krylov: lowest ||r|| / ||b|| of this solve */
+  int kfall;              /* Krylov: fallen back to defect correction for this solve */
   float sig, isig;        /* FP16 smoother: p stored as p * sig, a power of two */
   cf_schedule sch;
   int last_vcycles;
@@ -781,6 +785,19 @@ static double resid_norm(const S* s) {
   return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
 }
 
+/* FP16 smoother: the power of two bringing the rms of a residual with
+ * ||r||^2 = rr into [0.5, 1) scales the stored correction */
+static void set_sig(S* s, double rr) {
+  int ex = 0;
+  const double rms = sqrt(rr / ((double)s->nx * (double)s->ny));
+  if (rms > 0) {
+    frexp(rms, &ex);
+    ex = ex < -60 ? -60 : (ex > 60 ? 60 : ex);
+  }
+  s->sig = (float)ldexp(1.0, -ex);
+  s->isig = (float)ldexp(1.0, ex);
+}
+
 /* defect correction (the build's schedule option refine = 1): r = b - A p64
  * in FP64 into rhs[0] with p[0] = 0 as the correction's starting guess;
  * returns ||r|| / ||b|| */
@@ -804,22 +821,139 @@ static double dc_defect(S* s, int first) {
       if (j > 0) { off += so * p[k - nx]; diag += so; }
       if (j < ny - 1) { off += n * p[k + nx]; diag += n; } else diag += 2.0 * n;
       const double r = diag <= 0 ? 0.0 : b - (off - diag * p[k]) / h2;
+      if (first && s->sch.krylov) s->r64[k] = r;
       s->rhs[0][k] = (REAL)r;
       s->p[0][k] = 0;
       rr += r * r; bb += b * b;
     }
-  {
-    /* FP16 smoother: the power of two bringing the residual's rms into [0.5, 1) */
-    int ex = 0;
-    const double rms = sqrt(rr / ((double)nx * (double)ny));
-    if (rms > 0) {
-      frexp(rms, &ex);
-      ex = ex < -60 ? -60 : (ex > 60 ? 60 : ex);
-    }
-    s->sig = (float)ldexp(1.0, -ex);
-    s->isig = (float)ldexp(1.0, ex);
-  }
+  set_sig(s, rr);
   return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+}
+
+/* Sum of v[0..n) in the order of the GPU's two-pass reduction (the
+ * build's convention for FP64 norms and dot products, not the reference's):
+ * vg workgroups of 256 threads, thread t of workgroup b adding the cells
+ * b*256 + t + m*vg*256 in turn; each 64-lane wave folding its lanes as a tree
+ * (lane i += lane i+o for o = 32, 16, ..., 1); each workgroup's partial
+ * ((w0 + w1) + w2) + w3 of its four waves; then the partials folded the same
+ * way by one workgroup (thread t adding partials t, t+256, ...).  vg is
+ * ceil(n / 1024) capped at 1024. */
+static double gpu_sum(const double* v, int n) {
+  int vg = (n + 1023) / 1024;
+  if (vg > 1024) vg = 1024;
+  if (vg < 1) vg = 1;
+  double* part = malloc(sizeof(double) * (size_t)vg);
+  double lane[256];
+  for (int pass = 0; pass < 2; pass++) {
+    const int nb = pass == 0 ? vg : 1;
+    for (int b = 0; b < nb; b++) {
+      for (int t = 0; t < 256; t++) {
+        double a = 0;
+        if (pass == 0)
+          for (long k = (long)b * 256 + t; k < n; k += (long)vg * 256) a += v[k];
+        else
+          for (int q = t; q < vg; q += 256) a += part[q];
+        lane[t] = a;
+      }
+      for (int w = 0; w < 4; w++) {
+        double* L = lane + 64 * w;
+        for (int o = 32; o > 0; o >>= 1)
+          for (int i = 0; i < o; i++) L[i] = L[i] + L[i + o];
+      }
+      const double r = ((lane[0] + lane[64]) + lane[128]) + lane[192];
+      if (pass == 0) part[b] = r;
+      else { free(part); return r; }
+    }
+  }
+  free(part);
+  return 0;
+}
+
+/* B z at level 0 for the Krylov iteration: (sum a z_nb - diag z) / h2 in FP64
+ * with the relaxation's boundary terms, 0 at a closed cell */
+static double apply64(const S* s, const double* z, int i, int j) {
+  const int nx = s->nx, ny = s->ny;
+  const REAL *ax = s->ax[0], *ay = s->ay[0];
+  const double w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1];
+  const double so = ay[j * nx + i], n = ay[(j + 1) * nx + i];
+  const int k = j * nx + i;
+  double off = 0, diag = 0;
+  if (i > 0) { off += w * z[k - 1]; diag += w; } else diag += 2.0 * w;
+  if (i < nx - 1) { off += e * z[k + 1]; diag += e; } else diag += 2.0 * e;
+  if (j > 0) { off += so * z[k - nx]; diag += so; }
+  if (j < ny - 1) { off += n * z[k + nx]; diag += n; } else diag += 2.0 * n;
+  if (diag <= 0) return 0;
+  return (off - diag * z[k]) / (double)s->lh2[0];
+}
+
+/* Krylov start (schedule option krylov = 1, after dc_defect(s, 1) has left
+ * the FP64 residual b - B p64 in r64): ||b||^2, no stored direction, the
+ * starting residual as the lowest so far */
+static void kry_init(S* s) {
+  const int n = s->nx * s->ny;
+  double* v = malloc(sizeof(double) * (size_t)n);
+  for (int k = 0; k < n; k++) v[k] = (double)s->b0[k] * (double)s->b0[k];
+  s->kbb = gpu_sum(v, n);
+  for (int k = 0; k < n; k++) v[k] = s->r64[k] * s->r64[k];
+  const double rr = gpu_sum(v, n);
+  s->kbest = s->kbb > 0 ? sqrt(rr / s->kbb) : sqrt(rr);
+  s->kdq = 0;
+  s->kfall = 0;
+  free(v);
+}
+
+/* One flexible-CG iteration after the V-cycle has left z = M r in p[0]
+ * (Notay's FCG with one stored direction): beta = (z, B d) / (d, B d),
+ * d <- z - beta d, alpha = (r, z) / (d, B d), p64 += alpha d, r -= alpha B d.
+ * B d is carried as q <- B z - beta q, and (d, B d) expands to
+ * (z, B z) - beta (z, q), so the three dot products come from z alone.
+ * Then r (FP32) is the next V-cycle's right-hand side with e = 0 as its
+ * start; returns ||r|| / ||b||.
+ * A residual above twice the lowest of the solve so far ends the Krylov
+ * iteration for this solve: the later cycles are plain defect correction
+ * (p64 += z, r = b - B p64) from where it stands.  A singular level-0 system
+ * (open cells closed off from every Dirichlet side) with a right-hand side
+ * outside its range makes CG diverge once the compatible part is solved,
+ * while defect correction stalls at the incompatible part. */
+static double kry_step(S* s) {
+  const int nx = s->nx, ny = s->ny, n = nx * ny;
+  if (s->kfall) {
+    for (int k = 0; k < n; k++) s->p64[k] += (double)s->p[0][k];
+    return dc_defect(s, 0);
+  }
+  double* z = malloc(sizeof(double) * (size_t)n);
+  double* bz = malloc(sizeof(double) * (size_t)n);
+  double* v = malloc(sizeof(double) * (size_t)n);
+  for (int k = 0; k < n; k++) z[k] = (double)s->p[0][k];
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) bz[j * nx + i] = apply64(s, z, i, j);
+  for (int k = 0; k < n; k++) v[k] = z[k] * s->q64[k];
+  const double zq = gpu_sum(v, n);
+  for (int k = 0; k < n; k++) v[k] = s->r64[k] * z[k];
+  const double rz = gpu_sum(v, n);
+  for (int k = 0; k < n; k++) v[k] = z[k] * bz[k];
+  const double zbz = gpu_sum(v, n);
+  const double beta = s->kdq != 0 ? zq / s->kdq : 0.0;
+  const double dq = zbz - beta * zq;
+  const double alpha = dq != 0 ? rz / dq : 0.0;
+  s->kdq = dq;
+  for (int k = 0; k < n; k++) {
+    const double d = z[k] - beta * s->d64[k];
+    const double q = bz[k] - beta * s->q64[k];
+    s->p64[k] += alpha * d;
+    const double r = s->r64[k] - alpha * q;
+    s->d64[k] = d; s->q64[k] = q; s->r64[k] = r;
+    s->rhs[0][k] = (REAL)r;
+    s->p[0][k] = 0;
+    v[k] = r * r;
+  }
+  const double rr = gpu_sum(v, n);
+  set_sig(s, rr);
+  free(z); free(bz); free(v);
+  const double rel = s->kbb > 0 ? sqrt(rr / s->kbb) : sqrt(rr);
+  if (rel > 2.0 * s->kbest) s->kfall = 1;
+  if (rel < s->kbest) s->kbest = rel;
+  return rel;
 }
 
 #ifndef CFO_DOUBLE
@@ -1169,6 +1303,7 @@ void* PFX(create)(int nx, int ny, double dx, double dt, int* err) {             
   s->V = calloc(nV, sizeof(REAL)); s->Vh = calloc(nV, sizeof(REAL)); s->V2 = calloc(nV, sizeof(REAL));
   s->T = calloc(nC, sizeof(REAL)); s->Th = calloc(nC, sizeof(REAL)); s->T2 = calloc(nC, sizeof(REAL));
   s->p64 = calloc(nC, sizeof(double)); s->b0 = calloc(nC, sizeof(REAL));
+  s->r64 = calloc(nC, sizeof(double)); s->d64 = calloc(nC, sizeof(double)); s->q64 = calloc(nC, sizeof(double));
   s->Y = calloc(4 * nC, sizeof(REAL)); s->Yh = calloc(4 * nC, sizeof(REAL)); s->Y2 = calloc(4 * nC, sizeof(REAL));
   s->SQ = calloc(2 * nC, sizeof(REAL));
   s->sdf = calloc(nC, sizeof(REAL)); s->sid = calloc(nC, sizeof(uint32_t));
@@ -1193,7 +1328,7 @@ void* PFX(create)(int nx, int ny, double dx, double dt, int* err) {             
   memcpy(s->pv, def, sizeof(def));
   s->sch.nu = 2; s->sch.vcycles = 1; s->sch.tol = 0; s->sch.coarse_iters = 120; s->sch.smoother_fp16 = 0;
   s->sig = s->isig = 1.0f;
-  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0;
+  s->sch.cfl = 0; s->sch.umin = 1.0f; s->sch.refine = 0; s->sch.krylov = 0;
   s->diffSub = 1;
   return s;
 }
@@ -1203,7 +1338,7 @@ void PFX(destroy)(void* h) {
   if (!s) return;
   free(s->U); free(s->Uh); free(s->U2); free(s->V); free(s->Vh); free(s->V2);
   free(s->T); free(s->Th); free(s->T2); free(s->Y); free(s->Yh); free(s->Y2);
-  free(s->p64); free(s->b0);
+  free(s->p64); free(s->b0); free(s->r64); free(s->d64); free(s->q64);
   free(s->SQ); free(s->sdf); free(s->sid); free(s->fx); free(s->fy);
   for (int l = 0; l < s->nlev; l++) { free(s->p[l]); free(s->rhs[l]); free(s->ax[l]); free(s->ay[l]); }
   free(s);
@@ -1264,6 +1399,7 @@ int PFX(set_schedule)(void* h, const cf_schedule* sc) {
   if (sc->nu < 0 || sc->vcycles < 1 || sc->coarse_iters < 0) return CF_EINVAL;
   if (!(sc->cfl >= 0) || (sc->cfl > 0 && !(sc->umin > 0))) return CF_EINVAL;
   if (sc->refine && !(sc->tol > 0)) return CF_EINVAL;
+  if (sc->krylov && !sc->refine) return CF_EINVAL;
 #ifdef CFO_DOUBLE
   if (sc->smoother_fp16) return CF_EINVAL; /* an FP32-arithmetic option */
 #else
@@ -1279,6 +1415,7 @@ int PFX(last_vcycles)(void* h) { return ((S*)h)->last_vcycles; }
 #ifndef CFO_DOUBLE
 void cfo_f2h(const float* a, uint16_t* h, int n) { for (int k = 0; k < n; k++) h[k] = cfo_f2h1(a[k]); }
 void cfo_h2f(const uint16_t* h, float* a, int n) { for (int k = 0; k < n; k++) a[k] = cfo_h2f1(h[k]); }
+double cfo_gpu_sum(const double* v, int n) { return gpu_sum(v, n); }
 #endif
 double PFX(last_resid)(void* h) { return ((S*)h)->last_resid; }
 
@@ -1336,9 +1473,11 @@ int PFX(run_stage)(void* h, int stage) {
       double r = -1;
       if (s->sch.tol > 0 && s->sch.refine) {
         r = dc_defect(s, 1);
+        if (s->sch.krylov) kry_init(s);
         for (int c = 0; c < s->sch.vcycles && !(r < s->sch.tol); c++) {
           vcycle(s, s->sch.nu);
           nv++;
+          if (s->sch.krylov) { r = kry_step(s); continue; }
           for (int k = 0; k < s->nC; k++) s->p64[k] += (double)s->p[0][k];
           r = dc_defect(s, 0);
         }
@@ -1481,7 +1620,9 @@ int PFX(read_field64)(void* h, const char* name, double* dst, size_t count) {
   S* s = (S*)h;
   if (!strcmp(name, "sdf")) { if (count != (size_t)s->nC) return CF_EINVAL; for (int k = 0; k < s->nC; k++) dst[k] = s->sdf[k]; return 0; }
   if (!strcmp(name, "rad")) { if (count != CF_RAD_LEN * 4) return CF_EINVAL; for (int q = 0; q < CF_RAD_LEN * 4; q++) dst[q] = s->rad[q]; return 0; }
-  if (!strcmp(name, "p64")) { if (count != (size_t)s->nC) return CF_EINVAL; memcpy(dst, s->p64, sizeof(double) * s->nC); return 0; }
+  const double* src = !strcmp(name, "p64") ? s->p64 : !strcmp(name, "r64") ? s->r64 : !strcmp(name, "d64") ? s->d64
+                    : !strcmp(name, "q64") ? s->q64 : NULL;
+  if (src) { if (count != (size_t)s->nC) return CF_EINVAL; memcpy(dst, src, sizeof(double) * s->nC); return 0; }
   size_t c; int lev;
   REAL* f = field(s, name, &c, &lev);
   if (!f || count != c) return CF_EINVAL;

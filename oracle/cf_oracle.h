@@ -26,6 +26,8 @@ cfo_flame_result cfo_flame_speed_ex(double B, double Ta, double dx, double F, do
 /* IEEE binary16 conversions of the oracle (round to nearest even, subnormals kept) */
 void cfo_f2h(const float* a, uint16_t* h, int n);
 void cfo_h2f(const uint16_t* h, float* a, int n);
+/* sum of v[0..n) in the GPU's fixed reduction order (norms and dot products) */
+double cfo_gpu_sum(const double* v, int n);
 cfo_flame_result cfo_flame_speed32_ex(double B, double Ta, double dx, double F, double phi, double L, double tEnd,
                                       double Tamb, double Tign);
 

@@ -31,7 +31,7 @@ class CfParams(C.Structure):
 
 class CfSchedule(C.Structure):
     _fields_ = [("nu", C.c_int32), ("vcycles", C.c_int32), ("tol", C.c_float),
-                ("coarse_iters", C.c_int32), ("smoother_fp16", C.c_int32), ("cfl", C.c_float), ("umin", C.c_float), ("refine", C.c_int32)]
+                ("coarse_iters", C.c_int32), ("smoother_fp16", C.c_int32), ("cfl", C.c_float), ("umin", C.c_float), ("refine", C.c_int32), ("krylov", C.c_int32)]
 
 
 class FlameResult(C.Structure):
@@ -78,6 +78,8 @@ def lib():
             getattr(L, p + "closure").argtypes = [C.c_void_p, C.c_char_p, C.c_double]
             getattr(L, p + "closure").restype = C.c_double
         L.cfo_f2h.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.cfo_gpu_sum.argtypes = [C.c_void_p, C.c_int]
+        L.cfo_gpu_sum.restype = C.c_double
         L.cfo_f2h.restype = None
         L.cfo_h2f.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.cfo_h2f.restype = None
@@ -104,6 +106,12 @@ def h2f(h):
     a = np.zeros(h.size, np.float32)
     lib().cfo_h2f(h.ctypes.data, a.ctypes.data, h.size)
     return a
+
+
+def gpu_sum(v):
+    """Sum of a float64 array in the GPU's fixed reduction order."""
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    return float(lib().cfo_gpu_sum(a.ctypes.data, a.size))
 
 
 def flame_speed(B, Ta, dx, F=1.0, phi=1.0, L=0.12, tEnd=0.25, Tamb=None, Tign=None, fp32=False):
@@ -195,8 +203,9 @@ class Oracle:
         """dt of the parameter block (the CFL step writes it each step)."""
         return float(self.packed_params().dt)
 
-    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0):
-        s = CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine)
+    def set_schedule(self, nu=2, vcycles=1, tol=0.0, coarse_iters=120, smoother_fp16=0, cfl=0.0, umin=1.0, refine=0,
+                     krylov=0):
+        s = CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine, krylov)
         self._chk(self._f("set_schedule")(self.h, C.byref(s)), "set_schedule")
 
     def reset(self):
@@ -237,7 +246,8 @@ class Oracle:
         nC, nU, nV = nx * ny, (nx + 1) * ny, nx * (ny + 1)
         tab = {"U": nU, "Uh": nU, "U2": nU, "V": nV, "Vh": nV, "V2": nV, "T": nC, "Th": nC, "T2": nC,
                "Y": 4 * nC, "Yh": 4 * nC, "Y2": 4 * nC, "SQ": 2 * nC, "cellG": 2 * nC, "fx": nU, "fy": nV,
-               "acc": 528, "faceW": 256, "pieces": 1280, "rad": 1280, "p64": nC}
+               "acc": 528, "faceW": 256, "pieces": 1280, "rad": 1280, "p64": nC,
+               "r64": nC, "d64": nC, "q64": nC}
         if name in tab:
             return tab[name]
         lx, ly = nx, ny

@@ -16,5 +16,5 @@ def test_autotune_picks_fastest_converging_schedule():
     assert best == min(ok, key=lambda r: r[1])[0]
     # the snapshot makes the candidates comparable: the same schedule twice
     # gives the same V-cycle counts
-    again, t2 = tune(sizes=(128, 256), steps=4, warm=16, candidates=[{"nu": best["nu"], "coarse_iters": best["coarse_iters"]}] * 2)
+    again, t2 = tune(sizes=(128, 256), steps=4, warm=16, candidates=[{k: best[k] for k in ("nu", "coarse_iters", "krylov")}] * 2)
     assert t2[0][3] == t2[1][3]

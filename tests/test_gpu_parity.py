@@ -313,6 +313,50 @@ def test_direct_coarsest_vcycle_matches_oracle(nx, ny):
     o.close()
 
 
+@pytest.mark.parametrize("nx,ny,fp16", [(128, 128, 0), (256, 192, 1), (512, 512, 1)])
+def test_krylov_matches_oracle(nx, ny, fp16):
+    """Flexible CG at level 0 with the V-cycle as preconditioner against the
+    oracle's restatement: three iterations from the same start (pressure,
+    residual, direction and its image in FP64; the dot products sum in the
+    same fixed order on both sides), then a solve towards 1e-6.  The random
+    pile of the 256x192 and 512x512 cases encloses open cells, so their
+    systems are singular: there the solve falls back to defect correction and
+    stalls at the incompatible part of the right-hand side, on both sides."""
+    _flush("[parity] krylov", nx, ny, fp16)
+    g, o, pieces, n = make_pair(nx, ny, 0.7 / nx, 0.002, seed=31, npieces=5)
+    sync_geometry(g, o)
+    put(g, o, "SQ", np.random.default_rng(5).uniform(-0.5, 0.5, nx * ny * 2).astype(np.float32))
+    g.run_stage(4)
+    o.run_stage(4)
+    p0 = np.random.default_rng(6).uniform(-1, 1, nx * ny).astype(np.float32)
+    put(g, o, "p", p0)
+    for q in (g, o):  # switching refine on starts p64 from p
+        q.set_schedule(nu=2, vcycles=3, tol=1e-30, coarse_iters=24, refine=1, smoother_fp16=fp16, krylov=1)
+    g.run_stage(5)
+    o.run_stage(5)
+    assert g.info().last_vcycles == o.last_vcycles == 3
+    for name in ("p64", "r64", "d64", "q64"):
+        r = rel_l2(g.read_p64(name), o.read64(name))
+        _flush("  3 iterations: %s rel_l2=%.3g" % (name, r))
+        assert r <= 1e-12, name
+    assert abs(g.info().last_resid - o.last_resid) <= 1e-4 * o.last_resid
+    for q in (g, o):
+        q.set_schedule(nu=2, vcycles=40, tol=1e-6, coarse_iters=24, refine=1, smoother_fp16=fp16, krylov=1)
+    g.run_stage(5)
+    o.run_stage(5)
+    ig = g.info()
+    _flush("  solve: cycles %d / %d, resid %.3g / %.3g" % (ig.last_vcycles, o.last_vcycles, ig.last_resid,
+                                                        o.last_resid))
+    assert (ig.last_resid < 1e-6) == (o.last_resid < 1e-6)
+    if nx == 128:
+        assert o.last_resid < 1e-6
+    assert abs(ig.last_resid - o.last_resid) <= 1e-3 * o.last_resid
+    assert abs(ig.last_vcycles - o.last_vcycles) <= 1
+    assert rel_l2(g.read_p64(), o.read64("p64")) <= 1e-6
+    g.close()
+    o.close()
+
+
 @pytest.mark.parametrize("nx,ny,ntiled", [(128, 128, 1), (256, 192, 2), (512, 512, 3)])
 def test_fp16_smoother_vcycle_matches_oracle(nx, ny, ntiled):
     """The FP16 smoother (p and coefficients of the tiled levels in FP16, FP32

@@ -87,6 +87,11 @@ def test_batched_equals_alone_fp16_smoother_schedule():
                          "refine": 1, "smoother_fp16": 1}, frames=1, steps=4)
 
 
+def test_batched_equals_alone_krylov_schedule():
+    _compare_with_alone({"nu": 3, "vcycles": 30, "tol": 1e-6, "coarse_iters": 24, "cfl": 0.5, "umin": 1.0,
+                         "refine": 1, "smoother_fp16": 1, "krylov": 1}, frames=1, steps=4)
+
+
 def test_admission_follows_budget():
     from campfire_amd.pool import ScenePool, scene_bytes
     b256, b512 = scene_bytes(256, 256, 0.005), scene_bytes(512, 512, 0.0025)

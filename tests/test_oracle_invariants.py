@@ -663,3 +663,145 @@ def test_fp16_smoother_defect_correction_reaches_the_direct_solve(n):
         assert o3.last_resid < 0.05, (fp16, o3.last_resid)
     d = np.linalg.norm(outs[1] - outs[0]) / np.linalg.norm(outs[0])
     assert 1e-4 < d < 5e-2, d
+
+
+def _apply_B(o, z):
+    """B z = (sum of a z over the neighbours - diag z) / h2 at level 0 in
+    float64, with the relaxation's boundary terms (twice the face coefficient
+    at the open sides and the top), 0 at closed cells."""
+    nx, ny = o.nx, o.ny
+    ax = o.read64("ax").reshape(ny, nx + 1)
+    ay = o.read64("ay").reshape(ny + 1, nx)
+    z = np.asarray(z, np.float64).reshape(ny, nx)
+    off = np.zeros((ny, nx))
+    diag = np.zeros((ny, nx))
+    w, e, s, n = ax[:, :-1], ax[:, 1:], ay[:-1], ay[1:]
+    off[:, 1:] += w[:, 1:] * z[:, :-1]
+    diag[:, 1:] += w[:, 1:]
+    diag[:, 0] += 2 * w[:, 0]
+    off[:, :-1] += e[:, :-1] * z[:, 1:]
+    diag[:, :-1] += e[:, :-1]
+    diag[:, -1] += 2 * e[:, -1]
+    off[1:] += s[1:] * z[:-1]
+    diag[1:] += s[1:]
+    off[:-1] += n[:-1] * z[1:]
+    diag[:-1] += n[:-1]
+    diag[-1] += 2 * n[-1]
+    h2 = float(f32(o.dx ** 2))
+    return np.where(diag > 0, (off - diag * z) / h2, 0.0).reshape(-1)
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 256, 1023, 1024, 1025, 4097, 300001, (1 << 20) + 12345])
+def test_gpu_order_sum_is_a_sum(n):
+    """The fixed-order FP64 sum that the Krylov dot products use adds every
+    element once: it equals the exactly rounded sum to FP64 rounding, over
+    sizes with one partial workgroup, exactly 1024 workgroups and more
+    elements than 1024 x 256 threads."""
+    from oracle.pyoracle import gpu_sum
+    v = np.random.default_rng(n).standard_normal(n) * 10.0 ** np.random.default_rng(n + 1).integers(-3, 3, n)
+    s = gpu_sum(v)
+    assert abs(s - math.fsum(v)) <= 1e-13 * max(np.sum(np.abs(v)), 1e-300)
+    if n > 1:  # one element changed by a unit is seen, whichever it is
+        for k in (0, n // 2, n - 1):
+            w = v.copy()
+            w[k] += 1.0
+            assert abs(gpu_sum(w) - s - 1.0) < 1e-6 * max(1.0, np.sum(np.abs(v)) * 1e-9)
+
+
+@pytest.mark.parametrize("n,fp16", [(64, 0), (128, 0), (128, 1)])
+def test_krylov_reaches_the_direct_solve_in_fewer_cycles(n, fp16):
+    """Flexible CG with the V-cycle as preconditioner (24 coarsest sweeps, so
+    the V-cycle alone converges slowly): the FP64 pressure converges to the
+    sparse direct solve of the level-0 system, in fewer V-cycles than defect
+    correction alone with the same V-cycle."""
+    cyc = []
+    for kry in (0, 1):
+        o = _projection_case(n, n, fp64=False)
+        o.run_stage(4)
+        ref = _poisson_direct(o)
+        o.set_schedule(nu=2, vcycles=80, tol=1e-9, coarse_iters=24, refine=1, smoother_fp16=fp16, krylov=kry)
+        assert o.run_stage(5) == 0
+        p64 = o.read64("p64").reshape(n, n)
+        assert np.max(np.abs(p64 - ref)) < 1e-6 * np.max(np.abs(ref)), kry
+        assert o.last_resid < 1e-9
+        cyc.append(o.last_vcycles)
+        o.close()
+    assert cyc[1] < cyc[0], cyc
+
+
+def test_krylov_iteration_identities():
+    """After k = 1..4 iterations (cycle cap k): q = B d (the carried image of
+    the direction), r = b - B p64 (the carried residual), (r, d) = 0 (exact
+    line search along d) and (d_k, B d_{k-1}) = 0 (each direction conjugate to
+    the one before).  A wrong sign or a dropped term in the direction, step or
+    residual updates breaks one of them."""
+    n = 96
+    prev_q = None
+    for k in range(1, 5):
+        o = _projection_case(n, n, fp64=False)
+        o.run_stage(4)
+        b = o.read64("rhs")
+        o.set_schedule(nu=2, vcycles=k, tol=1e-30, coarse_iters=24, refine=1, krylov=1)
+        o.run_stage(5)
+        assert o.last_vcycles == k
+        r, d, q, x = o.read64("r64"), o.read64("d64"), o.read64("q64"), o.read64("p64")
+        nb = np.linalg.norm(b)
+        assert np.linalg.norm(q - _apply_B(o, d)) <= 1e-12 * np.linalg.norm(q), k
+        assert np.linalg.norm(r - (b - _apply_B(o, x))) <= 1e-11 * nb, k
+        assert abs(r @ d) <= 1e-9 * np.linalg.norm(r) * np.linalg.norm(d), k
+        if prev_q is not None:
+            assert abs(d @ prev_q) <= 1e-9 * np.linalg.norm(d) * np.linalg.norm(prev_q), k
+        prev_q = q
+        assert np.isclose(o.last_resid, np.linalg.norm(r) / nb, rtol=1e-9)
+        o.close()
+
+
+def test_krylov_needs_defect_correction():
+    o = _projection_case(32, 32, fp64=False)
+    with pytest.raises(Exception):
+        o.set_schedule(nu=2, vcycles=4, tol=1e-6, krylov=1)
+    o.close()
+
+
+def _pocket_case(n=48, seed=5):
+    """A ring of eight overlapping squares encloses open cells that reach no
+    Dirichlet side: the level-0 system is singular there, and random sources
+    put the right-hand side partly outside its range."""
+    o = Oracle(n, n, 0.002, 1e-3)
+    o.reset()
+    p = np.zeros((64, 20), np.float32)
+    for q in range(8):
+        a = 2 * np.pi * q / 8
+        p[q, :8] = [0.048 + 0.02 * np.cos(a), 0.045 + 0.02 * np.sin(a), a, 0.01, 0, 0, 0, 1]
+        p[q, 8:12] = 300.0
+        p[q, 16:20] = 1.0
+    o.upload_pieces(p.reshape(-1), 8)
+    o.write_params()
+    o.clear_geometry()
+    o.geometry()
+    rng = np.random.default_rng(seed)
+    o.write("T", rng.uniform(300, 1500, n * n).astype(np.float32))
+    o.write("U", rng.uniform(-1, 1, (n + 1) * n).astype(np.float32))
+    o.write("V", rng.uniform(-1, 1, n * (n + 1)).astype(np.float32))
+    sq = np.zeros((n * n, 2), np.float32)
+    sq[:, 0] = rng.uniform(-5, 5, n * n)
+    o.write("SQ", sq.reshape(-1))
+    return o
+
+
+def test_krylov_falls_back_on_a_singular_system():
+    """On a system with an enclosed pocket, defect correction stalls at the
+    incompatible part of the right-hand side; the Krylov iteration, which
+    would diverge there, falls back to defect correction and ends at the same
+    residual with the same bounded pressure."""
+    out = []
+    for kry in (0, 1):
+        o = _pocket_case()
+        o.run_stage(4)
+        o.set_schedule(nu=2, vcycles=30, tol=1e-9, coarse_iters=24, refine=1, krylov=kry)
+        assert o.run_stage(5) != 0  # no convergence: the target is out of reach
+        out.append((o.last_resid, np.max(np.abs(o.read64("p64")))))
+        o.close()
+    (r_dc, p_dc), (r_kr, p_kr) = out
+    assert 1e-5 < r_dc < 1e-2, r_dc
+    assert r_kr <= 1.5 * r_dc and p_kr <= 2 * p_dc, out
