@@ -386,7 +386,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
     ap.add_argument("--batch", type=int, default=64, help="scenes of the batched measurement (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=40)
-    ap.add_argument("--fcg-nu", type=int, default=2, help="smoother sweeps of the batched FCG form")
+    ap.add_argument("--fcg-nu", type=int, default=3, help="smoother sweeps of the batched FCG form")
     ap.add_argument("--fcg-coarse", type=int, default=24, help="coarsest sweeps of the batched FCG form (0: direct)")
     ap.add_argument("--conv-steps", type=int, default=6,
                     help="steps of the equal-accuracy batch comparison (0: skip)")

@@ -226,30 +226,33 @@ unsigned resnorm_blocks(const Level& L0);
 // defect correction at level 0: r = b - A p64 (FP64) into L0.rhs, L0.p = 0
 // (ph0 non-null: its FP16 p instead), ||r||/||b|| and the FP16 scale into
 // dyn; then, after a V-cycle on the correction, p64 += L0.p (or ph0 * isig)
-// Krylov iteration (schedule krylov = 1), after a V-cycle left z in p (or
-// ph0 * isig): kry_dots forms w = B z and the dot products (z, q), (r, z),
-// (z, w) and from them the step coefficients; kry_step updates the direction
-// d, q = B d, the pressure and the residual, and sets up the next V-cycle's
-// right-hand side and the convergence flag.  part holds 3 doubles per
+// Krylov iteration (schedule krylov = 1): flexible CG at level 0 with the
+// V-cycle as preconditioner.  After a V-cycle left z in p (or ph0 * isig),
+// kry_step runs kry_dots (the dot products with B z), kry_coef (alpha,
+// beta), kry_update (the direction and the FP64 pressure), then dc_defect
+// and resid_fold (the residual from the pressure, the next V-cycle's
+// right-hand side, the convergence flag).  part holds 3 doubles per
 // workgroup.  A residual above twice the lowest of the solve so far switches
-// the rest of the solve to defect correction (kry_dots: p64 += z; kry_step:
-// r = b - B p64), which stalls where CG diverges: at a right-hand side
-// outside the range of a singular system.
+// the rest of the solve to defect correction (alpha = 1, beta = 0), which
+// stalls where CG diverges: at a right-hand side outside the range of a
+// singular system.
 struct KryBufs {
-  double *p64, *r64, *d64, *q64, *w64;
-  float* b0;  // the defect correction's saved right-hand side (fallback)
+  double *p64, *d64;  // the pressure and the stored direction
+  float* b0;          // the saved right-hand side
 };
-void launch_kry_dots(const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k, double* part, hipStream_t s);
 void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
                      hipStream_t s);
-// fold: the phase of the one-workgroup-per-scene kernel that follows
-void phase_kry_dots(BatchPhase& ph, BatchPhase& fold, const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k,
-                    double* part);
-void phase_kry_step(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k,
-                    double* part, float tol);
-// r64 (Krylov, first call only; else null): the FP64 residual kept too
-void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, double* r64, Dyn* dyn, double* part,
-                      float tol, bool first, hipStream_t s);
+// *out = 1 when the converged flag of each of the n scenes is set, else 0
+void launch_all_converged(Dyn* const* d, int n, int* out, hipStream_t s);
+// the kernels of launch_kry_start / launch_kry_step, one phase each
+void phase_kry_step(BatchPhase (&ph)[5], const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part,
+                    float tol);
+void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
+                      bool first, hipStream_t s);
+// the Krylov start: a line search along the previous pressure (in p64) from
+// zero, which stays as the stored direction
+void launch_kry_start(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
+                      hipStream_t s);
 void launch_dc_accum(const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip, hipStream_t s);
 void launch_p32_to_64(const float* p, double* p64, int n, hipStream_t s);
 void launch_resnorm(const Level& L0, Dyn* dyn, double* part, float tol, hipStream_t s);
@@ -281,7 +284,9 @@ void phase_coef_restrict(BatchPhase& ph, const Level& f, const Level& c);
 void phase_cfl(BatchPhase& ph, const Fields& F, float cfl, float umin, int nsub);
 void phase_resnorm(BatchPhase& ph, BatchPhase& fold, const Level& L0, Dyn* dyn, double* part, float tol);
 void phase_dc_defect(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, double* p64, float* b0,
-                     double* r64, Dyn* dyn, double* part, float tol, bool first);
+                     Dyn* dyn, double* part, float tol, bool first);
+void phase_kry_start(BatchPhase (&ph)[5], const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part,
+                     float tol);
 void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip);
 void phase_coef_half(BatchPhase& ph, const Level& L, const LevelH& H);
 void phase_project(BatchPhase& ph, const Fields& F, const double* p64);

@@ -1028,7 +1028,6 @@ struct DcDefectOp {
     __half* ph0;  // FP16 smoother: the correction's start e = 0 goes here instead of L.p
     double* p64;
     float* b0;
-    double* r64;  // Krylov start: the FP64 residual is kept too
     Dyn* d;
     double* part;
     int first;
@@ -1042,7 +1041,6 @@ struct DcDefectOp {
     const Level& L = a_.L;
     const double* __restrict__ p64 = a_.p64;
     float* __restrict__ b0 = a_.b0;
-    double* __restrict__ r64 = a_.r64;
     const int n = L.nx * L.ny;
     for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
       const float bf = FIRST ? L.rhs[k] : b0[k];
@@ -1051,7 +1049,6 @@ struct DcDefectOp {
       const int j = k / L.nx, i = k - j * L.nx;
       const double r = residual64(L, p64, i, j, b);
       L.rhs[k] = (float)r;
-      if (FIRST && r64) r64[k] = r;
       if (H) a_.ph0[k] = __float2half(0.f);
       else L.p[k] = K(0);
       v[0] += r * r;
@@ -1075,14 +1072,13 @@ __global__ __launch_bounds__(256) void k_dc_defect(const DcDefectOp::Args a_) {
   DcDefectOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// One workgroup per scene after dc_defect or kry_step: folds their partials
-// (||r||^2, ||b||^2) and ends the residual evaluation: ||r|| / ||b||, the
-// convergence flag, the FP16 smoother's scale and the cycle count.  Krylov
-// (krylov = 1): the first evaluation also starts the iteration (||b||^2, the
-// residual as the lowest so far, no stored direction); a later one that
-// follows a Krylov step divides by the saved ||b||^2 and, when the residual
-// is above twice the lowest of the solve so far, switches the rest of the
-// solve to defect correction.
+// One workgroup per scene after dc_defect, resnorm or the Krylov start and
+// step kernels: folds their partials (||r||^2, ||b||^2) and ends the residual
+// evaluation: ||r|| / ||b||, the convergence flag, the FP16 smoother's scale
+// and the cycle count.  Krylov (krylov = 1) divides by the ||b||^2 saved at
+// the start; the start's residual is the lowest so far, and after a step one
+// above twice the lowest of the solve switches the rest of the solve to
+// defect correction.
 struct ResidFoldOp {
   struct Args {
     Dyn* d;
@@ -1100,19 +1096,18 @@ struct ResidFoldOp {
     double v[2];
     fold_partials<2>(v, a_.part, a_.vg);
     if (threadIdx.x != 0) return;
-    if (a_.krylov && !a_.first && !d->kfall) {
+    if (a_.krylov && a_.first) {
+      d->kbest = resid_done(d, v[0], d->kbb, a_.ncell, a_.tol, false);
+      d->kfall = 0;
+      return;
+    }
+    if (a_.krylov && !d->kfall) {
       const double r = resid_done(d, v[0], d->kbb, a_.ncell, a_.tol, true);
       if (r > 2.0 * d->kbest) d->kfall = 1;
       if (r < d->kbest) d->kbest = r;
       return;
     }
-    const double r = resid_done(d, v[0], v[1], a_.ncell, a_.tol, !a_.first);
-    if (a_.first && a_.krylov) {
-      d->kbb = v[1];
-      d->kbest = r;
-      d->kdq = 0;
-      d->kfall = 0;
-    }
+    resid_done(d, v[0], v[1], a_.ncell, a_.tol, !a_.first);
   }
 };
 __global__ __launch_bounds__(256) void k_resid_fold(const ResidFoldOp::Args a_) { ResidFoldOp::run(a_, 0); }
@@ -1155,15 +1150,17 @@ __device__ __forceinline__ double kry_z(const Level& L, const __half* __restrict
 }
 
 // Flexible conjugate gradients with one stored direction (Notay's FCG(1)),
-// the V-cycle's z = M r as the preconditioned residual:
-//   beta = (z, B d) / (d, B d),  d <- z - beta d,  q <- B z - beta q (= B d),
-//   alpha = (r, z) / (d, B d),   p64 += alpha d,   r -= alpha q,
-// where (d, B d) of the new direction expands to (z, B z) - beta (z, q_old),
-// so one pass over z gives all three dot products: kry_dots forms w = B z
-// and their partials, kry_coef folds them into alpha and beta, kry_step
-// updates the vectors and leaves the partials of ||r||^2 for resid_fold.
-// After a fallback to defect correction kry_dots adds e to p64 and kry_step
-// forms r = b - A p64 instead.
+// the V-cycle's z = M r as the preconditioned residual, the residual
+// recomputed from the pressure each iteration (dc_defect) rather than
+// carried:
+//   beta = (B z, d) / (d, B d)   (= (z, B d): B is symmetric),
+//   d <- z - beta d,  alpha = (r, z) / (d, B d),  p64 += alpha d,
+//   r = b - B p64,
+// where (d, B d) of the new direction expands to (z, B z) - beta (B z, d_old),
+// so one pass over z (kry_dots) gives the three dot products; r enters them
+// as the FP32 right-hand side the V-cycle solved for.  kry_coef folds them
+// into alpha and beta; after a fallback to defect correction it sets
+// alpha = 1, beta = 0 (p64 += z) without them.
 struct KryDotsOp {
   struct Args {
     Level L;
@@ -1177,30 +1174,23 @@ struct KryDotsOp {
   template <bool H>
   static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double (&v)[3]) {
     const Level& L = a_.L;
-    const double* __restrict__ q64 = a_.k.q64;
-    const double* __restrict__ r64 = a_.k.r64;
-    double* __restrict__ w64 = a_.k.w64;
+    const double* __restrict__ d64 = a_.k.d64;
     const auto zat = [&](int q) { return kry_z<H>(L, a_.ph0, is, q); };
     const int n = L.nx * L.ny;
     for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
       const int j = k / L.nx, i = k - j * L.nx;
-      const double z = zat(k), q = q64[k], r = r64[k];
-      double bz;
-      if (!apply64(L, zat, i, j, z, bz)) bz = 0.0;
-      w64[k] = bz;
-      v[0] += z * q;
+      const double z = zat(k), dk = d64[k], r = (double)L.rhs[k];
+      double w;
+      if (!apply64(L, zat, i, j, z, w)) w = 0.0;
+      v[0] += w * dk;
       v[1] += r * z;
-      v[2] += z * bz;
+      v[2] += z * w;
     }
   }
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     Dyn* d = a_.d;
-    if (d->converged) return;
-    if (d->kfall) {  // defect correction: p64 += z
-      DcAccumOp::run(DcAccumOp::Args{a_.L, a_.ph0, d, a_.k.p64, &d->converged, a_.vg}, vb);
-      return;
-    }
-    double v[3] = {0, 0, 0};  // (z, q), (r, z), (z, B z)
+    if (d->converged || d->kfall) return;
+    double v[3] = {0, 0, 0};  // (B z, d), (r, z), (z, B z)
     if (a_.ph0) sweep<true>(a_, vb, d->isig, v);
     else sweep<false>(a_, vb, 1.f, v);
     store_partials<3>(v, a_.part, vb);
@@ -1220,7 +1210,11 @@ struct KryCoefOp {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const Args& a_, int) {
     Dyn* d = a_.d;
-    if (d->converged || d->kfall) return;
+    if (d->converged) return;
+    if (d->kfall) {
+      if (threadIdx.x == 0) { d->kalpha = 1.0; d->kbeta = 0.0; }
+      return;
+    }
     double v[3];
     fold_partials<3>(v, a_.part, a_.vg);
     if (threadIdx.x != 0) return;
@@ -1233,57 +1227,118 @@ struct KryCoefOp {
 };
 __global__ __launch_bounds__(256) void k_kry_coef(const KryCoefOp::Args a_) { KryCoefOp::run(a_, 0); }
 
-struct KryStepOp {
+// d <- z - beta d, p64 += alpha d (own cell only; dc_defect follows)
+struct KryUpdateOp {
   struct Args {
     Level L;
-    __half* ph0;
+    const __half* ph0;
+    Dyn* d;
+    KryBufs k;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  template <bool H>
+  static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double alpha, double beta) {
+    const Level& L = a_.L;
+    double* __restrict__ p64 = a_.k.p64;
+    double* __restrict__ d64 = a_.k.d64;
+    const int n = L.nx * L.ny;
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      const double dn = kry_z<H>(L, a_.ph0, is, k) - beta * d64[k];
+      const double x = p64[k] + alpha * dn;
+      d64[k] = dn;
+      p64[k] = x;
+    }
+  }
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    Dyn* d = a_.d;
+    if (d->converged) return;
+    if (a_.ph0) sweep<true>(a_, vb, d->isig, d->kalpha, d->kbeta);
+    else sweep<false>(a_, vb, 1.f, d->kalpha, d->kbeta);
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_update(const KryUpdateOp::Args a_) {
+  KryUpdateOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Krylov start from the previous step's pressure p (in p64): a line search
+// along p from zero.  kry_start saves b and stores d = p with the partials
+// of (p, b), (p, B p), (b, b); kry_start_coef folds them into
+// a = (p, b) / (p, B p) ((p, B p) = 0: a = 0 and no stored direction);
+// kry_scale sets x = a p; dc_defect then forms r = b - B x.
+struct KryStartOp {
+  struct Args {
+    Level L;
     Dyn* d;
     KryBufs k;
     double* part;
     int vg;
   };
   static constexpr int kThreads = 256;
-  template <bool H>
-  static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double alpha, double beta,
-                                               double (&v)[2]) {
-    const Level& L = a_.L;
-    double* __restrict__ p64 = a_.k.p64;
-    double* __restrict__ r64 = a_.k.r64;
-    double* __restrict__ d64 = a_.k.d64;
-    double* __restrict__ q64 = a_.k.q64;
-    const double* __restrict__ w64 = a_.k.w64;
-    const int n = L.nx * L.ny;
-    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
-      const double z = kry_z<H>(L, a_.ph0, is, k);
-      const double dn = z - beta * d64[k];
-      const double qn = w64[k] - beta * q64[k];
-      const double x = p64[k] + alpha * dn;
-      const double r = r64[k] - alpha * qn;
-      p64[k] = x;
-      d64[k] = dn;
-      q64[k] = qn;
-      r64[k] = r;
-      L.rhs[k] = (float)r;
-      if (H) a_.ph0[k] = __float2half(0.f);
-      else L.p[k] = K(0);
-      v[0] += r * r;
-    }
-  }
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
-    Dyn* d = a_.d;
-    if (d->converged) return;
-    if (d->kfall) {  // defect correction: r = b - A p64
-      DcDefectOp::run(DcDefectOp::Args{a_.L, a_.ph0, a_.k.p64, a_.k.b0, nullptr, d, a_.part, 0, a_.vg}, vb);
-      return;
+    const Level& L = a_.L;
+    const double* __restrict__ p64 = a_.k.p64;
+    double* __restrict__ d64 = a_.k.d64;
+    float* __restrict__ b0 = a_.k.b0;
+    const auto pat = [&](int q) { return p64[q]; };
+    const int n = L.nx * L.ny;
+    double v[3] = {0, 0, 0};  // (p, b), (p, B p), (b, b)
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      const int j = k / L.nx, i = k - j * L.nx;
+      const float bf = L.rhs[k];
+      const double b = (double)bf, p = p64[k];
+      double bp;
+      if (!apply64(L, pat, i, j, p, bp)) bp = 0.0;
+      b0[k] = bf;
+      d64[k] = p;
+      v[0] += p * b;
+      v[1] += p * bp;
+      v[2] += b * b;
     }
-    double v[2] = {0, 0};  // ||r||^2 (the fold divides by the saved ||b||^2)
-    if (a_.ph0) sweep<true>(a_, vb, d->isig, d->kalpha, d->kbeta, v);
-    else sweep<false>(a_, vb, 1.f, d->kalpha, d->kbeta, v);
-    store_partials<2>(v, a_.part, vb);
+    store_partials<3>(v, a_.part, vb);
   }
 };
-__global__ __launch_bounds__(256) void k_kry_step(const KryStepOp::Args a_) {
-  KryStepOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+__global__ __launch_bounds__(256) void k_kry_start(const KryStartOp::Args a_) {
+  KryStartOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+struct KryStartCoefOp {
+  struct Args {
+    Dyn* d;
+    const double* part;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int) {
+    Dyn* d = a_.d;
+    double v[3];
+    fold_partials<3>(v, a_.part, a_.vg);
+    if (threadIdx.x != 0) return;
+    d->kalpha = v[1] != 0 ? v[0] / v[1] : 0.0;
+    d->kdq = v[1];
+    d->kbb = v[2];
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_start_coef(const KryStartCoefOp::Args a_) { KryStartCoefOp::run(a_, 0); }
+
+struct KryScaleOp {
+  struct Args {
+    Level L;
+    Dyn* d;
+    KryBufs k;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    double* __restrict__ p64 = a_.k.p64;
+    const double* __restrict__ d64 = a_.k.d64;
+    const double a = a_.d->kalpha;
+    const int n = a_.L.nx * a_.L.ny;
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) p64[k] = a * d64[k];
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_scale(const KryScaleOp::Args a_) {
+  KryScaleOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 __global__ __launch_bounds__(256) void k_p32_to_64(const float* __restrict__ p, double* __restrict__ p64, int n) {
@@ -1790,23 +1845,42 @@ void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s)
   if (b > 1024) b = 1024;
   launch(k_cfl, dim3(b), dim3(256), 0, s, CflOp::Args{F, cfl, umin, nsub, (int)b});
 }
-void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, double* r64, Dyn* dyn, double* part,
-                      float tol, bool first, hipStream_t s) {
+void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
+                      bool first, hipStream_t s) {
   const int nb = (int)resnorm_blocks(L0);
-  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, p64, b0, r64, dyn, part, first ? 1 : 0, nb});
+  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, p64, b0, dyn, part, first ? 1 : 0, nb});
   launch(k_resid_fold, dim3(1), dim3(256), 0, s,
-         ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, first ? 1 : 0, r64 ? 1 : 0});
+         ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, first ? 1 : 0, 0});
 }
-void launch_kry_dots(const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k, double* part, hipStream_t s) {
+void launch_kry_start(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
+                      hipStream_t s) {
   const int nb = (int)resnorm_blocks(L0);
-  launch(k_kry_dots, dim3(nb), dim3(256), 0, s, KryDotsOp::Args{L0, ph0, dyn, k, part, nb});
-  launch(k_kry_coef, dim3(1), dim3(256), 0, s, KryCoefOp::Args{dyn, part, nb});
+  launch(k_kry_start, dim3(nb), dim3(256), 0, s, KryStartOp::Args{L0, dyn, k, part, nb});
+  launch(k_kry_start_coef, dim3(1), dim3(256), 0, s, KryStartCoefOp::Args{dyn, part, nb});
+  launch(k_kry_scale, dim3(nb), dim3(256), 0, s, KryScaleOp::Args{L0, dyn, k, nb});
+  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb});
+  launch(k_resid_fold, dim3(1), dim3(256), 0, s, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 1, 1});
 }
 void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
                      hipStream_t s) {
   const int nb = (int)resnorm_blocks(L0);
-  launch(k_kry_step, dim3(nb), dim3(256), 0, s, KryStepOp::Args{L0, ph0, dyn, k, part, nb});
+  launch(k_kry_dots, dim3(nb), dim3(256), 0, s, KryDotsOp::Args{L0, ph0, dyn, k, part, nb});
+  launch(k_kry_coef, dim3(1), dim3(256), 0, s, KryCoefOp::Args{dyn, part, nb});
+  launch(k_kry_update, dim3(nb), dim3(256), 0, s, KryUpdateOp::Args{L0, ph0, dyn, k, nb});
+  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb});
   launch(k_resid_fold, dim3(1), dim3(256), 0, s, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 1});
+}
+__global__ __launch_bounds__(256) void k_all_converged(Dyn* const* __restrict__ d, int n, int* __restrict__ out) {
+  __shared__ int open;
+  if (threadIdx.x == 0) open = 0;
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += 256)
+    if (!d[q]->converged) atomicOr(&open, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = open ? 0 : 1;
+}
+void launch_all_converged(Dyn* const* d, int n, int* out, hipStream_t s) {
+  launch(k_all_converged, dim3(1), dim3(256), 0, s, d, n, out);
 }
 void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s) {
   const int nb = (int)nblk(L.nx + 1, L.ny + 1);
@@ -1870,24 +1944,31 @@ void phase_resnorm(BatchPhase& ph, BatchPhase& fold, const Level& L0, Dyn* dyn, 
                          "resid_fold");
 }
 void phase_dc_defect(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, double* p64, float* b0,
-                     double* r64, Dyn* dyn, double* part, float tol, bool first) {
+                     Dyn* dyn, double* part, float tol, bool first) {
   const int nb = (int)resnorm_blocks(L0);
-  phase_add<DcDefectOp>(ph, DcDefectOp::Args{L0, ph0, p64, b0, r64, dyn, part, first ? 1 : 0, nb}, nb, dim3(256),
+  phase_add<DcDefectOp>(ph, DcDefectOp::Args{L0, ph0, p64, b0, dyn, part, first ? 1 : 0, nb}, nb, dim3(256),
                         "dc_defect");
-  phase_add<ResidFoldOp>(fold, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, first ? 1 : 0, r64 ? 1 : 0}, 1,
-                         dim3(256), "resid_fold");
+  phase_add<ResidFoldOp>(fold, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, first ? 1 : 0, 0}, 1, dim3(256),
+                         "resid_fold");
 }
-void phase_kry_dots(BatchPhase& ph, BatchPhase& fold, const Level& L0, const __half* ph0, Dyn* dyn, const KryBufs& k,
-                    double* part) {
+void phase_kry_start(BatchPhase (&ph)[5], const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part,
+                     float tol) {
   const int nb = (int)resnorm_blocks(L0);
-  phase_add<KryDotsOp>(ph, KryDotsOp::Args{L0, ph0, dyn, k, part, nb}, nb, dim3(256), "kry_dots");
-  phase_add<KryCoefOp>(fold, KryCoefOp::Args{dyn, part, nb}, 1, dim3(256), "kry_coef");
+  phase_add<KryStartOp>(ph[0], KryStartOp::Args{L0, dyn, k, part, nb}, nb, dim3(256), "kry_start");
+  phase_add<KryStartCoefOp>(ph[1], KryStartCoefOp::Args{dyn, part, nb}, 1, dim3(256), "kry_start_coef");
+  phase_add<KryScaleOp>(ph[2], KryScaleOp::Args{L0, dyn, k, nb}, nb, dim3(256), "kry_scale");
+  phase_add<DcDefectOp>(ph[3], DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb}, nb, dim3(256), "dc_defect");
+  phase_add<ResidFoldOp>(ph[4], ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 1, 1}, 1, dim3(256),
+                         "resid_fold");
 }
-void phase_kry_step(BatchPhase& ph, BatchPhase& fold, const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k,
-                    double* part, float tol) {
+void phase_kry_step(BatchPhase (&ph)[5], const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part,
+                    float tol) {
   const int nb = (int)resnorm_blocks(L0);
-  phase_add<KryStepOp>(ph, KryStepOp::Args{L0, ph0, dyn, k, part, nb}, nb, dim3(256), "kry_step");
-  phase_add<ResidFoldOp>(fold, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 1}, 1, dim3(256),
+  phase_add<KryDotsOp>(ph[0], KryDotsOp::Args{L0, ph0, dyn, k, part, nb}, nb, dim3(256), "kry_dots");
+  phase_add<KryCoefOp>(ph[1], KryCoefOp::Args{dyn, part, nb}, 1, dim3(256), "kry_coef");
+  phase_add<KryUpdateOp>(ph[2], KryUpdateOp::Args{L0, ph0, dyn, k, nb}, nb, dim3(256), "kry_update");
+  phase_add<DcDefectOp>(ph[3], DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb}, nb, dim3(256), "dc_defect");
+  phase_add<ResidFoldOp>(ph[4], ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 1}, 1, dim3(256),
                          "resid_fold");
 }
 void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip) {

@@ -72,6 +72,17 @@ struct ParamUpload {
   float dsub, dt;
 };
 
+// graphs of a residual-controlled step in segments (see run_segments)
+static const int SEG_HEAD = 4, SEG_CHUNK = 2;
+struct Segments {
+  hipGraphExec_t head = nullptr, chunk = nullptr, rem = nullptr, tail = nullptr;
+  int h = 0, r = 0;  // cycles in the head and in the remainder chunk
+  void drop() {
+    for (hipGraphExec_t* g : {&head, &chunk, &rem, &tail})
+      if (*g) { hipGraphExecDestroy(*g); *g = nullptr; }
+  }
+};
+
 struct cf_sim {
   int dev = 0;
   hipStream_t st = nullptr;
@@ -97,7 +108,7 @@ struct cf_sim {
   double* dNorm = nullptr;   // residual-norm partials
   double* dP64 = nullptr;    // level-0 pressure in FP64 (defect correction)
   float* dB0 = nullptr;      // level-0 right-hand side kept across correction cycles
-  double *dR64 = nullptr, *dD64 = nullptr, *dQ64 = nullptr, *dW64 = nullptr;  // Krylov: r, d, B d, B z
+  double* dD64 = nullptr;    // Krylov: the stored direction
   double* dBand = nullptr;   // direct coarsest solve: Cholesky band
   // upload rings
   ParamUpload* hP = nullptr;
@@ -117,6 +128,8 @@ struct cf_sim {
   // graph of one step
   hipGraphExec_t gexec = nullptr;   // one step
   hipGraphExec_t gexec8 = nullptr;  // eight steps: one replay per 8 steps amortises the launch
+  Segments seg;                     // residual-controlled steps: the step in segments
+  int* hflag = nullptr;             // pinned host copy of the convergence flag
   int gkey[11] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   bool use_graph = true;
   bool mg_simple = false;  // one launch per reference dispatch pair (debug comparisons)
@@ -212,34 +225,47 @@ static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef)
 // relative residual norm; once the target is met the remaining launches
 // return at once, so the sequence is fixed and a graph replays it.  (The
 // per-dispatch path of CF_MG_SIMPLE has no skip and runs every cycle.)
-static void enqueue_mg(cf_sim* s, bool coarse_coef) {
+// enqueue_mg_pre: what precedes the cycles (the FP16 coefficient copies,
+// the control reset, the first defect); enqueue_mg_cycle: cycle c and the
+// update and residual evaluation that follow it.
+static __half* mg_ph0(cf_sim* s) { return s->sch.smoother_fp16 ? s->LH[0].p : nullptr; }
+static void enqueue_mg_pre(cf_sim* s) {
   const bool tol = s->sch.tol > 0;
   const bool dc = tol && s->sch.refine;  // defect correction with the FP64 pressure
-  const int* skip = tol ? &s->dDyn->converged : nullptr;
   // the FP16 smoother (defect correction only) reads FP16 copies of the coefficients
-  __half* ph0 = s->sch.smoother_fp16 ? s->LH[0].p : nullptr;
+  __half* ph0 = mg_ph0(s);
   if (ph0) for (int l = 0; l < s->mg_bottom; l++) { cf::launch_coef_half(s->L[l], s->LH[l], s->st); mark(s, "coef_half"); }
   if (tol) { cf::launch_fill_u32(&s->dDyn->converged, 0u, 2, s->st); mark(s, "mg_control"); }  // converged, cycles
-  const bool kry = dc && s->sch.krylov;
-  const cf::KryBufs kb{s->dP64, s->dR64, s->dD64, s->dQ64, s->dW64, s->dB0};
-  if (dc) {
-    cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, kry ? s->dR64 : nullptr, s->dDyn, s->dNorm, s->sch.tol, true,
-                         s->st);
+  if (dc && s->sch.krylov) {
+    const cf::KryBufs kb{s->dP64, s->dD64, s->dB0};
+    cf::launch_kry_start(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st);
+    mark(s, "kry_start");
+  } else if (dc) {
+    cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, true, s->st);
     mark(s, "dc_defect");
   }
-  for (int c = 0; c < s->sch.vcycles; c++) {
-    enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0);
-    if (kry) {
-      cf::launch_kry_dots(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->st); mark(s, "kry_dots");
-      cf::launch_kry_step(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st); mark(s, "kry_step");
-    } else if (dc) {
-      cf::launch_dc_accum(s->L[0], ph0, s->dDyn, s->dP64, skip, s->st); mark(s, "dc_accum");
-      cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, nullptr, s->dDyn, s->dNorm, s->sch.tol, false, s->st);
-      mark(s, "dc_defect");
-    } else if (tol) {
-      cf::launch_resnorm(s->L[0], s->dDyn, s->dNorm, s->sch.tol, s->st); mark(s, "resnorm");
-    }
+}
+static void enqueue_mg_cycle(cf_sim* s, int c, bool coarse_coef) {
+  const bool tol = s->sch.tol > 0;
+  const bool dc = tol && s->sch.refine;
+  const bool kry = dc && s->sch.krylov;
+  const int* skip = tol ? &s->dDyn->converged : nullptr;
+  __half* ph0 = mg_ph0(s);
+  const cf::KryBufs kb{s->dP64, s->dD64, s->dB0};
+  enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0);
+  if (kry) {
+    cf::launch_kry_step(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st); mark(s, "kry_step");
+  } else if (dc) {
+    cf::launch_dc_accum(s->L[0], ph0, s->dDyn, s->dP64, skip, s->st); mark(s, "dc_accum");
+    cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, false, s->st);
+    mark(s, "dc_defect");
+  } else if (tol) {
+    cf::launch_resnorm(s->L[0], s->dDyn, s->dNorm, s->sch.tol, s->st); mark(s, "resnorm");
   }
+}
+static void enqueue_mg(cf_sim* s, bool coarse_coef) {
+  enqueue_mg_pre(s);
+  for (int c = 0; c < s->sch.vcycles; c++) enqueue_mg_cycle(s, c, coarse_coef);
 }
 
 static void enqueue_coef(cf_sim* s) {
@@ -291,8 +317,10 @@ static int enqueue_stage(cf_sim* s, int stage) {
 // one step (js/fluid.js:225-253); the last diffusion substep runs inside
 // sources, forces through the coefficient cascade run as one pass which also
 // folds the accumulator partials, and the coarsest level's coefficients are
-// restricted inside the bottom solver
-static void enqueue_step(cf_sim* s) {
+// restricted inside the bottom solver.  Split at the V-cycles:
+// enqueue_step_pre runs up to them (returning the first cycle's coarse_coef),
+// enqueue_mg_cycle each cycle, enqueue_step_post the projection.
+static bool enqueue_step_pre(cf_sim* s) {
   Fields& F = s->F;
   const bool fused = !s->mg_simple;
   // CFL time step from the current velocities, on the device
@@ -310,38 +338,102 @@ static void enqueue_step(cf_sim* s) {
     Yin = F.Yh;
   }
   cf::launch_sources(F, true, Tin, Yin, s->dAccPart, !fused, s->st); mark(s, "diffuse_sources");
+  bool coarse_coef = false;
   if (!fused) {
     enqueue_stage(s, 3);
     enqueue_stage(s, 4);
-    enqueue_stage(s, 5);
   } else {
     cf::launch_forces_coef(F, s->L, s->nlev, s->dAccPart, (int)cf::sources_blocks(F), s->st); mark(s, "forces_coef");
     const int last_coef = s->nlev - 1 - (s->mg_bottom < s->nlev - 1 ? 1 : 0);
     for (int l = 1 + cf::forces_coef_cascade(s->nlev); l <= last_coef; l++) {
       cf::launch_coef_restrict(s->L[l - 1], s->L[l], s->st); mark(s, level_tag("coef_restrict", l));
     }
-    const bool coarse_coef = last_coef < s->nlev - 1 && s->nlev - 1 > cf::forces_coef_cascade(s->nlev);
     // the coarsest coefficients are restricted once, in the first V-cycle's bottom solve
-    enqueue_mg(s, coarse_coef);
+    coarse_coef = last_coef < s->nlev - 1 && s->nlev - 1 > cf::forces_coef_cascade(s->nlev);
   }
-  enqueue_stage(s, 6);
+  enqueue_mg_pre(s);
+  return coarse_coef;
+}
+static void enqueue_step_post(cf_sim* s) { enqueue_stage(s, 6); }
+static void enqueue_step(cf_sim* s) {
+  const bool coarse_coef = enqueue_step_pre(s);
+  for (int c = 0; c < s->sch.vcycles; c++) enqueue_mg_cycle(s, c, coarse_coef);
+  enqueue_step_post(s);
+}
+
+// ---- early exit of the residual-controlled cycles ------------------------------
+// A graph is a fixed sequence, so a captured step with a residual target
+// launches all `vcycles` cycles, and the ones after convergence return at
+// once but still cost a launch each (a few microseconds per kernel, which
+// adds up to more than the converged work on small scenes).  With a target
+// the step is captured in segments instead: the head (up to and including
+// the first SEG_HEAD cycles), a chunk of SEG_CHUNK cycles that is replayed
+// while any scene has not converged and cycles remain, a remainder chunk for
+// a cap that is not a whole number of chunks, and the tail (projection).
+// The host reads the convergence flag after the head and after each chunk.
+static void seg_plan(int vcycles, int* h, int* r) {
+  *h = vcycles < SEG_HEAD ? vcycles : SEG_HEAD;
+  *r = (vcycles - *h) % SEG_CHUNK;
 }
 
 static void drop_graph(cf_sim* s) {
   if (s->gexec) hipGraphExecDestroy(s->gexec);
   if (s->gexec8) hipGraphExecDestroy(s->gexec8);
   s->gexec = s->gexec8 = nullptr;
+  s->seg.drop();
   for (int q = 0; q < 11; q++) s->gkey[q] = -1;
 }
 
-static int capture(cf_sim* s, int nsteps, hipGraphExec_t* out) {
+// captures what fn enqueues on stream st
+template <class Fn>
+static int capture_fn(hipStream_t st, hipGraphExec_t* out, Fn fn) {
   hipGraph_t g = nullptr;
-  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-  for (int q = 0; q < nsteps; q++) enqueue_step(s);
-  HIPCHK(hipStreamEndCapture(s->st, &g));
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  fn();
+  HIPCHK(hipStreamEndCapture(st, &g));
   hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) { *out = nullptr; return set_err(CF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+  return CF_OK;
+}
+
+static int capture(cf_sim* s, int nsteps, hipGraphExec_t* out) {
+  return capture_fn(s->st, out, [&] { for (int q = 0; q < nsteps; q++) enqueue_step(s); });
+}
+
+// residual-controlled steps on the fused path run in segments
+static bool seg_mode(const cf_sim* s) { return s->sch.tol > 0 && !s->mg_simple; }
+
+static int capture_segments(cf_sim* s) {
+  Segments& g = s->seg;
+  seg_plan(s->sch.vcycles, &g.h, &g.r);
+  bool cc = false;
+  int rc = capture_fn(s->st, &g.head, [&] {
+    cc = enqueue_step_pre(s);
+    for (int c = 0; c < g.h; c++) enqueue_mg_cycle(s, c, cc);
+  });
+  if (!rc && s->sch.vcycles > g.h)
+    rc = capture_fn(s->st, &g.chunk, [&] { for (int c = 1; c <= SEG_CHUNK; c++) enqueue_mg_cycle(s, c, false); });
+  if (!rc && g.r > 0) rc = capture_fn(s->st, &g.rem, [&] { for (int c = 1; c <= g.r; c++) enqueue_mg_cycle(s, c, false); });
+  if (!rc) rc = capture_fn(s->st, &g.tail, [&] { enqueue_step_post(s); });
+  return rc;
+}
+
+// one step from the segments: the host waits for the convergence flag after
+// the head and after each chunk
+static int run_segments(cf_sim* s) {
+  Segments& g = s->seg;
+  HIPCHK(hipGraphLaunch(g.head, s->st));
+  int n = g.h;
+  while (n < s->sch.vcycles) {
+    HIPCHK(hipMemcpyAsync(s->hflag, &s->dDyn->converged, sizeof(int), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (*s->hflag) break;
+    const bool whole = s->sch.vcycles - n >= SEG_CHUNK;
+    HIPCHK(hipGraphLaunch(whole ? g.chunk : g.rem, s->st));
+    n += whole ? SEG_CHUNK : g.r;
+  }
+  HIPCHK(hipGraphLaunch(g.tail, s->st));
   return CF_OK;
 }
 
@@ -352,10 +444,15 @@ static int ensure_graph(cf_sim* s) {
   memcpy(&key[6], &s->sch.tol, 4);
   memcpy(&key[7], &s->sch.cfl, 4);
   memcpy(&key[8], &s->sch.umin, 4);
-  if (s->gexec && s->gexec8 && !memcmp(key, s->gkey, sizeof(key))) return CF_OK;
+  if ((s->gexec || s->seg.head) && !memcmp(key, s->gkey, sizeof(key))) return CF_OK;
   drop_graph(s);
-  int rc = capture(s, 1, &s->gexec);
-  if (!rc) rc = capture(s, 8, &s->gexec8);
+  int rc;
+  if (seg_mode(s)) {
+    rc = capture_segments(s);
+  } else {
+    rc = capture(s, 1, &s->gexec);
+    if (!rc) rc = capture(s, 8, &s->gexec8);
+  }
   if (rc) { drop_graph(s); return rc; }
   memcpy(s->gkey, key, sizeof(key));
   return CF_OK;
@@ -396,7 +493,7 @@ static bool lookup(cf_sim* s, const char* name, FieldRef& r) {
       {"SQ", F.SQ, nC * 8}, {"cellG", F.cellG, nC * 8}, {"fx", F.fx, nU * 4}, {"fy", F.fy, nV * 4},
       {"pieces", F.pieces, CF_MAX_PIECES * CF_PIECE_FLOATS * 4}, {"acc", F.acc, CF_ACC_LEN * 4},
       {"rad", F.rad, CF_RAD_LEN * 16}, {"faceW", F.faceW, CF_MAX_PIECES * 4 * 4},
-      {"p64", s->dP64, nC * 8}, {"r64", s->dR64, nC * 8}, {"d64", s->dD64, nC * 8}, {"q64", s->dQ64, nC * 8}};
+      {"p64", s->dP64, nC * 8}, {"d64", s->dD64, nC * 8}};
   for (auto& e : t)
     if (!strcmp(e.n, name)) { r.ptr = e.p; r.bytes = e.b; return true; }
   const char* pre[4] = {"rhs", "ax", "ay", "p"};
@@ -432,6 +529,12 @@ struct cf_pool {
   bool dirty = true;
   uint64_t key = 0, fkey = 0;
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
+  Segments seg;               // residual-controlled steps: the step in segments
+  std::vector<size_t> cyc;    // first phase of each V-cycle of `step`; the last entry: the projection's
+  cf::Dyn** dDyns = nullptr;  // the live scenes' control blocks, for the convergence check
+  int ndyn = 0;
+  int* dflag = nullptr;       // all scenes converged (device), and its pinned host copy
+  int* hflag = nullptr;
   bool use_graph = true;
   int nlaunch = 0;
   int64_t steps = 0;
@@ -469,6 +572,7 @@ static void pool_drop_graph(cf_pool* pl) {
   if (pl->g1) hipGraphExecDestroy(pl->g1);
   if (pl->g8) hipGraphExecDestroy(pl->g8);
   pl->g1 = pl->g8 = nullptr;
+  pl->seg.drop();
 }
 
 // the batched step of the live scenes, in the order of enqueue_step
@@ -528,7 +632,7 @@ static int pool_build_step(cf_pool* pl) {
   }
   // V-cycles; down leg d and up leg d hold level d of every scene that has one
   const bool tol = k.tol > 0, dc = tol && k.refine, kry = dc && k.krylov;
-  auto kb = [](cf_sim* s) { return cf::KryBufs{s->dP64, s->dR64, s->dD64, s->dQ64, s->dW64, s->dB0}; };
+  auto kb = [](cf_sim* s) { return cf::KryBufs{s->dP64, s->dD64, s->dB0}; };
   auto ph0 = [&](cf_sim* s) { return k.smoother_fp16 ? s->LH[0].p : (__half*)nullptr; };
   if (k.smoother_fp16) {
     for (int l = 0; l < MAXLEV; l++) {
@@ -545,11 +649,13 @@ static int pool_build_step(cf_pool* pl) {
     ph.push_back(std::move(a));
     ph.push_back(std::move(f));
   };
-  if (dc) {
+  if (kry) {
+    cf::BatchPhase a[5];
+    for (auto* s : sc) cf::phase_kry_start(a, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm, k.tol);
+    for (auto& x : a) ph.push_back(std::move(x));
+  } else if (dc) {
     cf::BatchPhase a, f;
-    for (auto* s : sc)
-      cf::phase_dc_defect(a, f, s->L[0], ph0(s), s->dP64, s->dB0, kry ? s->dR64 : nullptr, s->dDyn, s->dNorm, k.tol,
-                          true);
+    for (auto* s : sc) cf::phase_dc_defect(a, f, s->L[0], ph0(s), s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, true);
     pair(a, f);
   }
   int maxb = 0;
@@ -561,7 +667,9 @@ static int pool_build_step(cf_pool* pl) {
   auto leg_ok = [&](cf_sim* s, int d, int t, int m) {
     return d < s->mg_bottom && cf::mg_tile(s->L[d]) == t && mg_store(s, d).mode == m;
   };
+  pl->cyc.clear();
   for (int c = 0; c < k.vcycles; c++) {
+    pl->cyc.push_back(ph.size());
     for (int d = 0; d < maxb; d++) {
       for (int t : tiles)
         for (int m : modes) {
@@ -592,16 +700,9 @@ static int pool_build_step(cf_pool* pl) {
         }
     }
     if (kry) {
-      {
-        cf::BatchPhase a, f;
-        for (auto* s : sc) cf::phase_kry_dots(a, f, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm);
-        pair(a, f);
-      }
-      {
-        cf::BatchPhase a, f;
-        for (auto* s : sc) cf::phase_kry_step(a, f, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm, k.tol);
-        pair(a, f);
-      }
+      cf::BatchPhase a[5];
+      for (auto* s : sc) cf::phase_kry_step(a, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm, k.tol);
+      for (auto& x : a) ph.push_back(std::move(x));
     } else if (dc) {
       {
         cf::BatchPhase& a = P();
@@ -610,7 +711,7 @@ static int pool_build_step(cf_pool* pl) {
       {
         cf::BatchPhase a, f;
         for (auto* s : sc)
-          cf::phase_dc_defect(a, f, s->L[0], ph0(s), s->dP64, s->dB0, nullptr, s->dDyn, s->dNorm, k.tol, false);
+          cf::phase_dc_defect(a, f, s->L[0], ph0(s), s->dP64, s->dB0, s->dDyn, s->dNorm, k.tol, false);
         pair(a, f);
       }
     } else if (tol) {
@@ -619,9 +720,23 @@ static int pool_build_step(cf_pool* pl) {
       pair(a, f);
     }
   }
+  pl->cyc.push_back(ph.size());
   { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_project(a, s->F, dc ? s->dP64 : nullptr); }
   for (auto& a : ph)
     if (a.mixed) return set_err(CF_EINVAL, "pool: phase '%s' would run one kernel over scenes of another", a.tag);
+  {  // the control blocks the convergence check reads
+    std::vector<cf::Dyn*> dyns;
+    for (auto* s : sc) dyns.push_back(s->dDyn);
+    if (pl->dDyns) hipFree(pl->dDyns);
+    pl->dDyns = nullptr;
+    pl->ndyn = (int)dyns.size();
+    hipError_t e = hipMalloc((void**)&pl->dDyns, dyns.size() * sizeof(cf::Dyn*));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(pl->dDyns, dyns.data(), dyns.size() * sizeof(cf::Dyn*), hipMemcpyHostToDevice, pl->st);
+    if (e == hipSuccess && !pl->dflag) e = hipMalloc((void**)&pl->dflag, sizeof(int));
+    if (e == hipSuccess && !pl->hflag) e = hipHostMalloc((void**)&pl->hflag, sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables: %s", hipGetErrorString(e));
+  }
   for (auto& a : ph) {
     hipError_t e = a.upload(pl->st);
     if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables: %s", hipGetErrorString(e));
@@ -637,13 +752,43 @@ static void pool_enqueue_step(cf_pool* pl) {
 }
 
 static int pool_capture(cf_pool* pl, int n, hipGraphExec_t* out) {
-  hipGraph_t g = nullptr;
-  HIPCHK(hipStreamBeginCapture(pl->st, hipStreamCaptureModeThreadLocal));
-  for (int q = 0; q < n; q++) pool_enqueue_step(pl);
-  HIPCHK(hipStreamEndCapture(pl->st, &g));
-  hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
-  if (e != hipSuccess) return set_err(CF_EDEVICE, "pool graph: %s", hipGetErrorString(e));
+  return capture_fn(pl->st, out, [&] { for (int q = 0; q < n; q++) pool_enqueue_step(pl); });
+}
+
+// phases [a, b) of the step
+static void pool_enqueue_range(cf_pool* pl, size_t a, size_t b) {
+  for (size_t q = a; q < b; q++) pl->step[q].run(pl->st);
+}
+
+// the segments of a residual-controlled step (as capture_segments for one
+// scene): every cycle after the first has the same phases, so the chunks
+// replay cycles 1, 2, ...
+static int pool_capture_segments(cf_pool* pl) {
+  Segments& g = pl->seg;
+  const int V = pl->sch.vcycles;
+  seg_plan(V, &g.h, &g.r);
+  const std::vector<size_t>& cy = pl->cyc;
+  int rc = capture_fn(pl->st, &g.head, [&] { pool_enqueue_range(pl, 0, cy[g.h]); });
+  if (!rc && V > g.h) rc = capture_fn(pl->st, &g.chunk, [&] { pool_enqueue_range(pl, cy[1], cy[1 + SEG_CHUNK]); });
+  if (!rc && g.r > 0) rc = capture_fn(pl->st, &g.rem, [&] { pool_enqueue_range(pl, cy[1], cy[1 + g.r]); });
+  if (!rc) rc = capture_fn(pl->st, &g.tail, [&] { pool_enqueue_range(pl, cy[V], pl->step.size()); });
+  return rc;
+}
+
+static int pool_run_segments(cf_pool* pl) {
+  Segments& g = pl->seg;
+  HIPCHK(hipGraphLaunch(g.head, pl->st));
+  int n = g.h;
+  while (n < pl->sch.vcycles) {
+    cf::launch_all_converged(pl->dDyns, pl->ndyn, pl->dflag, pl->st);
+    HIPCHK(hipMemcpyAsync(pl->hflag, pl->dflag, sizeof(int), hipMemcpyDeviceToHost, pl->st));
+    HIPCHK(hipStreamSynchronize(pl->st));
+    if (*pl->hflag) break;
+    const bool whole = pl->sch.vcycles - n >= SEG_CHUNK;
+    HIPCHK(hipGraphLaunch(whole ? g.chunk : g.rem, pl->st));
+    n += whole ? SEG_CHUNK : g.r;
+  }
+  HIPCHK(hipGraphLaunch(g.tail, pl->st));
   return CF_OK;
 }
 
@@ -660,10 +805,15 @@ static int pool_ready(cf_pool* pl) {
     int rc = pool_build_step(pl);
     if (rc) return rc;
   }
-  if (pl->use_graph && !pl->g1) {
-    int rc = pool_capture(pl, 1, &pl->g1);
-    if (!rc) rc = pool_capture(pl, 8, &pl->g8);
-    if (rc) return rc;
+  if (pl->use_graph && !pl->g1 && !pl->seg.head) {
+    int rc;
+    if (pl->sch.tol > 0) {
+      rc = pool_capture_segments(pl);
+    } else {
+      rc = pool_capture(pl, 1, &pl->g1);
+      if (!rc) rc = pool_capture(pl, 8, &pl->g8);
+    }
+    if (rc) { pool_drop_graph(pl); return rc; }
   }
   return CF_OK;
 }
@@ -682,6 +832,7 @@ void cf_sim_destroy(cf_sim* s) {
   for (int q = 0; q < 3; q++) hipEventDestroy(s->ringEv[q]);
   if (s->hP) hipHostFree(s->hP);
   if (s->hPieces) hipHostFree(s->hPieces);
+  if (s->hflag) hipHostFree(s->hflag);
   if (s->ring) hipHostFree(s->ring);
   if (s->block) hipFree(s->block);
   if (s->st && s->own_stream) hipStreamDestroy(s->st);
@@ -745,8 +896,7 @@ static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
   add((void**)&s->dAccPart, (size_t)nparts * 16);
   add((void**)&s->dNorm, 4 * sizeof(double) * 1024);
   add((void**)&s->dP64, nC * sizeof(double));
-  add((void**)&s->dR64, nC * sizeof(double)); add((void**)&s->dD64, nC * sizeof(double));
-  add((void**)&s->dQ64, nC * sizeof(double)); add((void**)&s->dW64, nC * sizeof(double));
+  add((void**)&s->dD64, nC * sizeof(double));
   add((void**)&s->dB0, nC * sizeof(float));
   add((void**)&s->dBand, (256 * 33 + 256) * sizeof(double));
 }
@@ -827,7 +977,8 @@ static int sim_create(int nx, int ny, double dx, double dt, hipStream_t shared, 
   for (int q = 0; q < 3; q++) { hipEventCreateWithFlags(&s->ringEv[q], hipEventDisableTiming); s->busy[q] = false; }
   if (hipHostMalloc((void**)&s->hP, sizeof(ParamUpload) * UP_SLOTS, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&s->hPieces, sizeof(float) * CF_MAX_PIECES * CF_PIECE_FLOATS * UP_SLOTS, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&s->ring, RING_BYTES * 3, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&s->ring, RING_BYTES * 3, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->hflag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
     cf_sim_destroy(s);
     return set_err(CF_ENOMEM, "pinned host allocation failed");
   }
@@ -993,6 +1144,10 @@ int cf_sim_step(cf_sim* s, int nsteps) {  // js/fluid.js:225-253
     int rc = ensure_graph(s);
     if (rc) return rc;
     int q = 0;
+    if (s->seg.head) {
+      for (; q < nsteps; q++)
+        if ((rc = run_segments(s))) return rc;
+    }
     for (; q + 8 <= nsteps; q += 8) HIPCHK(hipGraphLaunch(s->gexec8, s->st));
     for (; q < nsteps; q++) HIPCHK(hipGraphLaunch(s->gexec, s->st));
   } else {
@@ -1092,8 +1247,14 @@ int cf_sim_step_latencies(cf_sim* s, int nsteps, float* ms) {
   }
   HIPCHK(hipEventRecord(ev[0], s->st));
   for (int q = 0; q < nsteps; q++) {
-    if (s->use_graph) HIPCHK(hipGraphLaunch(s->gexec, s->st));
-    else enqueue_step(s);
+    if (s->use_graph && s->seg.head) {
+      int rc = run_segments(s);
+      if (rc) return rc;
+    } else if (s->use_graph) {
+      HIPCHK(hipGraphLaunch(s->gexec, s->st));
+    } else {
+      enqueue_step(s);
+    }
     HIPCHK(hipEventRecord(ev[q + 1], s->st));
     if (s->sch.cfl <= 0) s->time += s->dt;
   }
@@ -1299,6 +1460,9 @@ void cf_pool_destroy(cf_pool* pl) {
   pool_release_phases(pl->step);
   pool_release_phases(pl->frame_pre);
   pool_release_phases(pl->frame_post);
+  if (pl->dDyns) hipFree(pl->dDyns);
+  if (pl->dflag) hipFree(pl->dflag);
+  if (pl->hflag) hipHostFree(pl->hflag);
   for (auto* s : pl->slots) if (s) cf_sim_destroy(s);
   hipStreamDestroy(pl->st);
   delete pl;
@@ -1367,7 +1531,10 @@ int cf_pool_step(cf_pool* pl, int nsteps) {
   int rc = pool_ready(pl);
   if (rc) return rc;
   int q = 0;
-  if (pl->use_graph) {
+  if (pl->use_graph && pl->seg.head) {
+    for (; q < nsteps; q++)
+      if ((rc = pool_run_segments(pl))) return rc;
+  } else if (pl->use_graph) {
     for (; q + 8 <= nsteps; q += 8) HIPCHK(hipGraphLaunch(pl->g8, pl->st));
     for (; q < nsteps; q++) HIPCHK(hipGraphLaunch(pl->g1, pl->st));
   } else {
@@ -1449,8 +1616,14 @@ int cf_pool_step_latencies(cf_pool* pl, int nsteps, float* ms) {
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], pl->st));
   for (int q = 0; q < nsteps; q++) {
-    if (pl->use_graph) HIPCHK(hipGraphLaunch(pl->g1, pl->st));
-    else pool_enqueue_step(pl);
+    if (pl->use_graph && pl->seg.head) {
+      rc = pool_run_segments(pl);
+      if (rc) return rc;
+    } else if (pl->use_graph) {
+      HIPCHK(hipGraphLaunch(pl->g1, pl->st));
+    } else {
+      pool_enqueue_step(pl);
+    }
     HIPCHK(hipEventRecord(ev[q + 1], pl->st));
   }
   HIPCHK(hipStreamSynchronize(pl->st));

@@ -272,9 +272,8 @@ class FluidSim:
 
     def read_p64(self, name="p64"):
         """The FP64 level-0 pressure of defect correction (refine = 1), or with
-        name "r64", "d64", "q64" the Krylov residual, direction and B times the
-        direction (krylov = 1)."""
-        if name not in ("p64", "r64", "d64", "q64"):
+        name "d64" the stored direction of the Krylov iteration (krylov = 1)."""
+        if name not in ("p64", "d64"):
             raise KeyError(name)
         a = np.empty(self.nx * self.ny, np.float64)
         check(L.lib().cf_sim_read_field(self._h, name.encode(), a.ctypes.data, a.nbytes), "read " + name)

@@ -106,7 +106,7 @@ typedef struct {
   REAL *p[MAXLEV], *rhs[MAXLEV], *ax[MAXLEV], *ay[MAXLEV];
   double* p64;            /* defect correction: level-0 pressure in FP64 */
   REAL* b0;               /* defect correction: level-0 right-hand side */
-  double *r64, *d64, *q64; /* Krylov: residual, direction, A times the direction */
+  double* d64;            /* Krylov: the stored direction */
   double kbb, kdq;        /* Krylov: ||b||^2, (d, A d) of the stored direction (0: none) */
   double kbest;           /* Krylov: lowest ||r|| / ||b|| of this solve */
   int kfall;              /* Krylov: fallen back to defect correction for this solve */
@@ -821,7 +821,6 @@ static double dc_defect(S* s, int first) {
       if (j > 0) { off += so * p[k - nx]; diag += so; }
       if (j < ny - 1) { off += n * p[k + nx]; diag += n; } else diag += 2.0 * n;
       const double r = diag <= 0 ? 0.0 : b - (off - diag * p[k]) / h2;
-      if (first && s->sch.krylov) s->r64[k] = r;
       s->rhs[0][k] = (REAL)r;
       s->p[0][k] = 0;
       rr += r * r; bb += b * b;
@@ -871,6 +870,16 @@ static double gpu_sum(const double* v, int n) {
 
 /* B z at level 0 for the Krylov iteration: (sum a z_nb - diag z) / h2 in FP64
  * with the relaxation's boundary terms, 0 at a closed cell */
+static int apply64_open(const S* s, int i, int j) {
+  const int nx = s->nx, ny = s->ny;
+  const REAL *ax = s->ax[0], *ay = s->ay[0];
+  double diag = 0;
+  diag += i > 0 ? (double)ax[j * (nx + 1) + i] : 2.0 * (double)ax[j * (nx + 1) + i];
+  diag += i < nx - 1 ? (double)ax[j * (nx + 1) + i + 1] : 2.0 * (double)ax[j * (nx + 1) + i + 1];
+  if (j > 0) diag += (double)ay[j * nx + i];
+  diag += j < ny - 1 ? (double)ay[(j + 1) * nx + i] : 2.0 * (double)ay[(j + 1) * nx + i];
+  return diag > 0;
+}
 static double apply64(const S* s, const double* z, int i, int j) {
   const int nx = s->nx, ny = s->ny;
   const REAL *ax = s->ax[0], *ay = s->ay[0];
@@ -886,73 +895,110 @@ static double apply64(const S* s, const double* z, int i, int j) {
   return (off - diag * z[k]) / (double)s->lh2[0];
 }
 
-/* Krylov start (schedule option krylov = 1, after dc_defect(s, 1) has left
- * the FP64 residual b - B p64 in r64): ||b||^2, no stored direction, the
- * starting residual as the lowest so far */
-static void kry_init(S* s) {
-  const int n = s->nx * s->ny;
+/* Krylov residual: r = b - B p64 from the saved b (the defect of
+ * dc_defect(s, 0)), into rhs[0] with e = 0 as the next V-cycle's start;
+ * ||r|| / ||b|| with the sums in the GPU's order */
+static double kry_defect(S* s) {
+  const int nx = s->nx, ny = s->ny, n = nx * ny;
   double* v = malloc(sizeof(double) * (size_t)n);
+  double* w = malloc(sizeof(double) * (size_t)n);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int k = j * nx + i;
+      const double b = (double)s->b0[k];
+      const double r = apply64_open(s, i, j) ? b - apply64(s, s->p64, i, j) : 0.0;
+      s->rhs[0][k] = (REAL)r;
+      s->p[0][k] = 0;
+      v[k] = r * r;
+      w[k] = b * b;
+    }
+  const double rr = gpu_sum(v, n), bb = gpu_sum(w, n);
+  set_sig(s, rr);
+  free(v);
+  free(w);
+  return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+}
+
+/* Krylov start (schedule option krylov = 1), from the previous step's
+ * pressure p: a line search along p from zero, x = a p with
+ * a = (p, b) / (p, B p), which keeps the previous pressure where it fits
+ * this step's right-hand side and scales it (or drops it, a = 0) where it
+ * does not; p stays as the stored direction (r = b - a B p is orthogonal to
+ * it), so the first FCG iteration is conjugated against it.  With p = 0 the
+ * start is x = 0, r = b and no stored direction.  Returns ||r|| / ||b||. */
+static double kry_start(S* s) {
+  const int nx = s->nx, ny = s->ny, n = nx * ny;
+  double* v = malloc(sizeof(double) * (size_t)n);
+  double* bp = malloc(sizeof(double) * (size_t)n);
+  for (int k = 0; k < n; k++) { s->b0[k] = s->rhs[0][k]; s->d64[k] = s->p64[k]; }
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) bp[j * nx + i] = apply64(s, s->d64, i, j);
+  for (int k = 0; k < n; k++) v[k] = s->d64[k] * (double)s->b0[k];
+  const double pb = gpu_sum(v, n);
+  for (int k = 0; k < n; k++) v[k] = s->d64[k] * bp[k];
+  const double pq = gpu_sum(v, n);
   for (int k = 0; k < n; k++) v[k] = (double)s->b0[k] * (double)s->b0[k];
   s->kbb = gpu_sum(v, n);
-  for (int k = 0; k < n; k++) v[k] = s->r64[k] * s->r64[k];
-  const double rr = gpu_sum(v, n);
-  s->kbest = s->kbb > 0 ? sqrt(rr / s->kbb) : sqrt(rr);
-  s->kdq = 0;
-  s->kfall = 0;
+  const double a = pq != 0 ? pb / pq : 0.0;
+  s->kdq = pq;
+  for (int k = 0; k < n; k++) s->p64[k] = a * s->d64[k];
   free(v);
+  free(bp);
+  const double rel = kry_defect(s);
+  s->kbest = rel;
+  s->kfall = 0;
+  return rel;
 }
 
 /* One flexible-CG iteration after the V-cycle has left z = M r in p[0]
- * (Notay's FCG with one stored direction): beta = (z, B d) / (d, B d),
- * d <- z - beta d, alpha = (r, z) / (d, B d), p64 += alpha d, r -= alpha B d.
- * B d is carried as q <- B z - beta q, and (d, B d) expands to
- * (z, B z) - beta (z, q), so the three dot products come from z alone.
- * Then r (FP32) is the next V-cycle's right-hand side with e = 0 as its
- * start; returns ||r|| / ||b||.
+ * (Notay's FCG with one stored direction), with the residual recomputed
+ * from the pressure each time rather than carried:
+ *   beta = (B z, d) / (d, B d)   (= (z, B d): B is symmetric),
+ *   d <- z - beta d,  alpha = (r, z) / (d, B d),  p64 += alpha d,
+ *   r = b - B p64,
+ * where (d, B d) of the new direction expands to (z, B z) - beta (B z, d_old),
+ * so one pass over z gives the three dot products; r enters them as the FP32
+ * right-hand side the V-cycle solved for.  Returns ||r|| / ||b||.
  * A residual above twice the lowest of the solve so far ends the Krylov
  * iteration for this solve: the later cycles are plain defect correction
- * (p64 += z, r = b - B p64) from where it stands.  A singular level-0 system
- * (open cells closed off from every Dirichlet side) with a right-hand side
- * outside its range makes CG diverge once the compatible part is solved,
- * while defect correction stalls at the incompatible part. */
+ * (alpha = 1, beta = 0: p64 += z) from where it stands.  A singular level-0
+ * system (open cells closed off from every Dirichlet side) with a
+ * right-hand side outside its range makes CG diverge once the compatible
+ * part is solved, while defect correction stalls at the incompatible part. */
 static double kry_step(S* s) {
   const int nx = s->nx, ny = s->ny, n = nx * ny;
-  if (s->kfall) {
-    for (int k = 0; k < n; k++) s->p64[k] += (double)s->p[0][k];
-    return dc_defect(s, 0);
-  }
+  double alpha = 1.0, beta = 0.0;
   double* z = malloc(sizeof(double) * (size_t)n);
-  double* bz = malloc(sizeof(double) * (size_t)n);
-  double* v = malloc(sizeof(double) * (size_t)n);
   for (int k = 0; k < n; k++) z[k] = (double)s->p[0][k];
-  for (int j = 0; j < ny; j++)
-    for (int i = 0; i < nx; i++) bz[j * nx + i] = apply64(s, z, i, j);
-  for (int k = 0; k < n; k++) v[k] = z[k] * s->q64[k];
-  const double zq = gpu_sum(v, n);
-  for (int k = 0; k < n; k++) v[k] = s->r64[k] * z[k];
-  const double rz = gpu_sum(v, n);
-  for (int k = 0; k < n; k++) v[k] = z[k] * bz[k];
-  const double zbz = gpu_sum(v, n);
-  const double beta = s->kdq != 0 ? zq / s->kdq : 0.0;
-  const double dq = zbz - beta * zq;
-  const double alpha = dq != 0 ? rz / dq : 0.0;
-  s->kdq = dq;
+  if (!s->kfall) {
+    double* w = malloc(sizeof(double) * (size_t)n);
+    double* v = malloc(sizeof(double) * (size_t)n);
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) w[j * nx + i] = apply64(s, z, i, j);
+    for (int k = 0; k < n; k++) v[k] = w[k] * s->d64[k];
+    const double wd = gpu_sum(v, n);
+    for (int k = 0; k < n; k++) v[k] = (double)s->rhs[0][k] * z[k];
+    const double rz = gpu_sum(v, n);
+    for (int k = 0; k < n; k++) v[k] = z[k] * w[k];
+    const double zw = gpu_sum(v, n);
+    beta = s->kdq != 0 ? wd / s->kdq : 0.0;
+    const double dq = zw - beta * wd;
+    alpha = dq != 0 ? rz / dq : 0.0;
+    s->kdq = dq;
+    free(w);
+    free(v);
+  }
   for (int k = 0; k < n; k++) {
     const double d = z[k] - beta * s->d64[k];
-    const double q = bz[k] - beta * s->q64[k];
+    s->d64[k] = d;
     s->p64[k] += alpha * d;
-    const double r = s->r64[k] - alpha * q;
-    s->d64[k] = d; s->q64[k] = q; s->r64[k] = r;
-    s->rhs[0][k] = (REAL)r;
-    s->p[0][k] = 0;
-    v[k] = r * r;
   }
-  const double rr = gpu_sum(v, n);
-  set_sig(s, rr);
-  free(z); free(bz); free(v);
-  const double rel = s->kbb > 0 ? sqrt(rr / s->kbb) : sqrt(rr);
-  if (rel > 2.0 * s->kbest) s->kfall = 1;
-  if (rel < s->kbest) s->kbest = rel;
+  free(z);
+  const double rel = kry_defect(s);
+  if (!s->kfall) {
+    if (rel > 2.0 * s->kbest) s->kfall = 1;
+    if (rel < s->kbest) s->kbest = rel;
+  }
   return rel;
 }
 
@@ -1303,7 +1349,7 @@ void* PFX(create)(int nx, int ny, double dx, double dt, int* err) {             
   s->V = calloc(nV, sizeof(REAL)); s->Vh = calloc(nV, sizeof(REAL)); s->V2 = calloc(nV, sizeof(REAL));
   s->T = calloc(nC, sizeof(REAL)); s->Th = calloc(nC, sizeof(REAL)); s->T2 = calloc(nC, sizeof(REAL));
   s->p64 = calloc(nC, sizeof(double)); s->b0 = calloc(nC, sizeof(REAL));
-  s->r64 = calloc(nC, sizeof(double)); s->d64 = calloc(nC, sizeof(double)); s->q64 = calloc(nC, sizeof(double));
+  s->d64 = calloc(nC, sizeof(double));
   s->Y = calloc(4 * nC, sizeof(REAL)); s->Yh = calloc(4 * nC, sizeof(REAL)); s->Y2 = calloc(4 * nC, sizeof(REAL));
   s->SQ = calloc(2 * nC, sizeof(REAL));
   s->sdf = calloc(nC, sizeof(REAL)); s->sid = calloc(nC, sizeof(uint32_t));
@@ -1338,7 +1384,7 @@ void PFX(destroy)(void* h) {
   if (!s) return;
   free(s->U); free(s->Uh); free(s->U2); free(s->V); free(s->Vh); free(s->V2);
   free(s->T); free(s->Th); free(s->T2); free(s->Y); free(s->Yh); free(s->Y2);
-  free(s->p64); free(s->b0); free(s->r64); free(s->d64); free(s->q64);
+  free(s->p64); free(s->b0); free(s->d64);
   free(s->SQ); free(s->sdf); free(s->sid); free(s->fx); free(s->fy);
   for (int l = 0; l < s->nlev; l++) { free(s->p[l]); free(s->rhs[l]); free(s->ax[l]); free(s->ay[l]); }
   free(s);
@@ -1472,8 +1518,7 @@ int PFX(run_stage)(void* h, int stage) {
       int nv = 0;
       double r = -1;
       if (s->sch.tol > 0 && s->sch.refine) {
-        r = dc_defect(s, 1);
-        if (s->sch.krylov) kry_init(s);
+        r = s->sch.krylov ? kry_start(s) : dc_defect(s, 1);
         for (int c = 0; c < s->sch.vcycles && !(r < s->sch.tol); c++) {
           vcycle(s, s->sch.nu);
           nv++;
@@ -1620,8 +1665,7 @@ int PFX(read_field64)(void* h, const char* name, double* dst, size_t count) {
   S* s = (S*)h;
   if (!strcmp(name, "sdf")) { if (count != (size_t)s->nC) return CF_EINVAL; for (int k = 0; k < s->nC; k++) dst[k] = s->sdf[k]; return 0; }
   if (!strcmp(name, "rad")) { if (count != CF_RAD_LEN * 4) return CF_EINVAL; for (int q = 0; q < CF_RAD_LEN * 4; q++) dst[q] = s->rad[q]; return 0; }
-  const double* src = !strcmp(name, "p64") ? s->p64 : !strcmp(name, "r64") ? s->r64 : !strcmp(name, "d64") ? s->d64
-                    : !strcmp(name, "q64") ? s->q64 : NULL;
+  const double* src = !strcmp(name, "p64") ? s->p64 : !strcmp(name, "d64") ? s->d64 : NULL;
   if (src) { if (count != (size_t)s->nC) return CF_EINVAL; memcpy(dst, src, sizeof(double) * s->nC); return 0; }
   size_t c; int lev;
   REAL* f = field(s, name, &c, &lev);
@@ -1654,6 +1698,11 @@ int PFX(write_field)(void* h, const char* name, const void* src, size_t bytes) {
 }
 int PFX(write_field64)(void* h, const char* name, const double* src, size_t count) {
   S* s = (S*)h;
+  if (!strcmp(name, "p64")) {
+    if (count != (size_t)s->nC) return CF_EINVAL;
+    memcpy(s->p64, src, sizeof(double) * s->nC);
+    return 0;
+  }
   size_t c; int lev;
   REAL* f = field(s, name, &c, &lev);
   if (!f || count != c) return CF_EINVAL;

@@ -316,9 +316,10 @@ def test_direct_coarsest_vcycle_matches_oracle(nx, ny):
 @pytest.mark.parametrize("nx,ny,fp16", [(128, 128, 0), (256, 192, 1), (512, 512, 1)])
 def test_krylov_matches_oracle(nx, ny, fp16):
     """Flexible CG at level 0 with the V-cycle as preconditioner against the
-    oracle's restatement: three iterations from the same start (pressure,
-    residual, direction and its image in FP64; the dot products sum in the
-    same fixed order on both sides), then a solve towards 1e-6.  The random
+    oracle's restatement: three iterations from the same start (the FP64
+    pressure and direction, the residual left for the next V-cycle; the dot
+    products sum in the same fixed order on both sides), then a solve towards
+    1e-6.  The random
     pile of the 256x192 and 512x512 cases encloses open cells, so their
     systems are singular: there the solve falls back to defect correction and
     stalls at the incompatible part of the right-hand side, on both sides."""
@@ -335,10 +336,13 @@ def test_krylov_matches_oracle(nx, ny, fp16):
     g.run_stage(5)
     o.run_stage(5)
     assert g.info().last_vcycles == o.last_vcycles == 3
-    for name in ("p64", "r64", "d64", "q64"):
+    for name in ("p64", "d64"):
         r = rel_l2(g.read_p64(name), o.read64(name))
         _flush("  3 iterations: %s rel_l2=%.3g" % (name, r))
         assert r <= 1e-12, name
+    r = rel_l2(g.read_field("rhs"), o.read("rhs"))
+    _flush("  3 iterations: residual rel_l2=%.3g" % r)
+    assert r <= 1e-12
     assert abs(g.info().last_resid - o.last_resid) <= 1e-4 * o.last_resid
     for q in (g, o):
         q.set_schedule(nu=2, vcycles=40, tol=1e-6, coarse_iters=24, refine=1, smoother_fp16=fp16, krylov=1)

@@ -730,13 +730,13 @@ def test_krylov_reaches_the_direct_solve_in_fewer_cycles(n, fp16):
 
 
 def test_krylov_iteration_identities():
-    """After k = 1..4 iterations (cycle cap k): q = B d (the carried image of
-    the direction), r = b - B p64 (the carried residual), (r, d) = 0 (exact
-    line search along d) and (d_k, B d_{k-1}) = 0 (each direction conjugate to
-    the one before).  A wrong sign or a dropped term in the direction, step or
-    residual updates breaks one of them."""
+    """After k = 1..4 iterations (cycle cap k): the right-hand side left for
+    the next V-cycle is the FP32 rounding of b - B p64 (the residual from the
+    pressure), (r, d) = 0 (exact line search along d), and (d_k, B d_{k-1}) = 0
+    (each direction conjugate to the one before).  A wrong sign or a dropped
+    term in the direction, step or residual breaks one of them."""
     n = 96
-    prev_q = None
+    prev_d = None
     for k in range(1, 5):
         o = _projection_case(n, n, fp64=False)
         o.run_stage(4)
@@ -744,15 +744,15 @@ def test_krylov_iteration_identities():
         o.set_schedule(nu=2, vcycles=k, tol=1e-30, coarse_iters=24, refine=1, krylov=1)
         o.run_stage(5)
         assert o.last_vcycles == k
-        r, d, q, x = o.read64("r64"), o.read64("d64"), o.read64("q64"), o.read64("p64")
-        nb = np.linalg.norm(b)
-        assert np.linalg.norm(q - _apply_B(o, d)) <= 1e-12 * np.linalg.norm(q), k
-        assert np.linalg.norm(r - (b - _apply_B(o, x))) <= 1e-11 * nb, k
-        assert abs(r @ d) <= 1e-9 * np.linalg.norm(r) * np.linalg.norm(d), k
-        if prev_q is not None:
-            assert abs(d @ prev_q) <= 1e-9 * np.linalg.norm(d) * np.linalg.norm(prev_q), k
-        prev_q = q
-        assert np.isclose(o.last_resid, np.linalg.norm(r) / nb, rtol=1e-9)
+        r, d, x = o.read64("rhs"), o.read64("d64"), o.read64("p64")
+        nb, nr = np.linalg.norm(b), np.linalg.norm(r)
+        assert np.linalg.norm(r - (b - _apply_B(o, x))) <= 1e-7 * nr + 1e-12 * nb, k
+        assert abs(r @ d) <= 1e-5 * nr * np.linalg.norm(d), k
+        if prev_d is not None:
+            bq = _apply_B(o, prev_d)
+            assert abs(d @ bq) <= 1e-9 * np.linalg.norm(d) * np.linalg.norm(bq), k
+        prev_d = d
+        assert np.isclose(o.last_resid, nr / nb, rtol=1e-6)
         o.close()
 
 
@@ -805,3 +805,44 @@ def test_krylov_falls_back_on_a_singular_system():
     (r_dc, p_dc), (r_kr, p_kr) = out
     assert 1e-5 < r_dc < 1e-2, r_dc
     assert r_kr <= 1.5 * r_dc and p_kr <= 2 * p_dc, out
+
+
+def test_krylov_start_rescales_the_previous_pressure():
+    """The Krylov start is the line search along the previous pressure p from
+    zero: when p is a multiple of this step's solution, x = (p, b)/(p, B p) p
+    is the solution itself, so the solve ends before any V-cycle; when p is
+    zero it starts from x = 0 with r = b."""
+    n = 64
+    o = _projection_case(n, n, fp64=False)
+    o.run_stage(4)
+    ref = _poisson_direct(o).reshape(-1)
+    o.set_schedule(nu=2, vcycles=10, tol=1e-6, coarse_iters=24, refine=1, krylov=1)
+    o.write64("p64", 3.7 * ref)
+    assert o.run_stage(5) == 0
+    assert o.last_vcycles == 0 and o.last_resid < 1e-9, (o.last_vcycles, o.last_resid)
+    assert np.max(np.abs(o.read64("p64") - ref)) < 1e-9 * np.max(np.abs(ref))
+    o2 = _projection_case(n, n, fp64=False)
+    o2.run_stage(4)
+    b = o2.read64("rhs")
+    o2.set_schedule(nu=2, vcycles=1, tol=1e-30, coarse_iters=24, refine=1, krylov=1)
+    o2.write64("p64", np.zeros(n * n))
+    o2.run_stage(5)
+    # one iteration from x = 0: the step is alpha z with z the V-cycle of b,
+    # so the new residual is b - alpha B z, orthogonal to z = d
+    r, d = o2.read64("rhs"), o2.read64("d64")
+    assert abs(r @ d) <= 1e-5 * np.linalg.norm(r) * np.linalg.norm(d)
+    assert np.linalg.norm(r - (b - _apply_B(o2, o2.read64("p64")))) <= 1e-7 * np.linalg.norm(r)
+    # from a previous pressure that is not a multiple of the solution, the
+    # first direction is conjugate to it: (d_1, B p) = 0
+    o3 = _projection_case(n, n, fp64=False)
+    o3.run_stage(4)
+    prev = ref + np.random.default_rng(7).standard_normal(n * n) * np.std(ref)
+    o3.set_schedule(nu=2, vcycles=1, tol=1e-30, coarse_iters=24, refine=1, krylov=1)
+    o3.write64("p64", prev)
+    o3.run_stage(5)
+    bp = _apply_B(o3, prev)
+    d1 = o3.read64("d64")
+    assert abs(d1 @ bp) <= 1e-9 * np.linalg.norm(d1) * np.linalg.norm(bp)
+    o.close()
+    o2.close()
+    o3.close()
