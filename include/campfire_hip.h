@@ -155,7 +155,10 @@ int  cf_sim_upload_pieces(cf_sim* sim, const float* pieces, int n);
 int  cf_sim_clear_geometry(cf_sim* sim);
 /* encodeGeometry(pass)  (js/fluid.js:215-218): cellG, fx, fy, faceW. */
 int  cf_sim_geometry(cf_sim* sim);
-/* nsteps x encodeStep(pass)  (js/fluid.js:225-253); time += dt each step. */
+/* nsteps x encodeStep(pass)  (js/fluid.js:225-253); time += dt each step.
+ * With a residual target (schedule tol > 0) each step waits on the host for
+ * the convergence flag between its V-cycles, so the call returns after the
+ * last step's V-cycles have run (the projection may still be in flight). */
 int  cf_sim_step(cf_sim* sim, int nsteps);
 /* encodeRadiation(pass)  (js/fluid.js:220-222). */
 int  cf_sim_radiation(cf_sim* sim);
@@ -252,7 +255,8 @@ int  cf_pool_admit(cf_pool* pool, const cf_scene_desc* desc, int* slot);
 int  cf_pool_scene(cf_pool* pool, int slot, cf_sim** sim);
 /* Frees a slot and its device memory. */
 int  cf_pool_release(cf_pool* pool, int slot);
-/* Every live scene advances nsteps steps (asynchronous). */
+/* Every live scene advances nsteps steps (asynchronous; with a residual
+ * target the host waits between V-cycles as in cf_sim_step). */
 int  cf_pool_step(cf_pool* pool, int nsteps);
 /* One app frame for every scene: parameters, geometry, nsteps steps, radiation. */
 int  cf_pool_frame(cf_pool* pool, int nsteps, int with_radiation);

@@ -35,6 +35,11 @@ SCHEDULES = {
     "fp32_dc_direct_coarse": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 0, "refine": 1},
     "fp16_dc_direct_coarse": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 0, "refine": 1,
                               "smoother_fp16": 1},
+    # the V-cycle as the preconditioner of flexible CG: the same target with
+    # the cheap coarsest sweeps
+    "fp32_fcg_120sweeps": {"nu": 2, "vcycles": 80, "tol": 1e-6, "coarse_iters": 120, "refine": 1, "krylov": 1},
+    "fp16_fcg_24sweeps": {"nu": 3, "vcycles": 80, "tol": 1e-6, "coarse_iters": 24, "refine": 1, "smoother_fp16": 1,
+                          "krylov": 1},
 }
 
 

@@ -35,3 +35,12 @@ def test_precision_study_1024():
     for k in ("U", "V", "T", "Y_F", "Y_O", "Y_P", "Y_S", "p", "S"):
         assert e["fp16_dc_direct_coarse"][k] <= 2 * e["fp32_dc_direct_coarse"][k] + 1e-6, k
     assert res["compat"]["fp16_dc_direct_coarse"]["rel_gap"] < 1e-6
+    # flexible CG with the V-cycle as preconditioner meets the same target
+    # with the cheap coarsest sweeps, in fewer cycles than defect correction
+    # with the direct coarsest solve
+    for name in ("fp32_fcg_120sweeps", "fp16_fcg_24sweeps"):
+        assert res["resid"][name] < 1e-6, name
+        assert res["vcycles"][name] < res["vcycles"]["fp32_dc_direct_coarse"], name
+        for k in ("U", "V", "T", "Y_F", "Y_O", "Y_P", "Y_S", "p", "S"):
+            assert e[name][k] <= 2 * e["fp32_dc_direct_coarse"][k] + 1e-6, (name, k)
+        assert res["compat"][name]["rel_gap"] < 1e-6, name
