@@ -390,10 +390,15 @@ def main():
     ap.add_argument("--fcg-coarse", type=int, default=24, help="coarsest sweeps of the batched FCG form (0: direct)")
     ap.add_argument("--conv-steps", type=int, default=6,
                     help="steps of the equal-accuracy batch comparison (0: skip)")
+    ap.add_argument("--converged-only", action="store_true",
+                    help="run only the equal-accuracy batch comparison and print its JSON (for profiling)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
+        return
+    if args.converged_only:
+        print(json.dumps(converged_batch(args)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build runs one scene per GPU on a single GPU; --gpus %d is not supported "
