@@ -58,13 +58,21 @@ def test_cfl_and_tolerance_match_the_oracle():
     o.close()
 
 
-def test_1000_steps_converge_and_stay_finite():
+# the workload's own schedule (FCG off), the V-cycle as FCG preconditioner with
+# the same smoother, and FCG with the FP16 smoother at nu 3, 24 coarsest sweeps
+VARIANTS = {"defect": {}, "fcg": {"krylov": 1},
+            "fcg_fp16": {"krylov": 1, "smoother_fp16": 1, "nu": 3, "coarse_iters": 24}}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_1000_steps_converge_and_stay_finite(variant):
     from campfire_amd.fluid import FluidSim
     cfg = scenes.config_plain_256(1)
+    cfg["schedule"] = dict(cfg["schedule"], **VARIANTS[variant])
     g = FluidSim({"nx": cfg["nx"], "ny": cfg["ny"], "dt": cfg["dt"]}, dx=cfg["dx"])
     scenes.apply(g, cfg)
     g.clear_geometry(); g.encode_geometry()
-    _flush("[plain256] 1000 steps")
+    _flush("[plain256] 1000 steps, %s schedule" % variant)
     g.step(1)
     g.synchronize()
     cycles, resid, dts = [], [], []

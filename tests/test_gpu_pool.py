@@ -137,3 +137,48 @@ def test_mixed_batch_of_64_runs():
         T = pool.scene(slot).read_field("T")
         assert np.all(np.isfinite(T))
     pool.close()
+
+
+def test_converged_batch_of_64_scenes_with_krylov_100_steps():
+    """The equal-accuracy batch of the benchmark (64 mixed scenes, FP16
+    smoother, the V-cycle as FCG preconditioner, ||r||/||rhs|| < 1e-4 at most
+    40 cycles) for 100 steps after the benchmark's 10 warm-up steps: every
+    scene-step meets the target, the fields stay finite and the fires burn.
+    (The warm-up is not checked: in the first step three piles enclose
+    pockets whose heating makes the level-0 system singular with an
+    incompatible right-hand side; defect correction and FCG both stall there
+    at ~1e-4, in the oracle as on the GPU.)"""
+    from campfire_amd.pool import ScenePool
+    cfgs = scenes.config_batch(1, 64)
+    pool = ScenePool(64 << 30)
+    pool.set_schedule(nu=3, vcycles=40, tol=1e-4, coarse_iters=24, refine=1, smoother_fp16=1, krylov=1)
+    views = []
+    for c in cfgs:
+        slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
+        scenes.apply(sim, c)
+        views.append(sim)
+    pool.frame(8, True)
+    pool.step(2)
+    worst, cyc = 0.0, []
+    for step in range(100):
+        if step % 10 == 0:
+            pool.frame(1, True)   # geometry, one step, radiation
+        else:
+            pool.step(1)
+        pool.synchronize()
+        infos = [v.info() for v in views]
+        worst = max(worst, max(i.last_resid for i in infos))
+        cyc.append(max(i.last_vcycles for i in infos))
+        bad = [(q, cfgs[q]["nx"], i.last_resid, i.last_vcycles) for q, i in enumerate(infos)
+               if not (i.last_resid < 1e-4 and i.last_vcycles < 40)]
+        if bad:
+            _flush("  step %d: %d scenes miss the target: %s" % (step, len(bad), bad[:8]))
+        assert not bad, step
+    cyc = cyc[::10]
+    _flush("[pool] 100 steps of 64 scenes with FCG: worst residual %.2e, most cycles every 10th step %s"
+           % (worst, cyc))
+    for v in views[::9]:
+        for name in ("T", "U", "V", "p"):
+            assert np.all(np.isfinite(v.read_field(name))), name
+    assert max(float(v.read_field("T").max()) for v in views) > 1000
+    pool.close()
