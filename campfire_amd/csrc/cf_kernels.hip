@@ -1561,121 +1561,154 @@ __device__ __forceinline__ float hitBox(float ox, float oy, float dxr, float dyr
   else face = ldy < K(0) ? K(1) : K(3);
   return tn;
 }
-__device__ __forceinline__ void gasAt(const cf_params& P, const float* __restrict__ T, const float4* __restrict__ Y,
-                                      float px, float py, float& g0, float& g1) {
-  const int i = toint(px / P.dx), j = toint(py / P.dx);
-  if (i < 0 || j < 0 || i >= P.nx || j >= P.ny) { g0 = K(0); g1 = K(0); return; }
-  const int k = j * P.nx + i;
-  const float Tg = T[k];
-  const float4 Yg = Y[k];
+// the gas terms of one nearest-cell sample with T and Y of that cell
+__device__ __forceinline__ void gasTerms(const cf_params& P, float Tg, float4 Yg, float& g0, float& g1) {
   const float fv = rhoOf(P, Tg) * Yg.w / P.sootDensity;
   const float kap = P.kappaSootC * fv * Tg + P.kappaGas * Yg.z;
   const float T2 = Tg * Tg;
   g0 = kap * P.sigma * (T2 * T2 - powf(P.Tamb, K(4.0))) / CF_PI;
   g1 = K(6.3) * fv / K(580e-9) * planck580(Tg);
 }
+// One workgroup per target, 64 rays (:64-158).  Each ray's march is split
+// over RG lanes: lane g takes the samples m = g, g + RG, ... of every
+// segment, and the lanes' partial integrals are added by a fixed tree
+// (shuffles over the group).  The per-ray terms then meet in LDS and one wave
+// adds them over the rays in the reference's workgroup tree order.  With
+// RG = 8 a target is 8 waves, so a scene's 4·nPieces + 64 targets fill the
+// chip's SIMDs instead of one wave each.
 struct RadiationOp {
   struct Args {
     Fields F;
     int vg;  // tiles of this scene's launch
   };
-  static constexpr int kThreads = 64;
+  static constexpr int RG = 8;
+  static constexpr int kThreads = 64 * RG;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Fields& F = a_.F;
-
-  const cf_params P = *F.P;
-  const int tgt = vb;
-  const int li = threadIdx.x;
-  const int nFaces = min(P.nPieces, CF_MAX_PIECES) * 4;
-  float ox = 0, oy = 0, nrx = 0, nry = 1;
-  bool valid = false;
-  if (tgt < nFaces) {
-    const float* pc = F.pieces + (tgt / 4) * CF_PIECE_FLOATS;
-    const int f = tgt % 4;
-    if (pc[7] > K(0)) {
-      const float c = cosf(pc[2]), s = sinf(pc[2]);
-      float lnx = 1, lny = 0;
-      if (f == 1) { lnx = 0; lny = 1; } else if (f == 2) { lnx = -1; lny = 0; } else if (f == 3) { lnx = 0; lny = -1; }
-      nrx = c * lnx - s * lny; nry = s * lnx + c * lny;
-      const float tgx = -nry, tgy = nrx;
-      const float u = (fractf_((float)li * K(0.618034) + (float)P.frame * K(0.1234)) - K(0.5)) * K(1.6) * pc[3];
-      const float hh = pc[3] + K(1e-4);
-      ox = pc[0] + nrx * hh + tgx * u;
-      oy = pc[1] + nry * hh + tgy * u;
-      valid = true;
-    }
-  } else {
-    const int fi = tgt - nFaces;
-    if (fi < CF_N_FLOOR) {
-      ox = ((float)fi + K(0.5)) / (float)CF_N_FLOOR * (float)P.nx * P.dx;
-      oy = K(1e-4);
-      nrx = 0; nry = 1;
-      valid = true;
-    }
-  }
-  float dq = K(0), dv = K(0);
-  if (valid) {
-    const float jit = fractf_((float)P.frame * K(0.7548777) + (float)tgt * K(0.5698403));
-    const float a = (((float)li + jit) / K(64) - K(0.5)) * CF_PI;
-    const float cosn = cosf(a);
-    const float base = atan2f(nry, nrx);
-    float dxr = cosf(base + a), dyr = sinf(base + a);
-    float J = K(0), Jv = K(0);
-    float Twall = P.Tamb;
-    bool bounced = false;
-    const float ds = P.dx;
-    const float W = (float)P.nx * P.dx, H = (float)P.ny * P.dx;
-    for (int step = 0; step < 4096; step++) {
-      float tHit = K(1e9), hitT = K(0);
-      for (int n = 0; n < min(P.nPieces, CF_MAX_PIECES); n++) {
-        const float* pc = F.pieces + n * CF_PIECE_FLOATS;
-        if (!(pc[7] > K(0))) continue;
-        float face;
-        const float hx = hitBox(ox, oy, dxr, dyr, pc, face);
-        if (hx > K(0) && hx < tHit) { tHit = hx; hitT = pc[8 + (int)face]; }
+    __shared__ float sq[64], sv[64];
+    const cf_params P = *F.P;
+    const int tgt = vb;
+    const int li = (int)threadIdx.x / RG;  // the ray
+    const int g = (int)threadIdx.x % RG;   // its lane in the ray's group
+    const int nFaces = min(P.nPieces, CF_MAX_PIECES) * 4;
+    float ox = 0, oy = 0, nrx = 0, nry = 1;
+    bool valid = false;
+    if (tgt < nFaces) {
+      const float* pc = F.pieces + (tgt / 4) * CF_PIECE_FLOATS;
+      const int f = tgt % 4;
+      if (pc[7] > K(0)) {
+        const float c = cosf(pc[2]), s = sinf(pc[2]);
+        float lnx = 1, lny = 0;
+        if (f == 1) { lnx = 0; lny = 1; } else if (f == 2) { lnx = -1; lny = 0; } else if (f == 3) { lnx = 0; lny = -1; }
+        nrx = c * lnx - s * lny; nry = s * lnx + c * lny;
+        const float tgx = -nry, tgy = nrx;
+        const float u = (fractf_((float)li * K(0.618034) + (float)P.frame * K(0.1234)) - K(0.5)) * K(1.6) * pc[3];
+        const float hh = pc[3] + K(1e-4);
+        ox = pc[0] + nrx * hh + tgx * u;
+        oy = pc[1] + nry * hh + tgy * u;
+        valid = true;
       }
-      float tFloor = K(1e9);
-      if (dyr < K(-1e-6)) tFloor = -oy / dyr;
-      const float tEnd = fminf(tHit, tFloor);
-      float tx = K(1e9), ty = K(1e9);
-      if (dxr > K(1e-6)) tx = (W - ox) / dxr; else if (dxr < K(-1e-6)) tx = -ox / dxr;
-      if (dyr > K(1e-6)) ty = (H - oy) / dyr;
-      const float tOut = fminf(fminf(tx, ty), tEnd);
-      const int ns = toint(ceilf(tOut / ds));
-      for (int m = 0; m < ns; m++) {
-        const float t = ((float)m + K(0.5)) * tOut / (float)ns;
-        float g0, g1;
-        gasAt(P, F.T, F.Y, ox + dxr * t, oy + dyr * t, g0, g1);
-        J += g0 * tOut / (float)ns;
-        Jv += g1 * tOut / (float)ns;
+    } else {
+      const int fi = tgt - nFaces;
+      if (fi < CF_N_FLOOR) {
+        ox = ((float)fi + K(0.5)) / (float)CF_N_FLOOR * (float)P.nx * P.dx;
+        oy = K(1e-4);
+        nrx = 0; nry = 1;
+        valid = true;
       }
-      if (tHit < tFloor && tHit < K(1e8)) { Twall = hitT; break; }
-      if (tFloor < K(1e8) && tFloor <= tHit && !bounced) {
-        ox = ox + dxr * tFloor + K(0);
-        oy = oy + dyr * tFloor + K(1e-5);
-        dyr = -dyr;
-        bounced = true;
-        continue;
-      }
-      Twall = P.Tamb;
-      break;
     }
-    const float w = CF_PI / K(64) * cosn;
-    const float T2 = Twall * Twall;
-    dq = w * (K(0.5) * P.sigma * T2 * T2 + K(2.0) * J);
-    dv = w * (K(2.0) * Jv);
-  }
-  dq = wave_tree_sum(dq);
-  dv = wave_tree_sum(dv);
-  if (li == 0 && valid) {
-    const float4 old = F.rad[tgt];
-    const float a = old.x <= K(0) ? K(1.0) : K(0.35);
-    F.rad[tgt] = make_float4(mixf(old.x, dq, a), mixf(old.y, dv, a), K(0), K(0));
-  }
-
+    float dq = K(0), dv = K(0);
+    if (valid) {  // uniform over the workgroup
+      const float jit = fractf_((float)P.frame * K(0.7548777) + (float)tgt * K(0.5698403));
+      const float a = (((float)li + jit) / K(64) - K(0.5)) * CF_PI;
+      const float cosn = cosf(a);
+      const float base = atan2f(nry, nrx);
+      float dxr = cosf(base + a), dyr = sinf(base + a);
+      float J = K(0), Jv = K(0);
+      float Twall = P.Tamb;
+      bool bounced = false;
+      const float ds = P.dx;
+      const float W = (float)P.nx * P.dx, H = (float)P.ny * P.dx;
+      for (int step = 0; step < 4096; step++) {
+        float tHit = K(1e9), hitT = K(0);
+        for (int n = 0; n < min(P.nPieces, CF_MAX_PIECES); n++) {
+          const float* pc = F.pieces + n * CF_PIECE_FLOATS;
+          if (!(pc[7] > K(0))) continue;
+          float face;
+          const float hx = hitBox(ox, oy, dxr, dyr, pc, face);
+          if (hx > K(0) && hx < tHit) { tHit = hx; hitT = pc[8 + (int)face]; }
+        }
+        float tFloor = K(1e9);
+        if (dyr < K(-1e-6)) tFloor = -oy / dyr;
+        const float tEnd = fminf(tHit, tFloor);
+        float tx = K(1e9), ty = K(1e9);
+        if (dxr > K(1e-6)) tx = (W - ox) / dxr; else if (dxr < K(-1e-6)) tx = -ox / dxr;
+        if (dyr > K(1e-6)) ty = (H - oy) / dyr;
+        const float tOut = fminf(fminf(tx, ty), tEnd);
+        const int ns = toint(ceilf(tOut / ds));
+        // gasAt (:53-62) at this lane's samples, RU at a time: every sample's
+        // T and Y are loaded before any is used (a sample outside the grid
+        // loads cell 0 and contributes 0)
+        constexpr int RU = 4;
+        for (int m0 = g; m0 < ns; m0 += RU * RG) {
+          float Tg[RU];
+          float4 Yg[RU];
+          bool in[RU];
+#pragma unroll
+          for (int u = 0; u < RU; u++) {
+            const int m = m0 + u * RG;
+            const float t = ((float)m + K(0.5)) * tOut / (float)ns;
+            const float px = ox + dxr * t, py = oy + dyr * t;
+            const int i = toint(px / P.dx), j = toint(py / P.dx);
+            in[u] = m < ns && !(i < 0 || j < 0 || i >= P.nx || j >= P.ny);
+            const int k = in[u] ? j * P.nx + i : 0;
+            Tg[u] = F.T[k];
+            Yg[u] = F.Y[k];
+          }
+#pragma unroll
+          for (int u = 0; u < RU; u++) {
+            float g0 = K(0), g1 = K(0);
+            if (in[u]) gasTerms(P, Tg[u], Yg[u], g0, g1);
+            J += g0 * tOut / (float)ns;
+            Jv += g1 * tOut / (float)ns;
+          }
+        }
+        if (tHit < tFloor && tHit < K(1e8)) { Twall = hitT; break; }
+        if (tFloor < K(1e8) && tFloor <= tHit && !bounced) {
+          ox = ox + dxr * tFloor + K(0);
+          oy = oy + dyr * tFloor + K(1e-5);
+          dyr = -dyr;
+          bounced = true;
+          continue;
+        }
+        Twall = P.Tamb;
+        break;
+      }
+      // the ray's integrals: the group's partials added by a fixed tree
+#pragma unroll
+      for (int o = RG / 2; o > 0; o >>= 1) {
+        J += __shfl_xor(J, o, RG);
+        Jv += __shfl_xor(Jv, o, RG);
+      }
+      const float w = CF_PI / K(64) * cosn;
+      const float T2 = Twall * Twall;
+      dq = w * (K(0.5) * P.sigma * T2 * T2 + K(2.0) * J);
+      dv = w * (K(2.0) * Jv);
+    }
+    if (g == 0) { sq[li] = dq; sv[li] = dv; }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      dq = wave_tree_sum(sq[threadIdx.x]);
+      dv = wave_tree_sum(sv[threadIdx.x]);
+      if (threadIdx.x == 0 && valid) {
+        const float4 old = F.rad[tgt];
+        const float am = old.x <= K(0) ? K(1.0) : K(0.35);
+        F.rad[tgt] = make_float4(mixf(old.x, dq, am), mixf(old.y, dv, am), K(0), K(0));
+      }
+    }
   }
 };
-__global__ __launch_bounds__(64) void k_radiation(const RadiationOp::Args a_) {
+__global__ __launch_bounds__(RadiationOp::kThreads) void k_radiation(const RadiationOp::Args a_) {
   RadiationOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
@@ -1837,7 +1870,8 @@ void launch_geometry(const Fields& F, hipStream_t s) {
   launch(k_geometry, dim3(nb), B328, 0, s, GeometryOp::Args{F, nb});
 }
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s) {
-  if (ntargets > 0) launch(k_radiation, dim3(ntargets), dim3(64), 0, s, RadiationOp::Args{F, ntargets});
+  if (ntargets > 0)
+    launch(k_radiation, dim3(ntargets), dim3(RadiationOp::kThreads), 0, s, RadiationOp::Args{F, ntargets});
 }
 void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s) {
   const int n = (F.nx + 1) * F.ny > F.nx * (F.ny + 1) ? (F.nx + 1) * F.ny : F.nx * (F.ny + 1);
@@ -1990,7 +2024,8 @@ void phase_geometry(BatchPhase& ph, const Fields& F) {
   phase_add<GeometryOp>(ph, GeometryOp::Args{F, nb}, nb, B328, "geometry");
 }
 void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets) {
-  if (ntargets > 0) phase_add<RadiationOp>(ph, RadiationOp::Args{F, ntargets}, ntargets, dim3(64), "radiation");
+  if (ntargets > 0)
+    phase_add<RadiationOp>(ph, RadiationOp::Args{F, ntargets}, ntargets, dim3(RadiationOp::kThreads), "radiation");
 }
 void phase_fill_u32(BatchPhase& ph, void* a, uint32_t v, size_t n) {
   int b = (int)((n + 255) / 256);
