@@ -1470,18 +1470,25 @@ __global__ __launch_bounds__(256) void k_project(const typename ProjectOp<P64>::
 // -------------------------------------------------------------------------
 // geometry: rasterise the pieces (shaders/geometry.wgsl:20-52, common.wgsl:47-62)
 // -------------------------------------------------------------------------
-__device__ __forceinline__ float nearest(const cf_params& P, const float* __restrict__ pieces, float px, float py,
-                                         uint32_t& bid) {
+// a piece's pose for the SDF: centre, cos and sin of the angle, half size,
+// and whether the piece is active (pieces[7] > 0)
+struct PiecePose {
+  float x, y, c, s, h;
+  int on;
+};
+// signed distance to the nearest active piece and its face id
+// (shaders/geometry.wgsl:20-29, the box SDF of common.wgsl:47-62), from the
+// poses computed once per workgroup
+__device__ __forceinline__ float nearest_pose(const PiecePose* __restrict__ pp, int np, float px, float py,
+                                              uint32_t& bid) {
   float best = K(1e9);
   bid = CF_SOLID_NONE;
-  const int np = min(P.nPieces, CF_MAX_PIECES);
   for (int n = 0; n < np; n++) {
-    const float* pc = pieces + n * CF_PIECE_FLOATS;
-    if (!(pc[7] > K(0))) continue;
-    const float c = cosf(pc[2]), s = sinf(pc[2]);
-    const float dxp = px - pc[0], dyp = py - pc[1];
-    const float lx = c * dxp + s * dyp, ly = -s * dxp + c * dyp;
-    const float qx = fabsf(lx) - pc[3], qy = fabsf(ly) - pc[3];
+    const PiecePose q = pp[n];
+    if (!q.on) continue;
+    const float dxp = px - q.x, dyp = py - q.y;
+    const float lx = q.c * dxp + q.s * dyp, ly = -q.s * dxp + q.c * dyp;
+    const float qx = fabsf(lx) - q.h, qy = fabsf(ly) - q.h;
     const float mxq = fmaxf(qx, K(0)), myq = fmaxf(qy, K(0));
     const float d = sqrtf(mxq * mxq + myq * myq) + fminf(fmaxf(qx, qy), K(0));
     if (d < best) {
@@ -1494,6 +1501,7 @@ __device__ __forceinline__ float nearest(const cf_params& P, const float* __rest
   }
   return best;
 }
+
 struct GeometryOp {
   struct Args {
     Fields F;
@@ -1505,12 +1513,23 @@ struct GeometryOp {
 
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
+  // the pieces' poses, once per workgroup: every thread evaluates each piece
+  // at three points, and the angle's cos and sin would dominate
+  __shared__ PiecePose pp[CF_MAX_PIECES];
+  const int np = min(P.nPieces, CF_MAX_PIECES);
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if (tid < np) {
+    const float* pc = F.pieces + tid * CF_PIECE_FLOATS;
+    const bool on = pc[7] > K(0);
+    pp[tid] = PiecePose{pc[0], pc[1], cosf(pc[2]), sinf(pc[2]), pc[3], on ? 1 : 0};
+  }
+  __syncthreads();
   int i, j;
   tile_ij<32, 8>(vb, nx + 1, ny + 1, i, j);
   const float dx = P.dx;
   uint32_t id;
   if (i < nx && j < ny) {
-    const float d = nearest(P, F.pieces, ((float)i + K(0.5)) * dx, ((float)j + K(0.5)) * dx, id);
+    const float d = nearest_pose(pp, np, ((float)i + K(0.5)) * dx, ((float)j + K(0.5)) * dx, id);
     F.cellG[j * nx + i] = make_float2(d, __uint_as_float(id));
     if (id != CF_SOLID_NONE && d >= K(0) && d < K(2.0) * dx) {
       const float w = fmaxf(K(0), K(1) - fabsf(d - dx) / dx) / dx;
@@ -1518,11 +1537,11 @@ struct GeometryOp {
     }
   }
   if (i <= nx && j < ny) {
-    const float d = nearest(P, F.pieces, (float)i * dx, ((float)j + K(0.5)) * dx, id);
+    const float d = nearest_pose(pp, np, (float)i * dx, ((float)j + K(0.5)) * dx, id);
     F.fx[j * (nx + 1) + i] = clampf(K(0.5) + d / dx, K(0), K(1));
   }
   if (i < nx && j <= ny) {
-    const float d = nearest(P, F.pieces, ((float)i + K(0.5)) * dx, (float)j * dx, id);
+    const float d = nearest_pose(pp, np, ((float)i + K(0.5)) * dx, (float)j * dx, id);
     F.fy[j * nx + i] = j == 0 ? K(0) : clampf(K(0.5) + d / dx, K(0), K(1));
   }
 
