@@ -341,6 +341,48 @@ def converged_batch(args):
     return res
 
 
+def plain_256(args, steps=300):
+    """The 256^2 plain-method workload (CFL dt on the device, V-cycles until
+    ||r||/||rhs|| < 1e-6 with the FP64 pressure) over `steps` steps after 20
+    warm-up steps: its own schedule (defect correction, nu 2, 120 coarsest
+    sweeps) against the V-cycle as FCG preconditioner (same smoother) and FCG
+    with the FP16 smoother (nu 3, 24 coarsest sweeps).  Every step's cycle
+    count and residual are recorded."""
+    from campfire_amd import scenes
+    from campfire_amd.fluid import FluidSim
+    variants = {"defect_correction": {}, "fcg": {"krylov": 1},
+                "fcg_fp16": {"krylov": 1, "smoother_fp16": 1, "nu": 3, "coarse_iters": 24}}
+    out = {}
+    for name, extra in variants.items():
+        cfg = scenes.config_plain_256(args.seed)
+        cfg["schedule"] = dict(cfg["schedule"], **extra)
+        g = FluidSim({"nx": cfg["nx"], "ny": cfg["ny"], "dt": cfg["dt"]}, dx=cfg["dx"])
+        scenes.apply(g, cfg)
+        g.clear_geometry()
+        g.encode_geometry()
+        g.step(20)
+        g.synchronize()
+        cyc, res = [], []
+        t = 0.0
+        for q in range(steps):
+            t0 = time.perf_counter()
+            g.step(1)
+            g.synchronize()
+            t += time.perf_counter() - t0
+            i = g.info()
+            cyc.append(int(i.last_vcycles))
+            res.append(float(i.last_resid))
+        g.close()
+        out[name] = {"mcell_steps_per_s": round(256 * 256 * steps / t / 1e6, 2),
+                     "ms_per_step": round(t * 1e3 / steps, 4),
+                     "cycles_mean": round(float(np.mean(cyc)), 2), "cycles_max": int(max(cyc)),
+                     "resid_max": max(res), "steps": steps}
+        log("[bench] 256^2 plain run, %-18s %.3f ms/step (%.1f Mcell-steps/s), %.1f cycles (max %d), resid max %.2e"
+            % (name, out[name]["ms_per_step"], out[name]["mcell_steps_per_s"], out[name]["cycles_mean"],
+               out[name]["cycles_max"], out[name]["resid_max"]))
+    return out
+
+
 def bench_config(args):
     from campfire_amd import scenes
     cfg = scenes.config_pile_1024(args.seed)
@@ -390,6 +432,8 @@ def main():
     ap.add_argument("--fcg-coarse", type=int, default=24, help="coarsest sweeps of the batched FCG form (0: direct)")
     ap.add_argument("--conv-steps", type=int, default=6,
                     help="steps of the equal-accuracy batch comparison (0: skip)")
+    ap.add_argument("--plain-steps", type=int, default=300,
+                    help="timed steps of the 256^2 plain-method workload (0: skip)")
     ap.add_argument("--converged-only", action="store_true",
                     help="run only the equal-accuracy batch comparison and print its JSON (for profiling)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -503,6 +547,8 @@ def main():
         out["step_latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 5),
                                   "p99": round(float(np.percentile(lat, 99)), 5)}
     sim.close()
+    if args.plain_steps > 0:
+        out["plain_256"] = plain_256(args, args.plain_steps)
     if args.batch > 0:
         out["batched"] = batched(args)
         if args.conv_steps > 0:
