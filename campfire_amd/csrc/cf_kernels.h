@@ -237,8 +237,9 @@ unsigned resnorm_blocks(const Level& L0);
 // stalls where CG diverges: at a right-hand side outside the range of a
 // singular system.
 struct KryBufs {
-  double *p64, *d64;  // the pressure and the stored direction
-  float* b0;          // the saved right-hand side
+  double* p64;  // the pressure
+  float* d32;   // the stored direction (FP32: the residual is recomputed from p64)
+  float* b0;    // the saved right-hand side
 };
 void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
                      hipStream_t s);

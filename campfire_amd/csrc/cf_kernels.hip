@@ -1174,12 +1174,12 @@ struct KryDotsOp {
   template <bool H>
   static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double (&v)[3]) {
     const Level& L = a_.L;
-    const double* __restrict__ d64 = a_.k.d64;
+    const float* __restrict__ d32 = a_.k.d32;
     const auto zat = [&](int q) { return kry_z<H>(L, a_.ph0, is, q); };
     const int n = L.nx * L.ny;
     for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
       const int j = k / L.nx, i = k - j * L.nx;
-      const double z = zat(k), dk = d64[k], r = (double)L.rhs[k];
+      const double z = zat(k), dk = (double)d32[k], r = (double)L.rhs[k];
       double w;
       if (!apply64(L, zat, i, j, z, w)) w = 0.0;
       v[0] += w * dk;
@@ -1227,7 +1227,8 @@ struct KryCoefOp {
 };
 __global__ __launch_bounds__(256) void k_kry_coef(const KryCoefOp::Args a_) { KryCoefOp::run(a_, 0); }
 
-// d <- z - beta d, p64 += alpha d (own cell only; dc_defect follows)
+// d <- z - beta d (stored in FP32), p64 += alpha d along the stored d (own
+// cell only; dc_defect follows)
 struct KryUpdateOp {
   struct Args {
     Level L;
@@ -1241,12 +1242,12 @@ struct KryUpdateOp {
   static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double alpha, double beta) {
     const Level& L = a_.L;
     double* __restrict__ p64 = a_.k.p64;
-    double* __restrict__ d64 = a_.k.d64;
+    float* __restrict__ d32 = a_.k.d32;
     const int n = L.nx * L.ny;
     for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
-      const double dn = kry_z<H>(L, a_.ph0, is, k) - beta * d64[k];
-      const double x = p64[k] + alpha * dn;
-      d64[k] = dn;
+      const float dn = (float)(kry_z<H>(L, a_.ph0, is, k) - beta * (double)d32[k]);
+      const double x = p64[k] + alpha * (double)dn;
+      d32[k] = dn;
       p64[k] = x;
     }
   }
@@ -1262,8 +1263,9 @@ __global__ __launch_bounds__(256) void k_kry_update(const KryUpdateOp::Args a_) 
 }
 
 // Krylov start from the previous step's pressure p (in p64): a line search
-// along p from zero.  kry_start saves b and stores d = p with the partials
-// of (p, b), (p, B p), (b, b); kry_start_coef folds them into
+// along p (rounded to FP32, the direction's storage) from zero.  kry_start
+// saves b and stores d = p with the partials of (p, b), (p, B p), (b, b);
+// kry_start_coef folds them into
 // a = (p, b) / (p, B p) ((p, B p) = 0: a = 0 and no stored direction);
 // kry_scale sets x = a p; dc_defect then forms r = b - B x.
 struct KryStartOp {
@@ -1278,19 +1280,19 @@ struct KryStartOp {
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Level& L = a_.L;
     const double* __restrict__ p64 = a_.k.p64;
-    double* __restrict__ d64 = a_.k.d64;
+    float* __restrict__ d32 = a_.k.d32;
     float* __restrict__ b0 = a_.k.b0;
-    const auto pat = [&](int q) { return p64[q]; };
+    const auto pat = [&](int q) { return (double)(float)p64[q]; };
     const int n = L.nx * L.ny;
     double v[3] = {0, 0, 0};  // (p, b), (p, B p), (b, b)
     for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
       const int j = k / L.nx, i = k - j * L.nx;
       const float bf = L.rhs[k];
-      const double b = (double)bf, p = p64[k];
+      const double b = (double)bf, p = pat(k);
       double bp;
       if (!apply64(L, pat, i, j, p, bp)) bp = 0.0;
       b0[k] = bf;
-      d64[k] = p;
+      d32[k] = (float)p;
       v[0] += p * b;
       v[1] += p * bp;
       v[2] += b * b;
@@ -1331,10 +1333,10 @@ struct KryScaleOp {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     double* __restrict__ p64 = a_.k.p64;
-    const double* __restrict__ d64 = a_.k.d64;
+    const float* __restrict__ d32 = a_.k.d32;
     const double a = a_.d->kalpha;
     const int n = a_.L.nx * a_.L.ny;
-    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) p64[k] = a * d64[k];
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) p64[k] = a * (double)d32[k];
   }
 };
 __global__ __launch_bounds__(256) void k_kry_scale(const KryScaleOp::Args a_) {

@@ -108,7 +108,7 @@ struct cf_sim {
   double* dNorm = nullptr;   // residual-norm partials
   double* dP64 = nullptr;    // level-0 pressure in FP64 (defect correction)
   float* dB0 = nullptr;      // level-0 right-hand side kept across correction cycles
-  double* dD64 = nullptr;    // Krylov: the stored direction
+  float* dD32 = nullptr;     // Krylov: the stored direction
   double* dBand = nullptr;   // direct coarsest solve: Cholesky band
   // upload rings
   ParamUpload* hP = nullptr;
@@ -237,7 +237,7 @@ static void enqueue_mg_pre(cf_sim* s) {
   if (ph0) for (int l = 0; l < s->mg_bottom; l++) { cf::launch_coef_half(s->L[l], s->LH[l], s->st); mark(s, "coef_half"); }
   if (tol) { cf::launch_fill_u32(&s->dDyn->converged, 0u, 2, s->st); mark(s, "mg_control"); }  // converged, cycles
   if (dc && s->sch.krylov) {
-    const cf::KryBufs kb{s->dP64, s->dD64, s->dB0};
+    const cf::KryBufs kb{s->dP64, s->dD32, s->dB0};
     cf::launch_kry_start(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st);
     mark(s, "kry_start");
   } else if (dc) {
@@ -251,7 +251,7 @@ static void enqueue_mg_cycle(cf_sim* s, int c, bool coarse_coef) {
   const bool kry = dc && s->sch.krylov;
   const int* skip = tol ? &s->dDyn->converged : nullptr;
   __half* ph0 = mg_ph0(s);
-  const cf::KryBufs kb{s->dP64, s->dD64, s->dB0};
+  const cf::KryBufs kb{s->dP64, s->dD32, s->dB0};
   enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0);
   if (kry) {
     cf::launch_kry_step(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st); mark(s, "kry_step");
@@ -493,7 +493,7 @@ static bool lookup(cf_sim* s, const char* name, FieldRef& r) {
       {"SQ", F.SQ, nC * 8}, {"cellG", F.cellG, nC * 8}, {"fx", F.fx, nU * 4}, {"fy", F.fy, nV * 4},
       {"pieces", F.pieces, CF_MAX_PIECES * CF_PIECE_FLOATS * 4}, {"acc", F.acc, CF_ACC_LEN * 4},
       {"rad", F.rad, CF_RAD_LEN * 16}, {"faceW", F.faceW, CF_MAX_PIECES * 4 * 4},
-      {"p64", s->dP64, nC * 8}, {"d64", s->dD64, nC * 8}};
+      {"p64", s->dP64, nC * 8}, {"d32", s->dD32, nC * 4}};
   for (auto& e : t)
     if (!strcmp(e.n, name)) { r.ptr = e.p; r.bytes = e.b; return true; }
   const char* pre[4] = {"rhs", "ax", "ay", "p"};
@@ -632,7 +632,7 @@ static int pool_build_step(cf_pool* pl) {
   }
   // V-cycles; down leg d and up leg d hold level d of every scene that has one
   const bool tol = k.tol > 0, dc = tol && k.refine, kry = dc && k.krylov;
-  auto kb = [](cf_sim* s) { return cf::KryBufs{s->dP64, s->dD64, s->dB0}; };
+  auto kb = [](cf_sim* s) { return cf::KryBufs{s->dP64, s->dD32, s->dB0}; };
   auto ph0 = [&](cf_sim* s) { return k.smoother_fp16 ? s->LH[0].p : (__half*)nullptr; };
   if (k.smoother_fp16) {
     for (int l = 0; l < MAXLEV; l++) {
@@ -896,7 +896,7 @@ static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
   add((void**)&s->dAccPart, (size_t)nparts * 16);
   add((void**)&s->dNorm, 4 * sizeof(double) * 1024);
   add((void**)&s->dP64, nC * sizeof(double));
-  add((void**)&s->dD64, nC * sizeof(double));
+  add((void**)&s->dD32, nC * sizeof(float));
   add((void**)&s->dB0, nC * sizeof(float));
   add((void**)&s->dBand, (256 * 33 + 256) * sizeof(double));
 }

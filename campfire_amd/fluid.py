@@ -270,13 +270,16 @@ class FluidSim:
 
     readBuffer = read_buffer
 
-    def read_p64(self, name="p64"):
-        """The FP64 level-0 pressure of defect correction (refine = 1), or with
-        name "d64" the stored direction of the Krylov iteration (krylov = 1)."""
-        if name not in ("p64", "d64"):
-            raise KeyError(name)
+    def read_p64(self):
+        """The FP64 level-0 pressure of defect correction (refine = 1)."""
         a = np.empty(self.nx * self.ny, np.float64)
-        check(L.lib().cf_sim_read_field(self._h, name.encode(), a.ctypes.data, a.nbytes), "read " + name)
+        check(L.lib().cf_sim_read_field(self._h, b"p64", a.ctypes.data, a.nbytes), "read p64")
+        return a
+
+    def read_direction(self):
+        """The stored FP32 direction of the Krylov iteration (krylov = 1)."""
+        a = np.empty(self.nx * self.ny, np.float32)
+        check(L.lib().cf_sim_read_field(self._h, b"d32", a.ctypes.data, a.nbytes), "read d32")
         return a
 
     def write_field(self, name, arr):

@@ -106,7 +106,7 @@ typedef struct {
   REAL *p[MAXLEV], *rhs[MAXLEV], *ax[MAXLEV], *ay[MAXLEV];
   double* p64;            /* defect correction: level-0 pressure in FP64 */
   REAL* b0;               /* defect correction: level-0 right-hand side */
-  double* d64;            /* Krylov: the stored direction */
+  float* d32;             /* Krylov: the stored direction (FP32) */
   double kbb, kdq;        /* Krylov: ||b||^2, (d, A d) of the stored direction (0: none) */
   double kbest;           /* Krylov: lowest ||r|| / ||b|| of this solve */
   int kfall;              /* Krylov: fallen back to defect correction for this solve */
@@ -920,7 +920,8 @@ static double kry_defect(S* s) {
 }
 
 /* Krylov start (schedule option krylov = 1), from the previous step's
- * pressure p: a line search along p from zero, x = a p with
+ * pressure p rounded to FP32 (the direction's storage): a line search along
+ * p from zero, x = a p with
  * a = (p, b) / (p, B p), which keeps the previous pressure where it fits
  * this step's right-hand side and scales it (or drops it, a = 0) where it
  * does not; p stays as the stored direction (r = b - a B p is orthogonal to
@@ -930,20 +931,26 @@ static double kry_start(S* s) {
   const int nx = s->nx, ny = s->ny, n = nx * ny;
   double* v = malloc(sizeof(double) * (size_t)n);
   double* bp = malloc(sizeof(double) * (size_t)n);
-  for (int k = 0; k < n; k++) { s->b0[k] = s->rhs[0][k]; s->d64[k] = s->p64[k]; }
+  double* d = malloc(sizeof(double) * (size_t)n);
+  for (int k = 0; k < n; k++) {
+    s->b0[k] = s->rhs[0][k];
+    s->d32[k] = (float)s->p64[k];
+    d[k] = (double)s->d32[k];
+  }
   for (int j = 0; j < ny; j++)
-    for (int i = 0; i < nx; i++) bp[j * nx + i] = apply64(s, s->d64, i, j);
-  for (int k = 0; k < n; k++) v[k] = s->d64[k] * (double)s->b0[k];
+    for (int i = 0; i < nx; i++) bp[j * nx + i] = apply64(s, d, i, j);
+  for (int k = 0; k < n; k++) v[k] = d[k] * (double)s->b0[k];
   const double pb = gpu_sum(v, n);
-  for (int k = 0; k < n; k++) v[k] = s->d64[k] * bp[k];
+  for (int k = 0; k < n; k++) v[k] = d[k] * bp[k];
   const double pq = gpu_sum(v, n);
   for (int k = 0; k < n; k++) v[k] = (double)s->b0[k] * (double)s->b0[k];
   s->kbb = gpu_sum(v, n);
   const double a = pq != 0 ? pb / pq : 0.0;
   s->kdq = pq;
-  for (int k = 0; k < n; k++) s->p64[k] = a * s->d64[k];
+  for (int k = 0; k < n; k++) s->p64[k] = a * d[k];
   free(v);
   free(bp);
+  free(d);
   const double rel = kry_defect(s);
   s->kbest = rel;
   s->kfall = 0;
@@ -952,7 +959,8 @@ static double kry_start(S* s) {
 
 /* One flexible-CG iteration after the V-cycle has left z = M r in p[0]
  * (Notay's FCG with one stored direction), with the residual recomputed
- * from the pressure each time rather than carried:
+ * from the pressure each time rather than carried, so the direction can be
+ * stored in FP32 (rounding it only perturbs the conjugacy):
  *   beta = (B z, d) / (d, B d)   (= (z, B d): B is symmetric),
  *   d <- z - beta d,  alpha = (r, z) / (d, B d),  p64 += alpha d,
  *   r = b - B p64,
@@ -975,7 +983,7 @@ static double kry_step(S* s) {
     double* v = malloc(sizeof(double) * (size_t)n);
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) w[j * nx + i] = apply64(s, z, i, j);
-    for (int k = 0; k < n; k++) v[k] = w[k] * s->d64[k];
+    for (int k = 0; k < n; k++) v[k] = w[k] * (double)s->d32[k];
     const double wd = gpu_sum(v, n);
     for (int k = 0; k < n; k++) v[k] = (double)s->rhs[0][k] * z[k];
     const double rz = gpu_sum(v, n);
@@ -989,9 +997,9 @@ static double kry_step(S* s) {
     free(v);
   }
   for (int k = 0; k < n; k++) {
-    const double d = z[k] - beta * s->d64[k];
-    s->d64[k] = d;
-    s->p64[k] += alpha * d;
+    const float d = (float)(z[k] - beta * (double)s->d32[k]);
+    s->d32[k] = d;
+    s->p64[k] += alpha * (double)d;
   }
   free(z);
   const double rel = kry_defect(s);
@@ -1349,7 +1357,7 @@ void* PFX(create)(int nx, int ny, double dx, double dt, int* err) {             
   s->V = calloc(nV, sizeof(REAL)); s->Vh = calloc(nV, sizeof(REAL)); s->V2 = calloc(nV, sizeof(REAL));
   s->T = calloc(nC, sizeof(REAL)); s->Th = calloc(nC, sizeof(REAL)); s->T2 = calloc(nC, sizeof(REAL));
   s->p64 = calloc(nC, sizeof(double)); s->b0 = calloc(nC, sizeof(REAL));
-  s->d64 = calloc(nC, sizeof(double));
+  s->d32 = calloc(nC, sizeof(float));
   s->Y = calloc(4 * nC, sizeof(REAL)); s->Yh = calloc(4 * nC, sizeof(REAL)); s->Y2 = calloc(4 * nC, sizeof(REAL));
   s->SQ = calloc(2 * nC, sizeof(REAL));
   s->sdf = calloc(nC, sizeof(REAL)); s->sid = calloc(nC, sizeof(uint32_t));
@@ -1384,7 +1392,7 @@ void PFX(destroy)(void* h) {
   if (!s) return;
   free(s->U); free(s->Uh); free(s->U2); free(s->V); free(s->Vh); free(s->V2);
   free(s->T); free(s->Th); free(s->T2); free(s->Y); free(s->Yh); free(s->Y2);
-  free(s->p64); free(s->b0); free(s->d64);
+  free(s->p64); free(s->b0); free(s->d32);
   free(s->SQ); free(s->sdf); free(s->sid); free(s->fx); free(s->fy);
   for (int l = 0; l < s->nlev; l++) { free(s->p[l]); free(s->rhs[l]); free(s->ax[l]); free(s->ay[l]); }
   free(s);
@@ -1665,8 +1673,8 @@ int PFX(read_field64)(void* h, const char* name, double* dst, size_t count) {
   S* s = (S*)h;
   if (!strcmp(name, "sdf")) { if (count != (size_t)s->nC) return CF_EINVAL; for (int k = 0; k < s->nC; k++) dst[k] = s->sdf[k]; return 0; }
   if (!strcmp(name, "rad")) { if (count != CF_RAD_LEN * 4) return CF_EINVAL; for (int q = 0; q < CF_RAD_LEN * 4; q++) dst[q] = s->rad[q]; return 0; }
-  const double* src = !strcmp(name, "p64") ? s->p64 : !strcmp(name, "d64") ? s->d64 : NULL;
-  if (src) { if (count != (size_t)s->nC) return CF_EINVAL; memcpy(dst, src, sizeof(double) * s->nC); return 0; }
+  if (!strcmp(name, "p64")) { if (count != (size_t)s->nC) return CF_EINVAL; memcpy(dst, s->p64, sizeof(double) * s->nC); return 0; }
+  if (!strcmp(name, "d32")) { if (count != (size_t)s->nC) return CF_EINVAL; for (int k = 0; k < s->nC; k++) dst[k] = s->d32[k]; return 0; }
   size_t c; int lev;
   REAL* f = field(s, name, &c, &lev);
   if (!f || count != c) return CF_EINVAL;

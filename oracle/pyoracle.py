@@ -247,7 +247,7 @@ class Oracle:
         tab = {"U": nU, "Uh": nU, "U2": nU, "V": nV, "Vh": nV, "V2": nV, "T": nC, "Th": nC, "T2": nC,
                "Y": 4 * nC, "Yh": 4 * nC, "Y2": 4 * nC, "SQ": 2 * nC, "cellG": 2 * nC, "fx": nU, "fy": nV,
                "acc": 528, "faceW": 256, "pieces": 1280, "rad": 1280, "p64": nC,
-               "d64": nC}
+               "d32": nC}
         if name in tab:
             return tab[name]
         lx, ly = nx, ny
