@@ -265,6 +265,37 @@ class FluidSim:
         check(L.lib().cf_sim_read_field(self._h, name.encode(), a.ctypes.data, a.nbytes), "read " + name)
         return a
 
+    # the buffers shaders/render.wgsl:20-29 binds that the simulation owns,
+    # as (binding, buffer, WGSL element type, bytes per element); bindings 1
+    # (the renderer's camera uniform) and 8 (the wood model's state) belong to
+    # the app and are not dumped
+    RENDER_BUFFERS = [(0, "P", "Params (uniform)", 160), (2, "T", "f32", 4), (3, "Y", "vec4f", 16),
+                      (4, "SQ", "vec2f", 8), (5, "pieces", "Piece", 4 * PIECE_FLOATS), (6, "U", "f32", 4),
+                      (7, "V", "f32", 4), (9, "rad", "vec4f", 16)]
+
+    def dump_render_buffers(self, path):
+        """Headless field dump for an offline renderer: every buffer that
+        shaders/render.wgsl:20-29 binds from the simulation, written as raw
+        little-endian bytes in the layout the shader reads (js/render.js:37
+        binds the simulation's own buffers), to <path>/<name>.bin, with
+        <path>/manifest.json naming the binding, element type, element count
+        and the grid.  Returns the manifest."""
+        import json
+        import os
+        os.makedirs(path, exist_ok=True)
+        info = self.info()
+        man = {"nx": self.nx, "ny": self.ny, "dx": float(self.dx), "time": float(info.time),
+               "frame": int(info.frame), "buffers": []}
+        for binding, name, wtype, esize in self.RENDER_BUFFERS:
+            data = bytes(self.packed_params()) if name == "P" else self.read_field(name).astype("<f4").tobytes()
+            with open(os.path.join(path, name + ".bin"), "wb") as fh:
+                fh.write(data)
+            man["buffers"].append({"binding": binding, "name": name, "type": wtype, "bytes": len(data),
+                                   "elements": len(data) // esize})
+        with open(os.path.join(path, "manifest.json"), "w") as fh:
+            json.dump(man, fh, indent=1)
+        return man
+
     def read_buffer(self, name):  # js/fluid.js:345-355: the raw buffer as Float32Array
         return self.read_field(name).view(np.float32)
 
