@@ -3,28 +3,31 @@
 // The reference runs it as two dispatch pairs (shaders/advect.wgsl):
 // scalForward + velForward write Th, Yh, Uh, Vh; scalCorrect + velCorrect
 // read them back at the backward-traced points and write T2, Y2, U2, V2.
-// Here one workgroup owns a 32x16 tile of cells and keeps everything of the
-// tile's neighbourhood in LDS:
-//   A. U, V, T, Y over the tile + 5 cells, as the values the boundary-aware
+// Here one workgroup of 512 threads owns a 32x32 tile of cells and keeps
+// everything of the tile's neighbourhood in LDS:
+//   A. U, V, T, Y over the tile + 3 cells, as the values the boundary-aware
 //      samplers of js/gpu.js:16-65 return for each index (ambient outside the
 //      sides and top, floor clamp, no-slip ghost for U), so sampling LDS is
 //      sampling the field;
-//   B. the forward pass (Th, Yh, Uh, Vh) over the tile + 2 cells, by the
+//   B. the forward pass (Th, Yh, Uh, Vh) over the tile + 1 cell, by the
 //      same rules;
 //   C. the correction and limiter for the tile's cells and faces.
-// A bilinear footprint that leaves a window (a displacement of more than
-// about one cell per step) reads the global field, or evaluates the forward
-// pass at that index, with the same arithmetic, so the result never depends
-// on the window sizes.  Th, Yh, Uh, Vh are not written to memory.
+// These halos hold every footprint of a displacement below one cell per
+// step.  A bilinear footprint that leaves a window reads the global field, or
+// evaluates the forward pass at that index, with the same arithmetic, so the
+// result never depends on the window sizes.  Two 73 KB workgroups fit in a
+// CU's LDS (16 waves); the forward pass over the ring around the tile adds
+// 13 % to the tile's own.  Th, Yh, Uh, Vh are not written to memory.
 #include "cf_kernels.h"
 
 namespace cf {
 
 namespace {
 
-constexpr int AX = 32, AY = 16;  // output tile (cells)
-constexpr int AH = 2;            // halo of the forward-pass window
-constexpr int AR = 5;            // halo of the staged input window
+constexpr int AX = 32, AY = 32;  // output tile (cells)
+constexpr int NT = 512;          // threads: two tile cells each
+constexpr int AH = 1;            // halo of the forward-pass window
+constexpr int AR = 3;            // halo of the staged input window
 constexpr int A_CX = AX + 2 * AR, A_CY = AY + 2 * AR;  // T, Y
 constexpr int A_UX = A_CX + 1, A_UY = A_CY;            // U faces
 constexpr int A_VX = A_CX, A_VY = A_CY + 1;            // V faces
@@ -82,7 +85,7 @@ struct Tile {
 // integer footprints and weights 0.5 or 0: they are read at fixed offsets
 // and mixed with constant weights, the arithmetic of bil() on those points.
 // Every B and C point's first footprint lies inside the A windows (their
-// halo is 5, the points are within 2 of the tile, the footprint reaches 1
+// halo is AR, the points are within AH of the tile, the footprint reaches 1
 // more), so `ok` is unchanged by these samples.
 __device__ __forceinline__ float2 trace_rest(const Tile& t, float gx, float gy, float u, float v, float dt, bool& ok) {
   const float vx = div_rcp_fast(u, t.dx, t.idx);
@@ -217,7 +220,7 @@ __device__ __forceinline__ float g_corr_v(const Fields& F, const cf_params& P, i
 
 struct AdvectOp {
   using Args = Fields;
-  static constexpr int kThreads = 256;
+  static constexpr int kThreads = NT;
   static __device__ __forceinline__ void run(const Fields& F, int tile);
   static int tiles(int nx, int ny) { return ((nx + AX - 1) / AX) * ((ny + AY - 1) / AY); }
 };
@@ -246,13 +249,13 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   t.idx = K(1) / P.dx;
 
   // A. inputs as the samplers see them
-  for (int q = tid; q < A_CX * A_CY; q += 256) {
+  for (int q = tid; q < A_CX * A_CY; q += NT) {
     const int i = ax0 + q % A_CX, j = ay0 + q / A_CX;
     sT[q] = fC(F.T, nx, ny, P.Tamb, i, j);
     sY[q] = fC4(F.Y, nx, ny, aY, i, j);
   }
-  for (int q = tid; q < A_UX * A_UY; q += 256) sU[q] = fU(F.U, nx, ny, ax0 + q % A_UX, ay0 + q / A_UX);
-  for (int q = tid; q < A_VX * A_VY; q += 256) sV[q] = fV(F.V, nx, ny, ax0 + q % A_VX, ay0 + q / A_VX);
+  for (int q = tid; q < A_UX * A_UY; q += NT) sU[q] = fU(F.U, nx, ny, ax0 + q % A_UX, ay0 + q / A_UX);
+  for (int q = tid; q < A_VX * A_VY; q += NT) sV[q] = fV(F.V, nx, ny, ax0 + q % A_VX, ay0 + q / A_VX);
   __syncthreads();
 
   // B. forward pass over the tile + AH, as the corrector's samplers see it.
@@ -263,7 +266,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   const float sO2i = K(1) / P.sO2;
   const bool ex = x0 + AX == nx, ey = y0 + AY == ny;
   constexpr int NQ = AX * AY + AY + AX;  // C's work items: the tile's cells (+ faces), then edge faces
-  constexpr int NS = (NQ + 255) / 256;
+  constexpr int NS = (NQ + NT - 1) / NT;
   // the points C corrects: item q -> (i, j) and which of its cell / U / V it owns
   auto item = [&](int q, int& i, int& j, bool& oc, bool& ou, bool& ov) {
     const bool tp = q < AX * AY, eu = q >= AX * AY && q < AX * AY + AY, ev = q >= AX * AY + AY && q < NQ;
@@ -303,9 +306,9 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
     float2 dC, dU, dV;
     bool okC, okU, okV;
   };
-  static_assert(NS == 3 && AX * AY == 512, "two tile slots and one edge slot per thread");
+  static_assert(NS == 3 && AX * AY == 2 * NT && AX + AY <= NT, "two tile slots and one edge slot per thread");
   Slot S0, S1;
-  // B1: the tile rectangle's points of work items tid and tid + 256: the
+  // B1: the tile rectangle's points of work items tid and tid + NT: the
   // forward pass of the points C corrects (traces kept), the outside values
   // for the rest (beyond the domain's right or top edge)
   auto b1 = [&](int q, Slot& S) {
@@ -324,7 +327,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
     bV[(j - by0) * B_VX + (i - bx0)] = ov ? fwd_v(i, j, S.dV, S.okV) : K(0);
   };
   b1(tid, S0);
-  b1(tid + 256, S1);
+  b1(tid + NT, S1);
   // B2: the ring of the window around the tile rectangle, enumerated without
   // gaps: r -> window (a, b) for a W x H window whose rectangle starts at
   // (AH, AH) with size AX x AY
@@ -337,7 +340,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       b = AH + (r - top) / (W - AX);
     } else { a = (r - top - mid) % W; b = AH + AY + (r - top - mid) / W; }
   };
-  for (int r = tid; r < B_CX * B_CY - AX * AY; r += 256) {
+  for (int r = tid; r < B_CX * B_CY - AX * AY; r += NT) {
     int a, b;
     ring(r, B_CX, a, b);
     const int i = bx0 + a, j = by0 + b;
@@ -351,7 +354,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
     bT[b * B_CX + a] = th;
     bY[b * B_CX + a] = yh;
   }
-  for (int r = tid; r < B_UX * B_UY - AX * AY; r += 256) {
+  for (int r = tid; r < B_UX * B_UY - AX * AY; r += NT) {
     int a, b;
     ring(r, B_UX, a, b);
     const int i = bx0 + a, j = by0 + b;
@@ -364,7 +367,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
     }
     bU[b * B_UX + a] = u;
   }
-  for (int r = tid; r < B_VX * B_VY - AX * AY; r += 256) {
+  for (int r = tid; r < B_VX * B_VY - AX * AY; r += NT) {
     int a, b;
     ring(r, B_VX, a, b);
     const int i = bx0 + a, j = by0 + b;
@@ -453,27 +456,27 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
     }
   };
   c1(tid, S0);
-  c1(tid + 256, S1);
+  c1(tid + NT, S1);
   {  // the domain-edge faces: traced here
     int i, j;
     bool oc, ou, ov;
-    item(tid + 512, i, j, oc, ou, ov);
+    item(tid + 2 * NT, i, j, oc, ou, ov);
     Slot S2;
     S2.dC = S2.dU = S2.dV = make_float2(0.f, 0.f);
     S2.okC = S2.okU = S2.okV = false;
     if (ou) { S2.okU = true; S2.dU = trace_uface(t, i, j, dt, S2.okU); }
     if (ov) { S2.okV = true; S2.dV = trace_vface(t, i, j, dt, S2.okV); }
-    c1(tid + 512, S2);
+    c1(tid + 2 * NT, S2);
   }
 }
 
-__global__ __launch_bounds__(256) void k_advect(Fields F) { AdvectOp::run(F, xcd_tile(blockIdx.x, gridDim.x)); }
+__global__ __launch_bounds__(NT) void k_advect(Fields F) { AdvectOp::run(F, xcd_tile(blockIdx.x, gridDim.x)); }
 
 void launch_advect(const Fields& F, hipStream_t s) {
-  launch(k_advect, dim3(AdvectOp::tiles(F.nx, F.ny)), dim3(256), 0, s, F);
+  launch(k_advect, dim3(AdvectOp::tiles(F.nx, F.ny)), dim3(NT), 0, s, F);
 }
 void phase_advect(BatchPhase& ph, const Fields& F) {
-  phase_add<AdvectOp>(ph, F, AdvectOp::tiles(F.nx, F.ny), dim3(256), "advect");
+  phase_add<AdvectOp>(ph, F, AdvectOp::tiles(F.nx, F.ny), dim3(NT), "advect");
 }
 
 }  // namespace cf
