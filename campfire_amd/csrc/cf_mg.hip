@@ -8,7 +8,7 @@
 // more ring from the halo edge and the tile interior stays exact.  That gives
 //   k_mg_down  : nu sweeps (red, black) + residual restriction, halo 2nu+1
 //   k_mg_up    : prolongation + nu sweeps (black, red), halo 2nu
-//   k_mg_bottom: every level of <= 1024 cells in one LDS-resident workgroup,
+//   k_mg_bottom: the last levels (<= 4096, 1024, 256 cells) in one LDS-resident workgroup,
 //                including the coarsest level's red-black sweeps
 // with the reference's arithmetic and operation order in every update.
 #include "cf_kernels.h"
@@ -307,8 +307,10 @@ __global__ __launch_bounds__(256) void k_mg_up(const typename MgUpOp<TX, TY, NU,
 // <= 256 cells (each level above the coarsest has more than 256 cells and
 // four times the cells of the next).  One workgroup of 1024 threads keeps p
 // and rhs of all of them in LDS and each thread's cells' coefficients in
-// registers (4, 1 and 1 cells per thread), so a red-black half-sweep is one
-// pass of branch-free updates between two barriers: neighbours missing at the
+// registers: two horizontal pairs of the first level, one pair of the second
+// (threads below 512), each pair one red and one black cell, sorted by colour
+// at load, so every lane updates cells in every half-sweep; the coarsest
+// level's two pairs per lane of the first wave.  Neighbours missing at the
 // domain edge are read at the cell itself with a zero coefficient.  The
 // coarsest level's face coefficients are restricted here from the level above
 // when the caller has not produced them.
@@ -332,33 +334,43 @@ struct CellSet {
 };
 
 // cell data of a level for the cells kidx[q] (ignored when outside the level);
-// coefficient arrays may be in global memory (ax, ay) or in LDS
+// coefficient arrays may be in global memory (ax, ay) or in LDS.  Every
+// load is issued before any arithmetic (a slot without a cell loads cell
+// 0's coefficients and discards them), so they are in flight together.
 template <int CPT>
 __device__ __forceinline__ void cells_load(CellSet<CPT>& C, int nx, int ny, const float* ax, const float* ay,
                                            const int (&kidx)[CPT]) {
+  float w[CPT], e[CPT], s[CPT], n[CPT];
+  bool has[CPT];
+  int ii[CPT], jj[CPT];
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
     const int k = kidx[q];
-    C.k[q] = -1;
-    C.w[q] = C.e[q] = C.s[q] = C.n[q] = C.diag[q] = C.y[q] = 0.f;
-    C.nb[q] = 0;
-    if (k >= 0 && k < nx * ny) {
-      const int i = k % nx, j = k / nx;
-      const float w = ax[j * (nx + 1) + i], e = ax[j * (nx + 1) + i + 1], s = ay[j * nx + i], n = ay[(j + 1) * nx + i];
-      float diag = 0.f;  // the diagonal of shaders/pressure.wgsl:77-85, same accumulation order
-      if (i > 0) diag += w; else diag += K(2.0) * w;
-      if (i < nx - 1) diag += e; else diag += K(2.0) * e;
-      if (j > 0) diag += s;
-      if (j < ny - 1) diag += n; else diag += K(2.0) * n;
-      C.k[q] = k;
-      C.diag[q] = diag;
-      C.y[q] = diag > K(0) ? K(1) / diag : K(0);
-      C.w[q] = i > 0 ? w : 0.f;
-      C.e[q] = i < nx - 1 ? e : 0.f;
-      C.s[q] = j > 0 ? s : 0.f;
-      C.n[q] = j < ny - 1 ? n : 0.f;
-      C.nb[q] = (i > 0 ? 1 : 0) | (i < nx - 1 ? 2 : 0) | (j > 0 ? 4 : 0) | (j < ny - 1 ? 8 : 0) | (((i + j) & 1) << 4);
-    }
+    has[q] = k >= 0 && k < nx * ny;
+    const int kk = has[q] ? k : 0;
+    const int i = kk % nx, j = kk / nx;
+    ii[q] = i; jj[q] = j;
+    w[q] = ax[j * (nx + 1) + i]; e[q] = ax[j * (nx + 1) + i + 1];
+    s[q] = ay[j * nx + i]; n[q] = ay[(j + 1) * nx + i];
+  }
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int i = ii[q], j = jj[q];
+    float diag = 0.f;  // the diagonal of shaders/pressure.wgsl:77-85, same accumulation order
+    if (i > 0) diag += w[q]; else diag += K(2.0) * w[q];
+    if (i < nx - 1) diag += e[q]; else diag += K(2.0) * e[q];
+    if (j > 0) diag += s[q];
+    if (j < ny - 1) diag += n[q]; else diag += K(2.0) * n[q];
+    C.k[q] = has[q] ? kidx[q] : -1;
+    C.diag[q] = has[q] ? diag : K(0);
+    C.y[q] = has[q] && diag > K(0) ? K(1) / diag : K(0);
+    C.w[q] = has[q] && i > 0 ? w[q] : 0.f;
+    C.e[q] = has[q] && i < nx - 1 ? e[q] : 0.f;
+    C.s[q] = has[q] && j > 0 ? s[q] : 0.f;
+    C.n[q] = has[q] && j < ny - 1 ? n[q] : 0.f;
+    C.nb[q] = has[q] ? ((i > 0 ? 1 : 0) | (i < nx - 1 ? 2 : 0) | (j > 0 ? 4 : 0) | (j < ny - 1 ? 8 : 0) |
+                        (((i + j) & 1) << 4))
+                     : 0;
   }
 }
 
@@ -367,32 +379,6 @@ __device__ __forceinline__ float cells_off(const CellSet<CPT>& C, int q, const f
   const int k = C.k[q], b = C.nb[q];
   return C.w[q] * p[k - (b & 1)] + C.e[q] * p[k + ((b >> 1) & 1)] + C.s[q] * p[k - nx * ((b >> 2) & 1)] +
          C.n[q] * p[k + nx * ((b >> 3) & 1)];
-}
-
-template <int CPT>
-__device__ __forceinline__ void cells_relax(const CellSet<CPT>& C, int nx, float* p, const float* rhs, float h2,
-                                            int col) {
-#pragma unroll
-  for (int q = 0; q < CPT; q++) {
-    if (C.k[q] >= 0 && ((C.nb[q] >> 4) & 1) == col) {
-      const int k = C.k[q];
-      const float off = cells_off<CPT>(C, q, p, nx);
-      p[k] = C.diag[q] > K(0) ? (off - rhs[k] * h2) / C.diag[q] : K(0);
-    }
-  }
-}
-
-template <int CPT>
-__device__ __forceinline__ void cells_residual(const CellSet<CPT>& C, int nx, const float* p, const float* rhs,
-                                               float h2, float* res) {
-#pragma unroll
-  for (int q = 0; q < CPT; q++) {
-    const int k = C.k[q];
-    if (k >= 0) {
-      const float off = cells_off<CPT>(C, q, p, nx);
-      res[k] = C.diag[q] <= K(0) ? K(0) : rhs[k] - (off - C.diag[q] * p[k]) / h2;
-    }
-  }
 }
 
 // One cell of the coarsest level's sweeps with everything in registers: the
@@ -442,6 +428,67 @@ __device__ __forceinline__ void sweep_relax2(const SweepCell& a, const SweepCell
   const float va = div_rcp_fast(offa - a.r, a.d, a.y), vb = div_rcp_fast(offb - b.r, b.d, b.y);
   p[a.k] = a.d > K(0) ? va : K(0);
   p[b.k] = b.d > K(0) ? vb : K(0);
+}
+
+// Entry q of a CellSet of slot 0 or 1 of the bottom solver (k = -1: none).
+// Neighbours missing at the domain edge are read at the cell itself with a
+// zero coefficient.
+template <int CPT>
+__device__ __forceinline__ float bot_off(const CellSet<CPT>& C, int q, const float* p, int nx) {
+  const int k = C.k[q], b = C.nb[q];
+  return C.w[q] * p[k - (b & 1)] + C.e[q] * p[k + ((b >> 1) & 1)] + C.s[q] * p[k - nx * ((b >> 2) & 1)] +
+         C.n[q] * p[k + nx * ((b >> 3) & 1)];
+}
+
+// the red-black update of shaders/pressure.wgsl:87-94 (terms in its order;
+// the quotient by the reciprocal-corrected division, as in the tiled legs)
+template <int CPT>
+__device__ __forceinline__ void bot_relax(const CellSet<CPT>& C, int q, int nx, float* p, const float* rhs, float h2) {
+  if (C.k[q] >= 0) {
+    const float off = bot_off<CPT>(C, q, p, nx);
+    const float v = div_rcp_fast(off - rhs[C.k[q]] * h2, C.diag[q], C.y[q]);
+    p[C.k[q]] = C.diag[q] > K(0) ? v : K(0);
+  }
+}
+
+// the residual (shaders/pressure.wgsl:96-101)
+template <int CPT>
+__device__ __forceinline__ void bot_residual(const CellSet<CPT>& C, int q, int nx, const float* p, const float* rhs,
+                                             float h2, float* res) {
+  if (C.k[q] >= 0) {
+    const float off = bot_off<CPT>(C, q, p, nx);
+    res[C.k[q]] = C.diag[q] <= K(0) ? K(0) : rhs[C.k[q]] - (off - C.diag[q] * p[C.k[q]]) / h2;
+  }
+}
+
+// bilinear coarse correction (shaders/pressure.wgsl:116-129)
+template <int CPT>
+__device__ __forceinline__ void bot_prolong(const CellSet<CPT>& C, int q, int nx, float* p, const float* pc, int nxc,
+                                            int nyc) {
+  const int k = C.k[q];
+  if (k >= 0) {
+    const int i = k % nx, j = k / nx;
+    const int I = i / 2, J = j / 2;
+    const int I2 = iclamp(I + ((i & 1) == 1 ? 1 : -1), 0, nxc - 1);
+    const int J2 = iclamp(J + ((j & 1) == 1 ? 1 : -1), 0, nyc - 1);
+    const float cc = K(0.5625) * pc[J * nxc + I] + K(0.1875) * (pc[J * nxc + I2] + pc[J2 * nxc + I]) +
+                     K(0.0625) * pc[J2 * nxc + I2];
+    p[k] += C.diag[q] > K(0) ? cc : K(0);
+  }
+}
+
+// entry q of a CellSet in LDS, word m at st[(w + m) * 64 + lane] (8 words)
+template <int CPT>
+__device__ __forceinline__ void stash_put(float* st, int lane, int w, const CellSet<CPT>& C, int q) {
+  float* o = st + w * 64 + lane;
+  o[0] = __int_as_float(C.k[q]); o[64] = __int_as_float(C.nb[q]);
+  o[128] = C.w[q]; o[192] = C.e[q]; o[256] = C.s[q]; o[320] = C.n[q]; o[384] = C.diag[q]; o[448] = C.y[q];
+}
+template <int CPT>
+__device__ __forceinline__ void stash_get(const float* st, int lane, int w, CellSet<CPT>& C, int q) {
+  const float* o = st + w * 64 + lane;
+  C.k[q] = __float_as_int(o[0]); C.nb[q] = __float_as_int(o[64]);
+  C.w[q] = o[128]; C.e[q] = o[192]; C.s[q] = o[256]; C.n[q] = o[320]; C.diag[q] = o[384]; C.y[q] = o[448];
 }
 
 // orders one wave's LDS traffic between half-sweeps without a workgroup barrier
@@ -533,28 +580,13 @@ __device__ __forceinline__ void restrict_to(const float* res, int nxf, float* rh
   }
 }
 
-// bilinear coarse correction (shaders/pressure.wgsl:116-129)
-template <int CPT>
-__device__ __forceinline__ void cells_prolong(const CellSet<CPT>& C, int nx, float* p, const float* pc, int nxc, int nyc) {
-#pragma unroll
-  for (int q = 0; q < CPT; q++) {
-    const int k = C.k[q];
-    if (k >= 0) {
-      const int i = k % nx, j = k / nx;
-      const int I = i / 2, J = j / 2;
-      const int I2 = iclamp(I + ((i & 1) == 1 ? 1 : -1), 0, nxc - 1);
-      const int J2 = iclamp(J + ((j & 1) == 1 ? 1 : -1), 0, nyc - 1);
-      const float cc = K(0.5625) * pc[J * nxc + I] + K(0.1875) * (pc[J * nxc + I2] + pc[J2 * nxc + I]) +
-                       K(0.0625) * pc[J2 * nxc + I2];
-      p[k] += C.diag[q] > K(0) ? cc : K(0);
-    }
-  }
-}
-
-// LDS of the bottom solver: p and rhs of the three slots, then the residual scratch
-// and the coarsest level's face coefficients (at most (nx+1)ny + nx(ny+1) <= 800)
+// LDS of the bottom solver: p and rhs of the three slots, the residual scratch,
+// the coarsest level's face coefficients (at most (nx+1)ny + nx(ny+1) <= 800),
+// and the first
+// wave's cell records while it sweeps the coarsest level (6 cells x 8 words)
 constexpr int BOT_P0 = 0, BOT_R0 = 4096, BOT_P1 = 8192, BOT_R1 = 9216, BOT_P2 = 10240, BOT_R2 = 10496,
-              BOT_RES = 10752, BOT_CAX = 14848, BOT_TOTAL = 15648;
+              BOT_RES = 10752, BOT_CAX = 14848, BOT_STASH = 15648,
+              BOT_TOTAL = BOT_STASH + 6 * 8 * 64;
 __device__ __forceinline__ int bot_p(int s) { return s == 0 ? BOT_P0 : (s == 1 ? BOT_P1 : BOT_P2); }
 __device__ __forceinline__ int bot_r(int s) { return s == 0 ? BOT_R0 : (s == 1 ? BOT_R1 : BOT_R2); }
 
@@ -566,136 +598,190 @@ struct MgBottomOp {
   static constexpr int kThreads = 1024;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Bottom& B = a_.B;
-
-  if (B.skip && *B.skip) return;
-  __shared__ float sm[BOT_TOTAL];
-  float* sp2 = sm + BOT_P2;
-  float* sr2 = sm + BOT_R2;
-  float* sres = sm + BOT_RES;  // fine residuals
-  auto P = [&](int s) { return sm + bot_p(s); };
-  auto R = [&](int s) { return sm + bot_r(s); };
-  const int nb = B.n, t = threadIdx.x;
-  const Level& Lc = B.L[nb - 1];
-  // coarsest-level coefficients into LDS: copied, or restricted here (:57-67)
-  float* axc = sm + BOT_CAX;
-  float* ayc = axc + (Lc.nx + 1) * Lc.ny;
-  if (B.coarse_coef) {
-    const Level& Lf = B.L[nb - 2];
-    for (int k = t; k < (Lc.nx + 1) * Lc.ny; k += BT) {
-      const int I = k % (Lc.nx + 1), J = k / (Lc.nx + 1);
-      const float v = K(0.5) * (Lf.ax[(2 * J) * (Lf.nx + 1) + 2 * I] + Lf.ax[(2 * J + 1) * (Lf.nx + 1) + 2 * I]);
-      axc[k] = v; Lc.ax[k] = v;
-    }
-    for (int k = t; k < Lc.nx * (Lc.ny + 1); k += BT) {
-      const int I = k % Lc.nx, J = k / Lc.nx;
-      const float v = K(0.5) * (Lf.ay[(2 * J) * Lf.nx + 2 * I] + Lf.ay[(2 * J) * Lf.nx + 2 * I + 1]);
-      ayc[k] = v; Lc.ay[k] = v;
-    }
-  } else {
-    for (int k = t; k < (Lc.nx + 1) * Lc.ny; k += BT) axc[k] = Lc.ax[k];
-    for (int k = t; k < Lc.nx * (Lc.ny + 1); k += BT) ayc[k] = Lc.ay[k];
-  }
-  // per-thread cell data: slot 0 (up to 4096 cells) cells t + 1024q, slot
-  // 1 (up to 1024) cell t.  The coarsest level is swept by the first wave
-  // alone: lane l owns cells 2l, 2l+1, 2l+128, 2l+129 (two of each colour),
-  // loaded just before its sweeps.
-  CellSet<4> C0, C2;
-  CellSet<1> C1;
-  // (re)loaded after the coarsest sweeps rather than kept through them, which
-  // leaves the coarsest level's per-colour cells room in registers
-  auto load_levels = [&]() {
-    const Level& L0 = B.L[0];
-    const int k0[4] = {t, t + 1024, t + 2048, t + 3072};
-    const int k1[1] = {t};
-    if (nb == 2) cells_load<1>(C1, L0.nx, L0.ny, L0.ax, L0.ay, k1);
-    if (nb == 3) {
-      cells_load<4>(C0, L0.nx, L0.ny, L0.ax, L0.ay, k0);
-      cells_load<1>(C1, B.L[1].nx, B.L[1].ny, B.L[1].ax, B.L[1].ay, k1);
-    }
-  };
-  {
-    const Level& L0 = B.L[0];
-    load_levels();
-    // p (warm start or zero) and rhs of the first level; slot = 3 - nb
+    if (B.skip && *B.skip) return;
+    __shared__ float sm[BOT_TOTAL];
+    float* sp2 = sm + BOT_P2;
+    float* sr2 = sm + BOT_R2;
+    float* sres = sm + BOT_RES;  // fine residuals
+    float* stash = sm + BOT_STASH;
+    auto P = [&](int s) { return sm + bot_p(s); };
+    auto R = [&](int s) { return sm + bot_r(s); };
+    const int nb = B.n, t = threadIdx.x;
+    // slot s holds level B.L[s - s0]
     const int s0 = 3 - nb;
-    for (int k = t; k < L0.nx * L0.ny; k += BT) { P(s0)[k] = L0.p[k]; R(s0)[k] = L0.rhs[k]; }
-  }
-  __syncthreads();
-  // slot s holds level B.L[s - (3 - nb)]
-  const int s0 = 3 - nb;
-  auto lev = [&](int s) -> const Level& { return B.L[s - s0]; };
-  // down (js/fluid.js:297-303)
-  for (int s = s0; s < 2; s++) {
-    const Level& L = lev(s);
-    for (int q = 0; q < B.nu; q++) {
-      for (int col = 0; col < 2; col++) {
-        if (s == 0) cells_relax<4>(C0, L.nx, P(0), R(0), L.h2, col); else cells_relax<1>(C1, L.nx, P(1), R(1), L.h2, col);
+    auto lev = [&](int s) -> const Level& { return B.L[s - s0]; };
+    const Level& Lc = B.L[nb - 1];
+    const Level& Lf0 = B.L[0];
+    float* axc = sm + BOT_CAX;
+    float* ayc = axc + (Lc.nx + 1) * Lc.ny;
+    const int nax = (Lc.nx + 1) * Lc.ny, nay = Lc.nx * (Lc.ny + 1), n0 = Lf0.nx * Lf0.ny;
+    // ---- set-up: every global load is issued before the first store, so
+    // they are in flight together.  The coarsest level's coefficients are
+    // copied, or restricted here (:57-67); p (warm start or zero) and rhs of
+    // the first level; the cell data of slots 0 and 1.
+    // (a thread past the face count loads face 0 and discards it)
+    float cax = 0.f, cay = 0.f;
+    const int tu = t < nax ? t : 0, tv = t < nay ? t : 0;
+    if (B.coarse_coef) {
+      const Level& Lf = B.L[nb - 2];
+      const int Iu = tu % (Lc.nx + 1), Ju = tu / (Lc.nx + 1), Iv = tv % Lc.nx, Jv = tv / Lc.nx;
+      const float a0 = Lf.ax[(2 * Ju) * (Lf.nx + 1) + 2 * Iu], a1 = Lf.ax[(2 * Ju + 1) * (Lf.nx + 1) + 2 * Iu];
+      const float b0 = Lf.ay[(2 * Jv) * Lf.nx + 2 * Iv], b1 = Lf.ay[(2 * Jv) * Lf.nx + 2 * Iv + 1];
+      cax = K(0.5) * (a0 + a1);
+      cay = K(0.5) * (b0 + b1);
+    } else {
+      cax = Lc.ax[tu];
+      cay = Lc.ay[tv];
+    }
+    float pv[4], rv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = t + BT * q;
+      pv[q] = rv[q] = 0.f;
+      if (k < n0) { pv[q] = Lf0.p[k]; rv[q] = Lf0.rhs[k]; }
+    }
+    // Slot 0 (<= 4096 cells): thread t owns the horizontal pairs starting at
+    // cells 2t and 2t + 2048; slot 1 (<= 1024): threads t < 512 own the pair
+    // at 2t.  The levels above the coarsest have an even width, so each pair
+    // is one red and one black cell, and every lane updates its cells of one
+    // colour in each half-sweep.
+    // A pair starting at an even column of row j has its red cell first
+    // when j is even.  C0 holds the red cells of the two pairs, then the
+    // black ones; C1 the red cell, then the black one.
+    CellSet<4> C0;
+    CellSet<2> C1;
+    auto load0 = [&]() {
+      const int nx = Lf0.nx, a = 2 * t, b = 2 * t + 2 * BT;
+      const int ra = ((a / nx) & 1) ? a + 1 : a, rb = ((b / nx) & 1) ? b + 1 : b;
+      const int k0[4] = {ra, rb, 2 * a + 1 - ra, 2 * b + 1 - rb};
+      cells_load<4>(C0, nx, Lf0.ny, Lf0.ax, Lf0.ay, k0);
+    };
+    auto load1 = [&]() {
+      const Level& L1 = lev(1);
+      const int a = 2 * t, ra = ((a / L1.nx) & 1) ? a + 1 : a;
+      const int k1[2] = {t < BT / 2 ? ra : -1, t < BT / 2 ? 2 * a + 1 - ra : -1};
+      cells_load<2>(C1, L1.nx, L1.ny, L1.ax, L1.ay, k1);
+    };
+    if (nb == 3) load0();
+    if (nb >= 2) load1();
+    if (t < nax) { axc[t] = cax; if (B.coarse_coef) Lc.ax[t] = cax; }
+    if (t < nay) { ayc[t] = cay; if (B.coarse_coef) Lc.ay[t] = cay; }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = t + BT * q;
+      if (k < n0) { P(s0)[k] = pv[q]; R(s0)[k] = rv[q]; }
+    }
+    __syncthreads();
+    auto relax = [&](int s, int col) {  // col 0: red, 1: black
+      const Level& L = lev(s);
+      if (s == 0) {
+        bot_relax<4>(C0, 2 * col, L.nx, P(0), R(0), L.h2);
+        bot_relax<4>(C0, 2 * col + 1, L.nx, P(0), R(0), L.h2);
+      } else {
+        bot_relax<2>(C1, col, L.nx, P(1), R(1), L.h2);
+      }
+    };
+    // ---- down (js/fluid.js:297-303)
+    for (int s = s0; s < 2; s++) {
+      const Level& L = lev(s);
+      for (int q = 0; q < B.nu; q++) {
+        relax(s, 0);
+        __syncthreads();
+        relax(s, 1);
+        __syncthreads();
+      }
+      if (s == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) bot_residual<4>(C0, q, L.nx, P(0), R(0), L.h2, sres);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 2; q++) bot_residual<2>(C1, q, L.nx, P(1), R(1), L.h2, sres);
+      }
+      __syncthreads();
+      const Level& Cn = lev(s + 1);
+      restrict_to(sres, L.nx, R(s + 1), P(s + 1), Cn.nx, Cn.ny);
+      __syncthreads();
+    }
+    // ---- coarsest: red-black sweeps from zero (shaders/pressure.wgsl:131-156)
+    // by the first wave; the other waves wait at the next workgroup barrier
+    for (int k = t; k < Lc.nx * Lc.ny; k += BT) sp2[k] = K(0);
+    const int kw[4] = {t < 64 ? 2 * t : -1, t < 64 ? 2 * t + 1 : -1, t < 64 ? 2 * t + 128 : -1,
+                       t < 64 ? 2 * t + 129 : -1};
+    if (B.coarse_iters <= 0) {
+      CellSet<4> C2;
+      cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
+      __syncthreads();
+      coarse_direct<4>(C2, Lc.nx, Lc.ny, Lc.h2, sr2, sp2, B.band, (double*)sres);
+      // the cell data of slots 0 and 1 again, rather than held through the
+      // factorization
+      if (nb == 3) load0();
+      if (nb >= 2) load1();
+    } else {
+      __syncthreads();
+      // a wave-uniform branch (the first wave), so the registers of values
+      // held only outside it are free inside it
+      if (__builtin_amdgcn_readfirstlane(t) < 64) {
+        // the wave's slot 0 and 1 cells wait in LDS, leaving their
+        // registers to the coarsest level's cells
+#pragma unroll
+        for (int q = 0; q < 4; q++) stash_put<4>(stash, t, 8 * q, C0, q);
+#pragma unroll
+        for (int q = 0; q < 2; q++) stash_put<2>(stash, t, 8 * (4 + q), C1, q);
+        CellSet<4> C2;
+        cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
+        float rh[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rh[q] = C2.k[q] >= 0 ? sr2[C2.k[q]] * Lc.h2 : K(0);
+        // each lane's red and black cell of its two pairs; no cell: sres[0]
+        SweepCell X[2][2];
+        const int dummy = BOT_RES - BOT_P2;
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          const SweepCell c0 = sweep_cell<4>(C2, rh, 2 * a, Lc.nx, dummy), c1 = sweep_cell<4>(C2, rh, 2 * a + 1, Lc.nx, dummy);
+          const bool first_red = ((C2.nb[2 * a] >> 4) & 1) == 0;
+          X[0][a] = sweep_sel(first_red, c0, c1);
+          X[1][a] = sweep_sel(first_red, c1, c0);
+        }
+        for (int it = 0; it < B.coarse_iters; it++) {
+          sweep_relax2(X[0][0], X[0][1], sp2);
+          wave_sync();
+          sweep_relax2(X[1][0], X[1][1], sp2);
+          wave_sync();
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) stash_get<4>(stash, t, 8 * q, C0, q);
+#pragma unroll
+        for (int q = 0; q < 2; q++) stash_get<2>(stash, t, 8 * (4 + q), C1, q);
+      }
+    }
+    __syncthreads();
+    // ---- up (js/fluid.js:305-311)
+    for (int s = 1; s >= s0; s--) {
+      const Level& L = lev(s);
+      const Level& Cn = lev(s + 1);
+      if (s == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) bot_prolong<4>(C0, q, L.nx, P(0), P(1), Cn.nx, Cn.ny);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 2; q++) bot_prolong<2>(C1, q, L.nx, P(1), P(2), Cn.nx, Cn.ny);
+      }
+      __syncthreads();
+      for (int q = 0; q < B.nu; q++) {  // black then red
+        relax(s, 1);
+        __syncthreads();
+        relax(s, 0);
         __syncthreads();
       }
     }
-    if (s == 0) cells_residual<4>(C0, L.nx, P(0), R(0), L.h2, sres); else cells_residual<1>(C1, L.nx, P(1), R(1), L.h2, sres);
-    __syncthreads();
-    const Level& Cn = lev(s + 1);
-    restrict_to(sres, L.nx, R(s + 1), P(s + 1), Cn.nx, Cn.ny);
-    __syncthreads();
-  }
-  // coarsest: red-black sweeps from zero (shaders/pressure.wgsl:131-156) by
-  // the first wave; the other waves wait at the next workgroup barrier
-  for (int k = t; k < Lc.nx * Lc.ny; k += BT) sp2[k] = K(0);
-  {
-    const int kw[4] = {t < 64 ? 2 * t : -1, t < 64 ? 2 * t + 1 : -1, t < 64 ? 2 * t + 128 : -1, t < 64 ? 2 * t + 129 : -1};
-    cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
-  }
-  __syncthreads();
-  if (B.coarse_iters <= 0) {
-    coarse_direct<4>(C2, Lc.nx, Lc.ny, Lc.h2, sr2, sp2, B.band, (double*)sres);
-  } else if (t < 64) {
-    float rh[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) rh[q] = C2.k[q] >= 0 ? sr2[C2.k[q]] * Lc.h2 : K(0);
-    // each lane's red and black cell of its two pairs; no cell: sres[0]
-    SweepCell X[2][2];
-    const int dummy = BOT_RES - BOT_P2;
-#pragma unroll
-    for (int a = 0; a < 2; a++) {
-      const SweepCell c0 = sweep_cell<4>(C2, rh, 2 * a, Lc.nx, dummy), c1 = sweep_cell<4>(C2, rh, 2 * a + 1, Lc.nx, dummy);
-      const bool first_red = ((C2.nb[2 * a] >> 4) & 1) == 0;
-      X[0][a] = sweep_sel(first_red, c0, c1);
-      X[1][a] = sweep_sel(first_red, c1, c0);
-    }
-    for (int it = 0; it < B.coarse_iters; it++) {
-      sweep_relax2(X[0][0], X[0][1], sp2);
-      wave_sync();
-      sweep_relax2(X[1][0], X[1][1], sp2);
-      wave_sync();
-    }
-  }
-  __syncthreads();
-  load_levels();
-  // up (js/fluid.js:305-311)
-  for (int s = 1; s >= s0; s--) {
-    const Level& L = lev(s);
-    const Level& Cn = lev(s + 1);
-    if (s == 0) cells_prolong<4>(C0, L.nx, P(0), P(1), Cn.nx, Cn.ny);
-    else cells_prolong<1>(C1, L.nx, P(1), P(2), Cn.nx, Cn.ny);
-    __syncthreads();
-    for (int q = 0; q < B.nu; q++) {
-      for (int c = 0; c < 2; c++) {
-        const int col = 1 - c;
-        if (s == 0) cells_relax<4>(C0, L.nx, P(0), R(0), L.h2, col); else cells_relax<1>(C1, L.nx, P(1), R(1), L.h2, col);
-        __syncthreads();
+    // write back p of every level and rhs of the levels below the first
+    for (int s = s0; s < 3; s++) {
+      const Level& L = lev(s);
+      for (int k = t; k < L.nx * L.ny; k += BT) {
+        L.p[k] = P(s)[k];
+        if (s > s0) L.rhs[k] = R(s)[k];
       }
     }
-  }
-  // write back p of every level and rhs of the levels below the first
-  for (int s = s0; s < 3; s++) {
-    const Level& L = lev(s);
-    for (int k = t; k < L.nx * L.ny; k += BT) {
-      L.p[k] = P(s)[k];
-      if (s > s0) L.rhs[k] = R(s)[k];
-    }
-  }
-
   }
 };
 __global__ __launch_bounds__(1024) void k_mg_bottom(const MgBottomOp::Args a_) {
