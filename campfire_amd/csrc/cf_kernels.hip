@@ -260,22 +260,33 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
     Tc = (r * Tc + madd * P.burnerT) / (r + madd);
     massSrc += mf;
   }
-  // one-step Arrhenius, 8 substeps with rho frozen (:109-120)
+  // one-step Arrhenius, 8 substeps with rho frozen (:109-120).  The rate's
+  // last factor, exp(-Ta/T), is positive and finite, so where the fuel
+  // fraction is zero the product is that zero (of the same sign) with any
+  // positive factor: the exponential (and its quotient) is evaluated only
+  // where the fuel fraction is not zero, a branch that waves of fuel-free
+  // cells skip.  Such a cell's fraction stays zero through the substeps.
   const float rho0 = rhoOf(P, Tc);
   const float h = dt / (float)8;
   float burned = K(0);
 #pragma unroll 1
   for (int s = 0; s < 8; s++) {
-    const float wr = P.reactB * rho0 * Yc.x * Yc.y * expf(-P.reactTa / Tc);
+    float ea = K(1);
+    if (Yc.x != K(0)) ea = expf(-P.reactTa / Tc);
+    const float wr = P.reactB * rho0 * Yc.x * Yc.y * ea;
     const float dF = fminf(wr * h, fminf(Yc.x, div_rcp(Yc.y, sO2, isO2)));
     Yc.x += -dF; Yc.y += -sO2 * dF; Yc.z += (K(1) + sO2) * dF; Yc.w += K(0);
     Tc += div_rcp(dF * dHc, cp, icp);
     burned += dF;
   }
   // soot formation and oxidation (:122-127)
-  const float dSf = fminf(P.sootFormA * expf(-P.sootFormTa / Tc) * Yc.x * dt, Yc.x);
+  // (the same argument for the soot rates: a zero factor decides the product)
+  float ef = K(1), eo = K(1);
+  if (Yc.x != K(0)) ef = expf(-P.sootFormTa / Tc);
+  const float dSf = fminf(P.sootFormA * ef * Yc.x * dt, Yc.x);
   Yc.x += -dSf; Yc.y += K(0); Yc.z += K(0); Yc.w += dSf;
-  const float dSo = fminf(P.sootOxA * Yc.w * Yc.y * expf(-P.sootOxTa / Tc) * dt, fminf(Yc.w, div_rcp(Yc.y, sO2, isO2)));
+  if (Yc.w != K(0) && Yc.y != K(0)) eo = expf(-P.sootOxTa / Tc);
+  const float dSo = fminf(P.sootOxA * Yc.w * Yc.y * eo * dt, fminf(Yc.w, div_rcp(Yc.y, sO2, isO2)));
   Yc.x += K(0); Yc.y += -sO2 * dSo; Yc.z += (K(1) + sO2) * dSo; Yc.w += -dSo;
   Tc += div_rcp(dSo * dHc, cp, icp);
   burned += dSo;
