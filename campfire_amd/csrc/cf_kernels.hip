@@ -453,8 +453,20 @@ __global__ __launch_bounds__(256) void k_acc_finish(const int* __restrict__ part
 __device__ __forceinline__ float Tcl(const float* __restrict__ T, int nx, int ny, int i, int j) {
   return T[iclamp(j, 0, ny - 1) * nx + iclamp(i, 0, nx - 1)];
 }
-__device__ __forceinline__ float exchangef(const cf_params& P, float spd, float Tf) {
-  return expf(K(-2.0) * P.exchAlpha * spd * (Tf / P.Tamb) / P.depth * P.dt);
+// reciprocals of the per-launch constants the momentum and coefficient
+// updates divide by: their quotients are correctly rounded through div_rcp
+struct MomRecip {
+  float iTamb, iDepth, iRT;  // 1/Tamb, 1/depth, 1/(rhoAmb Tamb)
+};
+__device__ __forceinline__ MomRecip mom_recip(const cf_params& P) {
+  return MomRecip{K(1) / P.Tamb, K(1) / P.depth, K(1) / (P.rhoAmb * P.Tamb)};
+}
+__device__ __forceinline__ float exchangef(const cf_params& P, float spd, float Tf, const MomRecip& r) {
+  return expf(div_rcp(K(-2.0) * P.exchAlpha * spd * div_rcp(Tf, P.Tamb, r.iTamb), P.depth, r.iDepth) * P.dt);
+}
+// invRho of the closures with the reciprocal of rhoAmb Tamb
+__device__ __forceinline__ float invRhoR(const cf_params& P, float T, const MomRecip& r) {
+  return div_rcp(T, P.rhoAmb * P.Tamb, r.iRT);
 }
 // Momentum updates, written against an accessor so the same arithmetic runs
 // from global memory (k_forces) or from an LDS tile (k_forces_coef).
@@ -465,6 +477,8 @@ struct GlobalAcc {
   const float *T, *U2, *V2, *fx, *fy;
   int nx, ny;
   __device__ float Tc(int i, int j) const { return T[iclamp(j, 0, ny - 1) * nx + iclamp(i, 0, nx - 1)]; }
+  // viscosity of the cell's temperature
+  __device__ float mu(const cf_params& P, int i, int j) const { return muOf(P, Tc(i, j)); }
   __device__ float U(int i, int j) const { return U2[j * (nx + 1) + i]; }
   __device__ float V(int i, int j) const { return V2[j * nx + i]; }
   __device__ float FX(int i, int j) const { return fx[j * (nx + 1) + i]; }
@@ -473,7 +487,7 @@ struct GlobalAcc {
 
 // new U at face (i, j), 0 <= i <= nx, 0 <= j < ny (shaders/forces.wgsl:44-71)
 template <class A>
-__device__ __forceinline__ float force_u(const cf_params& P, const A& a, int i, int j) {
+__device__ __forceinline__ float force_u(const cf_params& P, const A& a, int i, int j, const MomRecip& rc) {
   const int nx = P.nx, ny = P.ny;
   const float dt = P.dt;
   const float idx2 = K(1) / (P.dx * P.dx);
@@ -484,19 +498,19 @@ __device__ __forceinline__ float force_u(const cf_params& P, const A& a, int i, 
   const float u = a.U(i, j);
   const float TL = a.Tc(i - 1, j), TR = a.Tc(i, j);
   const float rhoF = rhoOf(P, K(0.5) * (TL + TR));
-  const float muE = muOf(P, TR), muW = muOf(P, TL);
+  const float muE = a.mu(P, i, j), muW = a.mu(P, i - 1, j);
   float uN = u, muN = muOf(P, K(0.5) * (TL + TR));
   if (j < ny - 1) { uN = u2(i, j + 1); muN = muOf(P, K(0.25) * (TL + TR + a.Tc(i - 1, j + 1) + a.Tc(i, j + 1))); }
   float uS = -u, muS = muOf(P, K(0.5) * (TL + TR));
   if (j > 0) { uS = u2(i, j - 1); muS = muOf(P, K(0.25) * (TL + TR + a.Tc(i - 1, j - 1) + a.Tc(i, j - 1))); }
   const float visc = (muE * (u2(i + 1, j) - u) + muW * (u2(i - 1, j) - u) + muN * (uN - u) + muS * (uS - u)) * idx2 / rhoF;
   const float vAvg = K(0.25) * (a.V(i - 1, j) + a.V(i, j) + a.V(i - 1, j + 1) + a.V(i, j + 1));
-  return (u + dt * visc) * exchangef(P, sqrtf(u * u + vAvg * vAvg), K(0.5) * (TL + TR));
+  return (u + dt * visc) * exchangef(P, sqrtf(u * u + vAvg * vAvg), K(0.5) * (TL + TR), rc);
 }
 
 // new V at face (i, j), 0 <= i < nx, 0 <= j <= ny (shaders/forces.wgsl:72-100)
 template <class A>
-__device__ __forceinline__ float force_v(const cf_params& P, const A& a, int i, int j) {
+__device__ __forceinline__ float force_v(const cf_params& P, const A& a, int i, int j, const MomRecip& rc) {
   const int nx = P.nx, ny = P.ny;
   const float dt = P.dt;
   const float idx2 = K(1) / (P.dx * P.dx);
@@ -507,29 +521,29 @@ __device__ __forceinline__ float force_v(const cf_params& P, const A& a, int i, 
   const float TB = a.Tc(i, j - 1), TT = a.Tc(i, j);
   const float Tf = K(0.5) * (TB + TT);
   const float rhoF = rhoOf(P, Tf);
-  const float muN = muOf(P, TT), muS = muOf(P, TB);
+  const float muN = a.mu(P, i, j), muS = a.mu(P, i, j - 1);
   float vE = v, muE = muOf(P, Tf);
   if (i < nx - 1) { vE = v2(i + 1, j); muE = muOf(P, K(0.25) * (TB + TT + a.Tc(i + 1, j - 1) + a.Tc(i + 1, j))); }
   float vW = v, muW = muOf(P, Tf);
   if (i > 0) { vW = v2(i - 1, j); muW = muOf(P, K(0.25) * (TB + TT + a.Tc(i - 1, j - 1) + a.Tc(i - 1, j))); }
   const float visc = (muN * (v2(i, j + 1) - v) + muS * (v2(i, j - 1) - v) + muE * (vE - v) + muW * (vW - v)) * idx2 / rhoF;
-  const float buoy = P.grav * (Tf / P.Tamb - K(1));
+  const float buoy = P.grav * (div_rcp(Tf, P.Tamb, rc.iTamb) - K(1));
   const float uAvg = K(0.25) * (a.U(i, j - 1) + a.U(i + 1, j - 1) + a.U(i, j) + a.U(i + 1, j));
-  return (v + dt * (visc + buoy)) * exchangef(P, sqrtf(uAvg * uAvg + v * v), Tf);
+  return (v + dt * (visc + buoy)) * exchangef(P, sqrtf(uAvg * uAvg + v * v), Tf, rc);
 }
 
 // face coefficients a = f / rho (shaders/pressure.wgsl:30-43)
 template <class A>
-__device__ __forceinline__ float coef_u(const cf_params& P, const A& a, int i, int j) {
+__device__ __forceinline__ float coef_u(const cf_params& P, const A& a, int i, int j, const MomRecip& rc) {
   const int nx = P.nx;
   const float Tf = K(0.5) * (a.Tc(iclamp(i - 1, 0, nx - 1), j) + a.Tc(min(i, nx - 1), j));
-  return a.FX(i, j) * invRho(P, Tf);
+  return a.FX(i, j) * invRhoR(P, Tf, rc);
 }
 template <class A>
-__device__ __forceinline__ float coef_v(const cf_params& P, const A& a, int i, int j) {
+__device__ __forceinline__ float coef_v(const cf_params& P, const A& a, int i, int j, const MomRecip& rc) {
   const int ny = P.ny;
   const float Tf = K(0.5) * (a.Tc(i, iclamp(j - 1, 0, ny - 1)) + a.Tc(i, min(j, ny - 1)));
-  return a.FY(i, j) * invRho(P, Tf);
+  return a.FY(i, j) * invRhoR(P, Tf, rc);
 }
 
 __global__ __launch_bounds__(256) void k_forces(Fields F) {
@@ -538,8 +552,9 @@ __global__ __launch_bounds__(256) void k_forces(Fields F) {
   int i, j;
   tile_ij<32, 8>(nx + 1, ny + 1, i, j);
   const GlobalAcc a{F.T, F.U2, F.V2, F.fx, F.fy, nx, ny};
-  if (i <= nx && j < ny) F.U[j * (nx + 1) + i] = force_u(P, a, i, j);
-  if (i < nx && j <= ny) F.V[j * nx + i] = force_v(P, a, i, j);
+  const MomRecip rc = mom_recip(P);
+  if (i <= nx && j < ny) F.U[j * (nx + 1) + i] = force_u(P, a, i, j, rc);
+  if (i < nx && j <= ny) F.V[j * nx + i] = force_v(P, a, i, j, rc);
 }
 
 // Fused momentum update + projection set-up on 32x32-cell tiles:
@@ -561,9 +576,10 @@ constexpr int FC_UX = FC_T + 3;          // U faces [x0-1, x0+33] x rows [y0-1, 
 constexpr int FC_VY = FC_T + 3;          // V faces cols [x0-1, x0+32] x rows [y0-1, y0+33]
 
 struct TileAcc {
-  const float *T, *U2, *V2, *fx, *fy;    // LDS, region-relative
+  const float *T, *MU, *U2, *V2, *fx, *fy;  // LDS, region-relative; MU: muOf of T
   int x0, y0;
   __device__ float Tc(int i, int j) const { return T[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
+  __device__ float mu(const cf_params&, int i, int j) const { return MU[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
   __device__ float U(int i, int j) const { return U2[(j - y0 + 1) * FC_UX + (i - x0 + 1)]; }
   __device__ float FX(int i, int j) const { return fx[(j - y0 + 1) * FC_UX + (i - x0 + 1)]; }
   __device__ float V(int i, int j) const { return V2[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
@@ -586,22 +602,41 @@ struct ForcesCoefOp {
     int nparts = a_.nparts;
 
   constexpr int TX = FC_T, TY = FC_T;
-  __shared__ float sT[FC_R * FC_R];
-  __shared__ float sU[FC_R * FC_UX], sfx[FC_R * FC_UX];
-  __shared__ float sV[FC_VY * FC_R], sfy[FC_VY * FC_R];
-  __shared__ float su[TY][TX + 1], sv[TY + 1][TX];
-  __shared__ float sax[TY][TX + 1], say[TY + 1][TX];
+  // One LDS pool: the staged T, its viscosity (muOf once per cell rather
+  // than once per adjacent face), U2 and V2, all dead once the faces are
+  // computed, then fx and fy.  The faces' forces and coefficients are kept
+  // in registers and stored over the dead region, the forces for rhs0 and
+  // then the coefficients for the cascade, so the tile needs 28 KB and five
+  // workgroups share a CU.
+  constexpr int OFF_MU = FC_R * FC_R, OFF_U = 2 * FC_R * FC_R, OFF_V = OFF_U + FC_R * FC_UX,
+                OFF_FX = OFF_V + FC_VY * FC_R, OFF_FY = OFF_FX + FC_R * FC_UX, POOL = OFF_FY + FC_VY * FC_R;
+  constexpr int NFU = (TX + 1) * TY, NFV = TX * (TY + 1), RF = (NFU + 255) / 256;
+  static_assert(NFU == NFV && 2 * NFU <= OFF_FX && 3 * 256 <= POOL, "face arrays fit over the staged inputs");
+  __shared__ float pool[POOL];
+  float* sT = pool;
+  float* sMU = pool + OFF_MU;
+  float* sU = pool + OFF_U;
+  float* sV = pool + OFF_V;
+  float* sfx = pool + OFF_FX;
+  float* sfy = pool + OFF_FY;
+  float (*su)[TX + 1] = reinterpret_cast<float (*)[TX + 1]>(pool);
+  float (*sv)[TX] = reinterpret_cast<float (*)[TX]>(pool + NFU);
+  float (*sax)[TX + 1] = reinterpret_cast<float (*)[TX + 1]>(pool);
+  float (*say)[TX] = reinterpret_cast<float (*)[TX]>(pool + NFU);
   const cf_params P = *F.P;
+  const MomRecip rc = mom_recip(P);
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;  // 32 x 8 threads
-  if (vb == 0 && nparts > 0) acc_fold(part, nparts, F.acc, (int*)&su[0][0]);
+  if (vb == 0 && nparts > 0) acc_fold(part, nparts, F.acc, (int*)pool);
   const int nx = P.nx, ny = P.ny;
-  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
+  const int gx = (nx + TX - 1) / TX;
   const int t = vb;
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
   // ---- stage inputs (cells at clamped indices; faces outside the domain read 0, never used)
   for (int q = tid; q < FC_R * FC_R; q += 256) {
     const int a = q % FC_R, b = q / FC_R;
-    sT[q] = F.T[iclamp(y0 - 1 + b, 0, ny - 1) * nx + iclamp(x0 - 1 + a, 0, nx - 1)];
+    const float tv = F.T[iclamp(y0 - 1 + b, 0, ny - 1) * nx + iclamp(x0 - 1 + a, 0, nx - 1)];
+    sT[q] = tv;
+    sMU[q] = muOf(P, tv);
   }
   for (int q = tid; q < FC_R * FC_UX; q += 256) {
     const int a = q % FC_UX, b = q / FC_UX;
@@ -618,26 +653,39 @@ struct ForcesCoefOp {
     sfy[q] = in ? F.fy[j * nx + i] : 0.f;
   }
   __syncthreads();
-  const TileAcc acc{sT, sU, sV, sfx, sfy, x0, y0};
-  // ---- U faces [x0, x0+32] x [y0, y0+32): forces and coefficients
-  for (int q = tid; q < (TX + 1) * TY; q += 256) {
-    const int li = q % (TX + 1), lj = q / (TX + 1);
-    const int i = x0 + li, j = y0 + lj;
-    if (i <= nx && j < ny) {
-      const float u = force_u(P, acc, i, j), a = coef_u(P, acc, i, j);
-      su[lj][li] = u; sax[lj][li] = a;
-      if (li < TX || i == nx) { F.U[j * (nx + 1) + i] = u; F.ax0[j * (nx + 1) + i] = a; }
+  const TileAcc acc{sT, sMU, sU, sV, sfx, sfy, x0, y0};
+  // ---- U faces [x0, x0+32] x [y0, y0+32) and V faces [x0, x0+32) x
+  // [y0, y0+32]: forces and coefficients (face r of this thread: q = tid + 256 r)
+  float fu[RF], fa[RF], fv[RF], fb[RF];
+#pragma unroll
+  for (int r = 0; r < RF; r++) {
+    const int q = tid + 256 * r;
+    fu[r] = fa[r] = fv[r] = fb[r] = 0.f;
+    if (q < NFU) {
+      const int li = q % (TX + 1), lj = q / (TX + 1);
+      const int i = x0 + li, j = y0 + lj;
+      if (i <= nx && j < ny) {
+        const float u = force_u(P, acc, i, j, rc), a = coef_u(P, acc, i, j, rc);
+        fu[r] = u; fa[r] = a;
+        if (li < TX || i == nx) { F.U[j * (nx + 1) + i] = u; F.ax0[j * (nx + 1) + i] = a; }
+      }
+    }
+    if (q < NFV) {
+      const int li = q % TX, lj = q / TX;
+      const int i = x0 + li, j = y0 + lj;
+      if (i < nx && j <= ny) {
+        const float v = force_v(P, acc, i, j, rc), b = coef_v(P, acc, i, j, rc);
+        fv[r] = v; fb[r] = b;
+        if (lj < TY || j == ny) { F.V[j * nx + i] = v; F.ay0[j * nx + i] = b; }
+      }
     }
   }
-  // ---- V faces [x0, x0+32) x [y0, y0+32]
-  for (int q = tid; q < TX * (TY + 1); q += 256) {
-    const int li = q % TX, lj = q / TX;
-    const int i = x0 + li, j = y0 + lj;
-    if (i < nx && j <= ny) {
-      const float v = force_v(P, acc, i, j), b = coef_v(P, acc, i, j);
-      sv[lj][li] = v; say[lj][li] = b;
-      if (lj < TY || j == ny) { F.V[j * nx + i] = v; F.ay0[j * nx + i] = b; }
-    }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RF; r++) {
+    const int q = tid + 256 * r;
+    if (q < NFU) su[q / (TX + 1)][q % (TX + 1)] = fu[r];
+    if (q < NFV) sv[q / TX][q % TX] = fv[r];
   }
   __syncthreads();
   // ---- rhs0 (:46-54)
@@ -650,6 +698,15 @@ struct ForcesCoefOp {
       const float div = (fe * su[lj][li + 1] - fw * su[lj][li] + fn * sv[lj + 1][li] - fs * sv[lj][li]) / P.dx;
       const float open = fe + fw + fn + fs;
       F.rhs0[j * nx + i] = open > K(0) ? (div - F.SQ[j * nx + i].x) / P.dt : K(0);
+    }
+  }
+  if (CL.ncascade > 0) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RF; r++) {
+      const int q = tid + 256 * r;
+      if (q < NFU) sax[q / (TX + 1)][q % (TX + 1)] = fa[r];
+      if (q < NFV) say[q / TX][q % TX] = fb[r];
     }
   }
   // ---- cascade of coefficient restrictions (:57-67)
