@@ -13,6 +13,7 @@
 #include <string.h>
 #include <string>
 #include <array>
+#include <algorithm>
 #include <vector>
 
 #include "cf_kernels.h"
@@ -663,7 +664,13 @@ static int pool_build_step(cf_pool* pl) {
   // a leg's kernel depends on its tile size (from the level's size) and on
   // the storage of its two levels (FP16 smoother: whether the coarse level is
   // tiled too), so each level's legs form one phase per (tile, storage)
-  const int modes[3] = {0, 1, 3}, tiles[2] = {32, 16};
+  const int modes[3] = {0, 1, 3};
+  std::vector<int> tiles;  // every tile size a scene's level uses
+  for (auto* s : sc)
+    for (int d = 0; d < s->mg_bottom; d++) {
+      const int t = cf::mg_tile(s->L[d]);
+      if (std::find(tiles.begin(), tiles.end(), t) == tiles.end()) tiles.push_back(t);
+    }
   auto leg_ok = [&](cf_sim* s, int d, int t, int m) {
     return d < s->mg_bottom && cf::mg_tile(s->L[d]) == t && mg_store(s, d).mode == m;
   };
