@@ -107,6 +107,7 @@ def lib():
         "cf_sim_profile_kernels": ([p, i, i, vp, vp, vp, C.POINTER(i)], i),
         "cf_sim_step_latencies": ([p, i, vp], i),
         "cf_sim_synchronize": ([p], i),
+        "cf_sim_status": ([p], i),
         "cf_sim_encode_readback": ([p], i),
         "cf_sim_take_readback": ([p, i, i, vp, vp, vp], i),
         "cf_sim_field_bytes": ([p, cp, C.POINTER(sz)], i),
@@ -128,6 +129,7 @@ def lib():
         "cf_pool_frame": ([p, i, i], i),
         "cf_pool_synchronize": ([p], i),
         "cf_pool_get_info": ([p, C.POINTER(CfPoolInfo)], i),
+        "cf_pool_status": ([p, C.POINTER(i)], i),
         "cf_pool_step_latencies": ([p, i, vp], i),
         "cf_last_error": ([], cp),
         "cf_version": ([], cp),

@@ -136,6 +136,9 @@ struct Dyn {
   double kbb, kdq, kalpha, kbeta;
   double kbest;        // Krylov: lowest relative residual of this solve
   int32_t kfall;       // Krylov: fallen back to defect correction for this solve
+  // non-finite values seen since the host last read this (bit 0: T or Y
+  // entering the sources pass, bit 1: U or V in the CFL pass)
+  int32_t nonfinite;
 };
 
 // Every device buffer of one scene (the `buf` dictionary of js/fluid.js:52-68

@@ -352,10 +352,11 @@ struct SourcesOp {
   __shared__ int sacc[CF_ACC_GLOBAL];
   __shared__ int sglob[4][3];
   __shared__ float stamb4;
+  __shared__ int sbad;
   const int tid = threadIdx.y * 32 + threadIdx.x;
   const cf_params P = *F.P;
   for (int q = tid; q < CF_ACC_GLOBAL; q += 256) sacc[q] = 0;
-  if (tid == 0) stamb4 = powf(P.Tamb, K(4.0));  // one wave evaluates the power
+  if (tid == 0) { stamb4 = powf(P.Tamb, K(4.0)); sbad = 0; }  // one wave evaluates the power
   __syncthreads();
   const float tamb4 = stamb4;
   const int nx = P.nx, ny = P.ny;
@@ -368,6 +369,8 @@ struct SourcesOp {
     float4 Yc;
     if (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, Tc, Yc);
     else { Tc = F.T[k]; Yc = F.Y[k]; }
+    // the clamps below would hide a NaN (max(NaN, x) = x), so the inputs are checked
+    if (!(isfinite(Tc) && isfinite(Yc.x) && isfinite(Yc.y) && isfinite(Yc.z) && isfinite(Yc.w))) sbad = 1;
     float To;
     float4 Yo;
     float2 sq;
@@ -404,6 +407,7 @@ struct SourcesOp {
     part[4 * vb + 0] = (int)sq_;
     part[4 * vb + 1] = (int)sr_;
     part[4 * vb + 2] = mt;
+    part[4 * vb + 3] = sbad;
   }
 
   }
@@ -415,18 +419,21 @@ __global__ __launch_bounds__(256) void k_sources(const typename SourcesOp<FUSE>:
 
 // Folds the per-workgroup partials of k_sources into acc[512..514]: integer
 // sums with two's-complement wrap-around and a max, so the order is irrelevant.
+// The partials' non-finite flags go into dyn->nonfinite (bit 0).
 // Called by 256 threads; `scratch` holds 3 x 256 ints of LDS.
-__device__ __forceinline__ void acc_fold(const int* __restrict__ part, int nparts, int* __restrict__ acc, int* scratch) {
+__device__ __forceinline__ void acc_fold(const int* __restrict__ part, int nparts, int* __restrict__ acc, Dyn* dyn,
+                                         int* scratch) {
   const int t = threadIdx.y * blockDim.x + threadIdx.x;
   unsigned* sq = (unsigned*)scratch;
   unsigned* sr = sq + 256;
   int* st = (int*)(sr + 256);
   unsigned a = 0, b = 0;
-  int m = INT_MIN;
+  int m = INT_MIN, bad = 0;
   for (int q = t; q < nparts; q += 256) {
     a += (unsigned)part[4 * q];
     b += (unsigned)part[4 * q + 1];
     m = max(m, part[4 * q + 2]);
+    bad |= part[4 * q + 3];
   }
   sq[t] = a; sr[t] = b; st[t] = m;
   __syncthreads();
@@ -439,12 +446,14 @@ __device__ __forceinline__ void acc_fold(const int* __restrict__ part, int npart
     atomicAdd(&acc[CF_ACC_GLOBAL + 1], (int)sr[0]);
     atomicMax(&acc[CF_ACC_GLOBAL + 2], st[0]);
   }
+  if (bad) atomicOr(&dyn->nonfinite, 1);
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_acc_finish(const int* __restrict__ part, int nparts, int* __restrict__ acc) {
+__global__ __launch_bounds__(256) void k_acc_finish(const int* __restrict__ part, int nparts, int* __restrict__ acc,
+                                                    Dyn* dyn) {
   __shared__ int scratch[3 * 256];
-  acc_fold(part, nparts, acc, scratch);
+  acc_fold(part, nparts, acc, dyn, scratch);
 }
 
 // -------------------------------------------------------------------------
@@ -626,7 +635,7 @@ struct ForcesCoefOp {
   const cf_params P = *F.P;
   const MomRecip rc = mom_recip(P);
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;  // 32 x 8 threads
-  if (vb == 0 && nparts > 0) acc_fold(part, nparts, F.acc, (int*)pool);
+  if (vb == 0 && nparts > 0) acc_fold(part, nparts, F.acc, F.dyn, (int*)pool);
   const int nx = P.nx, ny = P.ny;
   const int gx = (nx + TX - 1) / TX;
   const int t = vb;
@@ -887,23 +896,27 @@ struct CflOp {
     float umin = a_.umin;
     int nsub = a_.nsub;
 
-  __shared__ float wm[4];
+  __shared__ unsigned wm[4];
   __shared__ int last;
   const cf_params P = *F.P;
   const int nx = P.nx, ny = P.ny;
   const int nU = (nx + 1) * ny, nV = nx * (ny + 1), n = nU > nV ? nU : nV;
   const int tid = threadIdx.x;
-  float m = K(0);
+  // max |u| as the max of the bit patterns of |u|: the same order for finite
+  // values, and Inf and NaN (above every finite pattern) are not dropped as
+  // fmaxf would drop a NaN
+  unsigned m = 0u;
   for (int k = vb * 256 + tid; k < n; k += a_.vg * 256) {
-    if (k < nU) m = fmaxf(m, fabsf(F.U[k]));
-    if (k < nV) m = fmaxf(m, fabsf(F.V[k]));
+    if (k < nU) m = max(m, __float_as_uint(fabsf(F.U[k])));
+    if (k < nV) m = max(m, __float_as_uint(fabsf(F.V[k])));
   }
-  m = wave_tree_max(m);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
   if ((tid & 63) == 0) wm[tid >> 6] = m;
   __syncthreads();
   if (tid == 0) {
-    const float b = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
-    atomicMax(&F.dyn->umax_bits, __float_as_uint(b));
+    const unsigned b = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+    atomicMax(&F.dyn->umax_bits, b);
     __threadfence();
     last = atomicAdd(&F.dyn->cfl_done, 1u) == a_.vg - 1;
   }
@@ -911,7 +924,9 @@ struct CflOp {
   if (last && tid == 0) {
     __threadfence();
     Dyn* d = F.dyn;
-    const float umax = __uint_as_float(atomicOr(&d->umax_bits, 0u));
+    const unsigned ub = atomicOr(&d->umax_bits, 0u);
+    if (ub >= 0x7f800000u) atomicOr(&d->nonfinite, 2);  // Inf or NaN
+    const float umax = __uint_as_float(ub);
     const float dt = cfl * P.dx / fmaxf(umax, umin);
     ((cf_params*)F.P)->dt = dt;
     d->dt = dt;
@@ -1914,7 +1929,7 @@ void launch_sources(const Fields& F, bool fuse, const float* Tin, const float4* 
   const unsigned nb = nblk(F.nx, F.ny);
   if (fuse) launch(k_sources<true>, dim3(nb), B328, 0, s, SourcesOp<true>::Args{F, Tin, Yin, part, (int)nb});
   else launch(k_sources<false>, dim3(nb), B328, 0, s, SourcesOp<false>::Args{F, Tin, Yin, part, (int)nb});
-  if (finish) launch(k_acc_finish, dim3(1), dim3(256), 0, s, (const int*)part, (int)nb, F.acc);
+  if (finish) launch(k_acc_finish, dim3(1), dim3(256), 0, s, (const int*)part, (int)nb, F.acc, F.dyn);
 }
 void launch_forces(const Fields& F, hipStream_t s) {
   launch(k_forces, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);

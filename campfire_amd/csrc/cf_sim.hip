@@ -377,7 +377,10 @@ static void seg_plan(int vcycles, int* h, int* r) {
   *r = (vcycles - *h) % SEG_CHUNK;
 }
 
+// A graph exec must not be destroyed while a launch of it may still run: the
+// runtime (and a profiler's dispatch records) still refer to its kernel nodes.
 static void drop_graph(cf_sim* s) {
+  if ((s->gexec || s->gexec8 || s->seg.head) && s->st) hipStreamSynchronize(s->st);
   if (s->gexec) hipGraphExecDestroy(s->gexec);
   if (s->gexec8) hipGraphExecDestroy(s->gexec8);
   s->gexec = s->gexec8 = nullptr;
@@ -539,6 +542,9 @@ struct cf_pool {
   bool use_graph = true;
   int nlaunch = 0;
   int64_t steps = 0;
+  // bumped by every admit and release: a new scene may reuse a released one's
+  // host address, so the table keys include it besides the slot pointers
+  uint64_t gen = 0;
 };
 
 // ---- scene pool: many scenes stepped together ---------------------------------
@@ -570,6 +576,7 @@ static void pool_release_phases(std::vector<cf::BatchPhase>& v) {
 }
 
 static void pool_drop_graph(cf_pool* pl) {
+  if ((pl->g1 || pl->g8 || pl->seg.head) && pl->st) hipStreamSynchronize(pl->st);  // see drop_graph
   if (pl->g1) hipGraphExecDestroy(pl->g1);
   if (pl->g8) hipGraphExecDestroy(pl->g8);
   pl->g1 = pl->g8 = nullptr;
@@ -578,8 +585,8 @@ static void pool_drop_graph(cf_pool* pl) {
 
 // the batched step of the live scenes, in the order of enqueue_step
 static int pool_build_step(cf_pool* pl) {
+  if (pl->st) hipStreamSynchronize(pl->st);  // the old graphs and tables may still be in use
   pool_drop_graph(pl);
-  if (pl->st) hipStreamSynchronize(pl->st);  // the old tables may still be read
   pool_release_phases(pl->step);
   std::vector<cf_sim*> sc;
   for (auto* s : pl->slots) if (s) sc.push_back(s);
@@ -802,7 +809,7 @@ static int pool_run_segments(cf_pool* pl) {
 // the batched step for the current scenes, rebuilt when scenes come or go
 // or a scene's substep count changes
 static int pool_ready(cf_pool* pl) {
-  uint64_t key = 1469598103934665603ull;
+  uint64_t key = (1469598103934665603ull ^ pl->gen) * 1099511628211ull;
   for (auto* s : pl->slots) {
     key = (key ^ (uint64_t)(uintptr_t)s) * 1099511628211ull;
     if (s) key = (key ^ (uint64_t)s->diffSub) * 1099511628211ull;
@@ -1024,6 +1031,7 @@ int cf_sim_reset(cf_sim* s) {  // js/fluid.js:162-176
   cf::launch_fill_u32(s->dP64, 0, (size_t)s->nC * 2, s->st);
   s->time = 0;
   HIPCHK(hipMemsetAsync(&s->dDyn->time, 0, sizeof(double), s->st));
+  cf::launch_fill_u32(&s->dDyn->nonfinite, 0u, 1, s->st);
   HIPCHK(hipStreamSynchronize(s->st));  // the staging vector goes out of scope
   return CF_OK;
 }
@@ -1079,8 +1087,8 @@ int cf_sim_get_packed_params(const cf_sim* s, cf_params* out) {
   return CF_OK;
 }
 
-int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
-  if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
+// the rules a schedule must meet on scene s (CF_OK or CF_EINVAL with a message)
+static int sched_check(const cf_sim* s, const cf_schedule* sc) {
   if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 0 || !(sc->tol >= 0))
     return set_err(CF_EINVAL, "bad schedule (nu 1..3, vcycles >= 1, coarse_iters >= 0)");
   if (sc->coarse_iters == 0 && (s->lnx[s->nlev - 1] > 32 || s->mg_simple))
@@ -1089,15 +1097,31 @@ int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
     return set_err(CF_EINVAL, "bad schedule (cfl >= 0; umin > 0 with a CFL step)");
   if (sc->refine && !(sc->tol > 0)) return set_err(CF_EINVAL, "bad schedule (refine needs tol > 0)");
   if (sc->krylov && !sc->refine) return set_err(CF_EINVAL, "bad schedule (krylov needs refine = 1)");
-  // switching defect correction on: the FP64 pressure starts from the FP32 one
   if (sc->smoother_fp16 && (!sc->refine || s->mg_simple))
     return set_err(CF_EINVAL, "FP16 smoother needs defect correction (refine = 1, tol > 0) and the fused path");
+  return CF_OK;
+}
+
+// applies a schedule that sched_check accepted
+static int sched_apply(cf_sim* s, const cf_schedule* sc) {
+  // switching defect correction on: the FP64 pressure starts from the FP32 one
   if (sc->refine && !s->sch.refine) cf::launch_p32_to_64(s->L[0].p, s->dP64, s->nC, s->st);
   const bool cfl_changed = sc->cfl != s->sch.cfl || sc->umin != s->sch.umin;
   s->sch = *sc;
   // the diffusion substep count follows the largest dt of a CFL rule
   if (cfl_changed) return cf_sim_write_params(s);
   return CF_OK;
+}
+
+int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
+  if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
+  // a pool's batched step is built from the pool's schedule: a scene of a
+  // pool takes it from cf_pool_set_schedule only
+  if (s->in_pool && memcmp(sc, &s->sch, sizeof(*sc)))
+    return set_err(CF_EINVAL, "scene belongs to a pool: set the schedule with cf_pool_set_schedule");
+  int rc = sched_check(s, sc);
+  if (rc) return rc;
+  return sched_apply(s, sc);
 }
 
 int cf_sim_get_schedule(const cf_sim* s, cf_schedule* sc) {
@@ -1358,6 +1382,28 @@ int cf_sim_get_info(const cf_sim* s, cf_sim_info* info) {
   return CF_OK;
 }
 
+// the status of the steps run so far; reads and clears the non-finite flag
+static int sim_status(cf_sim* s) {
+  cf::Dyn d;
+  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpy(&d, s->dDyn, sizeof(d), hipMemcpyDeviceToHost));
+  if (d.nonfinite) {
+    cf::launch_fill_u32(&s->dDyn->nonfinite, 0u, 1, s->st);
+    HIPCHK(hipStreamSynchronize(s->st));
+    return set_err(CF_ENONFINITE, "non-finite values in the fields of the %dx%d scene (%s%s%s)", s->nx, s->ny,
+                   d.nonfinite & 1 ? "T or Y" : "", d.nonfinite == 3 ? ", " : "", d.nonfinite & 2 ? "U or V" : "");
+  }
+  if (s->sch.tol > 0 && !d.converged)
+    return set_err(CF_ENOCONV, "residual %.3g above the target %.3g after %d V-cycles (%dx%d scene)", d.resid,
+                   s->sch.tol, d.cycles, s->nx, s->ny);
+  return CF_OK;
+}
+
+int cf_sim_status(cf_sim* s) {
+  if (!s) return set_err(CF_EINVAL, "sim is NULL");
+  return sim_status(s);
+}
+
 int cf_sim_div_error(cf_sim* s, float* max_err, float* mean_err) {
   if (!s || !max_err || !mean_err) return set_err(CF_EINVAL, "NULL argument");
   const unsigned nb = cf::div_error_blocks(s->F);
@@ -1481,8 +1527,11 @@ int cf_pool_set_schedule(cf_pool* pl, const cf_schedule* sc) {
   if (sc->nu < 1 || sc->nu > 3 || sc->vcycles < 1 || sc->coarse_iters < 0 || !(sc->tol >= 0) || !(sc->cfl >= 0) ||
       (sc->refine && !(sc->tol > 0)) || (sc->smoother_fp16 && !sc->refine) || (sc->krylov && !sc->refine))
     return set_err(CF_EINVAL, "bad schedule");
+  // every scene must accept it before any of them changes
   for (auto* s : pl->slots)
-    if (s) { int rc = cf_sim_set_schedule(s, sc); if (rc) return rc; }
+    if (s) { int rc = sched_check(s, sc); if (rc) return rc; }
+  for (auto* s : pl->slots)
+    if (s) { int rc = sched_apply(s, sc); if (rc) return rc; }
   pl->sch = *sc;
   pl->dirty = true;
   return CF_OK;
@@ -1498,15 +1547,17 @@ int cf_pool_admit(cf_pool* pl, const cf_scene_desc* d, int* slot) {
   cf_sim* s = nullptr;
   int rc = sim_create(d->nx, d->ny, d->dx, d->dt, pl->st, &s);
   if (rc) return rc;
-  s->in_pool = true;
-  rc = cf_sim_set_schedule(s, &pl->sch);
+  rc = sched_check(s, &pl->sch);
+  if (!rc) rc = sched_apply(s, &pl->sch);
   if (rc) { cf_sim_destroy(s); return rc; }
+  s->in_pool = true;
   int q = 0;
   while (q < (int)pl->slots.size() && pl->slots[q]) q++;
   if (q == (int)pl->slots.size()) pl->slots.push_back(nullptr);
   pl->slots[q] = s;
   pl->used += s->block_bytes;
   pl->dirty = true;
+  pl->gen++;
   *slot = q;
   return CF_OK;
 }
@@ -1526,6 +1577,10 @@ int cf_pool_release(cf_pool* pl, int slot) {
   cf_sim_destroy(pl->slots[slot]);
   pl->slots[slot] = nullptr;
   pl->dirty = true;
+  pl->gen++;
+  // the per-frame tables hold the released scene's device pointers
+  pool_release_phases(pl->frame_pre);
+  pool_release_phases(pl->frame_post);
   return CF_OK;
 }
 
@@ -1563,8 +1618,8 @@ int cf_pool_frame(cf_pool* pl, int nsteps, int with_radiation) {
   for (auto* s : pl->slots) if (s) sc.push_back(s);
   if (sc.empty()) return CF_OK;
   for (auto* s : sc) { int rc = cf_sim_write_params(s); if (rc) return rc; }
-  // per-frame tables: rebuilt when the piece counts change
-  uint64_t key = 1469598103934665603ull;
+  // per-frame tables: rebuilt when the piece counts or the scenes change
+  uint64_t key = (1469598103934665603ull ^ pl->gen) * 1099511628211ull;
   for (auto* s : pl->slots) {
     key = (key ^ (uint64_t)(uintptr_t)s) * 1099511628211ull;
     if (s) key = (key ^ (uint64_t)s->P.nPieces) * 1099511628211ull;
@@ -1596,6 +1651,27 @@ int cf_pool_frame(cf_pool* pl, int nsteps, int with_radiation) {
 int cf_pool_synchronize(cf_pool* pl) {
   if (!pl) return set_err(CF_EINVAL, "pool is NULL");
   HIPCHK(hipStreamSynchronize(pl->st));
+  return CF_OK;
+}
+
+int cf_pool_status(cf_pool* pl, int* slot) {
+  if (!pl) return set_err(CF_EINVAL, "pool is NULL");
+  if (slot) *slot = -1;
+  HIPCHK(hipStreamSynchronize(pl->st));
+  int first = CF_OK;
+  char msg[512] = "";
+  for (size_t q = 0; q < pl->slots.size(); q++) {
+    if (!pl->slots[q]) continue;
+    const int rc = sim_status(pl->slots[q]);  // every scene's flag is read and cleared
+    if (rc && rc != CF_ENOCONV && rc != CF_ENONFINITE) return rc;
+    // a non-finite scene outranks one that only missed its residual target
+    if (rc && (first == CF_OK || (rc == CF_ENONFINITE && first == CF_ENOCONV))) {
+      first = rc;
+      snprintf(msg, sizeof(msg), "slot %d: %s", (int)q, g_err);
+      if (slot) *slot = (int)q;
+    }
+  }
+  if (first) return set_err(first, "%s", msg);
   return CF_OK;
 }
 

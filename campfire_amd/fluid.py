@@ -144,6 +144,12 @@ class FluidSim:
         s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine, krylov)
         check(L.lib().cf_sim_set_schedule(self._h, C.byref(s)), "set_schedule")
 
+    def get_schedule(self):
+        """The schedule as a dict of set_schedule's keyword arguments."""
+        s = L.CfSchedule()
+        check(L.lib().cf_sim_get_schedule(self._h, C.byref(s)), "get_schedule")
+        return {n: getattr(s, n) for n, _ in L.CfSchedule._fields_}
+
     def upload_pieces(self, data, n):  # js/fluid.js:201-204
         a = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
         if a.size != MAX_PIECES * PIECE_FLOATS:
@@ -197,6 +203,12 @@ class FluidSim:
 
     def synchronize(self):
         check(L.lib().cf_sim_synchronize(self._h), "synchronize")
+
+    def status(self):
+        """Waits for the enqueued steps; raises CampfireError with CF_ENONFINITE
+        (a NaN/Inf reached the fields; the flag is then cleared) or CF_ENOCONV
+        (a residual target was missed by the last step)."""
+        check(L.lib().cf_sim_status(self._h), "status")
 
     def frame_all(self, nsteps, with_radiation=True):
         """One app frame as js/main.js:266-283 encodes it."""

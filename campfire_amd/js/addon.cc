@@ -103,6 +103,7 @@ SIMPLE(ClearGeometry, cf_sim_clear_geometry, "clearGeometry")
 SIMPLE(Geometry, cf_sim_geometry, "encodeGeometry")
 SIMPLE(Radiation, cf_sim_radiation, "encodeRadiation")
 SIMPLE(Synchronize, cf_sim_synchronize, "synchronize")
+SIMPLE(Status, cf_sim_status, "status")
 
 // setParam(h, key, value)  (js/fluid.js:178)
 static napi_value SetParam(napi_env env, napi_callback_info info) {
@@ -332,6 +333,7 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"info", 0, Info, 0, 0, 0, napi_default, 0},
       {"profileSteps", 0, ProfileSteps, 0, 0, 0, napi_default, 0},
       {"synchronize", 0, Synchronize, 0, 0, 0, napi_default, 0},
+      {"status", 0, Status, 0, 0, 0, napi_default, 0},
       {"version", 0, Version, 0, 0, 0, napi_default, 0},
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);

@@ -108,4 +108,6 @@ export class FluidSim {
 
   writeBuffer(name, typedArray) { addon.writeField(this.h, name, typedArray); }
   synchronize() { addon.synchronize(this.h); }
+  // throws when a NaN/Inf reached the fields or a residual target was missed
+  status() { addon.status(this.h); }
 }

@@ -88,6 +88,12 @@ class ScenePool:
     def synchronize(self):
         L.check(L.lib().cf_pool_synchronize(self._h), "pool sync")
 
+    def status(self):
+        """cf_pool_status: raises CampfireError naming the first failing slot
+        (CF_ENONFINITE before CF_ENOCONV); every scene's flag is cleared."""
+        slot = C.c_int(-1)
+        L.check(L.lib().cf_pool_status(self._h, C.byref(slot)), "pool status")
+
     def info(self):
         i = L.CfPoolInfo()
         L.check(L.lib().cf_pool_get_info(self._h, C.byref(i)), "pool info")

@@ -141,6 +141,9 @@ int  cf_sim_get_packed_params(const cf_sim* sim, cf_params* out);
 /* PARAM_FIELDS name of field `index` (0..39), or NULL. */
 const char* cf_param_name(int index);
 
+/* The solver schedule (no counterpart: the reference's is fixed).  A scene of
+ * a pool takes the pool's schedule (cf_pool_set_schedule); setting another
+ * one on it fails with CF_EINVAL. */
 int  cf_sim_set_schedule(cf_sim* sim, const cf_schedule* s);
 int  cf_sim_get_schedule(const cf_sim* sim, cf_schedule* s);
 
@@ -189,6 +192,15 @@ int  cf_sim_step_latencies(cf_sim* sim, int nsteps, float* ms);
 
 /* Wait for all enqueued work. */
 int  cf_sim_synchronize(cf_sim* sim);
+
+/* Status of the steps run so far; waits for them.  CF_ENONFINITE when a NaN
+ * or Inf entered the sources pass (T, Y) or the CFL pass (U, V) since the
+ * last call (the flag is then cleared); CF_ENOCONV when the schedule has a
+ * residual target (tol > 0) and the last step ended above it; else CF_OK.
+ * The reference has no counterpart: WebGPU reports only device loss and
+ * validation errors (js/main.js:166-167), so a diverged field goes unnoticed
+ * there. */
+int  cf_sim_status(cf_sim* sim);
 
 /* ---- readback ---------------------------------------------------------- */
 
@@ -247,7 +259,8 @@ typedef struct cf_pool_info {
 uint64_t cf_scene_bytes(int nx, int ny, double dx);
 int  cf_pool_create(uint64_t hbm_budget, cf_pool** out);
 void cf_pool_destroy(cf_pool* pool);
-/* The schedule of every scene (cf_sim_set_schedule rules). */
+/* The schedule of every scene (cf_sim_set_schedule rules).  Every scene is
+ * checked before any changes: on an error no scene's schedule has changed. */
 int  cf_pool_set_schedule(cf_pool* pool, const cf_schedule* sc);
 /* A new scene in the first free slot; CF_EAGAIN when it does not fit the budget. */
 int  cf_pool_admit(cf_pool* pool, const cf_scene_desc* desc, int* slot);
@@ -262,6 +275,10 @@ int  cf_pool_step(cf_pool* pool, int nsteps);
 int  cf_pool_frame(cf_pool* pool, int nsteps, int with_radiation);
 int  cf_pool_synchronize(cf_pool* pool);
 int  cf_pool_get_info(const cf_pool* pool, cf_pool_info* info);
+/* cf_sim_status over every scene (each scene's flag is read and cleared):
+ * the first CF_ENONFINITE scene, else the first CF_ENOCONV scene; its slot
+ * in *slot (may be NULL; -1 when all are CF_OK). */
+int  cf_pool_status(cf_pool* pool, int* slot);
 /* Device time of each of nsteps pool steps (ms), events around every step. */
 int  cf_pool_step_latencies(cf_pool* pool, int nsteps, float* ms);
 

@@ -29,6 +29,16 @@
 #include "cf_oracle.h"
 #include "half.h"
 
+/* CFO_MUTANT = k > 0 builds the oracle with one deliberate mistake (mutant k,
+ * marked MUTANT(k) where it is planted) so that the oracle's own checks can
+ * be shown to catch it: tests/test_oracle_pins.py compiles each mutant and
+ * asserts that the check aimed at it fails.  The oracle proper is CFO_MUTANT
+ * 0, where every MUTANT(k) is false. */
+#ifndef CFO_MUTANT
+#define CFO_MUTANT 0
+#endif
+#define MUTANT(k) (CFO_MUTANT == (k))
+
 #ifdef CFO_DOUBLE
 typedef double REAL;
 #define PFX(n) cfo64_##n
@@ -550,7 +560,7 @@ static inline REAL exchange(const S* s, REAL spd, REAL Tf) {                    
 static void k_forces(S* s) {
   const int nx = s->nx, ny = s->ny;
   const REAL dx = PP(dx), dt = PP(dt);
-  const REAL idx2 = K(1) / (dx * dx);
+  const REAL idx2 = (MUTANT(7) ? K(2) : K(1)) / (dx * dx);
 #pragma omp parallel for schedule(static)
   for (int j = 0; j <= ny; j++)
     for (int i = 0; i <= nx; i++) {
@@ -569,7 +579,8 @@ static void k_forces(S* s) {
           REAL muE = muOf(s, TR), muW = muOf(s, TL);
           REAL uN = u, muN = muOf(s, K(0.5) * (TL + TR));
           if (j < ny - 1) { uN = u2f(s, i, j + 1); muN = muOf(s, K(0.25) * (TL + TR + Tcl(s, i - 1, j + 1) + Tcl(s, i, j + 1))); }
-          REAL uS = -u, muS = muOf(s, K(0.5) * (TL + TR));
+          if (MUTANT(8) && j < ny - 1) muN = muOf(s, K(0.5) * (TL + TR));  /* corner mu from the face's own cells */
+          REAL uS = MUTANT(9) ? u : -u, muS = muOf(s, K(0.5) * (TL + TR));
           if (j > 0) { uS = u2f(s, i, j - 1); muS = muOf(s, K(0.25) * (TL + TR + Tcl(s, i - 1, j - 1) + Tcl(s, i, j - 1))); }
           REAL visc = (muE * (u2f(s, i + 1, j) - u) + muW * (u2f(s, i - 1, j) - u) + muN * (uN - u) + muS * (uS - u)) * idx2 / rhoF;
           REAL vAvg = K(0.25) * (s->V2[idxV(s, i - 1, j)] + s->V2[idxV(s, i, j)] + s->V2[idxV(s, i - 1, j + 1)] + s->V2[idxV(s, i, j + 1)]);
@@ -691,8 +702,8 @@ static void k_restrictRes(S* s, int l) {                                        
     for (int I = 0; I < nxc; I++) {
       REAL r = residual(s, l, 2 * I, 2 * J) + residual(s, l, 2 * I + 1, 2 * J) +
                residual(s, l, 2 * I, 2 * J + 1) + residual(s, l, 2 * I + 1, 2 * J + 1);
-      s->rhs[l + 1][J * nxc + I] = K(0.25) * r;
-      s->p[l + 1][J * nxc + I] = K(0);
+      s->rhs[l + 1][J * nxc + I] = (MUTANT(5) ? K(0.5) : K(0.25)) * r;
+      if (!MUTANT(6)) s->p[l + 1][J * nxc + I] = K(0);
     }
 }
 
@@ -706,7 +717,12 @@ static void k_prolong(S* s, int l) {                                            
       int I = i / 2, J = j / 2;
       int I2 = iclamp(I + ((i & 1) == 1 ? 1 : -1), 0, nxc - 1);
       int J2 = iclamp(J + ((j & 1) == 1 ? 1 : -1), 0, nyc - 1);
-      REAL c = K(0.5625) * pc[J * nxc + I] + K(0.1875) * (pc[J * nxc + I2] + pc[J2 * nxc + I]) + K(0.0625) * pc[J2 * nxc + I2];
+      if (MUTANT(4)) {  /* mirror instead of clamp at the edges */
+        I2 = I + ((i & 1) == 1 ? 1 : -1); I2 = I2 < 0 ? 1 : (I2 > nxc - 1 ? nxc - 2 : I2);
+        J2 = J + ((j & 1) == 1 ? 1 : -1); J2 = J2 < 0 ? 1 : (J2 > nyc - 1 ? nyc - 2 : J2);
+      }
+      const REAL w3 = MUTANT(3) ? K(0.0625) : K(0.1875), w1 = MUTANT(3) ? K(0.1875) : K(0.0625);
+      REAL c = K(0.5625) * pc[J * nxc + I] + w3 * (pc[J * nxc + I2] + pc[J2 * nxc + I]) + w1 * pc[J2 * nxc + I2];
       REAL off, diag;
       stencil(s, l, p, i, j, &off, &diag);
       p[j * nx + i] += diag > K(0) ? c : K(0);
@@ -1085,15 +1101,16 @@ static void vcycle(S* s, int nu) {
 /* vcycle (js/fluid.js:294-312) */
 static void vcycle_ref(S* s, int nu) {
   const int L = s->nlev;
+  const int dn = MUTANT(2) ? 1 : 0, up = MUTANT(1) ? 0 : 1;  /* first colour of a sweep, down and up */
   for (int l = 0; l < L - 1; l++) {
-    for (int q = 0; q < nu; q++) { k_relax(s, l, 0); k_relax(s, l, 1); }
+    for (int q = 0; q < nu; q++) { k_relax(s, l, dn); k_relax(s, l, 1 - dn); }
     k_restrictRes(s, l);
   }
   if (s->sch.coarse_iters > 0) k_coarsest(s, L - 1, s->sch.coarse_iters);
   else coarse_direct(s, L - 1);
   for (int l = L - 2; l >= 0; l--) {
     k_prolong(s, l);
-    for (int q = 0; q < nu; q++) { k_relax(s, l, 1); k_relax(s, l, 0); }
+    for (int q = 0; q < nu; q++) { k_relax(s, l, up); k_relax(s, l, 1 - up); }
   }
 }
 
@@ -1180,7 +1197,7 @@ static void k_geometry(S* s) {                                                  
       if (i <= nx && j < ny) {
         v2 c = {(REAL)i * dx, ((REAL)j + K(0.5)) * dx};
         REAL d = nearest(s, c, &id).x;
-        s->fx[idxU(s, i, j)] = clampr(K(0.5) + d / dx, K(0), K(1));
+        s->fx[idxU(s, i, j)] = clampr((MUTANT(10) ? K(0) : K(0.5)) + d / dx, K(0), K(1));
       }
       if (i < nx && j <= ny) {
         v2 c = {((REAL)i + K(0.5)) * dx, (REAL)j * dx};
@@ -1214,6 +1231,7 @@ static REAL hitBox(v2 o, v2 d, const float* pc, REAL* face_o) {                 
   if (tf < K(0) || tn > tf || tn < K(1e-5)) { *face_o = K(0); return K(-1); }
   REAL face;
   if (tminx > tminy) face = ldx < K(0) ? K(0) : K(2); else face = ldy < K(0) ? K(1) : K(3);
+  if (MUTANT(12)) face = (REAL)(((int)face + 2) % 4);  /* the opposite face */
   *face_o = face;
   return tn;
 }
@@ -1303,7 +1321,7 @@ static void k_radiation(S* s) {                                                 
             Jv += g1 * tOut / (REAL)ns;
           }
           if (tHit < tFloor && tHit < K(1e8)) { Twall = hitT; break; }
-          if (tFloor < K(1e8) && tFloor <= tHit && !bounced) {
+          if (tFloor < K(1e8) && tFloor <= tHit && !bounced && !MUTANT(11)) {
             o.x = o.x + d.x * tFloor + K(0); o.y = o.y + d.y * tFloor + K(1e-5);
             d.y = -d.y;
             bounced = 1;

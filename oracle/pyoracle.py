@@ -44,13 +44,27 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
+_LIBS = {}
+
+
+def lib(path=None):
+    """The oracle library (liboracle.so beside this file), or the build at
+    `path` (a mutant of the oracle, built by the mutation tests)."""
     global _LIB
+    if path is not None:
+        if path not in _LIBS:
+            _LIBS[path] = _bind(C.CDLL(path))
+        return _LIBS[path]
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
+        _LIB = _bind(C.CDLL(path))
+    return _LIB
+
+
+def _bind(L):
+    if True:
         for p in ("cfo32_", "cfo64_"):
             f = getattr(L, p + "create"); f.restype = C.c_void_p
             f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int)]
@@ -88,8 +102,7 @@ def lib():
         for n in ("cfo_flame_speed_ex", "cfo_flame_speed32_ex"):
             getattr(L, n).restype = FlameResult
             getattr(L, n).argtypes = [C.c_double] * 9
-        _LIB = L
-    return _LIB
+    return L
 
 
 def f2h(a):
@@ -133,8 +146,8 @@ class OracleError(RuntimeError):
 class Oracle:
     """One scene in the CPU oracle (FP32 by default, FP64 with fp64=True)."""
 
-    def __init__(self, nx, ny, dx, dt, fp64=False):
-        self.L = lib()
+    def __init__(self, nx, ny, dx, dt, fp64=False, lib_path=None):
+        self.L = lib(lib_path)
         self.p = "cfo64_" if fp64 else "cfo32_"
         self.fp64 = fp64
         self.nx, self.ny, self.dx, self.dt = nx, ny, dx, dt

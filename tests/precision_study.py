@@ -119,6 +119,11 @@ def study(size=1024, warm=2000, seed=1, log=None):
         info = g.info()
         G = split({k: g.read_field(k) for k in ("T", "Y", "U", "V", "p", "SQ")})
         out["errors"][name] = {k: rel(G[k], R[k]) for k in R}
+        if name == "fp32_reference":
+            # the GPU's reference step against the FP32 oracle's, directly
+            out["gpu_vs_oracle_fp32"] = {k: rel(G[k], F[k]) for k in F}
+            log("[precision] GPU vs FP32 oracle, reference step: %s"
+                % " ".join("%s %.1e" % (k, v) for k, v in out["gpu_vs_oracle_fp32"].items()))
         out["vcycles"][name] = int(info.last_vcycles)
         out["resid"][name] = float(info.last_resid) if sch.get("tol", 0) > 0 else None
         src, outf = g.compat_sums()

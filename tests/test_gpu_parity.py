@@ -172,7 +172,7 @@ def test_radiation_parity():
     a, b = g.read_field("rad"), o.read("rad")
     r = rel_l2(a, b)
     _flush("  rad rel_l2=%.3g" % r)
-    assert r <= 1e-4
+    assert r <= 1e-5
 
 
 def test_readback_ring_and_reset():
