@@ -50,10 +50,11 @@ def test_cfl_and_tolerance_match_the_oracle():
         # count can differ by a few cycles
         assert (gi.last_resid < 1e-6) == (o.last_resid < 1e-6)
         assert abs(gi.last_vcycles - o.last_vcycles) <= 3
-        for name in ["T", "Y", "U", "V"]:
+        # measured over these 4 steps: T, Y <= 6e-9; U <= 1.1e-6, V <= 2.4e-6
+        for name, tol in (("T", 1e-7), ("Y", 1e-7), ("U", 2.5e-5), ("V", 2.5e-5)):
             r = rel_l2(g.read_field(name), o.read(name))
             _flush("    %s rel_l2=%.3g" % (name, r))
-            assert r <= 1e-4, (s, name, r)
+            assert r <= tol, (s, name, r)
     g.close()
     o.close()
 
@@ -116,13 +117,14 @@ def test_short_horizon_heat_release_and_centerline_temperature():
     _flush("  HRR sums GPU  ", hg)
     _flush("  HRR sums oracle", ho)
     hg, ho = np.array(hg, float), np.array(ho, float)
-    assert np.abs(hg - ho).sum() <= 0.02 * np.abs(ho).sum()
+    # measured: 6e-5 of the total; centreline T 5.1e-4
+    assert np.abs(hg - ho).sum() <= 1e-3 * np.abs(ho).sum()
     nx = cfg["nx"]
     cx = int(float(cfg["pieces"][0] + cfg["pieces"][20]) / 2 / cfg["dx"])
     Tg = g.read_field("T").reshape(-1, nx)[:, cx]
     To = o.read("T").reshape(-1, nx)[:, cx]
     r = rel_l2(Tg, To)
     _flush("  centerline T rel_l2 = %.3g (max %.1f K / %.1f K)" % (r, Tg.max(), To.max()))
-    assert r <= 1e-2
+    assert r <= 5e-3
     g.close()
     o.close()
