@@ -118,6 +118,7 @@ def lib():
         "cf_sim_copy_state": ([p, p], i),
         "cf_sim_compat_sums": ([p, C.POINTER(d), C.POINTER(d)], i),
         "cf_flame1d_run": ([C.POINTER(CfFlameCase), i, i, C.POINTER(CfFlameResult)], i),
+        "cf_flame1d_run_static": ([C.POINTER(CfFlameCase), i, i, C.POINTER(CfFlameResult), vp], i),
         "cf_scene_bytes": ([i, i, C.c_double], C.c_uint64),
         "cf_pool_create": ([C.c_uint64, C.POINTER(p)], i),
         "cf_pool_destroy": ([p], None),

@@ -12,7 +12,7 @@
 // single-precision variant.
 #include <hip/hip_runtime.h>
 
-#include "../../include/campfire_hip.h"
+#include "cf_kernels.h"
 
 namespace cf {
 
@@ -194,3 +194,80 @@ void launch_flame1d(const cf_flame_case* cases, int ncases, bool fp64, cf_flame_
 }
 
 }  // namespace cf
+
+// ---- the allocation-free variant of cf_flame1d_run -------------------------------
+// The device staging of the cases and the results lives in one file-scope
+// array, `scratch`, instead of per-call allocations: the cases at offset 0, the
+// results after CF_FLAME_STATIC_MAX cases.  Each device holds its own copy,
+// until the process ends.  Host code must not pass `scratch` itself to a
+// kernel or a copy (in host code the name is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, once per
+// device, and is kept in g_scratch.
+#include <mutex>
+#include <string.h>
+
+namespace {
+constexpr size_t SCR_CASES = 0;
+constexpr size_t SCR_RESULTS = ((CF_FLAME_STATIC_MAX * sizeof(cf_flame_case) + 255) / 256) * 256;
+constexpr size_t SCR_BYTES = SCR_RESULTS + CF_FLAME_STATIC_MAX * sizeof(cf_flame_result);
+constexpr int SCR_MAX_DEVICES = 64;
+__device__ __align__(256) unsigned char scratch[SCR_BYTES];
+
+std::mutex g_scratch_mu;
+void* g_scratch[SCR_MAX_DEVICES] = {};  // device address of `scratch` on each device, once looked up
+
+// the device address of this device's `scratch` (looked up on the first call
+// on the device, with it current); null with a message on failure
+void* scratch_ptr(int* code) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= SCR_MAX_DEVICES) {
+    *code = cf::set_error(CF_EDEVICE, "hipGetDevice failed or device out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *code = cf::set_error(CF_EDEVICE, "hipGetSymbolAddress(scratch) failed");
+      return nullptr;
+    }
+    g_scratch[dev] = p;
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+
+extern "C" int cf_flame1d_run_static(const cf_flame_case* cases, int ncases, int fp64, cf_flame_result* out,
+                                     void* stream) {
+  if (!cases || !out || ncases < 1) return cf::set_error(CF_EINVAL, "bad argument (cases, out non-null, ncases >= 1)");
+  if (ncases > CF_FLAME_STATIC_MAX)
+    return cf::set_error(CF_EINVAL, "more cases than the static staging holds (CF_FLAME_STATIC_MAX)");
+  for (int q = 0; q < ncases; q++) {
+    const int rc = cf::check_flame_case(&cases[q]);
+    if (rc) return rc;
+  }
+  int code = CF_OK;
+  unsigned char* scr = (unsigned char*)scratch_ptr(&code);
+  if (!scr) return code;
+  hipStream_t st = (hipStream_t)stream;
+  cf_flame_case* dc = (cf_flame_case*)(scr + SCR_CASES);
+  cf_flame_result* dr = (cf_flame_result*)(scr + SCR_RESULTS);
+  // the cases as cf_flame1d_run passes them: a zero step cap means 10 million
+  hipError_t e = hipMemcpyAsync(dc, cases, sizeof(cf_flame_case) * ncases, hipMemcpyHostToDevice, st);
+  for (int q = 0; q < ncases && e == hipSuccess; q++)
+    if (cases[q].max_steps <= 0) {
+      const int32_t cap = 10000000;
+      e = hipMemcpyAsync((unsigned char*)&dc[q] + offsetof(cf_flame_case, max_steps), &cap, sizeof cap,
+                         hipMemcpyHostToDevice, st);
+    }
+  if (e == hipSuccess) {
+    cf::launch_flame1d(dc, ncases, fp64 != 0, dr, st);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dr, sizeof(cf_flame_result) * ncases, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return cf::set_error(CF_EDEVICE, hipGetErrorString(e));
+  return CF_OK;
+}

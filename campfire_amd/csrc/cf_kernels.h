@@ -74,8 +74,8 @@ struct BatchPhase {
     return e;
   }
   void release() {
-    if (dargs) hipFree(dargs);
-    if (ditems) hipFree(ditems);
+    if (dargs) (void)hipFree(dargs);
+    if (ditems) (void)hipFree(ditems);
     dargs = nullptr;
     ditems = nullptr;
   }
@@ -272,6 +272,10 @@ void launch_radiation(const Fields& F, int ntargets, hipStream_t s);
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s);
 // tests/flame1d.mjs on the GPU (cf_flame1d.hip): one workgroup per case
 void launch_flame1d(const cf_flame_case* cases, int ncases, bool fp64, cf_flame_result* out, hipStream_t s);
+// the checks of one flame case (CF_OK, or a code with the message set)
+int check_flame_case(const cf_flame_case* k);
+// sets cf_last_error()'s text and returns code
+int set_error(int code, const char* msg);
 unsigned div_error_blocks(const Fields& F);
 // FP64 net source and open-boundary outflow: 2 doubles per workgroup
 unsigned compat_blocks(const Fields& F);

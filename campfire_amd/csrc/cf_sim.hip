@@ -832,6 +832,17 @@ static int pool_ready(cf_pool* pl) {
   return CF_OK;
 }
 
+namespace cf {
+int set_error(int code, const char* msg) { return set_err(code, "%s", msg); }
+int check_flame_case(const cf_flame_case* k) {
+  if (!(k->dx > 0) || !(k->L > 0) || !(k->tEnd >= 0) || !(k->F > 0))
+    return set_err(CF_EINVAL, "flame case: bad dx, L, tEnd or F");
+  const double n = floor(k->L / k->dx + 0.5);  // Math.round (tests/flame1d.mjs:15)
+  if (n < 2 || n > CF_FLAME_MAXN) return set_err(CF_EBADSIZE, "flame case: %g cells (2..%d)", n, CF_FLAME_MAXN);
+  return CF_OK;
+}
+}  // namespace cf
+
 extern "C" {
 
 const char* cf_last_error(void) { return g_err; }
@@ -1453,9 +1464,8 @@ int cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_re
   if (!cases || !out || ncases < 1) return set_err(CF_EINVAL, "bad argument");
   std::vector<cf_flame_case> c(cases, cases + ncases);
   for (auto& k : c) {
-    if (!(k.dx > 0) || !(k.L > 0) || !(k.tEnd >= 0) || !(k.F > 0)) return set_err(CF_EINVAL, "flame case: bad dx, L, tEnd or F");
-    const double n = floor(k.L / k.dx + 0.5);  // Math.round (tests/flame1d.mjs:15)
-    if (n < 2 || n > CF_FLAME_MAXN) return set_err(CF_EBADSIZE, "flame case: %g cells (2..%d)", n, CF_FLAME_MAXN);
+    const int rc = cf::check_flame_case(&k);
+    if (rc) return rc;
     if (k.max_steps <= 0) k.max_steps = 10000000;
   }
   cf_flame_case* dc = nullptr;

@@ -302,6 +302,18 @@ typedef struct cf_flame_result {
  * or fewer than 2 cells. */
 int  cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_result* out);
 
+/* cf_flame1d_run without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc).  The device copies of the cases and the
+ * results live in a static device array of the library (about 26 KB per GPU,
+ * held until the process ends) instead of per-call hipMalloc/hipFree, and the
+ * work runs on the caller's stream (a hipStream_t; NULL: the null stream),
+ * waited for before returning.  At most CF_FLAME_STATIC_MAX cases per call
+ * (more: CF_EINVAL).  Calls that overlap on one GPU share that array and race. */
+#define CF_FLAME_STATIC_MAX 256
+int  cf_flame1d_run_static(const cf_flame_case* cases, int ncases, int fp64, cf_flame_result* out,
+                           void* stream);
+
 /* Divergence compatibility, summed on the device in FP64: the net source of
  * the open cells (sum S dx^2, m^2/s) and the net outflow through the open
  * domain faces (dx sum f u.n).  Equal after a converged projection.  No
