@@ -304,40 +304,53 @@ def converged_batch(args):
 
     dcb, tdc = batched({"nu": 3, "coarse_iters": 24})
     kry, tkr = batched({"nu": args.fcg_nu, "coarse_iters": args.fcg_coarse, "krylov": 1})
-    # (a) plain per-scene FP32
-    sims = []
-    for c in cfgs:
-        sim = FluidSim({"nx": c["nx"], "ny": c["ny"], "dt": c["dt"]}, dx=c["dx"])
-        scenes.apply(sim, c)
-        sim.set_schedule(nu=2, vcycles=cap, tol=tol, coarse_iters=120)
-        sim.frame_all(8, True)
-        sim.step(2)
-        sims.append(sim)
-    for sim in sims:
-        sim.synchronize()
-    ts, ra, ca = 0.0, [], []
-    for q in range(K):
+
+    def per_scene(sched, label):
+        # one FluidSim per scene, stepped one scene after another, FP32 smoother
+        sims = []
+        for c in cfgs:
+            sim = FluidSim({"nx": c["nx"], "ny": c["ny"], "dt": c["dt"]}, dx=c["dx"])
+            scenes.apply(sim, c)
+            sim.set_schedule(vcycles=cap, tol=tol, **sched)
+            sim.frame_all(8, True)
+            sim.step(2)
+            sims.append(sim)
         for sim in sims:
-            t0 = time.perf_counter()
-            sim.step(1)
             sim.synchronize()
-            ts += time.perf_counter() - t0
-            ra.append(float(sim.info().last_resid))
-            ca.append(int(sim.info().last_vcycles))
-    for sim in sims:
-        sim.close()
+        ts, ra, ca = 0.0, [], []
+        for q in range(K):
+            for sim in sims:
+                t0 = time.perf_counter()
+                sim.step(1)
+                sim.synchronize()
+                ts += time.perf_counter() - t0
+                ra.append(float(sim.info().last_resid))
+                ca.append(int(sim.info().last_vcycles))
+        for sim in sims:
+            sim.close()
+        return {"mcell_steps_per_s": round(cells * K / ts / 1e6, 1), "schedule": label,
+                "resid_max": max(ra), "cycles_mean": round(float(np.mean(ca)), 2),
+                "steps_at_cap": int(sum(1 for c in ca if c >= cap))}, ts
+
+    # (a) the same solver as the batched FCG form, per scene and in FP32
+    same, tsame = per_scene({"nu": args.fcg_nu, "coarse_iters": args.fcg_coarse, "refine": 1, "krylov": 1},
+                            "nu %d, %d-sweep coarsest, FP32 smoother, FP64 flexible CG" % (args.fcg_nu, args.fcg_coarse))
+    # (b) the reference's smoother settings with plain FP32 V-cycles
+    plain, tplain = per_scene({"nu": 2, "coarse_iters": 120}, "nu 2, 120 coarsest sweeps, FP32 V-cycles")
     res = {"tol": tol, "cycle_cap": cap, "steps": K, "scenes": len(cfgs), "cells": cells,
            "batched_mixed_fcg": kry, "batched_mixed": dcb,
-           "per_scene_fp32": {"mcell_steps_per_s": round(cells * K / ts / 1e6, 1),
-                              "schedule": "nu 2, 120 coarsest sweeps, FP32",
-                              "resid_max": max(ra), "cycles_mean": round(float(np.mean(ca)), 2),
-                              "steps_at_cap": int(sum(1 for c in ca if c >= cap))},
-           "speedup": round(ts / tkr, 3), "speedup_defect_correction": round(ts / tdc, 3)}
-    for name, r in (("batched mixed + FCG", kry), ("batched mixed", dcb), ("per-scene FP32", res["per_scene_fp32"])):
+           "per_scene_fp32_fcg": same, "per_scene_fp32": plain,
+           # batching + the FP16 smoother alone: the same solver on both sides
+           "speedup": round(tsame / tkr, 3),
+           # against the plain per-scene FP32 V-cycles (which miss the target in some scene-steps)
+           "speedup_vs_plain_vcycles": round(tplain / tkr, 3),
+           "speedup_defect_correction_vs_plain": round(tplain / tdc, 3)}
+    for name, r in (("batched mixed + FCG", kry), ("batched mixed", dcb), ("per-scene FP32 + FCG", same),
+                    ("per-scene FP32 V-cycles", plain)):
         log("[bench] equal accuracy (tol %.0e): %s %.1f Mcell-steps/s (resid max %.2e, %.2f cycles, %d scene-steps "
             "at the cap)" % (tol, name, r["mcell_steps_per_s"], r["resid_max"], r["cycles_mean"], r["steps_at_cap"]))
-    log("[bench] equal accuracy: x%.2f (FCG), x%.2f (defect correction); batched FCG p99 %.2f ms"
-        % (res["speedup"], res["speedup_defect_correction"], kry["step_latency_ms"]["p99"]))
+    log("[bench] equal accuracy: x%.2f against the same solver per scene, x%.2f against plain V-cycles; "
+        "batched FCG p99 %.2f ms" % (res["speedup"], res["speedup_vs_plain_vcycles"], kry["step_latency_ms"]["p99"]))
     return res
 
 
@@ -383,6 +396,129 @@ def plain_256(args, steps=300):
     return out
 
 
+def _steps_per_frame(frames, dt, fps=60.0):
+    """Gas steps per app frame at a steady frame rate (js/main.js:252-257)."""
+    import math
+    acc, out, cap = 0.0, [], math.ceil(0.034 / dt)
+    for _ in range(frames):
+        acc += 1.0 / fps
+        n = math.floor(acc / dt)
+        if n > cap:
+            n, acc = cap, 0.0
+        else:
+            acc -= n * dt
+        out.append(n)
+    return out
+
+
+def reference_modes(args, frames=60):
+    """The reference's three modes (js/config.js:6-10; dx = 0.7 m / nx) on the
+    reference's own default scene (js/main.js:132-155, packed by
+    tests/golden/make_wood_coupling.py), stepped as the app does: per frame
+    the parameters, geometry, the frame's gas steps at 60 frames/s of
+    simulated time, radiation and a readback that the host waits for.  Against
+    the real-time rate each mode implies (steps of dt per second)."""
+    import json as _json
+    from campfire_amd.fluid import FluidSim
+    z = np.load(os.path.join(ROOT, "tests", "golden", "default_scene.npz"), allow_pickle=False)
+    pieces, n = z["pieces"], int(z["n"])
+    params = _json.loads(str(np.load(os.path.join(ROOT, "tests", "golden", "wood_coupling.npz"),
+                                     allow_pickle=False)["params"]))
+    out = {}
+    for name, (nx, ny, dt) in (("coarse", (160, 224, 0.004)), ("balanced", (224, 320, 0.003)),
+                               ("fine", (352, 512, 0.002))):
+        sim = FluidSim({"nx": nx, "ny": ny, "dt": dt}, dx=0.7 / nx)
+        for k, v in params.items():
+            sim.set_param(k, v)
+        sim.reset()
+        sim.upload_pieces(pieces, n)
+        per = _steps_per_frame(frames + 10, dt)
+        for f in range(10):
+            sim.frame = f
+            sim.frame_all(per[f], True)
+            sim.readback_now()
+        sim.synchronize()
+        t0 = time.perf_counter()
+        steps = 0
+        for f in range(10, 10 + frames):
+            sim.frame = f
+            sim.frame_all(per[f], True)
+            sim.readback_now()
+            steps += per[f]
+        t = time.perf_counter() - t0
+        sim.close()
+        rt = nx * ny / dt / 1e6
+        out[name] = {"grid": [nx, ny], "dt_ms": dt * 1e3, "frames": frames, "steps": steps,
+                     "ms_per_frame": round(t * 1e3 / frames, 4),
+                     "mcell_steps_per_s": round(nx * ny * steps / t / 1e6, 1),
+                     "real_time_mcell_steps_per_s": round(rt, 2),
+                     "x_real_time": round(nx * ny * steps / t / 1e6 / rt, 1)}
+        log("[bench] %-8s %dx%d: %.3f ms per app frame (%.1f steps), %.1f Mcell-steps/s = %.1fx real time"
+            % (name, nx, ny, out[name]["ms_per_frame"], steps / frames, out[name]["mcell_steps_per_s"],
+               out[name]["x_real_time"]))
+    return out
+
+
+def serving(args):
+    """The serving workload: Poisson arrivals in wall-clock time of scene
+    requests at 256^2, 512^2 and 1024^2, each for 500-5000 steps, admitted
+    into one pool by HBM budget and stepped together (continuous batching).
+    Reports p50/p99 of the batched step's device time, aggregate cell-steps/s
+    over the wall time, queueing delay and rebuild times."""
+    from campfire_amd.serve import make_requests, serve
+    reqs = make_requests(args.serve_requests, args.serve_rate, args.seed, steps=(500, 5000))
+    budget = int(args.serve_budget_gb * (1 << 30))
+    stats, _ = serve(reqs, budget)
+    stats["config"] = {"requests": args.serve_requests, "rate_per_s": args.serve_rate,
+                       "sizes": [256, 512, 1024], "steps_per_request": [500, 5000],
+                       "budget_gib": args.serve_budget_gb, "seed": args.seed, "clock": "wall"}
+    log("[bench] serving: %d requests over %.1f s of arrivals, %.1f s wall: %.1f Mcell-steps/s, step p50/p99 "
+        "%.3f/%.3f ms, queue p50/p99 %.2f/%.2f s, %d rebuilds p50 %.2f ms, up to %d live"
+        % (stats["requests"], stats["arrival_span_s"], stats["wall_s"], stats["mcell_steps_per_s"],
+           stats["step_latency_ms"]["p50"], stats["step_latency_ms"]["p99"], stats["queue_s"]["p50"],
+           stats["queue_s"]["p99"], stats["rebuild_ms"]["count"], stats["rebuild_ms"]["p50"] or 0, stats["max_live"]))
+    return stats
+
+
+def flame1d_leg():
+    """The 1D flame table of tests/flame1d.mjs (B = 2e10, seven dx/F rows) in
+    one GPU launch (FP64), the C port of the script on one host core, and the
+    reference script's own wall time under Node (recorded with its output in
+    tests/golden/flame1d.json)."""
+    import json as _json
+    from campfire_amd.flame1d import flame_speeds
+    from oracle.pyoracle import flame_speed as port_speed
+    cases = []
+    for dx in [5e-5, 2e-4, 1e-3, 3.125e-3]:
+        for F in ([1, max(1, 3 * dx / 5e-4)] if dx > 5e-5 else [1]):
+            cases.append(dict(B=2e10, Ta=15000.0, dx=dx, F=F, L=max(0.03, 60 * dx), tEnd=3.0))
+    flame_speeds(cases[-1:], fp64=True)
+    t0 = time.perf_counter()
+    gpu = flame_speeds(cases, fp64=True)
+    tg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    port = [port_speed(c["B"], c["Ta"], c["dx"], c["F"], 1.0, c["L"], c["tEnd"]) for c in cases]
+    tp = time.perf_counter() - t0
+    gold = _json.load(open(os.path.join(ROOT, "tests", "golden", "flame1d.json")))["runs"][0]
+    res = {"cases": len(cases), "gpu_s": round(tg, 4), "port_s": round(tp, 4), "port_cores": 1,
+           "script_s": round(gold["wall_s"], 3), "script_where": gold.get("source", "") + ", builder's container",
+           "SL_readme_case": round(gpu[0]["SL"], 5), "SL_table_gpu": [round(r["SL"], 4) for r in gpu],
+           "SL_table_port": [round(r["SL"], 4) for r in port], "SL_table_script": [r["SL"] for r in gold["rows"]]}
+    log("[bench] flame1d: 7 cases GPU %.3f s, C port %.3f s (1 core), script %.2f s (Node); S_L %.5f m/s"
+        % (tg, tp, gold["wall_s"], gpu[0]["SL"]))
+    return res
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def bench_config(args):
     from campfire_amd import scenes
     cfg = scenes.config_pile_1024(args.seed)
@@ -394,6 +530,7 @@ def bench_config(args):
 
 
 def cpu_baseline(cfg, budget_s=12.0, max_steps=30):
+    """The FP32 oracle (OpenMP) on a bounded sample of the workload `cfg`."""
     from campfire_amd import scenes
     from oracle.pyoracle import Oracle
     o = Oracle(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dt"])
@@ -410,7 +547,7 @@ def cpu_baseline(cfg, budget_s=12.0, max_steps=30):
     o.close()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     return {"value": round(cfg["nx"] * cfg["ny"] * n / t / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu": _cpu_model(),
             "sample": "%d steps of the same %dx%d workload in the FP32 C oracle (OpenMP, %d threads), %.1f s"
                       % (n, cfg["nx"], cfg["ny"], threads, t)}
 
@@ -436,6 +573,10 @@ def main():
                     help="timed steps of the 256^2 plain-method workload (0: skip)")
     ap.add_argument("--converged-only", action="store_true",
                     help="run only the equal-accuracy batch comparison and print its JSON (for profiling)")
+    ap.add_argument("--serve-requests", type=int, default=32, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="request arrivals per second")
+    ap.add_argument("--serve-budget-gb", type=float, default=2.0, help="HBM budget of the serving pool")
+    ap.add_argument("--no-modes", action="store_true", help="skip the reference-mode frames")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -553,9 +694,21 @@ def main():
         out["batched"] = batched(args)
         if args.conv_steps > 0:
             out["converged_batch"] = converged_batch(args)
+    if not args.no_modes:
+        out["reference_modes"] = reference_modes(args)
+    if args.serve_requests > 0:
+        out["serving"] = serving(args)
     if not args.no_cpu:
+        out["flame1d"] = flame1d_leg()
         log("[bench] CPU baseline (FP32 oracle)")
         out["cpu_baseline"] = cpu_baseline(cfg)
+        # the same FP32 oracle at the smaller grids of the workload family
+        sizes = {}
+        for n in (256, 512):
+            a2 = argparse.Namespace(**vars(args))
+            a2.size = n
+            sizes["%dx%d" % (n, n)] = cpu_baseline(bench_config(a2), budget_s=5.0, max_steps=100)
+        out["cpu_baseline_sizes"] = sizes
     print(json.dumps(out), flush=True)
 
 

@@ -126,6 +126,7 @@ def lib():
         "cf_pool_admit": ([p, C.POINTER(CfSceneDesc), C.POINTER(i)], i),
         "cf_pool_scene": ([p, i, C.POINTER(p)], i),
         "cf_pool_release": ([p, i], i),
+        "cf_pool_prepare": ([p], i),
         "cf_pool_step": ([p, i], i),
         "cf_pool_frame": ([p, i, i], i),
         "cf_pool_synchronize": ([p], i),

@@ -65,22 +65,79 @@ struct BatchPhase {
     nargs++;
   }
   bool empty() const { return hitems.empty(); }
+  bool owned = false;  // dargs/ditems allocated by upload() (else they live in a TableArena)
   hipError_t upload(hipStream_t s) {
     if (empty()) return hipSuccess;
     hipError_t e = hipMalloc(&dargs, hargs.size());
     if (e == hipSuccess) e = hipMalloc((void**)&ditems, hitems.size() * sizeof(Item));
+    owned = true;
     if (e == hipSuccess) e = hipMemcpyAsync(dargs, hargs.data(), hargs.size(), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(ditems, hitems.data(), hitems.size() * sizeof(Item), hipMemcpyHostToDevice, s);
     return e;
   }
   void release() {
-    if (dargs) (void)hipFree(dargs);
-    if (ditems) (void)hipFree(ditems);
+    if (owned) {
+      if (dargs) (void)hipFree(dargs);
+      if (ditems) (void)hipFree(ditems);
+    }
     dargs = nullptr;
     ditems = nullptr;
+    owned = false;
   }
   void run(hipStream_t s) const {
     if (fn && !empty() && !mixed) fn(*this, s);
+  }
+};
+
+// Device storage for the tables of a set of phases: one buffer that only
+// grows, filled by one host-to-device copy per rebuild, so rebuilding a pool's
+// step allocates nothing once the buffer is large enough.  The caller
+// guarantees that no launch still reads the old tables (a stream sync).
+struct TableArena {
+  void* dev = nullptr;
+  size_t cap = 0;
+  std::vector<unsigned char> host;  // kept until the next place(): the copy's source
+  static size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+  // lays out every phase's argument blocks and items, plus `nextra` bytes of
+  // `extra` (returned in *extra_dev), and points the phases at them
+  hipError_t place(const std::vector<BatchPhase*>& phases, hipStream_t s, const void* extra = nullptr,
+                   size_t nextra = 0, void** extra_dev = nullptr) {
+    size_t total = al(nextra);
+    for (BatchPhase* p : phases)
+      if (!p->empty()) total += al(p->hargs.size()) + al(p->hitems.size() * sizeof(Item));
+    if (total == 0) return hipSuccess;
+    if (total > cap) {
+      if (dev) (void)hipFree(dev);
+      dev = nullptr;
+      const size_t want = total + total / 2;
+      hipError_t e = hipMalloc(&dev, want);
+      if (e != hipSuccess) { cap = 0; return e; }
+      cap = want;
+    }
+    host.assign(total, 0);
+    size_t off = 0;
+    unsigned char* d = (unsigned char*)dev;
+    if (nextra) {
+      memcpy(&host[0], extra, nextra);
+      if (extra_dev) *extra_dev = d;
+      off = al(nextra);
+    }
+    for (BatchPhase* p : phases) {
+      p->release();
+      if (p->empty()) continue;
+      memcpy(&host[off], p->hargs.data(), p->hargs.size());
+      p->dargs = d + off;
+      off += al(p->hargs.size());
+      memcpy(&host[off], p->hitems.data(), p->hitems.size() * sizeof(Item));
+      p->ditems = (Item*)(d + off);
+      off += al(p->hitems.size() * sizeof(Item));
+    }
+    return hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, s);
+  }
+  void free() {
+    if (dev) (void)hipFree(dev);
+    dev = nullptr;
+    cap = 0;
   }
 };
 

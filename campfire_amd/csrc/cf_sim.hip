@@ -535,7 +535,9 @@ struct cf_pool {
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   Segments seg;               // residual-controlled steps: the step in segments
   std::vector<size_t> cyc;    // first phase of each V-cycle of `step`; the last entry: the projection's
-  cf::Dyn** dDyns = nullptr;  // the live scenes' control blocks, for the convergence check
+  cf::Dyn** dDyns = nullptr;  // the live scenes' control blocks, for the convergence check (in `arena`)
+  cf::TableArena arena;       // device tables of `step`
+  cf::TableArena farena;      // device tables of frame_pre and frame_post
   int ndyn = 0;
   int* dflag = nullptr;       // all scenes converged (device), and its pinned host copy
   int* hflag = nullptr;
@@ -738,21 +740,15 @@ static int pool_build_step(cf_pool* pl) {
   { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_project(a, s->F, dc ? s->dP64 : nullptr); }
   for (auto& a : ph)
     if (a.mixed) return set_err(CF_EINVAL, "pool: phase '%s' would run one kernel over scenes of another", a.tag);
-  {  // the control blocks the convergence check reads
+  {  // the tables, and the control blocks the convergence check reads, in the step's arena
     std::vector<cf::Dyn*> dyns;
     for (auto* s : sc) dyns.push_back(s->dDyn);
-    if (pl->dDyns) hipFree(pl->dDyns);
-    pl->dDyns = nullptr;
     pl->ndyn = (int)dyns.size();
-    hipError_t e = hipMalloc((void**)&pl->dDyns, dyns.size() * sizeof(cf::Dyn*));
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(pl->dDyns, dyns.data(), dyns.size() * sizeof(cf::Dyn*), hipMemcpyHostToDevice, pl->st);
+    std::vector<cf::BatchPhase*> all;
+    for (auto& a : ph) all.push_back(&a);
+    hipError_t e = pl->arena.place(all, pl->st, dyns.data(), dyns.size() * sizeof(cf::Dyn*), (void**)&pl->dDyns);
     if (e == hipSuccess && !pl->dflag) e = hipMalloc((void**)&pl->dflag, sizeof(int));
     if (e == hipSuccess && !pl->hflag) e = hipHostMalloc((void**)&pl->hflag, sizeof(int), hipHostMallocDefault);
-    if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables: %s", hipGetErrorString(e));
-  }
-  for (auto& a : ph) {
-    hipError_t e = a.upload(pl->st);
     if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables: %s", hipGetErrorString(e));
   }
   pl->nlaunch = 0;
@@ -1523,8 +1519,9 @@ void cf_pool_destroy(cf_pool* pl) {
   pool_release_phases(pl->step);
   pool_release_phases(pl->frame_pre);
   pool_release_phases(pl->frame_post);
-  if (pl->dDyns) hipFree(pl->dDyns);
-  if (pl->dflag) hipFree(pl->dflag);
+  pl->arena.free();
+  pl->farena.free();
+  if (pl->dflag) (void)hipFree(pl->dflag);
   if (pl->hflag) hipHostFree(pl->hflag);
   for (auto* s : pl->slots) if (s) cf_sim_destroy(s);
   hipStreamDestroy(pl->st);
@@ -1594,6 +1591,13 @@ int cf_pool_release(cf_pool* pl, int slot) {
   return CF_OK;
 }
 
+int cf_pool_prepare(cf_pool* pl) {
+  if (!pl) return set_err(CF_EINVAL, "pool is NULL");
+  for (auto* s : pl->slots)
+    if (s) return pool_ready(pl);
+  return CF_OK;
+}
+
 int cf_pool_step(cf_pool* pl, int nsteps) {
   if (!pl) return set_err(CF_EINVAL, "pool is NULL");
   if (nsteps < 0) return set_err(CF_EINVAL, "nsteps < 0");
@@ -1646,8 +1650,11 @@ int cf_pool_frame(cf_pool* pl, int nsteps, int with_radiation) {
       const int np = s->P.nPieces < CF_MAX_PIECES ? s->P.nPieces : CF_MAX_PIECES;
       cf::phase_radiation(pl->frame_post[0], s->F, np * 4 + CF_N_FLOOR);
     }
-    for (auto& a : pl->frame_pre) { hipError_t e = a.upload(pl->st); if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables"); }
-    for (auto& a : pl->frame_post) { hipError_t e = a.upload(pl->st); if (e != hipSuccess) return set_err(CF_ENOMEM, "pool tables"); }
+    std::vector<cf::BatchPhase*> all;
+    for (auto& a : pl->frame_pre) all.push_back(&a);
+    for (auto& a : pl->frame_post) all.push_back(&a);
+    hipError_t e = pl->farena.place(all, pl->st);
+    if (e != hipSuccess) return set_err(CF_ENOMEM, "pool frame tables: %s", hipGetErrorString(e));
     pl->fkey = key;
   }
   for (auto& a : pl->frame_pre) a.run(pl->st);

@@ -79,6 +79,10 @@ class ScenePool:
         if sim is not None:
             sim._h = None
 
+    def prepare(self):
+        """Builds the batched step of the current scenes now (tables, graphs)."""
+        L.check(L.lib().cf_pool_prepare(self._h), "pool prepare")
+
     def step(self, n=1):
         L.check(L.lib().cf_pool_step(self._h, int(n)), "pool step")
 

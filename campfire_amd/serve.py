@@ -1,18 +1,25 @@
 """Serving loop: continuous batching of gas-scene requests on one GPU.
 
-Requests arrive by a seeded Poisson process (times in pool steps), each for a
-scene of 256^2, 512^2 or 1024^2 cells and a number of steps.  Every round the
-loop
-  1. admits waiting requests, in arrival order, while their device memory
-     fits the pool's HBM budget (a request that does not fit waits for
-     memory to free up; later, smaller ones may go ahead of it),
-  2. advances every live scene together, one batched step graph per step,
-     for as many steps as the nearest finishing request still needs (<= 8),
-  3. releases the requests that are done, after taking their results.
-Requests that arrive while others run join at the next round and share
-every launch with them.  A scene's results are bitwise what it gives alone.
+Requests arrive by a seeded Poisson process in wall-clock time (`rate`
+requests per second from the start of the session), each for a scene of
+256^2, 512^2 or 1024^2 cells and a number of steps.  Each round the loop
+  1. takes in the requests whose arrival time has passed,
+  2. admits waiting requests, in arrival order, while their device memory
+     fits the pool's HBM budget (a request that does not fit waits for memory
+     to free up; later, smaller ones may go ahead of it), and rebuilds the
+     batched step for the new scene set at once (cf_pool_prepare), timed,
+  3. advances every live scene together for as many steps as the nearest
+     finishing request still needs, at most `max_round` (the steps are queued
+     back to back; the host waits once, at the end of the round),
+  4. releases the requests that are done, after taking their results.
+Requests that arrive while others run join at the next round and share every
+launch with them.  A scene's results are bitwise what it gives alone.
 
-    python -m campfire_amd.serve --requests 24 --rate 0.05 --budget-gb 8
+Reported: p50/p99 of the device time of one batched step (every live scene
+advancing one step), aggregate cell-steps/s over the session's wall time, the
+queueing delay from arrival to admission and the time of the rebuilds.
+
+    python -m campfire_amd.serve --requests 24 --rate 4 --budget-gb 2
 """
 import argparse
 import json
@@ -22,12 +29,12 @@ import time
 import numpy as np
 
 from . import scenes
-from .pool import ScenePool, scene_bytes
+from .pool import ScenePool
 
 
 def make_requests(n, rate, seed, sizes=(256, 512, 1024), steps=(500, 5000), width=1.28):
-    """n requests: Poisson arrivals at `rate` per pool step, sizes and step
-    counts drawn from the seeded PCG32 stream."""
+    """n requests: Poisson arrivals at `rate` per second (arrival times in
+    seconds), sizes and step counts drawn from the seeded PCG32 stream."""
     g = scenes.PCG32(seed)
     t = 0.0
     out = []
@@ -51,21 +58,27 @@ def result_of(sim):
             "acc": sim.readback_now()["acc"]}
 
 
-def serve(requests, budget, schedule=None, keep_results=False, max_round=8, log=None):
+def _pct(a, q):
+    return round(float(np.percentile(a, q)), 4) if len(a) else None
+
+
+def serve(requests, budget, schedule=None, keep_results=False, max_round=64, log=None):
     """Run the requests to completion; returns (stats, results by request id)."""
     pool = ScenePool(budget)
     if schedule:
         pool.set_schedule(**schedule)
     pending = sorted(requests, key=lambda r: r["arrive"])
     waiting, live, results, done = [], {}, {}, []
-    now = 0                      # pool steps taken so far
-    step_ms = []                 # wall time per step, per round
-    cell_steps = 0
+    step_ms, rebuild_ms, queue_s = [], [], []
+    cell_steps, steps, max_live = 0, 0, 0
     t_start = time.perf_counter()
+    clock = lambda: time.perf_counter() - t_start
     while pending or waiting or live:
+        now = clock()
         while pending and pending[0]["arrive"] <= now:
             waiting.append(pending.pop(0))
-        # 1. admission by memory (arrival order, smaller requests may pass a blocked one)
+        # admission by memory (arrival order; smaller requests may pass a blocked one)
+        changed = False
         for r in list(waiting):
             c = r["cfg"]
             got = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
@@ -75,57 +88,65 @@ def serve(requests, budget, schedule=None, keep_results=False, max_round=8, log=
             scenes.apply(sim, c)
             sim.clear_geometry()
             sim.encode_geometry()
-            live[slot] = {"req": r, "sim": sim, "left": r["steps"], "admitted": now,
-                          "t_admit": time.perf_counter()}
+            live[slot] = {"req": r, "sim": sim, "left": r["steps"], "t_admit": clock()}
+            queue_s.append(clock() - r["arrive"])
             waiting.remove(r)
+            changed = True
         if not live:
             if waiting:
                 raise ValueError("request %d needs more than the pool's HBM budget" % waiting[0]["id"])
-            now = int(np.ceil(pending[0]["arrive"])) if pending else now
+            if pending:   # idle until the next arrival
+                time.sleep(max(0.0, pending[0]["arrive"] - clock()))
             continue
-        # 2. advance every live scene together
+        if changed:
+            t0 = time.perf_counter()
+            pool.prepare()
+            rebuild_ms.append((time.perf_counter() - t0) * 1e3)
+        max_live = max(max_live, len(live))
+        # advance every live scene together
         k = min(max_round, min(v["left"] for v in live.values()))
-        if pending:
-            k = max(1, min(k, int(np.ceil(pending[0]["arrive"] - now))))
-        t0 = time.perf_counter()
-        pool.step(k)
-        pool.synchronize()
-        dt = time.perf_counter() - t0
-        step_ms.extend([dt * 1e3 / k] * k)
+        lat = pool.step_latencies(k)
+        step_ms.extend(float(x) for x in lat)
         cells = sum(v["sim"].nx * v["sim"].ny for v in live.values())
         cell_steps += cells * k
-        now += k
-        # 3. finished requests leave
+        steps += k
+        # finished requests leave
+        released = False
         for slot in [s for s, v in live.items() if v["left"] - k <= 0]:
             v = live.pop(slot)
             if keep_results:
                 results[v["req"]["id"]] = result_of(v["sim"])
             done.append({"id": v["req"]["id"], "n": v["sim"].nx, "steps": v["req"]["steps"],
-                         "queued_steps": v["admitted"] - v["req"]["arrive"],
-                         "wall_s": time.perf_counter() - v["t_admit"]})
+                         "wall_s": clock() - v["t_admit"]})
             pool.release(slot)
+            released = True
         for v in live.values():
             v["left"] -= k
-        if log and len(done) % 8 == 0 and done:
-            log("[serve] step %d: %d live, %d waiting, %d done" % (now, len(live), len(waiting), len(done)))
-    wall = time.perf_counter() - t_start
+        if released and live:
+            t0 = time.perf_counter()
+            pool.prepare()
+            rebuild_ms.append((time.perf_counter() - t0) * 1e3)
+        if log and released:
+            log("[serve] t %.2f s: %d live, %d waiting, %d done" % (clock(), len(live), len(waiting), len(done)))
+    wall = clock()
     pool.close()
-    lat = np.array(step_ms)
-    stats = {"requests": len(requests), "pool_steps": now, "wall_s": round(wall, 3),
-             "cell_steps_per_s": round(cell_steps / wall, 1),
+    stats = {"requests": len(requests), "pool_steps": steps, "wall_s": round(wall, 3),
+             "arrival_span_s": round(max(r["arrive"] for r in requests), 3),
+             "cell_steps": int(cell_steps), "max_live": max_live,
              "mcell_steps_per_s": round(cell_steps / wall / 1e6, 1),
-             "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
-                                 "p99": round(float(np.percentile(lat, 99)), 4)},
-             "mean_queued_steps": round(float(np.mean([d["queued_steps"] for d in done])), 1)}
+             "step_latency_ms": {"p50": _pct(step_ms, 50), "p99": _pct(step_ms, 99)},
+             "queue_s": {"p50": _pct(queue_s, 50), "p99": _pct(queue_s, 99)},
+             "rebuild_ms": {"p50": _pct(rebuild_ms, 50), "p99": _pct(rebuild_ms, 99), "count": len(rebuild_ms)},
+             "device_busy_frac": round(sum(step_ms) / 1e3 / wall, 3)}
     return stats, results
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=24)
-    ap.add_argument("--rate", type=float, default=0.05, help="arrivals per pool step")
+    ap.add_argument("--rate", type=float, default=4.0, help="arrivals per second")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--budget-gb", type=float, default=8.0)
+    ap.add_argument("--budget-gb", type=float, default=2.0)
     ap.add_argument("--min-steps", type=int, default=500)
     ap.add_argument("--max-steps", type=int, default=5000)
     a = ap.parse_args()

@@ -268,6 +268,12 @@ int  cf_pool_admit(cf_pool* pool, const cf_scene_desc* desc, int* slot);
 int  cf_pool_scene(cf_pool* pool, int slot, cf_sim** sim);
 /* Frees a slot and its device memory. */
 int  cf_pool_release(cf_pool* pool, int slot);
+/* Builds the batched step of the live scenes now (tables and step graphs),
+ * which cf_pool_step otherwise does on its first call after an admission or
+ * a release: a serving loop calls it right after changing the scene set, so
+ * the rebuild is not on the next step's latency.  Waits for the pool's
+ * stream. */
+int  cf_pool_prepare(cf_pool* pool);
 /* Every live scene advances nsteps steps (asynchronous; with a residual
  * target the host waits between V-cycles as in cf_sim_step). */
 int  cf_pool_step(cf_pool* pool, int nsteps);

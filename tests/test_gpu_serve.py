@@ -20,13 +20,16 @@ def test_served_results_equal_alone_runs():
     from campfire_amd.fluid import FluidSim
     from campfire_amd.pool import scene_bytes
     from campfire_amd.serve import make_requests, serve
-    reqs = make_requests(8, rate=0.08, seed=3, sizes=(128, 256), steps=(16, 64), width=0.64)
-    # room for about three scenes: later arrivals wait for memory
-    budget = 3 * scene_bytes(256, 256, 0.0025) + scene_bytes(128, 128, 0.005)
+    reqs = make_requests(10, rate=300.0, seed=3, sizes=(128, 256, 512, 1024), steps=(16, 64), width=0.64)
+    _flush("[serve] sizes", [r["cfg"]["nx"] for r in reqs], "steps", [r["steps"] for r in reqs])
+    assert {512, 1024} <= {r["cfg"]["nx"] for r in reqs}
+    # room for a 1024^2, a 512^2 and two 256^2 scenes: later arrivals wait for memory
+    budget = (scene_bytes(1024, 1024, 0.000625) + scene_bytes(512, 512, 0.00125)
+              + 2 * scene_bytes(256, 256, 0.0025))
     stats, results = serve(reqs, budget, keep_results=True)
     _flush("[serve]", stats)
-    assert stats["requests"] == 8 and len(results) == 8
-    assert stats["mean_queued_steps"] >= 0
+    assert stats["requests"] == 10 and len(results) == 10
+    assert stats["max_live"] >= 2
     for r in reqs:
         c = r["cfg"]
         a = FluidSim({"nx": c["nx"], "ny": c["ny"], "dt": c["dt"]}, dx=c["dx"])
@@ -40,15 +43,21 @@ def test_served_results_equal_alone_runs():
             assert np.array_equal(a.read_field(name), got[name]), (r["id"], name)
         assert np.array_equal(a.readback_now()["acc"], got["acc"]), r["id"]
         a.close()
-    _flush("[serve] all 8 requests identical to their alone runs")
+    _flush("[serve] all 10 requests identical to their alone runs")
 
 
 def test_serving_statistics_at_scale():
-    """A short mixed-size serving session at the real sizes (256^2 to 1024^2):
-    p50/p99 step latency and aggregate cell-steps/s."""
+    """A short mixed-size serving session at the real sizes (256^2 to 1024^2,
+    Poisson arrivals at 40/s, 200-800 steps each, a 1 GiB budget that holds
+    about six 1024^2 scenes): requests overlap on the device, some wait for
+    memory, and the aggregate rate is that of batched stepping, well above one
+    scene at a time."""
     from campfire_amd.serve import make_requests, serve
-    reqs = make_requests(16, rate=0.2, seed=5, steps=(40, 200))
-    stats, _ = serve(reqs, 4 << 30)
-    _flush("[serve] 16 requests (40-200 steps, 256^2..1024^2, 4 GiB budget):", stats)
+    reqs = make_requests(16, rate=40.0, seed=5, steps=(200, 800))
+    stats, _ = serve(reqs, 1 << 30)
+    _flush("[serve] 16 requests (200-800 steps, 256^2..1024^2, 1 GiB budget):", stats)
     assert stats["step_latency_ms"]["p99"] >= stats["step_latency_ms"]["p50"] > 0
-    assert stats["mcell_steps_per_s"] > 0
+    assert stats["max_live"] >= 3
+    assert stats["queue_s"]["p99"] > 0
+    assert stats["device_busy_frac"] > 0.5
+    assert stats["mcell_steps_per_s"] > 2000
