@@ -101,17 +101,14 @@ def _symbol(kname):
 
 
 def pmc_child(args):
-    """Body of a counter run: a plain 256 MiB elementwise device copy (the
-    calibration kernel), then a few steps of the workload dispatched one kernel
-    at a time."""
+    """Body of a counter run: the library's streaming copy of 256 MiB (the
+    calibration kernel, k_bw_copy), then a few steps of the workload
+    dispatched one kernel at a time."""
     os.environ["CF_NO_GRAPH"] = "1"
-    import torch
-    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        torch.mul(a, 1.0, out=b)      # an elementwise copy: plain vector loads and stores
-    torch.cuda.synchronize()
-    del a, b
+    import ctypes as C
+    from campfire_amd import _lib
+    g = C.c_double(0)
+    _lib.check(_lib.lib().cf_copy_bandwidth(COPY_BYTES, 3, C.byref(g), None), "copy")
     from campfire_amd import scenes
     from campfire_amd.fluid import FluidSim
     cfg = bench_config(args)
@@ -168,18 +165,18 @@ def measure_traffic(args):
         for r in rows:
             v = float(r["Counter_Value"])
             sym = _symbol(r["Kernel_Name"])
-            if sym is None:
+            if sym is not None and sym[0] == "k_bw_copy":
                 copy.append((v, r["Kernel_Name"]))
-            else:
+            elif sym is not None:
                 per.setdefault(sym + (int(r.get("Grid_Size") or 0),), []).append(v)
         out[counter] = (per, copy)
-    # calibration: the copy kernel is the non-gas kernel that moves the most
-    fetch_copy = max(out["FETCH_SIZE"][1], default=(0, ""))
-    name = fetch_copy[1]
-    write_copy = [v for v, n in out["WRITE_SIZE"][1] if n == name]
-    if fetch_copy[0] <= 0 or not write_copy or max(write_copy) <= 0:
+    # calibration: the library's 256 MiB copy (k_bw_copy), median over its launches
+    fetch_copy = [v for v, n in out["FETCH_SIZE"][1]]
+    write_copy = [v for v, n in out["WRITE_SIZE"][1]]
+    name = out["FETCH_SIZE"][1][0][1] if out["FETCH_SIZE"][1] else ""
+    if not fetch_copy or not write_copy or min(fetch_copy) <= 0 or min(write_copy) <= 0:
         return None, "copy calibration kernel not found"
-    f_scale = COPY_BYTES / (fetch_copy[0] * 1024.0)
+    f_scale = COPY_BYTES / (float(np.median(fetch_copy)) * 1024.0)
     w_scale = COPY_BYTES / (float(np.median(write_copy)) * 1024.0)
     per = {}
     for sym in set(out["FETCH_SIZE"][0]) | set(out["WRITE_SIZE"][0]):
@@ -577,6 +574,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0, help="request arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=2.0, help="HBM budget of the serving pool")
     ap.add_argument("--no-modes", action="store_true", help="skip the reference-mode frames")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default leg tiles (no autotuning)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -601,8 +599,15 @@ def main():
     cfg = bench_config(args)
     nx, ny = cfg["nx"], cfg["ny"]
     log("[bench] %dx%d h=%.3g mm dt=%.3g ms" % (nx, ny, cfg["dx"] * 1e3, cfg["dt"] * 1e3))
+    # the leg tiles of this grid, by the autotuner (results do not depend on them)
+    tiles = None
+    if not args.no_tune:
+        from campfire_amd.autotune import tune_leg_tiles
+        tiles, _ = tune_leg_tiles(nx, ny, cfg["dx"], cfg["dt"], log=log)
     sim = FluidSim({"nx": nx, "ny": ny, "dt": cfg["dt"]}, dx=cfg["dx"])
     scenes.apply(sim, cfg)
+    if tiles:
+        sim.set_leg_tiles(tiles)
     levels = sim.levels
     # burn-in: app frames (geometry, 8 steps, radiation) so the scene is reacting
     for f in range(args.burnin_frames):
@@ -662,6 +667,15 @@ def main():
     roof = {"kernel": dom[0], "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr,
             "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(per_launch_ms * 1e3, 2)}
+    # the practical peak: the library's streaming copy of 1 GiB (2 GiB moved per launch)
+    import ctypes as C
+    from campfire_amd import _lib
+    cg, cms = C.c_double(0), C.c_double(0)
+    if _lib.lib().cf_copy_bandwidth(1 << 30, 10, C.byref(cg), C.byref(cms)) == 0:
+        roof["copy_gbs"] = round(cg.value, 1)
+        roof["frac_of_copy"] = round(achieved / cg.value, 4)
+        log("[bench] copy kernel (k_bw_copy, 1 GiB): %.1f GB/s; %s at %.3f of it" % (cg.value, dom[0],
+                                                                                 achieved / cg.value))
     if traffic:
         roof["traffic_calibration"] = traffic[1]
     elif traffic_err:
@@ -677,7 +691,8 @@ def main():
         "config": {"workload": "%dx%d reacting gas step, %d-piece seeded pile, h=%.3g mm, dt=%.3g ms, "
                                "reference schedule (1 V-cycle, nu=2, 120-sweep coarsest)"
                                % (nx, ny, cfg["n"], cfg["dx"] * 1e3, cfg["dt"] * 1e3),
-                   "grid": [nx, ny], "levels": len(levels), "diff_sub": sim.diff_sub, "seed": args.seed},
+                   "grid": [nx, ny], "levels": len(levels), "diff_sub": sim.diff_sub, "seed": args.seed,
+                   "leg_tiles": sim.leg_tiles()},
         "roofline": roof,
         "kernels_ms_per_step": {r[0]: round(r[1], 5) for r in rows},
         "kernels_gbs": {r[0]: round(r[4], 1) for r in rows if r[4]},
@@ -687,6 +702,11 @@ def main():
     if lat is not None:
         out["step_latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 5),
                                   "p99": round(float(np.percentile(lat, 99)), 5)}
+    # the per-frame kernels of the same scene (geometry of 12 pieces; radiation
+    # of 4 x 12 faces + 64 floor probes)
+    geo_ms, rad_ms = sim.profile_frame(10)
+    out["per_frame_ms"] = {"geometry": round(geo_ms, 5), "radiation": round(rad_ms, 5)}
+    log("[bench] per frame: geometry %.4f ms, radiation %.4f ms" % (geo_ms, rad_ms))
     sim.close()
     if args.plain_steps > 0:
         out["plain_256"] = plain_256(args, args.plain_steps)

@@ -96,6 +96,8 @@ def lib():
         "cf_param_name": ([i], cp),
         "cf_sim_set_schedule": ([p, C.POINTER(CfSchedule)], i),
         "cf_sim_get_schedule": ([p, C.POINTER(CfSchedule)], i),
+        "cf_sim_set_leg_tiles": ([p, vp, i], i),
+        "cf_sim_get_leg_tiles": ([p, vp, i], i),
         "cf_sim_upload_pieces": ([p, vp, i], i),
         "cf_sim_clear_geometry": ([p], i),
         "cf_sim_geometry": ([p], i),
@@ -106,6 +108,7 @@ def lib():
         "cf_sim_profile_steps": ([p, i, vp], i),
         "cf_sim_profile_kernels": ([p, i, i, vp, vp, vp, C.POINTER(i)], i),
         "cf_sim_step_latencies": ([p, i, vp], i),
+        "cf_sim_profile_frame": ([p, i, C.POINTER(f), C.POINTER(f)], i),
         "cf_sim_synchronize": ([p], i),
         "cf_sim_status": ([p], i),
         "cf_sim_encode_readback": ([p], i),
@@ -120,6 +123,7 @@ def lib():
         "cf_flame1d_run": ([C.POINTER(CfFlameCase), i, i, C.POINTER(CfFlameResult)], i),
         "cf_flame1d_run_static": ([C.POINTER(CfFlameCase), i, i, C.POINTER(CfFlameResult), vp], i),
         "cf_scene_bytes": ([i, i, C.c_double], C.c_uint64),
+        "cf_copy_bandwidth": ([C.c_uint64, i, C.POINTER(d), C.POINTER(d)], i),
         "cf_pool_create": ([C.c_uint64, C.POINTER(p)], i),
         "cf_pool_destroy": ([p], None),
         "cf_pool_set_schedule": ([p, C.POINTER(CfSchedule)], i),

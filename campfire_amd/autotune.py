@@ -1,4 +1,14 @@
-"""Autotuner of the multigrid schedule for residual-controlled stepping.
+"""Autotuners: the multigrid schedule for residual-controlled stepping, and
+the launch shape (tile edge) of the tiled multigrid legs per level.
+
+The leg tiles do not change results (a tile with a halo of 2 nu + 1 cells
+reproduces the global red-black half-sweeps exactly), only speed, which
+depends on the level's size and the device:
+
+    tiles, table = tune_leg_tiles(1024, 1024, 0.00125, 0.00025)
+    sim.set_leg_tiles(tiles)
+
+Schedule autotuner:
 
 For a mix of scene sizes it builds a probe pool (one seeded burning scene
 per size), warms it up, snapshots every scene on the device, and then, from
@@ -83,3 +93,51 @@ def tune(sizes=(256, 512, 1024), tol=1e-6, refine=1, cfl=0.5, umin=1.0, vcycles=
     good = [r for r in table if r[2]]
     best = min(good, key=lambda r: r[1])[0] if good else None
     return best, table
+
+
+TILES = (16, 32, 64)
+
+
+def _median_step_ms(sim, twin, steps):
+    sim.copy_state_from(twin)
+    sim.step(1)                       # captures the step graph of the current tiles
+    sim.copy_state_from(twin)
+    return float(np.median(sim.step_latencies(steps)))
+
+
+def tune_leg_tiles(nx, ny, dx, dt, steps=20, warm=16, seed=1, log=None):
+    """The leg tile edge of each tiled level of an nx x ny scene (reference
+    schedule), chosen level by level from the finest: each candidate is timed
+    over `steps` steps from the same device snapshot of a burning probe scene
+    (median step time).  Returns (tiles, [(tiles tried, ms per step)])."""
+    log = log or (lambda *a: None)
+    cfg = probe_config(nx, seed, width=nx * dx)
+    cfg.update(ny=ny, dx=dx, dt=dt)
+    sim = FluidSim({"nx": nx, "ny": ny, "dt": dt}, dx=dx)
+    scenes.apply(sim, cfg)
+    for f in range(max(1, warm // 8)):
+        sim.frame = f
+        sim.frame_all(8, True)
+    sim.synchronize()
+    twin = FluidSim({"nx": nx, "ny": ny, "dt": dt}, dx=dx)
+    twin.copy_state_from(sim)
+    tiles = sim.leg_tiles()
+    table = [(list(tiles), _median_step_ms(sim, twin, steps))]
+    log("[autotune] %dx%d leg tiles %s (size rule): %.4f ms/step" % (nx, ny, tiles, table[0][1]))
+    best_ms = table[0][1]
+    for l in range(len(tiles)):
+        for t in TILES:
+            if t == tiles[l]:
+                continue
+            trial = list(tiles)
+            trial[l] = t
+            sim.set_leg_tiles(trial)
+            ms = _median_step_ms(sim, twin, steps)
+            table.append((trial, ms))
+            log("[autotune] %dx%d leg tiles %s: %.4f ms/step" % (nx, ny, trial, ms))
+            if ms < 0.99 * best_ms:  # a change must beat the noise of the timing
+                best_ms, tiles = ms, trial
+        sim.set_leg_tiles(tiles)
+    twin.close()
+    sim.close()
+    return tiles, table

@@ -211,6 +211,7 @@ struct Fields {
   int *acc, *faceW;
   float4* rad;
   float *p0, *rhs0, *ax0, *ay0;
+  float2* G;  // radiation: each cell's gas terms of the frame (emission, visible emission)
 };
 
 // One multigrid level (the `Level` uniform of shaders/pressure.wgsl:7 and its arrays).
@@ -224,6 +225,7 @@ struct Level {
   // smoothed values in its halo.  The down leg writes q, the up leg reads q
   // and writes p.
   float* q;
+  int tile;  // edge of the tiled legs' tiles at this level (16, 32, 64); 0: the size rule of mg_tile
 };
 
 // FP16 storage of a level for the FP16 smoother (schedule smoother_fp16 = 1):
@@ -267,8 +269,10 @@ bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, c
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
 // FP16 copies of a level's face coefficients (once per step, FP16 smoother)
 void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s);
-// tile edge (32 or 16) of a level's legs; a batched phase holds one tile size
+// tile edge (64, 32 or 16) of a level's legs: Level::tile, or by default a
+// rule on the level's size; a batched phase holds one tile size
 int mg_tile(const Level& f);
+int mg_tile_rule(const Level& f);
 // first level of the single-workgroup bottom solver (the last 1-3 levels)
 int mg_bottom_first(const Level* L, int nlev);
 // coarse_coef: restrict the coarsest level's coefficients inside the kernel
@@ -325,8 +329,11 @@ void launch_coarsest(const Level& L, int iters, hipStream_t s);
 // p64 (may be null): gradients from the FP64 pressure of defect correction
 void launch_project(const Fields& F, const double* p64, hipStream_t s);
 void launch_geometry(const Fields& F, hipStream_t s);
+// radiation: the gas terms of every cell into F.G, then the rays of ntargets targets
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s);
 void launch_fill_u32(void* a, uint32_t v, size_t n, hipStream_t s);
+// a streaming copy of `bytes` (a multiple of 16): the bandwidth reference
+void launch_bw_copy(const void* a, void* b, size_t bytes, hipStream_t s);
 // tests/flame1d.mjs on the GPU (cf_flame1d.hip): one workgroup per case
 void launch_flame1d(const cf_flame_case* cases, int ncases, bool fp64, cf_flame_result* out, hipStream_t s);
 // the checks of one flame case (CF_OK, or a code with the message set)
@@ -356,6 +363,7 @@ void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dy
 void phase_coef_half(BatchPhase& ph, const Level& L, const LevelH& H);
 void phase_project(BatchPhase& ph, const Fields& F, const double* p64);
 void phase_geometry(BatchPhase& ph, const Fields& F);
+void phase_gas_terms(BatchPhase& ph, const Fields& F);
 void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets);
 void phase_fill_u32(BatchPhase& ph, void* a, uint32_t v, size_t n);
 bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip);

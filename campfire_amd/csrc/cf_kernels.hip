@@ -1563,14 +1563,18 @@ struct PiecePose {
 };
 // signed distance to the nearest active piece and its face id
 // (shaders/geometry.wgsl:20-29, the box SDF of common.wgsl:47-62), from the
-// poses computed once per workgroup
-__device__ __forceinline__ float nearest_pose(const PiecePose* __restrict__ pp, int np, float px, float py,
-                                              uint32_t& bid) {
+// poses computed once per workgroup.  `cand` lists, in increasing index
+// order, the pieces that can be nearest anywhere in the workgroup's
+// rectangle (every other piece is farther everywhere there), so the loop
+// over them finds the same minimum and the same first piece reaching it as
+// the loop over all pieces.
+__device__ __forceinline__ float nearest_pose(const PiecePose* __restrict__ pp, const int* __restrict__ cand, int nc,
+                                              float px, float py, uint32_t& bid) {
   float best = K(1e9);
   bid = CF_SOLID_NONE;
-  for (int n = 0; n < np; n++) {
+  for (int c = 0; c < nc; c++) {
+    const int n = cand[c];
     const PiecePose q = pp[n];
-    if (!q.on) continue;
     const float dxp = px - q.x, dyp = py - q.y;
     const float lx = q.c * dxp + q.s * dyp, ly = -q.s * dxp + q.c * dyp;
     const float qx = fabsf(lx) - q.h, qy = fabsf(ly) - q.h;
@@ -1601,20 +1605,49 @@ struct GeometryOp {
   // the pieces' poses, once per workgroup: every thread evaluates each piece
   // at three points, and the angle's cos and sin would dominate
   __shared__ PiecePose pp[CF_MAX_PIECES];
+  __shared__ float lo[CF_MAX_PIECES], hi[CF_MAX_PIECES];
+  __shared__ int cand[CF_MAX_PIECES];
+  __shared__ int ncand;
   const int np = min(P.nPieces, CF_MAX_PIECES);
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const float dx = P.dx;
+  // the workgroup's points (cell centres and the faces of 32 x 8 indices)
+  // lie in the rectangle [rx0, rx1] x [ry0, ry1]
+  const int gx = (nx + 1 + 31) / 32;
+  const float rx0 = (float)((vb % gx) * 32) * dx, ry0 = (float)((vb / gx) * 8) * dx;
+  const float rx1 = rx0 + K(32) * dx, ry1 = ry0 + K(8) * dx;
   if (tid < np) {
     const float* pc = F.pieces + tid * CF_PIECE_FLOATS;
     const bool on = pc[7] > K(0);
     pp[tid] = PiecePose{pc[0], pc[1], cosf(pc[2]), sinf(pc[2]), pc[3], on ? 1 : 0};
+    // bounds of the piece's SDF over the rectangle: a point at distance r from
+    // the centre is at least r - sqrt(2) h (the circumscribed circle) and at
+    // most r - h (the inscribed circle) from the box, inside or out
+    const float ex = fmaxf(fmaxf(rx0 - pc[0], pc[0] - rx1), K(0)), ey = fmaxf(fmaxf(ry0 - pc[1], pc[1] - ry1), K(0));
+    const float fx_ = fmaxf(fabsf(pc[0] - rx0), fabsf(pc[0] - rx1)), fy_ = fmaxf(fabsf(pc[1] - ry0), fabsf(pc[1] - ry1));
+    lo[tid] = on ? sqrtf(ex * ex + ey * ey) - K(1.41421357) * pc[3] : K(3e38);
+    hi[tid] = on ? sqrtf(fx_ * fx_ + fy_ * fy_) - pc[3] : K(3e38);
   }
   __syncthreads();
+  if (tid == 0) {
+    // a piece can be nearest somewhere in the rectangle only if its lower
+    // bound is below every piece's upper bound there; the margin covers the
+    // rounding of the bounds and of the SDFs
+    float mhi = K(3e38);
+    for (int n = 0; n < np; n++) mhi = fminf(mhi, hi[n]);
+    const float lim = mhi + K(1e-5) + K(1e-3) * dx;
+    int c = 0;
+    for (int n = 0; n < np; n++)
+      if (pp[n].on && lo[n] <= lim) cand[c++] = n;
+    ncand = c;
+  }
+  __syncthreads();
+  const int nc = ncand;
   int i, j;
   tile_ij<32, 8>(vb, nx + 1, ny + 1, i, j);
-  const float dx = P.dx;
   uint32_t id;
   if (i < nx && j < ny) {
-    const float d = nearest_pose(pp, np, ((float)i + K(0.5)) * dx, ((float)j + K(0.5)) * dx, id);
+    const float d = nearest_pose(pp, cand, nc, ((float)i + K(0.5)) * dx, ((float)j + K(0.5)) * dx, id);
     F.cellG[j * nx + i] = make_float2(d, __uint_as_float(id));
     if (id != CF_SOLID_NONE && d >= K(0) && d < K(2.0) * dx) {
       const float w = fmaxf(K(0), K(1) - fabsf(d - dx) / dx) / dx;
@@ -1622,11 +1655,11 @@ struct GeometryOp {
     }
   }
   if (i <= nx && j < ny) {
-    const float d = nearest_pose(pp, np, (float)i * dx, ((float)j + K(0.5)) * dx, id);
+    const float d = nearest_pose(pp, cand, nc, (float)i * dx, ((float)j + K(0.5)) * dx, id);
     F.fx[j * (nx + 1) + i] = clampf(K(0.5) + d / dx, K(0), K(1));
   }
   if (i < nx && j <= ny) {
-    const float d = nearest_pose(pp, np, ((float)i + K(0.5)) * dx, (float)j * dx, id);
+    const float d = nearest_pose(pp, cand, nc, ((float)i + K(0.5)) * dx, (float)j * dx, id);
     F.fy[j * nx + i] = j == 0 ? K(0) : clampf(K(0.5) + d / dx, K(0), K(1));
   }
 
@@ -1673,19 +1706,45 @@ __device__ __forceinline__ void gasTerms(const cf_params& P, float Tg, float4 Yg
   g0 = kap * P.sigma * (T2 * T2 - powf(P.Tamb, K(4.0))) / CF_PI;
   g1 = K(6.3) * fv / K(580e-9) * planck580(Tg);
 }
+// gasAt's two terms (:53-62) of every cell, once per frame: a ray sample
+// then reads 8 bytes instead of T and Y (20 bytes) and evaluates no
+// exponential.  The terms depend on the cell alone, so the integrals are
+// unchanged.
+struct GasTermsOp {
+  struct Args {
+    Fields F;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Fields& F = a_.F;
+    const cf_params P = *F.P;
+    const int n = P.nx * P.ny;
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      float g0, g1;
+      gasTerms(P, F.T[k], F.Y[k], g0, g1);
+      F.G[k] = make_float2(g0, g1);
+    }
+  }
+};
+__global__ __launch_bounds__(256) void k_gas_terms(const GasTermsOp::Args a_) {
+  GasTermsOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // One workgroup per target, 64 rays (:64-158).  Each ray's march is split
 // over RG lanes: lane g takes the samples m = g, g + RG, ... of every
 // segment, and the lanes' partial integrals are added by a fixed tree
 // (shuffles over the group).  The per-ray terms then meet in LDS and one wave
 // adds them over the rays in the reference's workgroup tree order.  With
-// RG = 8 a target is 8 waves, so a scene's 4·nPieces + 64 targets fill the
-// chip's SIMDs instead of one wave each.
+// RG = 16 a target is 16 waves, so a scene's 4·nPieces + 64 targets fill the
+// chip's SIMDs instead of one wave each; the ray-piece tests of a segment are
+// shared among the group's lanes too.
 struct RadiationOp {
   struct Args {
     Fields F;
     int vg;  // tiles of this scene's launch
   };
-  static constexpr int RG = 8;
+  static constexpr int RG = 16;
   static constexpr int kThreads = 64 * RG;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Fields& F = a_.F;
@@ -1733,15 +1792,31 @@ struct RadiationOp {
       bool bounced = false;
       const float ds = P.dx;
       const float W = (float)P.nx * P.dx, H = (float)P.ny * P.dx;
+      const int npc = min(P.nPieces, CF_MAX_PIECES);
       for (int step = 0; step < 4096; step++) {
-        float tHit = K(1e9), hitT = K(0);
-        for (int n = 0; n < min(P.nPieces, CF_MAX_PIECES); n++) {
+        // the nearest piece along the ray: lane g of the ray's group tests
+        // pieces g, g + RG, ...; the group then takes the smallest entry
+        // distance, the lowest piece index among equal ones, which is the
+        // piece the reference's loop in index order keeps (:108-113)
+        float tHit = K(1e9);
+        int hitN = -1, hitF = 0;
+        for (int n = g; n < npc; n += RG) {
           const float* pc = F.pieces + n * CF_PIECE_FLOATS;
           if (!(pc[7] > K(0))) continue;
           float face;
           const float hx = hitBox(ox, oy, dxr, dyr, pc, face);
-          if (hx > K(0) && hx < tHit) { tHit = hx; hitT = pc[8 + (int)face]; }
+          if (hx > K(0) && hx < tHit) { tHit = hx; hitN = n; hitF = (int)face; }
         }
+#pragma unroll
+        for (int o = RG / 2; o > 0; o >>= 1) {
+          const float t2 = __shfl_xor(tHit, o, RG);
+          const int n2 = __shfl_xor(hitN, o, RG), f2 = __shfl_xor(hitF, o, RG);
+          const bool take = t2 < tHit || (t2 == tHit && n2 >= 0 && (hitN < 0 || n2 < hitN));
+          tHit = take ? t2 : tHit;
+          hitN = take ? n2 : hitN;
+          hitF = take ? f2 : hitF;
+        }
+        const float hitT = hitN >= 0 ? F.pieces[hitN * CF_PIECE_FLOATS + 8 + hitF] : K(0);
         float tFloor = K(1e9);
         if (dyr < K(-1e-6)) tFloor = -oy / dyr;
         const float tEnd = fminf(tHit, tFloor);
@@ -1751,12 +1826,11 @@ struct RadiationOp {
         const float tOut = fminf(fminf(tx, ty), tEnd);
         const int ns = toint(ceilf(tOut / ds));
         // gasAt (:53-62) at this lane's samples, RU at a time: every sample's
-        // T and Y are loaded before any is used (a sample outside the grid
+        // terms are loaded before any is used (a sample outside the grid
         // loads cell 0 and contributes 0)
-        constexpr int RU = 4;
+        constexpr int RU = 8;
         for (int m0 = g; m0 < ns; m0 += RU * RG) {
-          float Tg[RU];
-          float4 Yg[RU];
+          float2 gg[RU];
           bool in[RU];
 #pragma unroll
           for (int u = 0; u < RU; u++) {
@@ -1765,14 +1839,11 @@ struct RadiationOp {
             const float px = ox + dxr * t, py = oy + dyr * t;
             const int i = toint(px / P.dx), j = toint(py / P.dx);
             in[u] = m < ns && !(i < 0 || j < 0 || i >= P.nx || j >= P.ny);
-            const int k = in[u] ? j * P.nx + i : 0;
-            Tg[u] = F.T[k];
-            Yg[u] = F.Y[k];
+            gg[u] = F.G[in[u] ? j * P.nx + i : 0];
           }
 #pragma unroll
           for (int u = 0; u < RU; u++) {
-            float g0 = K(0), g1 = K(0);
-            if (in[u]) gasTerms(P, Tg[u], Yg[u], g0, g1);
+            const float g0 = in[u] ? gg[u].x : K(0), g1 = in[u] ? gg[u].y : K(0);
             J += g0 * tOut / (float)ns;
             Jv += g1 * tOut / (float)ns;
           }
@@ -1841,6 +1912,20 @@ struct FillOp {
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
   const FillOp::Args a_{a, v, n, (int)gridDim.x};
   FillOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// The bandwidth reference: a plain streaming copy, 16 B per lane per access,
+// as many workgroups as fill the chip a few times over, each striding the
+// array.  Its rate is the practical HBM peak the stencil kernels are held to.
+__global__ __launch_bounds__(256) void k_bw_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += stride) b[k] = a[k];
+}
+void launch_bw_copy(const void* a, void* b, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  unsigned g = (unsigned)((n + 255) / 256);
+  if (g > 256 * 16) g = 256 * 16;
+  launch(k_bw_copy, dim3(g), dim3(256), 0, s, (const float4*)a, (float4*)b, n);
 }
 
 // |div(f u) - S| after projection (the H.divErr diagnostic, dev/gas_test.html:101-111):
@@ -1973,9 +2058,15 @@ void launch_geometry(const Fields& F, hipStream_t s) {
   const int nb = (int)nblk(F.nx + 1, F.ny + 1);
   launch(k_geometry, dim3(nb), B328, 0, s, GeometryOp::Args{F, nb});
 }
+static int gas_terms_blocks(const Fields& F) {
+  int b = (F.nx * F.ny + 255) / 256;
+  return b > 2048 ? 2048 : b;
+}
 void launch_radiation(const Fields& F, int ntargets, hipStream_t s) {
-  if (ntargets > 0)
-    launch(k_radiation, dim3(ntargets), dim3(RadiationOp::kThreads), 0, s, RadiationOp::Args{F, ntargets});
+  if (ntargets <= 0) return;
+  const int gb = gas_terms_blocks(F);
+  launch(k_gas_terms, dim3(gb), dim3(256), 0, s, GasTermsOp::Args{F, gb});
+  launch(k_radiation, dim3(ntargets), dim3(RadiationOp::kThreads), 0, s, RadiationOp::Args{F, ntargets});
 }
 void launch_cfl(const Fields& F, float cfl, float umin, int nsub, hipStream_t s) {
   const int n = (F.nx + 1) * F.ny > F.nx * (F.ny + 1) ? (F.nx + 1) * F.ny : F.nx * (F.ny + 1);
@@ -2126,6 +2217,10 @@ void phase_project(BatchPhase& ph, const Fields& F, const double* p64) {
 void phase_geometry(BatchPhase& ph, const Fields& F) {
   const int nb = (int)nblk(F.nx + 1, F.ny + 1);
   phase_add<GeometryOp>(ph, GeometryOp::Args{F, nb}, nb, B328, "geometry");
+}
+void phase_gas_terms(BatchPhase& ph, const Fields& F) {
+  const int gb = gas_terms_blocks(F);
+  phase_add<GasTermsOp>(ph, GasTermsOp::Args{F, gb}, gb, dim3(256), "gas_terms");
 }
 void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets) {
   if (ntargets > 0)

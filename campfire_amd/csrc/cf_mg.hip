@@ -803,10 +803,13 @@ static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
 // where it has at least 1024 of those, 16x16 below: a small level then has
 // four times the workgroups, each with half the per-thread chain, and its
 // legs are a shorter latency chain
-int mg_tile(const Level& f) {
+int mg_tile_rule(const Level& f) {
   if (ntiles(f.nx, f.ny, 64, 64) >= 256) return 64;
   return ntiles(f.nx, f.ny, 32, 32) >= 1024 ? 32 : 16;
 }
+// a tile chosen per level (the autotuner's) overrides the rule; every tile
+// gives the same results, the legs being exact at any tile size
+int mg_tile(const Level& f) { return f.tile ? f.tile : mg_tile_rule(f); }
 
 // dispatch on (tile, nu, storage mode) to the instantiation
 template <template <int, int, int, int> class Op, class F>

@@ -131,7 +131,8 @@ struct cf_sim {
   hipGraphExec_t gexec8 = nullptr;  // eight steps: one replay per 8 steps amortises the launch
   Segments seg;                     // residual-controlled steps: the step in segments
   int* hflag = nullptr;             // pinned host copy of the convergence flag
-  int gkey[11] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int gkey[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  uint32_t tile_gen = 0;  // bumped when a level's leg tile changes (the graphs bake the tiles in)
   bool use_graph = true;
   bool mg_simple = false;  // one launch per reference dispatch pair (debug comparisons)
   int mg_bottom = 0;       // first level solved by the single-workgroup bottom kernel
@@ -385,7 +386,7 @@ static void drop_graph(cf_sim* s) {
   if (s->gexec8) hipGraphExecDestroy(s->gexec8);
   s->gexec = s->gexec8 = nullptr;
   s->seg.drop();
-  for (int q = 0; q < 11; q++) s->gkey[q] = -1;
+  for (int q = 0; q < 12; q++) s->gkey[q] = -1;
 }
 
 // captures what fn enqueues on stream st
@@ -443,8 +444,8 @@ static int run_segments(cf_sim* s) {
 
 static int ensure_graph(cf_sim* s) {
   // everything a captured step bakes in as launch arguments
-  int key[11] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev, 0, 0, 0,
-                 s->sch.refine, s->sch.krylov};
+  int key[12] = {s->diffSub, s->sch.nu, s->sch.vcycles, s->sch.coarse_iters, s->sch.smoother_fp16, s->nlev, 0, 0, 0,
+                 s->sch.refine, s->sch.krylov, (int)s->tile_gen};
   memcpy(&key[6], &s->sch.tol, 4);
   memcpy(&key[7], &s->sch.cfl, 4);
   memcpy(&key[8], &s->sch.umin, 4);
@@ -809,6 +810,7 @@ static int pool_ready(cf_pool* pl) {
   for (auto* s : pl->slots) {
     key = (key ^ (uint64_t)(uintptr_t)s) * 1099511628211ull;
     if (s) key = (key ^ (uint64_t)s->diffSub) * 1099511628211ull;
+    if (s) key = (key ^ (uint64_t)s->tile_gen) * 1099511628211ull;
   }
   if (key != pl->key) { pl->dirty = true; pl->key = key; }
   if (pl->dirty) {
@@ -895,7 +897,7 @@ static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
   add((void**)&F.V, nV * 4); add((void**)&F.Vh, nV * 4); add((void**)&F.V2, nV * 4);
   add((void**)&F.T, nC * 4); add((void**)&F.Th, nC * 4); add((void**)&F.T2, nC * 4);
   add((void**)&F.Y, nC * 16); add((void**)&F.Yh, nC * 16); add((void**)&F.Y2, nC * 16);
-  add((void**)&F.SQ, nC * 8); add((void**)&F.cellG, nC * 8);
+  add((void**)&F.SQ, nC * 8); add((void**)&F.cellG, nC * 8); add((void**)&F.G, nC * 8);
   add((void**)&F.fx, nU * 4); add((void**)&F.fy, nV * 4);
   add((void**)&F.pieces, CF_MAX_PIECES * CF_PIECE_FLOATS * 4);
   add((void**)&F.acc, CF_ACC_LEN * 4);
@@ -1131,6 +1133,27 @@ int cf_sim_set_schedule(cf_sim* s, const cf_schedule* sc) {
   return sched_apply(s, sc);
 }
 
+int cf_sim_set_leg_tiles(cf_sim* s, const int32_t* tiles, int n) {
+  if (!s || (n > 0 && !tiles) || n < 0) return set_err(CF_EINVAL, "bad argument");
+  if (n > s->mg_bottom) return set_err(CF_EINVAL, "%d tile entries for %d tiled levels", n, s->mg_bottom);
+  for (int l = 0; l < n; l++)
+    if (tiles[l] != 0 && tiles[l] != 16 && tiles[l] != 32 && tiles[l] != 64)
+      return set_err(CF_EINVAL, "leg tile %d of level %d is not 0, 16, 32 or 64", tiles[l], l);
+  bool changed = false;
+  for (int l = 0; l < s->mg_bottom; l++) {
+    const int t = l < n ? tiles[l] : 0;
+    if (s->L[l].tile != t) { s->L[l].tile = t; changed = true; }
+  }
+  if (changed) s->tile_gen++;
+  return CF_OK;
+}
+
+int cf_sim_get_leg_tiles(const cf_sim* s, int32_t* tiles, int n) {
+  if (!s || (n > 0 && !tiles)) return set_err(CF_EINVAL, "bad argument");
+  for (int l = 0; l < n; l++) tiles[l] = l < s->mg_bottom ? cf::mg_tile(s->L[l]) : 0;
+  return s->mg_bottom;
+}
+
 int cf_sim_get_schedule(const cf_sim* s, cf_schedule* sc) {
   if (!s || !sc) return set_err(CF_EINVAL, "NULL argument");
   *sc = s->sch;
@@ -1272,6 +1295,32 @@ int cf_sim_profile_kernels(cf_sim* s, int nsteps, int max, char* names, float* m
     n++;
   }
   *count = n;
+  return CF_OK;
+}
+
+int cf_sim_profile_frame(cf_sim* s, int reps, float* geometry_ms, float* radiation_ms) {
+  if (!s || !geometry_ms || !radiation_ms || reps < 1) return set_err(CF_EINVAL, "bad argument");
+  hipEvent_t ev[3];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  const int np = s->P.nPieces < CF_MAX_PIECES ? s->P.nPieces : CF_MAX_PIECES;
+  float g = 0, r = 0;
+  for (int q = 0; q < reps; q++) {
+    HIPCHK(hipEventRecord(ev[0], s->st));
+    cf::launch_fill_u32(s->F.faceW, 0, CF_MAX_PIECES * 4, s->st);
+    cf::launch_geometry(s->F, s->st);
+    HIPCHK(hipEventRecord(ev[1], s->st));
+    cf::launch_radiation(s->F, np * 4 + CF_N_FLOOR, s->st);
+    HIPCHK(hipEventRecord(ev[2], s->st));
+    HIPCHK(hipEventSynchronize(ev[2]));
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+    g += a;
+    r += b;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *geometry_ms = g / reps;
+  *radiation_ms = r / reps;
   return CF_OK;
 }
 
@@ -1485,6 +1534,45 @@ int cf_flame1d_run(const cf_flame_case* cases, int ncases, int fp64, cf_flame_re
   return rc;
 }
 
+int cf_copy_bandwidth(uint64_t bytes, int reps, double* gbps, double* ms) {
+  if (!gbps || reps < 1 || bytes < 4096 || bytes % 16) return set_err(CF_EINVAL, "bad argument");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = CF_OK;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float best = 1e30f, total = 0.f;
+  if (e == hipSuccess) {
+    cf::launch_fill_u32(a, 0x3f800000u, bytes / 4, st);
+    cf::launch_bw_copy(a, b, bytes, st);  // warm-up
+    for (int q = 0; q < reps && e == hipSuccess; q++) {
+      e = hipEventRecord(e0, st);
+      cf::launch_bw_copy(a, b, bytes, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      best = t < best ? t : best;
+      total += t;
+    }
+  }
+  if (e != hipSuccess) rc = set_err(CF_EDEVICE, "copy bandwidth: %s", hipGetErrorString(e));
+  else {
+    *gbps = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
+    if (ms) *ms = total / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
 uint64_t cf_scene_bytes(int nx, int ny, double dx) {
   cf_sim* s = new cf_sim();
   uint64_t total = 0;
@@ -1643,12 +1731,13 @@ int cf_pool_frame(cf_pool* pl, int nsteps, int with_radiation) {
     pool_release_phases(pl->frame_pre);
     pool_release_phases(pl->frame_post);
     pl->frame_pre.resize(2);
-    pl->frame_post.resize(1);
+    pl->frame_post.resize(2);
     for (auto* s : sc) {
       cf::phase_fill_u32(pl->frame_pre[0], s->F.faceW, 0u, CF_MAX_PIECES * 4);
       cf::phase_geometry(pl->frame_pre[1], s->F);
       const int np = s->P.nPieces < CF_MAX_PIECES ? s->P.nPieces : CF_MAX_PIECES;
-      cf::phase_radiation(pl->frame_post[0], s->F, np * 4 + CF_N_FLOOR);
+      cf::phase_gas_terms(pl->frame_post[0], s->F);
+      cf::phase_radiation(pl->frame_post[1], s->F, np * 4 + CF_N_FLOOR);
     }
     std::vector<cf::BatchPhase*> all;
     for (auto& a : pl->frame_pre) all.push_back(&a);

@@ -150,6 +150,18 @@ class FluidSim:
         check(L.lib().cf_sim_get_schedule(self._h, C.byref(s)), "get_schedule")
         return {n: getattr(s, n) for n, _ in L.CfSchedule._fields_}
 
+    def set_leg_tiles(self, tiles):
+        """Tile edge (16, 32, 64; 0 = default rule) of the tiled multigrid legs
+        of each level from level 0; bitwise the same results for any choice."""
+        a = np.ascontiguousarray(tiles, dtype=np.int32)
+        check(L.lib().cf_sim_set_leg_tiles(self._h, a.ctypes.data, int(a.size)), "set_leg_tiles")
+
+    def leg_tiles(self):
+        """The tile edges in use, one per tiled level."""
+        a = np.zeros(16, np.int32)
+        n = check(L.lib().cf_sim_get_leg_tiles(self._h, a.ctypes.data, 16), "leg_tiles")
+        return [int(x) for x in a[:n]]
+
     def upload_pieces(self, data, n):  # js/fluid.js:201-204
         a = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
         if a.size != MAX_PIECES * PIECE_FLOATS:
@@ -203,6 +215,12 @@ class FluidSim:
 
     def synchronize(self):
         check(L.lib().cf_sim_synchronize(self._h), "synchronize")
+
+    def profile_frame(self, reps=10):
+        """Mean device milliseconds of the per-frame geometry and radiation."""
+        g, r = C.c_float(0), C.c_float(0)
+        check(L.lib().cf_sim_profile_frame(self._h, int(reps), C.byref(g), C.byref(r)), "profile_frame")
+        return float(g.value), float(r.value)
 
     def status(self):
         """Waits for the enqueued steps; raises CampfireError with CF_ENONFINITE

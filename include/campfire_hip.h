@@ -147,6 +147,17 @@ const char* cf_param_name(int index);
 int  cf_sim_set_schedule(cf_sim* sim, const cf_schedule* s);
 int  cf_sim_get_schedule(const cf_sim* sim, cf_schedule* s);
 
+/* Tile edge of the tiled multigrid legs per level (levels 0 .. n-1 of those
+ * above the single-workgroup bottom solver): 16, 32 or 64, or 0 for the
+ * default size rule.  Any choice gives bitwise the same results (a tile with
+ * a halo of 2 nu + 1 cells reproduces the global half-sweeps exactly); it
+ * changes only the launch shape, which the autotuner picks by timing.  No
+ * counterpart in the reference (its relax dispatches are 8x8 workgroups). */
+int  cf_sim_set_leg_tiles(cf_sim* sim, const int32_t* tiles, int n);
+/* The tile edges in use for levels 0 .. n-1 (0 past the tiled levels);
+ * returns the number of tiled levels. */
+int  cf_sim_get_leg_tiles(const cf_sim* sim, int32_t* tiles, int n);
+
 /* uploadPieces(Float32Array(64*20), n)  (js/fluid.js:201-204); also sets
  * nPieces.  Layout per piece (shaders/common.wgsl:17-23):
  * [x, y, angle, half, vx, vy, omega, alive, Ts[4], mflux[4], charCov[4]]. */
@@ -185,6 +196,12 @@ int  cf_sim_profile_steps(cf_sim* sim, int nsteps, float* stage_ms);
  * milliseconds and launch counts; *count receives the number filled. */
 int  cf_sim_profile_kernels(cf_sim* sim, int nsteps, int max, char* names,
                             float* ms, int* launches, int* count);
+
+/* Device time of the per-frame work (the measurement counterpart of
+ * clearGeometry + encodeGeometry and of encodeRadiation, js/fluid.js:213-222):
+ * `reps` frames' geometry and radiation (which also advances the stateful
+ * smoothing of rad), the mean milliseconds of each. */
+int  cf_sim_profile_frame(cf_sim* sim, int reps, float* geometry_ms, float* radiation_ms);
 
 /* Per-step latency: nsteps steps, each bracketed by HIP events on the sim's
  * stream; ms[q] receives the device time of step q. */
@@ -329,6 +346,12 @@ int  cf_sim_compat_sums(cf_sim* sim, double* source, double* outflow);
 /* Copies every device buffer and parameter of src into dst (same grid): a
  * snapshot, or a restore of one.  No counterpart in the reference. */
 int  cf_sim_copy_state(cf_sim* dst, const cf_sim* src);
+
+/* The bandwidth reference of the measurements: a plain streaming copy kernel
+ * of this library moving `bytes` (read + write: 2 x bytes per launch), best of
+ * `reps` launches, as GB/s (*gbps) and the mean launch time (*ms, may be NULL).
+ * Allocates its two buffers per call.  No counterpart in the reference. */
+int  cf_copy_bandwidth(uint64_t bytes, int reps, double* gbps, double* ms);
 
 /* Thread-local text of the last error. */
 const char* cf_last_error(void);
