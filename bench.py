@@ -526,7 +526,7 @@ def bench_config(args):
     return cfg
 
 
-def cpu_baseline(cfg, budget_s=12.0, max_steps=30):
+def cpu_baseline(cfg, budget_s=12.0, max_steps=200):
     """The FP32 oracle (OpenMP) on a bounded sample of the workload `cfg`."""
     from campfire_amd import scenes
     from oracle.pyoracle import Oracle
@@ -727,7 +727,7 @@ def main():
         for n in (256, 512):
             a2 = argparse.Namespace(**vars(args))
             a2.size = n
-            sizes["%dx%d" % (n, n)] = cpu_baseline(bench_config(a2), budget_s=5.0, max_steps=100)
+            sizes["%dx%d" % (n, n)] = cpu_baseline(bench_config(a2), budget_s=6.0, max_steps=2000)
         out["cpu_baseline_sizes"] = sizes
     print(json.dumps(out), flush=True)
 
