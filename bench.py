@@ -72,6 +72,13 @@ def kernel_bytes(tag, nx, ny, levels, launches):
         return 22 * fine[lev]
     if base == "mg_up" and lev is not None:     # p, coarse p, rhs, ax, ay read; p written
         return 21 * fine[lev]
+    if base == "mg_down2" and lev is not None:
+        # levels lev, lev+1 in one launch: fine p, rhs, ax, ay read, q written;
+        # coarse ax, ay read, q, rhs written; next level's rhs, p written
+        return 20 * fine[lev] + 16 * fine[lev + 1] + 8 * fine[lev + 2]
+    if base == "mg_up2" and lev is not None:
+        # fine and coarse q, rhs, ax, ay read, p written; next level's p read
+        return 20 * fine[lev] + 20 * fine[lev + 1] + 4 * fine[lev + 2]
     if base == "coef_restrict" and lev is not None:   # fine ax, ay read; coarse ax, ay written
         return 8 * fine[lev - 1] + 8 * fine[lev]
     if tag == "restrict_res":

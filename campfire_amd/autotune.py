@@ -126,6 +126,8 @@ def tune_leg_tiles(nx, ny, dx, dt, steps=20, warm=16, seed=1, log=None):
     log("[autotune] %dx%d leg tiles %s (size rule): %.4f ms/step" % (nx, ny, tiles, table[0][1]))
     best_ms = table[0][1]
     for l in range(len(tiles)):
+        if tiles[l] == 0:  # the level's legs run in a two-level pair, whose tile is fixed
+            continue
         for t in TILES:
             if t == tiles[l]:
                 continue

@@ -267,6 +267,12 @@ void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* pa
 // once *skip is non-zero (the residual target of the step has been met)
 bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
+// the legs of two levels f, c = f + 1 in one launch (FP32 storage), n = f + 2;
+// tile: the edge of a workgroup's tile in cells of c (16 or 8; 0: mg_pair_tile)
+bool launch_mg_down2(const Level& f, const Level& c, const Level& n, int nu, int tile, const int* skip,
+                     hipStream_t s);
+bool launch_mg_up2(const Level& f, const Level& c, const Level& n, int nu, int tile, const int* skip, hipStream_t s);
+int mg_pair_tile(const Level& c);
 // FP16 copies of a level's face coefficients (once per step, FP16 smoother)
 void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s);
 // tile edge (64, 32 or 16) of a level's legs: Level::tile, or by default a

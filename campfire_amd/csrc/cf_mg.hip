@@ -35,19 +35,18 @@ constexpr int LEG_UPD = 1 << 16, LEG_IN = 1 << 17, LEG_IDX = 0xffff;
 __device__ __forceinline__ float ldc(const float* a, int k) { return a[k]; }
 __device__ __forceinline__ float ldc(const __half* a, int k) { return __half2float(a[k]); }
 
-// cell (x, y) at region position (rx, ry); CT: float, or __half for the FP16
-// smoother's coefficient copies.  rhs_out: its rhs (0 outside the domain).
+// cell (x, y) at region position (rx, ry) without its rhs (rh = 0); CT:
+// float, or __half for the FP16 smoother's coefficient copies
 template <class CT>
-__device__ __forceinline__ LegCell leg_cell(int nx, int ny, const CT* __restrict__ ax, const CT* __restrict__ ay,
-                                            const float* __restrict__ rhs, float h2, int x, int y, int rx, int ry,
-                                            int RX, int RY, int PX, float& rhs_out) {
+__device__ __forceinline__ LegCell leg_cell_coef(int nx, int ny, const CT* __restrict__ ax,
+                                                 const CT* __restrict__ ay, int x, int y, int rx, int ry, int RX,
+                                                 int RY, int PX) {
   LegCell c;
   const bool in = x >= 0 && x < nx && y >= 0 && y < ny;
   const bool upd = in && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
                    !(ry == RY - 1 && y < ny - 1);
   c.ci = ((ry + 1) * PX + rx + 1) | (upd ? LEG_UPD : 0) | (in ? LEG_IN : 0);
   c.w = c.e = c.s = c.n = c.d = c.y = c.rh = 0.f;
-  rhs_out = 0.f;
   if (in) {
     const float w = ldc(ax, y * (nx + 1) + x), e = ldc(ax, y * (nx + 1) + x + 1);
     const float s = ldc(ay, y * nx + x), n = ldc(ay, (y + 1) * nx + x);
@@ -62,6 +61,18 @@ __device__ __forceinline__ LegCell leg_cell(int nx, int ny, const CT* __restrict
     c.e = x < nx - 1 ? e : 0.f;
     c.s = y > 0 ? s : 0.f;
     c.n = y < ny - 1 ? n : 0.f;
+  }
+  return c;
+}
+
+// the same with its rhs; rhs_out: the rhs (0 outside the domain)
+template <class CT>
+__device__ __forceinline__ LegCell leg_cell(int nx, int ny, const CT* __restrict__ ax, const CT* __restrict__ ay,
+                                            const float* __restrict__ rhs, float h2, int x, int y, int rx, int ry,
+                                            int RX, int RY, int PX, float& rhs_out) {
+  LegCell c = leg_cell_coef(nx, ny, ax, ay, x, y, rx, ry, RX, RY, PX);
+  rhs_out = 0.f;
+  if (c.ci & LEG_IN) {
     rhs_out = rhs[y * nx + x];
     c.rh = rhs_out * h2;
   }
@@ -300,6 +311,372 @@ struct MgUpOp {
 template <int TX, int TY, int NU, int M>
 __global__ __launch_bounds__((MgUpOp<TX, TY, NU, M>::kThreads)) void k_mg_up(const typename MgUpOp<TX, TY, NU, M>::Args a_) {
   MgUpOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// ---- two levels per launch -------------------------------------------------------
+// The legs of a level f and of the next coarser one c in one workgroup, for a
+// tile of TC x TC cells of c (2TC x 2TC of f): c's rhs, its start from p = 0
+// and its sweeps never leave LDS, and a V-cycle has one launch less per pair
+// in each direction.  The halos compose level by level:
+//   down: c's leg needs the tile plus 2NU+1 coarse cells (the lone leg's halo),
+//         whose rhs is the restriction of the fine residual under them.  The
+//         fine leg's region is those fine cells plus 2NU-1: the residual of
+//         their two outer fine rings is wrong, but it reaches only the coarse
+//         region's outer ring, whose cells the coarse leg never updates (that
+//         ring never lies on the domain edge: the tile origin is even, the
+//         halo odd and the level's size even).
+//   up:   f's leg needs its tile plus 2NU fine cells, whose prolongation reads
+//         coarse cells up to NU+1 beyond the coarse tile, so c's region is the
+//         tile plus 3NU+1 coarse cells.
+// Every update, residual, restriction and prolongation is the lone legs'
+// arithmetic in their order, so the results are theirs to the last bit.  FP32
+// storage only.
+
+// the ring of zeros around an RX x RY region stored with row pitch PX = RX + 2
+__device__ __forceinline__ void leg_ring(float* sp, int PX, int PY, int RY, int nt) {
+  for (int k = threadIdx.x; k < 2 * PX + 2 * RY; k += nt) {
+    const int idx = k < PX ? k : (k < 2 * PX ? (PY - 1) * PX + (k - PX) : ((k - 2 * PX) / 2 + 1) * PX +
+                                                                             ((k & 1) ? PX - 1 : 0));
+    sp[idx] = 0.f;
+  }
+}
+
+// Levels f, c = f + 1, n = f + 2: the down legs of f and c and the
+// restriction into n.  Writes what the lone legs leave for the up legs: f.q,
+// c.q and c.rhs over the tile, n.rhs and n.p = 0.  c.p is not written: no leg
+// reads it before the up leg of c overwrites it.
+template <int TC, int NU, int NT>
+struct MgDown2Op {
+  struct Args {
+    Level f, c, n;
+    const int* skip;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = NT;
+  static constexpr int HC = 2 * NU + 1, HF = 2 * NU - 1;
+  static constexpr int RC = TC + 2 * HC, RR = 2 * RC, RF = RR + 2 * HF;
+  static constexpr int PC = RC + 2, PF = RF + 2;
+  static constexpr int NPF = RF * RF / 2, NPC = RC * RC / 2;
+  static constexpr int QF = (NPF + NT - 1) / NT, QC = (NPC + NT - 1) / NT;
+  static_assert(RC % 2 == 0 && RF % 2 == 0 && TC % 2 == 0, "pairs need even region widths");
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& f = a_.f;
+    const Level& c = a_.c;
+    const Level& n = a_.n;
+    if (a_.skip && *a_.skip) return;  // the step's residual target is met
+    __shared__ float spf[PF * PF];
+    __shared__ float sres[RR * RR];   // the fine rhs under the coarse region, then its residual
+    __shared__ float spc[PC * PC];
+    __shared__ float src[RC * RC];    // the coarse rhs over the coarse region
+    __shared__ float sresc[TC * TC];  // the coarse residual over the tile
+    const int tid = threadIdx.x;
+    const int nx = f.nx, nxc = c.nx, nyc = c.ny;
+    const int gx = (nxc + TC - 1) / TC;
+    const int X0 = (vb % gx) * TC, Y0 = (vb / gx) * TC;  // the coarse tile
+    const int xc0 = X0 - HC, yc0 = Y0 - HC;              // the coarse region
+    const int xf0 = 2 * xc0 - HF, yf0 = 2 * yc0 - HF;    // the fine region
+    leg_ring(spf, PF, PF, RF, NT);
+    leg_ring(spc, PC, PC, RC, NT);
+    // every global load before the first sweep: the fine cells, rhs and p,
+    // the coarse cells' coefficients
+    LegCell fR[QF], fB[QF];
+#pragma unroll
+    for (int q = 0; q < QF; q++) {
+      const int pi = tid + NT * q;
+      fR[q] = fB[q] = leg_none(PF);
+      if (pi < NPF) {
+        const int ry = pi / (RF / 2), rx = 2 * (pi % (RF / 2));
+        const int x = xf0 + rx, y = yf0 + ry;
+        float r0, r1;
+        const LegCell c0 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx, ry, RF, RF, PF, r0);
+        const LegCell c1 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x + 1, y, rx + 1, ry, RF, RF, PF, r1);
+        spf[c0.ci & LEG_IDX] = (c0.ci & LEG_IN) ? f.p[y * nx + x] : 0.f;
+        spf[c1.ci & LEG_IDX] = (c1.ci & LEG_IN) ? f.p[y * nx + x + 1] : 0.f;
+        if (ry >= HF && ry < HF + RR) {
+          if (rx >= HF && rx < HF + RR) sres[(ry - HF) * RR + (rx - HF)] = r0;
+          if (rx + 1 >= HF && rx + 1 < HF + RR) sres[(ry - HF) * RR + (rx + 1 - HF)] = r1;
+        }
+        const bool red0 = ((x + y) & 1) == 0;
+        fR[q] = red0 ? c0 : c1;
+        fB[q] = red0 ? c1 : c0;
+      }
+    }
+    LegCell cR[QC], cB[QC];
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      const int pi = tid + NT * q;
+      cR[q] = cB[q] = leg_none(PC);
+      if (pi < NPC) {
+        const int ry = pi / (RC / 2), rx = 2 * (pi % (RC / 2));
+        const int X = xc0 + rx, Y = yc0 + ry;
+        const LegCell c0 = leg_cell_coef(nxc, nyc, c.ax, c.ay, X, Y, rx, ry, RC, RC, PC);
+        const LegCell c1 = leg_cell_coef(nxc, nyc, c.ax, c.ay, X + 1, Y, rx + 1, ry, RC, RC, PC);
+        spc[c0.ci & LEG_IDX] = 0.f;  // the coarse leg starts from p = 0 (shaders/pressure.wgsl:104-113)
+        spc[c1.ci & LEG_IDX] = 0.f;
+        const bool red0 = ((X + Y) & 1) == 0;
+        cR[q] = red0 ? c0 : c1;
+        cB[q] = red0 ? c1 : c0;
+      }
+    }
+    __syncthreads();
+    for (int s = 0; s < NU; s++) {  // red then black (js/fluid.js:298-301)
+#pragma unroll
+      for (int q = 0; q < QF; q++) leg_relax<PF>(fR[q], spf);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < QF; q++) leg_relax<PF>(fB[q], spf);
+      __syncthreads();
+    }
+    // the fine residual under the coarse region (:96-101); f's leg buffer over the tile
+    auto finish_f = [&](const LegCell& a) {
+      if (!(a.ci & LEG_IN)) return;
+      const int ci = a.ci & LEG_IDX;
+      const int rx = ci % PF - 1, ry = ci / PF - 1;
+      const float pk = spf[ci];
+      if (rx >= HF && rx < HF + RR && ry >= HF && ry < HF + RR) {
+        const float off = a.w * spf[ci - 1] + a.e * spf[ci + 1] + a.s * spf[ci - PF] + a.n * spf[ci + PF];
+        const int r = (ry - HF) * RR + (rx - HF);
+        sres[r] = a.d <= K(0) ? K(0) : sres[r] - (off - a.d * pk) / f.h2;
+      }
+      constexpr int T0 = HF + 2 * HC;  // the fine tile's first region row and column
+      if (rx >= T0 && rx < T0 + 2 * TC && ry >= T0 && ry < T0 + 2 * TC) f.q[(yf0 + ry) * nx + (xf0 + rx)] = pk;
+    };
+#pragma unroll
+    for (int q = 0; q < QF; q++) {
+      finish_f(fR[q]);
+      finish_f(fB[q]);
+    }
+    __syncthreads();
+    // restriction to the coarse rhs over the coarse region (:104-113); the tile's goes out for c's up leg
+    for (int k = tid; k < RC * RC; k += NT) {
+      const int li = k % RC, lj = k / RC;
+      const int X = xc0 + li, Y = yc0 + lj;
+      float v = 0.f;
+      if (X >= 0 && X < nxc && Y >= 0 && Y < nyc) {
+        const float* r0 = sres + (2 * lj) * RR + 2 * li;
+        const float r = r0[0] + r0[1] + r0[RR] + r0[RR + 1];
+        v = K(0.25) * r;
+        if (li >= HC && li < HC + TC && lj >= HC && lj < HC + TC) c.rhs[Y * nxc + X] = v;
+      }
+      src[k] = v;
+    }
+    __syncthreads();
+    auto set_rh = [&](LegCell& a) {  // rhs * h2, as leg_cell forms it
+      if (a.ci & LEG_IN) {
+        const int ci = a.ci & LEG_IDX;
+        a.rh = src[(ci / PC - 1) * RC + (ci % PC - 1)] * c.h2;
+      }
+    };
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      set_rh(cR[q]);
+      set_rh(cB[q]);
+    }
+    for (int s = 0; s < NU; s++) {
+#pragma unroll
+      for (int q = 0; q < QC; q++) leg_relax<PC>(cR[q], spc);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < QC; q++) leg_relax<PC>(cB[q], spc);
+      __syncthreads();
+    }
+    // the coarse residual over the tile; c's leg buffer
+    auto finish_c = [&](const LegCell& a) {
+      const int ci = a.ci & LEG_IDX;
+      const int rx = ci % PC - 1, ry = ci / PC - 1;
+      if ((a.ci & LEG_IN) && rx >= HC && rx < HC + TC && ry >= HC && ry < HC + TC) {
+        const float pk = spc[ci];
+        const float off = a.w * spc[ci - 1] + a.e * spc[ci + 1] + a.s * spc[ci - PC] + a.n * spc[ci + PC];
+        const float rh = src[ry * RC + rx];
+        sresc[(ry - HC) * TC + (rx - HC)] = a.d <= K(0) ? K(0) : rh - (off - a.d * pk) / c.h2;
+        c.q[(yc0 + ry) * nxc + (xc0 + rx)] = pk;
+      }
+    };
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      finish_c(cR[q]);
+      finish_c(cB[q]);
+    }
+    __syncthreads();
+    // restriction to n's rhs, n's p = 0
+    for (int k = tid; k < (TC / 2) * (TC / 2); k += NT) {
+      const int li = k % (TC / 2), lj = k / (TC / 2);
+      const int I = X0 / 2 + li, J = Y0 / 2 + lj;
+      if (I < n.nx && J < n.ny) {
+        const float* r0 = sresc + (2 * lj) * TC + 2 * li;
+        const float r = r0[0] + r0[1] + r0[TC] + r0[TC + 1];
+        n.rhs[J * n.nx + I] = K(0.25) * r;
+        n.p[J * n.nx + I] = K(0);
+      }
+    }
+  }
+};
+template <int TC, int NU, int NT>
+__global__ __launch_bounds__(NT) void k_mg_down2(const typename MgDown2Op<TC, NU, NT>::Args a_) {
+  MgDown2Op<TC, NU, NT>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Levels f, c = f + 1, n = f + 2: the up legs of c and f, after n's.  Reads
+// f.q, c.q, the rhs of both and n.p; writes c.p over the tile and f.p over the
+// fine tile.
+template <int TC, int NU, int NT>
+struct MgUp2Op {
+  struct Args {
+    Level f, c, n;
+    const int* skip;
+    int vg;  // tiles of this scene's launch
+  };
+  static constexpr int kThreads = NT;
+  static constexpr int HF = 2 * NU, HC = 3 * NU + 1;
+  static constexpr int RF = 2 * TC + 2 * HF, RC = TC + 2 * HC;
+  static constexpr int PF = RF + 2, PC = RC + 2;
+  static constexpr int NPF = RF * RF / 2, NPC = RC * RC / 2;
+  static constexpr int QF = (NPF + NT - 1) / NT, QC = (NPC + NT - 1) / NT;
+  static_assert(RC % 2 == 0 && RF % 2 == 0, "pairs need even region widths");
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    const Level& f = a_.f;
+    const Level& c = a_.c;
+    const Level& n = a_.n;
+    if (a_.skip && *a_.skip) return;
+    __shared__ float spf[PF * PF];
+    __shared__ float spc[PC * PC];
+    const int tid = threadIdx.x;
+    const int nx = f.nx, nxc = c.nx, nyc = c.ny, nxn = n.nx, nyn = n.ny;
+    const int gx = (nxc + TC - 1) / TC;
+    const int X0 = (vb % gx) * TC, Y0 = (vb / gx) * TC;  // the coarse tile
+    const int xc0 = X0 - HC, yc0 = Y0 - HC;              // the coarse region
+    const int xf0 = 2 * X0 - HF, yf0 = 2 * Y0 - HF;      // the fine region
+    leg_ring(spf, PF, PF, RF, NT);
+    leg_ring(spc, PC, PC, RC, NT);
+    // the coarse cells: coefficients, rhs, start value q + the bilinear
+    // correction from n, skipped in closed cells (shaders/pressure.wgsl:116-129)
+    LegCell cR[QC], cB[QC];
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      const int pi = tid + NT * q;
+      cR[q] = cB[q] = leg_none(PC);
+      if (pi < NPC) {
+        const int ry = pi / (RC / 2), rx = 2 * (pi % (RC / 2));
+        const int Y = yc0 + ry;
+        LegCell cc[2];
+#pragma unroll
+        for (int sub = 0; sub < 2; sub++) {
+          const int X = xc0 + rx + sub;
+          float rr;
+          cc[sub] = leg_cell(nxc, nyc, c.ax, c.ay, c.rhs, c.h2, X, Y, rx + sub, ry, RC, RC, PC, rr);
+          float v = 0.f;
+          if (cc[sub].ci & LEG_IN) {
+            const int I = X / 2, J = Y / 2;
+            const int I2 = iclamp(I + ((X & 1) == 1 ? 1 : -1), 0, nxn - 1);
+            const int J2 = iclamp(J + ((Y & 1) == 1 ? 1 : -1), 0, nyn - 1);
+            const float c00 = n.p[J * nxn + I], c10 = n.p[J * nxn + I2];
+            const float c01 = n.p[J2 * nxn + I], c11 = n.p[J2 * nxn + I2];
+            const float ccr = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
+            v = c.q[Y * nxc + X] + (cc[sub].d > K(0) ? ccr : K(0));
+          }
+          spc[cc[sub].ci & LEG_IDX] = v;
+        }
+        const bool red0 = ((xc0 + rx + Y) & 1) == 0;
+        cR[q] = red0 ? cc[0] : cc[1];
+        cB[q] = red0 ? cc[1] : cc[0];
+      }
+    }
+    // the fine cells: coefficients, rhs and q (their correction waits for c's sweeps)
+    LegCell fR[QF], fB[QF];
+    float qR[QF], qB[QF];
+#pragma unroll
+    for (int q = 0; q < QF; q++) {
+      const int pi = tid + NT * q;
+      fR[q] = fB[q] = leg_none(PF);
+      qR[q] = qB[q] = 0.f;
+      if (pi < NPF) {
+        const int ry = pi / (RF / 2), rx = 2 * (pi % (RF / 2));
+        const int y = yf0 + ry;
+        LegCell cc[2];
+        float qv[2];
+#pragma unroll
+        for (int sub = 0; sub < 2; sub++) {
+          const int x = xf0 + rx + sub;
+          float rr;
+          cc[sub] = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx + sub, ry, RF, RF, PF, rr);
+          qv[sub] = (cc[sub].ci & LEG_IN) ? f.q[y * nx + x] : 0.f;
+        }
+        const bool red0 = ((xf0 + rx + y) & 1) == 0;
+        fR[q] = red0 ? cc[0] : cc[1];
+        fB[q] = red0 ? cc[1] : cc[0];
+        qR[q] = red0 ? qv[0] : qv[1];
+        qB[q] = red0 ? qv[1] : qv[0];
+      }
+    }
+    __syncthreads();
+    for (int s = 0; s < NU; s++) {  // black then red (js/fluid.js:307-310)
+#pragma unroll
+      for (int q = 0; q < QC; q++) leg_relax<PC>(cB[q], spc);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < QC; q++) leg_relax<PC>(cR[q], spc);
+      __syncthreads();
+    }
+    // c.p over the tile
+    auto out_c = [&](const LegCell& a) {
+      const int ci = a.ci & LEG_IDX;
+      const int rx = ci % PC - 1, ry = ci / PC - 1;
+      if ((a.ci & LEG_IN) && rx >= HC && rx < HC + TC && ry >= HC && ry < HC + TC)
+        c.p[(yc0 + ry) * nxc + (xc0 + rx)] = spc[ci];
+    };
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      out_c(cR[q]);
+      out_c(cB[q]);
+    }
+    // the fine start values: q + the bilinear correction from c's region
+    auto start = [&](const LegCell& a, float qv) {
+      const int ci = a.ci & LEG_IDX;
+      float v = 0.f;
+      if (a.ci & LEG_IN) {
+        const int x = xf0 + (ci % PF - 1), y = yf0 + (ci / PF - 1);
+        const int I = x / 2, J = y / 2;
+        const int I2 = iclamp(I + ((x & 1) == 1 ? 1 : -1), 0, nxc - 1);
+        const int J2 = iclamp(J + ((y & 1) == 1 ? 1 : -1), 0, nyc - 1);
+        const int r0 = (J - yc0 + 1) * PC - xc0 + 1, r2 = (J2 - yc0 + 1) * PC - xc0 + 1;
+        const float c00 = spc[r0 + I], c10 = spc[r0 + I2], c01 = spc[r2 + I], c11 = spc[r2 + I2];
+        const float ccr = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
+        v = qv + (a.d > K(0) ? ccr : K(0));
+      }
+      spf[ci] = v;
+    };
+#pragma unroll
+    for (int q = 0; q < QF; q++) {
+      if (tid + NT * q < NPF) {  // a slot without a cell must not write the first region cell
+        start(fR[q], qR[q]);
+        start(fB[q], qB[q]);
+      }
+    }
+    __syncthreads();
+    for (int s = 0; s < NU; s++) {
+#pragma unroll
+      for (int q = 0; q < QF; q++) leg_relax<PF>(fB[q], spf);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < QF; q++) leg_relax<PF>(fR[q], spf);
+      __syncthreads();
+    }
+    auto out_f = [&](const LegCell& a) {
+      const int ci = a.ci & LEG_IDX;
+      const int rx = ci % PF - 1, ry = ci / PF - 1;
+      if ((a.ci & LEG_IN) && rx >= HF && rx < HF + 2 * TC && ry >= HF && ry < HF + 2 * TC)
+        f.p[(yf0 + ry) * nx + (xf0 + rx)] = spf[ci];
+    };
+#pragma unroll
+    for (int q = 0; q < QF; q++) {
+      out_f(fR[q]);
+      out_f(fB[q]);
+    }
+  }
+};
+template <int TC, int NU, int NT>
+__global__ __launch_bounds__(NT) void k_mg_up2(const typename MgUp2Op<TC, NU, NT>::Args a_) {
+  MgUp2Op<TC, NU, NT>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---- bottom levels in one workgroup ------------------------------------------------
@@ -841,6 +1218,44 @@ bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, con
   return mg_dispatch<MgUpOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
     launch(k_mg_up_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, st, skip, nb});
+  });
+}
+
+// two-level legs: 16 x 16 coarse cells per workgroup where that still gives
+// the launch 128 workgroups, else 8 x 8
+int mg_pair_tile(const Level& c) { return ntiles(c.nx, c.ny, 16, 16) >= 128 ? 16 : 8; }
+
+template <int TC, int NU, int NT>
+static auto k_mg_down2_of(MgDown2Op<TC, NU, NT>) { return k_mg_down2<TC, NU, NT>; }
+template <int TC, int NU, int NT>
+static auto k_mg_up2_of(MgUp2Op<TC, NU, NT>) { return k_mg_up2<TC, NU, NT>; }
+
+// threads per workgroup: about one pair of fine cells per thread and sweep
+// (down: (2TC + 6NU + 4)^2 / 2 cells, up: (2TC + 4NU)^2 / 2)
+template <template <int, int, int> class Op, int NT16, int NT8, class F>
+static bool pair_dispatch(int tc, int nu, F&& f) {
+#define CF_PAIR_CASE(T, NU, NT) \
+  if (tc == T && nu == NU) { f(Op<T, NU, NT>{}); return true; }
+  CF_PAIR_CASE(16, 1, NT16) CF_PAIR_CASE(16, 2, NT16) CF_PAIR_CASE(16, 3, NT16)
+  CF_PAIR_CASE(8, 1, NT8) CF_PAIR_CASE(8, 2, NT8) CF_PAIR_CASE(8, 3, NT8)
+#undef CF_PAIR_CASE
+  return false;
+}
+
+bool launch_mg_down2(const Level& f, const Level& c, const Level& n, int nu, int tile, const int* skip,
+                     hipStream_t s) {
+  const int tc = tile ? tile : mg_pair_tile(c), nb = (int)ntiles(c.nx, c.ny, tc, tc);
+  return pair_dispatch<MgDown2Op, 1024, 512>(tc, nu, [&](auto op) {
+    using Op = decltype(op);
+    launch(k_mg_down2_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, n, skip, nb});
+  });
+}
+
+bool launch_mg_up2(const Level& f, const Level& c, const Level& n, int nu, int tile, const int* skip, hipStream_t s) {
+  const int tc = tile ? tile : mg_pair_tile(c), nb = (int)ntiles(c.nx, c.ny, tc, tc);
+  return pair_dispatch<MgUp2Op, 1024, 256>(tc, nu, [&](auto op) {
+    using Op = decltype(op);
+    launch(k_mg_up2_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, n, skip, nb});
   });
 }
 

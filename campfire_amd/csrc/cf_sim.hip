@@ -136,6 +136,12 @@ struct cf_sim {
   bool use_graph = true;
   bool mg_simple = false;  // one launch per reference dispatch pair (debug comparisons)
   int mg_bottom = 0;       // first level solved by the single-workgroup bottom kernel
+  // two-level legs (FP32 storage, not in pools): the levels from 1 run as
+  // pairs, the bottom kernel from mg_bottom_pairs on; CF_LEG_PAIRS=0 turns
+  // them off, CF_PAIR_TILE=8|16 fixes their tile
+  bool leg_pairs = true;
+  int pair_tile = 0;
+  int mg_bottom_pairs = 0;
   // profiling
   bool profiling = false;
   std::vector<cf::LaunchEvents> plaunch;  // one per launch of a profiling pass
@@ -154,9 +160,9 @@ thread_local std::vector<LaunchEvents>* prof_sink = nullptr;
 
 // names of per-level launch sites ("mg_down.0" ...), static for the tag table
 static const char* level_tag(const char* base, int l) {
-  static char names[4][16][24];
-  static const char* bases[4] = {"mg_down", "mg_up", "coef_restrict", "relax"};
-  for (int b = 0; b < 4; b++)
+  static char names[6][16][24];
+  static const char* bases[6] = {"mg_down", "mg_up", "coef_restrict", "relax", "mg_down2", "mg_up2"};
+  for (int b = 0; b < 6; b++)
     if (!strcmp(base, bases[b]) && l >= 0 && l < 16) {
       if (!names[b][l][0]) snprintf(names[b][l], sizeof(names[b][l]), "%s.%d", base, l);
       return names[b][l];
@@ -191,17 +197,44 @@ static cf::MgStore mg_store(const cf_sim* s, int l) {
   return m;
 }
 
+// whether a lone scene's V-cycle runs the two-level legs: FP32 storage only
+static bool use_pairs(const cf_sim* s) { return s->leg_pairs && !s->mg_simple && !s->sch.smoother_fp16; }
+// the first level of the bottom kernel in a lone scene's V-cycle
+static int vcycle_bottom(const cf_sim* s) { return use_pairs(s) ? s->mg_bottom_pairs : s->mg_bottom; }
+// whether level l's legs run in a pair with level l + 1 (first of the pair)
+// or l - 1 (second): level 0 alone, then pairs while two tiled levels remain
+static bool pair_first(const cf_sim* s, int l) {
+  if (!use_pairs(s) || l < 1) return false;
+  return (l - 1) % 2 == 0 && l + 1 < vcycle_bottom(s);
+}
+static bool paired(const cf_sim* s, int l) { return pair_first(s, l) || (l >= 2 && pair_first(s, l - 1)); }
+
 static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef) {
   const int L = s->nlev;
   if (!s->mg_simple) {
-    // fused: tiled down legs, one LDS-resident workgroup for the bottom levels, tiled up legs
-    const int b = s->mg_bottom;
-    for (int l = 0; l < b; l++) {
-      cf::launch_mg_down(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_down", l));
+    // fused: tiled down legs (one or two levels per launch), one LDS-resident
+    // workgroup for the bottom levels, tiled up legs
+    const int b = vcycle_bottom(s);
+    for (int l = 0; l < b;) {
+      if (pair_first(s, l)) {
+        cf::launch_mg_down2(s->L[l], s->L[l + 1], s->L[l + 2], nu, s->pair_tile, skip, s->st);
+        mark(s, level_tag("mg_down2", l));
+        l += 2;
+      } else {
+        cf::launch_mg_down(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_down", l));
+        l += 1;
+      }
     }
     cf::launch_mg_bottom(s->L, L, b, coarse_coef, nu, s->sch.coarse_iters, skip, s->dBand, s->st); mark(s, "mg_bottom");
-    for (int l = b - 1; l >= 0; l--) {
-      cf::launch_mg_up(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_up", l));
+    for (int l = b - 1; l >= 0;) {
+      if (l >= 2 && pair_first(s, l - 1)) {
+        cf::launch_mg_up2(s->L[l - 1], s->L[l], s->L[l + 1], nu, s->pair_tile, skip, s->st);
+        mark(s, level_tag("mg_up2", l - 1));
+        l -= 2;
+      } else {
+        cf::launch_mg_up(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_up", l));
+        l -= 1;
+      }
     }
     return;
   }
@@ -883,6 +916,10 @@ static int sim_levels(cf_sim* s, int nx, int ny, double dx, double dt) {
   for (int l = 0; l < s->nlev; l++) { s->L[l].nx = s->lnx[l]; s->L[l].ny = s->lny[l]; s->L[l].h2 = s->lh2[l]; }
   // bottom solver: the last 1-3 levels (<= 4096, 1024, 256 cells) in one workgroup
   s->mg_bottom = cf::mg_bottom_first(s->L, s->nlev);
+  // with two-level legs the levels from 1 go in pairs; when they would leave
+  // one over and the bottom kernel holds three levels, its first joins them
+  s->mg_bottom_pairs = s->mg_bottom;
+  if ((s->mg_bottom - 1) % 2 == 1 && s->nlev - s->mg_bottom == 3) s->mg_bottom_pairs = s->mg_bottom + 1;
   return CF_OK;
 }
 
@@ -914,6 +951,8 @@ static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
     add((void**)&s->LH[l].p, lx * ly * 2); add((void**)&s->LH[l].q, lx * ly * 2);
     add((void**)&s->LH[l].ax, (lx + 1) * ly * 2); add((void**)&s->LH[l].ay, lx * (ly + 1) * 2);
   }
+  for (int l = s->mg_bottom; l < s->mg_bottom_pairs; l++)  // the level the two-level legs take from the bottom
+    add((void**)&s->L[l].q, (size_t)s->lnx[l] * s->lny[l] * 4);
   const unsigned nparts = (unsigned)(((s->nx + 31) / 32) * ((s->ny + 7) / 8));
   add((void**)&s->dPart, (size_t)nparts * 8);
   add((void**)&s->dAccPart, (size_t)nparts * 16);
@@ -961,6 +1000,10 @@ static int sim_create(int nx, int ny, double dx, double dt, hipStream_t shared, 
   s->use_graph = !(ng && *ng && *ng != '0');
   const char* ms = getenv("CF_MG_SIMPLE");
   s->mg_simple = ms && *ms && *ms != '0';
+  const char* lp = getenv("CF_LEG_PAIRS");
+  s->leg_pairs = !(lp && *lp == '0');
+  const char* pt = getenv("CF_PAIR_TILE");
+  s->pair_tile = pt && (atoi(pt) == 8 || atoi(pt) == 16) ? atoi(pt) : 0;
 
   hipError_t e = hipGetDevice(&s->dev);
   if (e != hipSuccess) { delete s; return set_err(CF_EDEVICE, "hipGetDevice: %s", hipGetErrorString(e)); }
@@ -1150,7 +1193,7 @@ int cf_sim_set_leg_tiles(cf_sim* s, const int32_t* tiles, int n) {
 
 int cf_sim_get_leg_tiles(const cf_sim* s, int32_t* tiles, int n) {
   if (!s || (n > 0 && !tiles)) return set_err(CF_EINVAL, "bad argument");
-  for (int l = 0; l < n; l++) tiles[l] = l < s->mg_bottom ? cf::mg_tile(s->L[l]) : 0;
+  for (int l = 0; l < n; l++) tiles[l] = l < s->mg_bottom && !paired(s, l) ? cf::mg_tile(s->L[l]) : 0;
   return s->mg_bottom;
 }
 

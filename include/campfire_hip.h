@@ -154,8 +154,10 @@ int  cf_sim_get_schedule(const cf_sim* sim, cf_schedule* s);
  * changes only the launch shape, which the autotuner picks by timing.  No
  * counterpart in the reference (its relax dispatches are 8x8 workgroups). */
 int  cf_sim_set_leg_tiles(cf_sim* sim, const int32_t* tiles, int n);
-/* The tile edges in use for levels 0 .. n-1 (0 past the tiled levels);
- * returns the number of tiled levels. */
+/* The tile edges in use for levels 0 .. n-1 (0 past the tiled levels, and 0
+ * for a level whose legs run in a two-level pair: with FP32 storage a lone
+ * scene runs the legs of levels 1-2, 3-4, ... two per launch, with tiles of
+ * their own); returns the number of tiled levels. */
 int  cf_sim_get_leg_tiles(const cf_sim* sim, int32_t* tiles, int n);
 
 /* uploadPieces(Float32Array(64*20), n)  (js/fluid.js:201-204); also sets
