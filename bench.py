@@ -72,6 +72,10 @@ def kernel_bytes(tag, nx, ny, levels, launches):
         return 22 * fine[lev]
     if base == "mg_up" and lev is not None:     # p, coarse p, rhs, ax, ay read; p written
         return 21 * fine[lev]
+    if base == "mg_up_project" and lev is not None:
+        # the level-0 up leg (21 B/cell) and the projection without its p
+        # reads: U, V, fx, fy, T read, U, V written
+        return 21 * fine[0] + 28 * nC
     if base == "mg_down2" and lev is not None:
         # levels lev, lev+1 in one launch: fine p, rhs, ax, ay read, q written;
         # coarse ax, ay read, q, rhs written; next level's rhs, p written

@@ -46,21 +46,33 @@ template <class T, int W, int H>
 struct Win {
   const T* s;
   int ox, oy;
-  __device__ __forceinline__ const T* at(int i, int j, bool& ok) const {
-    const int a = i - ox, b = j - oy;
-    const bool in = a >= 0 && a < W - 1 && b >= 0 && b < H - 1;
+  float fox, foy;  // ox, oy as floats
+  // the footprint whose lower-left corner is at the floored position (bx, by):
+  // tested in float, exact for values in the window (small integers) and
+  // false for NaN or huge ones, so the conversion needs no clamp first
+  __device__ __forceinline__ const T* at(float bx, float by, bool& ok) const {
+    const float a = bx - fox, b = by - foy;
+    const bool in = a >= K(0) && a < (float)(W - 1) && b >= K(0) && b < (float)(H - 1);
     ok = ok && in;
-    return s + (in ? b * W + a : 0);
+    return s + (in ? (int)b * W + (int)a : 0);
   }
   __device__ __forceinline__ T own(int i, int j) const { return s[(j - oy) * W + (i - ox)]; }
   // bilinear sample at (fx, fy) = grid position minus the field's stagger
   __device__ __forceinline__ T bil(float fx, float fy, bool& ok) const {
     const float bx = floorf(fx), by = floorf(fy);
     const float tx = fx - bx, ty = fy - by;
-    const T* p = at(toint(bx), toint(by), ok);
+    const T* p = at(bx, by, ok);
     return mixv(mixv(p[0], p[1], tx), mixv(p[W], p[W + 1], tx), ty);
   }
 };
+
+// x / d for the traces: div_rcp_fast without the sign fix-up, so a zero
+// quotient may carry the wrong sign; every quotient here is subtracted from a
+// positive position (or +0), where either zero gives the same result
+__device__ __forceinline__ float div_trace(float x, float d, float y) {
+  const float q = x * y;
+  return fmaf(fmaf(-q, d, x), y, q);
+}
 
 __device__ __forceinline__ float2 mm4v(float a, float c, float d, float e) {
   return make_float2(fminf(fminf(a, c), fminf(d, e)), fmaxf(fmaxf(a, c), fmaxf(d, e)));
@@ -88,12 +100,12 @@ struct Tile {
 // halo is AR, the points are within AH of the tile, the footprint reaches 1
 // more), so `ok` is unchanged by these samples.
 __device__ __forceinline__ float2 trace_rest(const Tile& t, float gx, float gy, float u, float v, float dt, bool& ok) {
-  const float vx = div_rcp_fast(u, t.dx, t.idx);
-  const float vy = div_rcp_fast(v, t.dx, t.idx);
+  const float vx = div_trace(u, t.dx, t.idx);
+  const float vy = div_trace(v, t.dx, t.idx);
   const float hdt = K(0.5) * dt;
   const float mx_ = gx - hdt * vx, my_ = gy - hdt * vy;
-  const float wx = div_rcp_fast(t.U.bil(mx_ - K(0), my_ - K(0.5), ok), t.dx, t.idx);
-  const float wy = div_rcp_fast(t.V.bil(mx_ - K(0.5), my_ - K(0), ok), t.dx, t.idx);
+  const float wx = div_trace(t.U.bil(mx_ - K(0), my_ - K(0.5), ok), t.dx, t.idx);
+  const float wy = div_trace(t.V.bil(mx_ - K(0.5), my_ - K(0), ok), t.dx, t.idx);
   return make_float2(gx - dt * wx, gy - dt * wy);
 }
 // from cell centre (i + 1/2, j + 1/2): U footprint (i, j) with (0.5, 0), V (i, j) with (0, 0.5)
@@ -243,8 +255,11 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   const float4 aY = make_float4(0.f, P.YOamb, 0.f, 0.f);
   const int ax0 = x0 - AR, ay0 = y0 - AR, bx0 = x0 - AH, by0 = y0 - AH;
   Tile t;
-  t.T = {sT, ax0, ay0}; t.Y = {sY, ax0, ay0}; t.U = {sU, ax0, ay0}; t.V = {sV, ax0, ay0};
-  t.Th = {bT, bx0, by0}; t.Yh = {bY, bx0, by0}; t.Uh = {bU, bx0, by0}; t.Vh = {bV, bx0, by0};
+  const float fax0 = (float)ax0, fay0 = (float)ay0, fbx0 = (float)bx0, fby0 = (float)by0;
+  t.T = {sT, ax0, ay0, fax0, fay0}; t.Y = {sY, ax0, ay0, fax0, fay0};
+  t.U = {sU, ax0, ay0, fax0, fay0}; t.V = {sV, ax0, ay0, fax0, fay0};
+  t.Th = {bT, bx0, by0, fbx0, fby0}; t.Yh = {bY, bx0, by0, fbx0, fby0};
+  t.Uh = {bU, bx0, by0, fbx0, fby0}; t.Vh = {bV, bx0, by0, fbx0, fby0};
   t.dx = P.dx;
   t.idx = K(1) / P.dx;
 
@@ -392,8 +407,8 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       bool ok = S.okC;
       const float2 d = S.dC;
       const float2 f = trace_cell(t, i, j, -dt, ok);
-      const float* pT = t.T.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
-      const float4* pY = t.Y.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0.5))), ok);
+      const float* pT = t.T.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
+      const float4* pY = t.Y.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
       const float sbT = t.Th.bil(f.x - K(0.5), f.y - K(0.5), ok);
       const float4 sb = t.Yh.bil(f.x - K(0.5), f.y - K(0.5), ok);
       float To;
@@ -429,7 +444,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       const float2 b = trace_uface(t, i, j, -dt, ok);
       const float back = t.Uh.bil(b.x - K(0), b.y - K(0.5), ok);
       const float2 d = S.dU;
-      const float* pU = t.U.at(toint(floorf(d.x - K(0))), toint(floorf(d.y - K(0.5))), ok);
+      const float* pU = t.U.at(floorf(d.x - K(0)), floorf(d.y - K(0.5)), ok);
       float u;
       if (ok) {
         const float2 mm = mm4v(pU[0], pU[1], pU[A_UX], pU[A_UX + 1]);
@@ -444,7 +459,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       const float2 b = trace_vface(t, i, j, -dt, ok);
       const float back = t.Vh.bil(b.x - K(0.5), b.y - K(0), ok);
       const float2 d = S.dV;
-      const float* pV = t.V.at(toint(floorf(d.x - K(0.5))), toint(floorf(d.y - K(0))), ok);
+      const float* pV = t.V.at(floorf(d.x - K(0.5)), floorf(d.y - K(0)), ok);
       float v;
       if (ok) {
         const float2 mm = mm4v(pV[0], pV[1], pV[A_VX], pV[A_VX + 1]);

@@ -267,6 +267,9 @@ void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* pa
 // once *skip is non-zero (the residual target of the step has been met)
 bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
+// the level-0 up leg of a step's last V-cycle with the projection of the
+// velocities fused in (FP32 storage; replaces the k_project launch)
+bool launch_mg_up_project(const Level& f, const Level& c, const Fields& F, int nu, const int* skip, hipStream_t s);
 // the legs of two levels f, c = f + 1 in one launch (FP32 storage), n = f + 2;
 // tile: the edge of a workgroup's tile in cells of c (16 or 8; 0: mg_pair_tile)
 bool launch_mg_down2(const Level& f, const Level& c, const Level& n, int nu, int tile, const int* skip,

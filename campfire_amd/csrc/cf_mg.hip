@@ -46,22 +46,24 @@ __device__ __forceinline__ LegCell leg_cell_coef(int nx, int ny, const CT* __res
   const bool upd = in && !(rx == 0 && x > 0) && !(rx == RX - 1 && x < nx - 1) && !(ry == 0 && y > 0) &&
                    !(ry == RY - 1 && y < ny - 1);
   c.ci = ((ry + 1) * PX + rx + 1) | (upd ? LEG_UPD : 0) | (in ? LEG_IN : 0);
-  c.w = c.e = c.s = c.n = c.d = c.y = c.rh = 0.f;
-  if (in) {
-    const float w = ldc(ax, y * (nx + 1) + x), e = ldc(ax, y * (nx + 1) + x + 1);
-    const float s = ldc(ay, y * nx + x), n = ldc(ay, (y + 1) * nx + x);
-    float diag = 0.f;  // shaders/pressure.wgsl:77-85, same accumulation order
-    if (x > 0) diag += w; else diag += K(2.0) * w;
-    if (x < nx - 1) diag += e; else diag += K(2.0) * e;
-    if (y > 0) diag += s;
-    if (y < ny - 1) diag += n; else diag += K(2.0) * n;
-    c.d = diag;
-    c.y = diag > K(0) ? K(1) / diag : K(0);
-    c.w = x > 0 ? w : 0.f;
-    c.e = x < nx - 1 ? e : 0.f;
-    c.s = y > 0 ? s : 0.f;
-    c.n = y < ny - 1 ? n : 0.f;
-  }
+  // the loads are unconditional (a cell outside the domain reads cell 0's
+  // faces and discards them): no branch separates them, so every load of a
+  // thread's cells is in flight at once instead of one round trip per cell
+  const int xx = in ? x : 0, yy = in ? y : 0;
+  const float w = ldc(ax, yy * (nx + 1) + xx), e = ldc(ax, yy * (nx + 1) + xx + 1);
+  const float s = ldc(ay, yy * nx + xx), n = ldc(ay, (yy + 1) * nx + xx);
+  float diag = 0.f;  // shaders/pressure.wgsl:77-85, same accumulation order
+  if (x > 0) diag += w; else diag += K(2.0) * w;
+  if (x < nx - 1) diag += e; else diag += K(2.0) * e;
+  if (y > 0) diag += s;
+  if (y < ny - 1) diag += n; else diag += K(2.0) * n;
+  c.d = in ? diag : K(0);
+  c.y = in && diag > K(0) ? K(1) / diag : K(0);
+  c.w = in && x > 0 ? w : 0.f;
+  c.e = in && x < nx - 1 ? e : 0.f;
+  c.s = in && y > 0 ? s : 0.f;
+  c.n = in && y < ny - 1 ? n : 0.f;
+  c.rh = 0.f;
   return c;
 }
 
@@ -71,13 +73,15 @@ __device__ __forceinline__ LegCell leg_cell(int nx, int ny, const CT* __restrict
                                             const float* __restrict__ rhs, float h2, int x, int y, int rx, int ry,
                                             int RX, int RY, int PX, float& rhs_out) {
   LegCell c = leg_cell_coef(nx, ny, ax, ay, x, y, rx, ry, RX, RY, PX);
-  rhs_out = 0.f;
-  if (c.ci & LEG_IN) {
-    rhs_out = rhs[y * nx + x];
-    c.rh = rhs_out * h2;
-  }
+  const bool in = (c.ci & LEG_IN) != 0;
+  const float r = rhs[in ? y * nx + x : 0];
+  rhs_out = in ? r : 0.f;
+  c.rh = in ? r * h2 : 0.f;
   return c;
 }
+
+// a level's value at cell (x, y) when `in` (else cell 0's, to be discarded)
+__device__ __forceinline__ int leg_idx(bool in, int x, int y, int nx) { return in ? y * nx + x : 0; }
 
 template <int M>
 __device__ __forceinline__ LegCell leg_cell_m(const Level& f, const MgStore& st, int x, int y, int rx, int ry, int RX,
@@ -153,25 +157,38 @@ struct MgDownOp {
       sp[idx] = 0.f;
     }
     LegCell cR[PQ], cB[PQ];
+    float pv[PQ][2], rv[PQ][2];
+    // every global load first, branch-free (a slot without a pair loads pair 0's data)
 #pragma unroll
     for (int q = 0; q < PQ; q++) {
       const int pi = tid + kThreads * q;
-      cR[q] = cB[q] = leg_none(PX);
+      const int pc = pi < NP ? pi : 0;
+      const int ry = pc / (RX / 2), rx = 2 * (pc % (RX / 2));
+      const int x = x0 - H + rx, y = y0 - H + ry;
+      const LegCell c0 = leg_cell_m<M>(f, a_.st, x, y, rx, ry, RX, RY, PX, rv[q][0]);
+      const LegCell c1 = leg_cell_m<M>(f, a_.st, x + 1, y, rx + 1, ry, RX, RY, PX, rv[q][1]);
+      const bool in0 = (c0.ci & LEG_IN) != 0, in1 = (c1.ci & LEG_IN) != 0;
+      const float v0 = ldp<FH>(f.p, a_.st.f.p, leg_idx(in0, x, y, nx), isig);
+      const float v1 = ldp<FH>(f.p, a_.st.f.p, leg_idx(in1, x + 1, y, nx), isig);
+      pv[q][0] = in0 ? v0 : 0.f;
+      pv[q][1] = in1 ? v1 : 0.f;
+      const bool red0 = ((x + y) & 1) == 0;
+      cR[q] = red0 ? c0 : c1;
+      cB[q] = red0 ? c1 : c0;
+    }
+#pragma unroll
+    for (int q = 0; q < PQ; q++) {
+      const int pi = tid + kThreads * q;
       if (pi < NP) {
         const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
-        const int x = x0 - H + rx, y = y0 - H + ry;
-        float r0, r1;
-        const LegCell c0 = leg_cell_m<M>(f, a_.st, x, y, rx, ry, RX, RY, PX, r0);
-        const LegCell c1 = leg_cell_m<M>(f, a_.st, x + 1, y, rx + 1, ry, RX, RY, PX, r1);
-        sp[c0.ci & LEG_IDX] = (c0.ci & LEG_IN) ? ldp<FH>(f.p, a_.st.f.p, y * nx + x, isig) : 0.f;
-        sp[c1.ci & LEG_IDX] = (c1.ci & LEG_IN) ? ldp<FH>(f.p, a_.st.f.p, y * nx + x + 1, isig) : 0.f;
+        sp[(ry + 1) * PX + rx + 1] = pv[q][0];
+        sp[(ry + 1) * PX + rx + 2] = pv[q][1];
         if (ry >= H && ry < H + TY) {
-          if (rx >= H && rx < H + TX) sres[(ry - H) * TX + (rx - H)] = r0;
-          if (rx + 1 >= H && rx + 1 < H + TX) sres[(ry - H) * TX + (rx + 1 - H)] = r1;
+          if (rx >= H && rx < H + TX) sres[(ry - H) * TX + (rx - H)] = rv[q][0];
+          if (rx + 1 >= H && rx + 1 < H + TX) sres[(ry - H) * TX + (rx + 1 - H)] = rv[q][1];
         }
-        const bool red0 = ((x + y) & 1) == 0;
-        cR[q] = red0 ? c0 : c1;
-        cB[q] = red0 ? c1 : c0;
+      } else {
+        cR[q] = cB[q] = leg_none(PX);
       }
     }
     __syncthreads();
@@ -219,18 +236,27 @@ __global__ __launch_bounds__((MgDownOp<TX, TY, NU, M>::kThreads)) void k_mg_down
   MgDownOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// the fine p is read from the leg buffer q and written to p
-template <int TX, int TY, int NU, int M>
+// the fine p is read from the leg buffer q and written to p.  PROJ: the
+// level-0 leg of a step's last V-cycle also projects the velocities of its
+// tile's faces (shaders/pressure.wgsl's project, as k_project does it): one
+// more ring of halo makes p exact one cell beyond the tile, where the faces on
+// the tile's edges take their pressure differences.
+template <int TX, int TY, int NU, int M, bool PROJ = false>
 struct MgUpOp {
-  struct Args {
+  struct ArgsLeg {
     Level f;
     Level c;
     MgStore st;
     const int* skip;
     int vg;  // tiles of this scene's launch
   };
+  struct ArgsProj : ArgsLeg {
+    Fields F;
+  };
+  using Args = std::conditional_t<PROJ, ArgsProj, ArgsLeg>;
+  static_assert(!PROJ || M == 0, "the projection reads FP32 p");
   static constexpr int kThreads = TX * TY >= 4096 ? 1024 : 256;  // 64x64 tiles: 1024
-  static constexpr int H = 2 * NU;
+  static constexpr int H = 2 * NU + (PROJ ? 1 : 0);
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, PX = RX + 2, PY = RY + 2, NP = RX * RY / 2;
   static constexpr int PQ = (NP + kThreads - 1) / kThreads;
   static_assert(RX % 2 == 0, "pairs need an even region width");
@@ -253,39 +279,57 @@ struct MgUpOp {
       sp[idx] = 0.f;
     }
     LegCell cR[PQ], cB[PQ];
+    float pv[PQ][2], cv[PQ][2][4];
+    // every global load first, branch-free (a slot without a pair, or a cell
+    // outside the domain, loads cell 0's data and discards it); the values
+    // combine in a second pass, so no select pulls a load into a branch
 #pragma unroll
     for (int q = 0; q < PQ; q++) {
       const int pi = tid + kThreads * q;
-      cR[q] = cB[q] = leg_none(PX);
+      const int pc = pi < NP ? pi : 0;
+      const int ry = pc / (RX / 2), rx = 2 * (pc % (RX / 2));
+      const int y = y0 - H + ry;
+      LegCell cc[2];
+#pragma unroll
+      for (int sub = 0; sub < 2; sub++) {
+        const int x = x0 - H + rx + sub;
+        float rr;
+        cc[sub] = leg_cell_m<M>(f, a_.st, x, y, rx + sub, ry, RX, RY, PX, rr);
+        const bool in = (cc[sub].ci & LEG_IN) != 0;
+        const int xi = in ? x : 0, yi = in ? y : 0;
+        // bilinear coarse correction, skipped in closed cells (:116-129)
+        const int I = xi / 2, J = yi / 2;
+        const int I2 = iclamp(I + ((xi & 1) == 1 ? 1 : -1), 0, nxc - 1);
+        const int J2 = iclamp(J + ((yi & 1) == 1 ? 1 : -1), 0, nyc - 1);
+        cv[q][sub][0] = ldp<CH>(c.p, a_.st.c.p, J * nxc + I, isig);
+        cv[q][sub][1] = ldp<CH>(c.p, a_.st.c.p, J * nxc + I2, isig);
+        cv[q][sub][2] = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I, isig);
+        cv[q][sub][3] = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I2, isig);
+        pv[q][sub] = ldp<FH>(f.q, a_.st.f.q, yi * nx + xi, isig);
+      }
+      const bool red0 = ((x0 - H + rx + y) & 1) == 0;
+      cR[q] = red0 ? cc[0] : cc[1];
+      cB[q] = red0 ? cc[1] : cc[0];
+    }
+#pragma unroll
+    for (int q = 0; q < PQ; q++) {
+      const int pi = tid + kThreads * q;
       if (pi < NP) {
         const int ry = pi / (RX / 2), rx = 2 * (pi % (RX / 2));
-        const int y = y0 - H + ry;
-        LegCell cc[2];
+        const bool red0 = ((x0 - H + rx + y0 - H + ry) & 1) == 0;
 #pragma unroll
         for (int sub = 0; sub < 2; sub++) {
-          const int x = x0 - H + rx + sub;
-          float rr;
-          cc[sub] = leg_cell_m<M>(f, a_.st, x, y, rx + sub, ry, RX, RY, PX, rr);
-          float v = 0.f;
-          if (cc[sub].ci & LEG_IN) {
-            // bilinear coarse correction, skipped in closed cells (:116-129)
-            const int I = x / 2, J = y / 2;
-            const int I2 = iclamp(I + ((x & 1) == 1 ? 1 : -1), 0, nxc - 1);
-            const int J2 = iclamp(J + ((y & 1) == 1 ? 1 : -1), 0, nyc - 1);
-            const float c00 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I, isig);
-            const float c10 = ldp<CH>(c.p, a_.st.c.p, J * nxc + I2, isig);
-            const float c01 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I, isig);
-            const float c11 = ldp<CH>(c.p, a_.st.c.p, J2 * nxc + I2, isig);
-            const float ccr = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
-            v = ldp<FH>(f.q, a_.st.f.q, y * nx + x, isig) + (cc[sub].d > K(0) ? ccr : K(0));
-          }
-          sp[cc[sub].ci & LEG_IDX] = v;
+          const LegCell& a = (sub == 0) == red0 ? cR[q] : cB[q];
+          const float* cw = cv[q][sub];
+          const float ccr = K(0.5625) * cw[0] + K(0.1875) * (cw[1] + cw[2]) + K(0.0625) * cw[3];
+          sp[(ry + 1) * PX + rx + 1 + sub] = (a.ci & LEG_IN) ? pv[q][sub] + (a.d > K(0) ? ccr : K(0)) : 0.f;
         }
-        const bool red0 = ((x0 - H + rx + y) & 1) == 0;
-        cR[q] = red0 ? cc[0] : cc[1];
-        cB[q] = red0 ? cc[1] : cc[0];
+      } else {
+        cR[q] = cB[q] = leg_none(PX);
       }
     }
+    Pre pre;
+    if constexpr (PROJ) pre_load(a_, x0, y0, pre);
     __syncthreads();
     for (int s = 0; s < NU; s++) {  // black then red (js/fluid.js:307-310)
 #pragma unroll
@@ -306,11 +350,97 @@ struct MgUpOp {
       finish(cR[q]);
       finish(cB[q]);
     }
+    if constexpr (PROJ) project(a_, sp, x0, y0, pre);
+  }
+  // The faces of the tile: U faces x0 .. x0 + TX - 1 (and nx on the domain's
+  // right edge), V faces y0 .. y0 + TY - 1 (and ny on its top).  Their
+  // velocity and open fraction are loaded before the sweeps (they do not
+  // depend on p), so those loads overlap the leg; the temperatures follow
+  // the sweeps (holding them too would spill registers).
+  static constexpr int QU = ((TX + 1) * TY + kThreads - 1) / kThreads, QV = (TX * (TY + 1) + kThreads - 1) / kThreads;
+  struct Pre {
+    float uf[QU], uu[QU];
+    float vf[QV], vv[QV];
+  };
+  static __device__ __forceinline__ bool uface(int k, int x0, int y0, int nx, int ny, int& i, int& j) {
+    const int a = k % (TX + 1), b = k / (TX + 1);
+    i = x0 + a;
+    j = y0 + b;
+    return k < (TX + 1) * TY && j < ny && i <= nx && (a < TX || i == nx);
+  }
+  static __device__ __forceinline__ bool vface(int k, int x0, int y0, int nx, int ny, int& i, int& j) {
+    const int a = k % TX, b = k / TX;
+    i = x0 + a;
+    j = y0 + b;
+    return k < TX * (TY + 1) && i < nx && j <= ny && (b < TY || j == ny);
+  }
+  // (a slot without a face loads face 0's data and discards it)
+  static __device__ __forceinline__ void pre_load(const Args& a_, int x0, int y0, Pre& r) {
+    const Fields& F = a_.F;
+    const int nx = a_.f.nx, ny = a_.f.ny;
+#pragma unroll
+    for (int q = 0; q < QU; q++) {
+      int i, j;
+      const bool ok = uface(threadIdx.x + kThreads * q, x0, y0, nx, ny, i, j);
+      const int k = ok ? j * (nx + 1) + i : 0;
+      r.uf[q] = F.fx[k];
+      r.uu[q] = F.U[k];
+    }
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+      int i, j;
+      const bool ok = vface(threadIdx.x + kThreads * q, x0, y0, nx, ny, i, j);
+      const int k = ok ? j * nx + i : 0;
+      r.vf[q] = F.fy[k];
+      r.vv[q] = F.V[k];
+    }
+  }
+  // k_project's update of each face (shaders/pressure.wgsl's project), the
+  // pressure from the region (exact over the tile plus one ring)
+  static __device__ __forceinline__ void project(const Args& a_, const float* sp, int x0, int y0, const Pre& r) {
+    const Fields& F = a_.F;
+    const cf_params P = *F.P;
+    const int nx = P.nx, ny = P.ny;
+    const float dt = P.dt, dx = P.dx;
+    auto cell = [&](int i, int j) { return (j - y0 + H + 1) * PX + (i - x0 + H + 1); };
+#pragma unroll
+    for (int q = 0; q < QU; q++) {
+      int i, j;
+      if (!uface(threadIdx.x + kThreads * q, x0, y0, nx, ny, i, j)) continue;
+      const int kf = j * (nx + 1) + i;
+      if (r.uf[q] > K(0)) {
+        const int ci = cell(i, j);
+        const float pl = i > 0 ? sp[ci - 1] : -sp[ci];
+        const float pr = i < nx ? sp[ci] : -sp[ci - 1];
+        const float dp = pr - pl;
+        const float Tf = K(0.5) * (F.T[j * nx + iclamp(i - 1, 0, nx - 1)] + F.T[j * nx + min(i, nx - 1)]);
+        F.U[kf] = r.uu[q] - dt * invRho(P, Tf) * dp / dx;
+      } else F.U[kf] = K(0);
+    }
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+      int i, j;
+      if (!vface(threadIdx.x + kThreads * q, x0, y0, nx, ny, i, j)) continue;
+      const int kf = j * nx + i;
+      if (j > 0 && r.vf[q] > K(0)) {
+        const int ci = cell(i, j);
+        const float pb = sp[ci - PX];
+        const float pt = j < ny ? sp[ci] : -sp[ci - PX];
+        const float dp = pt - pb;
+        const float Tf = K(0.5) * (F.T[(j - 1) * nx + i] + F.T[min(j, ny - 1) * nx + i]);
+        F.V[kf] = r.vv[q] - dt * invRho(P, Tf) * dp / dx;
+      } else F.V[kf] = K(0);
+    }
   }
 };
 template <int TX, int TY, int NU, int M>
 __global__ __launch_bounds__((MgUpOp<TX, TY, NU, M>::kThreads)) void k_mg_up(const typename MgUpOp<TX, TY, NU, M>::Args a_) {
   MgUpOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+template <int TX, int TY, int NU>
+__global__ __launch_bounds__((MgUpOp<TX, TY, NU, 0, true>::kThreads)) void k_mg_up_project(
+    const typename MgUpOp<TX, TY, NU, 0, true>::Args a_) {
+  MgUpOp<TX, TY, NU, 0, true>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---- two levels per launch -------------------------------------------------------
@@ -379,43 +509,62 @@ struct MgDown2Op {
     leg_ring(spc, PC, PC, RC, NT);
     // every global load before the first sweep: the fine cells, rhs and p,
     // the coarse cells' coefficients
+    // (branch-free: a slot without a pair loads pair 0's data and discards it)
     LegCell fR[QF], fB[QF];
+    float pv[QF][2], rv[QF][2];
 #pragma unroll
     for (int q = 0; q < QF; q++) {
       const int pi = tid + NT * q;
-      fR[q] = fB[q] = leg_none(PF);
-      if (pi < NPF) {
-        const int ry = pi / (RF / 2), rx = 2 * (pi % (RF / 2));
-        const int x = xf0 + rx, y = yf0 + ry;
-        float r0, r1;
-        const LegCell c0 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx, ry, RF, RF, PF, r0);
-        const LegCell c1 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x + 1, y, rx + 1, ry, RF, RF, PF, r1);
-        spf[c0.ci & LEG_IDX] = (c0.ci & LEG_IN) ? f.p[y * nx + x] : 0.f;
-        spf[c1.ci & LEG_IDX] = (c1.ci & LEG_IN) ? f.p[y * nx + x + 1] : 0.f;
-        if (ry >= HF && ry < HF + RR) {
-          if (rx >= HF && rx < HF + RR) sres[(ry - HF) * RR + (rx - HF)] = r0;
-          if (rx + 1 >= HF && rx + 1 < HF + RR) sres[(ry - HF) * RR + (rx + 1 - HF)] = r1;
-        }
-        const bool red0 = ((x + y) & 1) == 0;
-        fR[q] = red0 ? c0 : c1;
-        fB[q] = red0 ? c1 : c0;
-      }
+      const int pc = pi < NPF ? pi : 0;
+      const int ry = pc / (RF / 2), rx = 2 * (pc % (RF / 2));
+      const int x = xf0 + rx, y = yf0 + ry;
+      const LegCell c0 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx, ry, RF, RF, PF, rv[q][0]);
+      const LegCell c1 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x + 1, y, rx + 1, ry, RF, RF, PF, rv[q][1]);
+      const bool in0 = (c0.ci & LEG_IN) != 0, in1 = (c1.ci & LEG_IN) != 0;
+      const float v0 = f.p[leg_idx(in0, x, y, nx)], v1 = f.p[leg_idx(in1, x + 1, y, nx)];
+      pv[q][0] = in0 ? v0 : 0.f;
+      pv[q][1] = in1 ? v1 : 0.f;
+      const bool red0 = ((x + y) & 1) == 0;
+      fR[q] = red0 ? c0 : c1;
+      fB[q] = red0 ? c1 : c0;
     }
     LegCell cR[QC], cB[QC];
 #pragma unroll
     for (int q = 0; q < QC; q++) {
       const int pi = tid + NT * q;
-      cR[q] = cB[q] = leg_none(PC);
+      const int pc = pi < NPC ? pi : 0;
+      const int ry = pc / (RC / 2), rx = 2 * (pc % (RC / 2));
+      const int X = xc0 + rx, Y = yc0 + ry;
+      const LegCell c0 = leg_cell_coef(nxc, nyc, c.ax, c.ay, X, Y, rx, ry, RC, RC, PC);
+      const LegCell c1 = leg_cell_coef(nxc, nyc, c.ax, c.ay, X + 1, Y, rx + 1, ry, RC, RC, PC);
+      const bool red0 = ((X + Y) & 1) == 0;
+      cR[q] = red0 ? c0 : c1;
+      cB[q] = red0 ? c1 : c0;
+    }
+#pragma unroll
+    for (int q = 0; q < QF; q++) {
+      const int pi = tid + NT * q;
+      if (pi < NPF) {
+        const int ry = pi / (RF / 2), rx = 2 * (pi % (RF / 2));
+        spf[(ry + 1) * PF + rx + 1] = pv[q][0];
+        spf[(ry + 1) * PF + rx + 2] = pv[q][1];
+        if (ry >= HF && ry < HF + RR) {
+          if (rx >= HF && rx < HF + RR) sres[(ry - HF) * RR + (rx - HF)] = rv[q][0];
+          if (rx + 1 >= HF && rx + 1 < HF + RR) sres[(ry - HF) * RR + (rx + 1 - HF)] = rv[q][1];
+        }
+      } else {
+        fR[q] = fB[q] = leg_none(PF);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      const int pi = tid + NT * q;
       if (pi < NPC) {
         const int ry = pi / (RC / 2), rx = 2 * (pi % (RC / 2));
-        const int X = xc0 + rx, Y = yc0 + ry;
-        const LegCell c0 = leg_cell_coef(nxc, nyc, c.ax, c.ay, X, Y, rx, ry, RC, RC, PC);
-        const LegCell c1 = leg_cell_coef(nxc, nyc, c.ax, c.ay, X + 1, Y, rx + 1, ry, RC, RC, PC);
-        spc[c0.ci & LEG_IDX] = 0.f;  // the coarse leg starts from p = 0 (shaders/pressure.wgsl:104-113)
-        spc[c1.ci & LEG_IDX] = 0.f;
-        const bool red0 = ((X + Y) & 1) == 0;
-        cR[q] = red0 ? c0 : c1;
-        cB[q] = red0 ? c1 : c0;
+        spc[(ry + 1) * PC + rx + 1] = 0.f;  // the coarse leg starts from p = 0 (shaders/pressure.wgsl:104-113)
+        spc[(ry + 1) * PC + rx + 2] = 0.f;
+      } else {
+        cR[q] = cB[q] = leg_none(PC);
       }
     }
     __syncthreads();
@@ -550,36 +699,36 @@ struct MgUp2Op {
     leg_ring(spc, PC, PC, RC, NT);
     // the coarse cells: coefficients, rhs, start value q + the bilinear
     // correction from n, skipped in closed cells (shaders/pressure.wgsl:116-129)
+    // (branch-free: a slot without a pair, or a cell outside the domain,
+    // loads cell 0's data and discards it)
     LegCell cR[QC], cB[QC];
+    float pv[QC][2];
 #pragma unroll
     for (int q = 0; q < QC; q++) {
       const int pi = tid + NT * q;
-      cR[q] = cB[q] = leg_none(PC);
-      if (pi < NPC) {
-        const int ry = pi / (RC / 2), rx = 2 * (pi % (RC / 2));
-        const int Y = yc0 + ry;
-        LegCell cc[2];
+      const int pc = pi < NPC ? pi : 0;
+      const int ry = pc / (RC / 2), rx = 2 * (pc % (RC / 2));
+      const int Y = yc0 + ry;
+      LegCell cc[2];
 #pragma unroll
-        for (int sub = 0; sub < 2; sub++) {
-          const int X = xc0 + rx + sub;
-          float rr;
-          cc[sub] = leg_cell(nxc, nyc, c.ax, c.ay, c.rhs, c.h2, X, Y, rx + sub, ry, RC, RC, PC, rr);
-          float v = 0.f;
-          if (cc[sub].ci & LEG_IN) {
-            const int I = X / 2, J = Y / 2;
-            const int I2 = iclamp(I + ((X & 1) == 1 ? 1 : -1), 0, nxn - 1);
-            const int J2 = iclamp(J + ((Y & 1) == 1 ? 1 : -1), 0, nyn - 1);
-            const float c00 = n.p[J * nxn + I], c10 = n.p[J * nxn + I2];
-            const float c01 = n.p[J2 * nxn + I], c11 = n.p[J2 * nxn + I2];
-            const float ccr = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
-            v = c.q[Y * nxc + X] + (cc[sub].d > K(0) ? ccr : K(0));
-          }
-          spc[cc[sub].ci & LEG_IDX] = v;
-        }
-        const bool red0 = ((xc0 + rx + Y) & 1) == 0;
-        cR[q] = red0 ? cc[0] : cc[1];
-        cB[q] = red0 ? cc[1] : cc[0];
+      for (int sub = 0; sub < 2; sub++) {
+        const int X = xc0 + rx + sub;
+        float rr;
+        cc[sub] = leg_cell(nxc, nyc, c.ax, c.ay, c.rhs, c.h2, X, Y, rx + sub, ry, RC, RC, PC, rr);
+        const bool in = (cc[sub].ci & LEG_IN) != 0;
+        const int Xi = in ? X : 0, Yi = in ? Y : 0;
+        const int I = Xi / 2, J = Yi / 2;
+        const int I2 = iclamp(I + ((Xi & 1) == 1 ? 1 : -1), 0, nxn - 1);
+        const int J2 = iclamp(J + ((Yi & 1) == 1 ? 1 : -1), 0, nyn - 1);
+        const float c00 = n.p[J * nxn + I], c10 = n.p[J * nxn + I2];
+        const float c01 = n.p[J2 * nxn + I], c11 = n.p[J2 * nxn + I2];
+        const float ccr = K(0.5625) * c00 + K(0.1875) * (c10 + c01) + K(0.0625) * c11;
+        const float qc = c.q[Yi * nxc + Xi];
+        pv[q][sub] = in ? qc + (cc[sub].d > K(0) ? ccr : K(0)) : 0.f;
       }
+      const bool red0 = ((xc0 + rx + Y) & 1) == 0;
+      cR[q] = red0 ? cc[0] : cc[1];
+      cB[q] = red0 ? cc[1] : cc[0];
     }
     // the fine cells: coefficients, rhs and q (their correction waits for c's sweeps)
     LegCell fR[QF], fB[QF];
@@ -587,25 +736,42 @@ struct MgUp2Op {
 #pragma unroll
     for (int q = 0; q < QF; q++) {
       const int pi = tid + NT * q;
-      fR[q] = fB[q] = leg_none(PF);
-      qR[q] = qB[q] = 0.f;
-      if (pi < NPF) {
-        const int ry = pi / (RF / 2), rx = 2 * (pi % (RF / 2));
-        const int y = yf0 + ry;
-        LegCell cc[2];
-        float qv[2];
+      const int pc = pi < NPF ? pi : 0;
+      const int ry = pc / (RF / 2), rx = 2 * (pc % (RF / 2));
+      const int y = yf0 + ry;
+      LegCell cc[2];
+      float qv[2];
 #pragma unroll
-        for (int sub = 0; sub < 2; sub++) {
-          const int x = xf0 + rx + sub;
-          float rr;
-          cc[sub] = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx + sub, ry, RF, RF, PF, rr);
-          qv[sub] = (cc[sub].ci & LEG_IN) ? f.q[y * nx + x] : 0.f;
-        }
-        const bool red0 = ((xf0 + rx + y) & 1) == 0;
-        fR[q] = red0 ? cc[0] : cc[1];
-        fB[q] = red0 ? cc[1] : cc[0];
-        qR[q] = red0 ? qv[0] : qv[1];
-        qB[q] = red0 ? qv[1] : qv[0];
+      for (int sub = 0; sub < 2; sub++) {
+        const int x = xf0 + rx + sub;
+        float rr;
+        cc[sub] = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx + sub, ry, RF, RF, PF, rr);
+        const bool in = (cc[sub].ci & LEG_IN) != 0;
+        const float v = f.q[leg_idx(in, x, y, nx)];
+        qv[sub] = in ? v : 0.f;
+      }
+      const bool red0 = ((xf0 + rx + y) & 1) == 0;
+      fR[q] = red0 ? cc[0] : cc[1];
+      fB[q] = red0 ? cc[1] : cc[0];
+      qR[q] = red0 ? qv[0] : qv[1];
+      qB[q] = red0 ? qv[1] : qv[0];
+    }
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      const int pi = tid + NT * q;
+      if (pi < NPC) {
+        const int ry = pi / (RC / 2), rx = 2 * (pi % (RC / 2));
+        spc[(ry + 1) * PC + rx + 1] = pv[q][0];
+        spc[(ry + 1) * PC + rx + 2] = pv[q][1];
+      } else {
+        cR[q] = cB[q] = leg_none(PC);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QF; q++) {
+      if (tid + NT * q >= NPF) {
+        fR[q] = fB[q] = leg_none(PF);
+        qR[q] = qB[q] = 0.f;
       }
     }
     __syncthreads();
@@ -1257,6 +1423,23 @@ bool launch_mg_up2(const Level& f, const Level& c, const Level& n, int nu, int t
     using Op = decltype(op);
     launch(k_mg_up2_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, n, skip, nb});
   });
+}
+
+bool launch_mg_up_project(const Level& f, const Level& c, const Fields& F, int nu, const int* skip, hipStream_t s) {
+  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+#define CF_UPP_CASE(T, NU)                                                                          \
+  if (t == T && nu == NU) {                                                                         \
+    using Op = MgUpOp<T, T, NU, 0, true>;                                                           \
+    Op::Args a{};                                                                                   \
+    a.f = f; a.c = c; a.st = MgStore{}; a.skip = skip; a.vg = nb; a.F = F;                          \
+    launch(k_mg_up_project<T, T, NU>, dim3(nb), dim3(Op::kThreads), 0, s, a);                       \
+    return true;                                                                                    \
+  }
+  CF_UPP_CASE(64, 1) CF_UPP_CASE(64, 2) CF_UPP_CASE(64, 3)
+  CF_UPP_CASE(32, 1) CF_UPP_CASE(32, 2) CF_UPP_CASE(32, 3)
+  CF_UPP_CASE(16, 1) CF_UPP_CASE(16, 2) CF_UPP_CASE(16, 3)
+#undef CF_UPP_CASE
+  return false;
 }
 
 bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip) {

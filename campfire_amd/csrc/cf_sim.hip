@@ -141,6 +141,8 @@ struct cf_sim {
   // them off, CF_PAIR_TILE=8|16 fixes their tile
   bool leg_pairs = true;
   int pair_tile = 0;
+  // the projection inside the last V-cycle's level-0 up leg (CF_UP_PROJECT=0: a launch of its own)
+  bool up_project = true;
   int mg_bottom_pairs = 0;
   // profiling
   bool profiling = false;
@@ -209,7 +211,14 @@ static bool pair_first(const cf_sim* s, int l) {
 }
 static bool paired(const cf_sim* s, int l) { return pair_first(s, l) || (l >= 2 && pair_first(s, l - 1)); }
 
-static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef) {
+// whether the step's projection runs inside the last V-cycle's level-0 up
+// leg: a fixed cycle count (no residual target, so p, not p64, is projected),
+// FP32 storage, the fused path
+static bool fuse_project(const cf_sim* s) {
+  return !s->mg_simple && s->up_project && s->sch.tol <= 0 && !s->sch.smoother_fp16 && s->mg_bottom >= 1;
+}
+
+static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef, bool project = false) {
   const int L = s->nlev;
   if (!s->mg_simple) {
     // fused: tiled down legs (one or two levels per launch), one LDS-resident
@@ -231,6 +240,14 @@ static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef)
         cf::launch_mg_up2(s->L[l - 1], s->L[l], s->L[l + 1], nu, s->pair_tile, skip, s->st);
         mark(s, level_tag("mg_up2", l - 1));
         l -= 2;
+      } else if (l == 0 && project) {
+        if (cf::launch_mg_up_project(s->L[0], s->L[1], s->F, nu, skip, s->st)) {
+          mark(s, "mg_up_project.0");
+        } else {  // no such instantiation: the leg, then the projection
+          cf::launch_mg_up(s->L[0], s->L[1], mg_store(s, 0), nu, skip, s->st); mark(s, level_tag("mg_up", 0));
+          cf::launch_project(s->F, nullptr, s->st); mark(s, "project");
+        }
+        l -= 1;
       } else {
         cf::launch_mg_up(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_up", l));
         l -= 1;
@@ -280,14 +297,14 @@ static void enqueue_mg_pre(cf_sim* s) {
     mark(s, "dc_defect");
   }
 }
-static void enqueue_mg_cycle(cf_sim* s, int c, bool coarse_coef) {
+static void enqueue_mg_cycle(cf_sim* s, int c, bool coarse_coef, bool project = false) {
   const bool tol = s->sch.tol > 0;
   const bool dc = tol && s->sch.refine;
   const bool kry = dc && s->sch.krylov;
   const int* skip = tol ? &s->dDyn->converged : nullptr;
   __half* ph0 = mg_ph0(s);
   const cf::KryBufs kb{s->dP64, s->dD32, s->dB0};
-  enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0);
+  enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0, project);
   if (kry) {
     cf::launch_kry_step(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st); mark(s, "kry_step");
   } else if (dc) {
@@ -392,8 +409,9 @@ static bool enqueue_step_pre(cf_sim* s) {
 static void enqueue_step_post(cf_sim* s) { enqueue_stage(s, 6); }
 static void enqueue_step(cf_sim* s) {
   const bool coarse_coef = enqueue_step_pre(s);
-  for (int c = 0; c < s->sch.vcycles; c++) enqueue_mg_cycle(s, c, coarse_coef);
-  enqueue_step_post(s);
+  const bool fp = fuse_project(s);
+  for (int c = 0; c < s->sch.vcycles; c++) enqueue_mg_cycle(s, c, coarse_coef, fp && c == s->sch.vcycles - 1);
+  if (!fp) enqueue_step_post(s);
 }
 
 // ---- early exit of the residual-controlled cycles ------------------------------
@@ -1002,6 +1020,8 @@ static int sim_create(int nx, int ny, double dx, double dt, hipStream_t shared, 
   s->mg_simple = ms && *ms && *ms != '0';
   const char* lp = getenv("CF_LEG_PAIRS");
   s->leg_pairs = !(lp && *lp == '0');
+  const char* up = getenv("CF_UP_PROJECT");
+  s->up_project = !(up && *up == '0');
   const char* pt = getenv("CF_PAIR_TILE");
   s->pair_tile = pt && (atoi(pt) == 8 || atoi(pt) == 16) ? atoi(pt) : 0;
 

@@ -1,10 +1,14 @@
-"""The two-level multigrid legs (k_mg_down2 / k_mg_up2: the legs of levels
-1-2, 3-4, ... two per launch) against the one-level legs they replace: every
-field and every level's p bitwise the same after burning steps, for both pair
-tiles, nu = 1, 2, 3, square and oblong grids with partial tiles, and a
-residual-controlled FCG schedule.  The fused V-cycle against the oracle's
-per-dispatch V-cycle is in test_gpu_parity.py; this file pins that the pairs
-change no bit of it."""
+"""Launch fusions of the multigrid against the launches they replace, every
+field and every level's p bitwise the same after burning steps:
+- the two-level legs (k_mg_down2 / k_mg_up2: the legs of levels 1-2, 3-4, ...
+  two per launch) against the one-level legs, for both pair tiles, nu = 1, 2,
+  3, square and oblong grids with partial tiles, and a residual-controlled
+  FCG schedule;
+- the projection inside the last V-cycle's level-0 up leg
+  (k_mg_up_project) against the up leg followed by k_project, for one and two
+  V-cycles per step and a grid whose bottom kernel holds every level.
+The fused V-cycle against the oracle's per-dispatch V-cycle is in
+test_gpu_parity.py; this file pins that the fusions change no bit of it."""
 import os
 import sys
 
@@ -33,11 +37,12 @@ def _scene(g, nx, ny, dx, seed):
     g.encode_geometry()
 
 
-def _run(nx, ny, pairs, tile, sched, steps, seed=3):
+def _run(nx, ny, pairs, tile, sched, steps, seed=3, up_project=True):
     from campfire_amd.fluid import FluidSim
-    old = {k: os.environ.get(k) for k in ("CF_LEG_PAIRS", "CF_PAIR_TILE")}
+    old = {k: os.environ.get(k) for k in ("CF_LEG_PAIRS", "CF_PAIR_TILE", "CF_UP_PROJECT")}
     os.environ["CF_LEG_PAIRS"] = "1" if pairs else "0"
     os.environ["CF_PAIR_TILE"] = str(tile)
+    os.environ["CF_UP_PROJECT"] = "1" if up_project else "0"
     try:
         dx = 1.28 / max(nx, ny)
         g = FluidSim({"nx": nx, "ny": ny, "dt": 0.4 * dx / 2.5}, dx=dx)
@@ -90,3 +95,16 @@ def test_pairs_equal_single_legs_residual_controlled():
     b, _ = _run(512, 512, False, 0, sched, 4)
     for k in b:
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("nx,ny,nu,vcycles", [(1024, 1024, 2, 1), (160, 224, 3, 2), (96, 80, 1, 1), (72, 48, 2, 1)])
+def test_projection_in_the_up_leg_equals_its_own_launch(nx, ny, nu, vcycles):
+    """72x48 has three levels, all in the bottom kernel: no level-0 leg, so
+    the projection keeps its own launch there."""
+    _flush("[fused projection] %dx%d nu=%d, %d V-cycles" % (nx, ny, nu, vcycles))
+    sched = {"nu": nu, "vcycles": vcycles, "coarse_iters": 120}
+    a, _ = _run(nx, ny, True, 0, sched, 5, up_project=True)
+    b, _ = _run(nx, ny, True, 0, sched, 5, up_project=False)
+    for k in b:
+        assert np.isfinite(b[k]).all(), k
+        assert np.array_equal(a[k], b[k]), (k, float(np.max(np.abs(a[k] - b[k]))))
