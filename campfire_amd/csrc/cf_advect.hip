@@ -263,14 +263,69 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   t.dx = P.dx;
   t.idx = K(1) / P.dx;
 
-  // A. inputs as the samplers see them
-  for (int q = tid; q < A_CX * A_CY; q += NT) {
-    const int i = ax0 + q % A_CX, j = ay0 + q / A_CX;
-    sT[q] = fC(F.T, nx, ny, P.Tamb, i, j);
-    sY[q] = fC4(F.Y, nx, ny, aY, i, j);
+  // A. inputs as the samplers see them (fC, fC4, fU, fV).  Every load is
+  // issued before the first store, branch-free (an index outside the field
+  // loads element 0, and the sampler's boundary value replaces it), so the
+  // thread's loads are in flight together instead of one round trip per
+  // element
+  {
+    constexpr int QC = (A_CX * A_CY + NT - 1) / NT, QU = (A_UX * A_UY + NT - 1) / NT,
+                  QV = (A_VX * A_VY + NT - 1) / NT;
+    float tv[QC], uv[QU], vv[QV];
+    float4 yv[QC];
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      const int k = min(tid + NT * q, A_CX * A_CY - 1);
+      const int i = ax0 + k % A_CX, j = ay0 + k / A_CX;
+      const bool in = i >= 0 && i < nx && j < ny;
+      const int idx = in ? max(j, 0) * nx + i : 0;
+      tv[q] = F.T[idx];
+      yv[q] = F.Y[idx];
+    }
+#pragma unroll
+    for (int q = 0; q < QU; q++) {
+      const int k = min(tid + NT * q, A_UX * A_UY - 1);
+      const int i = ax0 + k % A_UX, j = ay0 + k / A_UX;
+      const bool in = i >= 0 && i <= nx && j < ny;
+      uv[q] = F.U[in ? max(j, 0) * (nx + 1) + i : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+      const int k = min(tid + NT * q, A_VX * A_VY - 1);
+      const int i = ax0 + k % A_VX, j = ay0 + k / A_VX;
+      const bool in = i >= 0 && i < nx && j <= ny;
+      vv[q] = F.V[in ? max(j, 0) * nx + i : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < QC; q++) {
+      const int k = tid + NT * q;
+      if (k < A_CX * A_CY) {
+        const int i = ax0 + k % A_CX, j = ay0 + k / A_CX;
+        const bool in = i >= 0 && i < nx && j < ny;
+        sT[k] = in ? tv[q] : P.Tamb;
+        // per component: a select of two float4 values would go through scratch memory
+        sY[k] = make_float4(in ? yv[q].x : aY.x, in ? yv[q].y : aY.y, in ? yv[q].z : aY.z, in ? yv[q].w : aY.w);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QU; q++) {
+      const int k = tid + NT * q;
+      if (k < A_UX * A_UY) {  // outside: 0; below the floor: the no-slip ghost -U(i, 0)
+        const int i = ax0 + k % A_UX, j = ay0 + k / A_UX;
+        const bool in = i >= 0 && i <= nx && j < ny;
+        sU[k] = in ? (j < 0 ? -uv[q] : uv[q]) : K(0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+      const int k = tid + NT * q;
+      if (k < A_VX * A_VY) {
+        const int i = ax0 + k % A_VX, j = ay0 + k / A_VX;
+        const bool in = i >= 0 && i < nx && j <= ny;
+        sV[k] = in ? vv[q] : K(0);
+      }
+    }
   }
-  for (int q = tid; q < A_UX * A_UY; q += NT) sU[q] = fU(F.U, nx, ny, ax0 + q % A_UX, ay0 + q / A_UX);
-  for (int q = tid; q < A_VX * A_VY; q += NT) sV[q] = fV(F.V, nx, ny, ax0 + q % A_VX, ay0 + q / A_VX);
   __syncthreads();
 
   // B. forward pass over the tile + AH, as the corrector's samplers see it.

@@ -640,26 +640,73 @@ struct ForcesCoefOp {
   const int gx = (nx + TX - 1) / TX;
   const int t = vb;
   const int x0 = (t % gx) * TX, y0 = (t / gx) * TY;
-  // ---- stage inputs (cells at clamped indices; faces outside the domain read 0, never used)
-  for (int q = tid; q < FC_R * FC_R; q += 256) {
-    const int a = q % FC_R, b = q / FC_R;
-    const float tv = F.T[iclamp(y0 - 1 + b, 0, ny - 1) * nx + iclamp(x0 - 1 + a, 0, nx - 1)];
-    sT[q] = tv;
-    sMU[q] = muOf(P, tv);
-  }
-  for (int q = tid; q < FC_R * FC_UX; q += 256) {
-    const int a = q % FC_UX, b = q / FC_UX;
-    const int i = x0 - 1 + a, j = y0 - 1 + b;
-    const bool in = i >= 0 && i <= nx && j >= 0 && j < ny;
-    sU[q] = in ? F.U2[j * (nx + 1) + i] : 0.f;
-    sfx[q] = in ? F.fx[j * (nx + 1) + i] : 0.f;
-  }
-  for (int q = tid; q < FC_VY * FC_R; q += 256) {
-    const int a = q % FC_R, b = q / FC_R;
-    const int i = x0 - 1 + a, j = y0 - 1 + b;
-    const bool in = i >= 0 && i < nx && j >= 0 && j <= ny;
-    sV[q] = in ? F.V2[j * nx + i] : 0.f;
-    sfy[q] = in ? F.fy[j * nx + i] : 0.f;
+  // ---- stage inputs (cells at clamped indices; faces outside the domain read
+  // 0, never used).  Every load is issued before the first store, branch-free
+  // (a face outside the domain loads face 0 and stores 0), so a thread's loads
+  // are in flight together; the source term of the tile's cells for rhs0 is
+  // loaded here too
+  constexpr int NTC = FC_R * FC_R, NUC = FC_R * FC_UX, NVC = FC_VY * FC_R;
+  constexpr int QT = (NTC + 255) / 256, QU = (NUC + 255) / 256, QV = (NVC + 255) / 256, QS = TX * TY / 256;
+  static_assert(TX * TY % 256 == 0, "whole rounds of the tile's cells");
+  float sqv[QS];
+  {
+    float tv[QT], uv[QU], fxv[QU], vv[QV], fyv[QV];
+#pragma unroll
+    for (int q = 0; q < QT; q++) {
+      const int k = min(tid + 256 * q, NTC - 1);
+      const int a = k % FC_R, b = k / FC_R;
+      tv[q] = F.T[iclamp(y0 - 1 + b, 0, ny - 1) * nx + iclamp(x0 - 1 + a, 0, nx - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < QU; q++) {
+      const int k = min(tid + 256 * q, NUC - 1);
+      const int i = x0 - 1 + k % FC_UX, j = y0 - 1 + k / FC_UX;
+      const int idx = i >= 0 && i <= nx && j >= 0 && j < ny ? j * (nx + 1) + i : 0;
+      uv[q] = F.U2[idx];
+      fxv[q] = F.fx[idx];
+    }
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+      const int k = min(tid + 256 * q, NVC - 1);
+      const int i = x0 - 1 + k % FC_R, j = y0 - 1 + k / FC_R;
+      const int idx = i >= 0 && i < nx && j >= 0 && j <= ny ? j * nx + i : 0;
+      vv[q] = F.V2[idx];
+      fyv[q] = F.fy[idx];
+    }
+#pragma unroll
+    for (int q = 0; q < QS; q++) {
+      const int k = tid + 256 * q;
+      const int i = x0 + k % TX, j = y0 + k / TX;
+      sqv[q] = F.SQ[i < nx && j < ny ? j * nx + i : 0].x;
+    }
+#pragma unroll
+    for (int q = 0; q < QT; q++) {
+      const int k = tid + 256 * q;
+      if (k < NTC) {
+        sT[k] = tv[q];
+        sMU[k] = muOf(P, tv[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QU; q++) {
+      const int k = tid + 256 * q;
+      if (k < NUC) {
+        const int i = x0 - 1 + k % FC_UX, j = y0 - 1 + k / FC_UX;
+        const bool in = i >= 0 && i <= nx && j >= 0 && j < ny;
+        sU[k] = in ? uv[q] : 0.f;
+        sfx[k] = in ? fxv[q] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+      const int k = tid + 256 * q;
+      if (k < NVC) {
+        const int i = x0 - 1 + k % FC_R, j = y0 - 1 + k / FC_R;
+        const bool in = i >= 0 && i < nx && j >= 0 && j <= ny;
+        sV[k] = in ? vv[q] : 0.f;
+        sfy[k] = in ? fyv[q] : 0.f;
+      }
+    }
   }
   __syncthreads();
   const TileAcc acc{sT, sMU, sU, sV, sfx, sfy, x0, y0};
@@ -698,7 +745,9 @@ struct ForcesCoefOp {
   }
   __syncthreads();
   // ---- rhs0 (:46-54)
-  for (int q = tid; q < TX * TY; q += 256) {
+#pragma unroll
+  for (int r = 0; r < QS; r++) {
+    const int q = tid + 256 * r;
     const int li = q % TX, lj = q / TX;
     const int i = x0 + li, j = y0 + lj;
     if (i < nx && j < ny) {
@@ -706,7 +755,7 @@ struct ForcesCoefOp {
       const float fn = acc.FY(i, j + 1), fs = acc.FY(i, j);
       const float div = (fe * su[lj][li + 1] - fw * su[lj][li] + fn * sv[lj + 1][li] - fs * sv[lj][li]) / P.dx;
       const float open = fe + fw + fn + fs;
-      F.rhs0[j * nx + i] = open > K(0) ? (div - F.SQ[j * nx + i].x) / P.dt : K(0);
+      F.rhs0[j * nx + i] = open > K(0) ? (div - sqv[r]) / P.dt : K(0);
     }
   }
   if (CL.ncascade > 0) {
