@@ -233,6 +233,7 @@ __device__ __forceinline__ float g_corr_v(const Fields& F, const cf_params& P, i
 struct AdvectOp {
   using Args = Fields;
   static constexpr int kThreads = NT;
+  static constexpr int kMinWaves = 4;  // two workgroups per CU, as the LDS allows
   static __device__ __forceinline__ void run(const Fields& F, int tile);
   static int tiles(int nx, int ny) { return ((nx + AX - 1) / AX) * ((ny + AY - 1) / AY); }
 };

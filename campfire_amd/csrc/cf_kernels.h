@@ -3,6 +3,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 #include <string.h>
+#include <type_traits>
 #include <vector>
 #include "cf_device.h"
 
@@ -28,9 +29,21 @@ struct Item {
   int arg, tile;
 };
 
+// the waves per SIMD a batched kernel must fit (Op::kMinWaves, else 1): its
+// register budget, so the batched form keeps the single-scene kernel's
+// occupancy (the argument-block indirection costs the batched form registers)
+template <class Op, class = void>
+struct MinWaves {
+  static constexpr int v = 1;
+};
 template <class Op>
-__global__ __launch_bounds__(Op::kThreads) void k_batch(const typename Op::Args* __restrict__ args,
-                                                       const Item* __restrict__ items, int n) {
+struct MinWaves<Op, std::void_t<decltype(Op::kMinWaves)>> {
+  static constexpr int v = Op::kMinWaves;
+};
+
+template <class Op>
+__global__ __launch_bounds__(Op::kThreads, MinWaves<Op>::v) void k_batch(const typename Op::Args* __restrict__ args,
+                                                                         const Item* __restrict__ items, int n) {
   const Item it = items[xcd_tile(blockIdx.x, n)];
   Op::run(args[it.arg], it.tile);
 }

@@ -342,6 +342,7 @@ struct SourcesOp {
     int vg;  // tiles of this scene's launch
   };
   static constexpr int kThreads = 256;
+  static constexpr int kMinWaves = 6;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Fields& F = a_.F;
     const float* __restrict__ Tin = a_.Tin;
