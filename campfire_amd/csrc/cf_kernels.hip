@@ -120,28 +120,37 @@ __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny,
                                              const float* __restrict__ fy, float dts, int i, int j,
                                              float& To, float4& Yo) {
   const int k = j * nx + i;
+  // Every load is issued first, branch-free (a missing neighbour reads the
+  // cell itself, and its term is not added), so the cell's loads are in
+  // flight together; each term is then added only where the reference adds
+  // it, in its order (west, east, south, north)
+  const bool has[4] = {i > 0, i < nx - 1, j > 0, j < ny - 1};
+  const int kn[4] = {has[0] ? k - 1 : k, has[1] ? k + 1 : k, has[2] ? k - nx : k, has[3] ? k + nx : k};
+  float f[4], Tn[4];
+  float4 Yn[4];
+  f[0] = fx[j * (nx + 1) + i];
+  f[1] = fx[j * (nx + 1) + i + 1];
+  f[2] = fy[j * nx + i];
+  f[3] = fy[(j + 1) * nx + i];
   const float T0 = Tin[k];
   const float4 Y0 = Yin[k];
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    Tn[d] = Tin[kn[d]];
+    Yn[d] = Yin[kn[d]];
+  }
   const float iPr = K(1) / P.Pr;
   float sT = K(0);
   float4 sY = make_float4(0.f, 0.f, 0.f, 0.f);
-#define CF_DIFF_NB(cond, fidx, farr, n)                                       \
-  if (cond) {                                                                 \
-    const float f = farr[fidx];                                               \
-    if (f > K(0)) {                                                           \
-      const float Tn = Tin[n];                                                \
-      const float4 Yn = Yin[n];                                               \
-      const float a = f * rhoDOf(P, K(0.5) * (T0 + Tn), iPr);                 \
-      sT += a * (Tn - T0);                                                    \
-      sY.x += a * (Yn.x - Y0.x); sY.y += a * (Yn.y - Y0.y);                   \
-      sY.z += a * (Yn.z - Y0.z); sY.w += a * (Yn.w - Y0.w);                   \
-    }                                                                         \
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    if (has[d] && f[d] > K(0)) {
+      const float a = f[d] * rhoDOf(P, K(0.5) * (T0 + Tn[d]), iPr);
+      sT += a * (Tn[d] - T0);
+      sY.x += a * (Yn[d].x - Y0.x); sY.y += a * (Yn[d].y - Y0.y);
+      sY.z += a * (Yn[d].z - Y0.z); sY.w += a * (Yn[d].w - Y0.w);
+    }
   }
-  CF_DIFF_NB(i > 0, j * (nx + 1) + i, fx, k - 1)          // west
-  CF_DIFF_NB(i < nx - 1, j * (nx + 1) + i + 1, fx, k + 1) // east
-  CF_DIFF_NB(j > 0, j * nx + i, fy, k - nx)               // south
-  CF_DIFF_NB(j < ny - 1, (j + 1) * nx + i, fy, k + nx)    // north
-#undef CF_DIFF_NB
   const float c = dts / (rhoOf(P, T0) * P.dx * P.dx);
   To = T0 + c * sT;
   Yo = make_float4(fmaxf(Y0.x + c * sY.x * K(1), K(0)), fmaxf(Y0.y + c * sY.y * K(1), K(0)),
