@@ -196,15 +196,16 @@ __global__ __launch_bounds__(256) void k_diffuse(const DiffuseOp::Args a_) {
 // pointwise sources of one cell (shaders/sources.wgsl:41-157)
 // -------------------------------------------------------------------------
 // tamb4 = powf(P.Tamb, 4), evaluated once per workgroup by the caller
+// g: the cell's cellG (sdf, solid id), loaded by the caller with its other
+// loads
 __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F, int i, int j, float Tc, float4 Yc,
-                                             float& Tout, float4& Yout, float2& SQout,
+                                             float2 g, float& Tout, float4& Yout, float2& SQout,
                                              float& qrel, float& qrad, float& tpeak, int* sacc, float tamb4) {
   const int nx = P.nx;
   const int k = j * nx + i;
   const float dx = P.dx, dt = P.dt, cp = P.cp, sO2 = P.sO2, dHc = P.dHc;
   // reciprocals for the correctly rounded quotients of div_rcp
   const float isO2 = K(1) / sO2, icp = K(1) / cp;
-  const float2 g = F.cellG[k];
   const float sdf = g.x;
   // ids come from the geometry pass (piece*4 + face) or are SOLID_NONE;
   // anything else is treated as no solid rather than indexed
@@ -375,6 +376,7 @@ struct SourcesOp {
   float qrel = K(0), qrad = K(0), tpeak = K(0);
   if (i < nx && j < ny) {
     const int k = j * nx + i;
+    const float2 g = F.cellG[k];  // issued with the diffusion's loads
     float Tc;
     float4 Yc;
     if (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, Tc, Yc);
@@ -384,7 +386,7 @@ struct SourcesOp {
     float To;
     float4 Yo;
     float2 sq;
-    sources_cell(P, F, i, j, Tc, Yc, To, Yo, sq, qrel, qrad, tpeak, sacc, tamb4);
+    sources_cell(P, F, i, j, Tc, Yc, g, To, Yo, sq, qrel, qrad, tpeak, sacc, tamb4);
     F.T[k] = To;
     F.Y[k] = Yo;
     F.SQ[k] = sq;
