@@ -196,16 +196,32 @@ __global__ __launch_bounds__(256) void k_diffuse(const DiffuseOp::Args a_) {
 // pointwise sources of one cell (shaders/sources.wgsl:41-157)
 // -------------------------------------------------------------------------
 // tamb4 = powf(P.Tamb, 4), evaluated once per workgroup by the caller
-// g: the cell's cellG (sdf, solid id), loaded by the caller with its other
-// loads
+// What a cell's sources read besides T and Y, loaded by the caller with its
+// other loads: cellG (sdf, solid id), the velocities of its four faces (for
+// the entrainment) and T2 (the temperature after advection, for S)
+struct SourceIn {
+  float2 g;
+  float uw, ue, vs, vn;
+  float t2;
+};
+__device__ __forceinline__ SourceIn source_in(const Fields& F, int nx, int i, int j) {
+  SourceIn r;
+  r.g = F.cellG[j * nx + i];
+  r.uw = F.U[j * (nx + 1) + i];
+  r.ue = F.U[j * (nx + 1) + i + 1];
+  r.vs = F.V[j * nx + i];
+  r.vn = F.V[(j + 1) * nx + i];
+  r.t2 = F.T2[j * nx + i];
+  return r;
+}
+
 __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F, int i, int j, float Tc, float4 Yc,
-                                             float2 g, float& Tout, float4& Yout, float2& SQout,
+                                             const SourceIn& in, float& Tout, float4& Yout, float2& SQout,
                                              float& qrel, float& qrad, float& tpeak, int* sacc, float tamb4) {
-  const int nx = P.nx;
-  const int k = j * nx + i;
   const float dx = P.dx, dt = P.dt, cp = P.cp, sO2 = P.sO2, dHc = P.dHc;
   // reciprocals for the correctly rounded quotients of div_rcp
   const float isO2 = K(1) / sO2, icp = K(1) / cp;
+  const float2 g = in.g;
   const float sdf = g.x;
   // ids come from the geometry pass (piece*4 + face) or are SOLID_NONE;
   // anything else is treated as no solid rather than indexed
@@ -311,8 +327,8 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
   const float dTr = div_rcp(qrad * dt, rc, irc) / (K(1) + div_rcp(K(16.0) * kap * P.sigma * T3 * dt, rc, irc));
   Tc -= dTr;
   // entrainment through the front/back faces (:140-148)
-  const float ucx = K(0.5) * (F.U[j * (nx + 1) + i] + F.U[j * (nx + 1) + i + 1]);
-  const float ucy = K(0.5) * (F.V[j * nx + i] + F.V[(j + 1) * nx + i]);
+  const float ucx = K(0.5) * (in.uw + in.ue);
+  const float ucy = K(0.5) * (in.vs + in.vn);
   const float me = K(2.0) * P.exchAlpha * sqrtf(ucx * ucx + ucy * ucy) * P.rhoAmb / P.depth;
   if (me > K(0)) {
     const float madd = me * dt;
@@ -326,7 +342,7 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
   // write (:150-156)
   Tc = fmaxf(Tc, K(0.5) * P.Tamb);
   Yc = make_float4(fmaxf(Yc.x, K(0)), fmaxf(Yc.y, K(0)), fmaxf(Yc.z, K(0)), fmaxf(Yc.w, K(0)));
-  const float S = massSrc / rhoOf(P, Tc) + logf(Tc / F.T2[k]) / dt;
+  const float S = massSrc / rhoOf(P, Tc) + logf(Tc / in.t2) / dt;
   Tout = Tc;
   Yout = Yc;
   tpeak = Tc;
@@ -352,7 +368,7 @@ struct SourcesOp {
     int vg;  // tiles of this scene's launch
   };
   static constexpr int kThreads = 256;
-  static constexpr int kMinWaves = 6;
+  static constexpr int kMinWaves = 5;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Fields& F = a_.F;
     const float* __restrict__ Tin = a_.Tin;
@@ -376,7 +392,7 @@ struct SourcesOp {
   float qrel = K(0), qrad = K(0), tpeak = K(0);
   if (i < nx && j < ny) {
     const int k = j * nx + i;
-    const float2 g = F.cellG[k];  // issued with the diffusion's loads
+    const SourceIn sin_ = source_in(F, nx, i, j);  // issued with the diffusion's loads
     float Tc;
     float4 Yc;
     if (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, Tc, Yc);
@@ -386,7 +402,7 @@ struct SourcesOp {
     float To;
     float4 Yo;
     float2 sq;
-    sources_cell(P, F, i, j, Tc, Yc, g, To, Yo, sq, qrel, qrad, tpeak, sacc, tamb4);
+    sources_cell(P, F, i, j, Tc, Yc, sin_, To, Yo, sq, qrel, qrad, tpeak, sacc, tamb4);
     F.T[k] = To;
     F.Y[k] = Yo;
     F.SQ[k] = sq;
