@@ -39,10 +39,12 @@ __device__ __forceinline__ float mixv(float a, float b, float t) { return mixf(a
 __device__ __forceinline__ float4 mixv(float4 a, float4 b, float t) { return mix4(a, b, t); }
 
 // A window of an index space held in LDS, element (i, j) at
-// s[(j - oy) * W + (i - ox)].  Reads flag `ok` false when a bilinear
+// s[(j - oy) * W + (i - ox)].  CK: reads flag `ok` false when a bilinear
 // footprint leaves the window; the caller then redoes the point from global
-// memory.
-template <class T, int W, int H>
+// memory.  Without CK the caller has shown that every footprint lies inside
+// (a workgroup whose velocities move no point by a cell), and `ok` is never
+// touched.
+template <class T, int W, int H, bool CK>
 struct Win {
   const T* s;
   int ox, oy;
@@ -52,9 +54,13 @@ struct Win {
   // false for NaN or huge ones, so the conversion needs no clamp first
   __device__ __forceinline__ const T* at(float bx, float by, bool& ok) const {
     const float a = bx - fox, b = by - foy;
-    const bool in = a >= K(0) && a < (float)(W - 1) && b >= K(0) && b < (float)(H - 1);
-    ok = ok && in;
-    return s + (in ? (int)b * W + (int)a : 0);
+    if constexpr (!CK) {
+      return s + (int)b * W + (int)a;
+    } else {
+      const bool in = a >= K(0) && a < (float)(W - 1) && b >= K(0) && b < (float)(H - 1);
+      ok = ok && in;
+      return s + (in ? (int)b * W + (int)a : 0);
+    }
   }
   __device__ __forceinline__ T own(int i, int j) const { return s[(j - oy) * W + (i - ox)]; }
   // bilinear sample at (fx, fy) = grid position minus the field's stagger
@@ -79,15 +85,16 @@ __device__ __forceinline__ float2 mm4v(float a, float c, float d, float e) {
 }
 
 // what a tile's passes read (idx: 1/dx correctly rounded, for div_rcp)
+template <bool CK>
 struct Tile {
-  Win<float, A_CX, A_CY> T;
-  Win<float4, A_CX, A_CY> Y;
-  Win<float, A_UX, A_UY> U;
-  Win<float, A_VX, A_VY> V;
-  Win<float, B_CX, B_CY> Th;
-  Win<float4, B_CX, B_CY> Yh;
-  Win<float, B_UX, B_UY> Uh;
-  Win<float, B_VX, B_VY> Vh;
+  Win<float, A_CX, A_CY, CK> T;
+  Win<float4, A_CX, A_CY, CK> Y;
+  Win<float, A_UX, A_UY, CK> U;
+  Win<float, A_VX, A_VY, CK> V;
+  Win<float, B_CX, B_CY, CK> Th;
+  Win<float4, B_CX, B_CY, CK> Yh;
+  Win<float, B_UX, B_UY, CK> Uh;
+  Win<float, B_VX, B_VY, CK> Vh;
   float dx, idx;
 };
 
@@ -99,7 +106,8 @@ struct Tile {
 // Every B and C point's first footprint lies inside the A windows (their
 // halo is AR, the points are within AH of the tile, the footprint reaches 1
 // more), so `ok` is unchanged by these samples.
-__device__ __forceinline__ float2 trace_rest(const Tile& t, float gx, float gy, float u, float v, float dt, bool& ok) {
+template <class TL>
+__device__ __forceinline__ float2 trace_rest(const TL& t, float gx, float gy, float u, float v, float dt, bool& ok) {
   const float vx = div_trace(u, t.dx, t.idx);
   const float vy = div_trace(v, t.dx, t.idx);
   const float hdt = K(0.5) * dt;
@@ -109,7 +117,8 @@ __device__ __forceinline__ float2 trace_rest(const Tile& t, float gx, float gy, 
   return make_float2(gx - dt * wx, gy - dt * wy);
 }
 // from cell centre (i + 1/2, j + 1/2): U footprint (i, j) with (0.5, 0), V (i, j) with (0, 0.5)
-__device__ __forceinline__ float2 trace_cell(const Tile& t, int i, int j, float dt, bool& ok) {
+template <class TL>
+__device__ __forceinline__ float2 trace_cell(const TL& t, int i, int j, float dt, bool& ok) {
   const float u = mixf(mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0.5)), mixf(t.U.own(i, j + 1), t.U.own(i + 1, j + 1), K(0.5)),
                        K(0));
   const float v = mixf(mixf(t.V.own(i, j), t.V.own(i + 1, j), K(0)), mixf(t.V.own(i, j + 1), t.V.own(i + 1, j + 1), K(0)),
@@ -117,7 +126,8 @@ __device__ __forceinline__ float2 trace_cell(const Tile& t, int i, int j, float 
   return trace_rest(t, (float)i + K(0.5), (float)j + K(0.5), u, v, dt, ok);
 }
 // from U face (i, j + 1/2): U footprint (i, j) with (0, 0), V (i - 1, j) with (0.5, 0.5)
-__device__ __forceinline__ float2 trace_uface(const Tile& t, int i, int j, float dt, bool& ok) {
+template <class TL>
+__device__ __forceinline__ float2 trace_uface(const TL& t, int i, int j, float dt, bool& ok) {
   const float u = mixf(mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0)), mixf(t.U.own(i, j + 1), t.U.own(i + 1, j + 1), K(0)),
                        K(0));
   const float v = mixf(mixf(t.V.own(i - 1, j), t.V.own(i, j), K(0.5)), mixf(t.V.own(i - 1, j + 1), t.V.own(i, j + 1), K(0.5)),
@@ -125,7 +135,8 @@ __device__ __forceinline__ float2 trace_uface(const Tile& t, int i, int j, float
   return trace_rest(t, (float)i, (float)j + K(0.5), u, v, dt, ok);
 }
 // from V face (i + 1/2, j): U footprint (i, j - 1) with (0.5, 0.5), V (i, j) with (0, 0)
-__device__ __forceinline__ float2 trace_vface(const Tile& t, int i, int j, float dt, bool& ok) {
+template <class TL>
+__device__ __forceinline__ float2 trace_vface(const TL& t, int i, int j, float dt, bool& ok) {
   const float u = mixf(mixf(t.U.own(i, j - 1), t.U.own(i + 1, j - 1), K(0.5)), mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0.5)),
                        K(0.5));
   const float v = mixf(mixf(t.V.own(i, j), t.V.own(i + 1, j), K(0)), mixf(t.V.own(i, j + 1), t.V.own(i + 1, j + 1), K(0)),
@@ -255,14 +266,12 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   const float dt = P.dt;
   const float4 aY = make_float4(0.f, P.YOamb, 0.f, 0.f);
   const int ax0 = x0 - AR, ay0 = y0 - AR, bx0 = x0 - AH, by0 = y0 - AH;
-  Tile t;
-  const float fax0 = (float)ax0, fay0 = (float)ay0, fbx0 = (float)bx0, fby0 = (float)by0;
-  t.T = {sT, ax0, ay0, fax0, fay0}; t.Y = {sY, ax0, ay0, fax0, fay0};
-  t.U = {sU, ax0, ay0, fax0, fay0}; t.V = {sV, ax0, ay0, fax0, fay0};
-  t.Th = {bT, bx0, by0, fbx0, fby0}; t.Yh = {bY, bx0, by0, fbx0, fby0};
-  t.Uh = {bU, bx0, by0, fbx0, fby0}; t.Vh = {bV, bx0, by0, fbx0, fby0};
-  t.dx = P.dx;
-  t.idx = K(1) / P.dx;
+  // a staged velocity below vlim moves no trace of this workgroup by a cell
+  // (below), so every bilinear footprint of its passes lies in the windows
+  // (grids below 2^16 cells a side keep the rounding of a position under
+  // 2^-8 cell)
+  const float vlim = K(0.99) * P.dx / fabsf(dt);
+  bool fastv = nx <= 65536 && ny <= 65536;  // false: also a staged |U| or |V| >= vlim, or NaN
 
   // A. inputs as the samplers see them (fC, fC4, fU, fV).  Every load is
   // issued before the first store, branch-free (an index outside the field
@@ -314,7 +323,9 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       if (k < A_UX * A_UY) {  // outside: 0; below the floor: the no-slip ghost -U(i, 0)
         const int i = ax0 + k % A_UX, j = ay0 + k / A_UX;
         const bool in = i >= 0 && i <= nx && j < ny;
-        sU[k] = in ? (j < 0 ? -uv[q] : uv[q]) : K(0);
+        const float su = in ? (j < 0 ? -uv[q] : uv[q]) : K(0);
+        sU[k] = su;
+        fastv = fastv && fabsf(su) < vlim;
       }
     }
 #pragma unroll
@@ -323,222 +334,257 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       if (k < A_VX * A_VY) {
         const int i = ax0 + k % A_VX, j = ay0 + k / A_VX;
         const bool in = i >= 0 && i < nx && j <= ny;
-        sV[k] = in ? vv[q] : K(0);
+        const float sv = in ? vv[q] : K(0);
+        sV[k] = sv;
+        fastv = fastv && fabsf(sv) < vlim;
       }
     }
+  }
+  // every wave writes its own slot, so the slots need no clearing
+  __shared__ int sSlow[NT / 64];
+  {
+    const unsigned long long slow = __ballot(!fastv);
+    if ((tid & 63) == 0) sSlow[tid >> 6] = slow != 0ull;
   }
   __syncthreads();
+  int slow = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) slow |= sSlow[w];
 
-  // B. forward pass over the tile + AH, as the corrector's samplers see it.
-  // The corrector's departure trace of a cell or face (+dt) is the forward
-  // pass's trace of the same point, so each thread first does the forward
-  // pass of the points it will correct in C and keeps their traces in
-  // registers (B1); the rest of the window follows (B2).
-  const float sO2i = K(1) / P.sO2;
-  const bool ex = x0 + AX == nx, ey = y0 + AY == ny;
-  constexpr int NQ = AX * AY + AY + AX;  // C's work items: the tile's cells (+ faces), then edge faces
-  constexpr int NS = (NQ + NT - 1) / NT;
-  // the points C corrects: item q -> (i, j) and which of its cell / U / V it owns
-  auto item = [&](int q, int& i, int& j, bool& oc, bool& ou, bool& ov) {
-    const bool tp = q < AX * AY, eu = q >= AX * AY && q < AX * AY + AY, ev = q >= AX * AY + AY && q < NQ;
-    i = tp ? x0 + q % AX : (eu ? nx : x0 + (q - AX * AY - AY));
-    j = tp ? y0 + q / AX : (eu ? y0 + (q - AX * AY) : ny);
-    oc = tp && i < nx && j < ny;
-    ou = (tp && i <= nx && j < ny) || (eu && ex && j < ny);
-    ov = (tp && i < nx && j <= ny) || (ev && ey && i < nx);
-  };
-  // forward values of one point (i, j >= 0 inside the domain); d, okd: its trace
-  auto fwd_cell = [&](int i, int j, float& th, float4& yh, float2& d, bool& okd) {
-    okd = true;
-    d = trace_cell(t, i, j, dt, okd);
-    bool ok = okd;
-    th = t.T.bil(d.x - K(0.5), d.y - K(0.5), ok);
-    yh = t.Y.bil(d.x - K(0.5), d.y - K(0.5), ok);
-    if (!ok) g_fwd_cell(F, P, i, j, th, yh);
-  };
-  auto fwd_u = [&](int i, int j, float2& d, bool& okd) {
-    okd = true;
-    d = trace_uface(t, i, j, dt, okd);
-    bool ok = okd;
-    float u = t.U.bil(d.x - K(0), d.y - K(0.5), ok);
-    if (!ok) u = g_fwd_u(F, P, i, j);
-    return u;
-  };
-  auto fwd_v = [&](int i, int j, float2& d, bool& okd) {
-    okd = true;
-    d = trace_vface(t, i, j, dt, okd);
-    bool ok = okd;
-    float v = t.V.bil(d.x - K(0.5), d.y - K(0), ok);
-    if (!ok) v = g_fwd_v(F, P, i, j);
-    return v;
-  };
-  // the traces of one work item's cell and faces, kept from B1 for C
-  struct Slot {
-    float2 dC, dU, dV;
-    bool okC, okU, okV;
-  };
-  static_assert(NS == 3 && AX * AY == 2 * NT && AX + AY <= NT, "two tile slots and one edge slot per thread");
-  Slot S0, S1;
-  // B1: the tile rectangle's points of work items tid and tid + NT: the
-  // forward pass of the points C corrects (traces kept), the outside values
-  // for the rest (beyond the domain's right or top edge)
-  auto b1 = [&](int q, Slot& S) {
-    int i, j;
-    bool oc, ou, ov;
-    item(q, i, j, oc, ou, ov);
-    S.dC = S.dU = S.dV = make_float2(0.f, 0.f);
-    S.okC = S.okU = S.okV = false;
-    float th = P.Tamb;
-    float4 yh = aY;
-    if (oc) fwd_cell(i, j, th, yh, S.dC, S.okC);
-    const int qb = (j - by0) * B_CX + (i - bx0);
-    bT[qb] = th;
-    bY[qb] = yh;
-    bU[(j - by0) * B_UX + (i - bx0)] = ou ? fwd_u(i, j, S.dU, S.okU) : K(0);
-    bV[(j - by0) * B_VX + (i - bx0)] = ov ? fwd_v(i, j, S.dV, S.okV) : K(0);
-  };
-  b1(tid, S0);
-  b1(tid + NT, S1);
-  // B2: the ring of the window around the tile rectangle, enumerated without
-  // gaps: r -> window (a, b) for a W x H window whose rectangle starts at
-  // (AH, AH) with size AX x AY
-  auto ring = [&](int r, int W, int& a, int& b) {
-    const int top = AH * W, mid = AY * (W - AX);
-    if (r < top) { a = r % W; b = r / W; }
-    else if (r < top + mid) {
-      const int c = (r - top) % (W - AX);
-      a = c < AH ? c : c + AX;
-      b = AH + (r - top) / (W - AX);
-    } else { a = (r - top - mid) % W; b = AH + AY + (r - top - mid) / W; }
-  };
-  for (int r = tid; r < B_CX * B_CY - AX * AY; r += NT) {
-    int a, b;
-    ring(r, B_CX, a, b);
-    const int i = bx0 + a, j = by0 + b;
-    float th = P.Tamb;
-    float4 yh = aY;
-    if (!(i < 0 || i >= nx || j >= ny)) {
-      float2 d;
-      bool okd;
-      fwd_cell(i, max(j, 0), th, yh, d, okd);
-    }
-    bT[b * B_CX + a] = th;
-    bY[b * B_CX + a] = yh;
-  }
-  for (int r = tid; r < B_UX * B_UY - AX * AY; r += NT) {
-    int a, b;
-    ring(r, B_UX, a, b);
-    const int i = bx0 + a, j = by0 + b;
-    float u = K(0);
-    if (!(i < 0 || i > nx || j >= ny)) {
-      float2 d;
-      bool okd;
-      u = fwd_u(i, max(j, 0), d, okd);
-      if (j < 0) u = -u;
-    }
-    bU[b * B_UX + a] = u;
-  }
-  for (int r = tid; r < B_VX * B_VY - AX * AY; r += NT) {
-    int a, b;
-    ring(r, B_VX, a, b);
-    const int i = bx0 + a, j = by0 + b;
-    float v = K(0);
-    if (!(i < 0 || i >= nx || j > ny)) {
-      float2 d;
-      bool okd;
-      v = fwd_v(i, max(j, 0), d, okd);
-    }
-    bV[b * B_VX + a] = v;
-  }
-  __syncthreads();
+  // The displacement bound.  A trace's velocity samples interpolate staged
+  // values, so |u|, |v| < vlim, and the trace moves its point by at most
+  // dt*vlim/dx = 0.99 cell (plus a few ulps of rounding, far below the 1 %
+  // margin), in x and in y.  With the window halos (AR = 3 staged, AH = 1
+  // forward) a point of B lies within a cell of the tile and its footprints
+  // within 1 + 1 + 1 cells; a point of C lies in the tile and its backward
+  // footprint within 1 + 1 cells of it, inside the forward window.  So the
+  // workgroup runs B and C without window tests (CK = false), and a
+  // workgroup with a faster velocity (or a NaN) runs them tested, with the
+  // global-memory redo.  Both forms do the same arithmetic on the same values.
+  auto passes = [&](auto ck) {
+    constexpr bool CK = decltype(ck)::value;
+    Tile<CK> t;
+    const float fax0 = (float)ax0, fay0 = (float)ay0, fbx0 = (float)bx0, fby0 = (float)by0;
+    t.T = {sT, ax0, ay0, fax0, fay0}; t.Y = {sY, ax0, ay0, fax0, fay0};
+    t.U = {sU, ax0, ay0, fax0, fay0}; t.V = {sV, ax0, ay0, fax0, fay0};
+    t.Th = {bT, bx0, by0, fbx0, fby0}; t.Yh = {bY, bx0, by0, fbx0, fby0};
+    t.Uh = {bU, bx0, by0, fbx0, fby0}; t.Vh = {bV, bx0, by0, fbx0, fby0};
+    t.dx = P.dx;
+    t.idx = K(1) / P.dx;
 
-  // C. correction of the tile's cells and faces; the tile at the right (top)
-  // domain edge also owns the U (V) faces on that edge
-  auto c1 = [&](int q, const Slot& S) {
-    int i, j;
-    bool oc, ou, ov;
-    item(q, i, j, oc, ou, ov);
-    if (oc) {  // scalCorrect (:49-66)
-      const int k = j * nx + i;
-      bool ok = S.okC;
-      const float2 d = S.dC;
-      const float2 f = trace_cell(t, i, j, -dt, ok);
-      const float* pT = t.T.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
-      const float4* pY = t.Y.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
-      const float sbT = t.Th.bil(f.x - K(0.5), f.y - K(0.5), ok);
-      const float4 sb = t.Yh.bil(f.x - K(0.5), f.y - K(0.5), ok);
-      float To;
-      float4 Yo;
-      if (ok) {
-        const float2 mT = mm4v(pT[0], pT[1], pT[A_CX], pT[A_CX + 1]);
-        const float4 a = pY[0], c = pY[1], dd = pY[A_CX], e = pY[A_CX + 1];
-        const float4 lo = make_float4(fminf(fminf(a.x, c.x), fminf(dd.x, e.x)), fminf(fminf(a.y, c.y), fminf(dd.y, e.y)),
-                                      fminf(fminf(a.z, c.z), fminf(dd.z, e.z)), fminf(fminf(a.w, c.w), fminf(dd.w, e.w)));
-        const float4 hi = make_float4(fmaxf(fmaxf(a.x, c.x), fmaxf(dd.x, e.x)), fmaxf(fmaxf(a.y, c.y), fmaxf(dd.y, e.y)),
-                                      fmaxf(fmaxf(a.z, c.z), fmaxf(dd.z, e.z)), fmaxf(fmaxf(a.w, c.w), fmaxf(dd.w, e.w)));
-        const float Thk = t.Th.own(i, j);
-        const float4 Yhk = t.Yh.own(i, j), Yk = t.Y.own(i, j);
-        const float Tc = Thk + K(0.5) * (t.T.own(i, j) - sbT);
-        const float4 Yc = make_float4(Yhk.x + K(0.5) * (Yk.x - sb.x), Yhk.y + K(0.5) * (Yk.y - sb.y),
-                                      Yhk.z + K(0.5) * (Yk.z - sb.z), Yhk.w + K(0.5) * (Yk.w - sb.w));
-        const bool inT = Tc >= mT.x && Tc <= mT.y;
-        const bool inY = Yc.x >= lo.x && Yc.y >= lo.y && Yc.z >= lo.z && Yc.w >= lo.w &&
-                         Yc.x <= hi.x && Yc.y <= hi.y && Yc.z <= hi.z && Yc.w <= hi.w;
-        const float mixC = fminf(Yc.x, div_rcp(Yc.y, P.sO2, sO2i));
-        const float mixL = fminf(Yhk.x, div_rcp(Yhk.y, P.sO2, sO2i));
-        const bool take = inT && inY && mixC <= mixL + K(1e-5);
-        To = take ? Tc : Thk;
-        Yo = take ? Yc : Yhk;
-      } else {
-        g_corr_cell(F, P, i, j, To, Yo);
+    // B. forward pass over the tile + AH, as the corrector's samplers see it.
+    // The corrector's departure trace of a cell or face (+dt) is the forward
+    // pass's trace of the same point, so each thread first does the forward
+    // pass of the points it will correct in C and keeps their traces in
+    // registers (B1); the rest of the window follows (B2).
+    const float sO2i = K(1) / P.sO2;
+    const bool ex = x0 + AX == nx, ey = y0 + AY == ny;
+    constexpr int NQ = AX * AY + AY + AX;  // C's work items: the tile's cells (+ faces), then edge faces
+    constexpr int NS = (NQ + NT - 1) / NT;
+    // the points C corrects: item q -> (i, j) and which of its cell / U / V it owns
+    auto item = [&](int q, int& i, int& j, bool& oc, bool& ou, bool& ov) {
+      const bool tp = q < AX * AY, eu = q >= AX * AY && q < AX * AY + AY, ev = q >= AX * AY + AY && q < NQ;
+      i = tp ? x0 + q % AX : (eu ? nx : x0 + (q - AX * AY - AY));
+      j = tp ? y0 + q / AX : (eu ? y0 + (q - AX * AY) : ny);
+      oc = tp && i < nx && j < ny;
+      ou = (tp && i <= nx && j < ny) || (eu && ex && j < ny);
+      ov = (tp && i < nx && j <= ny) || (ev && ey && i < nx);
+    };
+    // forward values of one point (i, j >= 0 inside the domain); d, okd: its trace
+    auto fwd_cell = [&](int i, int j, float& th, float4& yh, float2& d, bool& okd) {
+      okd = true;
+      d = trace_cell(t, i, j, dt, okd);
+      bool ok = okd;
+      th = t.T.bil(d.x - K(0.5), d.y - K(0.5), ok);
+      yh = t.Y.bil(d.x - K(0.5), d.y - K(0.5), ok);
+      if (!ok) g_fwd_cell(F, P, i, j, th, yh);
+    };
+    auto fwd_u = [&](int i, int j, float2& d, bool& okd) {
+      okd = true;
+      d = trace_uface(t, i, j, dt, okd);
+      bool ok = okd;
+      float u = t.U.bil(d.x - K(0), d.y - K(0.5), ok);
+      if (!ok) u = g_fwd_u(F, P, i, j);
+      return u;
+    };
+    auto fwd_v = [&](int i, int j, float2& d, bool& okd) {
+      okd = true;
+      d = trace_vface(t, i, j, dt, okd);
+      bool ok = okd;
+      float v = t.V.bil(d.x - K(0.5), d.y - K(0), ok);
+      if (!ok) v = g_fwd_v(F, P, i, j);
+      return v;
+    };
+    // the traces of one work item's cell and faces, kept from B1 for C
+    struct Slot {
+      float2 dC, dU, dV;
+      bool okC, okU, okV;
+    };
+    static_assert(NS == 3 && AX * AY == 2 * NT && AX + AY <= NT, "two tile slots and one edge slot per thread");
+    Slot S0, S1;
+    // B1: the tile rectangle's points of work items tid and tid + NT: the
+    // forward pass of the points C corrects (traces kept), the outside values
+    // for the rest (beyond the domain's right or top edge)
+    auto b1 = [&](int q, Slot& S) {
+      int i, j;
+      bool oc, ou, ov;
+      item(q, i, j, oc, ou, ov);
+      S.dC = S.dU = S.dV = make_float2(0.f, 0.f);
+      S.okC = S.okU = S.okV = false;
+      float th = P.Tamb;
+      float4 yh = aY;
+      if (oc) fwd_cell(i, j, th, yh, S.dC, S.okC);
+      const int qb = (j - by0) * B_CX + (i - bx0);
+      bT[qb] = th;
+      bY[qb] = yh;
+      bU[(j - by0) * B_UX + (i - bx0)] = ou ? fwd_u(i, j, S.dU, S.okU) : K(0);
+      bV[(j - by0) * B_VX + (i - bx0)] = ov ? fwd_v(i, j, S.dV, S.okV) : K(0);
+    };
+    b1(tid, S0);
+    b1(tid + NT, S1);
+    // B2: the ring of the window around the tile rectangle, enumerated without
+    // gaps: r -> window (a, b) for a W x H window whose rectangle starts at
+    // (AH, AH) with size AX x AY
+    auto ring = [&](int r, int W, int& a, int& b) {
+      const int top = AH * W, mid = AY * (W - AX);
+      if (r < top) { a = r % W; b = r / W; }
+      else if (r < top + mid) {
+        const int c = (r - top) % (W - AX);
+        a = c < AH ? c : c + AX;
+        b = AH + (r - top) / (W - AX);
+      } else { a = (r - top - mid) % W; b = AH + AY + (r - top - mid) / W; }
+    };
+    for (int r = tid; r < B_CX * B_CY - AX * AY; r += NT) {
+      int a, b;
+      ring(r, B_CX, a, b);
+      const int i = bx0 + a, j = by0 + b;
+      float th = P.Tamb;
+      float4 yh = aY;
+      if (!(i < 0 || i >= nx || j >= ny)) {
+        float2 d;
+        bool okd;
+        fwd_cell(i, max(j, 0), th, yh, d, okd);
       }
-      F.T2[k] = To;
-      F.Y2[k] = Yo;
+      bT[b * B_CX + a] = th;
+      bY[b * B_CX + a] = yh;
     }
-    if (ou) {  // velCorrect, U (:78-90)
-      bool ok = S.okU;
-      const float2 b = trace_uface(t, i, j, -dt, ok);
-      const float back = t.Uh.bil(b.x - K(0), b.y - K(0.5), ok);
-      const float2 d = S.dU;
-      const float* pU = t.U.at(floorf(d.x - K(0)), floorf(d.y - K(0.5)), ok);
-      float u;
-      if (ok) {
-        const float2 mm = mm4v(pU[0], pU[1], pU[A_UX], pU[A_UX + 1]);
-        u = clampf(t.Uh.own(i, j) + K(0.5) * (t.U.own(i, j) - back), mm.x, mm.y);
-      } else {
-        u = g_corr_u(F, P, i, j);
+    for (int r = tid; r < B_UX * B_UY - AX * AY; r += NT) {
+      int a, b;
+      ring(r, B_UX, a, b);
+      const int i = bx0 + a, j = by0 + b;
+      float u = K(0);
+      if (!(i < 0 || i > nx || j >= ny)) {
+        float2 d;
+        bool okd;
+        u = fwd_u(i, max(j, 0), d, okd);
+        if (j < 0) u = -u;
       }
-      F.U2[j * (nx + 1) + i] = u;
+      bU[b * B_UX + a] = u;
     }
-    if (ov) {  // velCorrect, V (:91-103)
-      bool ok = S.okV;
-      const float2 b = trace_vface(t, i, j, -dt, ok);
-      const float back = t.Vh.bil(b.x - K(0.5), b.y - K(0), ok);
-      const float2 d = S.dV;
-      const float* pV = t.V.at(floorf(d.x - K(0.5)), floorf(d.y - K(0)), ok);
-      float v;
-      if (ok) {
-        const float2 mm = mm4v(pV[0], pV[1], pV[A_VX], pV[A_VX + 1]);
-        v = clampf(t.Vh.own(i, j) + K(0.5) * (t.V.own(i, j) - back), mm.x, mm.y);
-      } else {
-        v = g_corr_v(F, P, i, j);
+    for (int r = tid; r < B_VX * B_VY - AX * AY; r += NT) {
+      int a, b;
+      ring(r, B_VX, a, b);
+      const int i = bx0 + a, j = by0 + b;
+      float v = K(0);
+      if (!(i < 0 || i >= nx || j > ny)) {
+        float2 d;
+        bool okd;
+        v = fwd_v(i, max(j, 0), d, okd);
       }
-      F.V2[j * nx + i] = v;
+      bV[b * B_VX + a] = v;
+    }
+    __syncthreads();
+
+    // C. correction of the tile's cells and faces; the tile at the right (top)
+    // domain edge also owns the U (V) faces on that edge
+    auto c1 = [&](int q, const Slot& S) {
+      int i, j;
+      bool oc, ou, ov;
+      item(q, i, j, oc, ou, ov);
+      if (oc) {  // scalCorrect (:49-66)
+        const int k = j * nx + i;
+        bool ok = CK ? S.okC : true;
+        const float2 d = S.dC;
+        const float2 f = trace_cell(t, i, j, -dt, ok);
+        const float* pT = t.T.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
+        const float4* pY = t.Y.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
+        const float sbT = t.Th.bil(f.x - K(0.5), f.y - K(0.5), ok);
+        const float4 sb = t.Yh.bil(f.x - K(0.5), f.y - K(0.5), ok);
+        float To;
+        float4 Yo;
+        if (ok) {
+          const float2 mT = mm4v(pT[0], pT[1], pT[A_CX], pT[A_CX + 1]);
+          const float4 a = pY[0], c = pY[1], dd = pY[A_CX], e = pY[A_CX + 1];
+          const float4 lo = make_float4(fminf(fminf(a.x, c.x), fminf(dd.x, e.x)), fminf(fminf(a.y, c.y), fminf(dd.y, e.y)),
+                                        fminf(fminf(a.z, c.z), fminf(dd.z, e.z)), fminf(fminf(a.w, c.w), fminf(dd.w, e.w)));
+          const float4 hi = make_float4(fmaxf(fmaxf(a.x, c.x), fmaxf(dd.x, e.x)), fmaxf(fmaxf(a.y, c.y), fmaxf(dd.y, e.y)),
+                                        fmaxf(fmaxf(a.z, c.z), fmaxf(dd.z, e.z)), fmaxf(fmaxf(a.w, c.w), fmaxf(dd.w, e.w)));
+          const float Thk = t.Th.own(i, j);
+          const float4 Yhk = t.Yh.own(i, j), Yk = t.Y.own(i, j);
+          const float Tc = Thk + K(0.5) * (t.T.own(i, j) - sbT);
+          const float4 Yc = make_float4(Yhk.x + K(0.5) * (Yk.x - sb.x), Yhk.y + K(0.5) * (Yk.y - sb.y),
+                                        Yhk.z + K(0.5) * (Yk.z - sb.z), Yhk.w + K(0.5) * (Yk.w - sb.w));
+          const bool inT = Tc >= mT.x && Tc <= mT.y;
+          const bool inY = Yc.x >= lo.x && Yc.y >= lo.y && Yc.z >= lo.z && Yc.w >= lo.w &&
+                           Yc.x <= hi.x && Yc.y <= hi.y && Yc.z <= hi.z && Yc.w <= hi.w;
+          const float mixC = fminf(Yc.x, div_rcp(Yc.y, P.sO2, sO2i));
+          const float mixL = fminf(Yhk.x, div_rcp(Yhk.y, P.sO2, sO2i));
+          const bool take = inT && inY && mixC <= mixL + K(1e-5);
+          To = take ? Tc : Thk;
+          Yo = take ? Yc : Yhk;
+        } else {
+          g_corr_cell(F, P, i, j, To, Yo);
+        }
+        F.T2[k] = To;
+        F.Y2[k] = Yo;
+      }
+      if (ou) {  // velCorrect, U (:78-90)
+        bool ok = CK ? S.okU : true;
+        const float2 b = trace_uface(t, i, j, -dt, ok);
+        const float back = t.Uh.bil(b.x - K(0), b.y - K(0.5), ok);
+        const float2 d = S.dU;
+        const float* pU = t.U.at(floorf(d.x - K(0)), floorf(d.y - K(0.5)), ok);
+        float u;
+        if (ok) {
+          const float2 mm = mm4v(pU[0], pU[1], pU[A_UX], pU[A_UX + 1]);
+          u = clampf(t.Uh.own(i, j) + K(0.5) * (t.U.own(i, j) - back), mm.x, mm.y);
+        } else {
+          u = g_corr_u(F, P, i, j);
+        }
+        F.U2[j * (nx + 1) + i] = u;
+      }
+      if (ov) {  // velCorrect, V (:91-103)
+        bool ok = CK ? S.okV : true;
+        const float2 b = trace_vface(t, i, j, -dt, ok);
+        const float back = t.Vh.bil(b.x - K(0.5), b.y - K(0), ok);
+        const float2 d = S.dV;
+        const float* pV = t.V.at(floorf(d.x - K(0.5)), floorf(d.y - K(0)), ok);
+        float v;
+        if (ok) {
+          const float2 mm = mm4v(pV[0], pV[1], pV[A_VX], pV[A_VX + 1]);
+          v = clampf(t.Vh.own(i, j) + K(0.5) * (t.V.own(i, j) - back), mm.x, mm.y);
+        } else {
+          v = g_corr_v(F, P, i, j);
+        }
+        F.V2[j * nx + i] = v;
+      }
+    };
+    c1(tid, S0);
+    c1(tid + NT, S1);
+    {  // the domain-edge faces: traced here
+      int i, j;
+      bool oc, ou, ov;
+      item(tid + 2 * NT, i, j, oc, ou, ov);
+      Slot S2;
+      S2.dC = S2.dU = S2.dV = make_float2(0.f, 0.f);
+      S2.okC = S2.okU = S2.okV = false;
+      if (ou) { S2.okU = true; S2.dU = trace_uface(t, i, j, dt, S2.okU); }
+      if (ov) { S2.okV = true; S2.dV = trace_vface(t, i, j, dt, S2.okV); }
+      c1(tid + 2 * NT, S2);
     }
   };
-  c1(tid, S0);
-  c1(tid + NT, S1);
-  {  // the domain-edge faces: traced here
-    int i, j;
-    bool oc, ou, ov;
-    item(tid + 2 * NT, i, j, oc, ou, ov);
-    Slot S2;
-    S2.dC = S2.dU = S2.dV = make_float2(0.f, 0.f);
-    S2.okC = S2.okU = S2.okV = false;
-    if (ou) { S2.okU = true; S2.dU = trace_uface(t, i, j, dt, S2.okU); }
-    if (ov) { S2.okV = true; S2.dV = trace_vface(t, i, j, dt, S2.okV); }
-    c1(tid + 2 * NT, S2);
-  }
+  if (slow) passes(std::true_type{});
+  else passes(std::false_type{});
 }
 
 __global__ __launch_bounds__(NT) void k_advect(Fields F) { AdvectOp::run(F, xcd_tile(blockIdx.x, gridDim.x)); }

@@ -112,7 +112,10 @@ def test_single_pass_advect_equals_dispatch_pairs(nx, ny, dx, dt):
     """The single-pass advection kernel (LDS windows, global-memory redo of
     footprints that leave them) and the reference's two dispatch pairs give
     bitwise the same T2, Y2, U2, V2 on the same inputs: ragged tiles (136x72),
-    displacements of 0.2 to 6 cells."""
+    displacements of 0.2 to 6 cells.  Below a cell (dt = 0.5 ms and 1 ms, and
+    the slower tiles at 2 ms) a workgroup reads its windows untested; at 2 ms
+    both forms run in one launch, and at 8 ms nearly every tile takes the
+    tested reads and the redo."""
     from campfire_amd import scenes
     from campfire_amd.fluid import FluidSim
     _flush("[parity] single-pass advect vs dispatch pairs", nx, ny, dt)
