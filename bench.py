@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-steps/s at 1024² reacting step; batched scenes/s + p99 step latency"
+PLAIN_CAP = 400  # V-cycle cap of the plain per-scene FP32 form of the equal-accuracy comparison
 HBM_PEAK_GBS = 8000.0
 
 
@@ -263,7 +264,8 @@ def converged_batch(args):
     ||r||/||rhs|| < tol (capped at `cap` cycles per step) in
       (a) the plain per-scene FP32 form: one FluidSim per scene, stepped one
           scene after another, FP32 V-cycles with the reference smoother
-          settings (nu 2, 120 coarsest sweeps);
+          settings (nu 2, 120 coarsest sweeps), capped at PLAIN_CAP cycles,
+          which it never reaches;
       (b) the batched mixed-precision form: one ScenePool stepping every
           scene per launch, FP16 smoother with FP64 defect correction, nu 3
           and 24 coarsest sweeps;
@@ -313,7 +315,7 @@ def converged_batch(args):
     dcb, tdc = batched({"nu": 3, "coarse_iters": 24})
     kry, tkr = batched({"nu": args.fcg_nu, "coarse_iters": args.fcg_coarse, "krylov": 1})
 
-    def per_scene(sched, label):
+    def per_scene(sched, label, cap=cap):
         # one FluidSim per scene, stepped one scene after another, FP32 smoother
         sims = []
         for c in cfgs:
@@ -336,21 +338,24 @@ def converged_batch(args):
                 ca.append(int(sim.info().last_vcycles))
         for sim in sims:
             sim.close()
-        return {"mcell_steps_per_s": round(cells * K / ts / 1e6, 1), "schedule": label,
+        return {"mcell_steps_per_s": round(cells * K / ts / 1e6, 1), "schedule": label, "cycle_cap": cap,
                 "resid_max": max(ra), "cycles_mean": round(float(np.mean(ca)), 2),
-                "steps_at_cap": int(sum(1 for c in ca if c >= cap))}, ts
+                "cycles_max": int(max(ca)), "steps_at_cap": int(sum(1 for c in ca if c >= cap))}, ts
 
     # (a) the same solver as the batched FCG form, per scene and in FP32
     same, tsame = per_scene({"nu": args.fcg_nu, "coarse_iters": args.fcg_coarse, "refine": 1, "krylov": 1},
                             "nu %d, %d-sweep coarsest, FP32 smoother, FP64 flexible CG" % (args.fcg_nu, args.fcg_coarse))
-    # (b) the reference's smoother settings with plain FP32 V-cycles
-    plain, tplain = per_scene({"nu": 2, "coarse_iters": 120}, "nu 2, 120 coarsest sweeps, FP32 V-cycles")
+    # (b) the reference's smoother settings with plain FP32 V-cycles; a few
+    # scene-steps need more than 40 of them, so this form gets a cap it never
+    # reaches and meets the tolerance in every scene-step, as the others do
+    plain, tplain = per_scene({"nu": 2, "coarse_iters": 120}, "nu 2, 120 coarsest sweeps, FP32 V-cycles",
+                              cap=PLAIN_CAP)
     res = {"tol": tol, "cycle_cap": cap, "steps": K, "scenes": len(cfgs), "cells": cells,
            "batched_mixed_fcg": kry, "batched_mixed": dcb,
            "per_scene_fp32_fcg": same, "per_scene_fp32": plain,
            # batching + the FP16 smoother alone: the same solver on both sides
            "speedup": round(tsame / tkr, 3),
-           # against the plain per-scene FP32 V-cycles (which miss the target in some scene-steps)
+           # against the plain per-scene FP32 V-cycles, the reference's solver, at the same tolerance
            "speedup_vs_plain_vcycles": round(tplain / tkr, 3),
            "speedup_defect_correction_vs_plain": round(tplain / tdc, 3)}
     for name, r in (("batched mixed + FCG", kry), ("batched mixed", dcb), ("per-scene FP32 + FCG", same),

@@ -295,6 +295,7 @@ void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s);
 // rule on the level's size; a batched phase holds one tile size
 int mg_tile(const Level& f);
 int mg_tile_rule(const Level& f);
+int mg_tile_for_count(int n64, int n32);
 // first level of the single-workgroup bottom solver (the last 1-3 levels)
 int mg_bottom_first(const Level* L, int nlev);
 // coarse_coef: restrict the coarsest level's coefficients inside the kernel
@@ -388,8 +389,11 @@ void phase_geometry(BatchPhase& ph, const Fields& F);
 void phase_gas_terms(BatchPhase& ph, const Fields& F);
 void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets);
 void phase_fill_u32(BatchPhase& ph, void* a, uint32_t v, size_t n);
-bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip);
-bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip);
+// tile: the legs' tile edge (16, 32, 64), as the pool chose it for this level
+bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
+                   int tile);
+bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
+                 int tile);
 bool phase_mg_bottom(BatchPhase& ph, const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters,
                      const int* skip, double* band);
 

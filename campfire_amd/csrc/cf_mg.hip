@@ -1347,8 +1347,14 @@ static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
 // four times the workgroups, each with half the per-thread chain, and its
 // legs are a shorter latency chain
 int mg_tile_rule(const Level& f) {
-  if (ntiles(f.nx, f.ny, 64, 64) >= 256) return 64;
-  return ntiles(f.nx, f.ny, 32, 32) >= 1024 ? 32 : 16;
+  return mg_tile_for_count((int)ntiles(f.nx, f.ny, 64, 64), (int)ntiles(f.nx, f.ny, 32, 32));
+}
+// the same rule on the tile counts of a launch (n64 tiles of 64x64, n32 of
+// 32x32), which for a pool's phase sum over its scenes: many small levels
+// together fill the device with large tiles, whose halo costs less
+int mg_tile_for_count(int n64, int n32) {
+  if (n64 >= 256) return 64;
+  return n32 >= 1024 ? 32 : 16;
 }
 // a tile chosen per level (the autotuner's) overrides the rule; every tile
 // gives the same results, the legs being exact at any tile size
@@ -1442,16 +1448,18 @@ bool launch_mg_up_project(const Level& f, const Level& c, const Fields& F, int n
   return false;
 }
 
-bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip) {
-  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
+                   int t) {
+  const int nb = (int)ntiles(f.nx, f.ny, t, t);
   return mg_dispatch<MgDownOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
     phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(Op::kThreads), "mg_down");
   });
 }
 
-bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip) {
-  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
+                 int t) {
+  const int nb = (int)ntiles(f.nx, f.ny, t, t);
   return mg_dispatch<MgUpOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
     phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(Op::kThreads), "mg_up");

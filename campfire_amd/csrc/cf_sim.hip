@@ -722,18 +722,31 @@ static int pool_build_step(cf_pool* pl) {
   }
   int maxb = 0;
   for (auto* s : sc) maxb = s->mg_bottom > maxb ? s->mg_bottom : maxb;
-  // a leg's kernel depends on its tile size (from the level's size) and on
+  // a leg's kernel depends on its tile size (chosen below per depth) and on
   // the storage of its two levels (FP16 smoother: whether the coarse level is
   // tiled too), so each level's legs form one phase per (tile, storage)
   const int modes[3] = {0, 1, 3};
+  // the tile of each scene's tiled level d: the level's own (cf_sim_set_leg_tiles)
+  // where one is set, else the size rule applied to the tile counts of every
+  // such level at depth d together (the launch the pool makes of them)
+  std::vector<std::array<int, MAXLEV>> ltile(sc.size());
+  for (int d = 0; d < maxb; d++) {
+    int n64 = 0, n32 = 0;
+    for (auto* s : sc)
+      if (d < s->mg_bottom && !s->L[d].tile) {
+        n64 += ((s->L[d].nx + 63) / 64) * ((s->L[d].ny + 63) / 64);
+        n32 += ((s->L[d].nx + 31) / 32) * ((s->L[d].ny + 31) / 32);
+      }
+    const int pooled = cf::mg_tile_for_count(n64, n32);
+    for (size_t q = 0; q < sc.size(); q++)
+      ltile[q][d] = d < sc[q]->mg_bottom ? (sc[q]->L[d].tile ? sc[q]->L[d].tile : pooled) : 0;
+  }
   std::vector<int> tiles;  // every tile size a scene's level uses
-  for (auto* s : sc)
-    for (int d = 0; d < s->mg_bottom; d++) {
-      const int t = cf::mg_tile(s->L[d]);
-      if (std::find(tiles.begin(), tiles.end(), t) == tiles.end()) tiles.push_back(t);
-    }
-  auto leg_ok = [&](cf_sim* s, int d, int t, int m) {
-    return d < s->mg_bottom && cf::mg_tile(s->L[d]) == t && mg_store(s, d).mode == m;
+  for (size_t q = 0; q < sc.size(); q++)
+    for (int d = 0; d < sc[q]->mg_bottom; d++)
+      if (std::find(tiles.begin(), tiles.end(), ltile[q][d]) == tiles.end()) tiles.push_back(ltile[q][d]);
+  auto leg_ok = [&](size_t q, int d, int t, int m) {
+    return d < sc[q]->mg_bottom && ltile[q][d] == t && mg_store(sc[q], d).mode == m;
   };
   pl->cyc.clear();
   for (int c = 0; c < k.vcycles; c++) {
@@ -742,9 +755,11 @@ static int pool_build_step(cf_pool* pl) {
       for (int t : tiles)
         for (int m : modes) {
           cf::BatchPhase a;
-          for (auto* s : sc)
-            if (leg_ok(s, d, t, m))
-              cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+          for (size_t q = 0; q < sc.size(); q++)
+            if (leg_ok(q, d, t, m)) {
+              cf_sim* s = sc[q];
+              cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr, t);
+            }
           if (!a.empty()) ph.push_back(a);
         }
     }
@@ -761,9 +776,11 @@ static int pool_build_step(cf_pool* pl) {
       for (int t : tiles)
         for (int m : modes) {
           cf::BatchPhase a;
-          for (auto* s : sc)
-            if (leg_ok(s, d, t, m))
-              cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr);
+          for (size_t q = 0; q < sc.size(); q++)
+            if (leg_ok(q, d, t, m)) {
+              cf_sim* s = sc[q];
+              cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr, t);
+            }
           if (!a.empty()) ph.push_back(a);
         }
     }
