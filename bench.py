@@ -205,6 +205,17 @@ def measure_traffic(args):
     return (per, cal), None
 
 
+def pool_tiles(args, pool, views):
+    """The pool's leg tile per depth: the autotuner's pick from a snapshot of
+    the scenes as they are (left as they were), or the size rule with
+    --no-tune."""
+    if not args.no_tune:
+        from campfire_amd.autotune import tune_pool_leg_tiles
+        tune_pool_leg_tiles(pool, views, log=log)
+        pool.synchronize()
+    return pool.leg_tiles()
+
+
 def batched(args):
     """The mixed-size batch: args.batch scenes (sizes drawn from 256^2, 512^2,
     1024^2) stepped together by one ScenePool, against the same scenes stepped
@@ -222,6 +233,7 @@ def batched(args):
     pool.frame(8, True)          # geometry and a first burning frame
     pool.step(args.warmup)
     pool.synchronize()
+    tiles = pool_tiles(args, pool, sims)
     info = pool.info()
     cells = int(info.cells)
     K = args.batch_steps
@@ -252,7 +264,7 @@ def batched(args):
            "speedup_vs_per_scene": round(ts / tb, 3),
            "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                                "p99": round(float(np.percentile(lat, 99)), 4)},
-           "launches_per_step": int(info.launches_per_step)}
+           "launches_per_step": int(info.launches_per_step), "leg_tiles": tiles}
     log("[bench] batch: %.1f Mcell-steps/s batched, %.1f per scene (x%.2f); %.1f scene-steps/s; p99 %.3f ms"
         % (res["batched_mcell_steps_per_s"], res["per_scene_mcell_steps_per_s"], res["speedup_vs_per_scene"],
            res["batched_scene_steps_per_s"], res["step_latency_ms"]["p99"]))
@@ -293,6 +305,7 @@ def converged_batch(args):
         pool.frame(8, True)
         pool.step(2)
         pool.synchronize()
+        tiles = pool_tiles(args, pool, views)
         t, rb, cb = 0.0, [], []
         for q in range(K):
             t0 = time.perf_counter()
@@ -308,7 +321,7 @@ def converged_batch(args):
                             % (sched["nu"], "%d-sweep" % sched["coarse_iters"] if sched["coarse_iters"] else "direct",
                                "flexible CG" if sched.get("krylov") else "defect correction"),
                 "resid_max": max(rb), "cycles_mean": round(float(np.mean(cb)), 2),
-                "steps_at_cap": int(sum(1 for c in cb if c >= cap)),
+                "steps_at_cap": int(sum(1 for c in cb if c >= cap)), "leg_tiles": tiles,
                 "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                                     "p99": round(float(np.percentile(lat, 99)), 4)}}, t
 

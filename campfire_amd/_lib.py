@@ -127,6 +127,8 @@ def lib():
         "cf_pool_create": ([C.c_uint64, C.POINTER(p)], i),
         "cf_pool_destroy": ([p], None),
         "cf_pool_set_schedule": ([p, C.POINTER(CfSchedule)], i),
+        "cf_pool_set_leg_tiles": ([p, vp, i], i),
+        "cf_pool_get_leg_tiles": ([p, vp, i], i),
         "cf_pool_admit": ([p, C.POINTER(CfSceneDesc), C.POINTER(i)], i),
         "cf_pool_scene": ([p, i, C.POINTER(p)], i),
         "cf_pool_release": ([p, i], i),

@@ -3,10 +3,11 @@ the launch shape (tile edge) of the tiled multigrid legs per level.
 
 The leg tiles do not change results (a tile with a halo of 2 nu + 1 cells
 reproduces the global red-black half-sweeps exactly), only speed, which
-depends on the level's size and the device:
+depends on the level's size, the scenes launched with it and the device:
 
     tiles, table = tune_leg_tiles(1024, 1024, 0.00125, 0.00025)
     sim.set_leg_tiles(tiles)
+    tiles, table = tune_pool_leg_tiles(pool, views)   # a pool's depths, in place
 
 Schedule autotuner:
 
@@ -103,6 +104,57 @@ def _median_step_ms(sim, twin, steps):
     sim.step(1)                       # captures the step graph of the current tiles
     sim.copy_state_from(twin)
     return float(np.median(sim.step_latencies(steps)))
+
+
+def tune_pool_leg_tiles(pool, views, steps=10, log=None):
+    """The leg tile edge of each depth of a pool's tiled levels, for the
+    pool's scenes and schedule as they are: depth by depth from the finest,
+    each candidate is timed over `steps` pool steps from the same device
+    snapshot of every scene (median step time), and a change is kept only
+    when it beats the best by more than 1 %.  The scenes are left in their
+    snapshot state and the pool with the step of the chosen tiles built (and
+    run once, so its first replay is not in the caller's timing).  Returns (tiles,
+    [(tiles tried, ms per step)])."""
+    log = log or (lambda *a: None)
+    twins = []
+    for v in views:
+        t = FluidSim({"nx": v.nx, "ny": v.ny, "dt": v.dt}, dx=v.dx)
+        t.copy_state_from(v)
+        twins.append(t)
+
+    def timed(tiles):
+        pool.set_leg_tiles(tiles)
+        for v, t in zip(views, twins):
+            v.copy_state_from(t)
+        pool.step(1)                  # rebuilds and captures the batched step
+        for v, t in zip(views, twins):
+            v.copy_state_from(t)
+        ms = float(np.median(pool.step_latencies(steps)))
+        return ms
+
+    pool.set_leg_tiles([])
+    tiles = pool.leg_tiles()
+    best_ms = timed(tiles)
+    table = [(list(tiles), best_ms)]
+    log("[autotune] pool leg tiles %s (size rule on each depth's tile count): %.3f ms/step" % (tiles, best_ms))
+    for d in range(len(tiles)):
+        for t in TILES:
+            if t == tiles[d]:
+                continue
+            trial = list(tiles)
+            trial[d] = t
+            ms = timed(trial)
+            table.append((trial, ms))
+            log("[autotune] pool leg tiles %s: %.3f ms/step" % (trial, ms))
+            if ms < 0.99 * best_ms:
+                best_ms, tiles = ms, trial
+    pool.set_leg_tiles(tiles)
+    pool.step(1)                      # the step of the chosen tiles, built, captured and run once
+    pool.synchronize()
+    for v, t in zip(views, twins):
+        v.copy_state_from(t)
+        t.close()
+    return tiles, table
 
 
 def tune_leg_tiles(nx, ny, dx, dt, steps=20, warm=16, seed=1, log=None):

@@ -599,7 +599,31 @@ struct cf_pool {
   // bumped by every admit and release: a new scene may reuse a released one's
   // host address, so the table keys include it besides the slot pointers
   uint64_t gen = 0;
+  int tile_over[MAXLEV] = {};  // cf_pool_set_leg_tiles: the leg tile of each depth (0: chosen below)
 };
+
+// The leg tile of each depth d of the tiled levels of the scenes `sc`:
+// the pool's own (cf_pool_set_leg_tiles) where one is set, else the size
+// rule applied to the tile counts of all those levels together (the launch
+// the pool makes of them), whose many small levels fill the device with
+// large tiles.  Returns the number of depths.
+static int pool_depth_tiles(const cf_pool* pl, const std::vector<cf_sim*>& sc, int (&tile)[MAXLEV]) {
+  int maxb = 0;
+  for (auto* s : sc) maxb = s->mg_bottom > maxb ? s->mg_bottom : maxb;
+  for (int d = 0; d < MAXLEV; d++) {
+    tile[d] = 0;
+    if (d >= maxb) continue;
+    if (pl->tile_over[d]) { tile[d] = pl->tile_over[d]; continue; }
+    int n64 = 0, n32 = 0;
+    for (auto* s : sc)
+      if (d < s->mg_bottom) {
+        n64 += ((s->L[d].nx + 63) / 64) * ((s->L[d].ny + 63) / 64);
+        n32 += ((s->L[d].nx + 31) / 32) * ((s->L[d].ny + 31) / 32);
+      }
+    tile[d] = cf::mg_tile_for_count(n64, n32);
+  }
+  return maxb;
+}
 
 // ---- scene pool: many scenes stepped together ---------------------------------
 // Every phase of a step is one launch over the tiles of all live scenes
@@ -726,21 +750,12 @@ static int pool_build_step(cf_pool* pl) {
   // the storage of its two levels (FP16 smoother: whether the coarse level is
   // tiled too), so each level's legs form one phase per (tile, storage)
   const int modes[3] = {0, 1, 3};
-  // the tile of each scene's tiled level d: the level's own (cf_sim_set_leg_tiles)
-  // where one is set, else the size rule applied to the tile counts of every
-  // such level at depth d together (the launch the pool makes of them)
+  // the tile of each scene's tiled level d: the depth's (pool_depth_tiles)
+  int dtile[MAXLEV];
+  pool_depth_tiles(pl, sc, dtile);
   std::vector<std::array<int, MAXLEV>> ltile(sc.size());
-  for (int d = 0; d < maxb; d++) {
-    int n64 = 0, n32 = 0;
-    for (auto* s : sc)
-      if (d < s->mg_bottom && !s->L[d].tile) {
-        n64 += ((s->L[d].nx + 63) / 64) * ((s->L[d].ny + 63) / 64);
-        n32 += ((s->L[d].nx + 31) / 32) * ((s->L[d].ny + 31) / 32);
-      }
-    const int pooled = cf::mg_tile_for_count(n64, n32);
-    for (size_t q = 0; q < sc.size(); q++)
-      ltile[q][d] = d < sc[q]->mg_bottom ? (sc[q]->L[d].tile ? sc[q]->L[d].tile : pooled) : 0;
-  }
+  for (size_t q = 0; q < sc.size(); q++)
+    for (int d = 0; d < MAXLEV; d++) ltile[q][d] = d < sc[q]->mg_bottom ? dtile[d] : 0;
   std::vector<int> tiles;  // every tile size a scene's level uses
   for (size_t q = 0; q < sc.size(); q++)
     for (int d = 0; d < sc[q]->mg_bottom; d++)
@@ -1710,6 +1725,29 @@ int cf_pool_set_schedule(cf_pool* pl, const cf_schedule* sc) {
   pl->sch = *sc;
   pl->dirty = true;
   return CF_OK;
+}
+
+int cf_pool_set_leg_tiles(cf_pool* pl, const int32_t* tiles, int n) {
+  if (!pl || (n > 0 && !tiles) || n < 0) return set_err(CF_EINVAL, "bad argument");
+  if (n > MAXLEV) return set_err(CF_EINVAL, "%d tile entries for at most %d depths", n, MAXLEV);
+  for (int d = 0; d < n; d++)
+    if (tiles[d] != 0 && tiles[d] != 16 && tiles[d] != 32 && tiles[d] != 64)
+      return set_err(CF_EINVAL, "leg tile %d of depth %d is not 0, 16, 32 or 64", tiles[d], d);
+  for (int d = 0; d < MAXLEV; d++) {
+    const int t = d < n ? tiles[d] : 0;
+    if (pl->tile_over[d] != t) { pl->tile_over[d] = t; pl->dirty = true; }
+  }
+  return CF_OK;
+}
+
+int cf_pool_get_leg_tiles(const cf_pool* pl, int32_t* tiles, int n) {
+  if (!pl || (n > 0 && !tiles) || n < 0) return set_err(CF_EINVAL, "bad argument");
+  std::vector<cf_sim*> sc;
+  for (auto* s : pl->slots) if (s) sc.push_back(s);
+  int t[MAXLEV];
+  const int maxb = pool_depth_tiles(pl, sc, t);
+  for (int d = 0; d < n; d++) tiles[d] = d < MAXLEV ? t[d] : 0;
+  return maxb;
 }
 
 int cf_pool_admit(cf_pool* pl, const cf_scene_desc* d, int* slot) {

@@ -56,6 +56,19 @@ class ScenePool:
         s = L.CfSchedule(nu, vcycles, tol, coarse_iters, smoother_fp16, cfl, umin, refine, krylov)
         L.check(L.lib().cf_pool_set_schedule(self._h, C.byref(s)), "pool schedule")
 
+    def set_leg_tiles(self, tiles):
+        """Leg tile edge (16, 32, 64; 0 = the size rule on the depth's tile
+        count) of each depth of the tiled multigrid levels; speed only."""
+        a = np.asarray(tiles, np.int32)
+        L.check(L.lib().cf_pool_set_leg_tiles(self._h, a.ctypes.data, int(a.size)), "pool set_leg_tiles")
+
+    def leg_tiles(self):
+        """The tile of each tiled depth for the live scenes."""
+        a = np.zeros(8, np.int32)
+        n = L.lib().cf_pool_get_leg_tiles(self._h, a.ctypes.data, int(a.size))
+        L.check(0 if n >= 0 else n, "pool leg_tiles")
+        return [int(x) for x in a[:n]]
+
     def admit(self, nx, ny, dx, dt):
         """(slot, FluidSim view), or None when the scene does not fit the budget."""
         d = L.CfSceneDesc(int(nx), int(ny), float(dx), float(dt))

@@ -281,6 +281,14 @@ void cf_pool_destroy(cf_pool* pool);
 /* The schedule of every scene (cf_sim_set_schedule rules).  Every scene is
  * checked before any changes: on an error no scene's schedule has changed. */
 int  cf_pool_set_schedule(cf_pool* pool, const cf_schedule* sc);
+/* Leg tile edge (16, 32 or 64) of each depth of the tiled multigrid levels,
+ * for the legs of every scene at that depth; 0 (the default) takes the size
+ * rule applied to the tile count of the whole depth.  Results do not depend
+ * on tiles, only speed.  Takes effect at the next rebuild of the step. */
+int  cf_pool_set_leg_tiles(cf_pool* pool, const int32_t* tiles, int n);
+/* The tile of each depth for the live scenes (n entries, 0 beyond the
+ * deepest tiled level); returns the number of tiled depths. */
+int  cf_pool_get_leg_tiles(const cf_pool* pool, int32_t* tiles, int n);
 /* A new scene in the first free slot; CF_EAGAIN when it does not fit the budget. */
 int  cf_pool_admit(cf_pool* pool, const cf_scene_desc* desc, int* slot);
 /* The scene of a slot (owned by the pool: do not cf_sim_destroy it). */

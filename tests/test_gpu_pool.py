@@ -43,7 +43,7 @@ SPECS = [(96, 80, 0.004, 0.0016, 3, 3), (256, 256, 0.003, 0.0006, 4, 6), (160, 2
          (512, 512, 0.0025, 0.0005, 12, 5), (1024, 1024, 0.00125, 0.00025, 13, 8)]
 
 
-def _compare_with_alone(schedule, frames, steps):
+def _compare_with_alone(schedule, frames, steps, tiles=None):
     from campfire_amd.fluid import FluidSim
     from campfire_amd.pool import ScenePool
     pool = ScenePool(8 << 30)
@@ -54,6 +54,11 @@ def _compare_with_alone(schedule, frames, steps):
         slot, sim = pool.admit(nx, ny, dx, dt)
         _setup(sim, nx, ny, dx, seed, npc)
         views.append(sim)
+    if tiles is not None:
+        pool.set_leg_tiles(tiles)
+        got = pool.leg_tiles()
+        _flush("[pool] leg tiles per depth %s" % got)
+        assert got == list(tiles)[:len(got)]
     for f in range(frames):
         pool.frame(steps, True)
     pool.synchronize()
@@ -96,6 +101,15 @@ def test_batched_equals_alone_fp16_smoother_schedule():
 def test_batched_equals_alone_krylov_schedule():
     _compare_with_alone({"nu": 3, "vcycles": 30, "tol": 1e-6, "coarse_iters": 24, "cfl": 0.5, "umin": 1.0,
                          "refine": 1, "smoother_fp16": 1, "krylov": 1}, frames=1, steps=4)
+
+
+def test_pool_leg_tiles_change_no_result():
+    """Tiles forced per depth instead of the size rule's: every scene still
+    equals the scene alone, bitwise, under the reference schedule and the
+    FP16-smoother FCG schedule (whose legs have three storage modes)."""
+    _compare_with_alone(None, frames=1, steps=8, tiles=[16, 32, 16, 64])
+    _compare_with_alone({"nu": 3, "vcycles": 30, "tol": 1e-6, "coarse_iters": 24, "cfl": 0.5, "umin": 1.0,
+                         "refine": 1, "smoother_fp16": 1, "krylov": 1}, frames=1, steps=4, tiles=[32, 64, 16, 32])
 
 
 def test_admission_follows_budget():
