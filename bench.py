@@ -205,6 +205,63 @@ def measure_traffic(args):
     return (per, cal), None
 
 
+def step_bytes(n_sub=1, nu=2, vcycles=1):
+    """Algorithmic HBM bytes of one gas step per cell (FP32, each array read
+    and written once per global-synchronisation phase, faces as cells):
+    advect forward 60 + advect correct 84 + sources 68 + forces 28 + coef and
+    rhs0 40 + coefficient restriction over the levels 13.3 + project 32, then
+    48 per diffusion substep and (40 nu + 35) * 4/3 per V-cycle (4/3: the
+    levels' geometric sum).  526.7 B at the reference schedule."""
+    return 325.3 + 48.0 * n_sub + vcycles * (40.0 * nu + 35.0) * 4.0 / 3.0
+
+
+def batched_1024(args, n):
+    """n scenes of the 1024^2 workload (the same pile and disc, seeds
+    args.seed .. args.seed + n - 1) stepped together by one ScenePool at the
+    reference schedule: Mcell-steps/s, the pool step's latency, and the
+    step's rate in algorithmic bytes (step_bytes) against the 8 TB/s HBM
+    peak.  The n scenes' state (about 150 MB each) is larger than the
+    256 MB Infinity Cache."""
+    from campfire_amd import scenes
+    from campfire_amd.pool import ScenePool
+    pool = ScenePool(200 << 30)
+    views = []
+    for q in range(n):
+        a2 = argparse.Namespace(**vars(args))
+        a2.seed = args.seed + q
+        c = bench_config(a2)
+        slot, sim = pool.admit(c["nx"], c["ny"], c["dx"], c["dt"])
+        scenes.apply(sim, c)
+        views.append(sim)
+    for f in range(2):
+        pool.frame(8, True)
+    pool.step(args.warmup)
+    pool.synchronize()
+    tiles = pool_tiles(args, pool, views)
+    info = pool.info()
+    cells = int(info.cells)
+    K = args.batch_steps
+    t0 = time.perf_counter()
+    pool.step(K)
+    pool.synchronize()
+    tb = time.perf_counter() - t0
+    lat = pool.step_latencies(K)
+    pool.close()
+    rate = cells * K / tb
+    gbs = rate * step_bytes() / 1e9
+    res = {"scenes": n, "grid": [1024, 1024], "cells": cells, "hbm_bytes": int(info.used), "steps": K,
+           "mcell_steps_per_s": round(rate / 1e6, 1), "ms_per_step": round(tb * 1e3 / K, 4),
+           "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                               "p99": round(float(np.percentile(lat, 99)), 4)},
+           "launches_per_step": int(info.launches_per_step), "leg_tiles": tiles,
+           "step_bytes_per_cell": round(step_bytes(), 1), "achieved_gbs": round(gbs, 1),
+           "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    log("[bench] batch of %d x 1024^2: %.1f Mcell-steps/s, %.3f ms per pool step (p99 %.3f), %.0f GB/s at %.1f B per "
+        "cell-step = %.3f of the HBM peak" % (n, res["mcell_steps_per_s"], res["ms_per_step"],
+                                             res["step_latency_ms"]["p99"], gbs, step_bytes(), gbs / HBM_PEAK_GBS))
+    return res
+
+
 def pool_tiles(args, pool, views):
     """The pool's leg tile per depth: the autotuner's pick from a snapshot of
     the scenes as they are (left as they were), or the size rule with
@@ -591,6 +648,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs")
     ap.add_argument("--batch", type=int, default=64, help="scenes of the batched measurement (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=40)
+    ap.add_argument("--batch1024", type=int, default=16, help="1024^2 scenes of the one-size batch (0: skip)")
     ap.add_argument("--fcg-nu", type=int, default=3, help="smoother sweeps of the batched FCG form")
     ap.add_argument("--fcg-coarse", type=int, default=24, help="coarsest sweeps of the batched FCG form (0: direct)")
     ap.add_argument("--conv-steps", type=int, default=6,
@@ -723,6 +781,10 @@ def main():
                    "grid": [nx, ny], "levels": len(levels), "diff_sub": sim.diff_sub, "seed": args.seed,
                    "leg_tiles": sim.leg_tiles()},
         "roofline": roof,
+        # the whole step at the per-cell-step byte model, against the same peak
+        "step_roofline": {"bytes_per_cell": round(step_bytes(sim.diff_sub), 1),
+                          "achieved": round(value * 1e6 * step_bytes(sim.diff_sub) / 1e9, 1),
+                          "frac": round(value * 1e6 * step_bytes(sim.diff_sub) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels_ms_per_step": {r[0]: round(r[1], 5) for r in rows},
         "kernels_gbs": {r[0]: round(r[4], 1) for r in rows if r[4]},
         "profiled_ms_per_step": round(stage_sum, 5),
@@ -739,6 +801,8 @@ def main():
     sim.close()
     if args.plain_steps > 0:
         out["plain_256"] = plain_256(args, args.plain_steps)
+    if args.batch1024 > 0:
+        out["batched_1024"] = batched_1024(args, args.batch1024)
     if args.batch > 0:
         out["batched"] = batched(args)
         if args.conv_steps > 0:
