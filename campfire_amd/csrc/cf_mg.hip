@@ -99,13 +99,42 @@ __device__ __forceinline__ LegCell leg_none(int PX) {
   return c;
 }
 
-// the red-black update of shaders/pressure.wgsl:87-94 (terms in its order)
-template <int PX>
-__device__ __forceinline__ void leg_relax(const LegCell& c, float* sp) {
-  const int ci = c.ci & LEG_IDX;
-  const float off = c.w * sp[ci - 1] + c.e * sp[ci + 1] + c.s * sp[ci - PX] + c.n * sp[ci + PX];
-  const float v = c.d > K(0) ? div_rcp_fast(off - c.rh, c.d, c.y) : K(0);
-  if (c.ci & LEG_UPD) sp[ci] = v;
+// A region's LDS array holds LEG_DUMMY words past its cells: lane l of a
+// wave stores to word l there when its slot's cell is not updated.
+constexpr int LEG_DUMMY = 64;
+__device__ __forceinline__ float* leg_dummy(float* sp, int n) { return sp + n + (threadIdx.x & 63); }
+
+// One half-sweep of a thread's cells of one colour, the red-black update of
+// shaders/pressure.wgsl:87-94 (terms in its order).  The LDS reads of G
+// cells at a time are issued before their arithmetic (G = Q: all of them;
+// fewer where registers are short), and every update is stored without a
+// branch: to its cell where the cell is updated, else to the lane's dummy
+// word.  (The cells of one colour never neighbour each other, so no read of
+// this half-sweep sees a store of it.)  A slot without a cell (leg_none)
+// reads inside the region and stores to the dummy.
+template <int PX, int G = 0, int Q>
+__device__ __forceinline__ void leg_sweep(const LegCell (&c)[Q], float* sp, float* dummy) {
+  constexpr int NG = G > 0 && G < Q ? G : Q;
+#pragma unroll
+  for (int g = 0; g < Q; g += NG) {
+    float a0[NG], a1[NG], a2[NG], a3[NG];
+#pragma unroll
+    for (int q = 0; q < NG && g + q < Q; q++) {
+      const int ci = c[g + q].ci & LEG_IDX;
+      a0[q] = sp[ci - 1];
+      a1[q] = sp[ci + 1];
+      a2[q] = sp[ci - PX];
+      a3[q] = sp[ci + PX];
+    }
+#pragma unroll
+    for (int q = 0; q < NG && g + q < Q; q++) {
+      const LegCell& x = c[g + q];
+      const float off = x.w * a0[q] + x.e * a1[q] + x.s * a2[q] + x.n * a3[q];
+      const float v = div_rcp_fast(off - x.rh, x.d, x.y);
+      float* dst = (x.ci & LEG_UPD) ? sp + (x.ci & LEG_IDX) : dummy;
+      *dst = x.d > K(0) ? v : K(0);
+    }
+  }
 }
 
 // p of a level: FP32, or FP16 holding p * sig (read back as h * isig, exact)
@@ -142,7 +171,7 @@ struct MgDownOp {
     const Level& f = a_.f;
     const Level& c = a_.c;
     if (a_.skip && *a_.skip) return;  // the step's residual target is met
-    __shared__ float sp[PX * PY];
+    __shared__ float sp[PX * PY + LEG_DUMMY];
     __shared__ float sres[TX * TY];  // the interior's rhs, then its residual
     constexpr bool FH = (M & 1) != 0;
     const int nx = f.nx;
@@ -193,11 +222,9 @@ struct MgDownOp {
     }
     __syncthreads();
     for (int s = 0; s < NU; s++) {  // red then black (js/fluid.js:298-301)
-#pragma unroll
-      for (int q = 0; q < PQ; q++) leg_relax<PX>(cR[q], sp);
+      leg_sweep<PX>(cR, sp, leg_dummy(sp, PX * PY));
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < PQ; q++) leg_relax<PX>(cB[q], sp);
+      leg_sweep<PX>(cB, sp, leg_dummy(sp, PX * PY));
       __syncthreads();
     }
     // interior: p into the leg buffer, residual into sres (shaders/pressure.wgsl:96-101)
@@ -259,12 +286,16 @@ struct MgUpOp {
   static constexpr int H = 2 * NU + (PROJ ? 1 : 0);
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, PX = RX + 2, PY = RY + 2, NP = RX * RY / 2;
   static constexpr int PQ = (NP + kThreads - 1) / kThreads;
+  // cells whose sweep reads are in flight together: all, but one at a time in
+  // the projecting leg, which holds the faces' velocities and fractions in
+  // registers through the sweeps (more would spill)
+  static constexpr int SG = PROJ ? 1 : 0;
   static_assert(RX % 2 == 0, "pairs need an even region width");
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Level& f = a_.f;
     const Level& c = a_.c;
     if (a_.skip && *a_.skip) return;
-    __shared__ float sp[PX * PY];
+    __shared__ float sp[PX * PY + LEG_DUMMY];
     constexpr bool FH = (M & 1) != 0, CH = (M & 2) != 0;
     const int nx = f.nx;
     const int gx = (nx + TX - 1) / TX;
@@ -332,11 +363,9 @@ struct MgUpOp {
     if constexpr (PROJ) pre_load(a_, x0, y0, pre);
     __syncthreads();
     for (int s = 0; s < NU; s++) {  // black then red (js/fluid.js:307-310)
-#pragma unroll
-      for (int q = 0; q < PQ; q++) leg_relax<PX>(cB[q], sp);
+      leg_sweep<PX, SG>(cB, sp, leg_dummy(sp, PX * PY));
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < PQ; q++) leg_relax<PX>(cR[q], sp);
+      leg_sweep<PX, SG>(cR, sp, leg_dummy(sp, PX * PY));
       __syncthreads();
     }
     auto finish = [&](const LegCell& a) {
@@ -494,9 +523,9 @@ struct MgDown2Op {
     const Level& c = a_.c;
     const Level& n = a_.n;
     if (a_.skip && *a_.skip) return;  // the step's residual target is met
-    __shared__ float spf[PF * PF];
+    __shared__ float spf[PF * PF + LEG_DUMMY];
     __shared__ float sres[RR * RR];   // the fine rhs under the coarse region, then its residual
-    __shared__ float spc[PC * PC];
+    __shared__ float spc[PC * PC + LEG_DUMMY];
     __shared__ float src[RC * RC];    // the coarse rhs over the coarse region
     __shared__ float sresc[TC * TC];  // the coarse residual over the tile
     const int tid = threadIdx.x;
@@ -569,11 +598,9 @@ struct MgDown2Op {
     }
     __syncthreads();
     for (int s = 0; s < NU; s++) {  // red then black (js/fluid.js:298-301)
-#pragma unroll
-      for (int q = 0; q < QF; q++) leg_relax<PF>(fR[q], spf);
+      leg_sweep<PF>(fR, spf, leg_dummy(spf, PF * PF));
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < QF; q++) leg_relax<PF>(fB[q], spf);
+      leg_sweep<PF>(fB, spf, leg_dummy(spf, PF * PF));
       __syncthreads();
     }
     // the fine residual under the coarse region (:96-101); f's leg buffer over the tile
@@ -622,11 +649,9 @@ struct MgDown2Op {
       set_rh(cB[q]);
     }
     for (int s = 0; s < NU; s++) {
-#pragma unroll
-      for (int q = 0; q < QC; q++) leg_relax<PC>(cR[q], spc);
+      leg_sweep<PC>(cR, spc, leg_dummy(spc, PC * PC));
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < QC; q++) leg_relax<PC>(cB[q], spc);
+      leg_sweep<PC>(cB, spc, leg_dummy(spc, PC * PC));
       __syncthreads();
     }
     // the coarse residual over the tile; c's leg buffer
@@ -687,8 +712,8 @@ struct MgUp2Op {
     const Level& c = a_.c;
     const Level& n = a_.n;
     if (a_.skip && *a_.skip) return;
-    __shared__ float spf[PF * PF];
-    __shared__ float spc[PC * PC];
+    __shared__ float spf[PF * PF + LEG_DUMMY];
+    __shared__ float spc[PC * PC + LEG_DUMMY];
     const int tid = threadIdx.x;
     const int nx = f.nx, nxc = c.nx, nyc = c.ny, nxn = n.nx, nyn = n.ny;
     const int gx = (nxc + TC - 1) / TC;
@@ -776,11 +801,9 @@ struct MgUp2Op {
     }
     __syncthreads();
     for (int s = 0; s < NU; s++) {  // black then red (js/fluid.js:307-310)
-#pragma unroll
-      for (int q = 0; q < QC; q++) leg_relax<PC>(cB[q], spc);
+      leg_sweep<PC>(cB, spc, leg_dummy(spc, PC * PC));
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < QC; q++) leg_relax<PC>(cR[q], spc);
+      leg_sweep<PC>(cR, spc, leg_dummy(spc, PC * PC));
       __syncthreads();
     }
     // c.p over the tile
@@ -820,11 +843,9 @@ struct MgUp2Op {
     }
     __syncthreads();
     for (int s = 0; s < NU; s++) {
-#pragma unroll
-      for (int q = 0; q < QF; q++) leg_relax<PF>(fB[q], spf);
+      leg_sweep<PF>(fB, spf, leg_dummy(spf, PF * PF));
       __syncthreads();
-#pragma unroll
-      for (int q = 0; q < QF; q++) leg_relax<PF>(fR[q], spf);
+      leg_sweep<PF>(fR, spf, leg_dummy(spf, PF * PF));
       __syncthreads();
     }
     auto out_f = [&](const LegCell& a) {
