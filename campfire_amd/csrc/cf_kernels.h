@@ -278,7 +278,10 @@ int forces_coef_cascade(int nlev);
 void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* part, int nparts, hipStream_t s);
 // fused multigrid (cf_mg.hip); skip (may be null): the launch does nothing
 // once *skip is non-zero (the residual target of the step has been met)
-bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
+// zero_p: the leg starts from p = 0 without reading the fine p (a Krylov
+// iteration's correction, which kry_upd_def does not clear)
+bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s,
+                    int zero_p = 0);
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);
 // the level-0 up leg of a step's last V-cycle with the projection of the
 // velocities fused in (FP32 storage; replaces the k_project launch)
@@ -323,18 +326,27 @@ unsigned resnorm_blocks(const Level& L0);
 // the rest of the solve to defect correction (alpha = 1, beta = 0), which
 // stalls where CG diverges: at a right-hand side outside the range of a
 // singular system.
+//
+// With `fused` (the V-cycle's level-0 down leg starting from zero itself),
+// kry_update and dc_defect run as one pass, kry_upd_def: it writes the new
+// direction and pressure into the other half of p64 and d32 (2n entries
+// each), the current half given by the parity of the iterations done
+// (Dyn.cycles), and kry_settle copies them back into the first half after
+// the last iteration of a step, where every other reader finds them.
 struct KryBufs {
-  double* p64;  // the pressure
-  float* d32;   // the stored direction (FP32: the residual is recomputed from p64)
+  double* p64;  // the pressure (2 n: the halves kry_upd_def alternates)
+  float* d32;   // the stored direction (FP32: the residual is recomputed from p64; 2 n)
   float* b0;    // the saved right-hand side
 };
-void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
+void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol, bool fused,
                      hipStream_t s);
+void launch_kry_settle(const Level& L0, Dyn* dyn, const KryBufs& k, hipStream_t s);
+void phase_kry_settle(BatchPhase& ph, const Level& L0, Dyn* dyn, const KryBufs& k);
 // *out = 1 when the converged flag of each of the n scenes is set, else 0
 void launch_all_converged(Dyn* const* d, int n, int* out, hipStream_t s);
 // the kernels of launch_kry_start / launch_kry_step, one phase each
 void phase_kry_step(BatchPhase (&ph)[5], const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part,
-                    float tol);
+                    float tol, bool fused);
 void launch_dc_defect(const Level& L0, __half* ph0, double* p64, float* b0, Dyn* dyn, double* part, float tol,
                       bool first, hipStream_t s);
 // the Krylov start: a line search along the previous pressure (in p64) from
@@ -391,7 +403,7 @@ void phase_radiation(BatchPhase& ph, const Fields& F, int ntargets);
 void phase_fill_u32(BatchPhase& ph, void* a, uint32_t v, size_t n);
 // tile: the legs' tile edge (16, 32, 64), as the pool chose it for this level
 bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
-                   int tile);
+                   int tile, int zero_p = 0);
 bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
                  int tile);
 bool phase_mg_bottom(BatchPhase& ph, const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters,

@@ -1329,12 +1329,13 @@ struct KryDotsOp {
     KryBufs k;
     double* part;
     int vg;
+    int pp;  // kry_upd_def follows: the current direction is in the half of the iterations' parity
   };
   static constexpr int kThreads = 256;
   template <bool H>
   static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double (&v)[3]) {
     const Level& L = a_.L;
-    const float* __restrict__ d32 = a_.k.d32;
+    const float* __restrict__ d32 = a_.k.d32 + (a_.pp ? (size_t)(a_.d->cycles & 1) * (size_t)(L.nx * L.ny) : 0);
     const auto zat = [&](int q) { return kry_z<H>(L, a_.ph0, is, q); };
     const int n = L.nx * L.ny;
     for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
@@ -1420,6 +1421,91 @@ struct KryUpdateOp {
 };
 __global__ __launch_bounds__(256) void k_kry_update(const KryUpdateOp::Args a_) {
   KryUpdateOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// kry_update and dc_defect (after the start) in one pass: the new direction
+// d <- z - beta d and pressure p64 + alpha d of a cell and of its four
+// neighbours are formed from z, the stored d and p64 (kry_update's
+// arithmetic), so r = b - B p64 (dc_defect's) needs no second pass over
+// p64.  Half P = cycles & 1 of p64 and d32 holds the current values; the new
+// ones go to the other half, so no cell's store meets a neighbour's read.
+// The correction's start e = 0 is not stored: the next V-cycle's level-0
+// down leg starts from zero (zero_p).  Same workgroups, loop and partials as
+// dc_defect, so resid_fold sums the same values in the same order.
+struct KryUpdDefOp {
+  struct Args {
+    Level L;
+    const __half* ph0;
+    Dyn* d;
+    KryBufs k;
+    double* part;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  template <bool H>
+  static __device__ __forceinline__ void sweep(const Args& a_, int vb, float is, double alpha, double beta, int P,
+                                               double (&v)[2]) {
+    const Level& L = a_.L;
+    const int n = L.nx * L.ny;
+    const double* __restrict__ pin = a_.k.p64 + (size_t)P * n;
+    double* __restrict__ pout = a_.k.p64 + (size_t)(1 - P) * n;
+    const float* __restrict__ din = a_.k.d32 + (size_t)P * n;
+    float* __restrict__ dout = a_.k.d32 + (size_t)(1 - P) * n;
+    const float* __restrict__ b0 = a_.k.b0;
+    const auto dnew = [&](int q) { return (float)(kry_z<H>(L, a_.ph0, is, q) - beta * (double)din[q]); };
+    const auto xnew = [&](int q) { return pin[q] + alpha * (double)dnew(q); };
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      const float dk = dnew(k);
+      const double xk = pin[k] + alpha * (double)dk;
+      dout[k] = dk;
+      pout[k] = xk;
+      const double b = (double)b0[k];
+      const int j = k / L.nx, i = k - j * L.nx;
+      double bv;
+      const double r = apply64(L, xnew, i, j, xk, bv) ? b - bv : 0.0;
+      L.rhs[k] = (float)r;
+      v[0] += r * r;
+      v[1] += b * b;
+    }
+  }
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    Dyn* d = a_.d;
+    if (d->converged) return;
+    double v[2] = {0, 0};  // ||r||^2, ||b||^2
+    const int P = d->cycles & 1;
+    if (a_.ph0) sweep<true>(a_, vb, d->isig, d->kalpha, d->kbeta, P, v);
+    else sweep<false>(a_, vb, 1.f, d->kalpha, d->kbeta, P, v);
+    store_partials<2>(v, a_.part, vb);
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_upd_def(const KryUpdDefOp::Args a_) {
+  KryUpdDefOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// after a step's last iteration: the direction and pressure back into the
+// first half of d32 and p64 when an odd number of iterations left them in
+// the second (run whether or not the step converged)
+struct KrySettleOp {
+  struct Args {
+    Level L;
+    const Dyn* d;
+    KryBufs k;
+    int vg;
+  };
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const Args& a_, int vb) {
+    if (!(a_.d->cycles & 1)) return;
+    const int n = a_.L.nx * a_.L.ny;
+    double* __restrict__ p64 = a_.k.p64;
+    float* __restrict__ d32 = a_.k.d32;
+    for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
+      p64[k] = p64[n + k];
+      d32[k] = d32[n + k];
+    }
+  }
+};
+__global__ __launch_bounds__(256) void k_kry_settle(const KrySettleOp::Args a_) {
+  KrySettleOp::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Krylov start from the previous step's pressure p (in p64): a line search
@@ -2167,14 +2253,22 @@ void launch_kry_start(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, 
   launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb});
   launch(k_resid_fold, dim3(1), dim3(256), 0, s, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 1, 1});
 }
-void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol,
+void launch_kry_step(const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part, float tol, bool fused,
                      hipStream_t s) {
   const int nb = (int)resnorm_blocks(L0);
-  launch(k_kry_dots, dim3(nb), dim3(256), 0, s, KryDotsOp::Args{L0, ph0, dyn, k, part, nb});
+  launch(k_kry_dots, dim3(nb), dim3(256), 0, s, KryDotsOp::Args{L0, ph0, dyn, k, part, nb, fused ? 1 : 0});
   launch(k_kry_coef, dim3(1), dim3(256), 0, s, KryCoefOp::Args{dyn, part, nb});
-  launch(k_kry_update, dim3(nb), dim3(256), 0, s, KryUpdateOp::Args{L0, ph0, dyn, k, nb});
-  launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb});
+  if (fused) {
+    launch(k_kry_upd_def, dim3(nb), dim3(256), 0, s, KryUpdDefOp::Args{L0, ph0, dyn, k, part, nb});
+  } else {
+    launch(k_kry_update, dim3(nb), dim3(256), 0, s, KryUpdateOp::Args{L0, ph0, dyn, k, nb});
+    launch(k_dc_defect, dim3(nb), dim3(256), 0, s, DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb});
+  }
   launch(k_resid_fold, dim3(1), dim3(256), 0, s, ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 1});
+}
+void launch_kry_settle(const Level& L0, Dyn* dyn, const KryBufs& k, hipStream_t s) {
+  const int nb = (int)resnorm_blocks(L0);
+  launch(k_kry_settle, dim3(nb), dim3(256), 0, s, KrySettleOp::Args{L0, dyn, k, nb});
 }
 __global__ __launch_bounds__(256) void k_all_converged(Dyn* const* __restrict__ d, int n, int* __restrict__ out) {
   __shared__ int open;
@@ -2268,14 +2362,23 @@ void phase_kry_start(BatchPhase (&ph)[5], const Level& L0, __half* ph0, Dyn* dyn
                          "resid_fold");
 }
 void phase_kry_step(BatchPhase (&ph)[5], const Level& L0, __half* ph0, Dyn* dyn, const KryBufs& k, double* part,
-                    float tol) {
+                    float tol, bool fused) {
   const int nb = (int)resnorm_blocks(L0);
-  phase_add<KryDotsOp>(ph[0], KryDotsOp::Args{L0, ph0, dyn, k, part, nb}, nb, dim3(256), "kry_dots");
+  phase_add<KryDotsOp>(ph[0], KryDotsOp::Args{L0, ph0, dyn, k, part, nb, fused ? 1 : 0}, nb, dim3(256), "kry_dots");
   phase_add<KryCoefOp>(ph[1], KryCoefOp::Args{dyn, part, nb}, 1, dim3(256), "kry_coef");
-  phase_add<KryUpdateOp>(ph[2], KryUpdateOp::Args{L0, ph0, dyn, k, nb}, nb, dim3(256), "kry_update");
-  phase_add<DcDefectOp>(ph[3], DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb}, nb, dim3(256), "dc_defect");
+  if (fused) {
+    phase_add<KryUpdDefOp>(ph[2], KryUpdDefOp::Args{L0, ph0, dyn, k, part, nb}, nb, dim3(256), "kry_upd_def");
+  } else {
+    phase_add<KryUpdateOp>(ph[2], KryUpdateOp::Args{L0, ph0, dyn, k, nb}, nb, dim3(256), "kry_update");
+    phase_add<DcDefectOp>(ph[3], DcDefectOp::Args{L0, ph0, k.p64, k.b0, dyn, part, 0, nb}, nb, dim3(256),
+                          "dc_defect");
+  }
   phase_add<ResidFoldOp>(ph[4], ResidFoldOp::Args{dyn, part, nb, L0.nx * L0.ny, tol, 0, 1}, 1, dim3(256),
                          "resid_fold");
+}
+void phase_kry_settle(BatchPhase& ph, const Level& L0, Dyn* dyn, const KryBufs& k) {
+  const int nb = (int)resnorm_blocks(L0);
+  phase_add<KrySettleOp>(ph, KrySettleOp::Args{L0, dyn, k, nb}, nb, dim3(256), "kry_settle");
 }
 void phase_dc_accum(BatchPhase& ph, const Level& L0, const __half* ph0, const Dyn* dyn, double* p64, const int* skip) {
   int b = (L0.nx * L0.ny + 1023) / 1024;

@@ -160,7 +160,8 @@ struct MgDownOp {
     Level c;
     MgStore st;
     const int* skip;
-    int vg;  // tiles of this scene's launch
+    int vg;      // tiles of this scene's launch
+    int zero_p;  // start from p = 0 without reading the fine p
   };
   static constexpr int kThreads = TX * TY >= 4096 ? 1024 : 256;  // 64x64 tiles: 1024
   static constexpr int H = 2 * NU + 1;
@@ -197,8 +198,11 @@ struct MgDownOp {
       const LegCell c0 = leg_cell_m<M>(f, a_.st, x, y, rx, ry, RX, RY, PX, rv[q][0]);
       const LegCell c1 = leg_cell_m<M>(f, a_.st, x + 1, y, rx + 1, ry, RX, RY, PX, rv[q][1]);
       const bool in0 = (c0.ci & LEG_IN) != 0, in1 = (c1.ci & LEG_IN) != 0;
-      const float v0 = ldp<FH>(f.p, a_.st.f.p, leg_idx(in0, x, y, nx), isig);
-      const float v1 = ldp<FH>(f.p, a_.st.f.p, leg_idx(in1, x + 1, y, nx), isig);
+      float v0 = 0.f, v1 = 0.f;
+      if (!a_.zero_p) {  // uniform over the launch's scene
+        v0 = ldp<FH>(f.p, a_.st.f.p, leg_idx(in0, x, y, nx), isig);
+        v1 = ldp<FH>(f.p, a_.st.f.p, leg_idx(in1, x + 1, y, nx), isig);
+      }
       pv[q][0] = in0 ? v0 : 0.f;
       pv[q][1] = in1 ? v1 : 0.f;
       const bool red0 = ((x + y) & 1) == 0;
@@ -1398,11 +1402,12 @@ static bool mg_dispatch(int tile, int nu, int mode, F&& f) {
   return false;
 }
 
-bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s) {
+bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s,
+                    int zero_p) {
   const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
   return mg_dispatch<MgDownOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
-    launch(k_mg_down_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, st, skip, nb});
+    launch(k_mg_down_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, st, skip, nb, zero_p});
   });
 }
 
@@ -1470,11 +1475,11 @@ bool launch_mg_up_project(const Level& f, const Level& c, const Fields& F, int n
 }
 
 bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
-                   int t) {
+                   int t, int zero_p) {
   const int nb = (int)ntiles(f.nx, f.ny, t, t);
   return mg_dispatch<MgDownOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
-    phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(Op::kThreads), "mg_down");
+    phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb, zero_p}, nb, dim3(Op::kThreads), "mg_down");
   });
 }
 

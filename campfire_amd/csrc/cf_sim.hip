@@ -211,6 +211,15 @@ static bool pair_first(const cf_sim* s, int l) {
 }
 static bool paired(const cf_sim* s, int l) { return pair_first(s, l) || (l >= 2 && pair_first(s, l - 1)); }
 
+// whether a Krylov iteration runs kry_update and dc_defect as one pass
+// (kry_upd_def: the new direction and pressure go to the other half of d32
+// and p64, and kry_settle brings them back after the step's last iteration):
+// the fused path with a tiled level 0, whose down leg then starts from zero
+// itself, since the fused pass cannot clear the correction that it reads
+static bool kry_fused(const cf_sim* s) {
+  return s->sch.tol > 0 && s->sch.refine && s->sch.krylov && !s->mg_simple && s->mg_bottom >= 1;
+}
+
 // whether the step's projection runs inside the last V-cycle's level-0 up
 // leg: a fixed cycle count (no residual target, so p, not p64, is projected),
 // FP32 storage, the fused path
@@ -230,7 +239,8 @@ static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef,
         mark(s, level_tag("mg_down2", l));
         l += 2;
       } else {
-        cf::launch_mg_down(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st); mark(s, level_tag("mg_down", l));
+        cf::launch_mg_down(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st, l == 0 && kry_fused(s) ? 1 : 0);
+        mark(s, level_tag("mg_down", l));
         l += 1;
       }
     }
@@ -306,7 +316,7 @@ static void enqueue_mg_cycle(cf_sim* s, int c, bool coarse_coef, bool project = 
   const cf::KryBufs kb{s->dP64, s->dD32, s->dB0};
   enqueue_vcycle(s, s->sch.nu, skip, coarse_coef && c == 0, project);
   if (kry) {
-    cf::launch_kry_step(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, s->st); mark(s, "kry_step");
+    cf::launch_kry_step(s->L[0], ph0, s->dDyn, kb, s->dNorm, s->sch.tol, kry_fused(s), s->st); mark(s, "kry_step");
   } else if (dc) {
     cf::launch_dc_accum(s->L[0], ph0, s->dDyn, s->dP64, skip, s->st); mark(s, "dc_accum");
     cf::launch_dc_defect(s->L[0], ph0, s->dP64, s->dB0, s->dDyn, s->dNorm, s->sch.tol, false, s->st);
@@ -315,9 +325,17 @@ static void enqueue_mg_cycle(cf_sim* s, int c, bool coarse_coef, bool project = 
     cf::launch_resnorm(s->L[0], s->dDyn, s->dNorm, s->sch.tol, s->st); mark(s, "resnorm");
   }
 }
+// after a step's last V-cycle: the Krylov direction and pressure back into
+// the first halves of d32 and p64 (kry_fused)
+static void enqueue_mg_post(cf_sim* s) {
+  if (!kry_fused(s)) return;
+  cf::launch_kry_settle(s->L[0], s->dDyn, cf::KryBufs{s->dP64, s->dD32, s->dB0}, s->st);
+  mark(s, "kry_settle");
+}
 static void enqueue_mg(cf_sim* s, bool coarse_coef) {
   enqueue_mg_pre(s);
   for (int c = 0; c < s->sch.vcycles; c++) enqueue_mg_cycle(s, c, coarse_coef);
+  enqueue_mg_post(s);
 }
 
 static void enqueue_coef(cf_sim* s) {
@@ -411,6 +429,7 @@ static void enqueue_step(cf_sim* s) {
   const bool coarse_coef = enqueue_step_pre(s);
   const bool fp = fuse_project(s);
   for (int c = 0; c < s->sch.vcycles; c++) enqueue_mg_cycle(s, c, coarse_coef, fp && c == s->sch.vcycles - 1);
+  enqueue_mg_post(s);
   if (!fp) enqueue_step_post(s);
 }
 
@@ -471,7 +490,10 @@ static int capture_segments(cf_sim* s) {
   if (!rc && s->sch.vcycles > g.h)
     rc = capture_fn(s->st, &g.chunk, [&] { for (int c = 1; c <= SEG_CHUNK; c++) enqueue_mg_cycle(s, c, false); });
   if (!rc && g.r > 0) rc = capture_fn(s->st, &g.rem, [&] { for (int c = 1; c <= g.r; c++) enqueue_mg_cycle(s, c, false); });
-  if (!rc) rc = capture_fn(s->st, &g.tail, [&] { enqueue_step_post(s); });
+  if (!rc) rc = capture_fn(s->st, &g.tail, [&] {
+    enqueue_mg_post(s);
+    enqueue_step_post(s);
+  });
   return rc;
 }
 
@@ -735,6 +757,9 @@ static int pool_build_step(cf_pool* pl) {
     ph.push_back(std::move(a));
     ph.push_back(std::move(f));
   };
+  // the fused Krylov pass when every scene can take it (one kernel per phase)
+  bool fused = kry;
+  for (auto* s : sc) fused = fused && kry_fused(s);
   if (kry) {
     cf::BatchPhase a[5];
     for (auto* s : sc) cf::phase_kry_start(a, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm, k.tol);
@@ -773,7 +798,8 @@ static int pool_build_step(cf_pool* pl) {
           for (size_t q = 0; q < sc.size(); q++)
             if (leg_ok(q, d, t, m)) {
               cf_sim* s = sc[q];
-              cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr, t);
+              cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr, t,
+                                d == 0 && fused ? 1 : 0);
             }
           if (!a.empty()) ph.push_back(a);
         }
@@ -801,7 +827,7 @@ static int pool_build_step(cf_pool* pl) {
     }
     if (kry) {
       cf::BatchPhase a[5];
-      for (auto* s : sc) cf::phase_kry_step(a, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm, k.tol);
+      for (auto* s : sc) cf::phase_kry_step(a, s->L[0], ph0(s), s->dDyn, kb(s), s->dNorm, k.tol, fused);
       for (auto& x : a) ph.push_back(std::move(x));
     } else if (dc) {
       {
@@ -821,6 +847,7 @@ static int pool_build_step(cf_pool* pl) {
     }
   }
   pl->cyc.push_back(ph.size());
+  if (fused) { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_kry_settle(a, s->L[0], s->dDyn, kb(s)); }
   { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_project(a, s->F, dc ? s->dP64 : nullptr); }
   for (auto& a : ph)
     if (a.mixed) return set_err(CF_EINVAL, "pool: phase '%s' would run one kernel over scenes of another", a.tag);
@@ -1007,8 +1034,8 @@ static void sim_plan(cf_sim* s, std::vector<std::pair<void**, size_t>>& plan) {
   add((void**)&s->dPart, (size_t)nparts * 8);
   add((void**)&s->dAccPart, (size_t)nparts * 16);
   add((void**)&s->dNorm, 4 * sizeof(double) * 1024);
-  add((void**)&s->dP64, nC * sizeof(double));
-  add((void**)&s->dD32, nC * sizeof(float));
+  add((void**)&s->dP64, 2 * nC * sizeof(double));  // two halves: see kry_fused
+  add((void**)&s->dD32, 2 * nC * sizeof(float));
   add((void**)&s->dB0, nC * sizeof(float));
   add((void**)&s->dBand, (256 * 33 + 256) * sizeof(double));
 }
