@@ -536,10 +536,15 @@ __device__ __forceinline__ float force_u(const cf_params& P, const A& a, int i, 
   const float TL = a.Tc(i - 1, j), TR = a.Tc(i, j);
   const float rhoF = rhoOf(P, K(0.5) * (TL + TR));
   const float muE = a.mu(P, i, j), muW = a.mu(P, i - 1, j);
-  float uN = u, muN = muOf(P, K(0.5) * (TL + TR));
-  if (j < ny - 1) { uN = u2(i, j + 1); muN = muOf(P, K(0.25) * (TL + TR + a.Tc(i - 1, j + 1) + a.Tc(i, j + 1))); }
-  float uS = -u, muS = muOf(P, K(0.5) * (TL + TR));
-  if (j > 0) { uS = u2(i, j - 1); muS = muOf(P, K(0.25) * (TL + TR + a.Tc(i - 1, j - 1) + a.Tc(i, j - 1))); }
+  // the north and south corner viscosities: the mean of four cells, or at
+  // the top (floor) row the face's own two; the temperature is chosen first
+  // and muOf evaluated once (at the edge the neighbour face is read at the
+  // face itself and not used; Tc clamps)
+  const bool hn = j < ny - 1, hs = j > 0;
+  const float uN = hn ? u2(i, hn ? j + 1 : j) : u;
+  const float muN = muOf(P, hn ? K(0.25) * (TL + TR + a.Tc(i - 1, j + 1) + a.Tc(i, j + 1)) : K(0.5) * (TL + TR));
+  const float uS = hs ? u2(i, hs ? j - 1 : j) : -u;
+  const float muS = muOf(P, hs ? K(0.25) * (TL + TR + a.Tc(i - 1, j - 1) + a.Tc(i, j - 1)) : K(0.5) * (TL + TR));
   const float visc = (muE * (u2(i + 1, j) - u) + muW * (u2(i - 1, j) - u) + muN * (uN - u) + muS * (uS - u)) * idx2 / rhoF;
   const float vAvg = K(0.25) * (a.V(i - 1, j) + a.V(i, j) + a.V(i - 1, j + 1) + a.V(i, j + 1));
   return (u + dt * visc) * exchangef(P, sqrtf(u * u + vAvg * vAvg), K(0.5) * (TL + TR), rc);
@@ -559,10 +564,12 @@ __device__ __forceinline__ float force_v(const cf_params& P, const A& a, int i, 
   const float Tf = K(0.5) * (TB + TT);
   const float rhoF = rhoOf(P, Tf);
   const float muN = a.mu(P, i, j), muS = a.mu(P, i, j - 1);
-  float vE = v, muE = muOf(P, Tf);
-  if (i < nx - 1) { vE = v2(i + 1, j); muE = muOf(P, K(0.25) * (TB + TT + a.Tc(i + 1, j - 1) + a.Tc(i + 1, j))); }
-  float vW = v, muW = muOf(P, Tf);
-  if (i > 0) { vW = v2(i - 1, j); muW = muOf(P, K(0.25) * (TB + TT + a.Tc(i - 1, j - 1) + a.Tc(i - 1, j))); }
+  // the east and west corner viscosities, as the corners of force_u
+  const bool he = i < nx - 1, hw = i > 0;
+  const float vE = he ? v2(he ? i + 1 : i, j) : v;
+  const float muE = muOf(P, he ? K(0.25) * (TB + TT + a.Tc(i + 1, j - 1) + a.Tc(i + 1, j)) : Tf);
+  const float vW = hw ? v2(hw ? i - 1 : i, j) : v;
+  const float muW = muOf(P, hw ? K(0.25) * (TB + TT + a.Tc(i - 1, j - 1) + a.Tc(i - 1, j)) : Tf);
   const float visc = (muN * (v2(i, j + 1) - v) + muS * (v2(i, j - 1) - v) + muE * (vE - v) + muW * (vW - v)) * idx2 / rhoF;
   const float buoy = P.grav * (div_rcp(Tf, P.Tamb, rc.iTamb) - K(1));
   const float uAvg = K(0.25) * (a.U(i, j - 1) + a.U(i + 1, j - 1) + a.U(i, j) + a.U(i + 1, j));
@@ -623,13 +630,49 @@ struct TileAcc {
   __device__ float FY(int i, int j) const { return fy[(j - y0 + 1) * FC_R + (i - x0 + 1)]; }
 };
 
+// Coarse level LV's face coefficients over a forces tile (TX = FC_T) from
+// the tile's fine faces in LDS: sax (FC_T rows of FC_T + 1 U faces) and say
+// (FC_T + 1 rows of FC_T V faces).  The face (I, J) is the restriction
+// cascade's binary tree over 2^LV fine faces (ax: column 2^LV I, rows
+// 2^LV J + k; ay: row 2^LV J, columns 2^LV I + k), each node 0.5 (a + b) of
+// its two children as the level-by-level restriction forms it
+// (shaders/pressure.wgsl:57-67), so the values are bitwise the cascade's.
+template <int LV>
+__device__ __forceinline__ void cascade_level(const Level& C, const float* sax, const float* say, int x0, int y0,
+                                              int tid) {
+  constexpr int TX = FC_T, cw = TX >> LV, nax = (cw + 1) * cw, N = 1 << LV;  // square tile: nay = nax
+  const int xc = x0 >> LV, yc = y0 >> LV;
+  for (int kk = tid; kk < 2 * nax; kk += 256) {
+    const bool isx = kk < nax;
+    const int k2 = isx ? kk : kk - nax;
+    const int I = isx ? k2 % (cw + 1) : k2 % cw, J = isx ? k2 / (cw + 1) : k2 / cw;
+    // fine face k of the tree: ax at (row N J + k, column N I), ay at (row N J, column N I + k)
+    const int base = isx ? (J * N) * (TX + 1) + I * N : (J * N) * TX + I * N;
+    const int step = isx ? TX + 1 : 1;
+    const float* src = isx ? sax : say;
+    float v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = src[base + k * step];
+#pragma unroll
+    for (int w = N / 2; w >= 1; w /= 2)
+#pragma unroll
+      for (int k = 0; k < w; k++) v[k] = K(0.5) * (v[2 * k] + v[2 * k + 1]);
+    const int gI = xc + I, gJ = yc + J;
+    if (isx) {
+      if (gJ < C.ny && ((I < cw && gI < C.nx) || gI == C.nx)) C.ax[gJ * (C.nx + 1) + gI] = v[0];
+    } else {
+      if (gI < C.nx && ((J < cw && gJ < C.ny) || gJ == C.ny)) C.ay[gJ * C.nx + gI] = v[0];
+    }
+  }
+}
+
 struct ForcesCoefOp {
   struct Args {
     Fields F;
     CoefLevels CL;
     const int* part;
     int nparts;
-    int vg;  // tiles of this scene's launch
+    int vg;  // tiles of this scene's launch; workgroup vg (when nparts > 0) folds the partials
   };
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
@@ -663,7 +706,12 @@ struct ForcesCoefOp {
   const cf_params P = *F.P;
   const MomRecip rc = mom_recip(P);
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;  // 32 x 8 threads
-  if (vb == 0 && nparts > 0) acc_fold(part, nparts, F.acc, F.dyn, (int*)pool);
+  // the workgroup after the tiles folds the sources partials into acc, so
+  // that no tile's work waits for it
+  if (vb == a_.vg) {
+    if (nparts > 0) acc_fold(part, nparts, F.acc, F.dyn, (int*)pool);
+    return;
+  }
   const int nx = P.nx, ny = P.ny;
   const int gx = (nx + TX - 1) / TX;
   const int t = vb;
@@ -786,6 +834,12 @@ struct ForcesCoefOp {
       F.rhs0[j * nx + i] = open > K(0) ? (div - sqv[r]) / P.dt : K(0);
     }
   }
+  // ---- cascade of coefficient restrictions (:57-67), one pass: the face
+  // (I, J) of coarse level l over the tile is the cascade's binary tree of
+  // 2^l fine faces (ax: column 2^l I, rows 2^l J + k; ay: row 2^l J,
+  // columns 2^l I + k), each node 0.5 (a + b) of its two children as the
+  // level-by-level restriction forms it, so every level comes from the fine
+  // faces in LDS after one barrier
   if (CL.ncascade > 0) {
     __syncthreads();
 #pragma unroll
@@ -794,42 +848,13 @@ struct ForcesCoefOp {
       if (q < NFU) sax[q / (TX + 1)][q % (TX + 1)] = fa[r];
       if (q < NFV) say[q / TX][q % TX] = fb[r];
     }
-  }
-  // ---- cascade of coefficient restrictions (:57-67)
-  int tx_l = TX, ty_l = TY;
-  for (int l = 0; l < CL.ncascade; l++) {
-    const Level& C = CL.L[l + 1];
-    const int xc = x0 >> (l + 1), yc = y0 >> (l + 1);
-    const int cwx = tx_l / 2, cwy = ty_l / 2;  // coarse tile extent
-    const int nax = (cwx + 1) * cwy, nay = cwx * (cwy + 1);
-    float va[2], vb[2];
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int kk = tid + 256 * q;
-      va[q] = vb[q] = 0.f;
-      if (kk < nax) { const int I = kk % (cwx + 1), J = kk / (cwx + 1); va[q] = K(0.5) * (sax[2 * J][2 * I] + sax[2 * J + 1][2 * I]); }
-      if (kk < nay) { const int I = kk % cwx, J = kk / cwx; vb[q] = K(0.5) * (say[2 * J][2 * I] + say[2 * J][2 * I + 1]); }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int kk = tid + 256 * q;
-      if (kk < nax) {
-        const int I = kk % (cwx + 1), J = kk / (cwx + 1);
-        sax[J][I] = va[q];
-        const int gI = xc + I, gJ = yc + J;
-        if (gJ < C.ny && ((I < cwx && gI < C.nx) || gI == C.nx)) C.ax[gJ * (C.nx + 1) + gI] = va[q];
-      }
-      if (kk < nay) {
-        const int I = kk % cwx, J = kk / cwx;
-        say[J][I] = vb[q];
-        const int gI = xc + I, gJ = yc + J;
-        if (gI < C.nx && ((J < cwy && gJ < C.ny) || gJ == C.ny)) C.ay[gJ * C.nx + gI] = vb[q];
-      }
-    }
-    tx_l = cwx; ty_l = cwy;
   }
+  if (CL.ncascade >= 1) cascade_level<1>(CL.L[1], &sax[0][0], &say[0][0], x0, y0, tid);
+  if (CL.ncascade >= 2) cascade_level<2>(CL.L[2], &sax[0][0], &say[0][0], x0, y0, tid);
+  if (CL.ncascade >= 3) cascade_level<3>(CL.L[3], &sax[0][0], &say[0][0], x0, y0, tid);
+  if (CL.ncascade >= 4) cascade_level<4>(CL.L[4], &sax[0][0], &say[0][0], x0, y0, tid);
+  if (CL.ncascade >= 5) cascade_level<5>(CL.L[5], &sax[0][0], &say[0][0], x0, y0, tid);
 
   }
 };
@@ -2191,7 +2216,8 @@ void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* pa
   for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
   CL.ncascade = forces_coef_cascade(nlev);
   const unsigned nb = (unsigned)(((F.nx + FC_T - 1) / FC_T) * ((F.ny + FC_T - 1) / FC_T));
-  launch(k_forces_coef, dim3(nb), dim3(32, 8), 0, s, ForcesCoefOp::Args{F, CL, part, nparts, (int)nb});
+  launch(k_forces_coef, dim3(nb + (nparts > 0 ? 1 : 0)), dim3(32, 8), 0, s,
+         ForcesCoefOp::Args{F, CL, part, nparts, (int)nb});
 }
 void launch_coef_rhs(const Fields& F, hipStream_t s) {
   launch(k_coef_rhs, dim3(nblk(F.nx + 1, F.ny + 1)), B328, 0, s, F);
@@ -2325,7 +2351,8 @@ void phase_forces_coef(BatchPhase& ph, const Fields& F, const Level* L, int nlev
   for (int l = 0; l < nlev && l < 16; l++) CL.L[l] = L[l];
   CL.ncascade = forces_coef_cascade(nlev);
   const int nb = ((F.nx + FC_T - 1) / FC_T) * ((F.ny + FC_T - 1) / FC_T);
-  phase_add<ForcesCoefOp>(ph, ForcesCoefOp::Args{F, CL, part, nparts, nb}, nb, dim3(32, 8), "forces_coef");
+  phase_add<ForcesCoefOp>(ph, ForcesCoefOp::Args{F, CL, part, nparts, nb}, nb + (nparts > 0 ? 1 : 0), dim3(32, 8),
+                          "forces_coef");
 }
 void phase_coef_restrict(BatchPhase& ph, const Level& f, const Level& c) {
   const int nb = (int)nblk(c.nx + 1, c.ny + 1);
