@@ -15,6 +15,13 @@
 
 namespace cf {
 
+// A workgroup barrier that orders LDS only.  The threads' global loads and
+// stores stay in flight across it and are waited for where their values are
+// used (or at the kernel's end); __syncthreads' global fence would wait for
+// every one of them at each barrier.  For barriers that exchange data through
+// LDS alone.
+__device__ __forceinline__ void lds_sync() { __barrier(__CLK_LOCAL_MEM_FENCE); }
+
 __device__ __forceinline__ float mixf(float a, float b, float t) { return a * (K(1) - t) + b * t; }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ int iclamp(int x, int lo, int hi) { return min(max(x, lo), hi); }

@@ -365,12 +365,12 @@ struct MgUpOp {
     }
     Pre pre;
     if constexpr (PROJ) pre_load(a_, x0, y0, pre);
-    __syncthreads();
+    lds_sync();
     for (int s = 0; s < NU; s++) {  // black then red (js/fluid.js:307-310)
       leg_sweep<PX, SG>(cB, sp, leg_dummy(sp, PX * PY));
-      __syncthreads();
+      lds_sync();
       leg_sweep<PX, SG>(cR, sp, leg_dummy(sp, PX * PY));
-      __syncthreads();
+      lds_sync();
     }
     auto finish = [&](const LegCell& a) {
       const int ci = a.ci & LEG_IDX;
