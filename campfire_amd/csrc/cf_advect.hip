@@ -52,14 +52,17 @@ struct Win {
   // the footprint whose lower-left corner is at the floored position (bx, by):
   // tested in float, exact for values in the window (small integers) and
   // false for NaN or huge ones, so the conversion needs no clamp first
+  // The element index b * W + a is formed in float, exact for these small
+  // integers, and converted once: one conversion instead of two and no
+  // integer multiply.
   __device__ __forceinline__ const T* at(float bx, float by, bool& ok) const {
     const float a = bx - fox, b = by - foy;
     if constexpr (!CK) {
-      return s + (int)b * W + (int)a;
+      return s + (unsigned)(int)fmaf(b, (float)W, a);
     } else {
       const bool in = a >= K(0) && a < (float)(W - 1) && b >= K(0) && b < (float)(H - 1);
       ok = ok && in;
-      return s + (in ? (int)b * W + (int)a : 0);
+      return s + (in ? (unsigned)(int)fmaf(b, (float)W, a) : 0u);
     }
   }
   __device__ __forceinline__ T own(int i, int j) const { return s[(j - oy) * W + (i - ox)]; }

@@ -998,6 +998,92 @@ __device__ __forceinline__ void sweep_relax2(const SweepCell& a, const SweepCell
   p[b.k] = b.d > K(0) ? vb : K(0);
 }
 
+// ---- the coarsest sweeps of a 16 x 16 level in registers ----
+// Lane (x, q) = (t mod 16, t / 16) of the first wave holds the cells
+// (x, 4q .. 4q + 3), sorted by colour: R0 / R1 its red cells in rows
+// 4q + {0, 1} / 4q + {2, 3}, K0 / K1 its black ones.  A cell's west and
+// east neighbours are the same row in lanes x -+ 1, moved by a row shift
+// inside the wave's 16-lane rows (the edge lane keeps its own value, which the
+// zero coefficient multiplies, as the LDS form reads the cell itself); its
+// vertical neighbours are in the lane's other registers or, past the ends of
+// its four rows, in lane x of the next row of lanes below or above.  The
+// arithmetic is sweep_relax2's, term by term, so p is bitwise the same.
+struct Cell16 {
+  float w, e, s, n, d, y, r;
+};
+
+__device__ __forceinline__ float relax16(const Cell16& c, float pw, float pe, float ps, float pn) {
+  const float off = c.w * pw + c.e * pe + c.s * ps + c.n * pn;
+  const float v = div_rcp_fast(off - c.r, c.d, c.y);
+  return c.d > K(0) ? v : K(0);
+}
+
+// v of lane (x, q - 1) (below) or (x, q + 1) (above).  The 16-lane swap
+// gives rows [v0 v0 v2 v2] and [v1 v1 v3 v3], the 32-lane swap of those
+// [v0 v0 v1 v1] and [v2 v2 v3 v3]; each row of lanes takes its neighbour
+// row from one of them (rows 0 below and 3 above get a value they ignore)
+__device__ __forceinline__ float row_from(float v, int q, bool below) {
+  const unsigned a = __float_as_uint(v);
+  const auto p = __builtin_amdgcn_permlane16_swap(a, a, false, false);
+  const auto r = __builtin_amdgcn_permlane32_swap(p[0], p[1], false, false);
+  return __uint_as_float(below ? (q == 2 ? r[0] : p[0]) : (q == 1 ? r[1] : p[1]));
+}
+// v of lane x - 1 (west) / x + 1 (east) of the same 16-lane row; the lane at
+// the row's end gets `self`
+__device__ __forceinline__ float lane_west(float self, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(self), __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_east(float self, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(self), __float_as_int(v), 0x101, 0xf, 0xf, false));
+}
+
+// `iters` red-black sweeps from zero of the 16 x 16 level whose face
+// coefficients are axc, ayc and rhs sr2 (LDS); p to sp2.  The whole first
+// wave calls it.
+__device__ __forceinline__ void coarse_sweeps16(const float* axc, const float* ayc, const float* sr2, float* sp2,
+                                             float h2, int iters, int t) {
+  const int x = t & 15, q = t >> 4;
+  const bool ev = (x & 1) == 0;  // red cells in rows 4q and 4q + 2 (else 4q + 1, 4q + 3)
+  const int j0 = 4 * q;
+  const int kk[4] = {(j0 + (ev ? 0 : 1)) * 16 + x, (j0 + (ev ? 2 : 3)) * 16 + x, (j0 + (ev ? 1 : 0)) * 16 + x,
+                     (j0 + (ev ? 3 : 2)) * 16 + x};
+  CellSet<4> C;
+  cells_load<4>(C, 16, 16, axc, ayc, kk);
+  Cell16 c[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) c[m] = {C.w[m], C.e[m], C.s[m], C.n[m], C.diag[m], C.y[m], sr2[kk[m]] * h2};
+  const bool bottom = q == 0, top = q == 3;
+  float R0 = K(0), R1 = K(0), K0 = K(0), K1 = K(0);
+  for (int it = 0; it < iters; it++) {
+    {  // red: the south neighbour of an even lane's R0 is K1 of the lanes below,
+       // the north one of an odd lane's R1 is K0 of the lanes above
+      const float X = row_from(ev ? K1 : K0, q, ev);
+      const float s0 = ev ? (bottom ? R0 : X) : K0;
+      const float n0 = ev ? K0 : K1;  // also R1's south neighbour
+      const float n1 = ev ? K1 : (top ? R1 : X);
+      const float a = relax16(c[0], lane_west(R0, K0), lane_east(R0, K0), s0, n0);
+      const float b = relax16(c[1], lane_west(R1, K1), lane_east(R1, K1), n0, n1);
+      R0 = a;
+      R1 = b;
+    }
+    {  // black: the north neighbour of an even lane's K1 is R0 of the lanes
+       // above, the south one of an odd lane's K0 is R1 of the lanes below
+      const float X = row_from(ev ? R0 : R1, q, !ev);
+      const float s0 = ev ? R0 : (bottom ? K0 : X);
+      const float n0 = ev ? R1 : R0;  // also K1's south neighbour
+      const float n1 = ev ? (top ? K1 : X) : R1;
+      const float a = relax16(c[2], lane_west(K0, R0), lane_east(K0, R0), s0, n0);
+      const float b = relax16(c[3], lane_west(K1, R1), lane_east(K1, R1), n0, n1);
+      K0 = a;
+      K1 = b;
+    }
+  }
+  sp2[kk[0]] = R0;
+  sp2[kk[1]] = R1;
+  sp2[kk[2]] = K0;
+  sp2[kk[3]] = K1;
+}
+
 // Entry q of a CellSet of slot 0 or 1 of the bottom solver (k = -1: none).
 // Neighbours missing at the domain edge are read at the cell itself with a
 // zero coefficient.
@@ -1295,26 +1381,30 @@ struct MgBottomOp {
         for (int q = 0; q < 4; q++) stash_put<4>(stash, t, 8 * q, C0, q);
 #pragma unroll
         for (int q = 0; q < 2; q++) stash_put<2>(stash, t, 8 * (4 + q), C1, q);
-        CellSet<4> C2;
-        cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
-        float rh[4];
+        if (Lc.nx == 16 && Lc.ny == 16) {
+          coarse_sweeps16(axc, ayc, sr2, sp2, Lc.h2, B.coarse_iters, t);
+        } else {
+          CellSet<4> C2;
+          cells_load<4>(C2, Lc.nx, Lc.ny, axc, ayc, kw);
+          float rh[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) rh[q] = C2.k[q] >= 0 ? sr2[C2.k[q]] * Lc.h2 : K(0);
-        // each lane's red and black cell of its two pairs; no cell: sres[0]
-        SweepCell X[2][2];
-        const int dummy = BOT_RES - BOT_P2;
+          for (int q = 0; q < 4; q++) rh[q] = C2.k[q] >= 0 ? sr2[C2.k[q]] * Lc.h2 : K(0);
+          // each lane's red and black cell of its two pairs; no cell: sres[0]
+          SweepCell X[2][2];
+          const int dummy = BOT_RES - BOT_P2;
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-          const SweepCell c0 = sweep_cell<4>(C2, rh, 2 * a, Lc.nx, dummy), c1 = sweep_cell<4>(C2, rh, 2 * a + 1, Lc.nx, dummy);
-          const bool first_red = ((C2.nb[2 * a] >> 4) & 1) == 0;
-          X[0][a] = sweep_sel(first_red, c0, c1);
-          X[1][a] = sweep_sel(first_red, c1, c0);
-        }
-        for (int it = 0; it < B.coarse_iters; it++) {
-          sweep_relax2(X[0][0], X[0][1], sp2);
-          wave_sync();
-          sweep_relax2(X[1][0], X[1][1], sp2);
-          wave_sync();
+          for (int a = 0; a < 2; a++) {
+            const SweepCell c0 = sweep_cell<4>(C2, rh, 2 * a, Lc.nx, dummy), c1 = sweep_cell<4>(C2, rh, 2 * a + 1, Lc.nx, dummy);
+            const bool first_red = ((C2.nb[2 * a] >> 4) & 1) == 0;
+            X[0][a] = sweep_sel(first_red, c0, c1);
+            X[1][a] = sweep_sel(first_red, c1, c0);
+          }
+          for (int it = 0; it < B.coarse_iters; it++) {
+            sweep_relax2(X[0][0], X[0][1], sp2);
+            wave_sync();
+            sweep_relax2(X[1][0], X[1][1], sp2);
+            wave_sync();
+          }
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) stash_get<4>(stash, t, 8 * q, C0, q);
