@@ -38,6 +38,9 @@ constexpr int B_VX = B_CX, B_VY = B_CY + 1;            // Vh
 __device__ __forceinline__ float mixv(float a, float b, float t) { return mixf(a, b, t); }
 __device__ __forceinline__ float4 mixv(float4 a, float4 b, float t) { return mix4(a, b, t); }
 
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) const T*;
+
 // A window of an index space held in LDS, element (i, j) at
 // s[(j - oy) * W + (i - ox)].  CK: reads flag `ok` false when a bilinear
 // footprint leaves the window; the caller then redoes the point from global
@@ -49,17 +52,26 @@ struct Win {
   const T* s;
   int ox, oy;
   float fox, foy;  // ox, oy as floats
-  // the footprint whose lower-left corner is at the floored position (bx, by):
-  // tested in float, exact for values in the window (small integers) and
-  // false for NaN or huge ones, so the conversion needs no clamp first
-  // The element index b * W + a is formed in float, exact for these small
-  // integers, and converted once: one conversion instead of two and no
-  // integer multiply.
+  float cb;        // LDS byte address of index (0, 0): s - (oy * W + ox) elements (set by init)
+  // On grids up to 16384 cells a side every term of the byte address
+  // by * W * sizeof(T) + bx * sizeof(T) + cb is an integer below 2^24, so it
+  // is formed exactly in float with two fused multiply-adds and converted once.
+  __device__ __forceinline__ void init(const T* s_, int ox_, int oy_) {
+    s = s_; ox = ox_; oy = oy_;
+    fox = (float)ox_; foy = (float)oy_;
+    const float base = (float)(unsigned)(uintptr_t)(lds_ptr<T>)s_;
+    cb = base - (foy * (float)W + fox) * (float)sizeof(T);
+  }
+  // the footprint whose lower-left corner is at the floored position (bx, by).
+  // CK: tested in float, exact for values in the window (small integers) and
+  // false for NaN or huge ones, so the conversion needs no clamp first; the
+  // element index b * W + a is formed in float too
   __device__ __forceinline__ const T* at(float bx, float by, bool& ok) const {
-    const float a = bx - fox, b = by - foy;
     if constexpr (!CK) {
-      return s + (unsigned)(int)fmaf(b, (float)W, a);
+      const unsigned byte = (unsigned)(int)fmaf(by, (float)(W * sizeof(T)), fmaf(bx, (float)sizeof(T), cb));
+      return (const T*)(lds_ptr<T>)byte;
     } else {
+      const float a = bx - fox, b = by - foy;
       const bool in = a >= K(0) && a < (float)(W - 1) && b >= K(0) && b < (float)(H - 1);
       ok = ok && in;
       return s + (in ? (unsigned)(int)fmaf(b, (float)W, a) : 0u);
@@ -271,10 +283,10 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   const int ax0 = x0 - AR, ay0 = y0 - AR, bx0 = x0 - AH, by0 = y0 - AH;
   // a staged velocity below vlim moves no trace of this workgroup by a cell
   // (below), so every bilinear footprint of its passes lies in the windows
-  // (grids below 2^16 cells a side keep the rounding of a position under
-  // 2^-8 cell)
+  // (grids up to 2^14 cells a side keep the rounding of a position under
+  // 2^-10 cell, and the untested windows' byte addresses exact in float)
   const float vlim = K(0.99) * P.dx / fabsf(dt);
-  bool fastv = nx <= 65536 && ny <= 65536;  // false: also a staged |U| or |V| >= vlim, or NaN
+  bool fastv = nx <= 16384 && ny <= 16384;  // false: also a staged |U| or |V| >= vlim, or NaN
 
   // A. inputs as the samplers see them (fC, fC4, fU, fV).  Every load is
   // issued before the first store, branch-free (an index outside the field
@@ -367,11 +379,8 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   auto passes = [&](auto ck) {
     constexpr bool CK = decltype(ck)::value;
     Tile<CK> t;
-    const float fax0 = (float)ax0, fay0 = (float)ay0, fbx0 = (float)bx0, fby0 = (float)by0;
-    t.T = {sT, ax0, ay0, fax0, fay0}; t.Y = {sY, ax0, ay0, fax0, fay0};
-    t.U = {sU, ax0, ay0, fax0, fay0}; t.V = {sV, ax0, ay0, fax0, fay0};
-    t.Th = {bT, bx0, by0, fbx0, fby0}; t.Yh = {bY, bx0, by0, fbx0, fby0};
-    t.Uh = {bU, bx0, by0, fbx0, fby0}; t.Vh = {bV, bx0, by0, fbx0, fby0};
+    t.T.init(sT, ax0, ay0); t.Y.init(sY, ax0, ay0); t.U.init(sU, ax0, ay0); t.V.init(sV, ax0, ay0);
+    t.Th.init(bT, bx0, by0); t.Yh.init(bY, bx0, by0); t.Uh.init(bU, bx0, by0); t.Vh.init(bV, bx0, by0);
     t.dx = P.dx;
     t.idx = K(1) / P.dx;
 
