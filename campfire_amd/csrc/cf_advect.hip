@@ -120,44 +120,53 @@ struct Tile {
 // and mixed with constant weights, the arithmetic of bil() on those points.
 // Every B and C point's first footprint lies inside the A windows (their
 // halo is AR, the points are within AH of the tile, the footprint reaches 1
-// more), so `ok` is unchanged by these samples.
+// more), so `ok` is unchanged by these samples.  A point's forward (+dt) and
+// backward (-dt) traces start from the same first velocity (in cells per
+// unit time, w1), which the forward pass keeps for the correction.
 template <class TL>
-__device__ __forceinline__ float2 trace_rest(const TL& t, float gx, float gy, float u, float v, float dt, bool& ok) {
-  const float vx = div_trace(u, t.dx, t.idx);
-  const float vy = div_trace(v, t.dx, t.idx);
+__device__ __forceinline__ float2 trace_from(const TL& t, float2 g, float2 w1, float dt, bool& ok) {
+  const float gx = g.x, gy = g.y;
   const float hdt = K(0.5) * dt;
-  const float mx_ = gx - hdt * vx, my_ = gy - hdt * vy;
+  const float mx_ = gx - hdt * w1.x, my_ = gy - hdt * w1.y;
   const float wx = div_trace(t.U.bil(mx_ - K(0), my_ - K(0.5), ok), t.dx, t.idx);
   const float wy = div_trace(t.V.bil(mx_ - K(0.5), my_ - K(0), ok), t.dx, t.idx);
   return make_float2(gx - dt * wx, gy - dt * wy);
 }
+template <class TL>
+__device__ __forceinline__ float2 first_vel(const TL& t, float u, float v) {
+  return make_float2(div_trace(u, t.dx, t.idx), div_trace(v, t.dx, t.idx));
+}
 // from cell centre (i + 1/2, j + 1/2): U footprint (i, j) with (0.5, 0), V (i, j) with (0, 0.5)
 template <class TL>
-__device__ __forceinline__ float2 trace_cell(const TL& t, int i, int j, float dt, bool& ok) {
+__device__ __forceinline__ float2 vel_cell(const TL& t, int i, int j) {
   const float u = mixf(mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0.5)), mixf(t.U.own(i, j + 1), t.U.own(i + 1, j + 1), K(0.5)),
                        K(0));
   const float v = mixf(mixf(t.V.own(i, j), t.V.own(i + 1, j), K(0)), mixf(t.V.own(i, j + 1), t.V.own(i + 1, j + 1), K(0)),
                        K(0.5));
-  return trace_rest(t, (float)i + K(0.5), (float)j + K(0.5), u, v, dt, ok);
+  return first_vel(t, u, v);
 }
 // from U face (i, j + 1/2): U footprint (i, j) with (0, 0), V (i - 1, j) with (0.5, 0.5)
 template <class TL>
-__device__ __forceinline__ float2 trace_uface(const TL& t, int i, int j, float dt, bool& ok) {
+__device__ __forceinline__ float2 vel_uface(const TL& t, int i, int j) {
   const float u = mixf(mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0)), mixf(t.U.own(i, j + 1), t.U.own(i + 1, j + 1), K(0)),
                        K(0));
   const float v = mixf(mixf(t.V.own(i - 1, j), t.V.own(i, j), K(0.5)), mixf(t.V.own(i - 1, j + 1), t.V.own(i, j + 1), K(0.5)),
                        K(0.5));
-  return trace_rest(t, (float)i, (float)j + K(0.5), u, v, dt, ok);
+  return first_vel(t, u, v);
 }
 // from V face (i + 1/2, j): U footprint (i, j - 1) with (0.5, 0.5), V (i, j) with (0, 0)
 template <class TL>
-__device__ __forceinline__ float2 trace_vface(const TL& t, int i, int j, float dt, bool& ok) {
+__device__ __forceinline__ float2 vel_vface(const TL& t, int i, int j) {
   const float u = mixf(mixf(t.U.own(i, j - 1), t.U.own(i + 1, j - 1), K(0.5)), mixf(t.U.own(i, j), t.U.own(i + 1, j), K(0.5)),
                        K(0.5));
   const float v = mixf(mixf(t.V.own(i, j), t.V.own(i + 1, j), K(0)), mixf(t.V.own(i, j + 1), t.V.own(i + 1, j + 1), K(0)),
                        K(0));
-  return trace_rest(t, (float)i + K(0.5), (float)j, u, v, dt, ok);
+  return first_vel(t, u, v);
 }
+// the start points of the three kinds
+__device__ __forceinline__ float2 at_cell(int i, int j) { return make_float2((float)i + K(0.5), (float)j + K(0.5)); }
+__device__ __forceinline__ float2 at_uface(int i, int j) { return make_float2((float)i, (float)j + K(0.5)); }
+__device__ __forceinline__ float2 at_vface(int i, int j) { return make_float2((float)i + K(0.5), (float)j); }
 
 // ---- global-memory forms (points whose footprints leave the windows) ----
 // forward pass at any index, as the corrector's samplers see Th/Yh, Uh, Vh
@@ -403,33 +412,38 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       ov = (tp && i < nx && j <= ny) || (ev && ey && i < nx);
     };
     // forward values of one point (i, j >= 0 inside the domain); d, okd: its trace
-    auto fwd_cell = [&](int i, int j, float& th, float4& yh, float2& d, bool& okd) {
+    auto fwd_cell = [&](int i, int j, float& th, float4& yh, float2& d, float2& w1, bool& okd) {
       okd = true;
-      d = trace_cell(t, i, j, dt, okd);
+      w1 = vel_cell(t, i, j);
+      d = trace_from(t, at_cell(i, j), w1, dt, okd);
       bool ok = okd;
       th = t.T.bil(d.x - K(0.5), d.y - K(0.5), ok);
       yh = t.Y.bil(d.x - K(0.5), d.y - K(0.5), ok);
       if (!ok) g_fwd_cell(F, P, i, j, th, yh);
     };
-    auto fwd_u = [&](int i, int j, float2& d, bool& okd) {
+    auto fwd_u = [&](int i, int j, float2& d, float2& w1, bool& okd) {
       okd = true;
-      d = trace_uface(t, i, j, dt, okd);
+      w1 = vel_uface(t, i, j);
+      d = trace_from(t, at_uface(i, j), w1, dt, okd);
       bool ok = okd;
       float u = t.U.bil(d.x - K(0), d.y - K(0.5), ok);
       if (!ok) u = g_fwd_u(F, P, i, j);
       return u;
     };
-    auto fwd_v = [&](int i, int j, float2& d, bool& okd) {
+    auto fwd_v = [&](int i, int j, float2& d, float2& w1, bool& okd) {
       okd = true;
-      d = trace_vface(t, i, j, dt, okd);
+      w1 = vel_vface(t, i, j);
+      d = trace_from(t, at_vface(i, j), w1, dt, okd);
       bool ok = okd;
       float v = t.V.bil(d.x - K(0.5), d.y - K(0), ok);
       if (!ok) v = g_fwd_v(F, P, i, j);
       return v;
     };
-    // the traces of one work item's cell and faces, kept from B1 for C
+    // the traces of one work item's cell and faces and their first
+    // velocities, kept from B1 for C
     struct Slot {
       float2 dC, dU, dV;
+      float2 wC, wU, wV;
       bool okC, okU, okV;
     };
     static_assert(NS == 3 && AX * AY == 2 * NT && AX + AY <= NT, "two tile slots and one edge slot per thread");
@@ -441,16 +455,16 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       int i, j;
       bool oc, ou, ov;
       item(q, i, j, oc, ou, ov);
-      S.dC = S.dU = S.dV = make_float2(0.f, 0.f);
+      S.dC = S.dU = S.dV = S.wC = S.wU = S.wV = make_float2(0.f, 0.f);
       S.okC = S.okU = S.okV = false;
       float th = P.Tamb;
       float4 yh = aY;
-      if (oc) fwd_cell(i, j, th, yh, S.dC, S.okC);
+      if (oc) fwd_cell(i, j, th, yh, S.dC, S.wC, S.okC);
       const int qb = (j - by0) * B_CX + (i - bx0);
       bT[qb] = th;
       bY[qb] = yh;
-      bU[(j - by0) * B_UX + (i - bx0)] = ou ? fwd_u(i, j, S.dU, S.okU) : K(0);
-      bV[(j - by0) * B_VX + (i - bx0)] = ov ? fwd_v(i, j, S.dV, S.okV) : K(0);
+      bU[(j - by0) * B_UX + (i - bx0)] = ou ? fwd_u(i, j, S.dU, S.wU, S.okU) : K(0);
+      bV[(j - by0) * B_VX + (i - bx0)] = ov ? fwd_v(i, j, S.dV, S.wV, S.okV) : K(0);
     };
     b1(tid, S0);
     b1(tid + NT, S1);
@@ -473,9 +487,9 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       float th = P.Tamb;
       float4 yh = aY;
       if (!(i < 0 || i >= nx || j >= ny)) {
-        float2 d;
+        float2 d, w1;
         bool okd;
-        fwd_cell(i, max(j, 0), th, yh, d, okd);
+        fwd_cell(i, max(j, 0), th, yh, d, w1, okd);
       }
       bT[b * B_CX + a] = th;
       bY[b * B_CX + a] = yh;
@@ -486,9 +500,9 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       const int i = bx0 + a, j = by0 + b;
       float u = K(0);
       if (!(i < 0 || i > nx || j >= ny)) {
-        float2 d;
+        float2 d, w1;
         bool okd;
-        u = fwd_u(i, max(j, 0), d, okd);
+        u = fwd_u(i, max(j, 0), d, w1, okd);
         if (j < 0) u = -u;
       }
       bU[b * B_UX + a] = u;
@@ -499,9 +513,9 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       const int i = bx0 + a, j = by0 + b;
       float v = K(0);
       if (!(i < 0 || i >= nx || j > ny)) {
-        float2 d;
+        float2 d, w1;
         bool okd;
-        v = fwd_v(i, max(j, 0), d, okd);
+        v = fwd_v(i, max(j, 0), d, w1, okd);
       }
       bV[b * B_VX + a] = v;
     }
@@ -517,7 +531,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
         const int k = j * nx + i;
         bool ok = CK ? S.okC : true;
         const float2 d = S.dC;
-        const float2 f = trace_cell(t, i, j, -dt, ok);
+        const float2 f = trace_from(t, at_cell(i, j), S.wC, -dt, ok);
         const float* pT = t.T.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
         const float4* pY = t.Y.at(floorf(d.x - K(0.5)), floorf(d.y - K(0.5)), ok);
         const float sbT = t.Th.bil(f.x - K(0.5), f.y - K(0.5), ok);
@@ -552,7 +566,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       }
       if (ou) {  // velCorrect, U (:78-90)
         bool ok = CK ? S.okU : true;
-        const float2 b = trace_uface(t, i, j, -dt, ok);
+        const float2 b = trace_from(t, at_uface(i, j), S.wU, -dt, ok);
         const float back = t.Uh.bil(b.x - K(0), b.y - K(0.5), ok);
         const float2 d = S.dU;
         const float* pU = t.U.at(floorf(d.x - K(0)), floorf(d.y - K(0.5)), ok);
@@ -567,7 +581,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       }
       if (ov) {  // velCorrect, V (:91-103)
         bool ok = CK ? S.okV : true;
-        const float2 b = trace_vface(t, i, j, -dt, ok);
+        const float2 b = trace_from(t, at_vface(i, j), S.wV, -dt, ok);
         const float back = t.Vh.bil(b.x - K(0.5), b.y - K(0), ok);
         const float2 d = S.dV;
         const float* pV = t.V.at(floorf(d.x - K(0.5)), floorf(d.y - K(0)), ok);
@@ -588,10 +602,10 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
       bool oc, ou, ov;
       item(tid + 2 * NT, i, j, oc, ou, ov);
       Slot S2;
-      S2.dC = S2.dU = S2.dV = make_float2(0.f, 0.f);
+      S2.dC = S2.dU = S2.dV = S2.wC = S2.wU = S2.wV = make_float2(0.f, 0.f);
       S2.okC = S2.okU = S2.okV = false;
-      if (ou) { S2.okU = true; S2.dU = trace_uface(t, i, j, dt, S2.okU); }
-      if (ov) { S2.okV = true; S2.dV = trace_vface(t, i, j, dt, S2.okV); }
+      if (ou) { S2.okU = true; S2.wU = vel_uface(t, i, j); S2.dU = trace_from(t, at_uface(i, j), S2.wU, dt, S2.okU); }
+      if (ov) { S2.okV = true; S2.wV = vel_vface(t, i, j); S2.dV = trace_from(t, at_vface(i, j), S2.wV, dt, S2.okV); }
       c1(tid + 2 * NT, S2);
     }
   };
