@@ -406,6 +406,10 @@ bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore
                    int tile, int zero_p = 0);
 bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
                  int tile);
+// the last V-cycle's level-0 up leg with the projection (as launch_mg_up_project);
+// false: no instantiation for that tile and nu
+bool phase_mg_up_project(BatchPhase& ph, const Level& f, const Level& c, const Fields& F, int nu, const int* skip,
+                         int t);
 bool phase_mg_bottom(BatchPhase& ph, const Level* L, int nlev, int first, bool coarse_coef, int nu, int coarse_iters,
                      const int* skip, double* band);
 

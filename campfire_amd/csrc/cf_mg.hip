@@ -1582,6 +1582,24 @@ bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& 
   });
 }
 
+bool phase_mg_up_project(BatchPhase& ph, const Level& f, const Level& c, const Fields& F, int nu, const int* skip,
+                         int t) {
+  const int nb = (int)ntiles(f.nx, f.ny, t, t);
+#define CF_UPP_PHASE(T, NU)                                                \
+  if (t == T && nu == NU) {                                                \
+    using Op = MgUpOp<T, T, NU, 0, true>;                                  \
+    typename Op::Args a{};                                                 \
+    a.f = f; a.c = c; a.st = MgStore{}; a.skip = skip; a.vg = nb; a.F = F; \
+    phase_add<Op>(ph, a, nb, dim3(Op::kThreads), "mg_up_project");        \
+    return true;                                                           \
+  }
+  CF_UPP_PHASE(64, 1) CF_UPP_PHASE(64, 2) CF_UPP_PHASE(64, 3)
+  CF_UPP_PHASE(32, 1) CF_UPP_PHASE(32, 2) CF_UPP_PHASE(32, 3)
+  CF_UPP_PHASE(16, 1) CF_UPP_PHASE(16, 2) CF_UPP_PHASE(16, 3)
+#undef CF_UPP_PHASE
+  return false;
+}
+
 // the bottom kernel takes the last min(3, nlev) levels when each fits its slot
 int mg_bottom_first(const Level* L, int nlev) {
   const int cap[3] = {256, 1024, 4096};

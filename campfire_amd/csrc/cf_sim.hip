@@ -608,7 +608,7 @@ struct cf_pool {
   uint64_t key = 0, fkey = 0;
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   Segments seg;               // residual-controlled steps: the step in segments
-  std::vector<size_t> cyc;    // first phase of each V-cycle of `step`; the last entry: the projection's
+  std::vector<size_t> cyc;    // first phase of each V-cycle of `step`; the last entry: the tail after them
   cf::Dyn** dDyns = nullptr;  // the live scenes' control blocks, for the convergence check (in `arena`)
   cf::TableArena arena;       // device tables of `step`
   cf::TableArena farena;      // device tables of frame_pre and frame_post
@@ -788,6 +788,11 @@ static int pool_build_step(cf_pool* pl) {
   auto leg_ok = [&](size_t q, int d, int t, int m) {
     return d < sc[q]->mg_bottom && ltile[q][d] == t && mg_store(sc[q], d).mode == m;
   };
+  // with a fixed cycle count and FP32 legs the projection runs inside the last
+  // V-cycle's level-0 up leg of each scene whose level 0 is tiled, as the
+  // scene alone runs it (the same faces, the same p: bitwise the same)
+  const bool proj_in_leg = !tol && !k.smoother_fp16;
+  std::vector<char> projected(sc.size(), 0);
   pl->cyc.clear();
   for (int c = 0; c < k.vcycles; c++) {
     pl->cyc.push_back(ph.size());
@@ -816,13 +821,19 @@ static int pool_build_step(cf_pool* pl) {
     for (int d = maxb - 1; d >= 0; d--) {
       for (int t : tiles)
         for (int m : modes) {
-          cf::BatchPhase a;
+          cf::BatchPhase a, b;  // b: the level-0 legs that also project
           for (size_t q = 0; q < sc.size(); q++)
             if (leg_ok(q, d, t, m)) {
               cf_sim* s = sc[q];
+              if (d == 0 && c == k.vcycles - 1 && proj_in_leg && m == 0 && s->up_project && !s->mg_simple &&
+                  cf::phase_mg_up_project(b, s->L[0], s->L[1], s->F, k.nu, nullptr, t)) {
+                projected[q] = 1;
+                continue;
+              }
               cf::phase_mg_up(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr, t);
             }
           if (!a.empty()) ph.push_back(a);
+          if (!b.empty()) ph.push_back(b);
         }
     }
     if (kry) {
@@ -848,7 +859,12 @@ static int pool_build_step(cf_pool* pl) {
   }
   pl->cyc.push_back(ph.size());
   if (fused) { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_kry_settle(a, s->L[0], s->dDyn, kb(s)); }
-  { cf::BatchPhase& a = P(); for (auto* s : sc) cf::phase_project(a, s->F, dc ? s->dP64 : nullptr); }
+  {
+    cf::BatchPhase a;
+    for (size_t q = 0; q < sc.size(); q++)
+      if (!projected[q]) cf::phase_project(a, sc[q]->F, dc ? sc[q]->dP64 : nullptr);
+    if (!a.empty()) ph.push_back(std::move(a));
+  }
   for (auto& a : ph)
     if (a.mixed) return set_err(CF_EINVAL, "pool: phase '%s' would run one kernel over scenes of another", a.tag);
   {  // the tables, and the control blocks the convergence check reads, in the step's arena
