@@ -282,7 +282,7 @@ __device__ __forceinline__ void AdvectOp::run(const Fields& F, int tile) {
   __shared__ float4 bY[B_CX * B_CY];
   __shared__ float bU[B_UX * B_UY];
   __shared__ float bV[B_VX * B_VY];
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   const int gx = (nx + AX - 1) / AX;
   const int x0 = (tile % gx) * AX, y0 = (tile / gx) * AY;

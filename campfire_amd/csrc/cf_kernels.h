@@ -227,6 +227,20 @@ struct Fields {
   float2* G;  // radiation: each cell's gas terms of the frame (emission, visible emission)
 };
 
+// The parameter block, read through the constant address space: it does not
+// change while a kernel runs, so the compiler reads it with scalar loads into
+// SGPRs also where the pointer itself comes from memory (a batched kernel's
+// argument block), where it would otherwise load it into some 40 VGPRs.
+__device__ __forceinline__ cf_params load_params(const cf_params* p) {
+  static_assert(sizeof(cf_params) % 4 == 0, "whole words");
+  const __attribute__((address_space(4))) int* q = (const __attribute__((address_space(4))) int*)(uintptr_t)p;
+  cf_params r;
+  int* w = reinterpret_cast<int*>(&r);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(cf_params) / 4); k++) w[k] = q[k];
+  return r;
+}
+
 // One multigrid level (the `Level` uniform of shaders/pressure.wgsl:7 and its arrays).
 struct Level {
   int nx, ny;

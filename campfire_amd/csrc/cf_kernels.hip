@@ -36,7 +36,7 @@ __device__ __forceinline__ bool tile_ij(int ex, int ey, int& i, int& j) {
 // advection, forward pass: scalForward + velForward (shaders/advect.wgsl:36-47,68-76)
 // -------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_advect_fwd(Fields F) {
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx + 1, ny + 1, i, j);
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_advect_fwd(Fields F) {
 // advection, MacCormack correction: scalCorrect + velCorrect (:49-66,78-103)
 // -------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_advect_corr(Fields F) {
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx + 1, ny + 1, i, j);
@@ -115,23 +115,41 @@ __global__ __launch_bounds__(256) void k_advect_corr(Fields F) {
 // -------------------------------------------------------------------------
 // explicit diffusion of one cell (shaders/diffuse.wgsl:18-50)
 // -------------------------------------------------------------------------
+// The weight a = f * rhoD(0.5 (T0 + T1)) of a face is the same for both of its
+// cells (the face temperature is symmetric in them), so a 32 x 8 workgroup
+// evaluates each face once into LDS and both cells read the one value: every
+// thread forms its cell's east and north faces, and the first 40 threads (one
+// wave) the tile's 8 west and 32 south edge faces.  ax[r][c]: the x-face west
+// of the tile's column c in row r (c = 32: the east edge); ay[r][c]: the
+// y-face south of row r (r = 8: the north edge).
+struct DiffFaces {
+  float ax[8][33];
+  float ay[9][32];
+};
+
+// Cell (i, j) of a 32 x 8 tile at thread (threadIdx.x, threadIdx.y); every
+// thread of the workgroup calls it (a workgroup barrier inside), `in` false
+// for a thread whose cell lies outside the domain (To, Yo are then unused).
 __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny, const float* __restrict__ Tin,
                                              const float4* __restrict__ Yin, const float* __restrict__ fx,
-                                             const float* __restrict__ fy, float dts, int i, int j,
-                                             float& To, float4& Yo) {
-  const int k = j * nx + i;
+                                             const float* __restrict__ fy, float dts, int i, int j, bool in,
+                                             DiffFaces& sh, float& To, float4& Yo) {
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 32 + tx;
+  const int x0 = i - tx, y0 = j - ty;
+  const int k = in ? j * nx + i : 0;
   // Every load is issued first, branch-free (a missing neighbour reads the
   // cell itself, and its term is not added), so the cell's loads are in
   // flight together; each term is then added only where the reference adds
   // it, in its order (west, east, south, north)
-  const bool has[4] = {i > 0, i < nx - 1, j > 0, j < ny - 1};
+  const bool has[4] = {in && i > 0, in && i < nx - 1, in && j > 0, in && j < ny - 1};
   const int kn[4] = {has[0] ? k - 1 : k, has[1] ? k + 1 : k, has[2] ? k - nx : k, has[3] ? k + nx : k};
   float f[4], Tn[4];
   float4 Yn[4];
-  f[0] = fx[j * (nx + 1) + i];
-  f[1] = fx[j * (nx + 1) + i + 1];
-  f[2] = fy[j * nx + i];
-  f[3] = fy[(j + 1) * nx + i];
+  const int kx = in ? j * (nx + 1) + i : 0, ky = in ? j * nx + i : 0;
+  f[0] = fx[kx];
+  f[1] = fx[kx + 1];
+  f[2] = fy[ky];
+  f[3] = fy[ky + (in ? nx : 0)];
   const float T0 = Tin[k];
   const float4 Y0 = Yin[k];
 #pragma unroll
@@ -139,16 +157,35 @@ __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny,
     Tn[d] = Tin[kn[d]];
     Yn[d] = Yin[kn[d]];
   }
+  // an edge face of the first 40 threads: the inner cell (ci, cj) of the
+  // tile, the outer one to its west (tid < 8, row tid) or south (column tid - 8)
+  const bool west = tid < 8;
+  const int ci = west ? x0 : x0 + tid - 8, cj = west ? y0 + tid : y0;
+  const bool eface = tid < 40 && ci < nx && cj < ny && (west ? ci > 0 : cj > 0);
+  const int ek = eface ? cj * nx + ci : 0, eo = eface ? (west ? ek - 1 : ek - nx) : 0;
+  const float eT0 = Tin[ek], eT1 = Tin[eo];
+  const float ef = eface ? (west ? fx[cj * (nx + 1) + ci] : fy[ek]) : K(0);
   const float iPr = K(1) / P.Pr;
+  auto weight = [&](bool ex, float fd, float ta, float tb) {
+    return ex && fd > K(0) ? fd * rhoDOf(P, K(0.5) * (ta + tb), iPr) : K(0);
+  };
+  sh.ax[ty][tx + 1] = weight(has[1], f[1], T0, Tn[1]);
+  sh.ay[ty + 1][tx] = weight(has[3], f[3], T0, Tn[3]);
+  if (tid < 40) {
+    const float w = weight(eface, ef, eT0, eT1);
+    if (west) sh.ax[tid][0] = w;
+    else sh.ay[0][tid - 8] = w;
+  }
+  __syncthreads();
+  const float a[4] = {sh.ax[ty][tx], sh.ax[ty][tx + 1], sh.ay[ty][tx], sh.ay[ty + 1][tx]};
   float sT = K(0);
   float4 sY = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int d = 0; d < 4; d++) {
     if (has[d] && f[d] > K(0)) {
-      const float a = f[d] * rhoDOf(P, K(0.5) * (T0 + Tn[d]), iPr);
-      sT += a * (Tn[d] - T0);
-      sY.x += a * (Yn[d].x - Y0.x); sY.y += a * (Yn[d].y - Y0.y);
-      sY.z += a * (Yn[d].z - Y0.z); sY.w += a * (Yn[d].w - Y0.w);
+      sT += a[d] * (Tn[d] - T0);
+      sY.x += a[d] * (Yn[d].x - Y0.x); sY.y += a[d] * (Yn[d].y - Y0.y);
+      sY.z += a[d] * (Yn[d].z - Y0.z); sY.w += a[d] * (Yn[d].w - Y0.w);
     }
   }
   const float c = dts / (rhoOf(P, T0) * P.dx * P.dx);
@@ -174,15 +211,17 @@ struct DiffuseOp {
     float* __restrict__ Tout = a_.Tout;
     float4* __restrict__ Yout = a_.Yout;
 
-  const cf_params P = *F.P;
+  __shared__ DiffFaces sfaces;
+  const cf_params P = load_params(F.P);
   const float dts = F.dyn->dsub;
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(vb, nx, ny, i, j);
-  if (i >= nx || j >= ny) return;
+  const bool in = i < nx && j < ny;
   float To;
   float4 Yo;
-  diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, dts, i, j, To, Yo);
+  diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, dts, i, j, in, sfaces, To, Yo);
+  if (!in) return;
   Tout[j * nx + i] = To;
   Yout[j * nx + i] = Yo;
 
@@ -376,12 +415,13 @@ struct SourcesOp {
     int* __restrict__ part = a_.part;
 
   __shared__ float red[3][8][33];
+  __shared__ DiffFaces sfaces;  // FUSE: the diffusion substep's face weights
   __shared__ int sacc[CF_ACC_GLOBAL];
   __shared__ int sglob[4][3];
   __shared__ float stamb4;
   __shared__ int sbad;
   const int tid = threadIdx.y * 32 + threadIdx.x;
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   for (int q = tid; q < CF_ACC_GLOBAL; q += 256) sacc[q] = 0;
   if (tid == 0) { stamb4 = powf(P.Tamb, K(4.0)); sbad = 0; }  // one wave evaluates the power
   __syncthreads();
@@ -390,13 +430,15 @@ struct SourcesOp {
   int i, j;
   tile_ij<32, 8>(vb, nx, ny, i, j);
   float qrel = K(0), qrad = K(0), tpeak = K(0);
-  if (i < nx && j < ny) {
-    const int k = j * nx + i;
-    const SourceIn sin_ = source_in(F, nx, i, j);  // issued with the diffusion's loads
-    float Tc;
-    float4 Yc;
-    if (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, Tc, Yc);
-    else { Tc = F.T[k]; Yc = F.Y[k]; }
+  const bool in = i < nx && j < ny;
+  const int k = in ? j * nx + i : 0;
+  SourceIn sin_{};
+  if (in) sin_ = source_in(F, nx, i, j);  // issued with the diffusion's loads
+  float Tc = K(0);
+  float4 Yc = make_float4(0.f, 0.f, 0.f, 0.f);
+  if constexpr (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, in, sfaces, Tc, Yc);
+  else if (in) { Tc = F.T[k]; Yc = F.Y[k]; }
+  if (in) {
     // the clamps below would hide a NaN (max(NaN, x) = x), so the inputs are checked
     if (!(isfinite(Tc) && isfinite(Yc.x) && isfinite(Yc.y) && isfinite(Yc.z) && isfinite(Yc.w))) sbad = 1;
     float To;
@@ -591,7 +633,7 @@ __device__ __forceinline__ float coef_v(const cf_params& P, const A& a, int i, i
 }
 
 __global__ __launch_bounds__(256) void k_forces(Fields F) {
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx + 1, ny + 1, i, j);
@@ -703,7 +745,7 @@ struct ForcesCoefOp {
   float (*sv)[TX] = reinterpret_cast<float (*)[TX]>(pool + NFU);
   float (*sax)[TX + 1] = reinterpret_cast<float (*)[TX + 1]>(pool);
   float (*say)[TX] = reinterpret_cast<float (*)[TX]>(pool + NFU);
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const MomRecip rc = mom_recip(P);
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;  // 32 x 8 threads
   // the workgroup after the tiles folds the sources partials into acc, so
@@ -863,7 +905,7 @@ __global__ __launch_bounds__(256) void k_forces_coef(const ForcesCoefOp::Args a_
 }
 
 __global__ __launch_bounds__(256) void k_coef_rhs(Fields F) {
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx + 1, ny + 1, i, j);
@@ -1000,7 +1042,7 @@ struct CflOp {
 
   __shared__ unsigned wm[4];
   __shared__ int last;
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   const int nU = (nx + 1) * ny, nV = nx * (ny + 1), n = nU > nV ? nU : nV;
   const int tid = threadIdx.x;
@@ -1690,7 +1732,7 @@ struct ProjectOp {
     const Fields& F = a_.F;
     const double* __restrict__ p64 = a_.p64;
 
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(vb, nx + 1, ny + 1, i, j);
@@ -1788,7 +1830,7 @@ struct GeometryOp {
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Fields& F = a_.F;
 
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   // the pieces' poses, once per workgroup: every thread evaluates each piece
   // at three points, and the angle's cos and sin would dominate
@@ -1906,7 +1948,7 @@ struct GasTermsOp {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Fields& F = a_.F;
-    const cf_params P = *F.P;
+    const cf_params P = load_params(F.P);
     const int n = P.nx * P.ny;
     for (int k = vb * 256 + (int)threadIdx.x; k < n; k += a_.vg * 256) {
       float g0, g1;
@@ -1937,6 +1979,7 @@ struct RadiationOp {
   static __device__ __forceinline__ void run(const Args& a_, int vb) {
     const Fields& F = a_.F;
     __shared__ float sq[64], sv[64];
+    // a vector load here: held in SGPRs the block costs this kernel VGPRs elsewhere (84 against 60)
     const cf_params P = *F.P;
     const int tgt = vb;
     const int li = (int)threadIdx.x / RG;  // the ray
@@ -2120,7 +2163,7 @@ void launch_bw_copy(const void* a, void* b, size_t bytes, hipStream_t s) {
 // per-workgroup max (as uint bits) and sum
 __global__ __launch_bounds__(256) void k_div_error(Fields F, float* __restrict__ part) {
   __shared__ float smax[256], ssum[256];
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(nx, ny, i, j);
@@ -2146,7 +2189,7 @@ __global__ __launch_bounds__(256) void k_div_error(Fields F, float* __restrict__
 // a converged projection div(f u) = S in every open cell, so the two agree.
 __global__ __launch_bounds__(256) void k_compat(Fields F, double* __restrict__ part) {
   __shared__ double ws[4], wo[4];
-  const cf_params P = *F.P;
+  const cf_params P = load_params(F.P);
   const int nx = P.nx, ny = P.ny;
   const double dx = (double)P.dx;
   double src = 0, out = 0;

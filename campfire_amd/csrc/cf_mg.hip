@@ -432,7 +432,7 @@ struct MgUpOp {
   // pressure from the region (exact over the tile plus one ring)
   static __device__ __forceinline__ void project(const Args& a_, const float* sp, int x0, int y0, const Pre& r) {
     const Fields& F = a_.F;
-    const cf_params P = *F.P;
+    const cf_params P = load_params(F.P);
     const int nx = P.nx, ny = P.ny;
     const float dt = P.dt, dx = P.dx;
     auto cell = [&](int i, int j) { return (j - y0 + H + 1) * PX + (i - x0 + H + 1); };
