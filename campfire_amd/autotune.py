@@ -99,19 +99,35 @@ def tune(sizes=(256, 512, 1024), tol=1e-6, refine=1, cfl=0.5, umin=1.0, vcycles=
 TILES = (16, 32, 64)
 
 
-def _median_step_ms(sim, twin, steps):
+def _timed_steps(restore, step, sync, steps, reps=2):
+    """ms per step of `steps` steps launched back to back from the restored
+    state (the throughput the benchmarks measure), the better of `reps` runs.
+    The first replay of a rebuilt step runs slower than the next ones, so the
+    caller builds and runs the step once before."""
+    best = float("inf")
+    for _ in range(reps):
+        restore()
+        sync()
+        t0 = time.perf_counter()
+        step(steps)
+        sync()
+        best = min(best, (time.perf_counter() - t0) * 1e3 / steps)
+    return best
+
+
+def _step_ms(sim, twin, steps):
     sim.copy_state_from(twin)
     sim.step(1)                       # captures the step graph of the current tiles
-    sim.copy_state_from(twin)
-    return float(np.median(sim.step_latencies(steps)))
+    sim.synchronize()
+    return _timed_steps(lambda: sim.copy_state_from(twin), sim.step, sim.synchronize, steps)
 
 
 def tune_pool_leg_tiles(pool, views, steps=10, log=None):
     """The leg tile edge of each depth of a pool's tiled levels, for the
     pool's scenes and schedule as they are: depth by depth from the finest,
     each candidate is timed over `steps` pool steps from the same device
-    snapshot of every scene (median step time), and a change is kept only
-    when it beats the best by more than 1 %.  The scenes are left in their
+    snapshot of every scene (launched back to back, the better of two runs),
+    and a change is kept only when it beats the best by more than 1 %.  The scenes are left in their
     snapshot state and the pool with the step of the chosen tiles built (and
     run once, so its first replay is not in the caller's timing).  Returns (tiles,
     [(tiles tried, ms per step)])."""
@@ -122,15 +138,16 @@ def tune_pool_leg_tiles(pool, views, steps=10, log=None):
         t.copy_state_from(v)
         twins.append(t)
 
+    def restore():
+        pool.synchronize()            # no pool step may still write the scenes
+        for v, t in zip(views, twins):
+            v.copy_state_from(t)
+
     def timed(tiles):
         pool.set_leg_tiles(tiles)
-        for v, t in zip(views, twins):
-            v.copy_state_from(t)
+        restore()
         pool.step(1)                  # rebuilds and captures the batched step
-        for v, t in zip(views, twins):
-            v.copy_state_from(t)
-        ms = float(np.median(pool.step_latencies(steps)))
-        return ms
+        return _timed_steps(restore, pool.step, pool.synchronize, steps)
 
     pool.set_leg_tiles([])
     tiles = pool.leg_tiles()
@@ -161,7 +178,8 @@ def tune_leg_tiles(nx, ny, dx, dt, steps=20, warm=16, seed=1, log=None):
     """The leg tile edge of each tiled level of an nx x ny scene (reference
     schedule), chosen level by level from the finest: each candidate is timed
     over `steps` steps from the same device snapshot of a burning probe scene
-    (median step time).  Returns (tiles, [(tiles tried, ms per step)])."""
+    (launched back to back, the better of two runs).  Returns (tiles, [(tiles
+    tried, ms per step)])."""
     log = log or (lambda *a: None)
     cfg = probe_config(nx, seed, width=nx * dx)
     cfg.update(ny=ny, dx=dx, dt=dt)
@@ -174,7 +192,7 @@ def tune_leg_tiles(nx, ny, dx, dt, steps=20, warm=16, seed=1, log=None):
     twin = FluidSim({"nx": nx, "ny": ny, "dt": dt}, dx=dx)
     twin.copy_state_from(sim)
     tiles = sim.leg_tiles()
-    table = [(list(tiles), _median_step_ms(sim, twin, steps))]
+    table = [(list(tiles), _step_ms(sim, twin, steps))]
     log("[autotune] %dx%d leg tiles %s (size rule): %.4f ms/step" % (nx, ny, tiles, table[0][1]))
     best_ms = table[0][1]
     for l in range(len(tiles)):
@@ -186,7 +204,7 @@ def tune_leg_tiles(nx, ny, dx, dt, steps=20, warm=16, seed=1, log=None):
             trial = list(tiles)
             trial[l] = t
             sim.set_leg_tiles(trial)
-            ms = _median_step_ms(sim, twin, steps)
+            ms = _step_ms(sim, twin, steps)
             table.append((trial, ms))
             log("[autotune] %dx%d leg tiles %s: %.4f ms/step" % (nx, ny, trial, ms))
             if ms < 0.99 * best_ms:  # a change must beat the noise of the timing
