@@ -504,10 +504,11 @@ __device__ __forceinline__ void leg_ring(float* sp, int PX, int PY, int RY, int 
   }
 }
 
-// Levels f, c = f + 1, n = f + 2: the down legs of f and c and the
+// Levels f, c = f + 1, n = f + 2 (f >= 1): the down legs of f and c and the
 // restriction into n.  Writes what the lone legs leave for the up legs: f.q,
 // c.q and c.rhs over the tile, n.rhs and n.p = 0.  c.p is not written: no leg
-// reads it before the up leg of c overwrites it.
+// reads it before the up leg of c overwrites it.  f.p is not read either: it
+// holds the zero that the restriction into f wrote.
 template <int TC, int NU, int NT>
 struct MgDown2Op {
   struct Args {
@@ -553,10 +554,9 @@ struct MgDown2Op {
       const int x = xf0 + rx, y = yf0 + ry;
       const LegCell c0 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x, y, rx, ry, RF, RF, PF, rv[q][0]);
       const LegCell c1 = leg_cell(f.nx, f.ny, f.ax, f.ay, f.rhs, f.h2, x + 1, y, rx + 1, ry, RF, RF, PF, rv[q][1]);
-      const bool in0 = (c0.ci & LEG_IN) != 0, in1 = (c1.ci & LEG_IN) != 0;
-      const float v0 = f.p[leg_idx(in0, x, y, nx)], v1 = f.p[leg_idx(in1, x + 1, y, nx)];
-      pv[q][0] = in0 ? v0 : 0.f;
-      pv[q][1] = in1 ? v1 : 0.f;
+      // f is a level above 0, whose V-cycle starts from the p = 0 that the
+      // restriction into it wrote (shaders/pressure.wgsl:104-113): not read
+      pv[q][0] = pv[q][1] = 0.f;
       const bool red0 = ((x + y) & 1) == 0;
       fR[q] = red0 ? c0 : c1;
       fB[q] = red0 ? c1 : c0;
