@@ -292,8 +292,9 @@ int forces_coef_cascade(int nlev);
 void launch_forces_coef(const Fields& F, const Level* L, int nlev, const int* part, int nparts, hipStream_t s);
 // fused multigrid (cf_mg.hip); skip (may be null): the launch does nothing
 // once *skip is non-zero (the residual target of the step has been met)
-// zero_p: the leg starts from p = 0 without reading the fine p (a Krylov
-// iteration's correction, which kry_upd_def does not clear)
+// zero_p: the leg starts from p = 0 without reading the fine p (a level above
+// 0, which holds the zero the restriction into it wrote, or a Krylov
+// iteration's level-0 correction, which kry_upd_def does not clear)
 bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s,
                     int zero_p = 0);
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s);

@@ -239,7 +239,9 @@ static void enqueue_vcycle(cf_sim* s, int nu, const int* skip, bool coarse_coef,
         mark(s, level_tag("mg_down2", l));
         l += 2;
       } else {
-        cf::launch_mg_down(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st, l == 0 && kry_fused(s) ? 1 : 0);
+        // a level above 0 holds the zero that the restriction into it wrote, and
+        // the fused Krylov pass leaves level 0's correction to start from zero
+        cf::launch_mg_down(s->L[l], s->L[l + 1], mg_store(s, l), nu, skip, s->st, l >= 1 || kry_fused(s) ? 1 : 0);
         mark(s, level_tag("mg_down", l));
         l += 1;
       }
@@ -804,7 +806,7 @@ static int pool_build_step(cf_pool* pl) {
             if (leg_ok(q, d, t, m)) {
               cf_sim* s = sc[q];
               cf::phase_mg_down(a, s->L[d], s->L[d + 1], mg_store(s, d), k.nu, tol ? &s->dDyn->converged : nullptr, t,
-                                d == 0 && fused ? 1 : 0);
+                                d >= 1 || fused ? 1 : 0);
             }
           if (!a.empty()) ph.push_back(a);
         }
