@@ -891,6 +891,7 @@ struct Bottom {
   int n;            // 1..3
   int coarse_coef;  // 1: restrict the coarsest level's ax/ay from L[n-2] here
   int nu, coarse_iters;  // coarse_iters = 0: direct FP64 solve of the coarsest level
+  int zero_p;            // the first level is above level 0: its p is the zero the restriction wrote, not read
 };
 
 template <int CPT>
@@ -1271,8 +1272,9 @@ struct MgBottomOp {
     const int nax = (Lc.nx + 1) * Lc.ny, nay = Lc.nx * (Lc.ny + 1), n0 = Lf0.nx * Lf0.ny;
     // ---- set-up: every global load is issued before the first store, so
     // they are in flight together.  The coarsest level's coefficients are
-    // copied, or restricted here (:57-67); p (warm start or zero) and rhs of
-    // the first level; the cell data of slots 0 and 1.
+    // copied, or restricted here (:57-67); p (level 0's warm start; a level
+    // above holds the restriction's zero, which is not read) and rhs of the
+    // first level; the cell data of slots 0 and 1.
     // (a thread past the face count loads face 0 and discards it)
     float cax = 0.f, cay = 0.f;
     const int tu = t < nax ? t : 0, tv = t < nay ? t : 0;
@@ -1292,7 +1294,7 @@ struct MgBottomOp {
     for (int q = 0; q < 4; q++) {
       const int k = t + BT * q;
       pv[q] = rv[q] = 0.f;
-      if (k < n0) { pv[q] = Lf0.p[k]; rv[q] = Lf0.rhs[k]; }
+      if (k < n0) { pv[q] = B.zero_p ? K(0) : Lf0.p[k]; rv[q] = Lf0.rhs[k]; }
     }
     // Slot 0 (<= 4096 cells): thread t owns the horizontal pairs starting at
     // cells 2t and 2t + 2048; slot 1 (<= 1024): threads t < 512 own the pair
@@ -1619,6 +1621,7 @@ static bool bottom_args(const Level* L, int nlev, int first, bool coarse_coef, i
   if (coarse_iters <= 0 && (!band || L[nlev - 1].nx > 32)) return false;
   for (int q = 0; q < n; q++) B.L[q] = L[first + q];
   B.n = n; B.coarse_coef = coarse_coef ? 1 : 0; B.nu = nu; B.coarse_iters = coarse_iters; B.skip = skip;
+  B.zero_p = first >= 1 ? 1 : 0;
   B.band = band;
   return true;
 }
