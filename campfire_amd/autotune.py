@@ -96,7 +96,7 @@ def tune(sizes=(256, 512, 1024), tol=1e-6, refine=1, cfl=0.5, umin=1.0, vcycles=
     return best, table
 
 
-TILES = (16, 32, 64)
+TILES = (16, 32, 48, 64)
 
 
 def _timed_steps(restore, step, sync, steps, reps=2):

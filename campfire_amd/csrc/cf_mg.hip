@@ -163,7 +163,10 @@ struct MgDownOp {
     int vg;      // tiles of this scene's launch
     int zero_p;  // start from p = 0 without reading the fine p
   };
-  static constexpr int kThreads = TX * TY >= 4096 ? 1024 : 256;  // 64x64 tiles: 1024
+  static constexpr int kThreads = TX * TY >= 2048 ? TX * TY / 4 : 256;  // four cells a thread from 64x32 up
+  static constexpr int kTX = TX, kTY = TY;
+  // the wide tiles (64x48, 64x32): two workgroups share a CU
+  static constexpr int kMinWaves = TX * TY == 3072 ? 6 : (TX * TY == 2048 ? 4 : 1);
   static constexpr int H = 2 * NU + 1;
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, PX = RX + 2, PY = RY + 2, NP = RX * RY / 2;
   static constexpr int PQ = (NP + kThreads - 1) / kThreads;
@@ -263,7 +266,7 @@ struct MgDownOp {
   }
 };
 template <int TX, int TY, int NU, int M>
-__global__ __launch_bounds__((MgDownOp<TX, TY, NU, M>::kThreads)) void k_mg_down(const typename MgDownOp<TX, TY, NU, M>::Args a_) {
+__global__ __launch_bounds__((MgDownOp<TX, TY, NU, M>::kThreads), (MgDownOp<TX, TY, NU, M>::kMinWaves)) void k_mg_down(const typename MgDownOp<TX, TY, NU, M>::Args a_) {
   MgDownOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 
@@ -286,7 +289,9 @@ struct MgUpOp {
   };
   using Args = std::conditional_t<PROJ, ArgsProj, ArgsLeg>;
   static_assert(!PROJ || M == 0, "the projection reads FP32 p");
-  static constexpr int kThreads = TX * TY >= 4096 ? 1024 : 256;  // 64x64 tiles: 1024
+  static constexpr int kThreads = TX * TY >= 2048 ? TX * TY / 4 : 256;  // four cells a thread from 64x32 up
+  static constexpr int kTX = TX, kTY = TY;
+  static constexpr int kMinWaves = TX * TY == 2048 ? 4 : 1;  // 64x32: two workgroups share a CU
   static constexpr int H = 2 * NU + (PROJ ? 1 : 0);
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, PX = RX + 2, PY = RY + 2, NP = RX * RY / 2;
   static constexpr int PQ = (NP + kThreads - 1) / kThreads;
@@ -467,7 +472,7 @@ struct MgUpOp {
   }
 };
 template <int TX, int TY, int NU, int M>
-__global__ __launch_bounds__((MgUpOp<TX, TY, NU, M>::kThreads)) void k_mg_up(const typename MgUpOp<TX, TY, NU, M>::Args a_) {
+__global__ __launch_bounds__((MgUpOp<TX, TY, NU, M>::kThreads), (MgUpOp<TX, TY, NU, M>::kMinWaves)) void k_mg_up(const typename MgUpOp<TX, TY, NU, M>::Args a_) {
   MgUpOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 template <int TX, int TY, int NU>
@@ -1458,7 +1463,9 @@ static inline unsigned ntiles(int nx, int ny, int tx, int ty) {
   return (unsigned)(((nx + tx - 1) / tx) * ((ny + ty - 1) / ty));
 }
 
-// 64x64 tiles of 1024 threads where a level has at least 256 of them (the
+// The wide tiles (code 48) where a level has at least 512 tiles of 64x64
+// (more than one round of them: two workgroups per CU overlap loads and
+// sweeps), 64x64 tiles of 1024 threads where it has at least 256 (the
 // smoothing halo then adds 34 % to the tile's cells instead of 72 %), 32x32
 // where it has at least 1024 of those, 16x16 below: a small level then has
 // four times the workgroups, each with half the per-thread chain, and its
@@ -1470,6 +1477,7 @@ int mg_tile_rule(const Level& f) {
 // 32x32), which for a pool's phase sum over its scenes: many small levels
 // together fill the device with large tiles, whose halo costs less
 int mg_tile_for_count(int n64, int n32) {
+  if (n64 >= 512) return 48;  // more than one round of 64x64 workgroups: the wide tiles
   if (n64 >= 256) return 64;
   return n32 >= 1024 ? 32 : 16;
 }
@@ -1478,15 +1486,32 @@ int mg_tile_for_count(int n64, int n32) {
 int mg_tile(const Level& f) { return f.tile ? f.tile : mg_tile_rule(f); }
 
 // dispatch on (tile, nu, storage mode) to the instantiation
+// Tile code 48 is the wide tile: 64 cells wide, short enough that two
+// workgroups share a CU (their registers fit twice), so one workgroup's loads
+// overlap the other's sweeps where a launch has several rounds of tiles:
+// 64x48 down legs (768 threads) at nu <= 2, 64x32 at nu = 3, 64x32 up legs
+// (512 threads).  The projecting up leg keeps 64x64 tiles.
+template <template <int, int, int, int> class Op>
+struct WideRows {
+  static constexpr int rows(int) { return 32; }
+};
+template <>
+struct WideRows<MgDownOp> {
+  static constexpr int rows(int nu) { return nu <= 2 ? 48 : 32; }
+};
 template <template <int, int, int, int> class Op, class F>
 static bool mg_dispatch(int tile, int nu, int mode, F&& f) {
-#define CF_MG_CASE(T, NU, M) \
-  if (tile == T && nu == NU && mode == M) { f(Op<T, T, NU, M>{}); return true; }
+#define CF_MG_CASE(T, NU, M)                                                               \
+  if (tile == T && nu == NU && mode == M) {                                                \
+    f(Op<(T == 48 ? 64 : T), (T == 48 ? WideRows<Op>::rows(NU) : T), NU, M>{});          \
+    return true;                                                                           \
+  }
 #define CF_MG_TILE(T) \
   CF_MG_CASE(T, 1, 0) CF_MG_CASE(T, 2, 0) CF_MG_CASE(T, 3, 0) \
   CF_MG_CASE(T, 1, 1) CF_MG_CASE(T, 2, 1) CF_MG_CASE(T, 3, 1) \
   CF_MG_CASE(T, 1, 3) CF_MG_CASE(T, 2, 3) CF_MG_CASE(T, 3, 3)
   CF_MG_TILE(64)
+  CF_MG_TILE(48)
   CF_MG_TILE(32)
   CF_MG_TILE(16)
 #undef CF_MG_TILE
@@ -1496,17 +1521,19 @@ static bool mg_dispatch(int tile, int nu, int mode, F&& f) {
 
 bool launch_mg_down(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s,
                     int zero_p) {
-  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+  const int t = mg_tile(f);
   return mg_dispatch<MgDownOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
+    const int nb = (int)ntiles(f.nx, f.ny, Op::kTX, Op::kTY);
     launch(k_mg_down_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, st, skip, nb, zero_p});
   });
 }
 
 bool launch_mg_up(const Level& f, const Level& c, const MgStore& st, int nu, const int* skip, hipStream_t s) {
-  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+  const int t = mg_tile(f);
   return mg_dispatch<MgUpOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
+    const int nb = (int)ntiles(f.nx, f.ny, Op::kTX, Op::kTY);
     launch(k_mg_up_of(op), dim3(nb), dim3(Op::kThreads), 0, s, typename Op::Args{f, c, st, skip, nb});
   });
 }
@@ -1550,7 +1577,7 @@ bool launch_mg_up2(const Level& f, const Level& c, const Level& n, int nu, int t
 }
 
 bool launch_mg_up_project(const Level& f, const Level& c, const Fields& F, int nu, const int* skip, hipStream_t s) {
-  const int t = mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
+  const int t = mg_tile(f) == 48 ? 64 : mg_tile(f), nb = (int)ntiles(f.nx, f.ny, t, t);
 #define CF_UPP_CASE(T, NU)                                                                          \
   if (t == T && nu == NU) {                                                                         \
     using Op = MgUpOp<T, T, NU, 0, true>;                                                           \
@@ -1568,25 +1595,25 @@ bool launch_mg_up_project(const Level& f, const Level& c, const Fields& F, int n
 
 bool phase_mg_down(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
                    int t, int zero_p) {
-  const int nb = (int)ntiles(f.nx, f.ny, t, t);
   return mg_dispatch<MgDownOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
+    const int nb = (int)ntiles(f.nx, f.ny, Op::kTX, Op::kTY);
     phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb, zero_p}, nb, dim3(Op::kThreads), "mg_down");
   });
 }
 
 bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& st, int nu, const int* skip,
                  int t) {
-  const int nb = (int)ntiles(f.nx, f.ny, t, t);
   return mg_dispatch<MgUpOp>(t, nu, st.mode, [&](auto op) {
     using Op = decltype(op);
+    const int nb = (int)ntiles(f.nx, f.ny, Op::kTX, Op::kTY);
     phase_add<Op>(ph, typename Op::Args{f, c, st, skip, nb}, nb, dim3(Op::kThreads), "mg_up");
   });
 }
 
 bool phase_mg_up_project(BatchPhase& ph, const Level& f, const Level& c, const Fields& F, int nu, const int* skip,
-                         int t) {
-  const int nb = (int)ntiles(f.nx, f.ny, t, t);
+                         int tile) {
+  const int t = tile == 48 ? 64 : tile, nb = (int)ntiles(f.nx, f.ny, t, t);
 #define CF_UPP_PHASE(T, NU)                                                \
   if (t == T && nu == NU) {                                                \
     using Op = MgUpOp<T, T, NU, 0, true>;                                  \

@@ -1277,8 +1277,8 @@ int cf_sim_set_leg_tiles(cf_sim* s, const int32_t* tiles, int n) {
   if (!s || (n > 0 && !tiles) || n < 0) return set_err(CF_EINVAL, "bad argument");
   if (n > s->mg_bottom) return set_err(CF_EINVAL, "%d tile entries for %d tiled levels", n, s->mg_bottom);
   for (int l = 0; l < n; l++)
-    if (tiles[l] != 0 && tiles[l] != 16 && tiles[l] != 32 && tiles[l] != 64)
-      return set_err(CF_EINVAL, "leg tile %d of level %d is not 0, 16, 32 or 64", tiles[l], l);
+    if (tiles[l] != 0 && tiles[l] != 16 && tiles[l] != 32 && tiles[l] != 48 && tiles[l] != 64)
+      return set_err(CF_EINVAL, "leg tile %d of level %d is not 0, 16, 32, 48 or 64", tiles[l], l);
   bool changed = false;
   for (int l = 0; l < s->mg_bottom; l++) {
     const int t = l < n ? tiles[l] : 0;
@@ -1776,8 +1776,8 @@ int cf_pool_set_leg_tiles(cf_pool* pl, const int32_t* tiles, int n) {
   if (!pl || (n > 0 && !tiles) || n < 0) return set_err(CF_EINVAL, "bad argument");
   if (n > MAXLEV) return set_err(CF_EINVAL, "%d tile entries for at most %d depths", n, MAXLEV);
   for (int d = 0; d < n; d++)
-    if (tiles[d] != 0 && tiles[d] != 16 && tiles[d] != 32 && tiles[d] != 64)
-      return set_err(CF_EINVAL, "leg tile %d of depth %d is not 0, 16, 32 or 64", tiles[d], d);
+    if (tiles[d] != 0 && tiles[d] != 16 && tiles[d] != 32 && tiles[d] != 48 && tiles[d] != 64)
+      return set_err(CF_EINVAL, "leg tile %d of depth %d is not 0, 16, 32, 48 or 64", tiles[d], d);
   for (int d = 0; d < MAXLEV; d++) {
     const int t = d < n ? tiles[d] : 0;
     if (pl->tile_over[d] != t) { pl->tile_over[d] = t; pl->dirty = true; }

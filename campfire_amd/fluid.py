@@ -151,7 +151,8 @@ class FluidSim:
         return {n: getattr(s, n) for n, _ in L.CfSchedule._fields_}
 
     def set_leg_tiles(self, tiles):
-        """Tile edge (16, 32, 64; 0 = default rule) of the tiled multigrid legs
+        """Tile code (16, 32 or 64: square tiles of that edge; 48: the wide tiles,
+        64 cells wide with 48 or 32 rows; 0 = default rule) of the tiled multigrid legs
         of each level from level 0; bitwise the same results for any choice."""
         a = np.ascontiguousarray(tiles, dtype=np.int32)
         check(L.lib().cf_sim_set_leg_tiles(self._h, a.ctypes.data, int(a.size)), "set_leg_tiles")

@@ -57,7 +57,7 @@ class ScenePool:
         L.check(L.lib().cf_pool_set_schedule(self._h, C.byref(s)), "pool schedule")
 
     def set_leg_tiles(self, tiles):
-        """Leg tile edge (16, 32, 64; 0 = the size rule on the depth's tile
+        """Leg tile code (16, 32, 64, or 48 for the wide tiles; 0 = the size rule on the depth's tile
         count) of each depth of the tiled multigrid levels; speed only."""
         a = np.asarray(tiles, np.int32)
         L.check(L.lib().cf_pool_set_leg_tiles(self._h, a.ctypes.data, int(a.size)), "pool set_leg_tiles")

@@ -148,7 +148,9 @@ int  cf_sim_set_schedule(cf_sim* sim, const cf_schedule* s);
 int  cf_sim_get_schedule(const cf_sim* sim, cf_schedule* s);
 
 /* Tile edge of the tiled multigrid legs per level (levels 0 .. n-1 of those
- * above the single-workgroup bottom solver): 16, 32 or 64, or 0 for the
+ * above the single-workgroup bottom solver): 16, 32 or 64 (square tiles of
+ * that edge), 48 (the wide tiles: 64 cells wide and 48 rows for the down
+ * legs at nu <= 2, else 32, so that two workgroups share a CU), or 0 for the
  * default size rule.  Any choice gives bitwise the same results (a tile with
  * a halo of 2 nu + 1 cells reproduces the global half-sweeps exactly); it
  * changes only the launch shape, which the autotuner picks by timing.  No
@@ -281,7 +283,7 @@ void cf_pool_destroy(cf_pool* pool);
 /* The schedule of every scene (cf_sim_set_schedule rules).  Every scene is
  * checked before any changes: on an error no scene's schedule has changed. */
 int  cf_pool_set_schedule(cf_pool* pool, const cf_schedule* sc);
-/* Leg tile edge (16, 32 or 64) of each depth of the tiled multigrid levels,
+/* Leg tile code (16, 32, 48 or 64, as cf_sim_set_leg_tiles) of each depth of the tiled multigrid levels,
  * for the legs of every scene at that depth; 0 (the default) takes the size
  * rule applied to the tile count of the whole depth.  Results do not depend
  * on tiles, only speed.  Takes effect at the next rebuild of the step. */

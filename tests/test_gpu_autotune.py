@@ -29,7 +29,7 @@ def test_leg_tiles_change_speed_not_results():
     from campfire_amd.fluid import FluidSim
     cfg = probe_config(256, 3)
     out = []
-    for tiles in (None, [64, 64], [32, 16]):   # 256^2 has two tiled levels
+    for tiles in (None, [64, 64], [32, 16], [48, 48]):   # 256^2 has two tiled levels
         g = FluidSim({"nx": 256, "ny": 256, "dt": cfg["dt"]}, dx=cfg["dx"])
         scenes.apply(g, cfg)
         if tiles:

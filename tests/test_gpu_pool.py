@@ -104,12 +104,16 @@ def test_batched_equals_alone_krylov_schedule():
 
 
 def test_pool_leg_tiles_change_no_result():
-    """Tiles forced per depth instead of the size rule's: every scene still
-    equals the scene alone, bitwise, under the reference schedule and the
-    FP16-smoother FCG schedule (whose legs have three storage modes)."""
+    """Tiles forced per depth instead of the size rule's, the wide tiles
+    (code 48: 64x48 or 64x32 down legs, 64x32 up legs) among them: every
+    scene still equals the scene alone, bitwise, under the reference schedule
+    and the FP16-smoother FCG schedule (whose legs have three storage modes)."""
     _compare_with_alone(None, frames=1, steps=8, tiles=[16, 32, 16, 64])
+    _compare_with_alone(None, frames=1, steps=8, tiles=[48, 48, 32, 16])  # the wide tiles, nu = 2
     _compare_with_alone({"nu": 3, "vcycles": 30, "tol": 1e-6, "coarse_iters": 24, "cfl": 0.5, "umin": 1.0,
                          "refine": 1, "smoother_fp16": 1, "krylov": 1}, frames=1, steps=4, tiles=[32, 64, 16, 32])
+    _compare_with_alone({"nu": 3, "vcycles": 30, "tol": 1e-6, "coarse_iters": 24, "cfl": 0.5, "umin": 1.0,
+                         "refine": 1, "smoother_fp16": 1, "krylov": 1}, frames=1, steps=4, tiles=[48, 48, 48, 16])
 
 
 def test_admission_follows_budget():
