@@ -163,7 +163,7 @@ struct MgDownOp {
     int vg;      // tiles of this scene's launch
     int zero_p;  // start from p = 0 without reading the fine p
   };
-  static constexpr int kThreads = TX * TY >= 2048 ? TX * TY / 4 : 256;  // four cells a thread from 64x32 up
+  static constexpr int kThreads = TX == 64 ? (TX * TY / 4 + 63) / 64 * 64 : 256;  // about four cells a thread
   static constexpr int kTX = TX, kTY = TY;
   // the wide tiles (64x48, 64x32): two workgroups share a CU
   static constexpr int kMinWaves = TX * TY == 3072 ? 6 : (TX * TY == 2048 ? 4 : 1);
@@ -289,9 +289,9 @@ struct MgUpOp {
   };
   using Args = std::conditional_t<PROJ, ArgsProj, ArgsLeg>;
   static_assert(!PROJ || M == 0, "the projection reads FP32 p");
-  static constexpr int kThreads = TX * TY >= 2048 ? TX * TY / 4 : 256;  // four cells a thread from 64x32 up
+  static constexpr int kThreads = TX == 64 ? (TX * TY / 4 + 63) / 64 * 64 : 256;  // about four cells a thread
   static constexpr int kTX = TX, kTY = TY;
-  static constexpr int kMinWaves = TX * TY == 2048 ? 4 : 1;  // 64x32: two workgroups share a CU
+  static constexpr int kMinWaves = TX == 64 && TY <= 32 ? 4 : 1;  // 64x32, 64x30: two workgroups share a CU
   static constexpr int H = 2 * NU + (PROJ ? 1 : 0);
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H, PX = RX + 2, PY = RY + 2, NP = RX * RY / 2;
   static constexpr int PQ = (NP + kThreads - 1) / kThreads;
@@ -476,7 +476,7 @@ __global__ __launch_bounds__((MgUpOp<TX, TY, NU, M>::kThreads), (MgUpOp<TX, TY, 
   MgUpOp<TX, TY, NU, M>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
 template <int TX, int TY, int NU>
-__global__ __launch_bounds__((MgUpOp<TX, TY, NU, 0, true>::kThreads)) void k_mg_up_project(
+__global__ __launch_bounds__((MgUpOp<TX, TY, NU, 0, true>::kThreads), (MgUpOp<TX, TY, NU, 0, true>::kMinWaves)) void k_mg_up_project(
     const typename MgUpOp<TX, TY, NU, 0, true>::Args a_) {
   MgUpOp<TX, TY, NU, 0, true>::run(a_, xcd_tile(blockIdx.x, gridDim.x));
 }
@@ -1490,7 +1490,9 @@ int mg_tile(const Level& f) { return f.tile ? f.tile : mg_tile_rule(f); }
 // workgroups share a CU (their registers fit twice), so one workgroup's loads
 // overlap the other's sweeps where a launch has several rounds of tiles:
 // 64x48 down legs (768 threads) at nu <= 2, 64x32 at nu = 3, 64x32 up legs
-// (512 threads).  The projecting up leg keeps 64x64 tiles.
+// (512 threads), 64x30 projecting up legs in pools (512 threads: the one more
+// ring of halo of 64x32 would give each thread a fourth pair of cells); a lone
+// scene's projecting leg keeps 64x64 tiles.
 template <template <int, int, int, int> class Op>
 struct WideRows {
   static constexpr int rows(int) { return 32; }
@@ -1613,16 +1615,18 @@ bool phase_mg_up(BatchPhase& ph, const Level& f, const Level& c, const MgStore& 
 
 bool phase_mg_up_project(BatchPhase& ph, const Level& f, const Level& c, const Fields& F, int nu, const int* skip,
                          int tile) {
-  const int t = tile == 48 ? 64 : tile, nb = (int)ntiles(f.nx, f.ny, t, t);
-#define CF_UPP_PHASE(T, NU)                                                \
-  if (t == T && nu == NU) {                                                \
-    using Op = MgUpOp<T, T, NU, 0, true>;                                  \
-    typename Op::Args a{};                                                 \
-    a.f = f; a.c = c; a.st = MgStore{}; a.skip = skip; a.vg = nb; a.F = F; \
-    phase_add<Op>(ph, a, nb, dim3(Op::kThreads), "mg_up_project");        \
-    return true;                                                           \
+  const int t = tile;
+#define CF_UPP_PHASE(T, NU)                                                                \
+  if (t == T && nu == NU) {                                                                \
+    using Op = MgUpOp<(T == 48 ? 64 : T), (T == 48 ? 30 : T), NU, 0, true>;                \
+    const int nb = (int)ntiles(f.nx, f.ny, Op::kTX, Op::kTY);                              \
+    typename Op::Args a{};                                                                 \
+    a.f = f; a.c = c; a.st = MgStore{}; a.skip = skip; a.vg = nb; a.F = F;                 \
+    phase_add<Op>(ph, a, nb, dim3(Op::kThreads), "mg_up_project");                        \
+    return true;                                                                           \
   }
   CF_UPP_PHASE(64, 1) CF_UPP_PHASE(64, 2) CF_UPP_PHASE(64, 3)
+  CF_UPP_PHASE(48, 1) CF_UPP_PHASE(48, 2) CF_UPP_PHASE(48, 3)
   CF_UPP_PHASE(32, 1) CF_UPP_PHASE(32, 2) CF_UPP_PHASE(32, 3)
   CF_UPP_PHASE(16, 1) CF_UPP_PHASE(16, 2) CF_UPP_PHASE(16, 3)
 #undef CF_UPP_PHASE
