@@ -69,8 +69,10 @@ def kernel_bytes(tag, nx, ny, levels, launches):
         # ax, ay of min(L-1, 5) coarse levels written
         casc = min(len(levels) - 1, 5)
         return 48 * nC + sum(8 * f for f in fine[1:1 + casc])
-    if base == "mg_down" and lev is not None:   # p, rhs, ax, ay read; p, coarse rhs, coarse p written
-        return 22 * fine[lev]
+    if base == "mg_down" and lev is not None:
+        # p (level 0 only: a level above starts from zero unread), rhs, ax, ay
+        # read; p, coarse rhs, coarse p written
+        return (22 if lev == 0 else 18) * fine[lev]
     if base == "mg_up" and lev is not None:     # p, coarse p, rhs, ax, ay read; p written
         return 21 * fine[lev]
     if base == "mg_up_project" and lev is not None:
@@ -78,9 +80,10 @@ def kernel_bytes(tag, nx, ny, levels, launches):
         # reads: U, V, fx, fy, T read, U, V written
         return 21 * fine[0] + 28 * nC
     if base == "mg_down2" and lev is not None:
-        # levels lev, lev+1 in one launch: fine p, rhs, ax, ay read, q written;
-        # coarse ax, ay read, q, rhs written; next level's rhs, p written
-        return 20 * fine[lev] + 16 * fine[lev + 1] + 8 * fine[lev + 2]
+        # levels lev, lev+1 in one launch: fine rhs, ax, ay read, q written
+        # (the fine p starts from zero unread); coarse ax, ay read, q, rhs
+        # written; next level's rhs, p written
+        return 16 * fine[lev] + 16 * fine[lev + 1] + 8 * fine[lev + 2]
     if base == "mg_up2" and lev is not None:
         # fine and coarse q, rhs, ax, ay read, p written; next level's p read
         return 20 * fine[lev] + 20 * fine[lev + 1] + 4 * fine[lev + 2]
