@@ -1,5 +1,6 @@
 """Autotuners: the multigrid schedule for residual-controlled stepping, and
-the launch shape (tile edge) of the tiled multigrid legs per level.
+the launch shape (tile code: square tiles of edge 16, 32 or 64, or the wide
+64-cell tiles, code 48) of the tiled multigrid legs per level.
 
 The leg tiles do not change results (a tile with a halo of 2 nu + 1 cells
 reproduces the global red-black half-sweeps exactly), only speed, which
@@ -123,7 +124,7 @@ def _step_ms(sim, twin, steps):
 
 
 def tune_pool_leg_tiles(pool, views, steps=10, log=None):
-    """The leg tile edge of each depth of a pool's tiled levels, for the
+    """The leg tile code of each depth of a pool's tiled levels, for the
     pool's scenes and schedule as they are: depth by depth from the finest,
     each candidate is timed over `steps` pool steps from the same device
     snapshot of every scene (launched back to back, the better of two runs),
@@ -176,7 +177,7 @@ def tune_pool_leg_tiles(pool, views, steps=10, log=None):
 
 
 def tune_leg_tiles(nx, ny, dx, dt, steps=20, warm=16, seed=1, log=None):
-    """The leg tile edge of each tiled level of an nx x ny scene (reference
+    """The leg tile code of each tiled level of an nx x ny scene (reference
     schedule), chosen level by level from the finest: each candidate is timed
     over `steps` steps from the same device snapshot of a burning probe scene
     (launched back to back, the better of two runs).  Returns (tiles, [(tiles

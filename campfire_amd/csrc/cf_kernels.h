@@ -309,8 +309,9 @@ bool launch_mg_up2(const Level& f, const Level& c, const Level& n, int nu, int t
 int mg_pair_tile(const Level& c);
 // FP16 copies of a level's face coefficients (once per step, FP16 smoother)
 void launch_coef_half(const Level& L, const LevelH& H, hipStream_t s);
-// tile edge (64, 32 or 16) of a level's legs: Level::tile, or by default a
-// rule on the level's size; a batched phase holds one tile size
+// tile code of a level's legs (64, 32, 16: square tiles of that edge; 48: the
+// wide tiles, 64 cells wide): Level::tile, or by default a rule on the level's
+// size (on the tile count of a pool's depth); a batched phase holds one tile code
 int mg_tile(const Level& f);
 int mg_tile_rule(const Level& f);
 int mg_tile_for_count(int n64, int n32);

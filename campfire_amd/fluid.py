@@ -158,7 +158,7 @@ class FluidSim:
         check(L.lib().cf_sim_set_leg_tiles(self._h, a.ctypes.data, int(a.size)), "set_leg_tiles")
 
     def leg_tiles(self):
-        """The tile edges in use, one per tiled level; 0 for a level whose
+        """The tile codes in use, one per tiled level; 0 for a level whose
         legs run in a two-level pair (their tile is not set per level)."""
         a = np.zeros(16, np.int32)
         n = check(L.lib().cf_sim_get_leg_tiles(self._h, a.ctypes.data, 16), "leg_tiles")
