@@ -116,6 +116,15 @@ def test_pool_leg_tiles_change_no_result():
                          "refine": 1, "smoother_fp16": 1, "krylov": 1}, frames=1, steps=4, tiles=[48, 48, 48, 16])
 
 
+def test_pool_wide_tiles_every_nu():
+    """The wide tiles' other instantiations (nu = 1: 64x48 down legs; nu = 3:
+    64x32 down legs and the projecting up leg on 64x30) under FP32 V-cycles
+    with a fixed cycle count: every scene bitwise equal to the scene alone."""
+    for nu in (1, 3):
+        _flush("[pool] wide tiles, nu", nu)
+        _compare_with_alone({"nu": nu, "vcycles": 2, "coarse_iters": 40}, frames=1, steps=4, tiles=[48, 48, 48, 48])
+
+
 def test_admission_follows_budget():
     from campfire_amd.pool import ScenePool, scene_bytes
     b256, b512 = scene_bytes(256, 256, 0.005), scene_bytes(512, 512, 0.0025)
