@@ -330,19 +330,30 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
   // fraction is zero the product is that zero (of the same sign) with any
   // positive factor: the exponential (and its quotient) is evaluated only
   // where the fuel fraction is not zero, a branch that waves of fuel-free
-  // cells skip.  Such a cell's fraction stays zero through the substeps.
+  // cells skip.  Where the fraction is +0, every dF is a zero (fminf returns
+  // the zero even beside a NaN rate), so each substep adds zeros to Y, T and
+  // burned: x + (+-0) = x for any x but a zero (denormals are kept), +0 +
+  // (+-0) = +0, and burned starts at +0.  Such a cell leaves the substeps
+  // bitwise unchanged unless one of the values it adds to is -0 (-0 + +0 =
+  // +0), so those cells skip the whole loop.
   const float rho0 = rhoOf(P, Tc);
   const float h = dt / (float)8;
   float burned = K(0);
+  constexpr uint32_t kNegZero = 0x80000000u;
+  const bool inert = __float_as_uint(Yc.x) == 0u && __float_as_uint(Yc.y) != kNegZero &&
+                     __float_as_uint(Yc.z) != kNegZero && __float_as_uint(Yc.w) != kNegZero &&
+                     __float_as_uint(Tc) != kNegZero;
+  if (!inert) {
 #pragma unroll 1
-  for (int s = 0; s < 8; s++) {
-    float ea = K(1);
-    if (Yc.x != K(0)) ea = expf(-P.reactTa / Tc);
-    const float wr = P.reactB * rho0 * Yc.x * Yc.y * ea;
-    const float dF = fminf(wr * h, fminf(Yc.x, div_rcp(Yc.y, sO2, isO2)));
-    Yc.x += -dF; Yc.y += -sO2 * dF; Yc.z += (K(1) + sO2) * dF; Yc.w += K(0);
-    Tc += div_rcp(dF * dHc, cp, icp);
-    burned += dF;
+    for (int s = 0; s < 8; s++) {
+      float ea = K(1);
+      if (Yc.x != K(0)) ea = expf(-P.reactTa / Tc);
+      const float wr = P.reactB * rho0 * Yc.x * Yc.y * ea;
+      const float dF = fminf(wr * h, fminf(Yc.x, div_rcp(Yc.y, sO2, isO2)));
+      Yc.x += -dF; Yc.y += -sO2 * dF; Yc.z += (K(1) + sO2) * dF; Yc.w += K(0);
+      Tc += div_rcp(dF * dHc, cp, icp);
+      burned += dF;
+    }
   }
   // soot formation and oxidation (:122-127)
   // (the same argument for the soot rates: a zero factor decides the product)
