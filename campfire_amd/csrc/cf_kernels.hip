@@ -356,16 +356,23 @@ __device__ __forceinline__ void sources_cell(const cf_params& P, const Fields& F
     }
   }
   // soot formation and oxidation (:122-127)
-  // (the same argument for the soot rates: a zero factor decides the product)
+  // (the same argument for the soot rates: a zero factor decides the product.
+  // In a cell that skipped the substeps, formation adds zeros (dSf is a zero)
+  // to values none of which is -0, and so does oxidation where the soot
+  // fraction is +0 as well: those cells skip them too)
   float ef = K(1), eo = K(1);
-  if (Yc.x != K(0)) ef = expf(-P.sootFormTa / Tc);
-  const float dSf = fminf(P.sootFormA * ef * Yc.x * dt, Yc.x);
-  Yc.x += -dSf; Yc.y += K(0); Yc.z += K(0); Yc.w += dSf;
-  if (Yc.w != K(0) && Yc.y != K(0)) eo = expf(-P.sootOxTa / Tc);
-  const float dSo = fminf(P.sootOxA * Yc.w * Yc.y * eo * dt, fminf(Yc.w, div_rcp(Yc.y, sO2, isO2)));
-  Yc.x += K(0); Yc.y += -sO2 * dSo; Yc.z += (K(1) + sO2) * dSo; Yc.w += -dSo;
-  Tc += div_rcp(dSo * dHc, cp, icp);
-  burned += dSo;
+  if (!inert) {
+    if (Yc.x != K(0)) ef = expf(-P.sootFormTa / Tc);
+    const float dSf = fminf(P.sootFormA * ef * Yc.x * dt, Yc.x);
+    Yc.x += -dSf; Yc.y += K(0); Yc.z += K(0); Yc.w += dSf;
+  }
+  if (!inert || __float_as_uint(Yc.w) != 0u) {
+    if (Yc.w != K(0) && Yc.y != K(0)) eo = expf(-P.sootOxTa / Tc);
+    const float dSo = fminf(P.sootOxA * Yc.w * Yc.y * eo * dt, fminf(Yc.w, div_rcp(Yc.y, sO2, isO2)));
+    Yc.x += K(0); Yc.y += -sO2 * dSo; Yc.z += (K(1) + sO2) * dSo; Yc.w += -dSo;
+    Tc += div_rcp(dSo * dHc, cp, icp);
+    burned += dSo;
+  }
   qrel = rho0 * burned * dHc / dt;
   // optically thin loss, linearised implicit (:130-136)
   const float r1 = rhoOf(P, Tc);
