@@ -127,13 +127,20 @@ struct DiffFaces {
   float ay[9][32];
 };
 
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
 // Cell (i, j) of a 32 x 8 tile at thread (threadIdx.x, threadIdx.y); every
 // thread of the workgroup calls it (a workgroup barrier inside), `in` false
 // for a thread whose cell lies outside the domain (To, Yo are then unused).
+// `before_barrier` runs once the cell's loads are issued, before the barrier:
+// what it stores to LDS is visible to the whole workgroup after the call.
+template <class Hook = NoHook>
 __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny, const float* __restrict__ Tin,
                                              const float4* __restrict__ Yin, const float* __restrict__ fx,
                                              const float* __restrict__ fy, float dts, int i, int j, bool in,
-                                             DiffFaces& sh, float& To, float4& Yo) {
+                                             DiffFaces& sh, float& To, float4& Yo, Hook before_barrier = Hook{}) {
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 32 + tx;
   const int x0 = i - tx, y0 = j - ty;
   const int k = in ? j * nx + i : 0;
@@ -165,6 +172,7 @@ __device__ __forceinline__ void diffuse_cell(const cf_params& P, int nx, int ny,
   const int ek = eface ? cj * nx + ci : 0, eo = eface ? (west ? ek - 1 : ek - nx) : 0;
   const float eT0 = Tin[ek], eT1 = Tin[eo];
   const float ef = eface ? (west ? fx[cj * (nx + 1) + ci] : fy[ek]) : K(0);
+  before_barrier();
   const float iPr = K(1) / P.Pr;
   auto weight = [&](bool ex, float fd, float ta, float tb) {
     return ex && fd > K(0) ? fd * rhoDOf(P, K(0.5) * (ta + tb), iPr) : K(0);
@@ -440,10 +448,18 @@ struct SourcesOp {
   __shared__ int sbad;
   const int tid = threadIdx.y * 32 + threadIdx.x;
   const cf_params P = load_params(F.P);
-  for (int q = tid; q < CF_ACC_GLOBAL; q += 256) sacc[q] = 0;
-  if (tid == 0) { stamb4 = powf(P.Tamb, K(4.0)); sbad = 0; }  // one wave evaluates the power
-  __syncthreads();
-  const float tamb4 = stamb4;
+  // the workgroup's accumulators and flag, and Tamb^4 (one wave evaluates the
+  // power), ready after the next workgroup barrier: in the fused form the
+  // diffusion's, reached once every load is issued, so that the power holds
+  // back no wave's loads
+  auto init = [&]() {
+    for (int q = tid; q < CF_ACC_GLOBAL; q += 256) sacc[q] = 0;
+    if (tid == 0) { stamb4 = powf(P.Tamb, K(4.0)); sbad = 0; }
+  };
+  if constexpr (!FUSE) {
+    init();
+    __syncthreads();
+  }
   const int nx = P.nx, ny = P.ny;
   int i, j;
   tile_ij<32, 8>(vb, nx, ny, i, j);
@@ -454,8 +470,9 @@ struct SourcesOp {
   if (in) sin_ = source_in(F, nx, i, j);  // issued with the diffusion's loads
   float Tc = K(0);
   float4 Yc = make_float4(0.f, 0.f, 0.f, 0.f);
-  if constexpr (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, in, sfaces, Tc, Yc);
+  if constexpr (FUSE) diffuse_cell(P, nx, ny, Tin, Yin, F.fx, F.fy, F.dyn->dsub, i, j, in, sfaces, Tc, Yc, init);
   else if (in) { Tc = F.T[k]; Yc = F.Y[k]; }
+  const float tamb4 = stamb4;
   if (in) {
     // the clamps below would hide a NaN (max(NaN, x) = x), so the inputs are checked
     if (!(isfinite(Tc) && isfinite(Yc.x) && isfinite(Yc.y) && isfinite(Yc.z) && isfinite(Yc.w))) sbad = 1;
